@@ -1,0 +1,195 @@
+/*
+ * spec_spars_hip.h -- C ABI of the MI355X (gfx950) sparsification hot path.
+ *
+ * Plain C: pointers, sizes and status codes only.  Every entry point returns
+ * an int status (SS_OK == 0); the message of the last failure on the calling
+ * thread is available from ss_last_error().  Nothing throws or aborts.
+ *
+ * Threading: one HIP stream per context; calls on different contexts may run
+ * concurrently from different threads; calls on one context are serialized
+ * by the caller.  Host-only entry points (ss_factor_*, ss_graph_check_*) need
+ * no GPU.
+ *
+ * Layouts: dense n x k blocks (sketch Y, solution Z) are row-major, node i's
+ * k values contiguous -- the FlattenedVec layout of the reference
+ * (src/lib.rs:116-142).  Laplacians are CSC with int32 indices and the
+ * diagonal stored, as handed over by Sparsifier::get_probs
+ * (src/sparsifier.rs:589-593); being symmetric, CSC and CSR coincide.
+ * Canonical edge order is the Laplacian's CSC upper triangle: column
+ * ascending, row ascending, row < column (src/sparsifier.rs:547-548).
+ */
+#ifndef SPEC_SPARS_HIP_H
+#define SPEC_SPARS_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SS_OK 0
+#define SS_ERR_INVALID 1        /* bad argument / malformed input           */
+#define SS_ERR_HIP 2            /* HIP runtime error                        */
+#define SS_ERR_NOT_CONVERGED 3  /* solve ended with rel. residual >= limit  */
+#define SS_ERR_NOMEM 4          /* host or device allocation failed         */
+#define SS_ERR_TIMEOUT 5        /* a device wait exceeded its spin limit    */
+#define SS_ERR_CAPACITY 6       /* size beyond a documented maximum         */
+
+#define SS_PREC_FP64 0          /* FP64 everywhere                           */
+#define SS_PREC_MIXED 1         /* FP32 SpMM/tri-solves, FP64 reductions and
+                                   FP64 true-residual refinement             */
+#define SS_STOP_REF_ABS 0       /* ||r||^2 <= tol (include/custom_cg.hpp:159,
+                                   :213-222), tol = 2e-7                      */
+#define SS_STOP_REL 1           /* ||r|| <= tol * ||b||                      */
+
+typedef struct ss_context ss_context;
+typedef struct ss_factor ss_factor;
+typedef struct ss_graph ss_graph;
+
+/* Last error message of the calling thread (truncated to len-1 chars).
+ * Returns the full message length. */
+int ss_last_error(char *buf, size_t len);
+const char *ss_version(void);
+
+/* ---------------------------------------------------------------------- */
+/* Context: device + one HIP stream.                                       */
+int ss_context_create(int device, ss_context **out);
+int ss_context_destroy(ss_context *ctx);
+/* Free / total device memory seen by this context (bytes). */
+int ss_context_mem_info(ss_context *ctx, size_t *free_b, size_t *total_b);
+
+/* ---------------------------------------------------------------------- */
+/* Host approximate-Cholesky factor (no GPU).  Same randomized algorithm as
+ * factorization_driver / perform_factorization_device
+ * (src/example.cc:90-471, include/auxilliary.hpp:90-173), natural order,
+ * made deterministic: one random stream per column keyed by (seed, column),
+ * duplicate merges and value sorts with a total order.                    */
+int ss_factor_build(int64_t n, const int32_t *colptr, const int32_t *rowidx,
+                    const double *vals, uint64_t seed, ss_factor **out);
+/* nnz counts the unit diagonal, as "factor nnz" at src/example.cc:448-450;
+ * depth_* are the longest dependency paths (src/example.cc:531-545). */
+int ss_factor_info(const ss_factor *f, int64_t *n, int64_t *nnz,
+                   int32_t *depth_fwd, int32_t *depth_bwd);
+/* The factor in the reference's precond_M layout (src/example.cc:422-470):
+ * CSR, row c = column c of the unit-lower factor L (diagonal first, rows
+ * ascending), plus D.  Arrays sized n+1, nnz, nnz, n. */
+int ss_factor_export(const ss_factor *f, int32_t *urowptr, int32_t *ucol,
+                     double *uval, double *diag);
+/* 1-based level of every row: forward = the DP at src/example.cc:531-542,
+ * backward = the same DP run on L^T. */
+int ss_factor_levels(const ss_factor *f, int32_t *lvl_fwd, int32_t *lvl_bwd);
+int ss_factor_destroy(ss_factor *f);
+
+/* ---------------------------------------------------------------------- */
+/* (a) JL sketch Y = B Q, replaces jl_sketch_colwise_flat
+ * (src/sparsifier.rs:490-539).  Edge e = (v1[e], v2[e], w[e]) in insertion
+ * order; B[:,e] = -sqrt(w) at v1, +sqrt(w) at v2 (src/sparsifier.rs:90-114);
+ * Q[e,j] = scale * sign(CityHash64WithSeed(LE64(e)||LE64(j), seed))
+ * (Rademacher, src/sparsifier.rs:379-407) or scale * uniform(-1,1)
+ * (src/sparsifier.rs:385-420) when uniform != 0.  Y: n x k row-major FP64.
+ * Edges must have v1 != v2, w > 0.                                         */
+int ss_sketch(ss_context *ctx, const int32_t *v1, const int32_t *v2,
+              const double *w, int64_t m, int64_t n, int32_t k,
+              uint64_t sketch_seed, double scale, int uniform, double *Y);
+
+/* ---------------------------------------------------------------------- */
+/* (b)+(c) Laplacian solve L Z = Y - mean, replaces ffi::run_solve_lap
+ * (src/lib.rs:103, include/example.h:18, src/example.cc:668-703) and the
+ * MKL PCG it drives (include/custom_cg.hpp:95-274).                        */
+typedef struct ss_solve_opts {
+    uint64_t factor_seed;   /* seed of the randomized factor                 */
+    int32_t stop_rule;      /* SS_STOP_REF_ABS or SS_STOP_REL                */
+    int32_t max_iter;       /* 400 in the reference                          */
+    double tol;             /* 2e-7 (REF_ABS) or e.g. 1e-8 (REL)             */
+    double fail_relres;     /* final rel. residual >= this -> NOT_CONVERGED;
+                               1e-2 in the reference (custom_cg.hpp:254)     */
+    int32_t precision;      /* SS_PREC_FP64 or SS_PREC_MIXED                 */
+    int32_t reserved;
+} ss_solve_opts;
+
+typedef struct ss_solve_report {
+    int32_t *iters;         /* [k] or NULL: PCG iterations per column        */
+    double *relres;         /* [k] or NULL: final true relative residual     */
+    int64_t factor_nnz;     /* out */
+    int32_t depth_fwd;      /* out */
+    int32_t depth_bwd;      /* out */
+    int32_t max_iters;      /* out */
+    int32_t refinements;    /* out: refinement sweeps (mixed precision)      */
+    double max_relres;      /* out */
+    double ms_factor;       /* out: host factorization time                  */
+    double ms_solve;        /* out: device solve time                        */
+} ss_solve_report;
+
+void ss_solve_opts_default(ss_solve_opts *o);   /* reference settings */
+int ss_solve_lap(ss_context *ctx, const double *Y, int64_t n, int32_t k,
+                 const int32_t *colptr, const int32_t *rowidx,
+                 const double *vals, int64_t nnz, const ss_solve_opts *opts,
+                 double *Z, ss_solve_report *rep);
+
+/* ---------------------------------------------------------------------- */
+/* (d) effective resistance + probabilities + coins + sampling + compaction,
+ * replaces compute_diff_norms / compute_probs / flip_coins /
+ * sample_and_reweight (src/sparsifier.rs:542-680).  Edges (u<v, lval=L_uv)
+ * in canonical order; Z n x k row-major FP64.
+ *   p_e = min(1, beta * -1 * L_uv * d_e^2 / k),  d_e = ||Z_u - Z_v||
+ *   c_e = e-th StdRng(ChaCha12)::seed_from_u64(sampling_seed) draw in [0,1)
+ *   keep iff c_e < p_e; kept weight = -(L_uv + (w - w/sqrt(p))).
+ * p_in != NULL replaces the computed probabilities (Z may then be NULL).
+ * p_out, d_out, mask_out (one byte per edge) may be NULL.  Kept edges are
+ * written in canonical order to ku/kv/kw (capacity m).                    */
+int ss_er_sample(ss_context *ctx, const double *Z, int64_t n, int32_t k,
+                 const int32_t *u, const int32_t *v, const double *lval,
+                 int64_t m, double beta, uint64_t sampling_seed,
+                 const double *p_in, double *p_out, double *d_out,
+                 uint8_t *mask_out, int64_t *kept, int32_t *ku, int32_t *kv,
+                 double *kw);
+
+/* ---------------------------------------------------------------------- */
+/* Device-resident graph: canonical edge list, Laplacian CSR, incidence ids,
+ * factor, sketch/solution blocks and sampling output stay in HBM.          */
+int ss_graph_create(ss_context *ctx, int64_t n, int64_t m, const int32_t *u,
+                    const int32_t *v, const double *w, ss_graph **out);
+int ss_graph_destroy(ss_graph *g);
+/* Device bytes the graph would need for k columns at a precision. */
+int ss_graph_bytes(int64_t n, int64_t m, int64_t factor_nnz, int32_t k,
+                   int32_t precision, size_t *bytes);
+int ss_graph_set_factor(ss_graph *g, const ss_factor *f);
+/* Laplacian of the graph as CSC with diagonal (host copy, sizes n+1 and
+ * n+2m) -- the input ss_factor_build expects. */
+int ss_graph_laplacian(const ss_graph *g, int32_t *colptr, int32_t *rowidx,
+                       double *vals);
+
+typedef struct ss_sparsify_opts {
+    int32_t k;              /* sketch columns                                */
+    int32_t uniform;        /* 0 = Rademacher signs                          */
+    double scale;           /* jl_scaling_factor                             */
+    uint64_t sketch_seed;
+    uint64_t sampling_seed;
+    double beta;            /* eps^-2 * beta_constant * ln n                 */
+    ss_solve_opts solve;
+} ss_sparsify_opts;
+
+typedef struct ss_sparsify_report {
+    int64_t kept;
+    int32_t max_iters;
+    int32_t refinements;
+    double max_relres;
+    double ms_sketch;       /* device time per phase (HIP events)           */
+    double ms_solve;
+    double ms_sample;
+    double ms_total;
+} ss_sparsify_report;
+
+/* Sketch -> solve -> ER/sample/compact, all on the device; the factor must
+ * have been set.  Results stay resident until read back. */
+int ss_graph_sparsify(ss_graph *g, const ss_sparsify_opts *o,
+                      ss_sparsify_report *rep);
+int ss_graph_read_kept(const ss_graph *g, int64_t cap, int32_t *ku,
+                       int32_t *kv, double *kw, int64_t *kept);
+/* Sketch Y / solution Z of the last run as FP64 n x k (host). */
+int ss_graph_read_block(const ss_graph *g, int which /*0=Y,1=Z*/, double *out);
+int ss_graph_read_probs(const ss_graph *g, double *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,14 @@
+"""spec_spars_amd -- MI355X (gfx950) sparsification hot path of spec_spars.
+
+The compute path is the HIP library libspecspars_hip.so behind the C ABI of
+include/spec_spars_hip.h; this package is its Python binding (ctypes) plus
+synthetic graph generators.  Importing the package loads the library and
+fails if it has not been built.
+"""
+from . import _lib
+from ._lib import SSError, PREC_FP64, PREC_MIXED, STOP_REF_ABS, STOP_REL  # noqa: F401
+
+_lib.load()
+
+from .api import Context, Factor, Graph, solve_opts  # noqa: E402,F401
+from . import graphs  # noqa: E402,F401

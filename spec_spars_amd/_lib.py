@@ -1,0 +1,130 @@
+"""ctypes binding of libspecspars_hip.so (the C ABI in include/spec_spars_hip.h).
+
+The library is loaded from the package directory only; if it is missing the
+import fails loudly -- there is no fallback implementation.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libspecspars_hip.so")
+
+SS_OK = 0
+SS_ERR_INVALID = 1
+SS_ERR_HIP = 2
+SS_ERR_NOT_CONVERGED = 3
+SS_ERR_NOMEM = 4
+SS_ERR_TIMEOUT = 5
+SS_ERR_CAPACITY = 6
+PREC_FP64 = 0
+PREC_MIXED = 1
+STOP_REF_ABS = 0
+STOP_REL = 1
+
+c_i32p = ctypes.POINTER(ctypes.c_int32)
+c_i64p = ctypes.POINTER(ctypes.c_int64)
+c_f64p = ctypes.POINTER(ctypes.c_double)
+c_u8p = ctypes.POINTER(ctypes.c_uint8)
+c_vp = ctypes.c_void_p
+
+
+class SolveOpts(ctypes.Structure):
+    _fields_ = [("factor_seed", ctypes.c_uint64), ("stop_rule", ctypes.c_int32),
+                ("max_iter", ctypes.c_int32), ("tol", ctypes.c_double),
+                ("fail_relres", ctypes.c_double), ("precision", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+class SolveReport(ctypes.Structure):
+    _fields_ = [("iters", c_i32p), ("relres", c_f64p), ("factor_nnz", ctypes.c_int64),
+                ("depth_fwd", ctypes.c_int32), ("depth_bwd", ctypes.c_int32),
+                ("max_iters", ctypes.c_int32), ("refinements", ctypes.c_int32),
+                ("max_relres", ctypes.c_double), ("ms_factor", ctypes.c_double),
+                ("ms_solve", ctypes.c_double)]
+
+
+class SparsifyOpts(ctypes.Structure):
+    _fields_ = [("k", ctypes.c_int32), ("uniform", ctypes.c_int32), ("scale", ctypes.c_double),
+                ("sketch_seed", ctypes.c_uint64), ("sampling_seed", ctypes.c_uint64),
+                ("beta", ctypes.c_double), ("solve", SolveOpts)]
+
+
+class SparsifyReport(ctypes.Structure):
+    _fields_ = [("kept", ctypes.c_int64), ("max_iters", ctypes.c_int32),
+                ("refinements", ctypes.c_int32), ("max_relres", ctypes.c_double),
+                ("ms_sketch", ctypes.c_double), ("ms_solve", ctypes.c_double),
+                ("ms_sample", ctypes.c_double), ("ms_total", ctypes.c_double)]
+
+
+# name -> (restype, argtypes); every symbol include/spec_spars_hip.h declares
+SIGNATURES = {
+    "ss_last_error": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
+    "ss_version": (ctypes.c_char_p, []),
+    "ss_context_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(c_vp)]),
+    "ss_context_destroy": (ctypes.c_int, [c_vp]),
+    "ss_context_mem_info": (ctypes.c_int, [c_vp, ctypes.POINTER(ctypes.c_size_t),
+                                           ctypes.POINTER(ctypes.c_size_t)]),
+    "ss_factor_build": (ctypes.c_int, [ctypes.c_int64, c_i32p, c_i32p, c_f64p, ctypes.c_uint64,
+                                       ctypes.POINTER(c_vp)]),
+    "ss_factor_info": (ctypes.c_int, [c_vp, c_i64p, c_i64p, c_i32p, c_i32p]),
+    "ss_factor_export": (ctypes.c_int, [c_vp, c_i32p, c_i32p, c_f64p, c_f64p]),
+    "ss_factor_levels": (ctypes.c_int, [c_vp, c_i32p, c_i32p]),
+    "ss_factor_destroy": (ctypes.c_int, [c_vp]),
+    "ss_sketch": (ctypes.c_int, [c_vp, c_i32p, c_i32p, c_f64p, ctypes.c_int64, ctypes.c_int64,
+                                 ctypes.c_int32, ctypes.c_uint64, ctypes.c_double, ctypes.c_int,
+                                 c_f64p]),
+    "ss_solve_opts_default": (None, [ctypes.POINTER(SolveOpts)]),
+    "ss_solve_lap": (ctypes.c_int, [c_vp, c_f64p, ctypes.c_int64, ctypes.c_int32, c_i32p, c_i32p,
+                                    c_f64p, ctypes.c_int64, ctypes.POINTER(SolveOpts), c_f64p,
+                                    ctypes.POINTER(SolveReport)]),
+    "ss_er_sample": (ctypes.c_int, [c_vp, c_f64p, ctypes.c_int64, ctypes.c_int32, c_i32p, c_i32p,
+                                    c_f64p, ctypes.c_int64, ctypes.c_double, ctypes.c_uint64,
+                                    c_f64p, c_f64p, c_f64p, c_u8p, c_i64p, c_i32p, c_i32p,
+                                    c_f64p]),
+    "ss_graph_create": (ctypes.c_int, [c_vp, ctypes.c_int64, ctypes.c_int64, c_i32p, c_i32p,
+                                       c_f64p, ctypes.POINTER(c_vp)]),
+    "ss_graph_destroy": (ctypes.c_int, [c_vp]),
+    "ss_graph_bytes": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                      ctypes.c_int32, ctypes.c_int32,
+                                      ctypes.POINTER(ctypes.c_size_t)]),
+    "ss_graph_set_factor": (ctypes.c_int, [c_vp, c_vp]),
+    "ss_graph_laplacian": (ctypes.c_int, [c_vp, c_i32p, c_i32p, c_f64p]),
+    "ss_graph_sparsify": (ctypes.c_int, [c_vp, ctypes.POINTER(SparsifyOpts),
+                                         ctypes.POINTER(SparsifyReport)]),
+    "ss_graph_read_kept": (ctypes.c_int, [c_vp, ctypes.c_int64, c_i32p, c_i32p, c_f64p, c_i64p]),
+    "ss_graph_read_block": (ctypes.c_int, [c_vp, ctypes.c_int, c_f64p]),
+    "ss_graph_read_probs": (ctypes.c_int, [c_vp, c_f64p]),
+}
+
+_lib = None
+
+
+def load():
+    """Load the shared library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            "libspecspars_hip.so is not built; run __graft_entry__.build() "
+            "(or python -m spec_spars_amd.build) first")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class SSError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("status %d: %s" % (code, msg))
+        self.code = code
+
+
+def check(rc):
+    if rc != SS_OK:
+        buf = ctypes.create_string_buffer(1024)
+        load().ss_last_error(buf, 1024)
+        raise SSError(rc, buf.value.decode(errors="replace"))

@@ -1,0 +1,222 @@
+"""numpy-level wrappers over the C ABI.
+
+Every call goes to libspecspars_hip.so; device work runs on the HIP device of
+the Context.  Host-only pieces (Factor) need no GPU.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, load
+
+_i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+_f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(ctypes.POINTER(ct)) if a is not None else None
+
+
+def solve_opts(factor_seed=0, stop_rule=_lib.STOP_REF_ABS, max_iter=400, tol=None,
+               fail_relres=1e-2, precision=_lib.PREC_FP64):
+    o = _lib.SolveOpts()
+    load().ss_solve_opts_default(ctypes.byref(o))
+    o.factor_seed = factor_seed
+    o.stop_rule = stop_rule
+    o.max_iter = max_iter
+    o.tol = (2e-7 if stop_rule == _lib.STOP_REF_ABS else 1e-8) if tol is None else tol
+    o.fail_relres = fail_relres
+    o.precision = precision
+    return o
+
+
+class Factor:
+    """Host approximate-Cholesky factor of a Laplacian given as CSC."""
+
+    def __init__(self, colptr, rowidx, vals, seed=0):
+        lib = load()
+        self._cp, self._ri, self._v = _i32(colptr), _i32(rowidx), _f64(vals)
+        self.n = len(self._cp) - 1
+        h = ctypes.c_void_p()
+        check(lib.ss_factor_build(self.n, _p(self._cp, ctypes.c_int32), _p(self._ri, ctypes.c_int32),
+                                  _p(self._v, ctypes.c_double), seed, ctypes.byref(h)))
+        self._h = h
+
+    def info(self):
+        n, nnz = ctypes.c_int64(), ctypes.c_int64()
+        df, db = ctypes.c_int32(), ctypes.c_int32()
+        check(load().ss_factor_info(self._h, ctypes.byref(n), ctypes.byref(nnz), ctypes.byref(df),
+                                    ctypes.byref(db)))
+        return {"n": n.value, "nnz": nnz.value, "depth_fwd": df.value, "depth_bwd": db.value}
+
+    def export(self):
+        inf = self.info()
+        rp = np.zeros(self.n + 1, np.int32)
+        ci = np.zeros(inf["nnz"], np.int32)
+        va = np.zeros(inf["nnz"], np.float64)
+        d = np.zeros(self.n, np.float64)
+        check(load().ss_factor_export(self._h, _p(rp, ctypes.c_int32), _p(ci, ctypes.c_int32),
+                                      _p(va, ctypes.c_double), _p(d, ctypes.c_double)))
+        return rp, ci, va, d
+
+    def levels(self):
+        f = np.zeros(self.n, np.int32)
+        b = np.zeros(self.n, np.int32)
+        check(load().ss_factor_levels(self._h, _p(f, ctypes.c_int32), _p(b, ctypes.c_int32)))
+        return f, b
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().ss_factor_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+
+class Context:
+    """A device and its HIP stream."""
+
+    def __init__(self, device=0):
+        lib = load()
+        h = ctypes.c_void_p()
+        check(lib.ss_context_create(device, ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().ss_context_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def mem_info(self):
+        f, t = ctypes.c_size_t(), ctypes.c_size_t()
+        check(load().ss_context_mem_info(self._h, ctypes.byref(f), ctypes.byref(t)))
+        return f.value, t.value
+
+    def sketch(self, v1, v2, w, n, k, seed=0, scale=1.0, uniform=False):
+        v1, v2, w = _i32(v1), _i32(v2), _f64(w)
+        Y = np.zeros((n, k), np.float64)
+        check(load().ss_sketch(self._h, _p(v1, ctypes.c_int32), _p(v2, ctypes.c_int32),
+                               _p(w, ctypes.c_double), len(w), n, k, seed, scale, int(bool(uniform)),
+                               _p(Y, ctypes.c_double)))
+        return Y
+
+    def solve_lap(self, Y, colptr, rowidx, vals, opts=None):
+        Y = _f64(Y)
+        n, k = Y.shape
+        cp, ri, va = _i32(colptr), _i32(rowidx), _f64(vals)
+        Z = np.zeros((n, k), np.float64)
+        iters = np.zeros(k, np.int32)
+        rel = np.zeros(k, np.float64)
+        rep = _lib.SolveReport()
+        rep.iters = _p(iters, ctypes.c_int32)
+        rep.relres = _p(rel, ctypes.c_double)
+        o = opts if opts is not None else solve_opts()
+        check(load().ss_solve_lap(self._h, _p(Y, ctypes.c_double), n, k, _p(cp, ctypes.c_int32),
+                                  _p(ri, ctypes.c_int32), _p(va, ctypes.c_double), len(va),
+                                  ctypes.byref(o), _p(Z, ctypes.c_double), ctypes.byref(rep)))
+        info = {f: getattr(rep, f) for f, _ in _lib.SolveReport._fields_ if f not in ("iters", "relres")}
+        info["iters"] = iters
+        info["relres"] = rel
+        return Z, info
+
+    def er_sample(self, u, v, lval, k, beta, seed, Z=None, p_in=None, n=None):
+        u, v, lval = _i32(u), _i32(v), _f64(lval)
+        m = len(u)
+        Zc = _f64(Z) if Z is not None else None
+        if n is None:
+            n = Zc.shape[0]
+        pin = _f64(p_in) if p_in is not None else None
+        p = np.zeros(m, np.float64)
+        d = np.zeros(m, np.float64)
+        mask = np.zeros(m, np.uint8)
+        ku = np.zeros(max(m, 1), np.int32)
+        kv = np.zeros(max(m, 1), np.int32)
+        kw = np.zeros(max(m, 1), np.float64)
+        kept = ctypes.c_int64()
+        check(load().ss_er_sample(self._h, _p(Zc, ctypes.c_double), n, k, _p(u, ctypes.c_int32),
+                                  _p(v, ctypes.c_int32), _p(lval, ctypes.c_double), m, beta, seed,
+                                  _p(pin, ctypes.c_double), _p(p, ctypes.c_double),
+                                  _p(d, ctypes.c_double), _p(mask, ctypes.c_uint8), ctypes.byref(kept),
+                                  _p(ku, ctypes.c_int32), _p(kv, ctypes.c_int32), _p(kw, ctypes.c_double)))
+        c = kept.value
+        return {"p": p, "d": d if p_in is None else None, "mask": mask, "kept": c,
+                "ku": ku[:c].copy(), "kv": kv[:c].copy(), "kw": kw[:c].copy()}
+
+
+class Graph:
+    """Device-resident graph (canonical edges u < v sorted by (v, u))."""
+
+    def __init__(self, ctx, n, u, v, w):
+        self.ctx = ctx
+        self.n = int(n)
+        self._u, self._v, self._w = _i32(u), _i32(v), _f64(w)
+        self.m = len(self._u)
+        h = ctypes.c_void_p()
+        check(load().ss_graph_create(ctx._h, self.n, self.m, _p(self._u, ctypes.c_int32),
+                                     _p(self._v, ctypes.c_int32), _p(self._w, ctypes.c_double),
+                                     ctypes.byref(h)))
+        self._h = h
+        self.k = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().ss_graph_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    @staticmethod
+    def bytes_needed(n, m, factor_nnz, k, precision):
+        b = ctypes.c_size_t()
+        check(load().ss_graph_bytes(n, m, factor_nnz, k, precision, ctypes.byref(b)))
+        return b.value
+
+    def laplacian(self):
+        nnz = self.n + 2 * self.m
+        cp = np.zeros(self.n + 1, np.int32)
+        ri = np.zeros(nnz, np.int32)
+        va = np.zeros(nnz, np.float64)
+        check(load().ss_graph_laplacian(self._h, _p(cp, ctypes.c_int32), _p(ri, ctypes.c_int32),
+                                        _p(va, ctypes.c_double)))
+        return cp, ri, va
+
+    def set_factor(self, factor):
+        check(load().ss_graph_set_factor(self._h, factor._h))
+
+    def sparsify(self, k, beta, scale=None, sketch_seed=0, sampling_seed=0, uniform=False, solve=None):
+        o = _lib.SparsifyOpts()
+        o.k = k
+        o.uniform = int(bool(uniform))
+        o.scale = 1.0 / np.sqrt(k) if scale is None else scale
+        o.sketch_seed = sketch_seed
+        o.sampling_seed = sampling_seed
+        o.beta = beta
+        o.solve = solve if solve is not None else solve_opts(stop_rule=_lib.STOP_REL, tol=1e-8)
+        rep = _lib.SparsifyReport()
+        check(load().ss_graph_sparsify(self._h, ctypes.byref(o), ctypes.byref(rep)))
+        self.k = k
+        return {f: getattr(rep, f) for f, _ in _lib.SparsifyReport._fields_}
+
+    def read_kept(self):
+        kept = ctypes.c_int64()
+        cap = self.m
+        ku = np.zeros(max(cap, 1), np.int32)
+        kv = np.zeros(max(cap, 1), np.int32)
+        kw = np.zeros(max(cap, 1), np.float64)
+        check(load().ss_graph_read_kept(self._h, cap, _p(ku, ctypes.c_int32), _p(kv, ctypes.c_int32),
+                                        _p(kw, ctypes.c_double), ctypes.byref(kept)))
+        c = kept.value
+        return ku[:c].copy(), kv[:c].copy(), kw[:c].copy()
+
+    def read_block(self, which):
+        out = np.zeros((self.n, self.k), np.float64)
+        check(load().ss_graph_read_block(self._h, 0 if which == "Y" else 1, _p(out, ctypes.c_double)))
+        return out
+
+    def read_probs(self):
+        p = np.zeros(self.m, np.float64)
+        check(load().ss_graph_read_probs(self._h, _p(p, ctypes.c_double)))
+        return p

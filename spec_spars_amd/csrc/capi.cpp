@@ -1,0 +1,502 @@
+// extern "C" entry points (include/spec_spars_hip.h).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "device.h"
+
+namespace ss {
+static thread_local std::string g_last_error;
+void set_error(const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+}
+void build_incidence(int64_t n, int64_t m, const int32_t *v1, const int32_t *v2, const double *w,
+                     std::vector<int32_t> &iptr, std::vector<int32_t> &ieid, std::vector<double> &ib);
+}  // namespace ss
+
+struct ss_factor {
+    ss::Factor F;
+};
+
+using namespace ss;
+
+#define SS_GUARD_BEGIN try {
+#define SS_GUARD_END                                                        \
+    }                                                                       \
+    catch (const std::bad_alloc &) {                                        \
+        return fail(SS_ERR_NOMEM, "host allocation failed");                \
+    }                                                                       \
+    catch (const std::exception &ex) {                                      \
+        return fail(SS_ERR_INVALID, "unexpected exception: %s", ex.what()); \
+    }
+
+extern "C" {
+
+int ss_last_error(char *buf, size_t len) {
+    const std::string &e = g_last_error;
+    if (buf && len > 0) {
+        size_t c = std::min(len - 1, e.size());
+        memcpy(buf, e.data(), c);
+        buf[c] = 0;
+    }
+    return (int)e.size();
+}
+
+const char *ss_version(void) { return "spec_spars_amd 0.1 (gfx950)"; }
+
+void ss_solve_opts_default(ss_solve_opts *o) {
+    if (!o) return;
+    memset(o, 0, sizeof *o);
+    o->factor_seed = 0;
+    o->stop_rule = SS_STOP_REF_ABS;
+    o->max_iter = 400;
+    o->tol = 2e-7;
+    o->fail_relres = 1e-2;
+    o->precision = SS_PREC_FP64;
+}
+
+// ---- context ---------------------------------------------------------------
+int ss_context_create(int device, ss_context **out) {
+    if (!out) return fail(SS_ERR_INVALID, "null output pointer");
+    SS_GUARD_BEGIN
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) {
+        (void)hipGetLastError();
+        return fail(SS_ERR_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+    }
+    if (device < 0 || device >= count) return fail(SS_ERR_INVALID, "device %d out of range (%d devices)", device, count);
+    SS_HIP(hipSetDevice(device));
+    ss_context *c = new ss_context();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(SS_ERR_HIP, "hipStreamCreate failed");
+    }
+    for (auto &ev : c->ev) (void)hipEventCreate(&ev);
+    if (hipHostMalloc((void **)&c->h_status, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+        delete c;
+        return fail(SS_ERR_NOMEM, "pinned host allocation failed");
+    }
+    *out = c;
+    return SS_OK;
+    SS_GUARD_END
+}
+
+int ss_context_destroy(ss_context *c) {
+    if (!c) return SS_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto &ev : c->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (c->h_status) (void)hipHostFree(c->h_status);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SS_OK;
+}
+
+int ss_context_mem_info(ss_context *c, size_t *free_b, size_t *total_b) {
+    if (!c) return fail(SS_ERR_INVALID, "null context");
+    SS_HIP(hipSetDevice(c->device));
+    size_t f = 0, t = 0;
+    SS_HIP(hipMemGetInfo(&f, &t));
+    if (free_b) *free_b = f;
+    if (total_b) *total_b = t;
+    return SS_OK;
+}
+
+// ---- factor (host) -------------------------------------------------------------
+int ss_factor_build(int64_t n, const int32_t *colptr, const int32_t *rowidx, const double *vals, uint64_t seed,
+                    ss_factor **out) {
+    if (!out || !colptr || !rowidx || !vals) return fail(SS_ERR_INVALID, "null argument");
+    SS_GUARD_BEGIN
+    if (n <= 0) return fail(SS_ERR_INVALID, "n must be positive");
+    if (colptr[0] != 0) return fail(SS_ERR_INVALID, "colptr[0] must be 0");
+    for (int64_t i = 0; i < n; i++)
+        if (colptr[i + 1] < colptr[i]) return fail(SS_ERR_INVALID, "colptr not monotone at %lld", (long long)i);
+    ss_factor *f = new ss_factor();
+    int rc = factor_build(n, colptr, rowidx, vals, seed, f->F);
+    if (rc != SS_OK) {
+        delete f;
+        return rc;
+    }
+    *out = f;
+    return SS_OK;
+    SS_GUARD_END
+}
+
+int ss_factor_info(const ss_factor *f, int64_t *n, int64_t *nnz, int32_t *depth_fwd, int32_t *depth_bwd) {
+    if (!f) return fail(SS_ERR_INVALID, "null factor");
+    if (n) *n = f->F.n;
+    if (nnz) *nnz = f->F.nnz;
+    if (depth_fwd) *depth_fwd = f->F.depth_fwd;
+    if (depth_bwd) *depth_bwd = f->F.depth_bwd;
+    return SS_OK;
+}
+
+int ss_factor_export(const ss_factor *f, int32_t *urowptr, int32_t *ucol, double *uval, double *diag) {
+    if (!f) return fail(SS_ERR_INVALID, "null factor");
+    const Factor &F = f->F;
+    if (urowptr) memcpy(urowptr, F.urowptr.data(), sizeof(int32_t) * F.urowptr.size());
+    if (ucol) memcpy(ucol, F.ucol.data(), sizeof(int32_t) * F.ucol.size());
+    if (uval) memcpy(uval, F.uval.data(), sizeof(double) * F.uval.size());
+    if (diag) memcpy(diag, F.diag.data(), sizeof(double) * F.diag.size());
+    return SS_OK;
+}
+
+int ss_factor_levels(const ss_factor *f, int32_t *lvl_fwd, int32_t *lvl_bwd) {
+    if (!f) return fail(SS_ERR_INVALID, "null factor");
+    if (lvl_fwd) memcpy(lvl_fwd, f->F.lvl_fwd.data(), sizeof(int32_t) * f->F.lvl_fwd.size());
+    if (lvl_bwd) memcpy(lvl_bwd, f->F.lvl_bwd.data(), sizeof(int32_t) * f->F.lvl_bwd.size());
+    return SS_OK;
+}
+
+int ss_factor_destroy(ss_factor *f) {
+    delete f;
+    return SS_OK;
+}
+
+// ---- graph ---------------------------------------------------------------------
+int ss_graph_create(ss_context *ctx, int64_t n, int64_t m, const int32_t *u, const int32_t *v, const double *w,
+                    ss_graph **out) {
+    if (!ctx || !out || (m > 0 && (!u || !v || !w))) return fail(SS_ERR_INVALID, "null argument");
+    SS_GUARD_BEGIN
+    HostGraph G;
+    SS_TRY(graph_build(n, m, u, v, w, G));
+    return graph_upload(ctx, G, out);
+    SS_GUARD_END
+}
+
+int ss_graph_destroy(ss_graph *g) {
+    if (!g) return SS_OK;
+    (void)hipSetDevice(g->ctx->device);
+    (void)hipStreamSynchronize(g->ctx->stream);
+    delete g;
+    return SS_OK;
+}
+
+int ss_graph_bytes(int64_t n, int64_t m, int64_t factor_nnz, int32_t k, int32_t precision, size_t *bytes) {
+    if (!bytes || n <= 0 || m < 0 || k < 1) return fail(SS_ERR_INVALID, "bad arguments");
+    const size_t N = (size_t)n, M = (size_t)m, nnzL = N + 2 * M, F = (size_t)std::max<int64_t>(factor_nnz, n);
+    size_t b = 0;
+    b += M * (4 + 4 + 8);                      // canonical edges
+    b += (N + 1) * 4 + nnzL * (4 + 8 + 4);     // Laplacian CSR (FP64 + FP32 values)
+    b += (N + 1) * 4 + 2 * M * (4 + 8);        // incidence lists
+    b += N * 4 + 8;                            // flags, status
+    b += M * (8 + 8 + 1 + 4 + 4 + 8) + (M / 256 + 2) * 8 + 16;   // sampling
+    b += 2 * ((N + 1) * 4 + N * 4 + (F - N) * (4 + 8 + 4)) + N * 8;  // factor schedules
+    const size_t nk = N * (size_t)k;
+    if (precision == SS_PREC_FP64) b += nk * 8 * 8;
+    else b += nk * (4 * 8 + 8 * 4);
+    b += (size_t)2048 * k * 8 + (size_t)k * 64;
+    *bytes = b;
+    return SS_OK;
+}
+
+int ss_graph_set_factor(ss_graph *g, const ss_factor *f) {
+    if (!g || !f) return fail(SS_ERR_INVALID, "null argument");
+    SS_GUARD_BEGIN
+    return graph_upload_factor(g, f->F);
+    SS_GUARD_END
+}
+
+int ss_graph_laplacian(const ss_graph *g, int32_t *colptr, int32_t *rowidx, double *vals) {
+    if (!g || !colptr || !rowidx || !vals) return fail(SS_ERR_INVALID, "null argument");
+    const int64_t n = g->n, nnz = n + 2 * g->m;
+    SS_HIP(hipSetDevice(g->ctx->device));
+    SS_HIP(hipMemcpy(colptr, g->rowptr, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost));
+    SS_HIP(hipMemcpy(rowidx, g->col, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost));
+    SS_HIP(hipMemcpy(vals, g->val, sizeof(double) * nnz, hipMemcpyDeviceToHost));
+    return SS_OK;
+}
+
+static float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) ms = 0.f;
+    return ms;
+}
+
+int ss_graph_sparsify(ss_graph *g, const ss_sparsify_opts *o, ss_sparsify_report *rep) {
+    if (!g || !o) return fail(SS_ERR_INVALID, "null argument");
+    SS_GUARD_BEGIN
+    if (!g->has_factor) return fail(SS_ERR_INVALID, "graph has no factor (ss_graph_set_factor)");
+    ss_context *c = g->ctx;
+    SS_HIP(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    SS_HIP(hipEventRecord(c->ev[0], s));
+    SS_TRY(graph_sketch(g, o->k, o->sketch_seed, o->scale, o->uniform, o->solve.precision));
+    SS_HIP(hipEventRecord(c->ev[1], s));
+    SolveStats st;
+    SS_TRY(graph_solve(g, o->k, o->solve, st));
+    SS_HIP(hipEventRecord(c->ev[2], s));
+    SS_TRY(graph_sample(g, o->k, o->beta, o->sampling_seed, nullptr, false));
+    SS_HIP(hipEventRecord(c->ev[3], s));
+    SS_HIP(hipEventSynchronize(c->ev[3]));
+    g->last_k = o->k;
+    g->last_prec = o->solve.precision;
+    if (rep) {
+        rep->kept = g->last_kept;
+        rep->max_iters = st.max_iters;
+        rep->refinements = st.refinements;
+        rep->max_relres = st.max_relres;
+        rep->ms_sketch = elapsed(c->ev[0], c->ev[1]);
+        rep->ms_solve = elapsed(c->ev[1], c->ev[2]);
+        rep->ms_sample = elapsed(c->ev[2], c->ev[3]);
+        rep->ms_total = elapsed(c->ev[0], c->ev[3]);
+    }
+    if (st.max_relres >= o->solve.fail_relres)
+        return fail(SS_ERR_NOT_CONVERGED, "solve did not converge: max relative residual %.3e >= %.3e",
+                    st.max_relres, o->solve.fail_relres);
+    return SS_OK;
+    SS_GUARD_END
+}
+
+int ss_graph_read_kept(const ss_graph *g, int64_t cap, int32_t *ku, int32_t *kv, double *kw, int64_t *kept) {
+    if (!g) return fail(SS_ERR_INVALID, "null graph");
+    const int64_t k = g->last_kept;
+    if (kept) *kept = k;
+    if (k > cap) return fail(SS_ERR_CAPACITY, "capacity %lld < kept %lld", (long long)cap, (long long)k);
+    SS_HIP(hipSetDevice(g->ctx->device));
+    if (ku && k) SS_HIP(hipMemcpy(ku, g->ku, sizeof(int32_t) * k, hipMemcpyDeviceToHost));
+    if (kv && k) SS_HIP(hipMemcpy(kv, g->kv, sizeof(int32_t) * k, hipMemcpyDeviceToHost));
+    if (kw && k) SS_HIP(hipMemcpy(kw, g->kw, sizeof(double) * k, hipMemcpyDeviceToHost));
+    return SS_OK;
+}
+
+int ss_graph_read_block(const ss_graph *g, int which, double *out) {
+    if (!g || !out) return fail(SS_ERR_INVALID, "null argument");
+    if (g->wk == 0) return fail(SS_ERR_INVALID, "nothing computed yet");
+    SS_HIP(hipSetDevice(g->ctx->device));
+    const double *src = which == 0 ? g->Y64 : g->X64;
+    SS_HIP(hipMemcpy(out, src, sizeof(double) * (size_t)g->n * g->wk, hipMemcpyDeviceToHost));
+    return SS_OK;
+}
+
+int ss_graph_read_probs(const ss_graph *g, double *p) {
+    if (!g || !p) return fail(SS_ERR_INVALID, "null argument");
+    SS_HIP(hipSetDevice(g->ctx->device));
+    if (g->m) SS_HIP(hipMemcpy(p, g->p, sizeof(double) * g->m, hipMemcpyDeviceToHost));
+    return SS_OK;
+}
+
+// ---- drop-in entry points with host buffers -------------------------------------
+int ss_sketch(ss_context *ctx, const int32_t *v1, const int32_t *v2, const double *w, int64_t m, int64_t n,
+              int32_t k, uint64_t sketch_seed, double scale, int uniform, double *Y) {
+    if (!ctx || !Y || (m > 0 && (!v1 || !v2 || !w))) return fail(SS_ERR_INVALID, "null argument");
+    if (n <= 0 || n >= INT32_MAX || m < 0 || 2 * m >= INT32_MAX) return fail(SS_ERR_INVALID, "bad sizes");
+    if (k < 1 || k > 256) return fail(SS_ERR_INVALID, "k=%d outside 1..256", k);
+    SS_GUARD_BEGIN
+    for (int64_t e = 0; e < m; e++) {
+        if (v1[e] < 0 || v1[e] >= n || v2[e] < 0 || v2[e] >= n || v1[e] == v2[e])
+            return fail(SS_ERR_INVALID, "edge %lld = (%d,%d) invalid (self-loop or out of range)", (long long)e, v1[e], v2[e]);
+        if (!(w[e] > 0.0) || !std::isfinite(w[e]))
+            return fail(SS_ERR_INVALID, "edge %lld has weight %g (must be finite and > 0)", (long long)e, w[e]);
+    }
+    std::vector<int32_t> iptr, ieid;
+    std::vector<double> ib;
+    build_incidence(n, m, v1, v2, w, iptr, ieid, ib);
+    SS_HIP(hipSetDevice(ctx->device));
+    DevArena A;
+    int32_t *d_iptr, *d_ieid;
+    double *d_ib, *d_Y;
+    SS_TRY(A.alloc(&d_iptr, n + 1));
+    SS_TRY(A.alloc(&d_ieid, 2 * m));
+    SS_TRY(A.alloc(&d_ib, 2 * m));
+    SS_TRY(A.alloc(&d_Y, (size_t)n * k));
+    hipStream_t s = ctx->stream;
+    SS_HIP(hipMemcpyAsync(d_iptr, iptr.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, s));
+    if (m) {
+        SS_HIP(hipMemcpyAsync(d_ieid, ieid.data(), sizeof(int32_t) * 2 * m, hipMemcpyHostToDevice, s));
+        SS_HIP(hipMemcpyAsync(d_ib, ib.data(), sizeof(double) * 2 * m, hipMemcpyHostToDevice, s));
+    }
+    launch_sketch<double>(s, (int32_t)n, d_iptr, d_ieid, d_ib, k, sketch_seed, scale, uniform, 0, d_Y);
+    SS_HIP(hipGetLastError());
+    SS_HIP(hipMemcpyAsync(Y, d_Y, sizeof(double) * (size_t)n * k, hipMemcpyDeviceToHost, s));
+    SS_HIP(hipStreamSynchronize(s));
+    return SS_OK;
+    SS_GUARD_END
+}
+
+// Canonical edges of a CSC Laplacian's strict upper triangle.
+static int csc_upper_edges(int64_t n, const int32_t *colptr, const int32_t *rowidx, const double *vals,
+                           std::vector<int32_t> &u, std::vector<int32_t> &v, std::vector<double> &w) {
+    if (colptr[0] != 0) return fail(SS_ERR_INVALID, "colptr[0] must be 0");
+    for (int64_t c = 0; c < n; c++) {
+        if (colptr[c + 1] < colptr[c]) return fail(SS_ERR_INVALID, "colptr not monotone");
+        for (int32_t q = colptr[c]; q < colptr[c + 1]; q++) {
+            int32_t r = rowidx[q];
+            if (r < 0 || r >= n) return fail(SS_ERR_INVALID, "row index %d out of range", r);
+            if (r < c) {
+                u.push_back(r);
+                v.push_back((int32_t)c);
+                w.push_back(-1.0 * vals[q]);
+            }
+        }
+    }
+    return SS_OK;
+}
+
+int ss_solve_lap(ss_context *ctx, const double *Y, int64_t n, int32_t k, const int32_t *colptr,
+                 const int32_t *rowidx, const double *vals, int64_t nnz, const ss_solve_opts *opts, double *Z,
+                 ss_solve_report *rep) {
+    if (!ctx || !Y || !colptr || !rowidx || !vals || !Z) return fail(SS_ERR_INVALID, "null argument");
+    if (n <= 0 || k < 1 || k > 256) return fail(SS_ERR_INVALID, "bad sizes n=%lld k=%d", (long long)n, k);
+    SS_GUARD_BEGIN
+    if ((int64_t)colptr[n] != nnz) return fail(SS_ERR_INVALID, "colptr[n]=%d does not match nnz=%lld", colptr[n], (long long)nnz);
+    ss_solve_opts o;
+    if (opts) o = *opts;
+    else ss_solve_opts_default(&o);
+    std::vector<int32_t> u, v;
+    std::vector<double> w;
+    SS_TRY(csc_upper_edges(n, colptr, rowidx, vals, u, v, w));
+    HostGraph G;
+    SS_TRY(graph_build(n, (int64_t)u.size(), u.data(), v.data(), w.data(), G));
+    Factor F;
+    SS_TRY(factor_build(n, colptr, rowidx, vals, o.factor_seed, F));
+    ss_graph *g = nullptr;
+    SS_TRY(graph_upload(ctx, G, &g));
+    struct Del {
+        ss_graph *g;
+        ~Del() { ss_graph_destroy(g); }
+    } del{g};
+    SS_TRY(graph_upload_factor(g, F));
+    SS_TRY(graph_ensure_work(g, k, o.precision));
+    hipStream_t s = ctx->stream;
+    SS_HIP(hipMemcpyAsync(g->Y64, Y, sizeof(double) * (size_t)n * k, hipMemcpyHostToDevice, s));
+    SS_HIP(hipEventRecord(ctx->ev[4], s));
+    SolveStats st;
+    SS_TRY(graph_solve(g, k, o, st));
+    SS_HIP(hipEventRecord(ctx->ev[5], s));
+    SS_HIP(hipMemcpyAsync(Z, g->X64, sizeof(double) * (size_t)n * k, hipMemcpyDeviceToHost, s));
+    SS_HIP(hipStreamSynchronize(s));
+    if (rep) {
+        if (rep->iters) memcpy(rep->iters, st.iters.data(), sizeof(int32_t) * k);
+        if (rep->relres) memcpy(rep->relres, st.relres.data(), sizeof(double) * k);
+        rep->factor_nnz = F.nnz;
+        rep->depth_fwd = F.depth_fwd;
+        rep->depth_bwd = F.depth_bwd;
+        rep->max_iters = st.max_iters;
+        rep->refinements = st.refinements;
+        rep->max_relres = st.max_relres;
+        rep->ms_factor = F.ms_build;
+        rep->ms_solve = elapsed(ctx->ev[4], ctx->ev[5]);
+    }
+    if (st.max_relres >= o.fail_relres)
+        return fail(SS_ERR_NOT_CONVERGED, "solve did not converge: max relative residual %.3e >= %.3e",
+                    st.max_relres, o.fail_relres);
+    return SS_OK;
+    SS_GUARD_END
+}
+
+int ss_er_sample(ss_context *ctx, const double *Z, int64_t n, int32_t k, const int32_t *u, const int32_t *v,
+                 const double *lval, int64_t m, double beta, uint64_t sampling_seed, const double *p_in,
+                 double *p_out, double *d_out, uint8_t *mask_out, int64_t *kept, int32_t *ku, int32_t *kv,
+                 double *kw) {
+    if (!ctx || (m > 0 && (!u || !v || !lval))) return fail(SS_ERR_INVALID, "null argument");
+    if (!p_in && !Z) return fail(SS_ERR_INVALID, "need Z or p_in");
+    if (n <= 0 || k < 1 || m < 0) return fail(SS_ERR_INVALID, "bad sizes");
+    SS_GUARD_BEGIN
+    for (int64_t e = 0; e < m; e++) {
+        if (u[e] < 0 || v[e] >= n || u[e] >= v[e])
+            return fail(SS_ERR_INVALID, "edge %lld = (%d,%d) is not u < v < n", (long long)e, u[e], v[e]);
+        if (!(lval[e] < 0.0)) return fail(SS_ERR_INVALID, "edge %lld has L_uv=%g (must be < 0)", (long long)e, lval[e]);
+        if (p_in && !(p_in[e] >= 0.0 && p_in[e] <= 1.0))
+            return fail(SS_ERR_INVALID, "probability %g of edge %lld outside [0,1]", p_in[e], (long long)e);
+    }
+    SS_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    DevArena A;
+    int32_t *d_u, *d_v, *d_ku, *d_kv;
+    double *d_w, *d_Z = nullptr, *d_pin = nullptr, *d_p, *d_d, *d_kw;
+    uint8_t *d_mask;
+    uint64_t *d_tiles;
+    uint32_t *d_ticket;
+    int64_t *d_kept;
+    int *d_err;
+    const int64_t ntiles = (m + 255) / 256;
+    SS_TRY(A.alloc(&d_u, m));
+    SS_TRY(A.alloc(&d_v, m));
+    SS_TRY(A.alloc(&d_w, m));
+    SS_TRY(A.alloc(&d_p, m));
+    SS_TRY(A.alloc(&d_d, m));
+    SS_TRY(A.alloc(&d_mask, m));
+    SS_TRY(A.alloc(&d_ku, m));
+    SS_TRY(A.alloc(&d_kv, m));
+    SS_TRY(A.alloc(&d_kw, m));
+    SS_TRY(A.alloc(&d_tiles, ntiles + 1));
+    SS_TRY(A.alloc(&d_ticket, 1));
+    SS_TRY(A.alloc(&d_kept, 1));
+    SS_TRY(A.alloc(&d_err, 1));
+    std::vector<double> w((size_t)m);
+    for (int64_t e = 0; e < m; e++) w[e] = -1.0 * lval[e];
+    if (m) {
+        SS_HIP(hipMemcpyAsync(d_u, u, sizeof(int32_t) * m, hipMemcpyHostToDevice, s));
+        SS_HIP(hipMemcpyAsync(d_v, v, sizeof(int32_t) * m, hipMemcpyHostToDevice, s));
+        SS_HIP(hipMemcpyAsync(d_w, w.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
+    }
+    if (p_in) {
+        SS_TRY(A.alloc(&d_pin, m));
+        if (m) SS_HIP(hipMemcpyAsync(d_pin, p_in, sizeof(double) * m, hipMemcpyHostToDevice, s));
+    } else {
+        SS_TRY(A.alloc(&d_Z, (size_t)n * k));
+        SS_HIP(hipMemcpyAsync(d_Z, Z, sizeof(double) * (size_t)n * k, hipMemcpyHostToDevice, s));
+    }
+    SS_HIP(hipMemsetAsync(d_tiles, 0, sizeof(uint64_t) * (ntiles + 1), s));
+    SS_HIP(hipMemsetAsync(d_ticket, 0, sizeof(uint32_t), s));
+    SS_HIP(hipMemsetAsync(d_kept, 0, sizeof(int64_t), s));
+    SS_HIP(hipMemsetAsync(d_err, 0, sizeof(int), s));
+    int64_t h_kept = 0;
+    int h_err = 0;
+    if (m) {
+        ErSampleArgs<double> a;
+        a.m = m;
+        a.ntiles = ntiles;
+        a.eu = d_u;
+        a.ev = d_v;
+        a.ew = d_w;
+        a.Z = d_Z;
+        a.k = k;
+        a.beta = beta;
+        a.ck = coin_key_from_u64(sampling_seed);
+        a.p_in = d_pin;
+        a.p_out = d_p;
+        a.d_out = p_in ? nullptr : d_d;
+        a.mask = d_mask;
+        a.ku = d_ku;
+        a.kv = d_kv;
+        a.kw = d_kw;
+        a.ticket = d_ticket;
+        a.tile_state = d_tiles;
+        a.epoch = 1;
+        a.kept_total = d_kept;
+        a.err = d_err;
+        a.spin_limit = 1 << 22;
+        launch_er_sample<double>(s, a);
+        SS_HIP(hipGetLastError());
+    }
+    SS_HIP(hipMemcpyAsync(&h_kept, d_kept, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    SS_HIP(hipMemcpyAsync(&h_err, d_err, sizeof(int), hipMemcpyDeviceToHost, s));
+    SS_HIP(hipStreamSynchronize(s));
+    if (h_err) return fail(SS_ERR_TIMEOUT, "device wait exceeded its spin limit (compaction look-back)");
+    if (kept) *kept = h_kept;
+    if (m) {
+        if (p_out) SS_HIP(hipMemcpy(p_out, d_p, sizeof(double) * m, hipMemcpyDeviceToHost));
+        if (d_out && !p_in) SS_HIP(hipMemcpy(d_out, d_d, sizeof(double) * m, hipMemcpyDeviceToHost));
+        if (mask_out) SS_HIP(hipMemcpy(mask_out, d_mask, m, hipMemcpyDeviceToHost));
+        if (ku && h_kept) SS_HIP(hipMemcpy(ku, d_ku, sizeof(int32_t) * h_kept, hipMemcpyDeviceToHost));
+        if (kv && h_kept) SS_HIP(hipMemcpy(kv, d_kv, sizeof(int32_t) * h_kept, hipMemcpyDeviceToHost));
+        if (kw && h_kept) SS_HIP(hipMemcpy(kw, d_kw, sizeof(double) * h_kept, hipMemcpyDeviceToHost));
+    }
+    return SS_OK;
+    SS_GUARD_END
+}
+
+}  // extern "C"

@@ -1,0 +1,418 @@
+// Device-resident graph, multi-RHS PCG driver and the sparsify pipeline.
+//
+// The PCG is the method MKL's dcg runs for one column at a time in
+// include/custom_cg.hpp:95-274 (x0 = 0, b projected to zero mean, stop test
+// after every x/r update, M^{-1} = L^{-T} D^+ L^{-1}), run here on all k
+// columns at once: every column keeps its own alpha/beta/rz in FP64 and a
+// convergence mask that freezes it (alpha = 0) once its stop rule holds.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "device.h"
+
+namespace ss {
+
+static const int kSpinLimit = 1 << 22;   // polls per wait (~seconds)
+
+// ---------------------------------------------------------------------------
+// structure upload
+// ---------------------------------------------------------------------------
+void build_incidence(int64_t n, int64_t m, const int32_t *v1, const int32_t *v2, const double *w,
+                     std::vector<int32_t> &iptr, std::vector<int32_t> &ieid, std::vector<double> &ib) {
+    iptr.assign((size_t)n + 1, 0);
+    for (int64_t e = 0; e < m; e++) {
+        iptr[v1[e] + 1]++;
+        iptr[v2[e] + 1]++;
+    }
+    for (int64_t i = 0; i < n; i++) iptr[i + 1] += iptr[i];
+    ieid.resize((size_t)2 * m);
+    ib.resize((size_t)2 * m);
+    std::vector<int32_t> pos(iptr.begin(), iptr.end() - 1);
+    for (int64_t e = 0; e < m; e++) {
+        // B[v1,e] = sqrt(w) * -1, B[v2,e] = -1 * (sqrt(w) * -1)
+        // (src/sparsifier.rs:97-110)
+        const double nv = std::sqrt(-1.0 * (-w[e])) * -1.0;
+        int32_t p1 = pos[v1[e]]++;
+        ieid[p1] = (int32_t)e;
+        ib[p1] = nv;
+        int32_t p2 = pos[v2[e]]++;
+        ieid[p2] = (int32_t)e;
+        ib[p2] = -1.0 * nv;
+    }
+}
+
+template <typename T>
+static int h2d(hipStream_t s, T *dst, const T *src, size_t count) {
+    if (count == 0) return SS_OK;
+    SS_HIP(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+    return SS_OK;
+}
+
+int graph_upload(ss_context *ctx, const HostGraph &G, ss_graph **out) {
+    SS_HIP(hipSetDevice(ctx->device));
+    ss_graph *g = new ss_graph();
+    g->ctx = ctx;
+    g->n = G.n;
+    g->m = G.m;
+    const int64_t n = G.n, m = G.m, nnz = G.rowptr[n];
+    auto &A = g->arena_struct;
+    int rc = SS_OK;
+    auto bail = [&](int code) { delete g; return code; };
+    if ((rc = A.alloc(&g->eu, m)) || (rc = A.alloc(&g->ev, m)) || (rc = A.alloc(&g->ew, m)) ||
+        (rc = A.alloc(&g->rowptr, n + 1)) || (rc = A.alloc(&g->col, nnz)) || (rc = A.alloc(&g->val, nnz)) ||
+        (rc = A.alloc(&g->valf, nnz)) || (rc = A.alloc(&g->iptr, n + 1)) || (rc = A.alloc(&g->ieid, 2 * m)) ||
+        (rc = A.alloc(&g->ib, 2 * m)) || (rc = A.alloc(&g->flags, n)) || (rc = A.alloc(&g->status, 2)) ||
+        (rc = A.alloc(&g->p, m)) || (rc = A.alloc(&g->dnorm, m)) || (rc = A.alloc(&g->mask, m)) ||
+        (rc = A.alloc(&g->ku, m)) || (rc = A.alloc(&g->kv, m)) || (rc = A.alloc(&g->kw, m)) ||
+        (rc = A.alloc(&g->tile_state, (m + 255) / 256 + 1)) || (rc = A.alloc(&g->ticket, 1)) ||
+        (rc = A.alloc(&g->kept, 1)))
+        return bail(rc);
+    hipStream_t s = ctx->stream;
+    // canonical edges back from the CSR's upper part
+    std::vector<int32_t> eu((size_t)m), ev((size_t)m);
+    std::vector<double> ew((size_t)m);
+    for (int64_t r = 0; r < n; r++)
+        for (int32_t q = G.rowptr[r]; q < G.rowptr[r + 1]; q++)
+            if (G.col[q] > r) {
+                int32_t e = G.eid[q];
+                eu[e] = (int32_t)r;
+                ev[e] = G.col[q];
+                ew[e] = -1.0 * G.val[q];
+            }
+    std::vector<int32_t> iptr, ieid;
+    std::vector<double> ib;
+    build_incidence(n, m, eu.data(), ev.data(), ew.data(), iptr, ieid, ib);
+    std::vector<float> valf(G.val.begin(), G.val.end());
+    if ((rc = h2d(s, g->eu, eu.data(), m)) || (rc = h2d(s, g->ev, ev.data(), m)) ||
+        (rc = h2d(s, g->ew, ew.data(), m)) || (rc = h2d(s, g->rowptr, G.rowptr.data(), n + 1)) ||
+        (rc = h2d(s, g->col, G.col.data(), nnz)) || (rc = h2d(s, g->val, G.val.data(), nnz)) ||
+        (rc = h2d(s, g->valf, valf.data(), nnz)) || (rc = h2d(s, g->iptr, iptr.data(), n + 1)) ||
+        (rc = h2d(s, g->ieid, ieid.data(), 2 * m)) || (rc = h2d(s, g->ib, ib.data(), 2 * m)))
+        return bail(rc);
+    hipError_t e1 = hipMemsetAsync(g->flags, 0, sizeof(uint32_t) * (size_t)n, s);
+    hipError_t e2 = hipMemsetAsync(g->status, 0, sizeof(int) * 2, s);
+    hipError_t e3 = hipMemsetAsync(g->tile_state, 0, sizeof(uint64_t) * (size_t)((m + 255) / 256 + 1), s);
+    hipError_t e4 = hipStreamSynchronize(s);
+    if (e1 || e2 || e3 || e4) {
+        delete g;
+        return fail(SS_ERR_HIP, "graph upload: %s", hipGetErrorString(e1 ? e1 : e2 ? e2 : e3 ? e3 : e4));
+    }
+    *out = g;
+    return SS_OK;
+}
+
+int graph_upload_factor(ss_graph *g, const Factor &F) {
+    if (F.n != g->n) return fail(SS_ERR_INVALID, "factor has n=%lld, graph has n=%lld", (long long)F.n, (long long)g->n);
+    SS_HIP(hipSetDevice(g->ctx->device));
+    Schedule fw, bw;
+    factor_schedules(F, fw, bw);
+    g->arena_factor.release();
+    g->has_factor = false;
+    auto &A = g->arena_factor;
+    const int64_t n = g->n;
+    g->f_nnz = (int64_t)fw.col.size();
+    g->b_nnz = (int64_t)bw.col.size();
+    SS_TRY(A.alloc(&g->f_rows, n));
+    SS_TRY(A.alloc(&g->f_ptr, n + 1));
+    SS_TRY(A.alloc(&g->f_col, g->f_nnz));
+    SS_TRY(A.alloc(&g->f_val, g->f_nnz));
+    SS_TRY(A.alloc(&g->f_valf, g->f_nnz));
+    SS_TRY(A.alloc(&g->b_rows, n));
+    SS_TRY(A.alloc(&g->b_ptr, n + 1));
+    SS_TRY(A.alloc(&g->b_col, g->b_nnz));
+    SS_TRY(A.alloc(&g->b_val, g->b_nnz));
+    SS_TRY(A.alloc(&g->b_valf, g->b_nnz));
+    SS_TRY(A.alloc(&g->diag, n));
+    hipStream_t s = g->ctx->stream;
+    std::vector<float> fvf(fw.val.begin(), fw.val.end()), bvf(bw.val.begin(), bw.val.end());
+    SS_TRY(h2d(s, g->f_rows, fw.rows.data(), n));
+    SS_TRY(h2d(s, g->f_ptr, fw.ptr.data(), n + 1));
+    SS_TRY(h2d(s, g->f_col, fw.col.data(), g->f_nnz));
+    SS_TRY(h2d(s, g->f_val, fw.val.data(), g->f_nnz));
+    SS_TRY(h2d(s, g->f_valf, fvf.data(), g->f_nnz));
+    SS_TRY(h2d(s, g->b_rows, bw.rows.data(), n));
+    SS_TRY(h2d(s, g->b_ptr, bw.ptr.data(), n + 1));
+    SS_TRY(h2d(s, g->b_col, bw.col.data(), g->b_nnz));
+    SS_TRY(h2d(s, g->b_val, bw.val.data(), g->b_nnz));
+    SS_TRY(h2d(s, g->b_valf, bvf.data(), g->b_nnz));
+    SS_TRY(h2d(s, g->diag, F.diag.data(), n));
+    SS_HIP(hipStreamSynchronize(s));
+    g->factor_nnz = F.nnz;
+    g->depth_fwd = F.depth_fwd;
+    g->depth_bwd = F.depth_bwd;
+    g->has_factor = true;
+    return SS_OK;
+}
+
+int graph_ensure_work(ss_graph *g, int k, int prec) {
+    if (k < 1 || k > 256) return fail(SS_ERR_INVALID, "k=%d outside 1..256", k);
+    if (prec != SS_PREC_FP64 && prec != SS_PREC_MIXED) return fail(SS_ERR_INVALID, "unknown precision %d", prec);
+    if (g->wk == k && g->wprec == prec) return SS_OK;
+    SS_HIP(hipSetDevice(g->ctx->device));
+    g->arena_work.release();
+    g->wk = 0;
+    g->wprec = -1;
+    g->Y64 = g->X64 = g->R64 = g->B64 = nullptr;
+    g->Y32 = g->X32 = nullptr;
+    g->w64 = PcgWork<double>();
+    g->w32 = PcgWork<float>();
+    auto &A = g->arena_work;
+    const size_t nk = (size_t)g->n * (size_t)k;
+    SS_TRY(A.alloc(&g->Y64, nk));
+    SS_TRY(A.alloc(&g->X64, nk));
+    if (prec == SS_PREC_FP64) {
+        auto &w = g->w64;
+        SS_TRY(A.alloc(&w.B, nk));
+        SS_TRY(A.alloc(&w.R, nk));
+        SS_TRY(A.alloc(&w.P, nk));
+        SS_TRY(A.alloc(&w.Q, nk));
+        SS_TRY(A.alloc(&w.Zp, nk));
+        SS_TRY(A.alloc(&w.W, nk));
+    } else {
+        SS_TRY(A.alloc(&g->B64, nk));
+        SS_TRY(A.alloc(&g->R64, nk));
+        SS_TRY(A.alloc(&g->Y32, nk));
+        SS_TRY(A.alloc(&g->X32, nk));
+        auto &w = g->w32;
+        SS_TRY(A.alloc(&w.B, nk));
+        SS_TRY(A.alloc(&w.R, nk));
+        SS_TRY(A.alloc(&w.P, nk));
+        SS_TRY(A.alloc(&w.Q, nk));
+        SS_TRY(A.alloc(&w.Zp, nk));
+        SS_TRY(A.alloc(&w.W, nk));
+    }
+    SS_TRY(A.alloc(&g->partial, (size_t)1024 * k));
+    SS_TRY(A.alloc(&g->partial2, (size_t)1024 * k));
+    SS_TRY(A.alloc(&g->colbuf, (size_t)4 * k));
+    SS_TRY(A.alloc(&g->sc_buf, (size_t)5 * k));
+    SS_TRY(A.alloc(&g->sc_int, (size_t)2 * k));
+    g->sc.rz = g->sc_buf;
+    g->sc.alpha = g->sc_buf + k;
+    g->sc.beta = g->sc_buf + 2 * k;
+    g->sc.rr = g->sc_buf + 3 * k;
+    g->sc.bnorm = g->sc_buf + 4 * k;
+    g->sc.active = g->sc_int;
+    g->sc.iters = g->sc_int + k;
+    g->wk = k;
+    g->wprec = prec;
+    return SS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// preconditioner and PCG
+// ---------------------------------------------------------------------------
+template <typename T> static const T *fvals(const ss_graph *g);
+template <> const double *fvals<double>(const ss_graph *g) { return g->f_val; }
+template <> const float *fvals<float>(const ss_graph *g) { return g->f_valf; }
+template <typename T> static const T *bvals(const ss_graph *g);
+template <> const double *bvals<double>(const ss_graph *g) { return g->b_val; }
+template <> const float *bvals<float>(const ss_graph *g) { return g->b_valf; }
+template <typename T> static const T *avals(const ss_graph *g);
+template <> const double *avals<double>(const ss_graph *g) { return g->val; }
+template <> const float *avals<float>(const ss_graph *g) { return g->valf; }
+
+static uint32_t next_epoch(ss_graph *g) {
+    if (g->epoch >= 0xfffffff0u) {
+        launch_fill_u32(g->ctx->stream, g->n, g->flags, 0u);
+        g->epoch = 0;
+    }
+    return ++g->epoch;
+}
+
+// Zp = L^{-T} D^+ L^{-1} R   (W holds the forward result)
+template <typename T>
+static int precond(ss_graph *g, int k, const T *R, T *W, T *Zp) {
+    hipStream_t s = g->ctx->stream;
+    const int32_t n = (int32_t)g->n;
+    uint32_t e1 = next_epoch(g);
+    SS_HIP((launch_trisolve<T, false>(s, g->ctx->device, n, g->f_rows, g->f_ptr, g->f_col, fvals<T>(g), g->diag,
+                                      k, R, W, g->flags, e1, g->status + 1, kSpinLimit)));
+    uint32_t e2 = next_epoch(g);
+    SS_HIP((launch_trisolve<T, true>(s, g->ctx->device, n, g->b_rows, g->b_ptr, g->b_col, bvals<T>(g), g->diag, k,
+                                     W, Zp, g->flags, e2, g->status + 1, kSpinLimit)));
+    return SS_OK;
+}
+
+static int read_status(ss_graph *g, int *nactive) {
+    ss_context *c = g->ctx;
+    SS_HIP(hipMemcpyAsync(c->h_status, g->status, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    SS_HIP(hipStreamSynchronize(c->stream));
+    if (c->h_status[1] != 0) {
+        (void)hipMemsetAsync(g->status, 0, 2 * sizeof(int), c->stream);
+        (void)hipStreamSynchronize(c->stream);
+        return fail(SS_ERR_TIMEOUT, "device wait exceeded its spin limit (triangular solve dependency)");
+    }
+    if (nactive) *nactive = c->h_status[0];
+    return SS_OK;
+}
+
+// PCG on all k columns: X = approx A^+ (rhs - mean).  Work in precision T,
+// dots and scalars in FP64.  Writes per-column iteration counts to iters.
+template <typename T>
+static int pcg_run(ss_graph *g, int k, const T *rhs, T *X, const PcgWork<T> &w, int rule, double tol,
+                   int max_iter, std::vector<int32_t> &iters, std::vector<double> &bnorm) {
+    hipStream_t s = g->ctx->stream;
+    const int32_t n = (int32_t)g->n;
+    const int64_t nk = (int64_t)n * k;
+    const int nb = partial_blocks(n);
+    // b - mean(b)   (include/custom_cg.hpp:80-93, :131-132)
+    launch_col_partial<T, T>(s, n, k, rhs, rhs, 1, g->partial);
+    launch_col_mean(s, k, g->partial, nb, n, g->colbuf);
+    launch_project<T>(s, nk, k, rhs, g->colbuf, w.R, w.B, X);
+    SS_TRY(precond<T>(g, k, w.R, w.W, w.Zp));
+    launch_col_partial<T, T>(s, n, k, w.B, w.B, 0, g->partial);
+    launch_col_partial<T, T>(s, n, k, w.R, w.Zp, 0, g->partial2);
+    launch_pcg_init(s, k, g->partial, g->partial2, nb, g->sc);
+    SS_HIP(hipMemcpyAsync(w.P, w.Zp, sizeof(T) * (size_t)nk, hipMemcpyDeviceToDevice, s));
+    for (int it = 0; it < max_iter; it++) {
+        launch_spmm<T>(s, n, g->rowptr, g->col, avals<T>(g), k, w.P, w.Q);
+        launch_col_partial<T, T>(s, n, k, w.P, w.Q, 0, g->partial);
+        launch_pcg_alpha(s, k, g->partial, nb, g->sc);
+        launch_update_xr<T>(s, nk, k, g->sc.alpha, w.P, w.Q, X, w.R);
+        launch_col_partial<T, T>(s, n, k, w.R, w.R, 0, g->partial);
+        launch_pcg_check(s, k, g->partial, nb, rule, tol, g->sc, g->status);
+        int nactive = 0;
+        SS_TRY(read_status(g, &nactive));
+        if (nactive == 0) break;
+        SS_TRY(precond<T>(g, k, w.R, w.W, w.Zp));
+        launch_col_partial<T, T>(s, n, k, w.R, w.Zp, 0, g->partial);
+        launch_pcg_beta(s, k, g->partial, nb, g->sc);
+        launch_update_p<T>(s, nk, k, g->sc.beta, w.Zp, w.P);
+    }
+    SS_TRY(read_status(g, nullptr));
+    iters.resize(k);
+    bnorm.resize(k);
+    SS_HIP(hipMemcpyAsync(iters.data(), g->sc.iters, sizeof(int32_t) * k, hipMemcpyDeviceToHost, s));
+    SS_HIP(hipMemcpyAsync(bnorm.data(), g->sc.bnorm, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+    SS_HIP(hipStreamSynchronize(s));
+    return SS_OK;
+}
+
+// per-column ||M||^2 of an n x k FP64 block, to the host
+static int col_norms2(ss_graph *g, int k, const double *M, std::vector<double> &out) {
+    hipStream_t s = g->ctx->stream;
+    const int32_t n = (int32_t)g->n;
+    const int nb = partial_blocks(n);
+    launch_col_partial<double, double>(s, n, k, M, M, 0, g->partial);
+    launch_col_mean(s, k, g->partial, nb, 1, g->colbuf);
+    out.resize(k);
+    SS_HIP(hipMemcpyAsync(out.data(), g->colbuf, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+    SS_HIP(hipStreamSynchronize(s));
+    return SS_OK;
+}
+
+int graph_solve(ss_graph *g, int k, const ss_solve_opts &o, SolveStats &st) {
+    if (!g->has_factor) return fail(SS_ERR_INVALID, "graph has no factor (ss_graph_set_factor)");
+    if (g->wk != k) return fail(SS_ERR_INVALID, "work not allocated for k=%d", k);
+    if (o.max_iter < 0 || !(o.tol >= 0.0)) return fail(SS_ERR_INVALID, "bad solve options");
+    SS_HIP(hipSetDevice(g->ctx->device));
+    hipStream_t s = g->ctx->stream;
+    const int32_t n = (int32_t)g->n;
+    const int64_t nk = (int64_t)n * k;
+    std::vector<double> bnorm, rn2;
+    st.iters.assign(k, 0);
+    st.relres.assign(k, 0.0);
+    st.refinements = 0;
+    if (g->wprec == SS_PREC_FP64) {
+        SS_TRY(pcg_run<double>(g, k, g->Y64, g->X64, g->w64, o.stop_rule, o.tol, o.max_iter, st.iters, bnorm));
+        // true residual b - A x  (include/custom_cg.hpp:245-258)
+        launch_residual64(s, n, g->rowptr, g->col, g->val, k, g->X64, g->w64.B, g->w64.Q);
+        SS_TRY(col_norms2(g, k, g->w64.Q, rn2));
+    } else {
+        // FP64 outer refinement around an FP32 PCG.
+        const int nb = partial_blocks(n);
+        launch_col_partial<double, double>(s, n, k, g->Y64, g->Y64, 1, g->partial);
+        launch_col_mean(s, k, g->partial, nb, n, g->colbuf);
+        launch_project<double>(s, nk, k, g->Y64, g->colbuf, g->R64, g->B64, g->X64);
+        std::vector<double> bn2;
+        SS_TRY(col_norms2(g, k, g->B64, bn2));
+        bnorm.resize(k);
+        for (int j = 0; j < k; j++) bnorm[j] = std::sqrt(bn2[j]);
+        const double inner_tol = std::max(1e-5, o.stop_rule == SS_STOP_REL ? o.tol : 0.0);
+        std::vector<int32_t> it_sweep;
+        std::vector<double> bn_inner;
+        rn2 = bn2;
+        for (int sweep = 0; sweep < 8; sweep++) {
+            bool all_done = true;
+            for (int j = 0; j < k; j++) {
+                double r = std::sqrt(rn2[j]);
+                bool done = o.stop_rule == SS_STOP_REL ? r <= o.tol * bnorm[j] : rn2[j] <= o.tol;
+                if (!done) all_done = false;
+            }
+            if (all_done) break;
+            launch_convert<double, float>(s, nk, g->R64, g->Y32);
+            SS_TRY(pcg_run<float>(g, k, g->Y32, g->X32, g->w32, SS_STOP_REL, inner_tol, o.max_iter, it_sweep,
+                                  bn_inner));
+            for (int j = 0; j < k; j++) st.iters[j] += it_sweep[j];
+            launch_accumulate<float>(s, nk, g->X32, g->X64);
+            launch_residual64(s, n, g->rowptr, g->col, g->val, k, g->X64, g->B64, g->R64);
+            SS_TRY(col_norms2(g, k, g->R64, rn2));
+            st.refinements++;
+        }
+    }
+    st.max_iters = 0;
+    st.max_relres = 0.0;
+    for (int j = 0; j < k; j++) {
+        st.relres[j] = bnorm[j] > 0.0 ? std::sqrt(rn2[j]) / bnorm[j] : 0.0;
+        st.max_iters = std::max(st.max_iters, st.iters[j]);
+        st.max_relres = std::max(st.max_relres, st.relres[j]);
+    }
+    return SS_OK;
+}
+
+int graph_sketch(ss_graph *g, int k, uint64_t seed, double scale, int uniform, int prec) {
+    SS_TRY(graph_ensure_work(g, k, prec));
+    SS_HIP(hipSetDevice(g->ctx->device));
+    launch_sketch<double>(g->ctx->stream, (int32_t)g->n, g->iptr, g->ieid, g->ib, k, seed, scale, uniform, 0,
+                          g->Y64);
+    SS_HIP(hipGetLastError());
+    return SS_OK;
+}
+
+int graph_sample(ss_graph *g, int k, double beta, uint64_t sampling_seed, const double *p_in_dev, bool write_d) {
+    SS_HIP(hipSetDevice(g->ctx->device));
+    hipStream_t s = g->ctx->stream;
+    if (g->samp_epoch >= (1u << 24) - 2) {
+        SS_HIP(hipMemsetAsync(g->tile_state, 0, sizeof(uint64_t) * (size_t)((g->m + 255) / 256 + 1), s));
+        g->samp_epoch = 0;
+    }
+    g->samp_epoch++;
+    launch_fill_u32(s, 1, g->ticket, 0u);
+    ErSampleArgs<double> a;
+    a.m = g->m;
+    a.ntiles = (g->m + 255) / 256;
+    a.eu = g->eu;
+    a.ev = g->ev;
+    a.ew = g->ew;
+    a.Z = g->X64;
+    a.k = k;
+    a.beta = beta;
+    a.ck = coin_key_from_u64(sampling_seed);
+    a.p_in = p_in_dev;
+    a.p_out = g->p;
+    a.d_out = write_d ? g->dnorm : nullptr;
+    a.mask = g->mask;
+    a.ku = g->ku;
+    a.kv = g->kv;
+    a.kw = g->kw;
+    a.ticket = g->ticket;
+    a.tile_state = g->tile_state;
+    a.epoch = g->samp_epoch;
+    a.kept_total = g->kept;
+    a.err = g->status + 1;
+    a.spin_limit = kSpinLimit;
+    if (g->m == 0) {
+        g->last_kept = 0;
+        return SS_OK;
+    }
+    launch_er_sample<double>(s, a);
+    SS_HIP(hipGetLastError());
+    SS_HIP(hipMemcpyAsync(&g->last_kept, g->kept, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    SS_TRY(read_status(g, nullptr));
+    return SS_OK;
+}
+
+}  // namespace ss

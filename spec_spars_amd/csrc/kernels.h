@@ -1,0 +1,82 @@
+// Launchers of the gfx950 kernels (kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hash_dev.h"
+
+namespace ss {
+
+// Per-column PCG state, FP64, resident on the device.
+struct PcgScalars {
+    double *rz;      // r . z
+    double *alpha;
+    double *beta;
+    double *rr;      // ||r||^2 after the last update
+    double *bnorm;   // ||b|| (projected)
+    int *active;     // 1 while the column iterates
+    int *iters;      // updates applied
+};
+
+template <typename TZ>
+struct ErSampleArgs {
+    int64_t m;
+    int64_t ntiles;
+    const int32_t *eu;
+    const int32_t *ev;
+    const double *ew;
+    const TZ *Z;
+    int k;
+    double beta;
+    CoinKey ck;
+    const double *p_in;
+    double *p_out;
+    double *d_out;
+    uint8_t *mask;
+    int32_t *ku;
+    int32_t *kv;
+    double *kw;
+    uint32_t *ticket;
+    uint64_t *tile_state;
+    uint32_t epoch;
+    int64_t *kept_total;
+    int *err;
+    int spin_limit;
+};
+
+template <typename OutT>
+void launch_sketch(hipStream_t s, int32_t n, const int32_t *iptr, const int32_t *ieid, const double *ib, int k,
+                   uint64_t seed, double scale, int uniform, int variant, OutT *Y);
+template <typename T>
+void launch_spmm(hipStream_t s, int32_t n, const int32_t *rowptr, const int32_t *col, const T *val, int k,
+                 const T *P, T *out);
+void launch_residual64(hipStream_t s, int32_t n, const int32_t *rowptr, const int32_t *col, const double *val,
+                       int k, const double *X, const double *B, double *out);
+int partial_blocks(int64_t n);
+template <typename TA, typename TB>
+void launch_col_partial(hipStream_t s, int32_t n, int k, const TA *A, const TB *B, int op, double *partial);
+void launch_pcg_init(hipStream_t s, int k, const double *pbb, const double *prz, int nb, PcgScalars sc);
+void launch_pcg_alpha(hipStream_t s, int k, const double *ppq, int nb, PcgScalars sc);
+void launch_pcg_check(hipStream_t s, int k, const double *prr, int nb, int rule, double tol, PcgScalars sc,
+                      int *nactive);
+void launch_pcg_beta(hipStream_t s, int k, const double *prz, int nb, PcgScalars sc);
+void launch_col_mean(hipStream_t s, int k, const double *partial, int nb, int64_t n, double *mean);
+template <typename T>
+void launch_project(hipStream_t s, int64_t nk, int k, const T *Y, const double *mean, T *R, T *B, T *X);
+template <typename T>
+void launch_update_xr(hipStream_t s, int64_t nk, int k, const double *alpha, const T *P, const T *Q, T *X, T *R);
+template <typename T>
+void launch_update_p(hipStream_t s, int64_t nk, int k, const double *beta, const T *Z, T *P);
+template <typename TS, typename TD>
+void launch_convert(hipStream_t s, int64_t count, const TS *src, TD *dst);
+template <typename T>
+void launch_accumulate(hipStream_t s, int64_t count, const T *d, double *x);
+void launch_fill_u32(hipStream_t s, int64_t count, uint32_t *p, uint32_t v);
+template <typename T, bool BWD>
+hipError_t launch_trisolve(hipStream_t s, int device, int32_t n, const int32_t *rows, const int32_t *ptr,
+                           const int32_t *col, const T *val, const double *diag, int k, const T *in, T *out,
+                           uint32_t *flags, uint32_t epoch, int *err, int spin_limit);
+template <typename TZ>
+void launch_er_sample(hipStream_t s, const ErSampleArgs<TZ> &a);
+
+}  // namespace ss

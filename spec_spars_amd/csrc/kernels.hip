@@ -1,0 +1,587 @@
+// gfx950 kernels of the sparsification hot path.
+//
+// Dense n x k blocks are row-major (node i's k values contiguous); every
+// row kernel maps one 64-lane wavefront to one node and its lanes to the k
+// columns, so a node's k-vector moves as one coalesced 64*s-byte access.
+// Reductions over nodes are per-block partials in a fixed order followed by
+// a one-block fixed-order sum: results do not depend on timing.
+#include <hip/hip_runtime.h>
+
+#include "hash_dev.h"
+#include "kernels.h"
+
+namespace ss {
+
+#define SS_LANE (threadIdx.x & 63)
+#define SS_WAVE_IN_BLOCK (threadIdx.x >> 6)
+
+__device__ __forceinline__ int wave_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// ---------------------------------------------------------------------------
+// (a) sketch, gather form: Y[x, j] = sum over incident edges e of x, in
+// increasing e, of  fl(scale * sign(e, j)) * b(x, e),  b = +-sqrt(w).
+// Matches the sequential edge loop of jl_sketch_colwise
+// (src/sparsifier.rs:490-504) term for term; fp-contract is off so each
+// product rounds before the add.
+// ---------------------------------------------------------------------------
+template <typename OutT>
+__global__ __launch_bounds__(256) void k_sketch(int32_t n, const int32_t *__restrict__ iptr,
+                                                const int32_t *__restrict__ ieid,
+                                                const double *__restrict__ ib, int k,
+                                                uint64_t seed, double scale, int uniform,
+                                                int variant, OutT *__restrict__ Y) {
+#pragma clang fp contract(off)
+    const int row = wave_uniform(blockIdx.x * 4 + SS_WAVE_IN_BLOCK);
+    if (row >= n) return;
+    const int q0 = iptr[row], q1 = iptr[row + 1];
+    for (int jb = 0; jb < k; jb += 64) {
+        const int j = jb + SS_LANE;
+        double acc = 0.0;
+        for (int q = q0; q < q1; q++) {
+            const int e = ieid[q];
+            const double b = ib[q];
+            const double sv = sign_value((uint64_t)e, (uint64_t)j, seed, uniform, variant) * scale;
+            const double term = sv * b;
+            acc = acc + term;
+        }
+        if (j < k) Y[(size_t)row * k + j] = (OutT)acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// (b) Laplacian SpMM  T = A * P  (A full symmetric CSR with diagonal).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_spmm(int32_t n, const int32_t *__restrict__ rowptr,
+                                              const int32_t *__restrict__ col,
+                                              const T *__restrict__ val, int k,
+                                              const T *__restrict__ P, T *__restrict__ out) {
+    const int row = wave_uniform(blockIdx.x * 4 + SS_WAVE_IN_BLOCK);
+    if (row >= n) return;
+    const int q0 = rowptr[row], q1 = rowptr[row + 1];
+    for (int jb = 0; jb < k; jb += 64) {
+        const int j = jb + SS_LANE;
+        if (j >= k) break;
+        T acc = 0;
+        for (int q = q0; q < q1; q++) acc += val[q] * P[(size_t)col[q] * k + j];
+        out[(size_t)row * k + j] = acc;
+    }
+}
+
+// FP64 residual  out = B - A * X  (iterative refinement's true residual).
+template <typename T>
+__global__ __launch_bounds__(256) void k_residual64(int32_t n, const int32_t *__restrict__ rowptr,
+                                                    const int32_t *__restrict__ col,
+                                                    const double *__restrict__ val, int k,
+                                                    const double *__restrict__ X,
+                                                    const double *__restrict__ B,
+                                                    double *__restrict__ out) {
+    const int row = wave_uniform(blockIdx.x * 4 + SS_WAVE_IN_BLOCK);
+    if (row >= n) return;
+    const int q0 = rowptr[row], q1 = rowptr[row + 1];
+    for (int jb = 0; jb < k; jb += 64) {
+        const int j = jb + SS_LANE;
+        if (j >= k) break;
+        double acc = 0;
+        for (int q = q0; q < q1; q++) acc += val[q] * X[(size_t)col[q] * k + j];
+        out[(size_t)row * k + j] = B[(size_t)row * k + j] - acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Column reductions: partial[b*k + j] = sum over this block's rows of
+// f(A[i,j], B[i,j]); op 0: A*B, op 1: A (column sums).
+// ---------------------------------------------------------------------------
+template <typename TA, typename TB>
+__global__ __launch_bounds__(256) void k_col_partial(int32_t n, int k, const TA *__restrict__ A,
+                                                     const TB *__restrict__ B, int op,
+                                                     double *__restrict__ partial) {
+    __shared__ double sm[4][64];
+    const int w = SS_WAVE_IN_BLOCK, lane = SS_LANE;
+    for (int jb = 0; jb < k; jb += 64) {
+        const int j = jb + lane;
+        double acc = 0.0;
+        if (j < k) {
+            for (int64_t r = (int64_t)blockIdx.x * 4 + w; r < n; r += (int64_t)gridDim.x * 4) {
+                const double a = (double)A[(size_t)r * k + j];
+                acc += op == 0 ? a * (double)B[(size_t)r * k + j] : a;
+            }
+        }
+        sm[w][lane] = acc;
+        __syncthreads();
+        if (w == 0 && j < k)
+            partial[(size_t)blockIdx.x * k + j] = ((sm[0][lane] + sm[1][lane]) + sm[2][lane]) + sm[3][lane];
+        __syncthreads();
+    }
+}
+
+// Sum nb partials per column in index order (one block, thread per column).
+__device__ __forceinline__ double sum_partials(const double *partial, int nb, int k, int j) {
+    double s = 0.0;
+    for (int b = 0; b < nb; b++) s += partial[(size_t)b * k + j];
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// PCG scalar steps (one block of 256 threads, thread j = column j).
+// ---------------------------------------------------------------------------
+__global__ void k_pcg_init(int k, const double *partial_bb, const double *partial_rz, int nb,
+                           PcgScalars s) {
+    for (int j = threadIdx.x; j < k; j += blockDim.x) {
+        const double bb = sum_partials(partial_bb, nb, k, j);
+        const double rz = sum_partials(partial_rz, nb, k, j);
+        s.bnorm[j] = sqrt(bb);
+        s.rz[j] = rz;
+        s.active[j] = bb > 0.0 ? 1 : 0;
+        s.iters[j] = 0;
+        s.alpha[j] = 0.0;
+        s.beta[j] = 0.0;
+    }
+}
+
+__global__ void k_pcg_alpha(int k, const double *partial_pq, int nb, PcgScalars s) {
+    for (int j = threadIdx.x; j < k; j += blockDim.x) {
+        const double pq = sum_partials(partial_pq, nb, k, j);
+        double a = 0.0;
+        if (s.active[j]) {
+            if (pq > 0.0) a = s.rz[j] / pq;
+            else s.active[j] = 0;
+        }
+        s.alpha[j] = a;
+    }
+}
+
+// After x += alpha p, r -= alpha q: count the update, apply the stop rule.
+__global__ void k_pcg_check(int k, const double *partial_rr, int nb, int rule, double tol,
+                            PcgScalars s, int *nactive) {
+    __shared__ int cnt[256];
+    int c = 0;
+    for (int j = threadIdx.x; j < k; j += blockDim.x) {
+        const double rr = sum_partials(partial_rr, nb, k, j);
+        s.rr[j] = rr;
+        if (s.active[j]) {
+            s.iters[j] += 1;
+            const double rn = sqrt(rr);
+            const bool stop = rule == 0 ? (rn <= sqrt(tol)) : (rn <= tol * s.bnorm[j]);
+            if (stop) s.active[j] = 0;
+        }
+        c += s.active[j];
+    }
+    cnt[threadIdx.x] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int i = 0; i < (int)blockDim.x; i++) t += cnt[i];
+        *nactive = t;
+    }
+}
+
+__global__ void k_pcg_beta(int k, const double *partial_rz, int nb, PcgScalars s) {
+    for (int j = threadIdx.x; j < k; j += blockDim.x) {
+        const double rzn = sum_partials(partial_rz, nb, k, j);
+        double b = 0.0;
+        if (s.active[j] && s.rz[j] != 0.0) b = rzn / s.rz[j];
+        s.beta[j] = b;
+        s.rz[j] = rzn;
+    }
+}
+
+// mean[j] = colsum[j] / n
+__global__ void k_col_mean(int k, const double *partial, int nb, int64_t n, double *mean) {
+    for (int j = threadIdx.x; j < k; j += blockDim.x) mean[j] = sum_partials(partial, nb, k, j) / (double)n;
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise PCG updates (grid-stride over n*k).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_project(int64_t nk, int k, const T *__restrict__ Y,
+                                                 const double *__restrict__ mean,
+                                                 T *__restrict__ R, T *__restrict__ B,
+                                                 T *__restrict__ X) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nk;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int j = (int)(t % k);
+        const T r = (T)((double)Y[t] - mean[j]);
+        R[t] = r;
+        B[t] = r;
+        X[t] = 0;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_update_xr(int64_t nk, int k, const double *__restrict__ alpha,
+                                                   const T *__restrict__ P, const T *__restrict__ Q,
+                                                   T *__restrict__ X, T *__restrict__ R) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nk;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const T a = (T)alpha[t % k];
+        X[t] += a * P[t];
+        R[t] -= a * Q[t];
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_update_p(int64_t nk, int k, const double *__restrict__ beta,
+                                                  const T *__restrict__ Z, T *__restrict__ P) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nk;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        P[t] = Z[t] + (T)beta[t % k] * P[t];
+    }
+}
+
+template <typename TS, typename TD>
+__global__ __launch_bounds__(256) void k_convert(int64_t count, const TS *__restrict__ s, TD *__restrict__ d) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < count;
+         t += (int64_t)gridDim.x * blockDim.x)
+        d[t] = (TD)s[t];
+}
+
+// X64 += D (refinement accumulation)
+template <typename T>
+__global__ __launch_bounds__(256) void k_accumulate(int64_t count, const T *__restrict__ d,
+                                                    double *__restrict__ x) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < count;
+         t += (int64_t)gridDim.x * blockDim.x)
+        x[t] += (double)d[t];
+}
+
+__global__ void k_fill_u32(int64_t count, uint32_t *p, uint32_t v) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < count;
+         t += (int64_t)gridDim.x * blockDim.x)
+        p[t] = v;
+}
+
+// ---------------------------------------------------------------------------
+// (c) level-ordered triangular solve, sync-free.
+// Positions t = 0..n-1 are rows sorted by level; wave w of W resident waves
+// (cooperative launch) owns positions w, w+W, ...  A row waits only for rows
+// of lower level, i.e. lower positions, owned by resident waves that never
+// wait on it: the grid always drains.  flags[row] == epoch publishes a row;
+// the writer's agent-scope release fence orders its row stores before the
+// flag, the reader's acquire fence orders the flag before its loads.
+//   forward  (BWD=false): y_i = r_i - sum_{j<i} L[i,j] y_j
+//   backward (BWD=true):  z_i = y_i / D_i - sum_{j>i} L[j,i] z_j
+//                         (division skipped where D_i == 0, as
+//                          include/custom_cg.hpp:227-233 does)
+// Every wait is bounded by spin_limit polls; on expiry the wave raises
+// *err and every wave leaves at its next poll.
+// ---------------------------------------------------------------------------
+template <typename T, bool BWD>
+__global__ __launch_bounds__(256) void k_trisolve(int32_t n, const int32_t *__restrict__ rows,
+                                                  const int32_t *__restrict__ ptr,
+                                                  const int32_t *__restrict__ col,
+                                                  const T *__restrict__ val,
+                                                  const double *__restrict__ diag, int k,
+                                                  const T *__restrict__ in, T *out, uint32_t *flags,
+                                                  uint32_t epoch, int *err, int spin_limit) {
+    const int lane = SS_LANE;
+    const int wave = blockIdx.x * 4 + SS_WAVE_IN_BLOCK;
+    const int nwaves = gridDim.x * 4;
+    for (int t = wave; t < n; t += nwaves) {
+        const int i = wave_uniform(rows[t]);
+        const int q0 = wave_uniform(ptr[t]), q1 = wave_uniform(ptr[t + 1]);
+        for (int qb = q0; qb < q1; qb += 64) {
+            const int q = qb + lane;
+            const int c = q < q1 ? col[q] : -1;
+            int spins = 0;
+            while (true) {
+                const bool ready =
+                    c < 0 || __hip_atomic_load(&flags[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch;
+                if (__all(ready)) break;
+                if (++spins > spin_limit) {
+                    if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    return;
+                }
+                if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        double d = 1.0;
+        if (BWD) d = diag[i];
+        for (int jb = 0; jb < k; jb += 64) {
+            const int j = jb + lane;
+            if (j < k) {
+                T acc = in[(size_t)i * k + j];
+                if (BWD && d != 0.0) acc = (T)((double)acc / d);
+                for (int q = q0; q < q1; q++) acc -= val[q] * out[(size_t)col[q] * k + j];
+                out[(size_t)i * k + j] = acc;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (lane == 0) __hip_atomic_store(&flags[i], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// (d) fused effective resistance + probability + coin + sample + compaction.
+// One thread per edge (canonical order), 256 edges per tile.  Tiles take a
+// ticket so that every tile a tile waits on has started; the running kept
+// count moves through a decoupled look-back over per-tile words holding
+// (epoch, status, count) in one 64-bit value, so no fence is needed.
+//   d_e^2 = sum_j (Z_u - Z_v)^2, j in order     (src/sparsifier.rs:575-578)
+//   p_e = min(1, beta * -1 * L_uv * d_e.powi(2) / k)  (src/sparsifier.rs:553)
+//   keep = coin < p; kept weight -(L_uv + (w - w/sqrt(p)))
+//                                                (src/sparsifier.rs:639-661)
+// ---------------------------------------------------------------------------
+#define SS_TILE_EPOCH_SHIFT 40
+#define SS_TILE_FLAG_SHIFT 38
+#define SS_TILE_VALUE_MASK ((1ull << 38) - 1)
+
+template <typename TZ>
+__global__ __launch_bounds__(256) void k_er_sample(ErSampleArgs<TZ> a) {
+#pragma clang fp contract(off)
+    __shared__ uint32_t s_tile;
+    __shared__ int s_wcnt[4];
+    __shared__ int64_t s_excl;
+    if (threadIdx.x == 0) s_tile = atomicAdd(a.ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const int64_t e = (int64_t)tile * 256 + threadIdx.x;
+    bool keep = false;
+    int32_t eu = 0, ev = 0;
+    double kwv = 0.0;
+    if (e < a.m) {
+        eu = a.eu[e];
+        ev = a.ev[e];
+        const double w = a.ew[e];
+        const double lval = -1.0 * w;
+        double p;
+        if (a.p_in) {
+            p = a.p_in[e];
+        } else {
+            const TZ *zu = a.Z + (size_t)eu * a.k;
+            const TZ *zv = a.Z + (size_t)ev * a.k;
+            double s = 0.0;
+            for (int j = 0; j < a.k; j++) {
+                const double t = (double)zu[j] - (double)zv[j];
+                s = s + t * t;
+            }
+            const double d = sqrt(s);
+            if (a.d_out) a.d_out[e] = d;
+            const double d2 = d * d;
+            const double x = a.beta * -1.0 * lval * d2 / (double)a.k;
+            p = 1.0 * (x < 1.0 ? x : 1.0);
+        }
+        if (a.p_out) a.p_out[e] = p;
+        const double c = coin_value(a.ck, (uint64_t)e);
+        keep = c < p;
+        if (a.mask) a.mask[e] = keep ? 1 : 0;
+        if (keep) {
+            const double target = w / sqrt(p);
+            const double change = w - target;
+            const double lnew = lval + change;
+            kwv = -1.0 * lnew;
+        }
+    }
+    const int lane = SS_LANE, wv = SS_WAVE_IN_BLOCK;
+    const uint64_t bal = __ballot(keep);
+    const int wpre = __popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) s_wcnt[wv] = __popcll(bal);
+    __syncthreads();
+    int woff = 0;
+    for (int w2 = 0; w2 < wv; w2++) woff += s_wcnt[w2];
+    if (threadIdx.x == 0) {
+        const uint64_t total = (uint64_t)(s_wcnt[0] + s_wcnt[1] + s_wcnt[2] + s_wcnt[3]);
+        const uint64_t ep = (uint64_t)a.epoch << SS_TILE_EPOCH_SHIFT;
+        int64_t excl = 0;
+        if (tile == 0) {
+            __hip_atomic_store(&a.tile_state[0], ep | (2ull << SS_TILE_FLAG_SHIFT) | total,
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(&a.tile_state[tile], ep | (1ull << SS_TILE_FLAG_SHIFT) | total,
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int64_t j = (int64_t)tile - 1;
+            int spins = 0;
+            while (j >= 0) {
+                const uint64_t st =
+                    __hip_atomic_load(&a.tile_state[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint32_t sep = (uint32_t)(st >> SS_TILE_EPOCH_SHIFT);
+                const uint32_t flag = (uint32_t)(st >> SS_TILE_FLAG_SHIFT) & 3u;
+                if (sep != a.epoch || flag == 0) {
+                    if (++spins > a.spin_limit) {
+                        __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += (int64_t)(st & SS_TILE_VALUE_MASK);
+                if (flag == 2) break;
+                j--;
+            }
+            __hip_atomic_store(&a.tile_state[tile],
+                               ep | (2ull << SS_TILE_FLAG_SHIFT) | ((uint64_t)excl + total),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_excl = excl;
+        if ((int64_t)tile == a.ntiles - 1) *a.kept_total = excl + (int64_t)total;
+    }
+    __syncthreads();
+    if (keep) {
+        const int64_t pos = s_excl + woff + wpre;
+        a.ku[pos] = eu;
+        a.kv[pos] = ev;
+        a.kw[pos] = kwv;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+static inline unsigned grid_for(int64_t items, int per_block, unsigned cap = 0) {
+    int64_t g = (items + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    if (cap && g > cap) g = cap;
+    return (unsigned)g;
+}
+static const unsigned kStreamGrid = 2048;   // grid-stride kernels
+
+template <typename OutT>
+void launch_sketch(hipStream_t s, int32_t n, const int32_t *iptr, const int32_t *ieid, const double *ib,
+                   int k, uint64_t seed, double scale, int uniform, int variant, OutT *Y) {
+    hipLaunchKernelGGL(k_sketch<OutT>, dim3(grid_for(n, 4)), dim3(256), 0, s, n, iptr, ieid, ib, k, seed,
+                       scale, uniform, variant, Y);
+}
+template void launch_sketch<double>(hipStream_t, int32_t, const int32_t *, const int32_t *, const double *,
+                                    int, uint64_t, double, int, int, double *);
+template void launch_sketch<float>(hipStream_t, int32_t, const int32_t *, const int32_t *, const double *,
+                                   int, uint64_t, double, int, int, float *);
+
+template <typename T>
+void launch_spmm(hipStream_t s, int32_t n, const int32_t *rowptr, const int32_t *col, const T *val, int k,
+                 const T *P, T *out) {
+    hipLaunchKernelGGL(k_spmm<T>, dim3(grid_for(n, 4)), dim3(256), 0, s, n, rowptr, col, val, k, P, out);
+}
+template void launch_spmm<double>(hipStream_t, int32_t, const int32_t *, const int32_t *, const double *, int,
+                                  const double *, double *);
+template void launch_spmm<float>(hipStream_t, int32_t, const int32_t *, const int32_t *, const float *, int,
+                                 const float *, float *);
+
+void launch_residual64(hipStream_t s, int32_t n, const int32_t *rowptr, const int32_t *col, const double *val,
+                       int k, const double *X, const double *B, double *out) {
+    hipLaunchKernelGGL(k_residual64<double>, dim3(grid_for(n, 4)), dim3(256), 0, s, n, rowptr, col, val, k, X,
+                       B, out);
+}
+
+int partial_blocks(int64_t n) { return (int)grid_for(n, 64, 1024); }
+
+template <typename TA, typename TB>
+void launch_col_partial(hipStream_t s, int32_t n, int k, const TA *A, const TB *B, int op, double *partial) {
+    hipLaunchKernelGGL((k_col_partial<TA, TB>), dim3(partial_blocks(n)), dim3(256), 0, s, n, k, A, B, op,
+                       partial);
+}
+template void launch_col_partial<double, double>(hipStream_t, int32_t, int, const double *, const double *, int,
+                                                 double *);
+template void launch_col_partial<float, float>(hipStream_t, int32_t, int, const float *, const float *, int,
+                                               double *);
+
+void launch_pcg_init(hipStream_t s, int k, const double *pbb, const double *prz, int nb, PcgScalars sc) {
+    hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(256), 0, s, k, pbb, prz, nb, sc);
+}
+void launch_pcg_alpha(hipStream_t s, int k, const double *ppq, int nb, PcgScalars sc) {
+    hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(256), 0, s, k, ppq, nb, sc);
+}
+void launch_pcg_check(hipStream_t s, int k, const double *prr, int nb, int rule, double tol, PcgScalars sc,
+                      int *nactive) {
+    hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(256), 0, s, k, prr, nb, rule, tol, sc, nactive);
+}
+void launch_pcg_beta(hipStream_t s, int k, const double *prz, int nb, PcgScalars sc) {
+    hipLaunchKernelGGL(k_pcg_beta, dim3(1), dim3(256), 0, s, k, prz, nb, sc);
+}
+void launch_col_mean(hipStream_t s, int k, const double *partial, int nb, int64_t n, double *mean) {
+    hipLaunchKernelGGL(k_col_mean, dim3(1), dim3(256), 0, s, k, partial, nb, n, mean);
+}
+
+template <typename T>
+void launch_project(hipStream_t s, int64_t nk, int k, const T *Y, const double *mean, T *R, T *B, T *X) {
+    hipLaunchKernelGGL(k_project<T>, dim3(grid_for(nk, 256, kStreamGrid)), dim3(256), 0, s, nk, k, Y, mean, R, B,
+                       X);
+}
+template void launch_project<double>(hipStream_t, int64_t, int, const double *, const double *, double *,
+                                     double *, double *);
+template void launch_project<float>(hipStream_t, int64_t, int, const float *, const double *, float *, float *,
+                                    float *);
+
+template <typename T>
+void launch_update_xr(hipStream_t s, int64_t nk, int k, const double *alpha, const T *P, const T *Q, T *X, T *R) {
+    hipLaunchKernelGGL(k_update_xr<T>, dim3(grid_for(nk, 256, kStreamGrid)), dim3(256), 0, s, nk, k, alpha, P, Q,
+                       X, R);
+}
+template void launch_update_xr<double>(hipStream_t, int64_t, int, const double *, const double *,
+                                       const double *, double *, double *);
+template void launch_update_xr<float>(hipStream_t, int64_t, int, const double *, const float *, const float *,
+                                      float *, float *);
+
+template <typename T>
+void launch_update_p(hipStream_t s, int64_t nk, int k, const double *beta, const T *Z, T *P) {
+    hipLaunchKernelGGL(k_update_p<T>, dim3(grid_for(nk, 256, kStreamGrid)), dim3(256), 0, s, nk, k, beta, Z, P);
+}
+template void launch_update_p<double>(hipStream_t, int64_t, int, const double *, const double *, double *);
+template void launch_update_p<float>(hipStream_t, int64_t, int, const double *, const float *, float *);
+
+template <typename TS, typename TD>
+void launch_convert(hipStream_t s, int64_t count, const TS *src, TD *dst) {
+    hipLaunchKernelGGL((k_convert<TS, TD>), dim3(grid_for(count, 256, kStreamGrid)), dim3(256), 0, s, count, src,
+                       dst);
+}
+template void launch_convert<double, float>(hipStream_t, int64_t, const double *, float *);
+template void launch_convert<float, double>(hipStream_t, int64_t, const float *, double *);
+
+template <typename T>
+void launch_accumulate(hipStream_t s, int64_t count, const T *d, double *x) {
+    hipLaunchKernelGGL(k_accumulate<T>, dim3(grid_for(count, 256, kStreamGrid)), dim3(256), 0, s, count, d, x);
+}
+template void launch_accumulate<float>(hipStream_t, int64_t, const float *, double *);
+template void launch_accumulate<double>(hipStream_t, int64_t, const double *, double *);
+
+void launch_fill_u32(hipStream_t s, int64_t count, uint32_t *p, uint32_t v) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(count, 256, kStreamGrid)), dim3(256), 0, s, count, p, v);
+}
+
+template <typename T, bool BWD>
+int trisolve_grid(int device) {
+    static int cached[64] = {0};
+    if (device < 0 || device >= 64) return 0;
+    if (cached[device]) return cached[device];
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trisolve<T, BWD>, 256, 0) != hipSuccess) return 0;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 0;
+    cached[device] = per_cu * prop.multiProcessorCount;
+    return cached[device];
+}
+
+template <typename T, bool BWD>
+hipError_t launch_trisolve(hipStream_t s, int device, int32_t n, const int32_t *rows, const int32_t *ptr,
+                           const int32_t *col, const T *val, const double *diag, int k, const T *in, T *out,
+                           uint32_t *flags, uint32_t epoch, int *err, int spin_limit) {
+    int grid = trisolve_grid<T, BWD>(device);
+    if (grid <= 0) return hipErrorInvalidConfiguration;
+    int need = (n + 3) / 4;
+    if (need < grid) grid = need < 1 ? 1 : need;
+    void *args[] = {&n, &rows, &ptr, &col, &val, &diag, &k, &in, &out, &flags, &epoch, &err, &spin_limit};
+    return hipLaunchCooperativeKernel((const void *)k_trisolve<T, BWD>, dim3(grid), dim3(256), args, 0, s);
+}
+template hipError_t launch_trisolve<double, false>(hipStream_t, int, int32_t, const int32_t *, const int32_t *,
+                                                   const int32_t *, const double *, const double *, int,
+                                                   const double *, double *, uint32_t *, uint32_t, int *, int);
+template hipError_t launch_trisolve<double, true>(hipStream_t, int, int32_t, const int32_t *, const int32_t *,
+                                                  const int32_t *, const double *, const double *, int,
+                                                  const double *, double *, uint32_t *, uint32_t, int *, int);
+template hipError_t launch_trisolve<float, false>(hipStream_t, int, int32_t, const int32_t *, const int32_t *,
+                                                  const int32_t *, const float *, const double *, int,
+                                                  const float *, float *, uint32_t *, uint32_t, int *, int);
+template hipError_t launch_trisolve<float, true>(hipStream_t, int, int32_t, const int32_t *, const int32_t *,
+                                                 const int32_t *, const float *, const double *, int,
+                                                 const float *, float *, uint32_t *, uint32_t, int *, int);
+
+template <typename TZ>
+void launch_er_sample(hipStream_t s, const ErSampleArgs<TZ> &a) {
+    hipLaunchKernelGGL(k_er_sample<TZ>, dim3((unsigned)a.ntiles), dim3(256), 0, s, a);
+}
+template void launch_er_sample<double>(hipStream_t, const ErSampleArgs<double> &);
+template void launch_er_sample<float>(hipStream_t, const ErSampleArgs<float> &);
+
+}  // namespace ss

@@ -1,0 +1,82 @@
+// Internal declarations shared by the host C++ and the HIP sources.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/spec_spars_hip.h"
+
+namespace ss {
+
+// ---- errors: thread-local last message, returned status ----------------
+void set_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+inline int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+inline int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    set_error("%s", buf);
+    return code;
+}
+
+#define SS_HIP(call)                                                                    \
+    do {                                                                                \
+        hipError_t _e = (call);                                                         \
+        if (_e != hipSuccess)                                                           \
+            return ::ss::fail(SS_ERR_HIP, "%s: %s (%s:%d)", #call, hipGetErrorString(_e), \
+                              __FILE__, __LINE__);                                      \
+    } while (0)
+
+#define SS_TRY(expr)              \
+    do {                          \
+        int _s = (expr);          \
+        if (_s != SS_OK) return _s; \
+    } while (0)
+
+// ---- host factor --------------------------------------------------------
+struct Factor {
+    int64_t n = 0;
+    int64_t nnz = 0;                  // incl. unit diagonal
+    std::vector<int32_t> urowptr;     // reference precond_M layout
+    std::vector<int32_t> ucol;
+    std::vector<double> uval;
+    std::vector<double> diag;         // D (0 on each component's last column)
+    std::vector<int32_t> lvl_fwd, lvl_bwd;   // 1-based
+    int32_t depth_fwd = 0, depth_bwd = 0;
+    double ms_build = 0.0;
+};
+
+// Level-ordered pull-form CSR of one triangular solve.
+struct Schedule {
+    int64_t n = 0;
+    int32_t depth = 0;
+    std::vector<int32_t> rows;        // row processed at position t
+    std::vector<int32_t> level_ptr;   // positions of level l: [ptr[l-1], ptr[l])
+    std::vector<int32_t> ptr;         // entries of position t: [ptr[t], ptr[t+1])
+    std::vector<int32_t> col;
+    std::vector<double> val;
+};
+
+int factor_build(int64_t n, const int32_t *colptr, const int32_t *rowidx, const double *vals,
+                 uint64_t seed, Factor &F);
+void factor_levels(Factor &F);
+void factor_schedules(const Factor &F, Schedule &fwd, Schedule &bwd);
+
+// ---- host graph structure -----------------------------------------------
+// Canonical edges (u < v, sorted by (v, u)) -> full Laplacian CSR with the
+// diagonal and, for every off-diagonal entry, the id of its edge.
+struct HostGraph {
+    int64_t n = 0, m = 0;
+    std::vector<int32_t> rowptr, col, eid;   // eid = -1 on the diagonal
+    std::vector<double> val;
+};
+int graph_check_edges(int64_t n, int64_t m, const int32_t *u, const int32_t *v, const double *w);
+int graph_build(int64_t n, int64_t m, const int32_t *u, const int32_t *v, const double *w,
+                HostGraph &G);
+
+}  // namespace ss

@@ -1,0 +1,229 @@
+"""GPU parity of the HIP path against the CPU oracle (through the C ABI)."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import components, golden
+
+pytestmark = pytest.mark.gpu
+
+
+def _graph(kind, seed=0):
+    from spec_spars_amd import graphs
+    if kind == "grid":
+        n, u, v = graphs.grid2d(24, 20)
+        w = graphs.loguniform_weights(len(u), seed=seed)
+    elif kind == "grid3d":
+        n, u, v = graphs.grid3d(10)
+        w = graphs.loguniform_weights(len(u), seed=seed)
+    elif kind == "rgg":
+        n, u, v = graphs.rgg(900, seed=seed)
+        w = np.ones(len(u))
+    else:
+        n, u, v = graphs.chung_lu(1500, 6000, seed=seed)
+        w = graphs.uniform_weights(len(u), seed=seed)
+    return n, u, v, w
+
+
+# ---- (a) sketch --------------------------------------------------------------
+@pytest.mark.parametrize("kind", ["grid", "rgg", "chunglu"])
+@pytest.mark.parametrize("k", [1, 24, 48, 64, 80])
+def test_sketch_bit_exact(ctx, oracle_mod, kind, k):
+    n, u, v, w = _graph(kind)
+    Y = ctx.sketch(u, v, w, n, k, seed=5, scale=1.0 / math.sqrt(k))
+    Yo = oracle_mod.sketch(u, v, w, n, k, seed=5, scale=1.0 / math.sqrt(k))
+    assert np.array_equal(Y, Yo)
+
+
+def test_sketch_uniform_and_insertion_order(ctx, oracle_mod):
+    g = golden("sampling_test")          # edges inserted with v1 > v2
+    n = int(g["n"])
+    w = np.linspace(0.5, 2.0, len(g["v1"]))
+    for uniform in (False, True):
+        Y = ctx.sketch(g["v1"], g["v2"], w, n, 16, seed=9, scale=math.sqrt(3.0), uniform=uniform)
+        Yo = oracle_mod.sketch(g["v1"], g["v2"], w, n, 16, seed=9, scale=math.sqrt(3.0), uniform=uniform)
+        assert np.array_equal(Y, Yo)
+        assert np.allclose(Y.sum(0), 0.0, atol=1e-12)
+
+
+def test_sketch_empty_and_isolated(ctx, oracle_mod):
+    Y = ctx.sketch([], [], [], 5, 8, seed=1, scale=1.0)
+    assert np.array_equal(Y, np.zeros((5, 8)))
+    Y = ctx.sketch([0], [3], [4.0], 6, 8, seed=1, scale=1.0)
+    assert np.array_equal(Y, oracle_mod.sketch([0], [3], [4.0], 6, 8, seed=1, scale=1.0))
+
+
+# ---- (b)+(c) solve -----------------------------------------------------------
+def _oracle_solution(oracle_mod, n, u, v, w, Y, seed, tol):
+    import spec_spars_amd as ss
+    cp, ri, va = oracle_mod.laplacian(n, u, v, w)
+    f = ss.Factor(cp, ri, va, seed=seed)
+    urp, uci, uva, dg = f.export()
+    Zo, it, rr = oracle_mod.solve_lap(Y, cp, ri, va, urp, uci, uva, dg, stop_rule=1, tol=tol, max_iter=2000)
+    return cp, ri, va, Zo, it
+
+
+def _rel_centered(Z, Zo):
+    a, b = Z - Z.mean(0), Zo - Zo.mean(0)
+    return np.linalg.norm(a - b, axis=0) / np.linalg.norm(b, axis=0)
+
+
+@pytest.mark.parametrize("kind", ["grid", "grid3d", "rgg"])
+def test_solve_fp64_matches_oracle(ctx, oracle_mod, kind):
+    import spec_spars_amd as ss
+    n, u, v, w = _graph(kind)
+    if components(n, u, v) != 1:
+        pytest.skip("disconnected instance")
+    k = 12
+    Y = ctx.sketch(u, v, w, n, k, seed=2, scale=1.0 / math.sqrt(k))
+    cp, ri, va, Zo, it = _oracle_solution(oracle_mod, n, u, v, w, Y, 3, 1e-12)
+    Z, info = ctx.solve_lap(Y, cp, ri, va, ss.solve_opts(factor_seed=3, stop_rule=ss.STOP_REL, tol=1e-10,
+                                                         max_iter=2000))
+    assert info["max_relres"] < 1e-9
+    assert _rel_centered(Z, Zo).max() < 1e-7
+    # same factor, same method: iteration counts agree to within a couple
+    assert abs(int(info["iters"].max()) - int(it.max())) <= 3
+
+
+def test_solve_golden_exact(ctx, oracle_mod):
+    import spec_spars_amd as ss
+    for name in ("grid8x8_loguniform", "rgg120", "tree60"):
+        g = golden(name)
+        n = int(g["n"])
+        cp, ri, va = oracle_mod.laplacian(n, g["u"], g["v"], g["w"])
+        Z, info = ctx.solve_lap(g["Y"], cp, ri, va, ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-11))
+        assert _rel_centered(Z, g["Z"]).max() < 1e-8
+
+
+def test_solve_reference_stop_rule(ctx, oracle_mod):
+    import spec_spars_amd as ss
+    n, u, v, w = _graph("grid")
+    Y = ctx.sketch(u, v, w, n, 8, seed=2, scale=math.sqrt(3.0))
+    cp, ri, va = oracle_mod.laplacian(n, u, v, w)
+    Z, info = ctx.solve_lap(Y, cp, ri, va, ss.solve_opts(factor_seed=1))     # ||r||^2 <= 2e-7, 400 its
+    assert info["max_relres"] < 1e-2
+    f = ss.Factor(cp, ri, va, seed=1)
+    urp, uci, uva, dg = f.export()
+    Zo, it, rr = oracle_mod.solve_lap(Y, cp, ri, va, urp, uci, uva, dg, stop_rule=0, tol=2e-7)
+    assert abs(int(info["iters"].max()) - int(it.max())) <= 2
+
+
+def test_solve_mixed_precision(ctx, oracle_mod):
+    import spec_spars_amd as ss
+    n, u, v, w = _graph("grid3d")
+    k = 16
+    Y = ctx.sketch(u, v, w, n, k, seed=2, scale=1.0 / math.sqrt(k))
+    cp, ri, va, Zo, it = _oracle_solution(oracle_mod, n, u, v, w, Y, 3, 1e-12)
+    Z, info = ctx.solve_lap(Y, cp, ri, va, ss.solve_opts(factor_seed=3, stop_rule=ss.STOP_REL, tol=1e-8,
+                                                         precision=ss.PREC_MIXED, max_iter=2000))
+    assert info["max_relres"] <= 1e-8
+    assert info["refinements"] >= 1
+    assert _rel_centered(Z, Zo).max() < 1e-5
+
+
+def test_solve_not_converged_status(ctx, oracle_mod):
+    import spec_spars_amd as ss
+    n, u, v, w = _graph("grid")
+    Y = ctx.sketch(u, v, w, n, 4, seed=2, scale=1.0)
+    cp, ri, va = oracle_mod.laplacian(n, u, v, w)
+    with pytest.raises(ss.SSError) as ei:
+        ctx.solve_lap(Y, cp, ri, va, ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-12, max_iter=1))
+    assert ei.value.code == 3
+
+
+# ---- (d) ER + sample + compact -----------------------------------------------
+@pytest.mark.parametrize("kind", ["grid", "chunglu"])
+def test_er_sample_bit_exact(ctx, oracle_mod, kind):
+    from spec_spars_amd import graphs
+    n, u, v, w = _graph(kind, seed=1)
+    k = 24
+    rng = np.random.default_rng(0)
+    Z = rng.standard_normal((n, k)) * 0.05
+    cp, ri, va = oracle_mod.laplacian(n, u, v, w)
+    uu, vv, lv = oracle_mod.upper_edges(n, cp, ri, va)
+    beta = graphs.beta_for(n, 0.3)
+    d = oracle_mod.diff_norms(uu, vv, Z)
+    p = oracle_mod.probs(lv, d, beta, k)
+    assert 0.05 < np.mean(p < 1.0) < 0.95
+    for seed in (0, 7):
+        r = ctx.er_sample(uu, vv, lv, k, beta, seed, Z=Z)
+        assert np.array_equal(r["d"], d)
+        assert np.array_equal(r["p"], p)
+        ref = oracle_mod.sample_reweight(n, uu, vv, lv, p, oracle_mod.coins(seed, len(uu)))
+        assert np.array_equal(r["mask"], ref["mask"])
+        assert r["kept"] == ref["kept"]
+        assert np.array_equal(r["ku"], ref["ku"]) and np.array_equal(r["kv"], ref["kv"])
+        assert np.array_equal(r["kw"], ref["kw"])
+
+
+def test_sampling_test_kat_gpu(ctx, oracle_mod):
+    g = golden("sampling_test")
+    n = int(g["n"])
+    cp, ri, va = oracle_mod.laplacian(n, g["v1"], g["v2"], np.ones(len(g["v1"])))
+    u, v, lv = oracle_mod.upper_edges(n, cp, ri, va)
+    r = ctx.er_sample(u, v, lv, 4, 1.0, 0, p_in=g["probs"], n=n)
+    assert components(n, r["ku"], r["kv"]) == int(g["components"])
+
+
+def test_sampling_verify_gpu(ctx, oracle_mod):
+    from spec_spars_amd import graphs
+    n, u, v = graphs.grid2d(100, 100)
+    w = graphs.uniform_weights(len(u), seed=8)
+    for seed in range(3):
+        r = ctx.er_sample(u, v, -w, 4, 1.0, seed, p_in=np.full(len(u), 0.5), n=n)
+        ref = oracle_mod.sample_reweight(n, u, v, -w, np.full(len(u), 0.5), oracle_mod.coins(seed, len(u)))
+        assert np.array_equal(r["mask"], ref["mask"])
+        assert abs(len(u) / r["kept"] - 2.0) < 0.05
+        assert np.allclose(r["kw"], math.sqrt(2.0) * w[r["mask"].astype(bool)], rtol=1e-15)
+
+
+def test_er_sample_large_lookback(ctx, oracle_mod):
+    # many tiles: the decoupled look-back must give every kept edge its place
+    from spec_spars_amd import graphs
+    n, u, v = graphs.grid2d(600, 500)
+    w = np.ones(len(u))
+    rng = np.random.default_rng(3)
+    p = rng.random(len(u))
+    r = ctx.er_sample(u, v, -w, 4, 1.0, 11, p_in=p, n=n)
+    ref = oracle_mod.sample_reweight(n, u, v, -w, p, oracle_mod.coins(11, len(u)))
+    assert r["kept"] == ref["kept"]
+    assert np.array_equal(r["mask"], ref["mask"])
+    assert np.array_equal(r["ku"], ref["ku"]) and np.array_equal(r["kv"], ref["kv"])
+
+
+# ---- whole device pipeline -------------------------------------------------------
+def test_graph_sparsify_pipeline(ctx, oracle_mod):
+    import spec_spars_amd as ss
+    from spec_spars_amd import graphs
+    n, u, v, w = _graph("grid", seed=4)
+    k = 24
+    G = ss.Graph(ctx, n, u, v, w)
+    cp, ri, va = G.laplacian()
+    f = ss.Factor(cp, ri, va, seed=2)
+    G.set_factor(f)
+    beta = graphs.beta_for(n, 0.3)
+    rep = G.sparsify(k, beta, scale=1.0 / math.sqrt(k), sketch_seed=0, sampling_seed=0,
+                     solve=ss.solve_opts(factor_seed=2, stop_rule=ss.STOP_REL, tol=1e-10, max_iter=2000))
+    Y = G.read_block("Y")
+    Yo = oracle_mod.sketch(u, v, w, n, k, seed=0, scale=1.0 / math.sqrt(k))
+    assert np.array_equal(Y, Yo)
+    Z = G.read_block("Z")
+    urp, uci, uva, dg = f.export()
+    Zo, it, rr = oracle_mod.solve_lap(Yo, cp, ri, va, urp, uci, uva, dg, stop_rule=1, tol=1e-12, max_iter=2000)
+    assert _rel_centered(Z, Zo).max() < 1e-6
+    # leverage scores within 1e-4 of the FP64 oracle using the same signs
+    uu, vv, lv = oracle_mod.upper_edges(n, cp, ri, va)
+    lev_o = -lv * oracle_mod.diff_norms(uu, vv, Zo) ** 2
+    p = G.read_probs()
+    po = oracle_mod.probs(lv, oracle_mod.diff_norms(uu, vv, Zo), beta, k)
+    unclipped = (p < 1.0) & (po < 1.0)
+    assert np.all(np.abs(p[unclipped] - po[unclipped]) <= 1e-4 * po[unclipped])
+    assert lev_o.sum() > 0
+    ku, kv, kw = G.read_kept()
+    assert rep["kept"] == len(ku)
+    ge = set(zip(u.tolist(), v.tolist()))
+    assert all((a, b) in ge for a, b in zip(ku.tolist(), kv.tolist()))
+    # sample mask given these probabilities is the oracle's
+    ref = oracle_mod.sample_reweight(n, uu, vv, lv, p, oracle_mod.coins(0, len(uu)))
+    assert np.array_equal(ref["ku"], ku) and np.array_equal(ref["kw"], kw)

@@ -1,0 +1,142 @@
+"""Host-side logic of the library (no GPU): C-ABI exports, the approximate-
+Cholesky factor, level sets, graph generators and error paths."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, components
+
+
+def test_library_exports_every_declared_symbol():
+    import spec_spars_amd._lib as L
+    lib = ctypes.CDLL(L.LIB_PATH)
+    names = set()
+    for h in os.listdir(os.path.join(ROOT, "include")):
+        if h.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", h)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            names |= set(re.findall(r"\b(ss_[a-z0-9_]+)\s*\(", txt))
+    assert len(names) >= 20
+    for nm in sorted(names):
+        assert hasattr(lib, nm), nm
+    assert set(L.SIGNATURES) <= names
+
+
+def _factor(n, u, v, w, seed=1):
+    import oracle
+    import spec_spars_amd as ss
+    cp, ri, va = oracle.laplacian(n, u, v, w)
+    return ss.Factor(cp, ri, va, seed=seed), (cp, ri, va)
+
+
+def test_factor_invariants():
+    from spec_spars_amd import graphs
+    for n, u, v in (graphs.grid2d(20, 17), graphs.rgg(700, seed=2), graphs.chung_lu(800, 3000, seed=1)):
+        w = graphs.loguniform_weights(len(u), seed=3)
+        f, _ = _factor(n, u, v, w)
+        urp, uci, uva, d = f.export()
+        assert urp[-1] == f.info()["nnz"]
+        for c in range(n):
+            cols = uci[urp[c]:urp[c + 1]]
+            vals = uva[urp[c]:urp[c + 1]]
+            assert cols[0] == c and vals[0] == 1.0           # unit diagonal first
+            assert np.all(np.diff(cols) > 0)                   # rows ascending, > c
+            if len(cols) > 1:
+                assert d[c] > 0
+                assert abs(vals[1:].sum() + 1.0) < 1e-12        # -w/tot sums to -1
+                assert np.all(vals[1:] < 0)
+            else:
+                assert d[c] == 0.0
+        # exactly one grounded column per connected component
+        assert int((d == 0).sum()) == components(n, u, v)
+
+
+def test_factor_tree_is_exact():
+    # leaf-first tree: every column has one lower entry, so no fill is sampled
+    # and L D L^T reproduces the Laplacian (src/example.cc:262 loop is empty).
+    rng = np.random.default_rng(4)
+    n = 300
+    parent = np.array([rng.integers(i + 1, n) for i in range(n - 1)])
+    from spec_spars_amd import graphs
+    u, v, _ = graphs.canonical(np.arange(n - 1), parent)
+    w = graphs.uniform_weights(len(u), seed=1)
+    f, (cp, ri, va) = _factor(n, u, v, w)
+    urp, uci, uva, d = f.export()
+    L = np.zeros((n, n))
+    for c in range(n):
+        L[uci[urp[c]:urp[c + 1]], c] = uva[urp[c]:urp[c + 1]]
+    import scipy.sparse as sp
+    A = sp.csc_matrix((va, ri, cp), shape=(n, n)).toarray()
+    assert np.allclose(L @ np.diag(d) @ L.T, A, rtol=1e-12, atol=1e-12)
+
+
+def test_factor_unbiased():
+    # E[L D L^T] = L_G for the randomized elimination
+    from spec_spars_amd import graphs
+    n, u, v = graphs.grid2d(5, 5)
+    w = graphs.uniform_weights(len(u), seed=2)
+    acc = np.zeros((n, n))
+    trials = 300
+    for s in range(trials):
+        f, (cp, ri, va) = _factor(n, u, v, w, seed=s)
+        urp, uci, uva, d = f.export()
+        L = np.zeros((n, n))
+        for c in range(n):
+            L[uci[urp[c]:urp[c + 1]], c] = uva[urp[c]:urp[c + 1]]
+        acc += L @ np.diag(d) @ L.T
+    import scipy.sparse as sp
+    A = sp.csc_matrix((va, ri, cp), shape=(n, n)).toarray()
+    assert np.linalg.norm(acc / trials - A) / np.linalg.norm(A) < 0.05
+
+
+def test_factor_deterministic_and_seeded():
+    from spec_spars_amd import graphs
+    n, u, v = graphs.grid2d(30, 30)
+    w = graphs.loguniform_weights(len(u), seed=0)
+    a = _factor(n, u, v, w, seed=5)[0].export()
+    b = _factor(n, u, v, w, seed=5)[0].export()
+    c = _factor(n, u, v, w, seed=6)[0].export()
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
+    assert not all(len(x) == len(y) and np.array_equal(x, y) for x, y in zip(a, c))
+
+
+def test_levels_match_oracle_dp():
+    import oracle
+    from spec_spars_amd import graphs
+    for n, u, v in (graphs.grid2d(40, 30), graphs.grid3d(9), graphs.rgg(2000, seed=1)):
+        f, _ = _factor(n, u, v, graphs.uniform_weights(len(u), seed=1))
+        urp, uci, _, _ = f.export()
+        lf, lb = f.levels()
+        of, ob = oracle.levels(n, urp, uci)
+        assert np.array_equal(lf, of) and np.array_equal(lb, ob)
+        assert f.info()["depth_fwd"] == of.max() and f.info()["depth_bwd"] == ob.max()
+
+
+def test_factor_rejects_bad_input():
+    import spec_spars_amd as ss
+    with pytest.raises(ss.SSError) as e:
+        ss.Factor(np.array([0, 2, 1], np.int32), np.array([0, 1], np.int32), np.ones(2))
+    assert e.value.code == 1
+    with pytest.raises(ss.SSError):
+        ss.Factor(np.array([0, 1, 2], np.int32), np.array([0, 7], np.int32), np.ones(2))
+
+
+def test_generators_canonical_and_sizes():
+    from spec_spars_amd import graphs
+    n, u, v = graphs.grid2d(64, 64)
+    assert (n, len(u)) == (4096, 8064)                 # c1
+    assert 3 * 128 * 128 * 127 == 6242304             # c2 edge count
+    n3, u3, v3 = graphs.grid3d(12)
+    assert len(u3) == 3 * 12 * 12 * 11
+    for (nn, a, b) in ((n, u, v), (n3, u3, v3), graphs.rgg(3000, seed=3), graphs.chung_lu(2000, 9000, seed=2)):
+        assert np.all(a < b)
+        key = b.astype(np.int64) * nn + a
+        assert np.all(np.diff(key) > 0)
+    n, u, v = graphs.chung_lu(5000, 20000, seed=1)
+    assert len(u) == 20000
+    w = graphs.loguniform_weights(100000, seed=0)
+    assert w.min() >= 1e-3 and w.max() <= 1e3
+    assert abs(np.log10(w).mean()) < 0.05
