@@ -41,6 +41,12 @@ extern "C" {
                                    :213-222), tol = 2e-7                      */
 #define SS_STOP_REL 1           /* ||r|| <= tol * ||b||                      */
 
+#define SS_ORDER_NATURAL 0      /* the reference's natural elimination order */
+#define SS_ORDER_RANDOM 1       /* seeded random permutation                 */
+#define SS_ORDER_ROUNDS 2       /* rounds of independent low-degree vertices
+                                   of the fill-augmented graph: every round
+                                   is one level of both triangular solves   */
+
 typedef struct ss_context ss_context;
 typedef struct ss_factor ss_factor;
 typedef struct ss_graph ss_graph;
@@ -65,13 +71,24 @@ int ss_context_mem_info(ss_context *ctx, size_t *free_b, size_t *total_b);
  * duplicate merges and value sorts with a total order.                    */
 int ss_factor_build(int64_t n, const int32_t *colptr, const int32_t *rowidx,
                     const double *vals, uint64_t seed, ss_factor **out);
+/* Same elimination with a chosen order (SS_ORDER_*).  The natural order's
+ * triangular-solve DAG is nearly a chain (depth ~ n/2 on grids, fill edges
+ * link consecutive columns); the other orders keep the algorithm and give
+ * the level-scheduled solves a shallow DAG.  No reference counterpart. */
+int ss_factor_build_ordered(int64_t n, const int32_t *colptr, const int32_t *rowidx,
+                            const double *vals, uint64_t seed, int32_t ordering,
+                            ss_factor **out);
+/* Elimination order: order[p] = vertex eliminated at position p (identity
+ * for the natural order); rounds = number of rounds (SS_ORDER_ROUNDS). */
+int ss_factor_order(const ss_factor *f, int32_t *order, int32_t *rounds);
 /* nnz counts the unit diagonal, as "factor nnz" at src/example.cc:448-450;
  * depth_* are the longest dependency paths (src/example.cc:531-545). */
 int ss_factor_info(const ss_factor *f, int64_t *n, int64_t *nnz,
                    int32_t *depth_fwd, int32_t *depth_bwd);
 /* The factor in the reference's precond_M layout (src/example.cc:422-470):
  * CSR, row c = column c of the unit-lower factor L (diagonal first, rows
- * ascending), plus D.  Arrays sized n+1, nnz, nnz, n. */
+ * ascending), plus D.  Arrays sized n+1, nnz, nnz, n.  Indices are
+ * elimination positions (vertex = order[position]). */
 int ss_factor_export(const ss_factor *f, int32_t *urowptr, int32_t *ucol,
                      double *uval, double *diag);
 /* 1-based level of every row: forward = the DP at src/example.cc:531-542,
@@ -103,7 +120,7 @@ typedef struct ss_solve_opts {
     double fail_relres;     /* final rel. residual >= this -> NOT_CONVERGED;
                                1e-2 in the reference (custom_cg.hpp:254)     */
     int32_t precision;      /* SS_PREC_FP64 or SS_PREC_MIXED                 */
-    int32_t reserved;
+    int32_t ordering;       /* SS_ORDER_* of the factor ss_solve_lap builds  */
 } ss_solve_opts;
 
 typedef struct ss_solve_report {
@@ -188,6 +205,25 @@ int ss_graph_read_kept(const ss_graph *g, int64_t cap, int32_t *ku,
 /* Sketch Y / solution Z of the last run as FP64 n x k (host). */
 int ss_graph_read_block(const ss_graph *g, int which /*0=Y,1=Z*/, double *out);
 int ss_graph_read_probs(const ss_graph *g, double *p);
+
+/* Per-kernel device time, measured with HIP events on the graph's stream
+ * around every launch while enabled (enabling clears the counters). */
+#define SS_KERNEL_SKETCH 0      /* (a) sketch                                */
+#define SS_KERNEL_SPMM 1        /* (b) Laplacian SpMM and FP64 residuals     */
+#define SS_KERNEL_TRI_FWD 2     /* (c) forward solve  L y = r                */
+#define SS_KERNEL_TRI_BWD 3     /* (c) backward solve L^T z = D^+ y          */
+#define SS_KERNEL_ER_SAMPLE 4   /* (d) ER + coin + sample + compaction       */
+#define SS_KERNEL_COUNT 5
+typedef struct ss_kernel_stat {
+    double ms_total;
+    int64_t launches;
+} ss_kernel_stat;
+int ss_graph_profile(ss_graph *g, int enable);
+int ss_graph_profile_read(const ss_graph *g, ss_kernel_stat *out /* [SS_KERNEL_COUNT] */);
+
+/* Achievable-bandwidth probe: a plain FP64 copy kernel over `bytes` bytes
+ * (read + write), repeated reps times; *ms = average duration per copy. */
+int ss_bench_copy(ss_context *ctx, size_t bytes, int reps, double *ms);
 
 #ifdef __cplusplus
 }

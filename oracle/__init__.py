@@ -23,8 +23,9 @@ def load():
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB):
-        from spec_spars_amd.build import build_oracle
-        build_oracle()
+        import subprocess
+        subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-std=c99", "-o", LIB,
+                        os.path.join(_HERE, "oracle.c"), "-lm"], check=True)
     L = ctypes.CDLL(LIB)
     sig = {
         "or_hash128to64": (ctypes.c_uint64, [ctypes.c_uint64, ctypes.c_uint64]),

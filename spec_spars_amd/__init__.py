@@ -6,7 +6,8 @@ synthetic graph generators.  Importing the package loads the library and
 fails if it has not been built.
 """
 from . import _lib
-from ._lib import SSError, PREC_FP64, PREC_MIXED, STOP_REF_ABS, STOP_REL  # noqa: F401
+from ._lib import (SSError, PREC_FP64, PREC_MIXED, STOP_REF_ABS, STOP_REL,  # noqa: F401
+                   ORDER_NATURAL, ORDER_RANDOM, ORDER_ROUNDS)
 
 _lib.load()
 

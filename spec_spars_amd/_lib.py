@@ -20,6 +20,9 @@ PREC_FP64 = 0
 PREC_MIXED = 1
 STOP_REF_ABS = 0
 STOP_REL = 1
+ORDER_NATURAL = 0
+ORDER_RANDOM = 1
+ORDER_ROUNDS = 2
 
 c_i32p = ctypes.POINTER(ctypes.c_int32)
 c_i64p = ctypes.POINTER(ctypes.c_int64)
@@ -32,7 +35,7 @@ class SolveOpts(ctypes.Structure):
     _fields_ = [("factor_seed", ctypes.c_uint64), ("stop_rule", ctypes.c_int32),
                 ("max_iter", ctypes.c_int32), ("tol", ctypes.c_double),
                 ("fail_relres", ctypes.c_double), ("precision", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("ordering", ctypes.c_int32)]
 
 
 class SolveReport(ctypes.Structure):
@@ -56,6 +59,13 @@ class SparsifyReport(ctypes.Structure):
                 ("ms_sample", ctypes.c_double), ("ms_total", ctypes.c_double)]
 
 
+KERNELS = ["sketch", "spmm", "tri_fwd", "tri_bwd", "er_sample"]
+
+
+class KernelStat(ctypes.Structure):
+    _fields_ = [("ms_total", ctypes.c_double), ("launches", ctypes.c_int64)]
+
+
 # name -> (restype, argtypes); every symbol include/spec_spars_hip.h declares
 SIGNATURES = {
     "ss_last_error": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
@@ -66,6 +76,9 @@ SIGNATURES = {
                                            ctypes.POINTER(ctypes.c_size_t)]),
     "ss_factor_build": (ctypes.c_int, [ctypes.c_int64, c_i32p, c_i32p, c_f64p, ctypes.c_uint64,
                                        ctypes.POINTER(c_vp)]),
+    "ss_factor_build_ordered": (ctypes.c_int, [ctypes.c_int64, c_i32p, c_i32p, c_f64p, ctypes.c_uint64,
+                                               ctypes.c_int32, ctypes.POINTER(c_vp)]),
+    "ss_factor_order": (ctypes.c_int, [c_vp, c_i32p, c_i32p]),
     "ss_factor_info": (ctypes.c_int, [c_vp, c_i64p, c_i64p, c_i32p, c_i32p]),
     "ss_factor_export": (ctypes.c_int, [c_vp, c_i32p, c_i32p, c_f64p, c_f64p]),
     "ss_factor_levels": (ctypes.c_int, [c_vp, c_i32p, c_i32p]),
@@ -94,6 +107,9 @@ SIGNATURES = {
     "ss_graph_read_kept": (ctypes.c_int, [c_vp, ctypes.c_int64, c_i32p, c_i32p, c_f64p, c_i64p]),
     "ss_graph_read_block": (ctypes.c_int, [c_vp, ctypes.c_int, c_f64p]),
     "ss_graph_read_probs": (ctypes.c_int, [c_vp, c_f64p]),
+    "ss_graph_profile": (ctypes.c_int, [c_vp, ctypes.c_int]),
+    "ss_graph_profile_read": (ctypes.c_int, [c_vp, ctypes.POINTER(KernelStat)]),
+    "ss_bench_copy": (ctypes.c_int, [c_vp, ctypes.c_size_t, ctypes.c_int, c_f64p]),
 }
 
 _lib = None

@@ -19,7 +19,7 @@ def _p(a, ct):
 
 
 def solve_opts(factor_seed=0, stop_rule=_lib.STOP_REF_ABS, max_iter=400, tol=None,
-               fail_relres=1e-2, precision=_lib.PREC_FP64):
+               fail_relres=1e-2, precision=_lib.PREC_FP64, ordering=_lib.ORDER_NATURAL):
     o = _lib.SolveOpts()
     load().ss_solve_opts_default(ctypes.byref(o))
     o.factor_seed = factor_seed
@@ -28,20 +28,27 @@ def solve_opts(factor_seed=0, stop_rule=_lib.STOP_REF_ABS, max_iter=400, tol=Non
     o.tol = (2e-7 if stop_rule == _lib.STOP_REF_ABS else 1e-8) if tol is None else tol
     o.fail_relres = fail_relres
     o.precision = precision
+    o.ordering = ordering
     return o
 
 
 class Factor:
     """Host approximate-Cholesky factor of a Laplacian given as CSC."""
 
-    def __init__(self, colptr, rowidx, vals, seed=0):
+    def __init__(self, colptr, rowidx, vals, seed=0, ordering=_lib.ORDER_NATURAL):
         lib = load()
         self._cp, self._ri, self._v = _i32(colptr), _i32(rowidx), _f64(vals)
         self.n = len(self._cp) - 1
         h = ctypes.c_void_p()
-        check(lib.ss_factor_build(self.n, _p(self._cp, ctypes.c_int32), _p(self._ri, ctypes.c_int32),
-                                  _p(self._v, ctypes.c_double), seed, ctypes.byref(h)))
+        check(lib.ss_factor_build_ordered(self.n, _p(self._cp, ctypes.c_int32), _p(self._ri, ctypes.c_int32),
+                                          _p(self._v, ctypes.c_double), seed, ordering, ctypes.byref(h)))
         self._h = h
+
+    def order(self):
+        o = np.zeros(self.n, np.int32)
+        r = ctypes.c_int32()
+        check(load().ss_factor_order(self._h, _p(o, ctypes.c_int32), ctypes.byref(r)))
+        return o, r.value
 
     def info(self):
         n, nnz = ctypes.c_int64(), ctypes.c_int64()
@@ -94,6 +101,12 @@ class Context:
         f, t = ctypes.c_size_t(), ctypes.c_size_t()
         check(load().ss_context_mem_info(self._h, ctypes.byref(f), ctypes.byref(t)))
         return f.value, t.value
+
+    def bench_copy(self, nbytes, reps=20):
+        """Average ms of a plain FP64 copy kernel over nbytes (read + write)."""
+        ms = ctypes.c_double()
+        check(load().ss_bench_copy(self._h, nbytes, reps, ctypes.byref(ms)))
+        return ms.value
 
     def sketch(self, v1, v2, w, n, k, seed=0, scale=1.0, uniform=False):
         v1, v2, w = _i32(v1), _i32(v2), _f64(w)
@@ -215,6 +228,15 @@ class Graph:
         out = np.zeros((self.n, self.k), np.float64)
         check(load().ss_graph_read_block(self._h, 0 if which == "Y" else 1, _p(out, ctypes.c_double)))
         return out
+
+    def profile(self, enable=True):
+        check(load().ss_graph_profile(self._h, int(bool(enable))))
+
+    def profile_read(self):
+        arr = (_lib.KernelStat * len(_lib.KERNELS))()
+        check(load().ss_graph_profile_read(self._h, arr))
+        return {name: {"ms_total": arr[i].ms_total, "launches": arr[i].launches}
+                for i, name in enumerate(_lib.KERNELS)}
 
     def read_probs(self):
         p = np.zeros(self.m, np.float64)

@@ -116,6 +116,11 @@ int ss_context_mem_info(ss_context *c, size_t *free_b, size_t *total_b) {
 // ---- factor (host) -------------------------------------------------------------
 int ss_factor_build(int64_t n, const int32_t *colptr, const int32_t *rowidx, const double *vals, uint64_t seed,
                     ss_factor **out) {
+    return ss_factor_build_ordered(n, colptr, rowidx, vals, seed, SS_ORDER_NATURAL, out);
+}
+
+int ss_factor_build_ordered(int64_t n, const int32_t *colptr, const int32_t *rowidx, const double *vals,
+                            uint64_t seed, int32_t ordering, ss_factor **out) {
     if (!out || !colptr || !rowidx || !vals) return fail(SS_ERR_INVALID, "null argument");
     SS_GUARD_BEGIN
     if (n <= 0) return fail(SS_ERR_INVALID, "n must be positive");
@@ -123,7 +128,7 @@ int ss_factor_build(int64_t n, const int32_t *colptr, const int32_t *rowidx, con
     for (int64_t i = 0; i < n; i++)
         if (colptr[i + 1] < colptr[i]) return fail(SS_ERR_INVALID, "colptr not monotone at %lld", (long long)i);
     ss_factor *f = new ss_factor();
-    int rc = factor_build(n, colptr, rowidx, vals, seed, f->F);
+    int rc = factor_build(n, colptr, rowidx, vals, seed, ordering, f->F);
     if (rc != SS_OK) {
         delete f;
         return rc;
@@ -149,6 +154,13 @@ int ss_factor_export(const ss_factor *f, int32_t *urowptr, int32_t *ucol, double
     if (ucol) memcpy(ucol, F.ucol.data(), sizeof(int32_t) * F.ucol.size());
     if (uval) memcpy(uval, F.uval.data(), sizeof(double) * F.uval.size());
     if (diag) memcpy(diag, F.diag.data(), sizeof(double) * F.diag.size());
+    return SS_OK;
+}
+
+int ss_factor_order(const ss_factor *f, int32_t *order, int32_t *rounds) {
+    if (!f) return fail(SS_ERR_INVALID, "null factor");
+    if (order) memcpy(order, f->F.order.data(), sizeof(int32_t) * f->F.order.size());
+    if (rounds) *rounds = f->F.rounds;
     return SS_OK;
 }
 
@@ -179,6 +191,7 @@ int ss_graph_destroy(ss_graph *g) {
     if (!g) return SS_OK;
     (void)hipSetDevice(g->ctx->device);
     (void)hipStreamSynchronize(g->ctx->stream);
+    for (hipEvent_t e : g->ev_pool) (void)hipEventDestroy(e);
     delete g;
     return SS_OK;
 }
@@ -240,6 +253,7 @@ int ss_graph_sparsify(ss_graph *g, const ss_sparsify_opts *o, ss_sparsify_report
     SS_TRY(graph_sample(g, o->k, o->beta, o->sampling_seed, nullptr, false));
     SS_HIP(hipEventRecord(c->ev[3], s));
     SS_HIP(hipEventSynchronize(c->ev[3]));
+    SS_TRY(prof_resolve(g));
     g->last_k = o->k;
     g->last_prec = o->solve.precision;
     if (rep) {
@@ -285,6 +299,48 @@ int ss_graph_read_probs(const ss_graph *g, double *p) {
     SS_HIP(hipSetDevice(g->ctx->device));
     if (g->m) SS_HIP(hipMemcpy(p, g->p, sizeof(double) * g->m, hipMemcpyDeviceToHost));
     return SS_OK;
+}
+
+int ss_graph_profile(ss_graph *g, int enable) {
+    if (!g) return fail(SS_ERR_INVALID, "null graph");
+    g->prof = enable != 0;
+    g->prof_recs.clear();
+    g->ev_used = 0;
+    for (int i = 0; i < SS_KERNEL_COUNT; i++) {
+        g->prof_ms[i] = 0.0;
+        g->prof_cnt[i] = 0;
+    }
+    return SS_OK;
+}
+
+int ss_graph_profile_read(const ss_graph *g, ss_kernel_stat *out) {
+    if (!g || !out) return fail(SS_ERR_INVALID, "null argument");
+    for (int i = 0; i < SS_KERNEL_COUNT; i++) {
+        out[i].ms_total = g->prof_ms[i];
+        out[i].launches = g->prof_cnt[i];
+    }
+    return SS_OK;
+}
+
+int ss_bench_copy(ss_context *ctx, size_t bytes, int reps, double *ms) {
+    if (!ctx || !ms || bytes < 8 || reps < 1) return fail(SS_ERR_INVALID, "bad arguments");
+    SS_GUARD_BEGIN
+    SS_HIP(hipSetDevice(ctx->device));
+    DevArena A;
+    double *a, *b;
+    const size_t count = bytes / 8;
+    SS_TRY(A.alloc(&a, count));
+    SS_TRY(A.alloc(&b, count));
+    hipStream_t s = ctx->stream;
+    SS_HIP(hipMemsetAsync(a, 0, count * 8, s));
+    launch_copy_f64(s, (int64_t)count, a, b);   // warm
+    SS_HIP(hipEventRecord(ctx->ev[6], s));
+    for (int r = 0; r < reps; r++) launch_copy_f64(s, (int64_t)count, a, b);
+    SS_HIP(hipEventRecord(ctx->ev[7], s));
+    SS_HIP(hipEventSynchronize(ctx->ev[7]));
+    *ms = elapsed(ctx->ev[6], ctx->ev[7]) / reps;
+    return SS_OK;
+    SS_GUARD_END
 }
 
 // ---- drop-in entry points with host buffers -------------------------------------
@@ -360,7 +416,7 @@ int ss_solve_lap(ss_context *ctx, const double *Y, int64_t n, int32_t k, const i
     HostGraph G;
     SS_TRY(graph_build(n, (int64_t)u.size(), u.data(), v.data(), w.data(), G));
     Factor F;
-    SS_TRY(factor_build(n, colptr, rowidx, vals, o.factor_seed, F));
+    SS_TRY(factor_build(n, colptr, rowidx, vals, o.factor_seed, o.ordering, F));
     ss_graph *g = nullptr;
     SS_TRY(graph_upload(ctx, G, &g));
     struct Del {

@@ -97,12 +97,28 @@ struct ss_graph {
     uint32_t *ticket = nullptr;
     int64_t *kept = nullptr;
     uint32_t samp_epoch = 0;
+    // per-kernel HIP-event timing (ss_graph_profile)
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_recs;
+    double prof_ms[SS_KERNEL_COUNT] = {};
+    int64_t prof_cnt[SS_KERNEL_COUNT] = {};
     int64_t last_kept = 0;
     int last_k = 0;
     int last_prec = 0;
 };
 
 namespace ss {
+// Records HIP events around one launch on the graph's stream when profiling.
+struct ProfScope {
+    ss_graph *g;
+    int kind;
+    hipEvent_t a = nullptr;
+    ProfScope(ss_graph *g_, int kind_);
+    ~ProfScope();
+};
+int prof_resolve(ss_graph *g);
 int graph_upload(ss_context *ctx, const HostGraph &G, ss_graph **out);
 int graph_ensure_work(ss_graph *g, int k, int prec);
 int graph_upload_factor(ss_graph *g, const Factor &F);

@@ -18,6 +18,44 @@ namespace ss {
 static const int kSpinLimit = 1 << 22;   // polls per wait (~seconds)
 
 // ---------------------------------------------------------------------------
+// per-kernel timing
+// ---------------------------------------------------------------------------
+static hipEvent_t prof_event(ss_graph *g) {
+    if (g->ev_used == g->ev_pool.size()) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        g->ev_pool.push_back(e);
+    }
+    return g->ev_pool[g->ev_used++];
+}
+
+ProfScope::ProfScope(ss_graph *g_, int kind_) : g(g_), kind(kind_) {
+    if (g->prof && (a = prof_event(g))) (void)hipEventRecord(a, g->ctx->stream);
+}
+
+ProfScope::~ProfScope() {
+    if (!g->prof || !a) return;
+    hipEvent_t b = prof_event(g);
+    if (!b) return;
+    (void)hipEventRecord(b, g->ctx->stream);
+    g->prof_recs.push_back({kind, {a, b}});
+}
+
+int prof_resolve(ss_graph *g) {
+    if (!g->prof) return SS_OK;
+    if (!g->prof_recs.empty()) SS_HIP(hipEventSynchronize(g->prof_recs.back().second.second));
+    for (auto &r : g->prof_recs) {
+        float ms = 0.f;
+        SS_HIP(hipEventElapsedTime(&ms, r.second.first, r.second.second));
+        g->prof_ms[r.first] += ms;
+        g->prof_cnt[r.first] += 1;
+    }
+    g->prof_recs.clear();
+    g->ev_used = 0;
+    return SS_OK;
+}
+
+// ---------------------------------------------------------------------------
 // structure upload
 // ---------------------------------------------------------------------------
 void build_incidence(int64_t n, int64_t m, const int32_t *v1, const int32_t *v2, const double *w,
@@ -108,7 +146,10 @@ int graph_upload_factor(ss_graph *g, const Factor &F) {
     if (F.n != g->n) return fail(SS_ERR_INVALID, "factor has n=%lld, graph has n=%lld", (long long)F.n, (long long)g->n);
     SS_HIP(hipSetDevice(g->ctx->device));
     Schedule fw, bw;
-    factor_schedules(F, fw, bw);
+    SS_TRY(factor_schedules(F, fw, bw));
+    // D per vertex (the factor keeps it per elimination position)
+    std::vector<double> vdiag((size_t)F.n);
+    for (int64_t p = 0; p < F.n; p++) vdiag[F.order[p]] = F.diag[p];
     g->arena_factor.release();
     g->has_factor = false;
     auto &A = g->arena_factor;
@@ -138,7 +179,7 @@ int graph_upload_factor(ss_graph *g, const Factor &F) {
     SS_TRY(h2d(s, g->b_col, bw.col.data(), g->b_nnz));
     SS_TRY(h2d(s, g->b_val, bw.val.data(), g->b_nnz));
     SS_TRY(h2d(s, g->b_valf, bvf.data(), g->b_nnz));
-    SS_TRY(h2d(s, g->diag, F.diag.data(), n));
+    SS_TRY(h2d(s, g->diag, vdiag.data(), n));
     SS_HIP(hipStreamSynchronize(s));
     g->factor_nnz = F.nnz;
     g->depth_fwd = F.depth_fwd;
@@ -228,9 +269,13 @@ static int precond(ss_graph *g, int k, const T *R, T *W, T *Zp) {
     hipStream_t s = g->ctx->stream;
     const int32_t n = (int32_t)g->n;
     uint32_t e1 = next_epoch(g);
+    {
+    ProfScope ps(g, SS_KERNEL_TRI_FWD);
     SS_HIP((launch_trisolve<T, false>(s, g->ctx->device, n, g->f_rows, g->f_ptr, g->f_col, fvals<T>(g), g->diag,
                                       k, R, W, g->flags, e1, g->status + 1, kSpinLimit)));
+    }
     uint32_t e2 = next_epoch(g);
+    ProfScope ps(g, SS_KERNEL_TRI_BWD);
     SS_HIP((launch_trisolve<T, true>(s, g->ctx->device, n, g->b_rows, g->b_ptr, g->b_col, bvals<T>(g), g->diag, k,
                                      W, Zp, g->flags, e2, g->status + 1, kSpinLimit)));
     return SS_OK;
@@ -268,7 +313,10 @@ static int pcg_run(ss_graph *g, int k, const T *rhs, T *X, const PcgWork<T> &w, 
     launch_pcg_init(s, k, g->partial, g->partial2, nb, g->sc);
     SS_HIP(hipMemcpyAsync(w.P, w.Zp, sizeof(T) * (size_t)nk, hipMemcpyDeviceToDevice, s));
     for (int it = 0; it < max_iter; it++) {
-        launch_spmm<T>(s, n, g->rowptr, g->col, avals<T>(g), k, w.P, w.Q);
+        {
+            ProfScope ps(g, SS_KERNEL_SPMM);
+            launch_spmm<T>(s, n, g->rowptr, g->col, avals<T>(g), k, w.P, w.Q);
+        }
         launch_col_partial<T, T>(s, n, k, w.P, w.Q, 0, g->partial);
         launch_pcg_alpha(s, k, g->partial, nb, g->sc);
         launch_update_xr<T>(s, nk, k, g->sc.alpha, w.P, w.Q, X, w.R);
@@ -319,7 +367,10 @@ int graph_solve(ss_graph *g, int k, const ss_solve_opts &o, SolveStats &st) {
     if (g->wprec == SS_PREC_FP64) {
         SS_TRY(pcg_run<double>(g, k, g->Y64, g->X64, g->w64, o.stop_rule, o.tol, o.max_iter, st.iters, bnorm));
         // true residual b - A x  (include/custom_cg.hpp:245-258)
-        launch_residual64(s, n, g->rowptr, g->col, g->val, k, g->X64, g->w64.B, g->w64.Q);
+        {
+            ProfScope ps(g, SS_KERNEL_SPMM);
+            launch_residual64(s, n, g->rowptr, g->col, g->val, k, g->X64, g->w64.B, g->w64.Q);
+        }
         SS_TRY(col_norms2(g, k, g->w64.Q, rn2));
     } else {
         // FP64 outer refinement around an FP32 PCG.
@@ -348,7 +399,10 @@ int graph_solve(ss_graph *g, int k, const ss_solve_opts &o, SolveStats &st) {
                                   bn_inner));
             for (int j = 0; j < k; j++) st.iters[j] += it_sweep[j];
             launch_accumulate<float>(s, nk, g->X32, g->X64);
-            launch_residual64(s, n, g->rowptr, g->col, g->val, k, g->X64, g->B64, g->R64);
+            {
+                ProfScope ps(g, SS_KERNEL_SPMM);
+                launch_residual64(s, n, g->rowptr, g->col, g->val, k, g->X64, g->B64, g->R64);
+            }
             SS_TRY(col_norms2(g, k, g->R64, rn2));
             st.refinements++;
         }
@@ -366,6 +420,7 @@ int graph_solve(ss_graph *g, int k, const ss_solve_opts &o, SolveStats &st) {
 int graph_sketch(ss_graph *g, int k, uint64_t seed, double scale, int uniform, int prec) {
     SS_TRY(graph_ensure_work(g, k, prec));
     SS_HIP(hipSetDevice(g->ctx->device));
+    ProfScope ps(g, SS_KERNEL_SKETCH);
     launch_sketch<double>(g->ctx->stream, (int32_t)g->n, g->iptr, g->ieid, g->ib, k, seed, scale, uniform, 0,
                           g->Y64);
     SS_HIP(hipGetLastError());
@@ -408,7 +463,10 @@ int graph_sample(ss_graph *g, int k, double beta, uint64_t sampling_seed, const 
         g->last_kept = 0;
         return SS_OK;
     }
-    launch_er_sample<double>(s, a);
+    {
+        ProfScope ps(g, SS_KERNEL_ER_SAMPLE);
+        launch_er_sample<double>(s, a);
+    }
     SS_HIP(hipGetLastError());
     SS_HIP(hipMemcpyAsync(&g->last_kept, g->kept, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     SS_TRY(read_status(g, nullptr));

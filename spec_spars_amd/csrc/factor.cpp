@@ -59,25 +59,65 @@ struct Fill {
 
 }  // namespace
 
-int factor_build(int64_t n, const int32_t *colptr, const int32_t *rowidx,
-                 const double *vals, uint64_t seed, Factor &F) {
-    auto t0 = std::chrono::steady_clock::now();
-    if (n <= 0 || n > INT32_MAX - 1) return fail(SS_ERR_INVALID, "factor: bad n=%lld", (long long)n);
-    F.n = n;
+// Eliminate one vertex whose live lower neighbours (row, |L_rc| + fill) are
+// in `work`: merge equal rows, sort by value, sample one fill edge per entry
+// but the last, scale.  Fill edges go to emit(a, b, value); the column of L
+// (rows ascending, values -v/tot) to `col`.  Returns D = tot (0 if empty).
+template <typename Emit>
+static double eliminate(std::vector<Ent> &work, std::vector<double> &cum, ColumnRng &rng, std::vector<Ent> &col,
+                        Emit emit) {
+    col.clear();
+    if (work.empty()) return 0.0;
+    std::sort(work.begin(), work.end(), [](const Ent &a, const Ent &b) {
+        return a.row != b.row ? a.row < b.row : a.val < b.val;
+    });
+    size_t cnt = 0;
+    for (size_t i = 0; i < work.size(); i++) {
+        if (cnt > 0 && work[cnt - 1].row == work[i].row) work[cnt - 1].val += work[i].val;
+        else work[cnt++] = work[i];
+    }
+    work.resize(cnt);
+    std::sort(work.begin(), work.end(), [](const Ent &a, const Ent &b) {
+        return a.val != b.val ? a.val < b.val : a.row < b.row;
+    });
+    cum.resize(cnt);
+    double run = 0.0;
+    for (size_t i = 0; i < cnt; i++) { run += work[i].val; cum[i] = run; }
+    const double tot = run;
+    for (size_t i = 0; i + 1 < cnt; i++) {
+        const double rest = tot - cum[i];
+        const double target = rng.uniform() * rest;
+        // first j > i with cum[j] - cum[i] >= target (std::lower_bound in the
+        // reference, src/example.cc:266-269)
+        size_t lo = i + 1, hi = cnt;
+        while (lo < hi) {
+            size_t mid = (lo + hi) / 2;
+            if (cum[mid] - cum[i] < target) lo = mid + 1; else hi = mid;
+        }
+        const size_t j = lo < cnt ? lo : cnt - 1;
+        emit(work[i].row, work[j].row, work[i].val * rest / tot);
+    }
+    for (size_t i = 0; i < cnt; i++) col.push_back({work[i].row, -(work[i].val / tot)});
+    std::sort(col.begin(), col.end(), [](const Ent &a, const Ent &b) { return a.row < b.row; });
+    return tot;
+}
+
+// Natural order on the given CSC (positions == indices of the CSC).
+static int factor_natural(int64_t n, const int32_t *colptr, const int32_t *rowidx, const double *vals,
+                          uint64_t seed, std::vector<int64_t> &cstart, std::vector<int32_t> &crow,
+                          std::vector<double> &cval, std::vector<double> &diag) {
     std::vector<int32_t> head((size_t)n, -1);
     std::vector<Fill> pool;
     pool.reserve((size_t)colptr[n]);
-    // Per-column output: rows and values of L below the diagonal.
-    std::vector<int64_t> cstart((size_t)n + 1, 0);
-    std::vector<int32_t> crow;
-    std::vector<double> cval;
+    cstart.assign((size_t)n + 1, 0);
+    crow.clear();
+    cval.clear();
     crow.reserve((size_t)colptr[n]);
     cval.reserve((size_t)colptr[n]);
-    F.diag.assign((size_t)n, 0.0);
-
-    std::vector<Ent> work;
+    diag.assign((size_t)n, 0.0);
+    std::vector<Ent> work, col;
     std::vector<double> cum;
-    work.reserve(64);
+    bool overflow = false;
     for (int64_t c = 0; c < n; c++) {
         work.clear();
         for (int32_t t = colptr[c]; t < colptr[c + 1]; t++) {
@@ -86,58 +126,175 @@ int factor_build(int64_t n, const int32_t *colptr, const int32_t *rowidx,
             if (r > c) work.push_back({r, std::fabs(vals[t])});
         }
         for (int32_t f = head[c]; f >= 0; f = pool[f].next) work.push_back({pool[f].row, pool[f].val});
-        cstart[c] = (int64_t)crow.size();
-        if (work.empty()) {            // no lower entries: D = 0 (grounds the component)
-            F.diag[c] = 0.0;
-            cstart[c + 1] = cstart[c];
-            continue;
-        }
-        // merge equal rows, summing in ascending value order
-        std::sort(work.begin(), work.end(), [](const Ent &a, const Ent &b) {
-            return a.row != b.row ? a.row < b.row : a.val < b.val;
-        });
-        size_t cnt = 0;
-        for (size_t i = 0; i < work.size(); i++) {
-            if (cnt > 0 && work[cnt - 1].row == work[i].row) work[cnt - 1].val += work[i].val;
-            else work[cnt++] = work[i];
-        }
-        work.resize(cnt);
-        std::sort(work.begin(), work.end(), [](const Ent &a, const Ent &b) {
-            return a.val != b.val ? a.val < b.val : a.row < b.row;
-        });
-        cum.resize(cnt);
-        double run = 0.0;
-        for (size_t i = 0; i < cnt; i++) { run += work[i].val; cum[i] = run; }
-        const double tot = run;
-        ColumnRng rng(seed, c);
-        for (size_t i = 0; i + 1 < cnt; i++) {
-            double rest = tot - cum[i];
-            double target = rng.uniform() * rest;
-            // first j > i with cum[j] - cum[i] >= target
-            size_t lo = i + 1, hi = cnt;
-            while (lo < hi) {
-                size_t mid = (lo + hi) / 2;
-                if (cum[mid] - cum[i] < target) lo = mid + 1; else hi = mid;
-            }
-            size_t j = lo < cnt ? lo : cnt - 1;
-            int32_t a = work[i].row, b = work[j].row;
-            int32_t frow = std::max(a, b), fcol = std::min(a, b);
-            double fval = work[i].val * rest / tot;
-            if (pool.size() >= (size_t)INT32_MAX) return fail(SS_ERR_CAPACITY, "factor: fill pool exceeds 2^31 entries");
-            pool.push_back({frow, head[fcol], fval});
-            head[fcol] = (int32_t)(pool.size() - 1);
-        }
-        F.diag[c] = tot;
-        std::sort(work.begin(), work.end(), [](const Ent &a, const Ent &b) { return a.row < b.row; });
-        for (size_t i = 0; i < cnt; i++) {
-            crow.push_back(work[i].row);
-            cval.push_back(-(work[i].val / tot));
-        }
-        cstart[c + 1] = (int64_t)crow.size();
-        // the fill lists of column c are no longer needed
         head[c] = -1;
+        ColumnRng rng(seed, c);
+        // fill edge (max, min) goes to column min (src/example.cc:270-286)
+        diag[c] = eliminate(work, cum, rng, col, [&](int32_t a, int32_t b, double fv) {
+            if (pool.size() >= (size_t)INT32_MAX) { overflow = true; return; }
+            const int32_t fr = std::max(a, b), fc = std::min(a, b);
+            pool.push_back({fr, head[fc], fv});
+            head[fc] = (int32_t)(pool.size() - 1);
+        });
+        if (overflow) return fail(SS_ERR_CAPACITY, "factor: fill pool exceeds 2^31 entries");
+        for (const Ent &e : col) { crow.push_back(e.row); cval.push_back(e.val); }
+        cstart[c + 1] = (int64_t)crow.size();
     }
-    int64_t nnz_lower = (int64_t)crow.size();
+    return SS_OK;
+}
+
+// Rounds: every round eliminates an independent set of the current
+// fill-augmented graph, picked greedily in (live degree, random) order among
+// the vertices of degree at most twice the round's minimum.  A round's
+// vertices never depend on each other, so every round is one level of both
+// triangular solves; low degree first keeps fill (and PCG iterations) at or
+// below the natural order's.
+static int factor_rounds(int64_t n, const int32_t *colptr, const int32_t *rowidx, const double *vals,
+                         uint64_t seed, std::vector<int32_t> &order, std::vector<int64_t> &vstart,
+                         std::vector<int32_t> &crow, std::vector<double> &cval, std::vector<double> &vdiag,
+                         int32_t &rounds) {
+    std::vector<std::vector<Ent>> adj((size_t)n);
+    for (int64_t c = 0; c < n; c++)
+        for (int32_t t = colptr[c]; t < colptr[c + 1]; t++) {
+            int32_t r = rowidx[t];
+            if (r < 0 || r >= n) return fail(SS_ERR_INVALID, "factor: row index %d out of range", r);
+            if (r != c) adj[c].push_back({r, std::fabs(vals[t])});
+        }
+    std::vector<char> elim((size_t)n, 0);
+    std::vector<uint64_t> key((size_t)n);
+    std::vector<int32_t> live((size_t)n), sel, next, cand;
+    std::vector<int32_t> blocked((size_t)n, 0);   // == round + 1: a neighbour was picked
+    for (int64_t i = 0; i < n; i++) live[i] = (int32_t)i;
+    order.clear();
+    order.reserve((size_t)n);
+    vstart.assign((size_t)n, 0);
+    std::vector<int64_t> vcount((size_t)n, 0);
+    crow.clear();
+    cval.clear();
+    vdiag.assign((size_t)n, 0.0);
+    std::vector<Ent> work, col;
+    std::vector<double> cum;
+    rounds = 0;
+    while (!live.empty()) {
+        const uint64_t rkey = mix64(seed ^ (0x94d049bb133111ebULL * (uint64_t)(rounds + 1)));
+        for (int32_t v : live) {
+            auto &a = adj[v];
+            size_t w = 0;
+            for (const Ent &e : a)
+                if (!elim[e.row]) a[w++] = e;
+            a.resize(w);
+            const uint64_t deg = std::min<uint64_t>(w, 0xffffff);
+            key[v] = (deg << 40) | (mix64(rkey ^ (uint64_t)v) & ((1ULL << 40) - 1));
+        }
+        // greedy maximal independent set in key order, restricted to
+        // vertices of degree <= 2x the round's minimum (min-degree flavour)
+        cand.assign(live.begin(), live.end());
+        std::sort(cand.begin(), cand.end(),
+                  [&](int32_t x, int32_t y) { return key[x] != key[y] ? key[x] < key[y] : x < y; });
+        const uint64_t capdeg = std::max<uint64_t>(2 * (key[cand[0]] >> 40), 2);
+        sel.clear();
+        for (int32_t v : cand) {
+            if ((key[v] >> 40) > capdeg) break;
+            if (blocked[v] == rounds + 1) continue;
+            sel.push_back(v);
+            for (const Ent &e : adj[v]) blocked[e.row] = rounds + 1;
+        }
+        std::sort(sel.begin(), sel.end());
+        for (int32_t v : sel) {
+            order.push_back(v);
+            work.assign(adj[v].begin(), adj[v].end());
+            ColumnRng rng(seed, v);
+            vdiag[v] = eliminate(work, cum, rng, col, [&](int32_t a, int32_t b, double fv) {
+                adj[a].push_back({b, fv});
+                adj[b].push_back({a, fv});
+            });
+            vstart[v] = (int64_t)crow.size();
+            vcount[v] = (int64_t)col.size();
+            for (const Ent &e : col) { crow.push_back(e.row); cval.push_back(e.val); }
+            elim[v] = 1;
+            std::vector<Ent>().swap(adj[v]);
+        }
+        next.clear();
+        for (int32_t v : live)
+            if (!elim[v]) next.push_back(v);
+        live.swap(next);
+        rounds++;
+    }
+    // vstart/vcount are per vertex; pack them as [start, start+count)
+    std::vector<int64_t> packed((size_t)n * 2);
+    for (int64_t v = 0; v < n; v++) { packed[2 * v] = vstart[v]; packed[2 * v + 1] = vcount[v]; }
+    vstart.swap(packed);
+    return SS_OK;
+}
+
+int factor_build(int64_t n, const int32_t *colptr, const int32_t *rowidx, const double *vals, uint64_t seed,
+                 int ordering, Factor &F) {
+    auto t0 = std::chrono::steady_clock::now();
+    if (n <= 0 || n > INT32_MAX - 1) return fail(SS_ERR_INVALID, "factor: bad n=%lld", (long long)n);
+    if (ordering < ORDER_NATURAL || ordering > ORDER_ROUNDS) return fail(SS_ERR_INVALID, "factor: unknown ordering %d", ordering);
+    F.n = n;
+    F.ordering = ordering;
+    F.rounds = 0;
+    std::vector<int32_t> rank((size_t)n);
+    // per-position column of L in position ids: start, rows, values
+    std::vector<int64_t> pstart((size_t)n + 1, 0);
+    std::vector<int32_t> prow;
+    std::vector<double> pval;
+    if (ordering == ORDER_ROUNDS) {
+        std::vector<int64_t> vsc;
+        std::vector<int32_t> crow;
+        std::vector<double> cval, vdiag;
+        SS_TRY(factor_rounds(n, colptr, rowidx, vals, seed, F.order, vsc, crow, cval, vdiag, F.rounds));
+        for (int64_t p = 0; p < n; p++) rank[F.order[p]] = (int32_t)p;
+        F.diag.resize((size_t)n);
+        prow.reserve(crow.size());
+        pval.reserve(cval.size());
+        std::vector<Ent> tmp;
+        for (int64_t p = 0; p < n; p++) {
+            const int32_t v = F.order[p];
+            F.diag[p] = vdiag[v];
+            tmp.clear();
+            for (int64_t q = vsc[2 * v]; q < vsc[2 * v] + vsc[2 * v + 1]; q++) tmp.push_back({rank[crow[q]], cval[q]});
+            std::sort(tmp.begin(), tmp.end(), [](const Ent &a, const Ent &b) { return a.row < b.row; });
+            for (const Ent &e : tmp) { prow.push_back(e.row); pval.push_back(e.val); }
+            pstart[p + 1] = (int64_t)prow.size();
+        }
+    } else {
+        F.order.resize((size_t)n);
+        for (int64_t i = 0; i < n; i++) F.order[i] = (int32_t)i;
+        if (ordering == ORDER_RANDOM) {
+            // Fisher-Yates with the factor seed
+            ColumnRng rng(seed ^ 0x2545f4914f6cdd1dULL, -1);
+            for (int64_t i = n - 1; i > 0; i--) {
+                int64_t j = (int64_t)(rng.uniform() * (double)(i + 1));
+                if (j > i) j = i;
+                std::swap(F.order[i], F.order[j]);
+            }
+        }
+        for (int64_t p = 0; p < n; p++) rank[F.order[p]] = (int32_t)p;
+        if (ordering == ORDER_NATURAL) {
+            SS_TRY(factor_natural(n, colptr, rowidx, vals, seed, pstart, prow, pval, F.diag));
+        } else {
+            // the permuted Laplacian P L P^T as CSC
+            std::vector<int32_t> pc((size_t)n + 1, 0);
+            for (int64_t c = 0; c < n; c++) pc[rank[c] + 1] = colptr[c + 1] - colptr[c];
+            for (int64_t i = 0; i < n; i++) pc[i + 1] += pc[i];
+            std::vector<int32_t> pr((size_t)colptr[n]);
+            std::vector<double> pv((size_t)colptr[n]);
+            std::vector<Ent> tmp;
+            for (int64_t c = 0; c < n; c++) {
+                tmp.clear();
+                for (int32_t t = colptr[c]; t < colptr[c + 1]; t++) {
+                    if (rowidx[t] < 0 || rowidx[t] >= n) return fail(SS_ERR_INVALID, "factor: row index %d out of range", rowidx[t]);
+                    tmp.push_back({rank[rowidx[t]], vals[t]});
+                }
+                std::sort(tmp.begin(), tmp.end(), [](const Ent &a, const Ent &b) { return a.row < b.row; });
+                int32_t o = pc[rank[c]];
+                for (size_t q = 0; q < tmp.size(); q++) { pr[o + q] = tmp[q].row; pv[o + q] = tmp[q].val; }
+            }
+            SS_TRY(factor_natural(n, pc.data(), pr.data(), pv.data(), seed, pstart, prow, pval, F.diag));
+        }
+    }
+    const int64_t nnz_lower = (int64_t)prow.size();
     if (nnz_lower + n > (int64_t)INT32_MAX) return fail(SS_ERR_CAPACITY, "factor: nnz %lld exceeds int32", (long long)(nnz_lower + n));
     // U CSR = columns of L, diagonal first (src/example.cc:422-470)
     F.urowptr.assign((size_t)n + 1, 0);
@@ -147,9 +304,7 @@ int factor_build(int64_t n, const int32_t *colptr, const int32_t *rowidx,
     for (int64_t c = 0; c < n; c++) {
         F.urowptr[c] = (int32_t)p;
         F.ucol[p] = (int32_t)c; F.uval[p] = 1.0; p++;
-        for (int64_t t = cstart[c]; t < cstart[c + 1]; t++) {
-            F.ucol[p] = crow[t]; F.uval[p] = cval[t]; p++;
-        }
+        for (int64_t t = pstart[c]; t < pstart[c + 1]; t++) { F.ucol[p] = prow[t]; F.uval[p] = pval[t]; p++; }
     }
     F.urowptr[n] = (int32_t)p;
     F.nnz = p;
@@ -186,17 +341,18 @@ void factor_levels(Factor &F) {
     }
 }
 
-// Level-ordered CSR for the two solves.  Position t of the forward schedule
-// holds row perm[t] (rows sorted by (level, row)); its entries are the
-// strictly-lower row of L (pull form).  The backward schedule holds the
-// strictly-upper rows of L^T, i.e. the off-diagonal part of U's rows.
-void factor_schedules(const Factor &F, Schedule &fwd, Schedule &bwd) {
+// Level-ordered CSR for the two solves, in vertex ids.  Position t of the
+// forward schedule holds the vertex at elimination position perm[t]
+// (positions sorted by (level, position)); its entries are that row of L in
+// pull form (the vertices it waits for).  The backward schedule holds the
+// rows of L^T, i.e. the off-diagonal part of U's rows.  Every dependency
+// must have a strictly lower level -- the sync-free solve relies on it.
+int factor_schedules(const Factor &F, Schedule &fwd, Schedule &bwd) {
     const int64_t n = F.n;
-    auto order = [&](const std::vector<int32_t> &lvl, Schedule &S) {
+    auto order = [&](const std::vector<int32_t> &lvl, Schedule &S, std::vector<int32_t> &pos_at) {
         S.n = n;
         S.depth = 0;
         for (int64_t i = 0; i < n; i++) S.depth = std::max<int32_t>(S.depth, lvl[i]);
-        // counting sort by level, stable in row id
         std::vector<int64_t> cnt((size_t)S.depth + 2, 0);
         for (int64_t i = 0; i < n; i++) cnt[lvl[i]]++;
         S.level_ptr.assign((size_t)S.depth + 1, 0);
@@ -205,13 +361,14 @@ void factor_schedules(const Factor &F, Schedule &fwd, Schedule &bwd) {
         S.level_ptr[S.depth] = (int32_t)acc;
         std::vector<int64_t> pos((size_t)S.depth + 1);
         for (int32_t l = 1; l <= S.depth; l++) pos[l] = S.level_ptr[l - 1];
-        S.rows.resize((size_t)n);
-        for (int64_t i = 0; i < n; i++) S.rows[pos[lvl[i]]++] = (int32_t)i;
+        pos_at.resize((size_t)n);
+        for (int64_t i = 0; i < n; i++) pos_at[pos[lvl[i]]++] = (int32_t)i;   // schedule slot -> position
     };
-    order(F.lvl_fwd, fwd);
-    order(F.lvl_bwd, bwd);
+    std::vector<int32_t> fslot, bslot;
+    order(F.lvl_fwd, fwd, fslot);
+    order(F.lvl_bwd, bwd, bslot);
 
-    // strictly-lower rows of L: transpose of U's off-diagonal part
+    // strictly-lower rows of L (position space): transpose of U's off-diagonal
     std::vector<int32_t> lcnt((size_t)n + 1, 0);
     for (int64_t i = 0; i < n; i++)
         for (int32_t t = F.urowptr[i]; t < F.urowptr[i + 1]; t++)
@@ -230,11 +387,13 @@ void factor_schedules(const Factor &F, Schedule &fwd, Schedule &bwd) {
                 fillp[r]++;
             }
     }
-    auto pack = [&](Schedule &S, const int32_t *ptr, const int32_t *col, const double *val, bool skip_diag) {
+    auto pack = [&](Schedule &S, const std::vector<int32_t> &slot, const std::vector<int32_t> &lvl,
+                    const int32_t *ptr, const int32_t *col, const double *val, bool skip_diag) -> int {
         S.ptr.assign((size_t)n + 1, 0);
+        S.rows.resize((size_t)n);
         int64_t total = 0;
         for (int64_t t = 0; t < n; t++) {
-            int32_t i = S.rows[t];
+            const int32_t i = slot[t];
             S.ptr[t] = (int32_t)total;
             for (int32_t q = ptr[i]; q < ptr[i + 1]; q++)
                 if (!(skip_diag && col[q] == i)) total++;
@@ -244,17 +403,22 @@ void factor_schedules(const Factor &F, Schedule &fwd, Schedule &bwd) {
         S.val.resize((size_t)total);
         int64_t p = 0;
         for (int64_t t = 0; t < n; t++) {
-            int32_t i = S.rows[t];
+            const int32_t i = slot[t];
+            S.rows[t] = F.order[i];
             for (int32_t q = ptr[i]; q < ptr[i + 1]; q++) {
                 if (skip_diag && col[q] == i) continue;
-                S.col[p] = col[q];
+                if (lvl[col[q]] >= lvl[i])
+                    return fail(SS_ERR_INVALID, "schedule: position %d depends on position %d of level >= its own", i, col[q]);
+                S.col[p] = F.order[col[q]];
                 S.val[p] = val[q];
                 p++;
             }
         }
+        return SS_OK;
     };
-    pack(fwd, lcnt.data(), lcol.data(), lval.data(), false);
-    pack(bwd, F.urowptr.data(), F.ucol.data(), F.uval.data(), true);
+    SS_TRY(pack(fwd, fslot, F.lvl_fwd, lcnt.data(), lcol.data(), lval.data(), false));
+    SS_TRY(pack(bwd, bslot, F.lvl_bwd, F.urowptr.data(), F.ucol.data(), F.uval.data(), true));
+    return SS_OK;
 }
 
 }  // namespace ss

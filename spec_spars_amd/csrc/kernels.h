@@ -72,6 +72,7 @@ void launch_convert(hipStream_t s, int64_t count, const TS *src, TD *dst);
 template <typename T>
 void launch_accumulate(hipStream_t s, int64_t count, const T *d, double *x);
 void launch_fill_u32(hipStream_t s, int64_t count, uint32_t *p, uint32_t v);
+void launch_copy_f64(hipStream_t s, int64_t count, const double *a, double *b);
 template <typename T, bool BWD>
 hipError_t launch_trisolve(hipStream_t s, int device, int32_t n, const int32_t *rows, const int32_t *ptr,
                            const int32_t *col, const T *val, const double *diag, int k, const T *in, T *out,

@@ -246,6 +246,13 @@ __global__ __launch_bounds__(256) void k_accumulate(int64_t count, const T *__re
         x[t] += (double)d[t];
 }
 
+__global__ __launch_bounds__(256) void k_copy_f64(int64_t count, const double *__restrict__ a,
+                                                  double *__restrict__ b) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < count;
+         t += (int64_t)gridDim.x * blockDim.x)
+        b[t] = a[t];
+}
+
 __global__ void k_fill_u32(int64_t count, uint32_t *p, uint32_t v) {
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < count;
          t += (int64_t)gridDim.x * blockDim.x)
@@ -419,8 +426,8 @@ __global__ __launch_bounds__(256) void k_er_sample(ErSampleArgs<TZ> a) {
         if ((int64_t)tile == a.ntiles - 1) *a.kept_total = excl + (int64_t)total;
     }
     __syncthreads();
-    if (keep) {
-        const int64_t pos = s_excl + woff + wpre;
+    const int64_t pos = s_excl + woff + wpre;
+    if (keep && pos < a.m) {
         a.ku[pos] = eu;
         a.kv[pos] = ev;
         a.kw[pos] = kwv;
@@ -535,6 +542,10 @@ void launch_accumulate(hipStream_t s, int64_t count, const T *d, double *x) {
 }
 template void launch_accumulate<float>(hipStream_t, int64_t, const float *, double *);
 template void launch_accumulate<double>(hipStream_t, int64_t, const double *, double *);
+
+void launch_copy_f64(hipStream_t s, int64_t count, const double *a, double *b) {
+    hipLaunchKernelGGL(k_copy_f64, dim3(grid_for(count, 256, 8192)), dim3(256), 0, s, count, a, b);
+}
 
 void launch_fill_u32(hipStream_t s, int64_t count, uint32_t *p, uint32_t v) {
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(count, 256, kStreamGrid)), dim3(256), 0, s, count, p, v);

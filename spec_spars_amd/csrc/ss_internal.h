@@ -39,9 +39,18 @@ inline int fail(int code, const char *fmt, ...) {
     } while (0)
 
 // ---- host factor --------------------------------------------------------
+// Elimination orders (the randomized elimination itself is the same).
+enum Ordering { ORDER_NATURAL = 0, ORDER_RANDOM = 1, ORDER_ROUNDS = 2 };
+
+// The factor is stored in elimination-position space: position p is vertex
+// order[p]; row p of U lists column p of L (positions > p, diagonal first),
+// the reference precond_M layout.  D and the levels are per position.
 struct Factor {
     int64_t n = 0;
     int64_t nnz = 0;                  // incl. unit diagonal
+    int32_t ordering = ORDER_NATURAL;
+    int32_t rounds = 0;               // elimination rounds (ORDER_ROUNDS)
+    std::vector<int32_t> order;       // position -> vertex
     std::vector<int32_t> urowptr;     // reference precond_M layout
     std::vector<int32_t> ucol;
     std::vector<double> uval;
@@ -63,9 +72,11 @@ struct Schedule {
 };
 
 int factor_build(int64_t n, const int32_t *colptr, const int32_t *rowidx, const double *vals,
-                 uint64_t seed, Factor &F);
+                 uint64_t seed, int ordering, Factor &F);
 void factor_levels(Factor &F);
-void factor_schedules(const Factor &F, Schedule &fwd, Schedule &bwd);
+// Level-ordered schedules in vertex ids; checks that every dependency sits
+// at a strictly lower level.
+int factor_schedules(const Factor &F, Schedule &fwd, Schedule &bwd);
 
 // ---- host graph structure -----------------------------------------------
 // Canonical edges (u < v, sorted by (v, u)) -> full Laplacian CSR with the

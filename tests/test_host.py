@@ -140,3 +140,56 @@ def test_generators_canonical_and_sizes():
     w = graphs.loguniform_weights(100000, seed=0)
     assert w.min() >= 1e-3 and w.max() <= 1e3
     assert abs(np.log10(w).mean()) < 0.05
+
+
+@pytest.mark.parametrize("ordering", [1, 2])
+def test_ordered_factor_invariants_and_levels(ordering):
+    import oracle
+    import spec_spars_amd as ss
+    from spec_spars_amd import graphs
+    for n, u, v in (graphs.grid3d(8), graphs.rgg(900, seed=1), graphs.chung_lu(700, 2500, seed=3)):
+        w = graphs.loguniform_weights(len(u), seed=1)
+        cp, ri, va = oracle.laplacian(n, u, v, w)
+        f = ss.Factor(cp, ri, va, seed=2, ordering=ordering)
+        order, rounds = f.order()
+        assert np.array_equal(np.sort(order), np.arange(n))          # a permutation
+        urp, uci, uva, d = f.export()
+        for c in range(n):
+            cols = uci[urp[c]:urp[c + 1]]
+            assert cols[0] == c and np.all(np.diff(cols) > 0)
+            if len(cols) > 1:
+                assert abs(uva[urp[c] + 1:urp[c + 1]].sum() + 1.0) < 1e-12
+        assert int((d == 0).sum()) == components(n, u, v)
+        lf, lb = f.levels()
+        of, ob = oracle.levels(n, urp, uci)
+        assert np.array_equal(lf, of) and np.array_equal(lb, ob)
+        if ordering == 2:
+            # a round is an independent set, so the depth never exceeds the rounds
+            assert 1 <= f.info()["depth_fwd"] <= rounds and f.info()["depth_bwd"] <= rounds
+
+
+def test_ordered_factor_preconditions_the_permuted_system():
+    # the exported factor (position space) preconditions P L P^T: the oracle
+    # PCG on the permuted system converges like the natural one
+    import oracle
+    import spec_spars_amd as ss
+    from spec_spars_amd import graphs
+    n, u, v = graphs.grid3d(12)
+    w = graphs.loguniform_weights(len(u), seed=3)
+    cp, ri, va = oracle.laplacian(n, u, v, w)
+    Y = np.random.default_rng(0).standard_normal((n, 3))
+    fn = ss.Factor(cp, ri, va, seed=1)
+    Zn, itn, _ = oracle.solve_lap(Y, cp, ri, va, *fn.export(), stop_rule=1, tol=1e-10)
+    f = ss.Factor(cp, ri, va, seed=1, ordering=2)
+    order, _ = f.order()
+    rank = np.empty(n, np.int64)
+    rank[order] = np.arange(n)
+    pu, pv, pw = graphs.canonical(rank[u], rank[v], w)
+    pcp, pri, pva = oracle.laplacian(n, pu, pv, pw)
+    Zp, itp, rr = oracle.solve_lap(Y[order], pcp, pri, pva, *f.export(), stop_rule=1, tol=1e-10)
+    Z = np.empty_like(Zp)
+    Z[order] = Zp
+    a, b = Z - Z.mean(0), Zn - Zn.mean(0)
+    assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-7
+    assert itp.max() <= itn.max() + 5
+    assert f.info()["depth_fwd"] * 20 < fn.info()["depth_fwd"]
