@@ -13,3 +13,4 @@ _lib.load()
 
 from .api import Context, Factor, Graph, solve_opts  # noqa: E402,F401
 from . import graphs  # noqa: E402,F401
+from .sparsifier import SparsifierParameters, Sparsifier  # noqa: E402,F401

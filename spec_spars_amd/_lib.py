@@ -59,6 +59,17 @@ class SparsifyReport(ctypes.Structure):
                 ("ms_sample", ctypes.c_double), ("ms_total", ctypes.c_double)]
 
 
+class HostParams(ctypes.Structure):
+    _fields_ = [("epsilon", ctypes.c_double), ("beta_constant", ctypes.c_int32),
+                ("row_constant", ctypes.c_int32), ("verbose", ctypes.c_int32),
+                ("sketch_uniform", ctypes.c_int32), ("jl_factor", ctypes.c_double),
+                ("jl_scaling_factor", ctypes.c_double), ("sketch_seed", ctypes.c_uint64),
+                ("sampling_seed", ctypes.c_uint64), ("streaming", ctypes.c_int32),
+                ("ordering", ctypes.c_int32), ("factor_seed", ctypes.c_uint64),
+                ("precision", ctypes.c_int32), ("stop_rule", ctypes.c_int32),
+                ("max_iter", ctypes.c_int32), ("reserved", ctypes.c_int32), ("tol", ctypes.c_double)]
+
+
 KERNELS = ["sketch", "spmm", "tri_fwd", "tri_bwd", "er_sample"]
 
 
@@ -110,6 +121,20 @@ SIGNATURES = {
     "ss_graph_profile": (ctypes.c_int, [c_vp, ctypes.c_int]),
     "ss_graph_profile_read": (ctypes.c_int, [c_vp, ctypes.POINTER(KernelStat)]),
     "ss_bench_copy": (ctypes.c_int, [c_vp, ctypes.c_size_t, ctypes.c_int, c_f64p]),
+    # host layer (include/spec_spars_host.h)
+    "ssh_params_default": (None, [ctypes.POINTER(HostParams), ctypes.c_int]),
+    "ssh_sparsifier_create": (ctypes.c_int, [c_vp, ctypes.c_int32, ctypes.POINTER(HostParams),
+                                             ctypes.POINTER(c_vp)]),
+    "ssh_sparsifier_destroy": (ctypes.c_int, [c_vp]),
+    "ssh_sparsifier_insert_many": (ctypes.c_int, [c_vp, ctypes.c_int64, c_i32p, c_i32p, c_f64p]),
+    "ssh_sparsifier_form_laplacian": (ctypes.c_int, [c_vp, ctypes.c_int]),
+    "ssh_sparsifier_check_diagonal": (ctypes.c_int, [c_vp]),
+    "ssh_sparsifier_num_edges": (ctypes.c_int, [c_vp, c_i64p]),
+    "ssh_sparsifier_sparsify": (ctypes.c_int, [c_vp, ctypes.c_int, c_i64p, c_i64p]),
+    "ssh_sparsifier_sample_apply": (ctypes.c_int, [c_vp, c_f64p, ctypes.c_int64, c_i64p, c_i64p]),
+    "ssh_sparsifier_laplacian": (ctypes.c_int, [c_vp, c_i64p, c_i32p, c_i32p, c_f64p]),
+    "ssh_sparsifier_info": (ctypes.c_int, [c_vp, c_f64p, c_i32p, c_i64p]),
+    "ssh_sparsifier_sketch": (ctypes.c_int, [c_vp, c_f64p]),
 }
 
 _lib = None

@@ -13,7 +13,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libspecspars_hip.so")
 
 HIP_SOURCES = ["kernels.hip", "device.hip"]
-CPP_SOURCES = ["factor.cpp", "graph.cpp", "capi.cpp"]
+CPP_SOURCES = ["factor.cpp", "graph.cpp", "capi.cpp", "sparsifier.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

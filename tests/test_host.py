@@ -18,7 +18,7 @@ def test_library_exports_every_declared_symbol():
         if h.endswith(".h"):
             txt = open(os.path.join(ROOT, "include", h)).read()
             txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-            names |= set(re.findall(r"\b(ss_[a-z0-9_]+)\s*\(", txt))
+            names |= set(re.findall(r"\b(ssh?_[a-z0-9_]+)\s*\(", txt))
     assert len(names) >= 20
     for nm in sorted(names):
         assert hasattr(lib, nm), nm
