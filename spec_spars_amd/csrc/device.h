@@ -46,7 +46,7 @@ struct DevArena {
 // Work blocks of one PCG run in precision T (n x k each).
 template <typename T>
 struct PcgWork {
-    T *B = nullptr, *R = nullptr, *P = nullptr, *Q = nullptr, *Zp = nullptr, *W = nullptr;
+    T *B = nullptr, *R = nullptr, *P = nullptr, *P2 = nullptr, *Q = nullptr, *Zp = nullptr, *W = nullptr;
 };
 
 }  // namespace ss

@@ -209,6 +209,7 @@ int graph_ensure_work(ss_graph *g, int k, int prec) {
         SS_TRY(A.alloc(&w.B, nk));
         SS_TRY(A.alloc(&w.R, nk));
         SS_TRY(A.alloc(&w.P, nk));
+        SS_TRY(A.alloc(&w.P2, nk));
         SS_TRY(A.alloc(&w.Q, nk));
         SS_TRY(A.alloc(&w.Zp, nk));
         SS_TRY(A.alloc(&w.W, nk));
@@ -221,12 +222,13 @@ int graph_ensure_work(ss_graph *g, int k, int prec) {
         SS_TRY(A.alloc(&w.B, nk));
         SS_TRY(A.alloc(&w.R, nk));
         SS_TRY(A.alloc(&w.P, nk));
+        SS_TRY(A.alloc(&w.P2, nk));
         SS_TRY(A.alloc(&w.Q, nk));
         SS_TRY(A.alloc(&w.Zp, nk));
         SS_TRY(A.alloc(&w.W, nk));
     }
-    SS_TRY(A.alloc(&g->partial, (size_t)1024 * k));
-    SS_TRY(A.alloc(&g->partial2, (size_t)1024 * k));
+    SS_TRY(A.alloc(&g->partial, (size_t)kMaxPartialBlocks * k));
+    SS_TRY(A.alloc(&g->partial2, (size_t)kMaxPartialBlocks * k));
     SS_TRY(A.alloc(&g->colbuf, (size_t)4 * k));
     SS_TRY(A.alloc(&g->sc_buf, (size_t)5 * k));
     SS_TRY(A.alloc(&g->sc_int, (size_t)2 * k));
@@ -263,21 +265,24 @@ static uint32_t next_epoch(ss_graph *g) {
     return ++g->epoch;
 }
 
-// Zp = L^{-T} D^+ L^{-1} R   (W holds the forward result)
+// Zp = L^{-T} D^+ L^{-1} R   (W holds the forward result).  The forward
+// solve also leaves per-block partials of r . Zp = y^T D^+ y in rz_partial
+// (nb_rz blocks).
 template <typename T>
-static int precond(ss_graph *g, int k, const T *R, T *W, T *Zp) {
+static int precond(ss_graph *g, int k, const T *R, T *W, T *Zp, double *rz_partial, int *nb_rz) {
     hipStream_t s = g->ctx->stream;
     const int32_t n = (int32_t)g->n;
     uint32_t e1 = next_epoch(g);
     {
-    ProfScope ps(g, SS_KERNEL_TRI_FWD);
-    SS_HIP((launch_trisolve<T, false>(s, g->ctx->device, n, g->f_rows, g->f_ptr, g->f_col, fvals<T>(g), g->diag,
-                                      k, R, W, g->flags, e1, g->status + 1, kSpinLimit)));
+        ProfScope ps(g, SS_KERNEL_TRI_FWD);
+        SS_HIP((launch_trisolve<T, false>(s, g->ctx->device, n, g->f_rows, g->f_ptr, g->f_col, fvals<T>(g),
+                                          g->diag, k, R, W, g->flags, e1, g->status + 1, kSpinLimit, rz_partial,
+                                          nb_rz)));
     }
     uint32_t e2 = next_epoch(g);
     ProfScope ps(g, SS_KERNEL_TRI_BWD);
     SS_HIP((launch_trisolve<T, true>(s, g->ctx->device, n, g->b_rows, g->b_ptr, g->b_col, bvals<T>(g), g->diag, k,
-                                     W, Zp, g->flags, e2, g->status + 1, kSpinLimit)));
+                                     W, Zp, g->flags, e2, g->status + 1, kSpinLimit, nullptr, nullptr)));
     return SS_OK;
 }
 
@@ -307,28 +312,27 @@ static int pcg_run(ss_graph *g, int k, const T *rhs, T *X, const PcgWork<T> &w, 
     launch_col_partial<T, T>(s, n, k, rhs, rhs, 1, g->partial);
     launch_col_mean(s, k, g->partial, nb, n, g->colbuf);
     launch_project<T>(s, nk, k, rhs, g->colbuf, w.R, w.B, X);
-    SS_TRY(precond<T>(g, k, w.R, w.W, w.Zp));
+    int nb_rz = 0;
+    SS_TRY(precond<T>(g, k, w.R, w.W, w.Zp, g->partial2, &nb_rz));
     launch_col_partial<T, T>(s, n, k, w.B, w.B, 0, g->partial);
-    launch_col_partial<T, T>(s, n, k, w.R, w.Zp, 0, g->partial2);
-    launch_pcg_init(s, k, g->partial, g->partial2, nb, g->sc);
-    SS_HIP(hipMemcpyAsync(w.P, w.Zp, sizeof(T) * (size_t)nk, hipMemcpyDeviceToDevice, s));
+    launch_pcg_init(s, k, g->partial, nb, g->partial2, nb_rz, g->sc);     // beta = 0: P starts as Z
+    T *Pold = w.P, *Pnew = w.P2;
     for (int it = 0; it < max_iter; it++) {
         {
+            // P = Z + beta P, Q = A P, p.q partials
             ProfScope ps(g, SS_KERNEL_SPMM);
-            launch_spmm<T>(s, n, g->rowptr, g->col, avals<T>(g), k, w.P, w.Q);
+            launch_spmm_pq<T>(s, n, g->rowptr, g->col, avals<T>(g), k, g->sc.beta, w.Zp, Pold, Pnew, w.Q,
+                              g->partial);
         }
-        launch_col_partial<T, T>(s, n, k, w.P, w.Q, 0, g->partial);
         launch_pcg_alpha(s, k, g->partial, nb, g->sc);
-        launch_update_xr<T>(s, nk, k, g->sc.alpha, w.P, w.Q, X, w.R);
-        launch_col_partial<T, T>(s, n, k, w.R, w.R, 0, g->partial);
+        launch_update_xr_rr<T>(s, n, k, g->sc.alpha, Pnew, w.Q, X, w.R, g->partial);
         launch_pcg_check(s, k, g->partial, nb, rule, tol, g->sc, g->status);
         int nactive = 0;
         SS_TRY(read_status(g, &nactive));
         if (nactive == 0) break;
-        SS_TRY(precond<T>(g, k, w.R, w.W, w.Zp));
-        launch_col_partial<T, T>(s, n, k, w.R, w.Zp, 0, g->partial);
-        launch_pcg_beta(s, k, g->partial, nb, g->sc);
-        launch_update_p<T>(s, nk, k, g->sc.beta, w.Zp, w.P);
+        SS_TRY(precond<T>(g, k, w.R, w.W, w.Zp, g->partial2, &nb_rz));
+        launch_pcg_beta(s, k, g->partial2, nb_rz, g->sc);
+        std::swap(Pold, Pnew);
     }
     SS_TRY(read_status(g, nullptr));
     iters.resize(k);

@@ -55,7 +55,8 @@ void launch_residual64(hipStream_t s, int32_t n, const int32_t *rowptr, const in
 int partial_blocks(int64_t n);
 template <typename TA, typename TB>
 void launch_col_partial(hipStream_t s, int32_t n, int k, const TA *A, const TB *B, int op, double *partial);
-void launch_pcg_init(hipStream_t s, int k, const double *pbb, const double *prz, int nb, PcgScalars sc);
+void launch_pcg_init(hipStream_t s, int k, const double *pbb, int nb_bb, const double *prz, int nb_rz,
+                     PcgScalars sc);
 void launch_pcg_alpha(hipStream_t s, int k, const double *ppq, int nb, PcgScalars sc);
 void launch_pcg_check(hipStream_t s, int k, const double *prr, int nb, int rule, double tol, PcgScalars sc,
                       int *nactive);
@@ -73,10 +74,20 @@ template <typename T>
 void launch_accumulate(hipStream_t s, int64_t count, const T *d, double *x);
 void launch_fill_u32(hipStream_t s, int64_t count, uint32_t *p, uint32_t v);
 void launch_copy_f64(hipStream_t s, int64_t count, const double *a, double *b);
+// Upper bound of the block count of every kernel that writes per-block
+// partials (partial buffers hold kMaxPartialBlocks * k doubles).
+static const int kMaxPartialBlocks = 4096;
 template <typename T, bool BWD>
 hipError_t launch_trisolve(hipStream_t s, int device, int32_t n, const int32_t *rows, const int32_t *ptr,
                            const int32_t *col, const T *val, const double *diag, int k, const T *in, T *out,
-                           uint32_t *flags, uint32_t epoch, int *err, int spin_limit);
+                           uint32_t *flags, uint32_t epoch, int *err, int spin_limit, double *partial,
+                           int *grid_out);
+template <typename T>
+void launch_spmm_pq(hipStream_t s, int32_t n, const int32_t *rowptr, const int32_t *col, const T *val, int k,
+                    const double *beta, const T *Z, const T *Pold, T *Pnew, T *Q, double *partial);
+template <typename T>
+void launch_update_xr_rr(hipStream_t s, int32_t n, int k, const double *alpha, const T *P, const T *Q, T *X, T *R,
+                         double *partial);
 template <typename TZ>
 void launch_er_sample(hipStream_t s, const ErSampleArgs<TZ> &a);
 

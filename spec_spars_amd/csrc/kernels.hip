@@ -68,6 +68,82 @@ __global__ __launch_bounds__(256) void k_spmm(int32_t n, const int32_t *__restri
     }
 }
 
+// Fused PCG step 1: P_new = Z + beta P_old (beta == 0 takes Z alone),
+// Q = A P_new gathered as A Z + beta A P_old row by row, and per-block
+// partials of P_new . Q.  A fixed grid walks rows wave by wave so the
+// partials stay deterministic.
+template <typename T>
+__global__ __launch_bounds__(256) void k_spmm_pq(int32_t n, const int32_t *__restrict__ rowptr,
+                                                 const int32_t *__restrict__ col, const T *__restrict__ val,
+                                                 int k, const double *__restrict__ beta,
+                                                 const T *__restrict__ Z, const T *__restrict__ Pold,
+                                                 T *__restrict__ Pnew, T *__restrict__ Q,
+                                                 double *__restrict__ partial) {
+    __shared__ double sm[4][64];
+    const int w = SS_WAVE_IN_BLOCK, lane = SS_LANE;
+    for (int jb = 0; jb < k; jb += 64) {
+        const int j = jb + lane;
+        const bool on = j < k;
+        const T b = on ? (T)beta[j] : (T)0;
+        const bool useb = b != (T)0;
+        double acc = 0.0;
+        for (int row = blockIdx.x * 4 + w; row < n; row += gridDim.x * 4) {
+            const int r = wave_uniform(row);
+            const int q0 = rowptr[r], q1 = rowptr[r + 1];
+            if (on) {
+                T qv = 0;
+                for (int q = q0; q < q1; q++) {
+                    const size_t o = (size_t)col[q] * k + j;
+                    T pc = Z[o];
+                    if (useb) pc += b * Pold[o];
+                    qv += val[q] * pc;
+                }
+                const size_t o = (size_t)r * k + j;
+                T pn = Z[o];
+                if (useb) pn += b * Pold[o];
+                Pnew[o] = pn;
+                Q[o] = qv;
+                acc += (double)pn * (double)qv;
+            }
+        }
+        sm[w][lane] = acc;
+        __syncthreads();
+        if (w == 0 && on)
+            partial[(size_t)blockIdx.x * k + j] = ((sm[0][lane] + sm[1][lane]) + sm[2][lane]) + sm[3][lane];
+        __syncthreads();
+    }
+}
+
+// Fused PCG step 2: X += alpha P, R -= alpha Q, per-block partials of R . R.
+template <typename T>
+__global__ __launch_bounds__(256) void k_update_xr_rr(int32_t n, int k, const double *__restrict__ alpha,
+                                                      const T *__restrict__ P, const T *__restrict__ Q,
+                                                      T *__restrict__ X, T *__restrict__ R,
+                                                      double *__restrict__ partial) {
+    __shared__ double sm[4][64];
+    const int w = SS_WAVE_IN_BLOCK, lane = SS_LANE;
+    for (int jb = 0; jb < k; jb += 64) {
+        const int j = jb + lane;
+        const bool on = j < k;
+        const T a = on ? (T)alpha[j] : (T)0;
+        double acc = 0.0;
+        if (on) {
+            for (int row = blockIdx.x * 4 + w; row < n; row += gridDim.x * 4) {
+                const size_t o = (size_t)row * k + j;
+                X[o] += a * P[o];
+                const T r = R[o] - a * Q[o];
+                R[o] = r;
+                acc += (double)r * (double)r;
+            }
+        }
+        sm[w][lane] = acc;
+        __syncthreads();
+        if (w == 0 && on)
+            partial[(size_t)blockIdx.x * k + j] = ((sm[0][lane] + sm[1][lane]) + sm[2][lane]) + sm[3][lane];
+        __syncthreads();
+    }
+}
+
 // FP64 residual  out = B - A * X  (iterative refinement's true residual).
 template <typename T>
 __global__ __launch_bounds__(256) void k_residual64(int32_t n, const int32_t *__restrict__ rowptr,
@@ -125,11 +201,11 @@ __device__ __forceinline__ double sum_partials(const double *partial, int nb, in
 // ---------------------------------------------------------------------------
 // PCG scalar steps (one block of 256 threads, thread j = column j).
 // ---------------------------------------------------------------------------
-__global__ void k_pcg_init(int k, const double *partial_bb, const double *partial_rz, int nb,
+__global__ void k_pcg_init(int k, const double *partial_bb, int nb_bb, const double *partial_rz, int nb_rz,
                            PcgScalars s) {
     for (int j = threadIdx.x; j < k; j += blockDim.x) {
-        const double bb = sum_partials(partial_bb, nb, k, j);
-        const double rz = sum_partials(partial_rz, nb, k, j);
+        const double bb = sum_partials(partial_bb, nb_bb, k, j);
+        const double rz = sum_partials(partial_rz, nb_rz, k, j);
         s.bnorm[j] = sqrt(bb);
         s.rz[j] = rz;
         s.active[j] = bb > 0.0 ? 1 : 0;
@@ -281,14 +357,18 @@ __global__ __launch_bounds__(256) void k_trisolve(int32_t n, const int32_t *__re
                                                   const T *__restrict__ val,
                                                   const double *__restrict__ diag, int k,
                                                   const T *__restrict__ in, T *out, uint32_t *flags,
-                                                  uint32_t epoch, int *err, int spin_limit) {
-    const int lane = SS_LANE;
-    const int wave = blockIdx.x * 4 + SS_WAVE_IN_BLOCK;
+                                                  uint32_t epoch, int *err, int spin_limit,
+                                                  double *__restrict__ partial) {
+    __shared__ double sm[4][64];
+    const int lane = SS_LANE, wv = SS_WAVE_IN_BLOCK;
+    const int wave = blockIdx.x * 4 + wv;
     const int nwaves = gridDim.x * 4;
-    for (int t = wave; t < n; t += nwaves) {
+    double rz[4] = {0.0, 0.0, 0.0, 0.0};     // forward: y^T D^+ y per column chunk (k <= 256)
+    bool dead = false;
+    for (int t = wave; t < n && !dead; t += nwaves) {
         const int i = wave_uniform(rows[t]);
         const int q0 = wave_uniform(ptr[t]), q1 = wave_uniform(ptr[t + 1]);
-        for (int qb = q0; qb < q1; qb += 64) {
+        for (int qb = q0; qb < q1 && !dead; qb += 64) {
             const int q = qb + lane;
             const int c = q < q1 ? col[q] : -1;
             int spins = 0;
@@ -298,26 +378,47 @@ __global__ __launch_bounds__(256) void k_trisolve(int32_t n, const int32_t *__re
                 if (__all(ready)) break;
                 if (++spins > spin_limit) {
                     if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    return;
+                    dead = true;
+                    break;
                 }
-                if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+                if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                    dead = true;
+                    break;
+                }
                 __builtin_amdgcn_s_sleep(1);
             }
         }
+        if (dead) break;
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        double d = 1.0;
-        if (BWD) d = diag[i];
-        for (int jb = 0; jb < k; jb += 64) {
-            const int j = jb + lane;
-            if (j < k) {
+        const double d = diag[i];
+#pragma unroll
+        for (int ch = 0; ch < 4; ch++) {
+            const int j = ch * 64 + lane;
+            if (ch * 64 < k && j < k) {
                 T acc = in[(size_t)i * k + j];
                 if (BWD && d != 0.0) acc = (T)((double)acc / d);
                 for (int q = q0; q < q1; q++) acc -= val[q] * out[(size_t)col[q] * k + j];
                 out[(size_t)i * k + j] = acc;
+                if (!BWD) {
+                    const double y = (double)acc;
+                    rz[ch] += d != 0.0 ? y * (y / d) : y * y;
+                }
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         if (lane == 0) __hip_atomic_store(&flags[i], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (!BWD && partial) {
+#pragma unroll
+        for (int ch = 0; ch < 4; ch++) {
+            if (ch * 64 >= k) break;
+            sm[wv][lane] = rz[ch];
+            __syncthreads();
+            const int j = ch * 64 + lane;
+            if (wv == 0 && j < k)
+                partial[(size_t)blockIdx.x * k + j] = ((sm[0][lane] + sm[1][lane]) + sm[2][lane]) + sm[3][lane];
+            __syncthreads();
+        }
     }
 }
 
@@ -484,8 +585,9 @@ template void launch_col_partial<double, double>(hipStream_t, int32_t, int, cons
 template void launch_col_partial<float, float>(hipStream_t, int32_t, int, const float *, const float *, int,
                                                double *);
 
-void launch_pcg_init(hipStream_t s, int k, const double *pbb, const double *prz, int nb, PcgScalars sc) {
-    hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(256), 0, s, k, pbb, prz, nb, sc);
+void launch_pcg_init(hipStream_t s, int k, const double *pbb, int nb_bb, const double *prz, int nb_rz,
+                     PcgScalars sc) {
+    hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(256), 0, s, k, pbb, nb_bb, prz, nb_rz, sc);
 }
 void launch_pcg_alpha(hipStream_t s, int k, const double *ppq, int nb, PcgScalars sc) {
     hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(256), 0, s, k, ppq, nb, sc);
@@ -551,6 +653,28 @@ void launch_fill_u32(hipStream_t s, int64_t count, uint32_t *p, uint32_t v) {
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(count, 256, kStreamGrid)), dim3(256), 0, s, count, p, v);
 }
 
+template <typename T>
+void launch_spmm_pq(hipStream_t s, int32_t n, const int32_t *rowptr, const int32_t *col, const T *val, int k,
+                    const double *beta, const T *Z, const T *Pold, T *Pnew, T *Q, double *partial) {
+    hipLaunchKernelGGL(k_spmm_pq<T>, dim3(partial_blocks(n)), dim3(256), 0, s, n, rowptr, col, val, k, beta, Z,
+                       Pold, Pnew, Q, partial);
+}
+template void launch_spmm_pq<double>(hipStream_t, int32_t, const int32_t *, const int32_t *, const double *, int,
+                                     const double *, const double *, const double *, double *, double *, double *);
+template void launch_spmm_pq<float>(hipStream_t, int32_t, const int32_t *, const int32_t *, const float *, int,
+                                    const double *, const float *, const float *, float *, float *, double *);
+
+template <typename T>
+void launch_update_xr_rr(hipStream_t s, int32_t n, int k, const double *alpha, const T *P, const T *Q, T *X, T *R,
+                         double *partial) {
+    hipLaunchKernelGGL(k_update_xr_rr<T>, dim3(partial_blocks(n)), dim3(256), 0, s, n, k, alpha, P, Q, X, R,
+                       partial);
+}
+template void launch_update_xr_rr<double>(hipStream_t, int32_t, int, const double *, const double *, const double *,
+                                          double *, double *, double *);
+template void launch_update_xr_rr<float>(hipStream_t, int32_t, int, const double *, const float *, const float *,
+                                         float *, float *, double *);
+
 template <typename T, bool BWD>
 int trisolve_grid(int device) {
     static int cached[64] = {0};
@@ -567,26 +691,25 @@ int trisolve_grid(int device) {
 template <typename T, bool BWD>
 hipError_t launch_trisolve(hipStream_t s, int device, int32_t n, const int32_t *rows, const int32_t *ptr,
                            const int32_t *col, const T *val, const double *diag, int k, const T *in, T *out,
-                           uint32_t *flags, uint32_t epoch, int *err, int spin_limit) {
+                           uint32_t *flags, uint32_t epoch, int *err, int spin_limit, double *partial,
+                           int *grid_out) {
     int grid = trisolve_grid<T, BWD>(device);
     if (grid <= 0) return hipErrorInvalidConfiguration;
+    if (grid > kMaxPartialBlocks) grid = kMaxPartialBlocks;
     int need = (n + 3) / 4;
     if (need < grid) grid = need < 1 ? 1 : need;
-    void *args[] = {&n, &rows, &ptr, &col, &val, &diag, &k, &in, &out, &flags, &epoch, &err, &spin_limit};
+    if (grid_out) *grid_out = grid;
+    void *args[] = {&n, &rows, &ptr, &col, &val, &diag, &k, &in, &out, &flags, &epoch, &err, &spin_limit, &partial};
     return hipLaunchCooperativeKernel((const void *)k_trisolve<T, BWD>, dim3(grid), dim3(256), args, 0, s);
 }
-template hipError_t launch_trisolve<double, false>(hipStream_t, int, int32_t, const int32_t *, const int32_t *,
-                                                   const int32_t *, const double *, const double *, int,
-                                                   const double *, double *, uint32_t *, uint32_t, int *, int);
-template hipError_t launch_trisolve<double, true>(hipStream_t, int, int32_t, const int32_t *, const int32_t *,
-                                                  const int32_t *, const double *, const double *, int,
-                                                  const double *, double *, uint32_t *, uint32_t, int *, int);
-template hipError_t launch_trisolve<float, false>(hipStream_t, int, int32_t, const int32_t *, const int32_t *,
-                                                  const int32_t *, const float *, const double *, int,
-                                                  const float *, float *, uint32_t *, uint32_t, int *, int);
-template hipError_t launch_trisolve<float, true>(hipStream_t, int, int32_t, const int32_t *, const int32_t *,
-                                                 const int32_t *, const float *, const double *, int,
-                                                 const float *, float *, uint32_t *, uint32_t, int *, int);
+#define SS_INST_TRI(T, B)                                                                                      \
+    template hipError_t launch_trisolve<T, B>(hipStream_t, int, int32_t, const int32_t *, const int32_t *,       \
+                                              const int32_t *, const T *, const double *, int, const T *, T *,   \
+                                              uint32_t *, uint32_t, int *, int, double *, int *);
+SS_INST_TRI(double, false)
+SS_INST_TRI(double, true)
+SS_INST_TRI(float, false)
+SS_INST_TRI(float, true)
 
 template <typename TZ>
 void launch_er_sample(hipStream_t s, const ErSampleArgs<TZ> &a) {
