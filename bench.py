@@ -58,8 +58,8 @@ def kernel_bytes(W, fnnz_off, k, kept):
     return {
         # sketch: incidence lists (edge id + signed sqrt(w)) and Y
         "sketch": 4 * (n + 1) + 2 * m * (4 + 8) + n * k * s,
-        # SpMM: rowptr, (col, val) per nonzero, read P, write Q
-        "spmm": 4 * (n + 1) + nnzL * (4 + s) + 2 * n * k * s,
+        # fused SpMM: rowptr, (col, val) per nonzero, read Z and P_old, write P_new and Q
+        "spmm": 4 * (n + 1) + nnzL * (4 + s) + 4 * n * k * s,
         # triangular solves: schedule rows + rowptr, (col, val), read r, write y (+ D)
         "tri_fwd": 4 * n + 4 * (n + 1) + fnnz_off * (4 + s) + 2 * n * k * s,
         "tri_bwd": 4 * n + 4 * (n + 1) + fnnz_off * (4 + s) + 2 * n * k * s + n * 8,

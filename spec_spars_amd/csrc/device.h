@@ -74,6 +74,10 @@ struct ss_graph {
     double *f_val = nullptr, *b_val = nullptr, *diag = nullptr;
     float *f_valf = nullptr, *b_valf = nullptr;
     int64_t f_nnz = 0, b_nnz = 0;
+    std::vector<int32_t> f_level_ptr, b_level_ptr;   // host copies: launch bounds per level
+    std::vector<int32_t> f_level_long, b_level_long; // long rows leading each level
+    int32_t f_small_lo = 0, f_small_hi = 0, b_small_lo = 0, b_small_hi = 0;
+    int32_t *f_lp_dev = nullptr, *b_lp_dev = nullptr, *f_split = nullptr;
     // sync state
     uint32_t *flags = nullptr;
     uint32_t epoch = 0;

@@ -146,7 +146,7 @@ int graph_upload_factor(ss_graph *g, const Factor &F) {
     if (F.n != g->n) return fail(SS_ERR_INVALID, "factor has n=%lld, graph has n=%lld", (long long)F.n, (long long)g->n);
     SS_HIP(hipSetDevice(g->ctx->device));
     Schedule fw, bw;
-    SS_TRY(factor_schedules(F, fw, bw));
+    SS_TRY(factor_schedules(F, fw, bw, kLongRow));
     // D per vertex (the factor keeps it per elimination position)
     std::vector<double> vdiag((size_t)F.n);
     for (int64_t p = 0; p < F.n; p++) vdiag[F.order[p]] = F.diag[p];
@@ -167,6 +167,9 @@ int graph_upload_factor(ss_graph *g, const Factor &F) {
     SS_TRY(A.alloc(&g->b_val, g->b_nnz));
     SS_TRY(A.alloc(&g->b_valf, g->b_nnz));
     SS_TRY(A.alloc(&g->diag, n));
+    SS_TRY(A.alloc(&g->f_lp_dev, fw.level_ptr.size()));
+    SS_TRY(A.alloc(&g->b_lp_dev, bw.level_ptr.size()));
+    SS_TRY(A.alloc(&g->f_split, n));
     hipStream_t s = g->ctx->stream;
     std::vector<float> fvf(fw.val.begin(), fw.val.end()), bvf(bw.val.begin(), bw.val.end());
     SS_TRY(h2d(s, g->f_rows, fw.rows.data(), n));
@@ -180,7 +183,18 @@ int graph_upload_factor(ss_graph *g, const Factor &F) {
     SS_TRY(h2d(s, g->b_val, bw.val.data(), g->b_nnz));
     SS_TRY(h2d(s, g->b_valf, bvf.data(), g->b_nnz));
     SS_TRY(h2d(s, g->diag, vdiag.data(), n));
+    SS_TRY(h2d(s, g->f_lp_dev, fw.level_ptr.data(), fw.level_ptr.size()));
+    SS_TRY(h2d(s, g->b_lp_dev, bw.level_ptr.data(), bw.level_ptr.size()));
+    SS_TRY(h2d(s, g->f_split, fw.split.data(), n));
     SS_HIP(hipStreamSynchronize(s));
+    g->f_level_ptr = fw.level_ptr;
+    g->b_level_ptr = bw.level_ptr;
+    g->f_level_long = fw.level_long;
+    g->b_level_long = bw.level_long;
+    g->f_small_lo = fw.small_lo;
+    g->f_small_hi = fw.small_hi;
+    g->b_small_lo = bw.small_lo;
+    g->b_small_hi = bw.small_hi;
     g->factor_nnz = F.nnz;
     g->depth_fwd = F.depth_fwd;
     g->depth_bwd = F.depth_bwd;
@@ -257,32 +271,36 @@ template <typename T> static const T *avals(const ss_graph *g);
 template <> const double *avals<double>(const ss_graph *g) { return g->val; }
 template <> const float *avals<float>(const ss_graph *g) { return g->valf; }
 
-static uint32_t next_epoch(ss_graph *g) {
-    if (g->epoch >= 0xfffffff0u) {
-        launch_fill_u32(g->ctx->stream, g->n, g->flags, 0u);
-        g->epoch = 0;
-    }
-    return ++g->epoch;
-}
-
-// Zp = L^{-T} D^+ L^{-1} R   (W holds the forward result).  The forward
-// solve also leaves per-block partials of r . Zp = y^T D^+ y in rz_partial
-// (nb_rz blocks).
+// Zp = L^{-T} D^+ L^{-1} R   (W holds the forward result), one launch per
+// level.  Also leaves per-block partials of r . Zp = y^T D^+ y in
+// rz_partial (nb_rz blocks).
 template <typename T>
 static int precond(ss_graph *g, int k, const T *R, T *W, T *Zp, double *rz_partial, int *nb_rz) {
     hipStream_t s = g->ctx->stream;
     const int32_t n = (int32_t)g->n;
-    uint32_t e1 = next_epoch(g);
     {
         ProfScope ps(g, SS_KERNEL_TRI_FWD);
-        SS_HIP((launch_trisolve<T, false>(s, g->ctx->device, n, g->f_rows, g->f_ptr, g->f_col, fvals<T>(g),
-                                          g->diag, k, R, W, g->flags, e1, g->status + 1, kSpinLimit, rz_partial,
-                                          nb_rz)));
+        for (int32_t l = 0; l < g->f_small_lo; l++)
+            launch_trisolve_level<T, false>(s, g->f_level_ptr[l], g->f_level_long[l], g->f_level_ptr[l + 1],
+                                            g->f_rows, g->f_ptr, g->f_col, fvals<T>(g), g->diag, k, R, W);
+        if (g->f_small_lo < g->f_small_hi) {
+            launch_tri_tail_head<T>(s, g->f_level_ptr[g->f_small_lo], (int32_t)g->n, g->f_rows, g->f_ptr, g->f_split,
+                                    g->f_col, fvals<T>(g), k, R, W);
+            launch_tri_small<T, false>(s, g->f_small_lo, g->f_small_hi, g->f_lp_dev, g->f_rows, g->f_ptr, g->f_split,
+                                       g->f_col, fvals<T>(g), g->diag, k, R, W);
+        }
     }
-    uint32_t e2 = next_epoch(g);
-    ProfScope ps(g, SS_KERNEL_TRI_BWD);
-    SS_HIP((launch_trisolve<T, true>(s, g->ctx->device, n, g->b_rows, g->b_ptr, g->b_col, bvals<T>(g), g->diag, k,
-                                     W, Zp, g->flags, e2, g->status + 1, kSpinLimit, nullptr, nullptr)));
+    launch_ydy_partial<T>(s, n, k, W, g->diag, rz_partial);
+    *nb_rz = partial_blocks(n);
+    {
+        ProfScope ps(g, SS_KERNEL_TRI_BWD);
+        launch_tri_small<T, true>(s, g->b_small_lo, g->b_small_hi, g->b_lp_dev, g->b_rows, g->b_ptr, nullptr,
+                                  g->b_col, bvals<T>(g), g->diag, k, W, Zp);
+        for (int32_t l = g->b_small_hi; l + 1 < (int32_t)g->b_level_ptr.size(); l++)
+            launch_trisolve_level<T, true>(s, g->b_level_ptr[l], g->b_level_long[l], g->b_level_ptr[l + 1],
+                                           g->b_rows, g->b_ptr, g->b_col, bvals<T>(g), g->diag, k, W, Zp);
+    }
+    SS_HIP(hipGetLastError());
     return SS_OK;
 }
 
@@ -316,23 +334,20 @@ static int pcg_run(ss_graph *g, int k, const T *rhs, T *X, const PcgWork<T> &w, 
     SS_TRY(precond<T>(g, k, w.R, w.W, w.Zp, g->partial2, &nb_rz));
     launch_col_partial<T, T>(s, n, k, w.B, w.B, 0, g->partial);
     launch_pcg_init(s, k, g->partial, nb, g->partial2, nb_rz, g->sc);     // beta = 0: P starts as Z
-    T *Pold = w.P, *Pnew = w.P2;
     for (int it = 0; it < max_iter; it++) {
+        launch_update_p<T>(s, nk, k, g->sc.beta, w.Zp, w.P);          // P = Z + beta P (beta = 0: P = Z)
         {
-            // P = Z + beta P, Q = A P, p.q partials
             ProfScope ps(g, SS_KERNEL_SPMM);
-            launch_spmm_pq<T>(s, n, g->rowptr, g->col, avals<T>(g), k, g->sc.beta, w.Zp, Pold, Pnew, w.Q,
-                              g->partial);
+            launch_spmm_dot<T>(s, n, g->rowptr, g->col, avals<T>(g), k, w.P, w.Q, g->partial);   // Q = A P
         }
-        launch_pcg_alpha(s, k, g->partial, nb, g->sc);
-        launch_update_xr_rr<T>(s, n, k, g->sc.alpha, Pnew, w.Q, X, w.R, g->partial);
+        launch_pcg_alpha(s, k, g->partial, nb, g->sc, g->status);
+        launch_update_xr_rr<T>(s, n, k, g->sc.alpha, w.P, w.Q, X, w.R, g->partial);
         launch_pcg_check(s, k, g->partial, nb, rule, tol, g->sc, g->status);
         int nactive = 0;
         SS_TRY(read_status(g, &nactive));
         if (nactive == 0) break;
         SS_TRY(precond<T>(g, k, w.R, w.W, w.Zp, g->partial2, &nb_rz));
         launch_pcg_beta(s, k, g->partial2, nb_rz, g->sc);
-        std::swap(Pold, Pnew);
     }
     SS_TRY(read_status(g, nullptr));
     iters.resize(k);
@@ -386,11 +401,14 @@ int graph_solve(ss_graph *g, int k, const ss_solve_opts &o, SolveStats &st) {
         SS_TRY(col_norms2(g, k, g->B64, bn2));
         bnorm.resize(k);
         for (int j = 0; j < k; j++) bnorm[j] = std::sqrt(bn2[j]);
-        const double inner_tol = std::max(1e-5, o.stop_rule == SS_STOP_REL ? o.tol : 0.0);
+        // Each FP32 sweep reduces the FP64 residual by ~1e-3 (well above the
+        // FP32 stagnation level); sweeps repeat until the FP64 stop rule holds.
+        const double inner_tol = std::max(1e-3, o.stop_rule == SS_STOP_REL ? o.tol : 0.0);
+        const int inner_max = std::min(o.max_iter, 300);
         std::vector<int32_t> it_sweep;
         std::vector<double> bn_inner;
         rn2 = bn2;
-        for (int sweep = 0; sweep < 8; sweep++) {
+        for (int sweep = 0; sweep < 16; sweep++) {
             bool all_done = true;
             for (int j = 0; j < k; j++) {
                 double r = std::sqrt(rn2[j]);
@@ -399,7 +417,7 @@ int graph_solve(ss_graph *g, int k, const ss_solve_opts &o, SolveStats &st) {
             }
             if (all_done) break;
             launch_convert<double, float>(s, nk, g->R64, g->Y32);
-            SS_TRY(pcg_run<float>(g, k, g->Y32, g->X32, g->w32, SS_STOP_REL, inner_tol, o.max_iter, it_sweep,
+            SS_TRY(pcg_run<float>(g, k, g->Y32, g->X32, g->w32, SS_STOP_REL, inner_tol, inner_max, it_sweep,
                                   bn_inner));
             for (int j = 0; j < k; j++) st.iters[j] += it_sweep[j];
             launch_accumulate<float>(s, nk, g->X32, g->X64);

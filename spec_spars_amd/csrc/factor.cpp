@@ -347,9 +347,16 @@ void factor_levels(Factor &F) {
 // pull form (the vertices it waits for).  The backward schedule holds the
 // rows of L^T, i.e. the off-diagonal part of U's rows.  Every dependency
 // must have a strictly lower level -- the sync-free solve relies on it.
-int factor_schedules(const Factor &F, Schedule &fwd, Schedule &bwd) {
+int factor_schedules(const Factor &F, Schedule &fwd, Schedule &bwd, int32_t long_row, int32_t small_rows) {
     const int64_t n = F.n;
-    auto order = [&](const std::vector<int32_t> &lvl, Schedule &S, std::vector<int32_t> &pos_at) {
+    // entries per position: forward = row of L (count of U rows naming it),
+    // backward = U's off-diagonal row
+    std::vector<int32_t> flen((size_t)n, 0), blen((size_t)n, 0);
+    for (int64_t i = 0; i < n; i++)
+        for (int32_t t = F.urowptr[i]; t < F.urowptr[i + 1]; t++)
+            if (F.ucol[t] != i) { flen[F.ucol[t]]++; blen[i]++; }
+    auto order = [&](const std::vector<int32_t> &lvl, const std::vector<int32_t> &len, Schedule &S,
+                     std::vector<int32_t> &pos_at) {
         S.n = n;
         S.depth = 0;
         for (int64_t i = 0; i < n; i++) S.depth = std::max<int32_t>(S.depth, lvl[i]);
@@ -363,10 +370,17 @@ int factor_schedules(const Factor &F, Schedule &fwd, Schedule &bwd) {
         for (int32_t l = 1; l <= S.depth; l++) pos[l] = S.level_ptr[l - 1];
         pos_at.resize((size_t)n);
         for (int64_t i = 0; i < n; i++) pos_at[pos[lvl[i]]++] = (int32_t)i;   // schedule slot -> position
+        // within a level, long rows first (they get a block each)
+        S.level_long.assign((size_t)S.depth, 0);
+        for (int32_t l = 0; l < S.depth; l++) {
+            auto b = pos_at.begin() + S.level_ptr[l], e = pos_at.begin() + S.level_ptr[l + 1];
+            auto mid = std::stable_partition(b, e, [&](int32_t p) { return len[p] > long_row; });
+            S.level_long[l] = (int32_t)(mid - b);
+        }
     };
     std::vector<int32_t> fslot, bslot;
-    order(F.lvl_fwd, fwd, fslot);
-    order(F.lvl_bwd, bwd, bslot);
+    order(F.lvl_fwd, flen, fwd, fslot);
+    order(F.lvl_bwd, blen, bwd, bslot);
 
     // strictly-lower rows of L (position space): transpose of U's off-diagonal
     std::vector<int32_t> lcnt((size_t)n + 1, 0);
@@ -418,6 +432,30 @@ int factor_schedules(const Factor &F, Schedule &fwd, Schedule &bwd) {
     };
     SS_TRY(pack(fwd, fslot, F.lvl_fwd, lcnt.data(), lcol.data(), lval.data(), false));
     SS_TRY(pack(bwd, bslot, F.lvl_bwd, F.urowptr.data(), F.ucol.data(), F.uval.data(), true));
+
+    // runs of small levels (one workgroup walks them)
+    auto small = [&](const Schedule &S, int32_t l) { return S.level_ptr[l + 1] - S.level_ptr[l] <= small_rows; };
+    fwd.small_hi = fwd.depth;
+    fwd.small_lo = fwd.depth;
+    while (fwd.small_lo > 0 && small(fwd, fwd.small_lo - 1)) fwd.small_lo--;
+    bwd.small_lo = 0;
+    bwd.small_hi = 0;
+    while (bwd.small_hi < bwd.depth && small(bwd, bwd.small_hi)) bwd.small_hi++;
+    // forward tail rows: entries from head levels first (1-based level of a
+    // vertex < small_lo + 1 means it is done before the tail starts)
+    fwd.split.assign((size_t)n, 0);
+    std::vector<int32_t> vlvl((size_t)n);
+    for (int64_t p = 0; p < n; p++) vlvl[F.order[p]] = F.lvl_fwd[p];
+    for (int64_t t = 0; t < n; t++) {
+        const int32_t b = fwd.ptr[t], e = fwd.ptr[t + 1];
+        if (t < fwd.level_ptr[fwd.small_lo]) { fwd.split[t] = e - b; continue; }
+        std::vector<std::pair<int32_t, double>> ent;
+        for (int32_t q = b; q < e; q++) ent.push_back({fwd.col[q], fwd.val[q]});
+        auto mid = std::stable_partition(ent.begin(), ent.end(),
+                                         [&](const std::pair<int32_t, double> &x) { return vlvl[x.first] <= fwd.small_lo; });
+        fwd.split[t] = (int32_t)(mid - ent.begin());
+        for (int32_t q = b; q < e; q++) { fwd.col[q] = ent[q - b].first; fwd.val[q] = ent[q - b].second; }
+    }
     return SS_OK;
 }
 

@@ -57,7 +57,7 @@ template <typename TA, typename TB>
 void launch_col_partial(hipStream_t s, int32_t n, int k, const TA *A, const TB *B, int op, double *partial);
 void launch_pcg_init(hipStream_t s, int k, const double *pbb, int nb_bb, const double *prz, int nb_rz,
                      PcgScalars sc);
-void launch_pcg_alpha(hipStream_t s, int k, const double *ppq, int nb, PcgScalars sc);
+void launch_pcg_alpha(hipStream_t s, int k, const double *ppq, int nb, PcgScalars sc, int *nactive);
 void launch_pcg_check(hipStream_t s, int k, const double *prr, int nb, int rule, double tol, PcgScalars sc,
                       int *nactive);
 void launch_pcg_beta(hipStream_t s, int k, const double *prz, int nb, PcgScalars sc);
@@ -77,11 +77,25 @@ void launch_copy_f64(hipStream_t s, int64_t count, const double *a, double *b);
 // Upper bound of the block count of every kernel that writes per-block
 // partials (partial buffers hold kMaxPartialBlocks * k doubles).
 static const int kMaxPartialBlocks = 4096;
+// Rows of one level, positions [t0, t1); the first nlong of them are long
+// rows processed by a whole block each.
+static const int kLongRow = 64;
 template <typename T, bool BWD>
-hipError_t launch_trisolve(hipStream_t s, int device, int32_t n, const int32_t *rows, const int32_t *ptr,
-                           const int32_t *col, const T *val, const double *diag, int k, const T *in, T *out,
-                           uint32_t *flags, uint32_t epoch, int *err, int spin_limit, double *partial,
-                           int *grid_out);
+void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nlong, int32_t t1, const int32_t *rows,
+                           const int32_t *ptr, const int32_t *col, const T *val, const double *diag, int k,
+                           const T *in, T *out);
+template <typename T>
+void launch_spmm_dot(hipStream_t s, int32_t n, const int32_t *rowptr, const int32_t *col, const T *val, int k,
+                     const T *P, T *Q, double *partial);
+template <typename T>
+void launch_tri_tail_head(hipStream_t s, int32_t t0, int32_t t1, const int32_t *rows, const int32_t *ptr,
+                          const int32_t *split, const int32_t *col, const T *val, int k, const T *in, T *out);
+template <typename T, bool BWD>
+void launch_tri_small(hipStream_t s, int32_t l0, int32_t l1, const int32_t *level_ptr, const int32_t *rows,
+                      const int32_t *ptr, const int32_t *split, const int32_t *col, const T *val, const double *diag,
+                      int k, const T *in, T *out);
+template <typename T>
+void launch_ydy_partial(hipStream_t s, int32_t n, int k, const T *Yv, const double *diag, double *partial);
 template <typename T>
 void launch_spmm_pq(hipStream_t s, int32_t n, const int32_t *rowptr, const int32_t *col, const T *val, int k,
                     const double *beta, const T *Z, const T *Pold, T *Pnew, T *Q, double *partial);

@@ -17,6 +17,27 @@ namespace ss {
 
 __device__ __forceinline__ int wave_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
+// acc -= sum_q val[q] * X[col[q]*k + j] over q in [q0, q1) with stride
+// `step`, 8 gathers in flight, terms subtracted in q order.
+template <typename T>
+__device__ __forceinline__ T row_dot_sub(T acc, int q0, int q1, int step, const int32_t *__restrict__ col,
+                                         const T *__restrict__ val, const T *X, int k, int j) {
+    int q = q0;
+    for (; q + 7 * step < q1; q += 8 * step) {
+        T x[8], v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            x[u] = X[(size_t)col[q + u * step] * k + j];
+            v[u] = val[q + u * step];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc -= v[u] * x[u];
+    }
+    for (; q < q1; q += step) acc -= val[q] * X[(size_t)col[q] * k + j];
+    return acc;
+}
+
+
 // ---------------------------------------------------------------------------
 // (a) sketch, gather form: Y[x, j] = sum over incident edges e of x, in
 // increasing e, of  fl(scale * sign(e, j)) * b(x, e),  b = +-sqrt(w).
@@ -114,6 +135,37 @@ __global__ __launch_bounds__(256) void k_spmm_pq(int32_t n, const int32_t *__res
     }
 }
 
+// PCG: Q = A P and per-block partials of P . Q (P[i] is read with the row).
+template <typename T>
+__global__ __launch_bounds__(256) void k_spmm_dot(int32_t n, const int32_t *__restrict__ rowptr,
+                                                  const int32_t *__restrict__ col, const T *__restrict__ val, int k,
+                                                  const T *__restrict__ P, T *__restrict__ Q,
+                                                  double *__restrict__ partial) {
+    __shared__ double sm[4][64];
+    const int w = SS_WAVE_IN_BLOCK, lane = SS_LANE;
+    for (int jb = 0; jb < k; jb += 64) {
+        const int j = jb + lane;
+        const bool on = j < k;
+        double acc = 0.0;
+        for (int row = blockIdx.x * 4 + w; row < n; row += gridDim.x * 4) {
+            const int r = wave_uniform(row);
+            const int q0 = rowptr[r], q1 = rowptr[r + 1];
+            if (on) {
+                T qv = 0;
+                qv = -row_dot_sub<T>(qv, q0, q1, 1, col, val, P, k, j);
+                const size_t o = (size_t)r * k + j;
+                Q[o] = qv;
+                acc += (double)P[o] * (double)qv;
+            }
+        }
+        sm[w][lane] = acc;
+        __syncthreads();
+        if (w == 0 && on)
+            partial[(size_t)blockIdx.x * k + j] = ((sm[0][lane] + sm[1][lane]) + sm[2][lane]) + sm[3][lane];
+        __syncthreads();
+    }
+}
+
 // Fused PCG step 2: X += alpha P, R -= alpha Q, per-block partials of R . R.
 template <typename T>
 __global__ __launch_bounds__(256) void k_update_xr_rr(int32_t n, int k, const double *__restrict__ alpha,
@@ -191,21 +243,33 @@ __global__ __launch_bounds__(256) void k_col_partial(int32_t n, int k, const TA 
     }
 }
 
-// Sum nb partials per column in index order (one block, thread per column).
-__device__ __forceinline__ double sum_partials(const double *partial, int nb, int k, int j) {
+// Column j's sum of nb block partials, by block j of the launch: thread t
+// adds partials t, t+256, ... in order, then a fixed-shape tree over the
+// 256 threads.  Deterministic, and ~nb/256 dependent loads per thread.
+__device__ __forceinline__ double block_sum_partials(const double *partial, int nb, int k, int j, double *sm) {
     double s = 0.0;
-    for (int b = 0; b < nb; b++) s += partial[(size_t)b * k + j];
-    return s;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) s += partial[(size_t)b * k + j];
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+        __syncthreads();
+    }
+    const double r = sm[0];
+    __syncthreads();
+    return r;
 }
 
 // ---------------------------------------------------------------------------
-// PCG scalar steps (one block of 256 threads, thread j = column j).
+// PCG scalar steps: one block of 256 threads per column j (grid = k).
 // ---------------------------------------------------------------------------
-__global__ void k_pcg_init(int k, const double *partial_bb, int nb_bb, const double *partial_rz, int nb_rz,
-                           PcgScalars s) {
-    for (int j = threadIdx.x; j < k; j += blockDim.x) {
-        const double bb = sum_partials(partial_bb, nb_bb, k, j);
-        const double rz = sum_partials(partial_rz, nb_rz, k, j);
+__global__ __launch_bounds__(256) void k_pcg_init(int k, const double *partial_bb, int nb_bb,
+                                                  const double *partial_rz, int nb_rz, PcgScalars s) {
+    __shared__ double sm[256];
+    const int j = blockIdx.x;
+    const double bb = block_sum_partials(partial_bb, nb_bb, k, j, sm);
+    const double rz = block_sum_partials(partial_rz, nb_rz, k, j, sm);
+    if (threadIdx.x == 0) {
         s.bnorm[j] = sqrt(bb);
         s.rz[j] = rz;
         s.active[j] = bb > 0.0 ? 1 : 0;
@@ -215,25 +279,29 @@ __global__ void k_pcg_init(int k, const double *partial_bb, int nb_bb, const dou
     }
 }
 
-__global__ void k_pcg_alpha(int k, const double *partial_pq, int nb, PcgScalars s) {
-    for (int j = threadIdx.x; j < k; j += blockDim.x) {
-        const double pq = sum_partials(partial_pq, nb, k, j);
+__global__ __launch_bounds__(256) void k_pcg_alpha(int k, const double *partial_pq, int nb, PcgScalars s,
+                                                   int *nactive) {
+    __shared__ double sm[256];
+    const int j = blockIdx.x;
+    const double pq = block_sum_partials(partial_pq, nb, k, j, sm);
+    if (threadIdx.x == 0) {
         double a = 0.0;
         if (s.active[j]) {
             if (pq > 0.0) a = s.rz[j] / pq;
             else s.active[j] = 0;
         }
         s.alpha[j] = a;
+        if (j == 0) *nactive = 0;      // k_pcg_check counts afresh
     }
 }
 
 // After x += alpha p, r -= alpha q: count the update, apply the stop rule.
-__global__ void k_pcg_check(int k, const double *partial_rr, int nb, int rule, double tol,
-                            PcgScalars s, int *nactive) {
-    __shared__ int cnt[256];
-    int c = 0;
-    for (int j = threadIdx.x; j < k; j += blockDim.x) {
-        const double rr = sum_partials(partial_rr, nb, k, j);
+__global__ __launch_bounds__(256) void k_pcg_check(int k, const double *partial_rr, int nb, int rule, double tol,
+                                                   PcgScalars s, int *nactive) {
+    __shared__ double sm[256];
+    const int j = blockIdx.x;
+    const double rr = block_sum_partials(partial_rr, nb, k, j, sm);
+    if (threadIdx.x == 0) {
         s.rr[j] = rr;
         if (s.active[j]) {
             s.iters[j] += 1;
@@ -241,20 +309,15 @@ __global__ void k_pcg_check(int k, const double *partial_rr, int nb, int rule, d
             const bool stop = rule == 0 ? (rn <= sqrt(tol)) : (rn <= tol * s.bnorm[j]);
             if (stop) s.active[j] = 0;
         }
-        c += s.active[j];
-    }
-    cnt[threadIdx.x] = c;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int t = 0;
-        for (int i = 0; i < (int)blockDim.x; i++) t += cnt[i];
-        *nactive = t;
+        if (s.active[j]) atomicAdd(nactive, 1);
     }
 }
 
-__global__ void k_pcg_beta(int k, const double *partial_rz, int nb, PcgScalars s) {
-    for (int j = threadIdx.x; j < k; j += blockDim.x) {
-        const double rzn = sum_partials(partial_rz, nb, k, j);
+__global__ __launch_bounds__(256) void k_pcg_beta(int k, const double *partial_rz, int nb, PcgScalars s) {
+    __shared__ double sm[256];
+    const int j = blockIdx.x;
+    const double rzn = block_sum_partials(partial_rz, nb, k, j, sm);
+    if (threadIdx.x == 0) {
         double b = 0.0;
         if (s.active[j] && s.rz[j] != 0.0) b = rzn / s.rz[j];
         s.beta[j] = b;
@@ -262,9 +325,12 @@ __global__ void k_pcg_beta(int k, const double *partial_rz, int nb, PcgScalars s
     }
 }
 
-// mean[j] = colsum[j] / n
-__global__ void k_col_mean(int k, const double *partial, int nb, int64_t n, double *mean) {
-    for (int j = threadIdx.x; j < k; j += blockDim.x) mean[j] = sum_partials(partial, nb, k, j) / (double)n;
+// mean[j] = colsum[j] / n   (n = 1 gives plain sums)
+__global__ __launch_bounds__(256) void k_col_mean(int k, const double *partial, int nb, int64_t n, double *mean) {
+    __shared__ double sm[256];
+    const int j = blockIdx.x;
+    const double t = block_sum_partials(partial, nb, k, j, sm);
+    if (threadIdx.x == 0) mean[j] = t / (double)n;
 }
 
 // ---------------------------------------------------------------------------
@@ -302,7 +368,8 @@ __global__ __launch_bounds__(256) void k_update_p(int64_t nk, int k, const doubl
                                                   const T *__restrict__ Z, T *__restrict__ P) {
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nk;
          t += (int64_t)gridDim.x * blockDim.x) {
-        P[t] = Z[t] + (T)beta[t % k] * P[t];
+        const T b = (T)beta[t % k];
+        P[t] = b == (T)0 ? Z[t] : Z[t] + b * P[t];
     }
 }
 
@@ -336,89 +403,145 @@ __global__ void k_fill_u32(int64_t count, uint32_t *p, uint32_t v) {
 }
 
 // ---------------------------------------------------------------------------
-// (c) level-ordered triangular solve, sync-free.
-// Positions t = 0..n-1 are rows sorted by level; wave w of W resident waves
-// (cooperative launch) owns positions w, w+W, ...  A row waits only for rows
-// of lower level, i.e. lower positions, owned by resident waves that never
-// wait on it: the grid always drains.  flags[row] == epoch publishes a row;
-// the writer's agent-scope release fence orders its row stores before the
-// flag, the reader's acquire fence orders the flag before its loads.
+// (c) level-scheduled triangular solves: one launch per level.
+// Schedule positions [t0, t1) hold the rows of one level (sorted by level,
+// so every row they read was finished by an earlier launch on the stream);
+// one wavefront per row, lanes over the k columns.
 //   forward  (BWD=false): y_i = r_i - sum_{j<i} L[i,j] y_j
 //   backward (BWD=true):  z_i = y_i / D_i - sum_{j>i} L[j,i] z_j
 //                         (division skipped where D_i == 0, as
 //                          include/custom_cg.hpp:227-233 does)
-// Every wait is bounded by spin_limit polls; on expiry the wave raises
-// *err and every wave leaves at its next poll.
 // ---------------------------------------------------------------------------
 template <typename T, bool BWD>
-__global__ __launch_bounds__(256) void k_trisolve(int32_t n, const int32_t *__restrict__ rows,
-                                                  const int32_t *__restrict__ ptr,
-                                                  const int32_t *__restrict__ col,
-                                                  const T *__restrict__ val,
-                                                  const double *__restrict__ diag, int k,
-                                                  const T *__restrict__ in, T *out, uint32_t *flags,
-                                                  uint32_t epoch, int *err, int spin_limit,
-                                                  double *__restrict__ partial) {
-    __shared__ double sm[4][64];
+__global__ __launch_bounds__(256) void k_trisolve_level(int32_t t0, int32_t nlong, int32_t t1,
+                                                        const int32_t *__restrict__ rows,
+                                                        const int32_t *__restrict__ ptr,
+                                                        const int32_t *__restrict__ col,
+                                                        const T *__restrict__ val,
+                                                        const double *__restrict__ diag, int k,
+                                                        const T *__restrict__ in, T *__restrict__ out) {
     const int lane = SS_LANE, wv = SS_WAVE_IN_BLOCK;
-    const int wave = blockIdx.x * 4 + wv;
-    const int nwaves = gridDim.x * 4;
-    double rz[4] = {0.0, 0.0, 0.0, 0.0};     // forward: y^T D^+ y per column chunk (k <= 256)
-    bool dead = false;
-    for (int t = wave; t < n && !dead; t += nwaves) {
+    if ((int)blockIdx.x < nlong) {
+        // a long row: the block's 4 waves take every 4th entry, then combine
+        __shared__ T sm[4][64];
+        const int t = t0 + blockIdx.x;
         const int i = wave_uniform(rows[t]);
         const int q0 = wave_uniform(ptr[t]), q1 = wave_uniform(ptr[t + 1]);
-        for (int qb = q0; qb < q1 && !dead; qb += 64) {
-            const int q = qb + lane;
-            const int c = q < q1 ? col[q] : -1;
-            int spins = 0;
-            while (true) {
-                const bool ready =
-                    c < 0 || __hip_atomic_load(&flags[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch;
-                if (__all(ready)) break;
-                if (++spins > spin_limit) {
-                    if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    dead = true;
-                    break;
-                }
-                if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-                    dead = true;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
-        }
-        if (dead) break;
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        const double d = diag[i];
-#pragma unroll
-        for (int ch = 0; ch < 4; ch++) {
-            const int j = ch * 64 + lane;
-            if (ch * 64 < k && j < k) {
+        for (int jb = 0; jb < k; jb += 64) {
+            const int j = jb + lane;
+            T part = 0;
+            if (j < k) part = row_dot_sub<T>((T)0, q0 + wv, q1, 4, col, val, out, k, j);
+            sm[wv][lane] = part;
+            __syncthreads();
+            if (wv == 0 && j < k) {
                 T acc = in[(size_t)i * k + j];
-                if (BWD && d != 0.0) acc = (T)((double)acc / d);
-                for (int q = q0; q < q1; q++) acc -= val[q] * out[(size_t)col[q] * k + j];
-                out[(size_t)i * k + j] = acc;
-                if (!BWD) {
-                    const double y = (double)acc;
-                    rz[ch] += d != 0.0 ? y * (y / d) : y * y;
+                if (BWD) {
+                    const double d = diag[i];
+                    if (d != 0.0) acc = (T)((double)acc / d);
                 }
+                acc += ((sm[0][lane] + sm[1][lane]) + sm[2][lane]) + sm[3][lane];
+                out[(size_t)i * k + j] = acc;
+            }
+            __syncthreads();
+        }
+        return;
+    }
+    const int t = wave_uniform(t0 + nlong + ((int)blockIdx.x - nlong) * 4 + wv);
+    if (t >= t1) return;
+    const int i = wave_uniform(rows[t]);
+    const int q0 = wave_uniform(ptr[t]), q1 = wave_uniform(ptr[t + 1]);
+    const double d = BWD ? diag[i] : 1.0;
+    for (int jb = 0; jb < k; jb += 64) {
+        const int j = jb + lane;
+        if (j >= k) break;
+        T acc = in[(size_t)i * k + j];
+        if (BWD && d != 0.0) acc = (T)((double)acc / d);
+        acc = row_dot_sub<T>(acc, q0, q1, 1, col, val, out, k, j);
+        out[(size_t)i * k + j] = acc;
+    }
+}
+
+// Forward tail, step 1: for every row of the trailing run of small levels,
+// the contribution of the entries whose columns were finished by the head
+// levels (the first split[t] entries): out_i = in_i - sum.  Wave per row.
+template <typename T>
+__global__ __launch_bounds__(256) void k_tri_tail_head(int32_t t0, int32_t t1, const int32_t *__restrict__ rows,
+                                                       const int32_t *__restrict__ ptr,
+                                                       const int32_t *__restrict__ split,
+                                                       const int32_t *__restrict__ col, const T *__restrict__ val,
+                                                       int k, const T *__restrict__ in, T *__restrict__ out) {
+    const int t = wave_uniform(t0 + blockIdx.x * 4 + SS_WAVE_IN_BLOCK);
+    if (t >= t1) return;
+    const int i = wave_uniform(rows[t]);
+    const int q0 = wave_uniform(ptr[t]);
+    const int qs = q0 + wave_uniform(split[t]);
+    for (int jb = 0; jb < k; jb += 64) {
+        const int j = jb + SS_LANE;
+        if (j >= k) break;
+        out[(size_t)i * k + j] = row_dot_sub<T>(in[(size_t)i * k + j], q0, qs, 1, col, val, out, k, j);
+    }
+}
+
+// A run of small levels [l0, l1) walked by one workgroup of 16 waves, a
+// workgroup barrier between levels (all waves share one CU, so the barrier's
+// workgroup-scope fence orders the row stores before the next level's loads).
+//   forward tail: out_i (head part from k_tri_tail_head) -= entries split..end
+//   backward head: out_i = in_i / D_i - all entries
+template <typename T, bool BWD>
+__global__ __launch_bounds__(1024) void k_tri_small(int32_t l0, int32_t l1, const int32_t *__restrict__ level_ptr,
+                                                    const int32_t *__restrict__ rows,
+                                                    const int32_t *__restrict__ ptr,
+                                                    const int32_t *__restrict__ split,
+                                                    const int32_t *__restrict__ col, const T *__restrict__ val,
+                                                    const double *__restrict__ diag, int k,
+                                                    const T *__restrict__ in, T *out) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int l = l0; l < l1; l++) {
+        const int tb = level_ptr[l], te = level_ptr[l + 1];
+        for (int t = tb + wv; t < te; t += 16) {
+            const int i = wave_uniform(rows[t]);
+            const int q0 = wave_uniform(ptr[t]), q1 = wave_uniform(ptr[t + 1]);
+            const int qs = BWD ? q0 : q0 + wave_uniform(split[t]);
+            const double d = BWD ? diag[i] : 1.0;
+            for (int jb = 0; jb < k; jb += 64) {
+                const int j = jb + lane;
+                if (j >= k) break;
+                T acc;
+                if (BWD) {
+                    acc = in[(size_t)i * k + j];
+                    if (d != 0.0) acc = (T)((double)acc / d);
+                } else {
+                    acc = out[(size_t)i * k + j];
+                }
+                out[(size_t)i * k + j] = row_dot_sub<T>(acc, qs, q1, 1, col, val, out, k, j);
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        if (lane == 0) __hip_atomic_store(&flags[i], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
     }
-    if (!BWD && partial) {
-#pragma unroll
-        for (int ch = 0; ch < 4; ch++) {
-            if (ch * 64 >= k) break;
-            sm[wv][lane] = rz[ch];
-            __syncthreads();
-            const int j = ch * 64 + lane;
-            if (wv == 0 && j < k)
-                partial[(size_t)blockIdx.x * k + j] = ((sm[0][lane] + sm[1][lane]) + sm[2][lane]) + sm[3][lane];
-            __syncthreads();
+}
+
+// Per-block partials of y^T D^+ y (= r . M^{-1} r) from the forward result y.
+template <typename T>
+__global__ __launch_bounds__(256) void k_ydy_partial(int32_t n, int k, const T *__restrict__ Yv,
+                                                     const double *__restrict__ diag,
+                                                     double *__restrict__ partial) {
+    __shared__ double sm[4][64];
+    const int w = SS_WAVE_IN_BLOCK, lane = SS_LANE;
+    for (int jb = 0; jb < k; jb += 64) {
+        const int j = jb + lane;
+        double acc = 0.0;
+        if (j < k) {
+            for (int64_t r = (int64_t)blockIdx.x * 4 + w; r < n; r += (int64_t)gridDim.x * 4) {
+                const double y = (double)Yv[(size_t)r * k + j];
+                const double d = diag[r];
+                acc += d != 0.0 ? y * (y / d) : y * y;
+            }
         }
+        sm[w][lane] = acc;
+        __syncthreads();
+        if (w == 0 && j < k)
+            partial[(size_t)blockIdx.x * k + j] = ((sm[0][lane] + sm[1][lane]) + sm[2][lane]) + sm[3][lane];
+        __syncthreads();
     }
 }
 
@@ -587,20 +710,20 @@ template void launch_col_partial<float, float>(hipStream_t, int32_t, int, const 
 
 void launch_pcg_init(hipStream_t s, int k, const double *pbb, int nb_bb, const double *prz, int nb_rz,
                      PcgScalars sc) {
-    hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(256), 0, s, k, pbb, nb_bb, prz, nb_rz, sc);
+    hipLaunchKernelGGL(k_pcg_init, dim3(k), dim3(256), 0, s, k, pbb, nb_bb, prz, nb_rz, sc);
 }
-void launch_pcg_alpha(hipStream_t s, int k, const double *ppq, int nb, PcgScalars sc) {
-    hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(256), 0, s, k, ppq, nb, sc);
+void launch_pcg_alpha(hipStream_t s, int k, const double *ppq, int nb, PcgScalars sc, int *nactive) {
+    hipLaunchKernelGGL(k_pcg_alpha, dim3(k), dim3(256), 0, s, k, ppq, nb, sc, nactive);
 }
 void launch_pcg_check(hipStream_t s, int k, const double *prr, int nb, int rule, double tol, PcgScalars sc,
                       int *nactive) {
-    hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(256), 0, s, k, prr, nb, rule, tol, sc, nactive);
+    hipLaunchKernelGGL(k_pcg_check, dim3(k), dim3(256), 0, s, k, prr, nb, rule, tol, sc, nactive);
 }
 void launch_pcg_beta(hipStream_t s, int k, const double *prz, int nb, PcgScalars sc) {
-    hipLaunchKernelGGL(k_pcg_beta, dim3(1), dim3(256), 0, s, k, prz, nb, sc);
+    hipLaunchKernelGGL(k_pcg_beta, dim3(k), dim3(256), 0, s, k, prz, nb, sc);
 }
 void launch_col_mean(hipStream_t s, int k, const double *partial, int nb, int64_t n, double *mean) {
-    hipLaunchKernelGGL(k_col_mean, dim3(1), dim3(256), 0, s, k, partial, nb, n, mean);
+    hipLaunchKernelGGL(k_col_mean, dim3(k), dim3(256), 0, s, k, partial, nb, n, mean);
 }
 
 template <typename T>
@@ -665,6 +788,46 @@ template void launch_spmm_pq<float>(hipStream_t, int32_t, const int32_t *, const
                                     const double *, const float *, const float *, float *, float *, double *);
 
 template <typename T>
+void launch_spmm_dot(hipStream_t s, int32_t n, const int32_t *rowptr, const int32_t *col, const T *val, int k,
+                     const T *P, T *Q, double *partial) {
+    hipLaunchKernelGGL(k_spmm_dot<T>, dim3(partial_blocks(n)), dim3(256), 0, s, n, rowptr, col, val, k, P, Q, partial);
+}
+template void launch_spmm_dot<double>(hipStream_t, int32_t, const int32_t *, const int32_t *, const double *, int,
+                                      const double *, double *, double *);
+template void launch_spmm_dot<float>(hipStream_t, int32_t, const int32_t *, const int32_t *, const float *, int,
+                                     const float *, float *, double *);
+
+template <typename T>
+void launch_tri_tail_head(hipStream_t s, int32_t t0, int32_t t1, const int32_t *rows, const int32_t *ptr,
+                          const int32_t *split, const int32_t *col, const T *val, int k, const T *in, T *out) {
+    if (t1 <= t0) return;
+    hipLaunchKernelGGL(k_tri_tail_head<T>, dim3(grid_for(t1 - t0, 4)), dim3(256), 0, s, t0, t1, rows, ptr, split, col,
+                       val, k, in, out);
+}
+template void launch_tri_tail_head<double>(hipStream_t, int32_t, int32_t, const int32_t *, const int32_t *,
+                                           const int32_t *, const int32_t *, const double *, int, const double *,
+                                           double *);
+template void launch_tri_tail_head<float>(hipStream_t, int32_t, int32_t, const int32_t *, const int32_t *,
+                                          const int32_t *, const int32_t *, const float *, int, const float *, float *);
+
+template <typename T, bool BWD>
+void launch_tri_small(hipStream_t s, int32_t l0, int32_t l1, const int32_t *level_ptr, const int32_t *rows,
+                      const int32_t *ptr, const int32_t *split, const int32_t *col, const T *val, const double *diag,
+                      int k, const T *in, T *out) {
+    if (l1 <= l0) return;
+    hipLaunchKernelGGL((k_tri_small<T, BWD>), dim3(1), dim3(1024), 0, s, l0, l1, level_ptr, rows, ptr, split, col, val,
+                       diag, k, in, out);
+}
+#define SS_INST_SMALL(T, B)                                                                                       \
+    template void launch_tri_small<T, B>(hipStream_t, int32_t, int32_t, const int32_t *, const int32_t *,          \
+                                         const int32_t *, const int32_t *, const int32_t *, const T *,              \
+                                         const double *, int, const T *, T *);
+SS_INST_SMALL(double, false)
+SS_INST_SMALL(double, true)
+SS_INST_SMALL(float, false)
+SS_INST_SMALL(float, true)
+
+template <typename T>
 void launch_update_xr_rr(hipStream_t s, int32_t n, int k, const double *alpha, const T *P, const T *Q, T *X, T *R,
                          double *partial) {
     hipLaunchKernelGGL(k_update_xr_rr<T>, dim3(partial_blocks(n)), dim3(256), 0, s, n, k, alpha, P, Q, X, R,
@@ -676,40 +839,29 @@ template void launch_update_xr_rr<float>(hipStream_t, int32_t, int, const double
                                          float *, float *, double *);
 
 template <typename T, bool BWD>
-int trisolve_grid(int device) {
-    static int cached[64] = {0};
-    if (device < 0 || device >= 64) return 0;
-    if (cached[device]) return cached[device];
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trisolve<T, BWD>, 256, 0) != hipSuccess) return 0;
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 0;
-    cached[device] = per_cu * prop.multiProcessorCount;
-    return cached[device];
-}
-
-template <typename T, bool BWD>
-hipError_t launch_trisolve(hipStream_t s, int device, int32_t n, const int32_t *rows, const int32_t *ptr,
-                           const int32_t *col, const T *val, const double *diag, int k, const T *in, T *out,
-                           uint32_t *flags, uint32_t epoch, int *err, int spin_limit, double *partial,
-                           int *grid_out) {
-    int grid = trisolve_grid<T, BWD>(device);
-    if (grid <= 0) return hipErrorInvalidConfiguration;
-    if (grid > kMaxPartialBlocks) grid = kMaxPartialBlocks;
-    int need = (n + 3) / 4;
-    if (need < grid) grid = need < 1 ? 1 : need;
-    if (grid_out) *grid_out = grid;
-    void *args[] = {&n, &rows, &ptr, &col, &val, &diag, &k, &in, &out, &flags, &epoch, &err, &spin_limit, &partial};
-    return hipLaunchCooperativeKernel((const void *)k_trisolve<T, BWD>, dim3(grid), dim3(256), args, 0, s);
+void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nlong, int32_t t1, const int32_t *rows,
+                           const int32_t *ptr, const int32_t *col, const T *val, const double *diag, int k,
+                           const T *in, T *out) {
+    if (t1 <= t0) return;
+    const unsigned grid = (unsigned)nlong + grid_for(t1 - t0 - nlong, 4);
+    hipLaunchKernelGGL((k_trisolve_level<T, BWD>), dim3(grid), dim3(256), 0, s, t0, nlong, t1, rows, ptr, col, val,
+                       diag, k, in, out);
 }
 #define SS_INST_TRI(T, B)                                                                                      \
-    template hipError_t launch_trisolve<T, B>(hipStream_t, int, int32_t, const int32_t *, const int32_t *,       \
-                                              const int32_t *, const T *, const double *, int, const T *, T *,   \
-                                              uint32_t *, uint32_t, int *, int, double *, int *);
+    template void launch_trisolve_level<T, B>(hipStream_t, int32_t, int32_t, int32_t, const int32_t *,          \
+                                              const int32_t *, const int32_t *, const T *, const double *, int,  \
+                                              const T *, T *);
 SS_INST_TRI(double, false)
 SS_INST_TRI(double, true)
 SS_INST_TRI(float, false)
 SS_INST_TRI(float, true)
+
+template <typename T>
+void launch_ydy_partial(hipStream_t s, int32_t n, int k, const T *Yv, const double *diag, double *partial) {
+    hipLaunchKernelGGL(k_ydy_partial<T>, dim3(partial_blocks(n)), dim3(256), 0, s, n, k, Yv, diag, partial);
+}
+template void launch_ydy_partial<double>(hipStream_t, int32_t, int, const double *, const double *, double *);
+template void launch_ydy_partial<float>(hipStream_t, int32_t, int, const float *, const double *, double *);
 
 template <typename TZ>
 void launch_er_sample(hipStream_t s, const ErSampleArgs<TZ> &a) {

@@ -66,6 +66,13 @@ struct Schedule {
     int32_t depth = 0;
     std::vector<int32_t> rows;        // row processed at position t
     std::vector<int32_t> level_ptr;   // positions of level l: [ptr[l-1], ptr[l])
+    std::vector<int32_t> level_long;  // rows of level l with more than long_row entries (placed first)
+    // Small-level run handled by one workgroup: levels [small_lo, small_hi)
+    // (0-based).  Forward: the trailing run; each of its rows lists the
+    // entries of earlier (head) levels first, split[t] of them.  Backward:
+    // the leading run.
+    int32_t small_lo = 0, small_hi = 0;
+    std::vector<int32_t> split;       // per position (forward only)
     std::vector<int32_t> ptr;         // entries of position t: [ptr[t], ptr[t+1])
     std::vector<int32_t> col;
     std::vector<double> val;
@@ -76,7 +83,8 @@ int factor_build(int64_t n, const int32_t *colptr, const int32_t *rowidx, const 
 void factor_levels(Factor &F);
 // Level-ordered schedules in vertex ids; checks that every dependency sits
 // at a strictly lower level.
-int factor_schedules(const Factor &F, Schedule &fwd, Schedule &bwd);
+int factor_schedules(const Factor &F, Schedule &fwd, Schedule &bwd, int32_t long_row = 64,
+                     int32_t small_rows = 256);
 
 // ---- host graph structure -----------------------------------------------
 // Canonical edges (u < v, sorted by (v, u)) -> full Laplacian CSR with the
