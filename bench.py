@@ -50,21 +50,38 @@ def workload(name):
     raise SystemExit("unknown workload " + name)
 
 
-def kernel_bytes(W, fnnz_off, k, kept):
-    """Algorithmic (compulsory) HBM bytes per launch of each kernel, FP64."""
+def kernel_bytes(W, G, k, kept, s=8):
+    """Algorithmic (compulsory) HBM bytes per launch of each kernel kind:
+    every array counted once per pass, gathers beyond first touch not
+    counted."""
     n, m = W["n"], len(W["u"])
     nnzL = n + 2 * m
-    s = 8
+    fi, bi = G.schedule_info(0), G.schedule_info(1)
+
+    def per_level(info, extra_per_row):
+        # schedule row id + row pointer, (col, val) per entry, read in-row, write out-row
+        if not info["level_launches"]:
+            return None
+        b = info["level_rows"] * (4 + 4 + 2 * k * s + extra_per_row) + info["level_nnz"] * (4 + s)
+        return b / info["level_launches"]
+
+    def small(info, extra_per_row, passes):
+        if not info["small_levels"]:
+            return None
+        return info["small_rows"] * (4 + 4 + passes * 2 * k * s + extra_per_row) + info["small_nnz"] * (4 + s)
+
     return {
         # sketch: incidence lists (edge id + signed sqrt(w)) and Y
         "sketch": 4 * (n + 1) + 2 * m * (4 + 8) + n * k * s,
-        # fused SpMM: rowptr, (col, val) per nonzero, read Z and P_old, write P_new and Q
-        "spmm": 4 * (n + 1) + nnzL * (4 + s) + 4 * n * k * s,
-        # triangular solves: schedule rows + rowptr, (col, val), read r, write y (+ D)
-        "tri_fwd": 4 * n + 4 * (n + 1) + fnnz_off * (4 + s) + 2 * n * k * s,
-        "tri_bwd": 4 * n + 4 * (n + 1) + fnnz_off * (4 + s) + 2 * n * k * s + n * 8,
+        # SpMM: rowptr, (col, val) per nonzero, read P, write Q
+        "spmm": 4 * (n + 1) + nnzL * (4 + s) + 2 * n * k * s,
+        "tri_fwd": per_level(fi, 0),
+        "tri_bwd": per_level(bi, 8),
+        "tri_fwd_small": small(fi, 4, 2),
+        "tri_bwd_small": small(bi, 8, 1),
         # ER/sample: edges (u, v, w), Z once, p out, mask out, kept (u, v, w) out
         "er_sample": m * (4 + 4 + 8) + n * k * s + 8 * m + m + kept * 16,
+        "vector": None,
     }
 
 
@@ -176,16 +193,18 @@ def main():
     value = m / (ms_per_step / 1000.0)
     last = reps[-1]
 
-    kb = kernel_bytes(W, finfo["nnz"] - n, k, last["kept"])
+    kb = kernel_bytes(W, G, k, last["kept"], s=8 if prec == ss.PREC_FP64 else 4)
     kernels = {}
     for name, st in prof.items():
         if st["launches"]:
             avg = st["ms_total"] / st["launches"]
             kernels[name] = {"launches": st["launches"], "avg_ms": avg, "bytes_per_launch": kb[name],
-                             "GBps": kb[name] / (avg * 1e-3) / 1e9, "share": st["ms_total"] / (t_timed * 1e3)}
+                             "GBps": (kb[name] / (avg * 1e-3) / 1e9) if kb[name] else None,
+                             "share": st["ms_total"] / (t_timed * 1e3)}
     roofline = None
-    if kernels:
-        dom = max(kernels, key=lambda x: kernels[x]["avg_ms"] * kernels[x]["launches"])
+    ranked = [x for x in kernels if kernels[x]["bytes_per_launch"]]
+    if ranked:
+        dom = max(ranked, key=lambda x: kernels[x]["avg_ms"] * kernels[x]["launches"])
         kd = kernels[dom]
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(kd["GBps"], 1), "peak": PEAK_HBM_GBS,
                     "unit": "GB/s", "frac": round(kd["GBps"] / PEAK_HBM_GBS, 4), "traffic": None,
@@ -212,7 +231,8 @@ def main():
                    "ms_sketch": last["ms_sketch"], "ms_solve": last["ms_solve"], "ms_sample": last["ms_sample"],
                    "rhs_solves_per_s": k / (last["ms_solve"] / 1000.0),
                    "host_factor_s": t_factor, "host_graph_s": t_graph, "upload_s": t_upload,
-                   "copy_kernel_GBps": copy_gbs, "parallelism": "1 GPU"},
+                   "copy_kernel_GBps": copy_gbs, "parallelism": "1 GPU",
+                   "schedule_fwd": G.schedule_info(0), "schedule_bwd": G.schedule_info(1)},
         "roofline": roofline,
         "kernels": kernels,
         "cpu_baseline": cpu,

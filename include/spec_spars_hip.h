@@ -209,16 +209,27 @@ int ss_graph_read_probs(const ss_graph *g, double *p);
 /* Per-kernel device time, measured with HIP events on the graph's stream
  * around every launch while enabled (enabling clears the counters). */
 #define SS_KERNEL_SKETCH 0      /* (a) sketch                                */
-#define SS_KERNEL_SPMM 1        /* (b) Laplacian SpMM and FP64 residuals     */
-#define SS_KERNEL_TRI_FWD 2     /* (c) forward solve  L y = r                */
-#define SS_KERNEL_TRI_BWD 3     /* (c) backward solve L^T z = D^+ y          */
+#define SS_KERNEL_SPMM 1        /* (b) Laplacian SpMM (+ p.q) and residuals  */
+#define SS_KERNEL_TRI_FWD 2     /* (c) one level of the forward solve        */
+#define SS_KERNEL_TRI_BWD 3     /* (c) one level of the backward solve       */
 #define SS_KERNEL_ER_SAMPLE 4   /* (d) ER + coin + sample + compaction       */
-#define SS_KERNEL_COUNT 5
+#define SS_KERNEL_TRI_FWD_SMALL 5 /* forward small-level run (2 kernels)     */
+#define SS_KERNEL_TRI_BWD_SMALL 6 /* backward small-level run                */
+#define SS_KERNEL_VECTOR 7      /* PCG vector updates and reductions         */
+#define SS_KERNEL_COUNT 8
 typedef struct ss_kernel_stat {
     double ms_total;
     int64_t launches;
 } ss_kernel_stat;
 int ss_graph_profile(ss_graph *g, int enable);
+/* Shape of the uploaded solve schedules: per direction (0 = forward,
+ * 1 = backward) the level count, the rows and entries handled by per-level
+ * launches and by the single-workgroup small-level run. */
+typedef struct ss_schedule_info {
+    int32_t levels, level_launches, small_levels, reserved;
+    int64_t level_rows, level_nnz, small_rows, small_nnz;
+} ss_schedule_info;
+int ss_graph_schedule_info(const ss_graph *g, int direction, ss_schedule_info *out);
 int ss_graph_profile_read(const ss_graph *g, ss_kernel_stat *out /* [SS_KERNEL_COUNT] */);
 
 /* Achievable-bandwidth probe: a plain FP64 copy kernel over `bytes` bytes

@@ -70,7 +70,14 @@ class HostParams(ctypes.Structure):
                 ("max_iter", ctypes.c_int32), ("reserved", ctypes.c_int32), ("tol", ctypes.c_double)]
 
 
-KERNELS = ["sketch", "spmm", "tri_fwd", "tri_bwd", "er_sample"]
+KERNELS = ["sketch", "spmm", "tri_fwd", "tri_bwd", "er_sample", "tri_fwd_small", "tri_bwd_small", "vector"]
+
+
+class ScheduleInfo(ctypes.Structure):
+    _fields_ = [("levels", ctypes.c_int32), ("level_launches", ctypes.c_int32),
+                ("small_levels", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("level_rows", ctypes.c_int64), ("level_nnz", ctypes.c_int64),
+                ("small_rows", ctypes.c_int64), ("small_nnz", ctypes.c_int64)]
 
 
 class KernelStat(ctypes.Structure):
@@ -120,6 +127,7 @@ SIGNATURES = {
     "ss_graph_read_probs": (ctypes.c_int, [c_vp, c_f64p]),
     "ss_graph_profile": (ctypes.c_int, [c_vp, ctypes.c_int]),
     "ss_graph_profile_read": (ctypes.c_int, [c_vp, ctypes.POINTER(KernelStat)]),
+    "ss_graph_schedule_info": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.POINTER(ScheduleInfo)]),
     "ss_bench_copy": (ctypes.c_int, [c_vp, ctypes.c_size_t, ctypes.c_int, c_f64p]),
     # host layer (include/spec_spars_host.h)
     "ssh_params_default": (None, [ctypes.POINTER(HostParams), ctypes.c_int]),

@@ -238,6 +238,11 @@ class Graph:
         return {name: {"ms_total": arr[i].ms_total, "launches": arr[i].launches}
                 for i, name in enumerate(_lib.KERNELS)}
 
+    def schedule_info(self, direction):
+        si = _lib.ScheduleInfo()
+        check(load().ss_graph_schedule_info(self._h, direction, ctypes.byref(si)))
+        return {f: getattr(si, f) for f, _ in _lib.ScheduleInfo._fields_ if f != "reserved"}
+
     def read_probs(self):
         p = np.zeros(self.m, np.float64)
         check(load().ss_graph_read_probs(self._h, _p(p, ctypes.c_double)))

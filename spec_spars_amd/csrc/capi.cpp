@@ -313,6 +313,13 @@ int ss_graph_profile(ss_graph *g, int enable) {
     return SS_OK;
 }
 
+int ss_graph_schedule_info(const ss_graph *g, int direction, ss_schedule_info *out) {
+    if (!g || !out || direction < 0 || direction > 1) return fail(SS_ERR_INVALID, "bad arguments");
+    if (!g->has_factor) return fail(SS_ERR_INVALID, "graph has no factor");
+    *out = g->sched_info[direction];
+    return SS_OK;
+}
+
 int ss_graph_profile_read(const ss_graph *g, ss_kernel_stat *out) {
     if (!g || !out) return fail(SS_ERR_INVALID, "null argument");
     for (int i = 0; i < SS_KERNEL_COUNT; i++) {

@@ -78,6 +78,7 @@ struct ss_graph {
     std::vector<int32_t> f_level_long, b_level_long; // long rows leading each level
     int32_t f_small_lo = 0, f_small_hi = 0, b_small_lo = 0, b_small_hi = 0;
     int32_t *f_lp_dev = nullptr, *b_lp_dev = nullptr, *f_split = nullptr;
+    ss_schedule_info sched_info[2] = {};
     // sync state
     uint32_t *flags = nullptr;
     uint32_t epoch = 0;

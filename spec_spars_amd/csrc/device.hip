@@ -191,6 +191,21 @@ int graph_upload_factor(ss_graph *g, const Factor &F) {
     g->b_level_ptr = bw.level_ptr;
     g->f_level_long = fw.level_long;
     g->b_level_long = bw.level_long;
+    for (int dir = 0; dir < 2; dir++) {
+        const Schedule &S = dir == 0 ? fw : bw;
+        ss_schedule_info &I = g->sched_info[dir];
+        I = ss_schedule_info{};
+        I.levels = S.depth;
+        I.small_levels = S.small_hi - S.small_lo;
+        I.level_launches = S.depth - I.small_levels;
+        for (int32_t l = 0; l < S.depth; l++) {
+            const bool sm = l >= S.small_lo && l < S.small_hi;
+            const int64_t rows = S.level_ptr[l + 1] - S.level_ptr[l];
+            const int64_t nz = S.ptr[S.level_ptr[l + 1]] - S.ptr[S.level_ptr[l]];
+            (sm ? I.small_rows : I.level_rows) += rows;
+            (sm ? I.small_nnz : I.level_nnz) += nz;
+        }
+    }
     g->f_small_lo = fw.small_lo;
     g->f_small_hi = fw.small_hi;
     g->b_small_lo = bw.small_lo;
@@ -278,11 +293,13 @@ template <typename T>
 static int precond(ss_graph *g, int k, const T *R, T *W, T *Zp, double *rz_partial, int *nb_rz) {
     hipStream_t s = g->ctx->stream;
     const int32_t n = (int32_t)g->n;
-    {
+    for (int32_t l = 0; l < g->f_small_lo; l++) {
         ProfScope ps(g, SS_KERNEL_TRI_FWD);
-        for (int32_t l = 0; l < g->f_small_lo; l++)
-            launch_trisolve_level<T, false>(s, g->f_level_ptr[l], g->f_level_long[l], g->f_level_ptr[l + 1],
-                                            g->f_rows, g->f_ptr, g->f_col, fvals<T>(g), g->diag, k, R, W);
+        launch_trisolve_level<T, false>(s, g->f_level_ptr[l], g->f_level_long[l], g->f_level_ptr[l + 1], g->f_rows,
+                                        g->f_ptr, g->f_col, fvals<T>(g), g->diag, k, R, W);
+    }
+    {
+        ProfScope ps(g, SS_KERNEL_TRI_FWD_SMALL);
         if (g->f_small_lo < g->f_small_hi) {
             launch_tri_tail_head<T>(s, g->f_level_ptr[g->f_small_lo], (int32_t)g->n, g->f_rows, g->f_ptr, g->f_split,
                                     g->f_col, fvals<T>(g), k, R, W);
@@ -290,15 +307,20 @@ static int precond(ss_graph *g, int k, const T *R, T *W, T *Zp, double *rz_parti
                                        g->f_col, fvals<T>(g), g->diag, k, R, W);
         }
     }
-    launch_ydy_partial<T>(s, n, k, W, g->diag, rz_partial);
-    *nb_rz = partial_blocks(n);
     {
-        ProfScope ps(g, SS_KERNEL_TRI_BWD);
+        ProfScope ps(g, SS_KERNEL_VECTOR);
+        launch_ydy_partial<T>(s, n, k, W, g->diag, rz_partial);
+    }
+    *nb_rz = partial_blocks(n);
+    if (g->b_small_lo < g->b_small_hi) {
+        ProfScope ps(g, SS_KERNEL_TRI_BWD_SMALL);
         launch_tri_small<T, true>(s, g->b_small_lo, g->b_small_hi, g->b_lp_dev, g->b_rows, g->b_ptr, nullptr,
                                   g->b_col, bvals<T>(g), g->diag, k, W, Zp);
-        for (int32_t l = g->b_small_hi; l + 1 < (int32_t)g->b_level_ptr.size(); l++)
-            launch_trisolve_level<T, true>(s, g->b_level_ptr[l], g->b_level_long[l], g->b_level_ptr[l + 1],
-                                           g->b_rows, g->b_ptr, g->b_col, bvals<T>(g), g->diag, k, W, Zp);
+    }
+    for (int32_t l = g->b_small_hi; l + 1 < (int32_t)g->b_level_ptr.size(); l++) {
+        ProfScope ps(g, SS_KERNEL_TRI_BWD);
+        launch_trisolve_level<T, true>(s, g->b_level_ptr[l], g->b_level_long[l], g->b_level_ptr[l + 1], g->b_rows,
+                                       g->b_ptr, g->b_col, bvals<T>(g), g->diag, k, W, Zp);
     }
     SS_HIP(hipGetLastError());
     return SS_OK;
@@ -335,13 +357,19 @@ static int pcg_run(ss_graph *g, int k, const T *rhs, T *X, const PcgWork<T> &w, 
     launch_col_partial<T, T>(s, n, k, w.B, w.B, 0, g->partial);
     launch_pcg_init(s, k, g->partial, nb, g->partial2, nb_rz, g->sc);     // beta = 0: P starts as Z
     for (int it = 0; it < max_iter; it++) {
-        launch_update_p<T>(s, nk, k, g->sc.beta, w.Zp, w.P);          // P = Z + beta P (beta = 0: P = Z)
+        {
+            ProfScope ps(g, SS_KERNEL_VECTOR);
+            launch_update_p<T>(s, nk, k, g->sc.beta, w.Zp, w.P);      // P = Z + beta P (beta = 0: P = Z)
+        }
         {
             ProfScope ps(g, SS_KERNEL_SPMM);
             launch_spmm_dot<T>(s, n, g->rowptr, g->col, avals<T>(g), k, w.P, w.Q, g->partial);   // Q = A P
         }
         launch_pcg_alpha(s, k, g->partial, nb, g->sc, g->status);
-        launch_update_xr_rr<T>(s, n, k, g->sc.alpha, w.P, w.Q, X, w.R, g->partial);
+        {
+            ProfScope ps(g, SS_KERNEL_VECTOR);
+            launch_update_xr_rr<T>(s, n, k, g->sc.alpha, w.P, w.Q, X, w.R, g->partial);
+        }
         launch_pcg_check(s, k, g->partial, nb, rule, tol, g->sc, g->status);
         int nactive = 0;
         SS_TRY(read_status(g, &nactive));

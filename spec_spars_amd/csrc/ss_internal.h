@@ -84,7 +84,7 @@ void factor_levels(Factor &F);
 // Level-ordered schedules in vertex ids; checks that every dependency sits
 // at a strictly lower level.
 int factor_schedules(const Factor &F, Schedule &fwd, Schedule &bwd, int32_t long_row = 64,
-                     int32_t small_rows = 256);
+                     int32_t small_rows = 16);
 
 // ---- host graph structure -----------------------------------------------
 // Canonical edges (u < v, sorted by (v, u)) -> full Laplacian CSR with the
