@@ -488,6 +488,16 @@ __global__ __launch_bounds__(256) void k_tri_tail_head(int32_t t0, int32_t t1, c
 //   forward tail: out_i (head part from k_tri_tail_head) -= entries split..end
 //   backward head: out_i = in_i / D_i - all entries
 template <typename T, bool BWD>
+__device__ __forceinline__ T small_start(int i, int j, int k, double d, const T *in, const T *out) {
+    if (BWD) {
+        T acc = in[(size_t)i * k + j];
+        if (d != 0.0) acc = (T)((double)acc / d);
+        return acc;
+    }
+    return out[(size_t)i * k + j];
+}
+
+template <typename T, bool BWD>
 __global__ __launch_bounds__(1024) void k_tri_small(int32_t l0, int32_t l1, const int32_t *__restrict__ level_ptr,
                                                     const int32_t *__restrict__ rows,
                                                     const int32_t *__restrict__ ptr,
@@ -495,25 +505,48 @@ __global__ __launch_bounds__(1024) void k_tri_small(int32_t l0, int32_t l1, cons
                                                     const int32_t *__restrict__ col, const T *__restrict__ val,
                                                     const double *__restrict__ diag, int k,
                                                     const T *__restrict__ in, T *out) {
+    __shared__ T sm[16][64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int l = l0; l < l1; l++) {
         const int tb = level_ptr[l], te = level_ptr[l + 1];
-        for (int t = tb + wv; t < te; t += 16) {
-            const int i = wave_uniform(rows[t]);
-            const int q0 = wave_uniform(ptr[t]), q1 = wave_uniform(ptr[t + 1]);
-            const int qs = BWD ? q0 : q0 + wave_uniform(split[t]);
-            const double d = BWD ? diag[i] : 1.0;
+        const int r = te - tb;
+        if (r >= 16) {
+            // a wave per row
+            for (int t = tb + wv; t < te; t += 16) {
+                const int i = wave_uniform(rows[t]);
+                const int q0 = wave_uniform(ptr[t]), q1 = wave_uniform(ptr[t + 1]);
+                const int qs = BWD ? q0 : q0 + wave_uniform(split[t]);
+                const double d = BWD ? diag[i] : 1.0;
+                for (int jb = 0; jb < k; jb += 64) {
+                    const int j = jb + lane;
+                    if (j >= k) break;
+                    const T acc = small_start<T, BWD>(i, j, k, d, in, out);
+                    out[(size_t)i * k + j] = row_dot_sub<T>(acc, qs, q1, 1, col, val, out, k, j);
+                }
+            }
+        } else {
+            // g = 16 / r (rounded down to a power of two) waves per row, entries strided by g
+            int g = 16 / r;
+            g = 1 << (31 - __builtin_clz(g));
+            const int slot = wv / g, sub = wv - slot * g;
+            const int t = tb + slot;
+            const bool act = t < te;
+            const int i = act ? wave_uniform(rows[t]) : 0;
+            const int q0 = act ? wave_uniform(ptr[t]) : 0, q1 = act ? wave_uniform(ptr[t + 1]) : 0;
+            const int qs = BWD ? q0 : (act ? q0 + wave_uniform(split[t]) : 0);
+            const double d = (BWD && act) ? diag[i] : 1.0;
             for (int jb = 0; jb < k; jb += 64) {
                 const int j = jb + lane;
-                if (j >= k) break;
-                T acc;
-                if (BWD) {
-                    acc = in[(size_t)i * k + j];
-                    if (d != 0.0) acc = (T)((double)acc / d);
-                } else {
-                    acc = out[(size_t)i * k + j];
+                T part = 0;
+                if (act && j < k) part = row_dot_sub<T>((T)0, qs + sub, q1, g, col, val, out, k, j);
+                sm[wv][lane] = part;
+                __syncthreads();
+                if (act && sub == 0 && j < k) {
+                    T acc = small_start<T, BWD>(i, j, k, d, in, out);
+                    for (int u = 0; u < g; u++) acc += sm[wv + u][lane];
+                    out[(size_t)i * k + j] = acc;
                 }
-                out[(size_t)i * k + j] = row_dot_sub<T>(acc, qs, q1, 1, col, val, out, k, j);
+                __syncthreads();
             }
         }
         __syncthreads();
