@@ -106,7 +106,11 @@ struct ss_graph {
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
-    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_recs;
+    struct ProfRec {
+        int kind, launches;
+        hipEvent_t a, b;
+    };
+    std::vector<ProfRec> prof_recs;
     double prof_ms[SS_KERNEL_COUNT] = {};
     int64_t prof_cnt[SS_KERNEL_COUNT] = {};
     int64_t last_kept = 0;
@@ -116,11 +120,13 @@ struct ss_graph {
 
 namespace ss {
 // Records HIP events around one launch on the graph's stream when profiling.
+// A scope covering `launches` kernel launches of one kind.
 struct ProfScope {
     ss_graph *g;
     int kind;
+    int launches;
     hipEvent_t a = nullptr;
-    ProfScope(ss_graph *g_, int kind_);
+    ProfScope(ss_graph *g_, int kind_, int launches_ = 1);
     ~ProfScope();
 };
 int prof_resolve(ss_graph *g);

@@ -29,7 +29,7 @@ static hipEvent_t prof_event(ss_graph *g) {
     return g->ev_pool[g->ev_used++];
 }
 
-ProfScope::ProfScope(ss_graph *g_, int kind_) : g(g_), kind(kind_) {
+ProfScope::ProfScope(ss_graph *g_, int kind_, int launches_) : g(g_), kind(kind_), launches(launches_) {
     if (g->prof && (a = prof_event(g))) (void)hipEventRecord(a, g->ctx->stream);
 }
 
@@ -38,17 +38,17 @@ ProfScope::~ProfScope() {
     hipEvent_t b = prof_event(g);
     if (!b) return;
     (void)hipEventRecord(b, g->ctx->stream);
-    g->prof_recs.push_back({kind, {a, b}});
+    if (launches > 0) g->prof_recs.push_back({kind, launches, a, b});
 }
 
 int prof_resolve(ss_graph *g) {
     if (!g->prof) return SS_OK;
-    if (!g->prof_recs.empty()) SS_HIP(hipEventSynchronize(g->prof_recs.back().second.second));
+    if (!g->prof_recs.empty()) SS_HIP(hipEventSynchronize(g->prof_recs.back().b));
     for (auto &r : g->prof_recs) {
         float ms = 0.f;
-        SS_HIP(hipEventElapsedTime(&ms, r.second.first, r.second.second));
-        g->prof_ms[r.first] += ms;
-        g->prof_cnt[r.first] += 1;
+        SS_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+        g->prof_ms[r.kind] += ms;
+        g->prof_cnt[r.kind] += r.launches;
     }
     g->prof_recs.clear();
     g->ev_used = 0;
@@ -293,13 +293,16 @@ template <typename T>
 static int precond(ss_graph *g, int k, const T *R, T *W, T *Zp, double *rz_partial, int *nb_rz) {
     hipStream_t s = g->ctx->stream;
     const int32_t n = (int32_t)g->n;
-    for (int32_t l = 0; l < g->f_small_lo; l++) {
-        ProfScope ps(g, SS_KERNEL_TRI_FWD);
-        launch_trisolve_level<T, false>(s, g->f_level_ptr[l], g->f_level_long[l], g->f_level_ptr[l + 1], g->f_rows,
-                                        g->f_ptr, g->f_col, fvals<T>(g), g->diag, k, R, W);
+    {
+        // the level launches as one timed group; the kernel's average is
+        // the group time over its launch count
+        ProfScope ps(g, SS_KERNEL_TRI_FWD, g->f_small_lo);
+        for (int32_t l = 0; l < g->f_small_lo; l++)
+            launch_trisolve_level<T, false>(s, g->f_level_ptr[l], g->f_level_long[l], g->f_level_ptr[l + 1],
+                                            g->f_rows, g->f_ptr, g->f_col, fvals<T>(g), g->diag, k, R, W);
     }
     {
-        ProfScope ps(g, SS_KERNEL_TRI_FWD_SMALL);
+        ProfScope ps(g, SS_KERNEL_TRI_FWD_SMALL, g->f_small_lo < g->f_small_hi ? 2 : 0);
         if (g->f_small_lo < g->f_small_hi) {
             launch_tri_tail_head<T>(s, g->f_level_ptr[g->f_small_lo], (int32_t)g->n, g->f_rows, g->f_ptr, g->f_split,
                                     g->f_col, fvals<T>(g), k, R, W);
@@ -317,10 +320,11 @@ static int precond(ss_graph *g, int k, const T *R, T *W, T *Zp, double *rz_parti
         launch_tri_small<T, true>(s, g->b_small_lo, g->b_small_hi, g->b_lp_dev, g->b_rows, g->b_ptr, nullptr,
                                   g->b_col, bvals<T>(g), g->diag, k, W, Zp);
     }
-    for (int32_t l = g->b_small_hi; l + 1 < (int32_t)g->b_level_ptr.size(); l++) {
-        ProfScope ps(g, SS_KERNEL_TRI_BWD);
-        launch_trisolve_level<T, true>(s, g->b_level_ptr[l], g->b_level_long[l], g->b_level_ptr[l + 1], g->b_rows,
-                                       g->b_ptr, g->b_col, bvals<T>(g), g->diag, k, W, Zp);
+    {
+        ProfScope ps(g, SS_KERNEL_TRI_BWD, (int)g->b_level_ptr.size() - 1 - g->b_small_hi);
+        for (int32_t l = g->b_small_hi; l + 1 < (int32_t)g->b_level_ptr.size(); l++)
+            launch_trisolve_level<T, true>(s, g->b_level_ptr[l], g->b_level_long[l], g->b_level_ptr[l + 1],
+                                           g->b_rows, g->b_ptr, g->b_col, bvals<T>(g), g->diag, k, W, Zp);
     }
     SS_HIP(hipGetLastError());
     return SS_OK;
