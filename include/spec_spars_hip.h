@@ -206,6 +206,44 @@ int ss_graph_read_kept(const ss_graph *g, int64_t cap, int32_t *ku,
 int ss_graph_read_block(const ss_graph *g, int which /*0=Y,1=Z*/, double *out);
 int ss_graph_read_probs(const ss_graph *g, double *p);
 
+/* ---------------------------------------------------------------------- */
+/* Batches.  No reference counterpart: the reference sparsifies one graph per
+ * Sparsifier (src/sparsifier.rs:722-770).  G independent graphs are held as
+ * one block-diagonal system and every stage runs once over all of them;
+ * all reductions are segmented per (graph, column) in a fixed order, so each
+ * graph's results are bit-identical to running it alone.  An ss_graph from
+ * ss_graph_create is a batch of one; the ss_graph_* calls above operate on
+ * batches of one only.
+ *
+ * Graph i has n[i] vertices and the canonical edges (u < v, sorted by
+ * (v, u)) at positions [edge_ptr[i], edge_ptr[i+1]) of u/v/w, with vertex
+ * ids local to the graph. */
+int ss_batch_create(ss_context *ctx, int32_t count, const int64_t *n,
+                    const int64_t *edge_ptr, const int32_t *u, const int32_t *v,
+                    const double *w, ss_graph **out);
+int ss_batch_count(const ss_graph *g, int32_t *count);
+/* One factor per graph (factors[i] built from graph i's Laplacian). */
+int ss_batch_set_factors(ss_graph *g, int32_t count,
+                         const ss_factor *const *factors);
+/* Sparsify every graph of the batch: opts[i] for graph i.  k, uniform and
+ * the solve options must agree across the batch; seeds, scale and beta are
+ * per graph.  rep[i] (may be NULL) gets graph i's kept count, iterations
+ * and residual; the phase times are the batch's. */
+int ss_batch_sparsify(ss_graph *g, const ss_sparsify_opts *opts,
+                      ss_sparsify_report *rep);
+/* Solve L_i Z_i = Y_i for every graph with the right-hand sides uploaded by
+ * ss_batch_write_rhs; iters/relres (may be NULL) have count*k entries. */
+int ss_batch_write_rhs(ss_graph *g, int32_t i, int32_t k, int32_t precision,
+                       const double *Y /* n_i x k */);
+int ss_batch_solve(ss_graph *g, int32_t k, const ss_solve_opts *opts,
+                   int32_t *iters, double *relres, int32_t *refinements);
+/* Graph i's part of the last run's results (local vertex ids). */
+int ss_batch_read_kept(const ss_graph *g, int32_t i, int64_t cap, int32_t *ku,
+                       int32_t *kv, double *kw, int64_t *kept);
+int ss_batch_read_block(const ss_graph *g, int32_t i, int which /*0=Y,1=Z*/,
+                        double *out);
+int ss_batch_read_probs(const ss_graph *g, int32_t i, double *p);
+
 /* Per-kernel device time, measured with HIP events on the graph's stream
  * around every launch while enabled (enabling clears the counters). */
 #define SS_KERNEL_SKETCH 0      /* (a) sketch                                */

@@ -128,6 +128,20 @@ SIGNATURES = {
     "ss_graph_profile": (ctypes.c_int, [c_vp, ctypes.c_int]),
     "ss_graph_profile_read": (ctypes.c_int, [c_vp, ctypes.POINTER(KernelStat)]),
     "ss_graph_schedule_info": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.POINTER(ScheduleInfo)]),
+    "ss_batch_create": (ctypes.c_int, [c_vp, ctypes.c_int32, c_i64p, c_i64p, c_i32p, c_i32p,
+                                       c_f64p, ctypes.POINTER(c_vp)]),
+    "ss_batch_count": (ctypes.c_int, [c_vp, c_i32p]),
+    "ss_batch_set_factors": (ctypes.c_int, [c_vp, ctypes.c_int32, ctypes.POINTER(c_vp)]),
+    "ss_batch_sparsify": (ctypes.c_int, [c_vp, ctypes.POINTER(SparsifyOpts),
+                                         ctypes.POINTER(SparsifyReport)]),
+    "ss_batch_write_rhs": (ctypes.c_int, [c_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                          c_f64p]),
+    "ss_batch_solve": (ctypes.c_int, [c_vp, ctypes.c_int32, ctypes.POINTER(SolveOpts), c_i32p,
+                                      c_f64p, c_i32p]),
+    "ss_batch_read_kept": (ctypes.c_int, [c_vp, ctypes.c_int32, ctypes.c_int64, c_i32p, c_i32p,
+                                          c_f64p, c_i64p]),
+    "ss_batch_read_block": (ctypes.c_int, [c_vp, ctypes.c_int32, ctypes.c_int, c_f64p]),
+    "ss_batch_read_probs": (ctypes.c_int, [c_vp, ctypes.c_int32, c_f64p]),
     "ss_bench_copy": (ctypes.c_int, [c_vp, ctypes.c_size_t, ctypes.c_int, c_f64p]),
     # host layer (include/spec_spars_host.h)
     "ssh_params_default": (None, [ctypes.POINTER(HostParams), ctypes.c_int]),

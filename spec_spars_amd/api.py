@@ -247,3 +247,117 @@ class Graph:
         p = np.zeros(self.m, np.float64)
         check(load().ss_graph_read_probs(self._h, _p(p, ctypes.c_double)))
         return p
+
+
+class Batch:
+    """Many graphs resident as one block-diagonal system.
+
+    ``graphs`` is a list of (n, u, v, w) with canonical edges (u < v sorted by
+    (v, u)) and vertex ids local to each graph.  Every stage runs once over
+    the whole batch; each graph's results equal those of running it alone.
+    """
+
+    def __init__(self, ctx, graphs):
+        self.ctx = ctx
+        self.count = len(graphs)
+        self.n = np.array([int(g[0]) for g in graphs], np.int64)
+        self.m = np.array([len(g[1]) for g in graphs], np.int64)
+        self.edge_ptr = np.zeros(self.count + 1, np.int64)
+        self.edge_ptr[1:] = np.cumsum(self.m)
+        self._u = _i32(np.concatenate([np.asarray(g[1]) for g in graphs]) if self.count else [])
+        self._v = _i32(np.concatenate([np.asarray(g[2]) for g in graphs]) if self.count else [])
+        self._w = _f64(np.concatenate([np.asarray(g[3]) for g in graphs]) if self.count else [])
+        h = ctypes.c_void_p()
+        check(load().ss_batch_create(ctx._h, self.count, _p(self.n, ctypes.c_int64),
+                                     _p(self.edge_ptr, ctypes.c_int64), _p(self._u, ctypes.c_int32),
+                                     _p(self._v, ctypes.c_int32), _p(self._w, ctypes.c_double),
+                                     ctypes.byref(h)))
+        self._h = h
+        self.k = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().ss_graph_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def laplacian(self, i):
+        """Host CSC Laplacian of graph i (built on the host from its edges)."""
+        from .graphs import laplacian_csc
+        e0, e1 = self.edge_ptr[i], self.edge_ptr[i + 1]
+        return laplacian_csc(int(self.n[i]), self._u[e0:e1], self._v[e0:e1], self._w[e0:e1])
+
+    def set_factors(self, factors):
+        arr = (ctypes.c_void_p * self.count)(*[f._h for f in factors])
+        check(load().ss_batch_set_factors(self._h, self.count, arr))
+
+    def sparsify(self, k, betas, scales=None, sketch_seeds=None, sampling_seeds=None, uniform=False,
+                 solve=None):
+        """Sparsify every graph; per-graph lists of beta, scale and seeds."""
+        o = (_lib.SparsifyOpts * self.count)()
+        sv = solve if solve is not None else solve_opts(stop_rule=_lib.STOP_REL, tol=1e-8)
+        for i in range(self.count):
+            o[i].k = k
+            o[i].uniform = int(bool(uniform))
+            o[i].scale = 1.0 / np.sqrt(k) if scales is None else float(scales[i])
+            o[i].sketch_seed = 0 if sketch_seeds is None else int(sketch_seeds[i])
+            o[i].sampling_seed = 0 if sampling_seeds is None else int(sampling_seeds[i])
+            o[i].beta = float(betas[i])
+            o[i].solve = sv
+        rep = (_lib.SparsifyReport * self.count)()
+        check(load().ss_batch_sparsify(self._h, o, rep))
+        self.k = k
+        return [{f: getattr(rep[i], f) for f, _ in _lib.SparsifyReport._fields_}
+                for i in range(self.count)]
+
+    def solve(self, rhs, solve=None):
+        """Solve L_i Z_i = rhs[i] for every graph (rhs[i] is n_i x k)."""
+        sv = solve if solve is not None else solve_opts(stop_rule=_lib.STOP_REL, tol=1e-8)
+        k = int(np.asarray(rhs[0]).shape[1])
+        for i in range(self.count):
+            y = _f64(np.asarray(rhs[i]).reshape(int(self.n[i]), k))
+            check(load().ss_batch_write_rhs(self._h, i, k, sv.precision, _p(y, ctypes.c_double)))
+        iters = np.zeros(self.count * k, np.int32)
+        relres = np.zeros(self.count * k, np.float64)
+        refin = ctypes.c_int32()
+        check(load().ss_batch_solve(self._h, k, ctypes.byref(sv), _p(iters, ctypes.c_int32),
+                                    _p(relres, ctypes.c_double), ctypes.byref(refin)))
+        self.k = k
+        return {"iters": iters.reshape(self.count, k), "relres": relres.reshape(self.count, k),
+                "refinements": refin.value}
+
+    def read_kept(self, i):
+        kept = ctypes.c_int64()
+        cap = int(self.m[i])
+        ku = np.zeros(max(cap, 1), np.int32)
+        kv = np.zeros(max(cap, 1), np.int32)
+        kw = np.zeros(max(cap, 1), np.float64)
+        check(load().ss_batch_read_kept(self._h, i, cap, _p(ku, ctypes.c_int32), _p(kv, ctypes.c_int32),
+                                        _p(kw, ctypes.c_double), ctypes.byref(kept)))
+        c = kept.value
+        return ku[:c].copy(), kv[:c].copy(), kw[:c].copy()
+
+    def read_block(self, i, which):
+        out = np.zeros((int(self.n[i]), self.k), np.float64)
+        check(load().ss_batch_read_block(self._h, i, 0 if which == "Y" else 1, _p(out, ctypes.c_double)))
+        return out
+
+    def read_probs(self, i):
+        p = np.zeros(int(self.m[i]), np.float64)
+        check(load().ss_batch_read_probs(self._h, i, _p(p, ctypes.c_double)))
+        return p
+
+    def profile(self, enable=True):
+        check(load().ss_graph_profile(self._h, int(bool(enable))))
+
+    def profile_read(self):
+        arr = (_lib.KernelStat * len(_lib.KERNELS))()
+        check(load().ss_graph_profile_read(self._h, arr))
+        return {name: {"ms_total": arr[i].ms_total, "launches": arr[i].launches}
+                for i, name in enumerate(_lib.KERNELS)}
+
+    def schedule_info(self, direction):
+        si = _lib.ScheduleInfo()
+        check(load().ss_graph_schedule_info(self._h, direction, ctypes.byref(si)))
+        return {f: getattr(si, f) for f, _ in _lib.ScheduleInfo._fields_ if f != "reserved"}

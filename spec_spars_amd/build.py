@@ -12,7 +12,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libspecspars_hip.so")
 
-HIP_SOURCES = ["kernels.hip", "device.hip"]
+HIP_SOURCES = ["kernels.hip", "batch_kernels.hip", "device.hip"]
 CPP_SOURCES = ["factor.cpp", "graph.cpp", "capi.cpp", "sparsifier.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -1,0 +1,77 @@
+// Launchers of the tile-based batched kernels (batch_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hash_dev.h"
+#include "kernels.h"
+
+namespace ss {
+
+static const int kRowTile = 2048;    // rows per row tile
+static const int kEdgeTile = 256;    // edges per edge tile (one block)
+
+// Tiles of one kind: tile t covers [start[t], start[t+1]) of graph graph[t].
+struct Tiles {
+    const int32_t *start;
+    const int32_t *graph;
+    int count;
+};
+
+struct BatchSampleArgs {
+    int64_t m;                 // total edges (bound for kept positions)
+    Tiles et;                  // edge tiles
+    const int32_t *edge_off;   // per graph: first global edge
+    const int32_t *eu, *ev;
+    const double *ew;
+    const double *Z;
+    int k;
+    const double *beta;        // per graph
+    const uint32_t *keys;      // per graph: 8 ChaCha key words
+    const double *p_in;
+    double *p_out, *d_out;
+    uint8_t *mask;
+    int32_t *ku, *kv;
+    double *kw;
+    uint32_t *ticket;
+    uint64_t *tile_state;
+    int64_t *tile_incl;        // inclusive kept count after each tile
+    uint32_t epoch;
+    int *err;
+    int spin_limit;
+};
+
+void bt_sketch(hipStream_t s, Tiles rt, const int32_t *iptr, const int32_t *ieid, const double *ib, int k,
+               const uint64_t *seeds, const double *scales, int uniform, double *Y);
+template <typename TA, typename TB>
+void bt_tile_partial(hipStream_t s, Tiles rt, int k, const TA *A, const TB *B, const double *diag, int op,
+                     double *partial);
+void bt_pcg_init(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p_bb, const double *p_rz,
+                 PcgScalars sc);
+void bt_pcg_alpha(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, PcgScalars sc, int *nactive);
+void bt_pcg_check(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, int rule, double tol,
+                  PcgScalars sc, int *nactive);
+void bt_pcg_beta(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, PcgScalars sc);
+void bt_graph_sums(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, double *out);
+void bt_small_comp_sums(hipStream_t s, int nsmall, const int32_t *small_id, const int32_t *cptr, const int32_t *cvert,
+                        int k, const double *Y, double *csum);
+void bt_comp_means(hipStream_t s, int ncomp, int k, const int32_t *comp_graph, const int32_t *comp_off,
+                   const int32_t *cptr, const double *gsum, double *csum, double *cmean);
+template <typename TS, typename T>
+void bt_project(hipStream_t s, Tiles rt, int k, const TS *Y, const int32_t *comp_of_row, const double *cmean, T *R,
+                T *B, T *X);
+template <typename T>
+void bt_update_p(hipStream_t s, Tiles rt, int k, const double *beta, const T *Z, T *P);
+template <typename T>
+void bt_spmm_dot(hipStream_t s, Tiles rt, const int32_t *rowptr, const int32_t *col, const T *val, int k, const T *P,
+                 T *Q, double *partial);
+template <typename T>
+void bt_update_xr_rr(hipStream_t s, Tiles rt, int k, const double *alpha, const T *P, const T *Q, T *X, T *R,
+                     double *partial);
+void bt_residual64(hipStream_t s, Tiles rt, const int32_t *rowptr, const int32_t *col, const double *val, int k,
+                   const double *X, const double *B, double *out, double *partial);
+void bt_convert_masked(hipStream_t s, Tiles rt, int k, const double *src, const int *keep, float *dst);
+void bt_accumulate(hipStream_t s, Tiles rt, int k, const float *d, double *x);
+void bt_er_sample(hipStream_t s, const BatchSampleArgs &a, int ntiles);
+
+}  // namespace ss

@@ -1,0 +1,576 @@
+// Tile-based kernels of the batched device path.
+//
+// A batch packs G graphs into one block-diagonal system: graph g owns rows
+// [row_off[g], row_off[g+1]) and edges [edge_off[g], edge_off[g+1]).  Rows
+// are cut into tiles of kRowTile rows and edges into tiles of kEdgeTile
+// edges, never straddling two graphs; tile t knows its first row/edge and
+// its graph.  Every reduction is a per-tile partial followed by a per
+// (graph, column) sum over that graph's tiles in a fixed shape, so a graph's
+// results are bit-identical alone and inside any batch.
+#include <hip/hip_runtime.h>
+
+#include "batch_kernels.h"
+#include "hash_dev.h"
+
+namespace ss {
+
+#define BT_LANE (threadIdx.x & 63)
+#define BT_WAVE (threadIdx.x >> 6)
+
+__device__ __forceinline__ int bt_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+template <typename T>
+__device__ __forceinline__ T bt_dot_sub(T acc, int q0, int q1, const int32_t *__restrict__ col,
+                                        const T *__restrict__ val, const T *X, int k, int j) {
+    int q = q0;
+    for (; q + 7 < q1; q += 8) {
+        T x[8], v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            x[u] = X[(size_t)col[q + u] * k + j];
+            v[u] = val[q + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc -= v[u] * x[u];
+    }
+    for (; q < q1; q++) acc -= val[q] * X[(size_t)col[q] * k + j];
+    return acc;
+}
+
+// Per-tile column partial of f over the tile's rows; 4 waves stride the
+// rows, lanes take columns, waves combine in order.  Writes partial[t*k+j].
+__device__ __forceinline__ void bt_tile_store(double acc, int k, int jb, double *partial, double (*sm)[64]) {
+    const int w = BT_WAVE, lane = BT_LANE, j = jb + lane;
+    sm[w][lane] = acc;
+    __syncthreads();
+    if (w == 0 && j < k)
+        partial[(size_t)blockIdx.x * k + j] = ((sm[0][lane] + sm[1][lane]) + sm[2][lane]) + sm[3][lane];
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// (a) sketch: block per row tile, wave per row, seed/scale of the tile's graph
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void kb_sketch(Tiles rt, const int32_t *__restrict__ iptr,
+                                                 const int32_t *__restrict__ ieid, const double *__restrict__ ib,
+                                                 int k, const uint64_t *__restrict__ seeds,
+                                                 const double *__restrict__ scales, int uniform, int variant,
+                                                 double *__restrict__ Y) {
+#pragma clang fp contract(off)
+    const int t = blockIdx.x;
+    const int g = rt.graph[t];
+    const uint64_t seed = seeds[g];
+    const double scale = scales[g];
+    const int r0 = rt.start[t], r1 = rt.start[t + 1];
+    for (int row = r0 + BT_WAVE; row < r1; row += 4) {
+        const int r = bt_uniform(row);
+        const int q0 = iptr[r], q1 = iptr[r + 1];
+        for (int jb = 0; jb < k; jb += 64) {
+            const int j = jb + BT_LANE;
+            double acc = 0.0;
+            for (int q = q0; q < q1; q++) {
+                const double sv = sign_value((uint64_t)ieid[q], (uint64_t)j, seed, uniform, variant) * scale;
+                const double term = sv * ib[q];
+                acc = acc + term;
+            }
+            if (j < k) Y[(size_t)r * k + j] = acc;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// per-tile column partials: op 0 sum A*B, op 1 sum A, op 2 sum y (y / D)
+// (y^T D^+ y with the reference's skip of D == 0)
+// ---------------------------------------------------------------------------
+template <typename TA, typename TB>
+__global__ __launch_bounds__(256) void kb_tile_partial(Tiles rt, int k, const TA *__restrict__ A,
+                                                       const TB *__restrict__ B, const double *__restrict__ diag,
+                                                       int op, double *__restrict__ partial) {
+    __shared__ double sm[4][64];
+    const int t = blockIdx.x;
+    const int r0 = rt.start[t], r1 = rt.start[t + 1];
+    for (int jb = 0; jb < k; jb += 64) {
+        const int j = jb + BT_LANE;
+        double acc = 0.0;
+        if (j < k) {
+            for (int r = r0 + BT_WAVE; r < r1; r += 4) {
+                const double a = (double)A[(size_t)r * k + j];
+                if (op == 0) acc += a * (double)B[(size_t)r * k + j];
+                else if (op == 1) acc += a;
+                else {
+                    const double d = diag[r];
+                    acc += d != 0.0 ? a * (a / d) : a * a;
+                }
+            }
+        }
+        bt_tile_store(acc, k, jb, partial, sm);
+    }
+}
+
+// Per (graph, column) sum of the graph's tile partials by one 64-lane block:
+// lane l adds tiles l, l+64, ... in order, then a fixed butterfly.
+__device__ __forceinline__ double bt_graph_sum(const double *partial, const int32_t *gtile, int g, int k, int j) {
+    double s = 0.0;
+    for (int t = gtile[g] + BT_LANE; t < gtile[g + 1]; t += 64) s += partial[(size_t)t * k + j];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// PCG scalars, one 64-lane block per (graph, column), index gj = g*k + j
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void kb_pcg_init(int k, const int32_t *gtile, const double *p_bb,
+                                                  const double *p_rz, PcgScalars s) {
+    const int gj = blockIdx.x, g = gj / k, j = gj - g * k;
+    const double bb = bt_graph_sum(p_bb, gtile, g, k, j);
+    const double rz = bt_graph_sum(p_rz, gtile, g, k, j);
+    if (BT_LANE == 0) {
+        s.bnorm[gj] = sqrt(bb);
+        s.rz[gj] = rz;
+        s.active[gj] = bb > 0.0 ? 1 : 0;
+        s.iters[gj] = 0;
+        s.alpha[gj] = 0.0;
+        s.beta[gj] = 0.0;
+    }
+}
+
+__global__ __launch_bounds__(64) void kb_pcg_alpha(int k, const int32_t *gtile, const double *p_pq, PcgScalars s,
+                                                   int *nactive) {
+    const int gj = blockIdx.x, g = gj / k, j = gj - g * k;
+    const double pq = bt_graph_sum(p_pq, gtile, g, k, j);
+    if (BT_LANE == 0) {
+        double a = 0.0;
+        if (s.active[gj]) {
+            if (pq > 0.0) a = s.rz[gj] / pq;
+            else s.active[gj] = 0;
+        }
+        s.alpha[gj] = a;
+        if (gj == 0) *nactive = 0;
+    }
+}
+
+__global__ __launch_bounds__(64) void kb_pcg_check(int k, const int32_t *gtile, const double *p_rr, int rule,
+                                                   double tol, PcgScalars s, int *nactive) {
+    const int gj = blockIdx.x, g = gj / k, j = gj - g * k;
+    const double rr = bt_graph_sum(p_rr, gtile, g, k, j);
+    if (BT_LANE == 0) {
+        s.rr[gj] = rr;
+        if (s.active[gj]) {
+            s.iters[gj] += 1;
+            const double rn = sqrt(rr);
+            const bool stop = rule == 0 ? (rn <= sqrt(tol)) : (rn <= tol * s.bnorm[gj]);
+            if (stop) s.active[gj] = 0;
+        }
+        if (s.active[gj]) atomicAdd(nactive, 1);
+    }
+}
+
+__global__ __launch_bounds__(64) void kb_pcg_beta(int k, const int32_t *gtile, const double *p_rz, PcgScalars s) {
+    const int gj = blockIdx.x, g = gj / k, j = gj - g * k;
+    const double rzn = bt_graph_sum(p_rz, gtile, g, k, j);
+    if (BT_LANE == 0) {
+        double b = 0.0;
+        if (s.active[gj] && s.rz[gj] != 0.0) b = rzn / s.rz[gj];
+        s.beta[gj] = b;
+        s.rz[gj] = rzn;
+    }
+}
+
+// out[gj] = graph sum (plain)
+__global__ __launch_bounds__(64) void kb_graph_sums(int k, const int32_t *gtile, const double *partial,
+                                                    double *out) {
+    const int gj = blockIdx.x, g = gj / k, j = gj - g * k;
+    const double v = bt_graph_sum(partial, gtile, g, k, j);
+    if (BT_LANE == 0) out[gj] = v;
+}
+
+// ---------------------------------------------------------------------------
+// component means: small components summed by one wave each (vertices in
+// order); the graph's main component gets (graph sum - small sums) / size
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void kb_small_comp_sums(int nsmall, const int32_t *__restrict__ small_id,
+                                                          const int32_t *__restrict__ cptr,
+                                                          const int32_t *__restrict__ cvert, int k,
+                                                          const double *__restrict__ Y, double *__restrict__ csum) {
+    const int w = blockIdx.x * 4 + BT_WAVE;
+    if (w >= nsmall) return;
+    const int c = bt_uniform(small_id[w]);
+    const int v0 = cptr[c], v1 = cptr[c + 1];
+    for (int jb = 0; jb < k; jb += 64) {
+        const int j = jb + BT_LANE;
+        if (j >= k) break;
+        double s = 0.0;
+        for (int q = v0; q < v1; q++) s += Y[(size_t)cvert[q] * k + j];
+        csum[(size_t)c * k + j] = s;
+    }
+}
+
+__global__ __launch_bounds__(256) void kb_comp_means(int ncomp, int k, const int32_t *__restrict__ comp_graph,
+                                                     const int32_t *__restrict__ comp_off,
+                                                     const int32_t *__restrict__ cptr, const double *__restrict__ gsum,
+                                                     double *__restrict__ csum, double *__restrict__ cmean) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)ncomp * k) return;
+    const int c = (int)(idx / k), j = (int)(idx - (int64_t)c * k);
+    const int g = comp_graph[c];
+    const double size = (double)(cptr[c + 1] - cptr[c]);
+    double s;
+    if (c == comp_off[g]) {          // the graph's main component
+        s = gsum[(size_t)g * k + j];
+        for (int c2 = comp_off[g] + 1; c2 < comp_off[g + 1]; c2++) s -= csum[(size_t)c2 * k + j];
+    } else {
+        s = csum[(size_t)c * k + j];
+    }
+    cmean[(size_t)c * k + j] = s / size;
+}
+
+template <typename TS, typename T>
+__global__ __launch_bounds__(256) void kb_project(Tiles rt, int k, const TS *__restrict__ Y,
+                                                  const int32_t *__restrict__ comp_of_row,
+                                                  const double *__restrict__ cmean, T *__restrict__ R,
+                                                  T *__restrict__ B, T *__restrict__ X) {
+    const int t = blockIdx.x;
+    const int r0 = rt.start[t], r1 = rt.start[t + 1];
+    for (int r = r0 + BT_WAVE; r < r1; r += 4) {
+        const int c = comp_of_row[r];
+        for (int jb = 0; jb < k; jb += 64) {
+            const int j = jb + BT_LANE;
+            if (j >= k) break;
+            const size_t o = (size_t)r * k + j;
+            const T v = (T)((double)Y[o] - cmean[(size_t)c * k + j]);
+            R[o] = v;
+            B[o] = v;
+            X[o] = 0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// PCG vector kernels (block per row tile, per-(graph, column) scalars)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void kb_update_p(Tiles rt, int k, const double *__restrict__ beta,
+                                                   const T *__restrict__ Z, T *__restrict__ P) {
+    const int t = blockIdx.x, g = rt.graph[t];
+    const int r0 = rt.start[t], r1 = rt.start[t + 1];
+    for (int jb = 0; jb < k; jb += 64) {
+        const int j = jb + BT_LANE;
+        if (j >= k) break;
+        const T b = (T)beta[(size_t)g * k + j];
+        for (int r = r0 + BT_WAVE; r < r1; r += 4) {
+            const size_t o = (size_t)r * k + j;
+            P[o] = b == (T)0 ? Z[o] : Z[o] + b * P[o];
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void kb_spmm_dot(Tiles rt, const int32_t *__restrict__ rowptr,
+                                                   const int32_t *__restrict__ col, const T *__restrict__ val, int k,
+                                                   const T *__restrict__ P, T *__restrict__ Q,
+                                                   double *__restrict__ partial) {
+    __shared__ double sm[4][64];
+    const int t = blockIdx.x;
+    const int r0 = rt.start[t], r1 = rt.start[t + 1];
+    for (int jb = 0; jb < k; jb += 64) {
+        const int j = jb + BT_LANE;
+        double acc = 0.0;
+        for (int row = r0 + BT_WAVE; row < r1; row += 4) {
+            const int r = bt_uniform(row);
+            const int q0 = rowptr[r], q1 = rowptr[r + 1];
+            if (j < k) {
+                const T qv = -bt_dot_sub<T>((T)0, q0, q1, col, val, P, k, j);
+                const size_t o = (size_t)r * k + j;
+                Q[o] = qv;
+                acc += (double)P[o] * (double)qv;
+            }
+        }
+        bt_tile_store(acc, k, jb, partial, sm);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void kb_update_xr_rr(Tiles rt, int k, const double *__restrict__ alpha,
+                                                       const T *__restrict__ P, const T *__restrict__ Q,
+                                                       T *__restrict__ X, T *__restrict__ R,
+                                                       double *__restrict__ partial) {
+    __shared__ double sm[4][64];
+    const int t = blockIdx.x, g = rt.graph[t];
+    const int r0 = rt.start[t], r1 = rt.start[t + 1];
+    for (int jb = 0; jb < k; jb += 64) {
+        const int j = jb + BT_LANE;
+        double acc = 0.0;
+        if (j < k) {
+            const T a = (T)alpha[(size_t)g * k + j];
+            for (int r = r0 + BT_WAVE; r < r1; r += 4) {
+                const size_t o = (size_t)r * k + j;
+                X[o] += a * P[o];
+                const T rv = R[o] - a * Q[o];
+                R[o] = rv;
+                acc += (double)rv * (double)rv;
+            }
+        }
+        bt_tile_store(acc, k, jb, partial, sm);
+    }
+}
+
+// out = B - A X in FP64, with per-tile partials of |out|^2
+__global__ __launch_bounds__(256) void kb_residual64(Tiles rt, const int32_t *__restrict__ rowptr,
+                                                     const int32_t *__restrict__ col, const double *__restrict__ val,
+                                                     int k, const double *__restrict__ X, const double *__restrict__ B,
+                                                     double *__restrict__ out, double *__restrict__ partial) {
+    __shared__ double sm[4][64];
+    const int t = blockIdx.x;
+    const int r0 = rt.start[t], r1 = rt.start[t + 1];
+    for (int jb = 0; jb < k; jb += 64) {
+        const int j = jb + BT_LANE;
+        double acc = 0.0;
+        for (int row = r0 + BT_WAVE; row < r1; row += 4) {
+            const int r = bt_uniform(row);
+            const int q0 = rowptr[r], q1 = rowptr[r + 1];
+            if (j < k) {
+                const size_t o = (size_t)r * k + j;
+                const double v = bt_dot_sub<double>(B[o], q0, q1, col, val, X, k, j);
+                out[o] = v;
+                acc += v * v;
+            }
+        }
+        bt_tile_store(acc, k, jb, partial, sm);
+    }
+}
+
+// FP64 -> FP32 with columns of converged (graph, column) pairs zeroed
+__global__ __launch_bounds__(256) void kb_convert_masked(Tiles rt, int k, const double *__restrict__ src,
+                                                         const int *__restrict__ keep, float *__restrict__ dst) {
+    const int t = blockIdx.x, g = rt.graph[t];
+    const int r0 = rt.start[t], r1 = rt.start[t + 1];
+    for (int jb = 0; jb < k; jb += 64) {
+        const int j = jb + BT_LANE;
+        if (j >= k) break;
+        const bool on = keep[(size_t)g * k + j] != 0;
+        for (int r = r0 + BT_WAVE; r < r1; r += 4) {
+            const size_t o = (size_t)r * k + j;
+            dst[o] = on ? (float)src[o] : 0.0f;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void kb_accumulate(Tiles rt, int k, const float *__restrict__ d,
+                                                     double *__restrict__ x) {
+    const int t = blockIdx.x;
+    const int r0 = rt.start[t], r1 = rt.start[t + 1];
+    for (int jb = 0; jb < k; jb += 64) {
+        const int j = jb + BT_LANE;
+        if (j >= k) break;
+        for (int r = r0 + BT_WAVE; r < r1; r += 4) x[(size_t)r * k + j] += (double)d[(size_t)r * k + j];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// (d) ER + probability + coin + sample + compaction over edge tiles.
+// Coins use the graph's key and the edge's index within its graph.
+// ---------------------------------------------------------------------------
+#define BT_EPOCH_SHIFT 40
+#define BT_FLAG_SHIFT 38
+#define BT_VALUE_MASK ((1ull << 38) - 1)
+
+__global__ __launch_bounds__(256) void kb_er_sample(BatchSampleArgs a) {
+#pragma clang fp contract(off)
+    __shared__ uint32_t s_tile;
+    __shared__ int s_wcnt[4];
+    __shared__ int64_t s_excl;
+    if (threadIdx.x == 0) s_tile = atomicAdd(a.ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const int g = a.et.graph[tile];
+    const int64_t e0 = a.et.start[tile], e1 = a.et.start[tile + 1];
+    const int64_t e = e0 + threadIdx.x;
+    bool keep = false;
+    int32_t eu = 0, ev = 0;
+    double kwv = 0.0;
+    if (e < e1) {
+        eu = a.eu[e];
+        ev = a.ev[e];
+        const double w = a.ew[e];
+        const double lval = -1.0 * w;
+        double p;
+        if (a.p_in) {
+            p = a.p_in[e];
+        } else {
+            const double *zu = a.Z + (size_t)eu * a.k;
+            const double *zv = a.Z + (size_t)ev * a.k;
+            double s = 0.0;
+            for (int j = 0; j < a.k; j++) {
+                const double t = zu[j] - zv[j];
+                s = s + t * t;
+            }
+            const double d = sqrt(s);
+            if (a.d_out) a.d_out[e] = d;
+            const double d2 = d * d;
+            const double x = a.beta[g] * -1.0 * lval * d2 / (double)a.k;
+            p = 1.0 * (x < 1.0 ? x : 1.0);
+        }
+        if (a.p_out) a.p_out[e] = p;
+        CoinKey ck;
+#pragma unroll
+        for (int i = 0; i < 8; i++) ck.k[i] = a.keys[g * 8 + i];
+        const double c = coin_value(ck, (uint64_t)(e - a.edge_off[g]));
+        keep = c < p;
+        if (a.mask) a.mask[e] = keep ? 1 : 0;
+        if (keep) {
+            const double target = w / sqrt(p);
+            const double change = w - target;
+            kwv = -1.0 * (lval + change);
+        }
+    }
+    const int lane = BT_LANE, wv = BT_WAVE;
+    const uint64_t bal = __ballot(keep);
+    const int wpre = __popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) s_wcnt[wv] = __popcll(bal);
+    __syncthreads();
+    int woff = 0;
+    for (int w2 = 0; w2 < wv; w2++) woff += s_wcnt[w2];
+    if (threadIdx.x == 0) {
+        const uint64_t total = (uint64_t)(s_wcnt[0] + s_wcnt[1] + s_wcnt[2] + s_wcnt[3]);
+        const uint64_t ep = (uint64_t)a.epoch << BT_EPOCH_SHIFT;
+        int64_t excl = 0;
+        if (tile == 0) {
+            __hip_atomic_store(&a.tile_state[0], ep | (2ull << BT_FLAG_SHIFT) | total, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(&a.tile_state[tile], ep | (1ull << BT_FLAG_SHIFT) | total, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            int64_t j = (int64_t)tile - 1;
+            int spins = 0;
+            while (j >= 0) {
+                const uint64_t st = __hip_atomic_load(&a.tile_state[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint32_t sep = (uint32_t)(st >> BT_EPOCH_SHIFT);
+                const uint32_t flag = (uint32_t)(st >> BT_FLAG_SHIFT) & 3u;
+                if (sep != a.epoch || flag == 0) {
+                    if (++spins > a.spin_limit) {
+                        __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += (int64_t)(st & BT_VALUE_MASK);
+                if (flag == 2) break;
+                j--;
+            }
+            __hip_atomic_store(&a.tile_state[tile], ep | (2ull << BT_FLAG_SHIFT) | ((uint64_t)excl + total),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_excl = excl;
+        a.tile_incl[tile] = excl + (int64_t)total;
+    }
+    __syncthreads();
+    const int64_t pos = s_excl + woff + wpre;
+    if (keep && pos < a.m) {
+        a.ku[pos] = eu;
+        a.kv[pos] = ev;
+        a.kw[pos] = kwv;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline unsigned bt_grid(int64_t items, int per) {
+    int64_t g = (items + per - 1) / per;
+    return (unsigned)(g < 1 ? 1 : g);
+}
+
+void bt_sketch(hipStream_t s, Tiles rt, const int32_t *iptr, const int32_t *ieid, const double *ib, int k,
+               const uint64_t *seeds, const double *scales, int uniform, double *Y) {
+    hipLaunchKernelGGL(kb_sketch, dim3(rt.count), dim3(256), 0, s, rt, iptr, ieid, ib, k, seeds, scales, uniform, 0,
+                       Y);
+}
+
+template <typename TA, typename TB>
+void bt_tile_partial(hipStream_t s, Tiles rt, int k, const TA *A, const TB *B, const double *diag, int op,
+                     double *partial) {
+    hipLaunchKernelGGL((kb_tile_partial<TA, TB>), dim3(rt.count), dim3(256), 0, s, rt, k, A, B, diag, op, partial);
+}
+template void bt_tile_partial<double, double>(hipStream_t, Tiles, int, const double *, const double *,
+                                              const double *, int, double *);
+template void bt_tile_partial<float, float>(hipStream_t, Tiles, int, const float *, const float *, const double *,
+                                            int, double *);
+
+void bt_pcg_init(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p_bb, const double *p_rz,
+                 PcgScalars sc) {
+    hipLaunchKernelGGL(kb_pcg_init, dim3(gk), dim3(64), 0, s, k, gtile, p_bb, p_rz, sc);
+}
+void bt_pcg_alpha(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, PcgScalars sc, int *nactive) {
+    hipLaunchKernelGGL(kb_pcg_alpha, dim3(gk), dim3(64), 0, s, k, gtile, p, sc, nactive);
+}
+void bt_pcg_check(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, int rule, double tol,
+                  PcgScalars sc, int *nactive) {
+    hipLaunchKernelGGL(kb_pcg_check, dim3(gk), dim3(64), 0, s, k, gtile, p, rule, tol, sc, nactive);
+}
+void bt_pcg_beta(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, PcgScalars sc) {
+    hipLaunchKernelGGL(kb_pcg_beta, dim3(gk), dim3(64), 0, s, k, gtile, p, sc);
+}
+void bt_graph_sums(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, double *out) {
+    hipLaunchKernelGGL(kb_graph_sums, dim3(gk), dim3(64), 0, s, k, gtile, p, out);
+}
+void bt_small_comp_sums(hipStream_t s, int nsmall, const int32_t *small_id, const int32_t *cptr, const int32_t *cvert,
+                        int k, const double *Y, double *csum) {
+    if (nsmall <= 0) return;
+    hipLaunchKernelGGL(kb_small_comp_sums, dim3(bt_grid(nsmall, 4)), dim3(256), 0, s, nsmall, small_id, cptr, cvert, k,
+                       Y, csum);
+}
+void bt_comp_means(hipStream_t s, int ncomp, int k, const int32_t *comp_graph, const int32_t *comp_off,
+                   const int32_t *cptr, const double *gsum, double *csum, double *cmean) {
+    hipLaunchKernelGGL(kb_comp_means, dim3(bt_grid((int64_t)ncomp * k, 256)), dim3(256), 0, s, ncomp, k, comp_graph,
+                       comp_off, cptr, gsum, csum, cmean);
+}
+template <typename TS, typename T>
+void bt_project(hipStream_t s, Tiles rt, int k, const TS *Y, const int32_t *comp_of_row, const double *cmean, T *R,
+                T *B, T *X) {
+    hipLaunchKernelGGL((kb_project<TS, T>), dim3(rt.count), dim3(256), 0, s, rt, k, Y, comp_of_row, cmean, R, B, X);
+}
+template void bt_project<double, double>(hipStream_t, Tiles, int, const double *, const int32_t *, const double *,
+                                         double *, double *, double *);
+template void bt_project<float, float>(hipStream_t, Tiles, int, const float *, const int32_t *, const double *,
+                                       float *, float *, float *);
+template <typename T>
+void bt_update_p(hipStream_t s, Tiles rt, int k, const double *beta, const T *Z, T *P) {
+    hipLaunchKernelGGL(kb_update_p<T>, dim3(rt.count), dim3(256), 0, s, rt, k, beta, Z, P);
+}
+template void bt_update_p<double>(hipStream_t, Tiles, int, const double *, const double *, double *);
+template void bt_update_p<float>(hipStream_t, Tiles, int, const double *, const float *, float *);
+template <typename T>
+void bt_spmm_dot(hipStream_t s, Tiles rt, const int32_t *rowptr, const int32_t *col, const T *val, int k, const T *P,
+                 T *Q, double *partial) {
+    hipLaunchKernelGGL(kb_spmm_dot<T>, dim3(rt.count), dim3(256), 0, s, rt, rowptr, col, val, k, P, Q, partial);
+}
+template void bt_spmm_dot<double>(hipStream_t, Tiles, const int32_t *, const int32_t *, const double *, int,
+                                  const double *, double *, double *);
+template void bt_spmm_dot<float>(hipStream_t, Tiles, const int32_t *, const int32_t *, const float *, int,
+                                 const float *, float *, double *);
+template <typename T>
+void bt_update_xr_rr(hipStream_t s, Tiles rt, int k, const double *alpha, const T *P, const T *Q, T *X, T *R,
+                     double *partial) {
+    hipLaunchKernelGGL(kb_update_xr_rr<T>, dim3(rt.count), dim3(256), 0, s, rt, k, alpha, P, Q, X, R, partial);
+}
+template void bt_update_xr_rr<double>(hipStream_t, Tiles, int, const double *, const double *, const double *,
+                                      double *, double *, double *);
+template void bt_update_xr_rr<float>(hipStream_t, Tiles, int, const double *, const float *, const float *, float *,
+                                     float *, double *);
+void bt_residual64(hipStream_t s, Tiles rt, const int32_t *rowptr, const int32_t *col, const double *val, int k,
+                   const double *X, const double *B, double *out, double *partial) {
+    hipLaunchKernelGGL(kb_residual64, dim3(rt.count), dim3(256), 0, s, rt, rowptr, col, val, k, X, B, out, partial);
+}
+void bt_convert_masked(hipStream_t s, Tiles rt, int k, const double *src, const int *keep, float *dst) {
+    hipLaunchKernelGGL(kb_convert_masked, dim3(rt.count), dim3(256), 0, s, rt, k, src, keep, dst);
+}
+void bt_accumulate(hipStream_t s, Tiles rt, int k, const float *d, double *x) {
+    hipLaunchKernelGGL(kb_accumulate, dim3(rt.count), dim3(256), 0, s, rt, k, d, x);
+}
+void bt_er_sample(hipStream_t s, const BatchSampleArgs &a, int ntiles) {
+    hipLaunchKernelGGL(kb_er_sample, dim3(ntiles), dim3(256), 0, s, a);
+}
+
+}  // namespace ss

@@ -18,8 +18,6 @@ void set_error(const char *fmt, ...) {
     va_end(ap);
     g_last_error = buf;
 }
-void build_incidence(int64_t n, int64_t m, const int32_t *v1, const int32_t *v2, const double *w,
-                     std::vector<int32_t> &iptr, std::vector<int32_t> &ieid, std::vector<double> &ib);
 }  // namespace ss
 
 struct ss_factor {
@@ -176,15 +174,52 @@ int ss_factor_destroy(ss_factor *f) {
     return SS_OK;
 }
 
-// ---- graph ---------------------------------------------------------------------
+// ---- graph batches ---------------------------------------------------------------
+static int batch_create(ss_context *ctx, const std::vector<int64_t> &n, const std::vector<int64_t> &eptr,
+                        const int32_t *u, const int32_t *v, const double *w, ss_graph **out) {
+    const size_t G = n.size();
+    std::vector<HostGraph> hg(G);
+    std::vector<const HostGraph *> ptrs(G);
+    for (size_t i = 0; i < G; i++) {
+        const int64_t e0 = eptr[i], m = eptr[i + 1] - eptr[i];
+        const int rc = graph_build(n[i], m, m ? u + e0 : nullptr, m ? v + e0 : nullptr, m ? w + e0 : nullptr, hg[i]);
+        if (rc != SS_OK) {
+            if (G > 1) {
+                std::string msg = g_last_error;
+                return fail(rc, "graph %zu: %s", i, msg.c_str());
+            }
+            return rc;
+        }
+        ptrs[i] = &hg[i];
+    }
+    return batch_upload(ctx, ptrs, out);
+}
+
 int ss_graph_create(ss_context *ctx, int64_t n, int64_t m, const int32_t *u, const int32_t *v, const double *w,
                     ss_graph **out) {
     if (!ctx || !out || (m > 0 && (!u || !v || !w))) return fail(SS_ERR_INVALID, "null argument");
     SS_GUARD_BEGIN
-    HostGraph G;
-    SS_TRY(graph_build(n, m, u, v, w, G));
-    return graph_upload(ctx, G, out);
+    return batch_create(ctx, {n}, {0, m}, u, v, w, out);
     SS_GUARD_END
+}
+
+int ss_batch_create(ss_context *ctx, int32_t count, const int64_t *n, const int64_t *edge_ptr, const int32_t *u,
+                    const int32_t *v, const double *w, ss_graph **out) {
+    if (!ctx || !out || !n || !edge_ptr || count < 1) return fail(SS_ERR_INVALID, "null argument or empty batch");
+    SS_GUARD_BEGIN
+    if (edge_ptr[0] != 0) return fail(SS_ERR_INVALID, "edge_ptr[0] must be 0");
+    for (int32_t i = 0; i < count; i++)
+        if (edge_ptr[i + 1] < edge_ptr[i]) return fail(SS_ERR_INVALID, "edge_ptr not monotone at %d", i);
+    if (edge_ptr[count] > 0 && (!u || !v || !w)) return fail(SS_ERR_INVALID, "null edge arrays");
+    return batch_create(ctx, std::vector<int64_t>(n, n + count), std::vector<int64_t>(edge_ptr, edge_ptr + count + 1),
+                        u, v, w, out);
+    SS_GUARD_END
+}
+
+int ss_batch_count(const ss_graph *g, int32_t *count) {
+    if (!g || !count) return fail(SS_ERR_INVALID, "null argument");
+    *count = g->G;
+    return SS_OK;
 }
 
 int ss_graph_destroy(ss_graph *g) {
@@ -199,31 +234,53 @@ int ss_graph_destroy(ss_graph *g) {
 int ss_graph_bytes(int64_t n, int64_t m, int64_t factor_nnz, int32_t k, int32_t precision, size_t *bytes) {
     if (!bytes || n <= 0 || m < 0 || k < 1) return fail(SS_ERR_INVALID, "bad arguments");
     const size_t N = (size_t)n, M = (size_t)m, nnzL = N + 2 * M, F = (size_t)std::max<int64_t>(factor_nnz, n);
+    const size_t rt = (N + kRowTile - 1) / kRowTile + 1, et = (M + kEdgeTile - 1) / kEdgeTile + 1;
     size_t b = 0;
     b += M * (4 + 4 + 8);                      // canonical edges
     b += (N + 1) * 4 + nnzL * (4 + 8 + 4);     // Laplacian CSR (FP64 + FP32 values)
     b += (N + 1) * 4 + 2 * M * (4 + 8);        // incidence lists
-    b += N * 4 + 8;                            // flags, status
-    b += M * (8 + 8 + 1 + 4 + 4 + 8) + (M / 256 + 2) * 8 + 16;   // sampling
-    b += 2 * ((N + 1) * 4 + N * 4 + (F - N) * (4 + 8 + 4)) + N * 8;  // factor schedules
+    b += N * 4 * 2 + 64;                       // components
+    b += rt * 8 + et * 8 + 64;                 // tiles
+    b += M * (8 + 8 + 1 + 4 + 4 + 8) + et * 16 + 16;                  // sampling
+    b += 2 * ((N + 1) * 4 + N * 4 + (F - N) * (4 + 8 + 4)) + N * 12;  // factor schedules
     const size_t nk = N * (size_t)k;
     if (precision == SS_PREC_FP64) b += nk * 8 * 8;
     else b += nk * (4 * 8 + 8 * 4);
-    b += (size_t)2048 * k * 8 + (size_t)k * 64;
+    b += rt * k * 16 + (size_t)k * 128;
     *bytes = b;
+    return SS_OK;
+}
+
+static int need_single(const ss_graph *g) {
+    if (g->G != 1) return fail(SS_ERR_INVALID, "this call takes a single graph; use the ss_batch_* call for a batch of %d", g->G);
     return SS_OK;
 }
 
 int ss_graph_set_factor(ss_graph *g, const ss_factor *f) {
     if (!g || !f) return fail(SS_ERR_INVALID, "null argument");
+    SS_TRY(need_single(g));
     SS_GUARD_BEGIN
-    return graph_upload_factor(g, f->F);
+    return batch_upload_factors(g, {&f->F});
+    SS_GUARD_END
+}
+
+int ss_batch_set_factors(ss_graph *g, int32_t count, const ss_factor *const *factors) {
+    if (!g || !factors) return fail(SS_ERR_INVALID, "null argument");
+    if (count != g->G) return fail(SS_ERR_INVALID, "batch holds %d graphs, got %d factors", g->G, count);
+    SS_GUARD_BEGIN
+    std::vector<const Factor *> fs((size_t)count);
+    for (int32_t i = 0; i < count; i++) {
+        if (!factors[i]) return fail(SS_ERR_INVALID, "factor %d is null", i);
+        fs[i] = &factors[i]->F;
+    }
+    return batch_upload_factors(g, fs);
     SS_GUARD_END
 }
 
 int ss_graph_laplacian(const ss_graph *g, int32_t *colptr, int32_t *rowidx, double *vals) {
     if (!g || !colptr || !rowidx || !vals) return fail(SS_ERR_INVALID, "null argument");
-    const int64_t n = g->n, nnz = n + 2 * g->m;
+    SS_TRY(need_single(g));
+    const int64_t n = g->N, nnz = n + 2 * g->M;
     SS_HIP(hipSetDevice(g->ctx->device));
     SS_HIP(hipMemcpy(colptr, g->rowptr, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost));
     SS_HIP(hipMemcpy(rowidx, g->col, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost));
@@ -237,68 +294,205 @@ static float elapsed(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
-int ss_graph_sparsify(ss_graph *g, const ss_sparsify_opts *o, ss_sparsify_report *rep) {
-    if (!g || !o) return fail(SS_ERR_INVALID, "null argument");
-    SS_GUARD_BEGIN
-    if (!g->has_factor) return fail(SS_ERR_INVALID, "graph has no factor (ss_graph_set_factor)");
+static bool same_solve(const ss_solve_opts &a, const ss_solve_opts &b) {
+    return a.stop_rule == b.stop_rule && a.max_iter == b.max_iter && a.tol == b.tol && a.precision == b.precision &&
+           a.fail_relres == b.fail_relres;
+}
+
+// Sketch -> solve -> sample over the whole batch (opts has G entries).
+static int batch_sparsify(ss_graph *g, const ss_sparsify_opts *opts, ss_sparsify_report *rep) {
+    if (!g->has_factor) return fail(SS_ERR_INVALID, "no factor set (ss_graph_set_factor / ss_batch_set_factors)");
+    const int G = g->G;
+    const ss_sparsify_opts &o0 = opts[0];
+    std::vector<GraphParams> prm((size_t)G);
+    for (int i = 0; i < G; i++) {
+        const ss_sparsify_opts &o = opts[i];
+        if (o.k != o0.k || o.uniform != o0.uniform || !same_solve(o.solve, o0.solve))
+            return fail(SS_ERR_INVALID, "graph %d: k, uniform and solve options must agree across the batch", i);
+        if (!(o.beta > 0.0) || !std::isfinite(o.beta) || !std::isfinite(o.scale))
+            return fail(SS_ERR_INVALID, "graph %d: bad beta %g or scale %g", i, o.beta, o.scale);
+        prm[i].sketch_seed = o.sketch_seed;
+        prm[i].sampling_seed = o.sampling_seed;
+        prm[i].scale = o.scale;
+        prm[i].beta = o.beta;
+    }
+    const int k = o0.k;
     ss_context *c = g->ctx;
     SS_HIP(hipSetDevice(c->device));
     hipStream_t s = c->stream;
+    SS_TRY(graph_ensure_work(g, k, o0.solve.precision));
+    SS_TRY(graph_set_params(g, prm));
     SS_HIP(hipEventRecord(c->ev[0], s));
-    SS_TRY(graph_sketch(g, o->k, o->sketch_seed, o->scale, o->uniform, o->solve.precision));
+    SS_TRY(graph_sketch(g, k, o0.uniform, o0.solve.precision));
     SS_HIP(hipEventRecord(c->ev[1], s));
     SolveStats st;
-    SS_TRY(graph_solve(g, o->k, o->solve, st));
+    SS_TRY(graph_solve(g, k, o0.solve, st));
     SS_HIP(hipEventRecord(c->ev[2], s));
-    SS_TRY(graph_sample(g, o->k, o->beta, o->sampling_seed, nullptr, false));
+    SS_TRY(graph_sample(g, k, nullptr, false));
     SS_HIP(hipEventRecord(c->ev[3], s));
     SS_HIP(hipEventSynchronize(c->ev[3]));
     SS_TRY(prof_resolve(g));
-    g->last_k = o->k;
-    g->last_prec = o->solve.precision;
-    if (rep) {
-        rep->kept = g->last_kept;
-        rep->max_iters = st.max_iters;
-        rep->refinements = st.refinements;
-        rep->max_relres = st.max_relres;
-        rep->ms_sketch = elapsed(c->ev[0], c->ev[1]);
-        rep->ms_solve = elapsed(c->ev[1], c->ev[2]);
-        rep->ms_sample = elapsed(c->ev[2], c->ev[3]);
-        rep->ms_total = elapsed(c->ev[0], c->ev[3]);
+    g->last_k = k;
+    double worst = 0.0;
+    int worst_g = 0;
+    for (int i = 0; i < G; i++) {
+        int32_t mi = 0;
+        double mr = 0.0;
+        for (int j = 0; j < k; j++) {
+            mi = std::max(mi, st.iters[(size_t)i * k + j]);
+            mr = std::max(mr, st.relres[(size_t)i * k + j]);
+        }
+        if (mr > worst) {
+            worst = mr;
+            worst_g = i;
+        }
+        if (rep) {
+            ss_sparsify_report &r = rep[i];
+            r.kept = g->kept_count[i];
+            r.max_iters = mi;
+            r.refinements = st.refinements;
+            r.max_relres = mr;
+            r.ms_sketch = elapsed(c->ev[0], c->ev[1]);
+            r.ms_solve = elapsed(c->ev[1], c->ev[2]);
+            r.ms_sample = elapsed(c->ev[2], c->ev[3]);
+            r.ms_total = elapsed(c->ev[0], c->ev[3]);
+        }
     }
-    if (st.max_relres >= o->solve.fail_relres)
-        return fail(SS_ERR_NOT_CONVERGED, "solve did not converge: max relative residual %.3e >= %.3e",
-                    st.max_relres, o->solve.fail_relres);
+    if (worst >= o0.solve.fail_relres)
+        return fail(SS_ERR_NOT_CONVERGED, "solve did not converge: graph %d has relative residual %.3e >= %.3e",
+                    worst_g, worst, o0.solve.fail_relres);
+    return SS_OK;
+}
+
+int ss_graph_sparsify(ss_graph *g, const ss_sparsify_opts *o, ss_sparsify_report *rep) {
+    if (!g || !o) return fail(SS_ERR_INVALID, "null argument");
+    SS_TRY(need_single(g));
+    SS_GUARD_BEGIN
+    return batch_sparsify(g, o, rep);
+    SS_GUARD_END
+}
+
+int ss_batch_sparsify(ss_graph *g, const ss_sparsify_opts *opts, ss_sparsify_report *rep) {
+    if (!g || !opts) return fail(SS_ERR_INVALID, "null argument");
+    SS_GUARD_BEGIN
+    return batch_sparsify(g, opts, rep);
+    SS_GUARD_END
+}
+
+int ss_batch_write_rhs(ss_graph *g, int32_t i, int32_t k, int32_t precision, const double *Y) {
+    if (!g || !Y) return fail(SS_ERR_INVALID, "null argument");
+    if (i < 0 || i >= g->G) return fail(SS_ERR_INVALID, "graph %d outside the batch of %d", i, g->G);
+    SS_GUARD_BEGIN
+    SS_TRY(graph_ensure_work(g, k, precision));
+    const int64_t r0 = g->row_off[i], ni = g->row_off[i + 1] - r0;
+    SS_HIP(hipSetDevice(g->ctx->device));
+    if (ni) SS_HIP(hipMemcpy(g->Y64 + (size_t)r0 * k, Y, sizeof(double) * (size_t)ni * k, hipMemcpyHostToDevice));
     return SS_OK;
     SS_GUARD_END
 }
 
+int ss_batch_solve(ss_graph *g, int32_t k, const ss_solve_opts *opts, int32_t *iters, double *relres,
+                   int32_t *refinements) {
+    if (!g) return fail(SS_ERR_INVALID, "null graph");
+    SS_GUARD_BEGIN
+    ss_solve_opts o;
+    if (opts) o = *opts;
+    else ss_solve_opts_default(&o);
+    if (g->wk != k || g->wprec != o.precision)
+        return fail(SS_ERR_INVALID, "right-hand sides were written for k=%d precision %d, solve asks k=%d precision %d",
+                    g->wk, g->wprec, k, o.precision);
+    SS_HIP(hipSetDevice(g->ctx->device));
+    SolveStats st;
+    SS_TRY(graph_solve(g, k, o, st));
+    SS_TRY(prof_resolve(g));
+    g->last_k = k;
+    const size_t gk = (size_t)g->G * k;
+    if (iters) memcpy(iters, st.iters.data(), sizeof(int32_t) * gk);
+    if (relres) memcpy(relres, st.relres.data(), sizeof(double) * gk);
+    if (refinements) *refinements = st.refinements;
+    double worst = 0.0;
+    for (double r : st.relres) worst = std::max(worst, r);
+    if (worst >= o.fail_relres)
+        return fail(SS_ERR_NOT_CONVERGED, "solve did not converge: max relative residual %.3e >= %.3e", worst,
+                    o.fail_relres);
+    return SS_OK;
+    SS_GUARD_END
+}
+
+static int read_kept(const ss_graph *g, int32_t i, int64_t cap, int32_t *ku, int32_t *kv, double *kw,
+                     int64_t *kept) {
+    if (i < 0 || i >= g->G) return fail(SS_ERR_INVALID, "graph %d outside the batch of %d", i, g->G);
+    const int64_t start = g->kept_start[i], cnt = g->kept_count[i];
+    if (kept) *kept = cnt;
+    if (cnt > cap) return fail(SS_ERR_CAPACITY, "capacity %lld < kept %lld", (long long)cap, (long long)cnt);
+    if (cnt == 0) return SS_OK;
+    SS_HIP(hipSetDevice(g->ctx->device));
+    const int32_t off = (int32_t)g->row_off[i];
+    if (ku) {
+        SS_HIP(hipMemcpy(ku, g->ku + start, sizeof(int32_t) * cnt, hipMemcpyDeviceToHost));
+        for (int64_t q = 0; q < cnt; q++) ku[q] -= off;
+    }
+    if (kv) {
+        SS_HIP(hipMemcpy(kv, g->kv + start, sizeof(int32_t) * cnt, hipMemcpyDeviceToHost));
+        for (int64_t q = 0; q < cnt; q++) kv[q] -= off;
+    }
+    if (kw) SS_HIP(hipMemcpy(kw, g->kw + start, sizeof(double) * cnt, hipMemcpyDeviceToHost));
+    return SS_OK;
+}
+
 int ss_graph_read_kept(const ss_graph *g, int64_t cap, int32_t *ku, int32_t *kv, double *kw, int64_t *kept) {
     if (!g) return fail(SS_ERR_INVALID, "null graph");
-    const int64_t k = g->last_kept;
-    if (kept) *kept = k;
-    if (k > cap) return fail(SS_ERR_CAPACITY, "capacity %lld < kept %lld", (long long)cap, (long long)k);
+    SS_TRY(need_single(g));
+    return read_kept(g, 0, cap, ku, kv, kw, kept);
+}
+
+int ss_batch_read_kept(const ss_graph *g, int32_t i, int64_t cap, int32_t *ku, int32_t *kv, double *kw,
+                       int64_t *kept) {
+    if (!g) return fail(SS_ERR_INVALID, "null graph");
+    return read_kept(g, i, cap, ku, kv, kw, kept);
+}
+
+static int read_block(const ss_graph *g, int32_t i, int which, double *out) {
+    if (!out) return fail(SS_ERR_INVALID, "null argument");
+    if (i < 0 || i >= g->G) return fail(SS_ERR_INVALID, "graph %d outside the batch of %d", i, g->G);
+    if (which != 0 && which != 1) return fail(SS_ERR_INVALID, "which must be 0 (Y) or 1 (Z)");
+    if (g->wk == 0) return fail(SS_ERR_INVALID, "nothing computed yet");
     SS_HIP(hipSetDevice(g->ctx->device));
-    if (ku && k) SS_HIP(hipMemcpy(ku, g->ku, sizeof(int32_t) * k, hipMemcpyDeviceToHost));
-    if (kv && k) SS_HIP(hipMemcpy(kv, g->kv, sizeof(int32_t) * k, hipMemcpyDeviceToHost));
-    if (kw && k) SS_HIP(hipMemcpy(kw, g->kw, sizeof(double) * k, hipMemcpyDeviceToHost));
+    const double *src = which == 0 ? g->Y64 : g->X64;
+    const int64_t r0 = g->row_off[i], ni = g->row_off[i + 1] - r0;
+    if (ni) SS_HIP(hipMemcpy(out, src + (size_t)r0 * g->wk, sizeof(double) * (size_t)ni * g->wk, hipMemcpyDeviceToHost));
     return SS_OK;
 }
 
 int ss_graph_read_block(const ss_graph *g, int which, double *out) {
-    if (!g || !out) return fail(SS_ERR_INVALID, "null argument");
-    if (g->wk == 0) return fail(SS_ERR_INVALID, "nothing computed yet");
+    if (!g) return fail(SS_ERR_INVALID, "null graph");
+    SS_TRY(need_single(g));
+    return read_block(g, 0, which, out);
+}
+
+int ss_batch_read_block(const ss_graph *g, int32_t i, int which, double *out) {
+    if (!g) return fail(SS_ERR_INVALID, "null graph");
+    return read_block(g, i, which, out);
+}
+
+static int read_probs(const ss_graph *g, int32_t i, double *p) {
+    if (!p) return fail(SS_ERR_INVALID, "null argument");
+    if (i < 0 || i >= g->G) return fail(SS_ERR_INVALID, "graph %d outside the batch of %d", i, g->G);
     SS_HIP(hipSetDevice(g->ctx->device));
-    const double *src = which == 0 ? g->Y64 : g->X64;
-    SS_HIP(hipMemcpy(out, src, sizeof(double) * (size_t)g->n * g->wk, hipMemcpyDeviceToHost));
+    const int64_t e0 = g->edge_off[i], mi = g->edge_off[i + 1] - e0;
+    if (mi) SS_HIP(hipMemcpy(p, g->p + e0, sizeof(double) * mi, hipMemcpyDeviceToHost));
     return SS_OK;
 }
 
 int ss_graph_read_probs(const ss_graph *g, double *p) {
-    if (!g || !p) return fail(SS_ERR_INVALID, "null argument");
-    SS_HIP(hipSetDevice(g->ctx->device));
-    if (g->m) SS_HIP(hipMemcpy(p, g->p, sizeof(double) * g->m, hipMemcpyDeviceToHost));
-    return SS_OK;
+    if (!g) return fail(SS_ERR_INVALID, "null graph");
+    SS_TRY(need_single(g));
+    return read_probs(g, 0, p);
+}
+
+int ss_batch_read_probs(const ss_graph *g, int32_t i, double *p) {
+    if (!g) return fail(SS_ERR_INVALID, "null graph");
+    return read_probs(g, i, p);
 }
 
 int ss_graph_profile(ss_graph *g, int enable) {
@@ -425,12 +619,12 @@ int ss_solve_lap(ss_context *ctx, const double *Y, int64_t n, int32_t k, const i
     Factor F;
     SS_TRY(factor_build(n, colptr, rowidx, vals, o.factor_seed, o.ordering, F));
     ss_graph *g = nullptr;
-    SS_TRY(graph_upload(ctx, G, &g));
+    SS_TRY(batch_upload(ctx, {&G}, &g));
     struct Del {
         ss_graph *g;
         ~Del() { ss_graph_destroy(g); }
     } del{g};
-    SS_TRY(graph_upload_factor(g, F));
+    SS_TRY(batch_upload_factors(g, {&F}));
     SS_TRY(graph_ensure_work(g, k, o.precision));
     hipStream_t s = ctx->stream;
     SS_HIP(hipMemcpyAsync(g->Y64, Y, sizeof(double) * (size_t)n * k, hipMemcpyHostToDevice, s));
@@ -440,21 +634,27 @@ int ss_solve_lap(ss_context *ctx, const double *Y, int64_t n, int32_t k, const i
     SS_HIP(hipEventRecord(ctx->ev[5], s));
     SS_HIP(hipMemcpyAsync(Z, g->X64, sizeof(double) * (size_t)n * k, hipMemcpyDeviceToHost, s));
     SS_HIP(hipStreamSynchronize(s));
+    int32_t max_iters = 0;
+    double max_relres = 0.0;
+    for (int j = 0; j < k; j++) {
+        max_iters = std::max(max_iters, st.iters[j]);
+        max_relres = std::max(max_relres, st.relres[j]);
+    }
     if (rep) {
         if (rep->iters) memcpy(rep->iters, st.iters.data(), sizeof(int32_t) * k);
         if (rep->relres) memcpy(rep->relres, st.relres.data(), sizeof(double) * k);
         rep->factor_nnz = F.nnz;
         rep->depth_fwd = F.depth_fwd;
         rep->depth_bwd = F.depth_bwd;
-        rep->max_iters = st.max_iters;
+        rep->max_iters = max_iters;
         rep->refinements = st.refinements;
-        rep->max_relres = st.max_relres;
+        rep->max_relres = max_relres;
         rep->ms_factor = F.ms_build;
         rep->ms_solve = elapsed(ctx->ev[4], ctx->ev[5]);
     }
-    if (st.max_relres >= o.fail_relres)
+    if (max_relres >= o.fail_relres)
         return fail(SS_ERR_NOT_CONVERGED, "solve did not converge: max relative residual %.3e >= %.3e",
-                    st.max_relres, o.fail_relres);
+                    max_relres, o.fail_relres);
     return SS_OK;
     SS_GUARD_END
 }

@@ -1,9 +1,11 @@
-// Device-resident state: context (device + stream) and graph.
+// Device-resident state: context (device + stream) and the batch of graphs
+// (a single graph is a batch of one).
 #pragma once
 #include <hip/hip_runtime.h>
 
 #include <vector>
 
+#include "batch_kernels.h"
 #include "kernels.h"
 #include "ss_internal.h"
 
@@ -43,29 +45,43 @@ struct DevArena {
     ~DevArena() { release(); }
 };
 
-// Work blocks of one PCG run in precision T (n x k each).
+// Work blocks of one PCG run in precision T (N x k each).
 template <typename T>
 struct PcgWork {
-    T *B = nullptr, *R = nullptr, *P = nullptr, *P2 = nullptr, *Q = nullptr, *Zp = nullptr, *W = nullptr;
+    T *B = nullptr, *R = nullptr, *P = nullptr, *Q = nullptr, *Zp = nullptr, *W = nullptr;
 };
 
 }  // namespace ss
 
 struct ss_graph {
     ss_context *ctx = nullptr;
-    int64_t n = 0, m = 0;
-    ss::DevArena arena_struct, arena_factor, arena_work;
-    // canonical edges
+    int G = 0;                       // graphs in the batch
+    int64_t N = 0, M = 0;            // total rows / edges
+    std::vector<int64_t> row_off, edge_off;       // host, G+1
+    std::vector<int32_t> rtile_off, etile_off;    // host, G+1: tile ranges per graph
+    int nrt = 0, net = 0;
+    ss::DevArena arena_struct, arena_factor, arena_work, arena_params;
+    // tiles
+    int32_t *rt_start = nullptr, *rt_graph = nullptr, *et_start = nullptr, *et_graph = nullptr;
+    int32_t *gtile = nullptr;        // device copy of rtile_off
+    int32_t *edge_off_dev = nullptr; // device copy of edge_off (int32)
+    // canonical edges (global vertex ids)
     int32_t *eu = nullptr, *ev = nullptr;
     double *ew = nullptr;
-    // Laplacian CSR (with diagonal), FP64 and FP32 values
+    // block-diagonal Laplacian CSR (with diagonal), FP64 and FP32 values
     int32_t *rowptr = nullptr, *col = nullptr;
     double *val = nullptr;
     float *valf = nullptr;
-    // incidence lists for the sketch: per node, (edge id, +-sqrt(w)) by edge
+    // incidence lists for the sketch: per node, (local edge id, +-sqrt(w))
     int32_t *iptr = nullptr, *ieid = nullptr;
     double *ib = nullptr;
-    // factor schedules
+    // connected components: comp_of_row, per comp graph / vertex list;
+    // per graph the main (largest) component comes first
+    int ncomp = 0, nsmall = 0;
+    int32_t *comp_of_row = nullptr, *comp_graph = nullptr, *comp_off = nullptr, *cptr = nullptr, *cvert = nullptr,
+            *small_id = nullptr;
+    std::vector<int32_t> h_comp_count;   // components per graph
+    // factor schedules (of the concatenated factor)
     bool has_factor = false;
     int64_t factor_nnz = 0;
     int32_t depth_fwd = 0, depth_bwd = 0;
@@ -73,35 +89,40 @@ struct ss_graph {
     int32_t *b_rows = nullptr, *b_ptr = nullptr, *b_col = nullptr;
     double *f_val = nullptr, *b_val = nullptr, *diag = nullptr;
     float *f_valf = nullptr, *b_valf = nullptr;
-    int64_t f_nnz = 0, b_nnz = 0;
     std::vector<int32_t> f_level_ptr, b_level_ptr;   // host copies: launch bounds per level
     std::vector<int32_t> f_level_long, b_level_long; // long rows leading each level
     int32_t f_small_lo = 0, f_small_hi = 0, b_small_lo = 0, b_small_hi = 0;
-    int32_t *f_lp_dev = nullptr, *b_lp_dev = nullptr, *f_split = nullptr;
+    int32_t *f_split = nullptr;
+    int32_t *f_unit_ptr = nullptr, *f_unit_t = nullptr, *f_unit_s = nullptr;   // small-run work units
+    int32_t *b_unit_ptr = nullptr, *b_unit_t = nullptr, *b_unit_s = nullptr;
     ss_schedule_info sched_info[2] = {};
-    // sync state
-    uint32_t *flags = nullptr;
-    uint32_t epoch = 0;
-    int *status = nullptr;          // {nactive, err}
+    // status
+    int *status = nullptr;           // {nactive, err}
     // work (allocated for wk columns at wprec)
     int wk = 0, wprec = -1;
     double *Y64 = nullptr, *X64 = nullptr, *R64 = nullptr, *B64 = nullptr;
     float *Y32 = nullptr, *X32 = nullptr;
     ss::PcgWork<double> w64;
     ss::PcgWork<float> w32;
-    double *partial = nullptr, *partial2 = nullptr, *colbuf = nullptr;
+    double *partial = nullptr, *partial2 = nullptr;
+    double *gsum = nullptr, *csum = nullptr, *cmean = nullptr;
     double *sc_buf = nullptr;
-    int *sc_int = nullptr;
+    int *sc_int = nullptr, *colkeep = nullptr;
     ss::PcgScalars sc{};
+    // per-call parameters (per graph)
+    uint64_t *p_seeds = nullptr;
+    double *p_scales = nullptr, *p_betas = nullptr;
+    uint32_t *p_keys = nullptr;
     // sampling
     double *p = nullptr, *dnorm = nullptr;
     uint8_t *mask = nullptr;
     int32_t *ku = nullptr, *kv = nullptr;
     double *kw = nullptr;
     uint64_t *tile_state = nullptr;
+    int64_t *tile_incl = nullptr;
     uint32_t *ticket = nullptr;
-    int64_t *kept = nullptr;
     uint32_t samp_epoch = 0;
+    std::vector<int64_t> kept_start, kept_count;   // per graph, last sample
     // per-kernel HIP-event timing (ss_graph_profile)
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
@@ -113,13 +134,10 @@ struct ss_graph {
     std::vector<ProfRec> prof_recs;
     double prof_ms[SS_KERNEL_COUNT] = {};
     int64_t prof_cnt[SS_KERNEL_COUNT] = {};
-    int64_t last_kept = 0;
     int last_k = 0;
-    int last_prec = 0;
 };
 
 namespace ss {
-// Records HIP events around one launch on the graph's stream when profiling.
 // A scope covering `launches` kernel launches of one kind.
 struct ProfScope {
     ss_graph *g;
@@ -130,20 +148,30 @@ struct ProfScope {
     ~ProfScope();
 };
 int prof_resolve(ss_graph *g);
-int graph_upload(ss_context *ctx, const HostGraph &G, ss_graph **out);
+
+// Incidence lists of B for the sketch: per vertex, (edge id, B entry).
+void build_incidence(int64_t n, int64_t m, const int32_t *v1, const int32_t *v2, const double *w,
+                     std::vector<int32_t> &iptr, std::vector<int32_t> &ieid, std::vector<double> &ib);
+// Upload a batch of graphs (host structures built by graph_build).
+int batch_upload(ss_context *ctx, const std::vector<const HostGraph *> &graphs, ss_graph **out);
+// Concatenate the graphs' factors into one schedule and upload it.
+int batch_upload_factors(ss_graph *g, const std::vector<const Factor *> &factors);
 int graph_ensure_work(ss_graph *g, int k, int prec);
-int graph_upload_factor(ss_graph *g, const Factor &F);
+
 struct SolveStats {
-    int32_t max_iters = 0;
+    std::vector<int32_t> iters;      // [G*k]
+    std::vector<double> relres;      // [G*k]
     int32_t refinements = 0;
-    double max_relres = 0.0;
-    std::vector<int32_t> iters;
-    std::vector<double> relres;
 };
-// Solve L Z = Y - mean for the graph's resident sketch (Y64 or Y32); the
-// FP64 solution lands in X64.
+// Solve L Z = Y - (component means) for every graph; Y64 must hold the
+// right-hand sides; the FP64 solution lands in X64.
 int graph_solve(ss_graph *g, int k, const ss_solve_opts &o, SolveStats &st);
-int graph_sketch(ss_graph *g, int k, uint64_t seed, double scale, int uniform, int prec);
-int graph_sample(ss_graph *g, int k, double beta, uint64_t sampling_seed, const double *p_in_dev,
-                 bool write_d);
+// Per-graph parameters of the next sketch / sample.
+struct GraphParams {
+    uint64_t sketch_seed = 0, sampling_seed = 0;
+    double scale = 1.0, beta = 1.0;
+};
+int graph_set_params(ss_graph *g, const std::vector<GraphParams> &prm);
+int graph_sketch(ss_graph *g, int k, int uniform, int prec);
+int graph_sample(ss_graph *g, int k, const double *p_in_dev, bool write_d);
 }  // namespace ss

@@ -1,21 +1,30 @@
-// Device-resident graph, multi-RHS PCG driver and the sparsify pipeline.
+// Device-resident batch of graphs (a single graph is a batch of one),
+// multi-RHS PCG driver and the sparsify pipeline.
 //
 // The PCG is the method MKL's dcg runs for one column at a time in
 // include/custom_cg.hpp:95-274 (x0 = 0, b projected to zero mean, stop test
 // after every x/r update, M^{-1} = L^{-T} D^+ L^{-1}), run here on all k
-// columns at once: every column keeps its own alpha/beta/rz in FP64 and a
-// convergence mask that freezes it (alpha = 0) once its stop rule holds.
+// columns of all G graphs at once: every (graph, column) pair keeps its own
+// alpha/beta/rz in FP64 and a convergence mask that freezes it (alpha = 0)
+// once its stop rule holds.  The projection removes the mean of every
+// connected component (the reference removes one global mean, the same
+// thing on a connected graph).  Every reduction is segmented per
+// (graph, column) over row tiles that never straddle two graphs, and summed
+// in a fixed order, so a graph's results do not depend on what else is in
+// the batch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
+#include <numeric>
 
 #include "device.h"
 
 namespace ss {
 
-static const int kSpinLimit = 1 << 22;   // polls per wait (~seconds)
+static const int kSpinLimit = 1 << 22;   // polls per look-back wait (~seconds)
 
 // ---------------------------------------------------------------------------
 // per-kernel timing
@@ -56,8 +65,11 @@ int prof_resolve(ss_graph *g) {
 }
 
 // ---------------------------------------------------------------------------
-// structure upload
+// host structure helpers
 // ---------------------------------------------------------------------------
+// Incidence lists of B (m x n, one row per edge): per vertex, the edges that
+// touch it with B's entry.  B[v1,e] = sqrt(w) * -1, B[v2,e] = -1 * that
+// (src/sparsifier.rs:97-110, the edge's smaller endpoint is v1).
 void build_incidence(int64_t n, int64_t m, const int32_t *v1, const int32_t *v2, const double *w,
                      std::vector<int32_t> &iptr, std::vector<int32_t> &ieid, std::vector<double> &ib) {
     iptr.assign((size_t)n + 1, 0);
@@ -70,15 +82,57 @@ void build_incidence(int64_t n, int64_t m, const int32_t *v1, const int32_t *v2,
     ib.resize((size_t)2 * m);
     std::vector<int32_t> pos(iptr.begin(), iptr.end() - 1);
     for (int64_t e = 0; e < m; e++) {
-        // B[v1,e] = sqrt(w) * -1, B[v2,e] = -1 * (sqrt(w) * -1)
-        // (src/sparsifier.rs:97-110)
         const double nv = std::sqrt(-1.0 * (-w[e])) * -1.0;
-        int32_t p1 = pos[v1[e]]++;
+        const int32_t p1 = pos[v1[e]]++;
         ieid[p1] = (int32_t)e;
         ib[p1] = nv;
-        int32_t p2 = pos[v2[e]]++;
+        const int32_t p2 = pos[v2[e]]++;
         ieid[p2] = (int32_t)e;
         ib[p2] = -1.0 * nv;
+    }
+}
+
+// Connected components of one graph (union-find over its CSR): comp[v] is
+// the component index, the largest component first (ties: the one holding
+// the smaller vertex), the others ordered by their smallest vertex.
+static void graph_components(const HostGraph &H, std::vector<int32_t> &comp, std::vector<int32_t> &sizes) {
+    const int64_t n = H.n;
+    std::vector<int32_t> parent((size_t)n);
+    std::iota(parent.begin(), parent.end(), 0);
+    auto find = [&](int32_t x) {
+        while (parent[x] != x) {
+            parent[x] = parent[parent[x]];
+            x = parent[x];
+        }
+        return x;
+    };
+    for (int64_t r = 0; r < n; r++)
+        for (int32_t q = H.rowptr[r]; q < H.rowptr[r + 1]; q++) {
+            const int32_t a = find((int32_t)r), b = find(H.col[q]);
+            if (a != b) parent[std::max(a, b)] = std::min(a, b);
+        }
+    std::vector<int32_t> root_id((size_t)n, -1);
+    std::vector<int64_t> cnt;
+    for (int64_t v = 0; v < n; v++) {
+        const int32_t r = find((int32_t)v);
+        if (root_id[r] < 0) {
+            root_id[r] = (int32_t)cnt.size();
+            cnt.push_back(0);
+        }
+        cnt[root_id[r]]++;
+    }
+    const int32_t nc = (int32_t)cnt.size();
+    int32_t mainc = 0;
+    for (int32_t c = 1; c < nc; c++)
+        if (cnt[c] > cnt[mainc]) mainc = c;
+    std::vector<int32_t> remap((size_t)nc);
+    int32_t next = 1;
+    for (int32_t c = 0; c < nc; c++) remap[c] = c == mainc ? 0 : next++;
+    comp.resize((size_t)n);
+    sizes.assign((size_t)nc, 0);
+    for (int64_t v = 0; v < n; v++) {
+        comp[v] = remap[root_id[find((int32_t)v)]];
+        sizes[comp[v]]++;
     }
 }
 
@@ -89,62 +143,238 @@ static int h2d(hipStream_t s, T *dst, const T *src, size_t count) {
     return SS_OK;
 }
 
-int graph_upload(ss_context *ctx, const HostGraph &G, ss_graph **out) {
+// ---------------------------------------------------------------------------
+// batch upload
+// ---------------------------------------------------------------------------
+int batch_upload(ss_context *ctx, const std::vector<const HostGraph *> &graphs, ss_graph **out) {
     SS_HIP(hipSetDevice(ctx->device));
-    ss_graph *g = new ss_graph();
+    const int G = (int)graphs.size();
+    if (G < 1) return fail(SS_ERR_INVALID, "a batch needs at least one graph");
+    std::unique_ptr<ss_graph> g(new ss_graph());
     g->ctx = ctx;
-    g->n = G.n;
-    g->m = G.m;
-    const int64_t n = G.n, m = G.m, nnz = G.rowptr[n];
-    auto &A = g->arena_struct;
-    int rc = SS_OK;
-    auto bail = [&](int code) { delete g; return code; };
-    if ((rc = A.alloc(&g->eu, m)) || (rc = A.alloc(&g->ev, m)) || (rc = A.alloc(&g->ew, m)) ||
-        (rc = A.alloc(&g->rowptr, n + 1)) || (rc = A.alloc(&g->col, nnz)) || (rc = A.alloc(&g->val, nnz)) ||
-        (rc = A.alloc(&g->valf, nnz)) || (rc = A.alloc(&g->iptr, n + 1)) || (rc = A.alloc(&g->ieid, 2 * m)) ||
-        (rc = A.alloc(&g->ib, 2 * m)) || (rc = A.alloc(&g->flags, n)) || (rc = A.alloc(&g->status, 2)) ||
-        (rc = A.alloc(&g->p, m)) || (rc = A.alloc(&g->dnorm, m)) || (rc = A.alloc(&g->mask, m)) ||
-        (rc = A.alloc(&g->ku, m)) || (rc = A.alloc(&g->kv, m)) || (rc = A.alloc(&g->kw, m)) ||
-        (rc = A.alloc(&g->tile_state, (m + 255) / 256 + 1)) || (rc = A.alloc(&g->ticket, 1)) ||
-        (rc = A.alloc(&g->kept, 1)))
-        return bail(rc);
-    hipStream_t s = ctx->stream;
-    // canonical edges back from the CSR's upper part
-    std::vector<int32_t> eu((size_t)m), ev((size_t)m);
-    std::vector<double> ew((size_t)m);
-    for (int64_t r = 0; r < n; r++)
-        for (int32_t q = G.rowptr[r]; q < G.rowptr[r + 1]; q++)
-            if (G.col[q] > r) {
-                int32_t e = G.eid[q];
-                eu[e] = (int32_t)r;
-                ev[e] = G.col[q];
-                ew[e] = -1.0 * G.val[q];
-            }
-    std::vector<int32_t> iptr, ieid;
-    std::vector<double> ib;
-    build_incidence(n, m, eu.data(), ev.data(), ew.data(), iptr, ieid, ib);
-    std::vector<float> valf(G.val.begin(), G.val.end());
-    if ((rc = h2d(s, g->eu, eu.data(), m)) || (rc = h2d(s, g->ev, ev.data(), m)) ||
-        (rc = h2d(s, g->ew, ew.data(), m)) || (rc = h2d(s, g->rowptr, G.rowptr.data(), n + 1)) ||
-        (rc = h2d(s, g->col, G.col.data(), nnz)) || (rc = h2d(s, g->val, G.val.data(), nnz)) ||
-        (rc = h2d(s, g->valf, valf.data(), nnz)) || (rc = h2d(s, g->iptr, iptr.data(), n + 1)) ||
-        (rc = h2d(s, g->ieid, ieid.data(), 2 * m)) || (rc = h2d(s, g->ib, ib.data(), 2 * m)))
-        return bail(rc);
-    hipError_t e1 = hipMemsetAsync(g->flags, 0, sizeof(uint32_t) * (size_t)n, s);
-    hipError_t e2 = hipMemsetAsync(g->status, 0, sizeof(int) * 2, s);
-    hipError_t e3 = hipMemsetAsync(g->tile_state, 0, sizeof(uint64_t) * (size_t)((m + 255) / 256 + 1), s);
-    hipError_t e4 = hipStreamSynchronize(s);
-    if (e1 || e2 || e3 || e4) {
-        delete g;
-        return fail(SS_ERR_HIP, "graph upload: %s", hipGetErrorString(e1 ? e1 : e2 ? e2 : e3 ? e3 : e4));
+    g->G = G;
+    g->row_off.assign((size_t)G + 1, 0);
+    g->edge_off.assign((size_t)G + 1, 0);
+    g->rtile_off.assign((size_t)G + 1, 0);
+    g->etile_off.assign((size_t)G + 1, 0);
+    for (int i = 0; i < G; i++) {
+        g->row_off[i + 1] = g->row_off[i] + graphs[i]->n;
+        g->edge_off[i + 1] = g->edge_off[i] + graphs[i]->m;
+        const int64_t rt = g->rtile_off[i] + (graphs[i]->n + kRowTile - 1) / kRowTile;
+        const int64_t et = g->etile_off[i] + (graphs[i]->m + kEdgeTile - 1) / kEdgeTile;
+        if (rt >= INT32_MAX || et >= INT32_MAX) return fail(SS_ERR_CAPACITY, "batch: too many tiles");
+        g->rtile_off[i + 1] = (int32_t)rt;
+        g->etile_off[i + 1] = (int32_t)et;
     }
-    *out = g;
+    g->N = g->row_off[G];
+    g->M = g->edge_off[G];
+    if (g->N + 2 * g->M >= (int64_t)INT32_MAX)
+        return fail(SS_ERR_CAPACITY, "batch: n + 2m = %lld exceeds int32 indexing", (long long)(g->N + 2 * g->M));
+    g->nrt = g->rtile_off[G];
+    g->net = g->etile_off[G];
+    const int64_t N = g->N, M = g->M, nnz = N + 2 * M;
+
+    // tiles
+    std::vector<int32_t> rt_start((size_t)g->nrt + 1), rt_graph((size_t)std::max(g->nrt, 1)),
+        et_start((size_t)g->net + 1), et_graph((size_t)std::max(g->net, 1));
+    for (int i = 0; i < G; i++) {
+        for (int32_t t = g->rtile_off[i]; t < g->rtile_off[i + 1]; t++) {
+            rt_start[t] = (int32_t)(g->row_off[i] + (int64_t)(t - g->rtile_off[i]) * kRowTile);
+            rt_graph[t] = i;
+        }
+        for (int32_t t = g->etile_off[i]; t < g->etile_off[i + 1]; t++) {
+            et_start[t] = (int32_t)(g->edge_off[i] + (int64_t)(t - g->etile_off[i]) * kEdgeTile);
+            et_graph[t] = i;
+        }
+    }
+    rt_start[g->nrt] = (int32_t)N;
+    et_start[g->net] = (int32_t)M;
+
+    // block-diagonal Laplacian, canonical edges, incidence lists, components
+    std::vector<int32_t> rowptr((size_t)N + 1), col((size_t)nnz), iptr((size_t)N + 1), ieid((size_t)2 * M);
+    std::vector<double> val((size_t)nnz), ib((size_t)2 * M), ew((size_t)M);
+    std::vector<int32_t> eu((size_t)M), ev((size_t)M), comp_of_row((size_t)N), comp_graph, cptr(1, 0), cvert,
+        small_id, comp_off((size_t)G + 1, 0), edge_off32((size_t)G + 1);
+    g->h_comp_count.assign((size_t)G, 0);
+    for (int i = 0; i < G; i++) {
+        const HostGraph &H = *graphs[i];
+        const int64_t ro = g->row_off[i], eo = g->edge_off[i];
+        const int64_t po = ro + 2 * eo;   // first nonzero of graph i
+        for (int64_t r = 0; r <= H.n; r++) rowptr[ro + r] = (int32_t)(po + H.rowptr[r]);
+        for (size_t q = 0; q < H.col.size(); q++) {
+            col[po + q] = (int32_t)(H.col[q] + ro);
+            val[po + q] = H.val[q];
+        }
+        // canonical edges back from the CSR's upper part (local ids)
+        std::vector<int32_t> lu((size_t)H.m), lv((size_t)H.m);
+        std::vector<double> lw((size_t)H.m);
+        for (int64_t r = 0; r < H.n; r++)
+            for (int32_t q = H.rowptr[r]; q < H.rowptr[r + 1]; q++)
+                if (H.col[q] > r) {
+                    const int32_t e = H.eid[q];
+                    lu[e] = (int32_t)r;
+                    lv[e] = H.col[q];
+                    lw[e] = -1.0 * H.val[q];
+                }
+        for (int64_t e = 0; e < H.m; e++) {
+            eu[eo + e] = (int32_t)(lu[e] + ro);
+            ev[eo + e] = (int32_t)(lv[e] + ro);
+            ew[eo + e] = lw[e];
+        }
+        std::vector<int32_t> li, le;
+        std::vector<double> lb;
+        build_incidence(H.n, H.m, lu.data(), lv.data(), lw.data(), li, le, lb);
+        for (int64_t r = 0; r <= H.n; r++) iptr[ro + r] = (int32_t)(2 * eo + li[r]);
+        for (int64_t q = 0; q < 2 * H.m; q++) {
+            ieid[2 * eo + q] = le[q];   // the edge's index within its graph: the sign hash input
+            ib[2 * eo + q] = lb[q];
+        }
+        std::vector<int32_t> comp, sizes;
+        graph_components(H, comp, sizes);
+        const int32_t cbase = (int32_t)comp_graph.size();
+        comp_off[i] = cbase;
+        g->h_comp_count[i] = (int32_t)sizes.size();
+        std::vector<std::vector<int32_t>> members(sizes.size());
+        for (int64_t v = 0; v < H.n; v++) {
+            comp_of_row[ro + v] = cbase + comp[v];
+            members[comp[v]].push_back((int32_t)(ro + v));
+        }
+        for (size_t c = 0; c < sizes.size(); c++) {
+            comp_graph.push_back(i);
+            cvert.insert(cvert.end(), members[c].begin(), members[c].end());
+            cptr.push_back((int32_t)cvert.size());
+            if (c > 0) small_id.push_back(cbase + (int32_t)c);
+        }
+        edge_off32[i] = (int32_t)eo;
+    }
+    comp_off[G] = (int32_t)comp_graph.size();
+    edge_off32[G] = (int32_t)M;
+    g->ncomp = (int)comp_graph.size();
+    g->nsmall = (int)small_id.size();
+    std::vector<float> valf(val.begin(), val.end());
+
+    auto &A = g->arena_struct;
+    SS_TRY(A.alloc(&g->rt_start, rt_start.size()));
+    SS_TRY(A.alloc(&g->rt_graph, rt_graph.size()));
+    SS_TRY(A.alloc(&g->et_start, et_start.size()));
+    SS_TRY(A.alloc(&g->et_graph, et_graph.size()));
+    SS_TRY(A.alloc(&g->gtile, g->rtile_off.size()));
+    SS_TRY(A.alloc(&g->edge_off_dev, edge_off32.size()));
+    SS_TRY(A.alloc(&g->eu, M));
+    SS_TRY(A.alloc(&g->ev, M));
+    SS_TRY(A.alloc(&g->ew, M));
+    SS_TRY(A.alloc(&g->rowptr, N + 1));
+    SS_TRY(A.alloc(&g->col, nnz));
+    SS_TRY(A.alloc(&g->val, nnz));
+    SS_TRY(A.alloc(&g->valf, nnz));
+    SS_TRY(A.alloc(&g->iptr, N + 1));
+    SS_TRY(A.alloc(&g->ieid, 2 * M));
+    SS_TRY(A.alloc(&g->ib, 2 * M));
+    SS_TRY(A.alloc(&g->comp_of_row, N));
+    SS_TRY(A.alloc(&g->comp_graph, comp_graph.size()));
+    SS_TRY(A.alloc(&g->comp_off, comp_off.size()));
+    SS_TRY(A.alloc(&g->cptr, cptr.size()));
+    SS_TRY(A.alloc(&g->cvert, cvert.size()));
+    SS_TRY(A.alloc(&g->small_id, small_id.size()));
+    SS_TRY(A.alloc(&g->status, 2));
+    SS_TRY(A.alloc(&g->p, M));
+    SS_TRY(A.alloc(&g->dnorm, M));
+    SS_TRY(A.alloc(&g->mask, M));
+    SS_TRY(A.alloc(&g->ku, M));
+    SS_TRY(A.alloc(&g->kv, M));
+    SS_TRY(A.alloc(&g->kw, M));
+    SS_TRY(A.alloc(&g->tile_state, (size_t)g->net + 1));
+    SS_TRY(A.alloc(&g->tile_incl, (size_t)g->net + 1));
+    SS_TRY(A.alloc(&g->ticket, 1));
+    auto &P = g->arena_params;
+    SS_TRY(P.alloc(&g->p_seeds, G));
+    SS_TRY(P.alloc(&g->p_scales, G));
+    SS_TRY(P.alloc(&g->p_betas, G));
+    SS_TRY(P.alloc(&g->p_keys, (size_t)8 * G));
+    hipStream_t s = ctx->stream;
+    SS_TRY(h2d(s, g->rt_start, rt_start.data(), rt_start.size()));
+    SS_TRY(h2d(s, g->rt_graph, rt_graph.data(), rt_graph.size()));
+    SS_TRY(h2d(s, g->et_start, et_start.data(), et_start.size()));
+    SS_TRY(h2d(s, g->et_graph, et_graph.data(), et_graph.size()));
+    SS_TRY(h2d(s, g->gtile, g->rtile_off.data(), g->rtile_off.size()));
+    SS_TRY(h2d(s, g->edge_off_dev, edge_off32.data(), edge_off32.size()));
+    SS_TRY(h2d(s, g->eu, eu.data(), M));
+    SS_TRY(h2d(s, g->ev, ev.data(), M));
+    SS_TRY(h2d(s, g->ew, ew.data(), M));
+    SS_TRY(h2d(s, g->rowptr, rowptr.data(), N + 1));
+    SS_TRY(h2d(s, g->col, col.data(), nnz));
+    SS_TRY(h2d(s, g->val, val.data(), nnz));
+    SS_TRY(h2d(s, g->valf, valf.data(), nnz));
+    SS_TRY(h2d(s, g->iptr, iptr.data(), N + 1));
+    SS_TRY(h2d(s, g->ieid, ieid.data(), 2 * M));
+    SS_TRY(h2d(s, g->ib, ib.data(), 2 * M));
+    SS_TRY(h2d(s, g->comp_of_row, comp_of_row.data(), N));
+    SS_TRY(h2d(s, g->comp_graph, comp_graph.data(), comp_graph.size()));
+    SS_TRY(h2d(s, g->comp_off, comp_off.data(), comp_off.size()));
+    SS_TRY(h2d(s, g->cptr, cptr.data(), cptr.size()));
+    SS_TRY(h2d(s, g->cvert, cvert.data(), cvert.size()));
+    SS_TRY(h2d(s, g->small_id, small_id.data(), small_id.size()));
+    SS_HIP(hipMemsetAsync(g->status, 0, sizeof(int) * 2, s));
+    SS_HIP(hipMemsetAsync(g->tile_state, 0, sizeof(uint64_t) * ((size_t)g->net + 1), s));
+    SS_HIP(hipStreamSynchronize(s));
+    g->kept_start.assign((size_t)G, 0);
+    g->kept_count.assign((size_t)G, 0);
+    *out = g.release();
     return SS_OK;
 }
 
-int graph_upload_factor(ss_graph *g, const Factor &F) {
-    if (F.n != g->n) return fail(SS_ERR_INVALID, "factor has n=%lld, graph has n=%lld", (long long)F.n, (long long)g->n);
+// One factor of the block-diagonal system from the graphs' own factors:
+// positions of graph i shift by its row offset; levels stay, since no entry
+// couples two graphs.
+static void factor_concat(const std::vector<const Factor *> &fs, const std::vector<int64_t> &row_off, Factor &F) {
+    F = Factor();
+    F.n = row_off.back();
+    F.order.resize((size_t)F.n);
+    F.diag.resize((size_t)F.n);
+    F.lvl_fwd.resize((size_t)F.n);
+    F.lvl_bwd.resize((size_t)F.n);
+    F.urowptr.assign((size_t)F.n + 1, 0);
+    int64_t nnz = 0;
+    for (const Factor *f : fs) nnz += f->nnz;
+    F.ucol.resize((size_t)nnz);
+    F.uval.resize((size_t)nnz);
+    int64_t q = 0;
+    for (size_t i = 0; i < fs.size(); i++) {
+        const Factor &f = *fs[i];
+        const int64_t o = row_off[i];
+        for (int64_t p = 0; p < f.n; p++) {
+            F.order[o + p] = (int32_t)(f.order[p] + o);
+            F.diag[o + p] = f.diag[p];
+            F.lvl_fwd[o + p] = f.lvl_fwd[p];
+            F.lvl_bwd[o + p] = f.lvl_bwd[p];
+            F.urowptr[o + p] = (int32_t)q;
+            for (int32_t t = f.urowptr[p]; t < f.urowptr[p + 1]; t++) {
+                F.ucol[q] = (int32_t)(f.ucol[t] + o);
+                F.uval[q] = f.uval[t];
+                q++;
+            }
+        }
+        F.depth_fwd = std::max(F.depth_fwd, f.depth_fwd);
+        F.depth_bwd = std::max(F.depth_bwd, f.depth_bwd);
+    }
+    F.urowptr[F.n] = (int32_t)q;
+    F.nnz = q;
+}
+
+int batch_upload_factors(ss_graph *g, const std::vector<const Factor *> &factors) {
+    if ((int)factors.size() != g->G) return fail(SS_ERR_INVALID, "need %d factors, got %zu", g->G, factors.size());
+    for (int i = 0; i < g->G; i++)
+        if (!factors[i] || factors[i]->n != g->row_off[i + 1] - g->row_off[i])
+            return fail(SS_ERR_INVALID, "factor %d does not match graph %d (n=%lld)", i, i,
+                        (long long)(g->row_off[i + 1] - g->row_off[i]));
     SS_HIP(hipSetDevice(g->ctx->device));
+    Factor Fc;
+    const Factor *Fp = factors[0];
+    if (g->G > 1) {
+        factor_concat(factors, g->row_off, Fc);
+        Fp = &Fc;
+    }
+    const Factor &F = *Fp;
     Schedule fw, bw;
     SS_TRY(factor_schedules(F, fw, bw, kLongRow));
     // D per vertex (the factor keeps it per elimination position)
@@ -153,39 +383,46 @@ int graph_upload_factor(ss_graph *g, const Factor &F) {
     g->arena_factor.release();
     g->has_factor = false;
     auto &A = g->arena_factor;
-    const int64_t n = g->n;
-    g->f_nnz = (int64_t)fw.col.size();
-    g->b_nnz = (int64_t)bw.col.size();
+    const int64_t n = g->N;
+    const int64_t fnz = (int64_t)fw.col.size(), bnz = (int64_t)bw.col.size();
     SS_TRY(A.alloc(&g->f_rows, n));
     SS_TRY(A.alloc(&g->f_ptr, n + 1));
-    SS_TRY(A.alloc(&g->f_col, g->f_nnz));
-    SS_TRY(A.alloc(&g->f_val, g->f_nnz));
-    SS_TRY(A.alloc(&g->f_valf, g->f_nnz));
+    SS_TRY(A.alloc(&g->f_col, fnz));
+    SS_TRY(A.alloc(&g->f_val, fnz));
+    SS_TRY(A.alloc(&g->f_valf, fnz));
     SS_TRY(A.alloc(&g->b_rows, n));
     SS_TRY(A.alloc(&g->b_ptr, n + 1));
-    SS_TRY(A.alloc(&g->b_col, g->b_nnz));
-    SS_TRY(A.alloc(&g->b_val, g->b_nnz));
-    SS_TRY(A.alloc(&g->b_valf, g->b_nnz));
+    SS_TRY(A.alloc(&g->b_col, bnz));
+    SS_TRY(A.alloc(&g->b_val, bnz));
+    SS_TRY(A.alloc(&g->b_valf, bnz));
     SS_TRY(A.alloc(&g->diag, n));
-    SS_TRY(A.alloc(&g->f_lp_dev, fw.level_ptr.size()));
-    SS_TRY(A.alloc(&g->b_lp_dev, bw.level_ptr.size()));
     SS_TRY(A.alloc(&g->f_split, n));
+    SS_TRY(A.alloc(&g->f_unit_ptr, fw.unit_ptr.size()));
+    SS_TRY(A.alloc(&g->f_unit_t, fw.unit_t.size()));
+    SS_TRY(A.alloc(&g->f_unit_s, fw.unit_s.size()));
+    SS_TRY(A.alloc(&g->b_unit_ptr, bw.unit_ptr.size()));
+    SS_TRY(A.alloc(&g->b_unit_t, bw.unit_t.size()));
+    SS_TRY(A.alloc(&g->b_unit_s, bw.unit_s.size()));
     hipStream_t s = g->ctx->stream;
     std::vector<float> fvf(fw.val.begin(), fw.val.end()), bvf(bw.val.begin(), bw.val.end());
     SS_TRY(h2d(s, g->f_rows, fw.rows.data(), n));
     SS_TRY(h2d(s, g->f_ptr, fw.ptr.data(), n + 1));
-    SS_TRY(h2d(s, g->f_col, fw.col.data(), g->f_nnz));
-    SS_TRY(h2d(s, g->f_val, fw.val.data(), g->f_nnz));
-    SS_TRY(h2d(s, g->f_valf, fvf.data(), g->f_nnz));
+    SS_TRY(h2d(s, g->f_col, fw.col.data(), fnz));
+    SS_TRY(h2d(s, g->f_val, fw.val.data(), fnz));
+    SS_TRY(h2d(s, g->f_valf, fvf.data(), fnz));
     SS_TRY(h2d(s, g->b_rows, bw.rows.data(), n));
     SS_TRY(h2d(s, g->b_ptr, bw.ptr.data(), n + 1));
-    SS_TRY(h2d(s, g->b_col, bw.col.data(), g->b_nnz));
-    SS_TRY(h2d(s, g->b_val, bw.val.data(), g->b_nnz));
-    SS_TRY(h2d(s, g->b_valf, bvf.data(), g->b_nnz));
+    SS_TRY(h2d(s, g->b_col, bw.col.data(), bnz));
+    SS_TRY(h2d(s, g->b_val, bw.val.data(), bnz));
+    SS_TRY(h2d(s, g->b_valf, bvf.data(), bnz));
     SS_TRY(h2d(s, g->diag, vdiag.data(), n));
-    SS_TRY(h2d(s, g->f_lp_dev, fw.level_ptr.data(), fw.level_ptr.size()));
-    SS_TRY(h2d(s, g->b_lp_dev, bw.level_ptr.data(), bw.level_ptr.size()));
     SS_TRY(h2d(s, g->f_split, fw.split.data(), n));
+    SS_TRY(h2d(s, g->f_unit_ptr, fw.unit_ptr.data(), fw.unit_ptr.size()));
+    SS_TRY(h2d(s, g->f_unit_t, fw.unit_t.data(), fw.unit_t.size()));
+    SS_TRY(h2d(s, g->f_unit_s, fw.unit_s.data(), fw.unit_s.size()));
+    SS_TRY(h2d(s, g->b_unit_ptr, bw.unit_ptr.data(), bw.unit_ptr.size()));
+    SS_TRY(h2d(s, g->b_unit_t, bw.unit_t.data(), bw.unit_t.size()));
+    SS_TRY(h2d(s, g->b_unit_s, bw.unit_s.data(), bw.unit_s.size()));
     SS_HIP(hipStreamSynchronize(s));
     g->f_level_ptr = fw.level_ptr;
     g->b_level_ptr = bw.level_ptr;
@@ -230,44 +467,42 @@ int graph_ensure_work(ss_graph *g, int k, int prec) {
     g->w64 = PcgWork<double>();
     g->w32 = PcgWork<float>();
     auto &A = g->arena_work;
-    const size_t nk = (size_t)g->n * (size_t)k;
+    const size_t nk = (size_t)g->N * (size_t)k, gk = (size_t)g->G * k;
     SS_TRY(A.alloc(&g->Y64, nk));
     SS_TRY(A.alloc(&g->X64, nk));
-    if (prec == SS_PREC_FP64) {
-        auto &w = g->w64;
+    auto alloc_work = [&](auto &w) -> int {
         SS_TRY(A.alloc(&w.B, nk));
         SS_TRY(A.alloc(&w.R, nk));
         SS_TRY(A.alloc(&w.P, nk));
-        SS_TRY(A.alloc(&w.P2, nk));
         SS_TRY(A.alloc(&w.Q, nk));
         SS_TRY(A.alloc(&w.Zp, nk));
         SS_TRY(A.alloc(&w.W, nk));
+        return SS_OK;
+    };
+    if (prec == SS_PREC_FP64) {
+        SS_TRY(alloc_work(g->w64));
     } else {
         SS_TRY(A.alloc(&g->B64, nk));
         SS_TRY(A.alloc(&g->R64, nk));
         SS_TRY(A.alloc(&g->Y32, nk));
         SS_TRY(A.alloc(&g->X32, nk));
-        auto &w = g->w32;
-        SS_TRY(A.alloc(&w.B, nk));
-        SS_TRY(A.alloc(&w.R, nk));
-        SS_TRY(A.alloc(&w.P, nk));
-        SS_TRY(A.alloc(&w.P2, nk));
-        SS_TRY(A.alloc(&w.Q, nk));
-        SS_TRY(A.alloc(&w.Zp, nk));
-        SS_TRY(A.alloc(&w.W, nk));
+        SS_TRY(alloc_work(g->w32));
     }
-    SS_TRY(A.alloc(&g->partial, (size_t)kMaxPartialBlocks * k));
-    SS_TRY(A.alloc(&g->partial2, (size_t)kMaxPartialBlocks * k));
-    SS_TRY(A.alloc(&g->colbuf, (size_t)4 * k));
-    SS_TRY(A.alloc(&g->sc_buf, (size_t)5 * k));
-    SS_TRY(A.alloc(&g->sc_int, (size_t)2 * k));
+    SS_TRY(A.alloc(&g->partial, (size_t)std::max(g->nrt, 1) * k));
+    SS_TRY(A.alloc(&g->partial2, (size_t)std::max(g->nrt, 1) * k));
+    SS_TRY(A.alloc(&g->gsum, gk));
+    SS_TRY(A.alloc(&g->csum, (size_t)g->ncomp * k));
+    SS_TRY(A.alloc(&g->cmean, (size_t)g->ncomp * k));
+    SS_TRY(A.alloc(&g->sc_buf, 5 * gk));
+    SS_TRY(A.alloc(&g->sc_int, 2 * gk));
+    SS_TRY(A.alloc(&g->colkeep, gk));
     g->sc.rz = g->sc_buf;
-    g->sc.alpha = g->sc_buf + k;
-    g->sc.beta = g->sc_buf + 2 * k;
-    g->sc.rr = g->sc_buf + 3 * k;
-    g->sc.bnorm = g->sc_buf + 4 * k;
+    g->sc.alpha = g->sc_buf + gk;
+    g->sc.beta = g->sc_buf + 2 * gk;
+    g->sc.rr = g->sc_buf + 3 * gk;
+    g->sc.bnorm = g->sc_buf + 4 * gk;
     g->sc.active = g->sc_int;
-    g->sc.iters = g->sc_int + k;
+    g->sc.iters = g->sc_int + gk;
     g->wk = k;
     g->wprec = prec;
     return SS_OK;
@@ -286,16 +521,16 @@ template <typename T> static const T *avals(const ss_graph *g);
 template <> const double *avals<double>(const ss_graph *g) { return g->val; }
 template <> const float *avals<float>(const ss_graph *g) { return g->valf; }
 
-// Zp = L^{-T} D^+ L^{-1} R   (W holds the forward result), one launch per
-// level.  Also leaves per-block partials of r . Zp = y^T D^+ y in
-// rz_partial (nb_rz blocks).
+static Tiles rtiles(const ss_graph *g) { return Tiles{g->rt_start, g->rt_graph, g->nrt}; }
+
+// Zp = L^{-T} D^+ L^{-1} R (W holds the forward result y): one launch per
+// level plus the single-workgroup small-level runs.  rz_partial receives the
+// per-tile partials of r . Zp = y^T D^+ y.
 template <typename T>
-static int precond(ss_graph *g, int k, const T *R, T *W, T *Zp, double *rz_partial, int *nb_rz) {
+static int precond(ss_graph *g, int k, const T *R, T *W, T *Zp, double *rz_partial) {
     hipStream_t s = g->ctx->stream;
-    const int32_t n = (int32_t)g->n;
     {
-        // the level launches as one timed group; the kernel's average is
-        // the group time over its launch count
+        // the level launches are timed as one group over their count
         ProfScope ps(g, SS_KERNEL_TRI_FWD, g->f_small_lo);
         for (int32_t l = 0; l < g->f_small_lo; l++)
             launch_trisolve_level<T, false>(s, g->f_level_ptr[l], g->f_level_long[l], g->f_level_ptr[l + 1],
@@ -304,25 +539,25 @@ static int precond(ss_graph *g, int k, const T *R, T *W, T *Zp, double *rz_parti
     {
         ProfScope ps(g, SS_KERNEL_TRI_FWD_SMALL, g->f_small_lo < g->f_small_hi ? 2 : 0);
         if (g->f_small_lo < g->f_small_hi) {
-            launch_tri_tail_head<T>(s, g->f_level_ptr[g->f_small_lo], (int32_t)g->n, g->f_rows, g->f_ptr, g->f_split,
-                                    g->f_col, fvals<T>(g), k, R, W);
-            launch_tri_small<T, false>(s, g->f_small_lo, g->f_small_hi, g->f_lp_dev, g->f_rows, g->f_ptr, g->f_split,
-                                       g->f_col, fvals<T>(g), g->diag, k, R, W);
+            launch_tri_tail_head<T>(s, g->f_level_ptr[g->f_small_lo], (int32_t)g->N, g->f_rows, g->f_ptr,
+                                    g->f_split, g->f_col, fvals<T>(g), k, R, W);
+            launch_tri_small<T, false>(s, g->f_small_lo, g->f_small_hi, g->f_unit_ptr, g->f_unit_t, g->f_unit_s,
+                                       g->f_rows, g->f_ptr, g->f_split, g->f_col, fvals<T>(g), g->diag, k, R, W);
         }
     }
     {
         ProfScope ps(g, SS_KERNEL_VECTOR);
-        launch_ydy_partial<T>(s, n, k, W, g->diag, rz_partial);
+        bt_tile_partial<T, T>(s, rtiles(g), k, W, W, g->diag, 2, rz_partial);
     }
-    *nb_rz = partial_blocks(n);
     if (g->b_small_lo < g->b_small_hi) {
         ProfScope ps(g, SS_KERNEL_TRI_BWD_SMALL);
-        launch_tri_small<T, true>(s, g->b_small_lo, g->b_small_hi, g->b_lp_dev, g->b_rows, g->b_ptr, nullptr,
-                                  g->b_col, bvals<T>(g), g->diag, k, W, Zp);
+        launch_tri_small<T, true>(s, g->b_small_lo, g->b_small_hi, g->b_unit_ptr, g->b_unit_t, g->b_unit_s,
+                                  g->b_rows, g->b_ptr, nullptr, g->b_col, bvals<T>(g), g->diag, k, W, Zp);
     }
     {
-        ProfScope ps(g, SS_KERNEL_TRI_BWD, (int)g->b_level_ptr.size() - 1 - g->b_small_hi);
-        for (int32_t l = g->b_small_hi; l + 1 < (int32_t)g->b_level_ptr.size(); l++)
+        const int32_t nlev = (int32_t)g->b_level_ptr.size() - 1;
+        ProfScope ps(g, SS_KERNEL_TRI_BWD, nlev - g->b_small_hi);
+        for (int32_t l = g->b_small_hi; l < nlev; l++)
             launch_trisolve_level<T, true>(s, g->b_level_ptr[l], g->b_level_long[l], g->b_level_ptr[l + 1],
                                            g->b_rows, g->b_ptr, g->b_col, bvals<T>(g), g->diag, k, W, Zp);
     }
@@ -337,169 +572,205 @@ static int read_status(ss_graph *g, int *nactive) {
     if (c->h_status[1] != 0) {
         (void)hipMemsetAsync(g->status, 0, 2 * sizeof(int), c->stream);
         (void)hipStreamSynchronize(c->stream);
-        return fail(SS_ERR_TIMEOUT, "device wait exceeded its spin limit (triangular solve dependency)");
+        return fail(SS_ERR_TIMEOUT, "device wait exceeded its spin limit (compaction look-back)");
     }
     if (nactive) *nactive = c->h_status[0];
     return SS_OK;
 }
 
-// PCG on all k columns: X = approx A^+ (rhs - mean).  Work in precision T,
-// dots and scalars in FP64.  Writes per-column iteration counts to iters.
-template <typename T>
-static int pcg_run(ss_graph *g, int k, const T *rhs, T *X, const PcgWork<T> &w, int rule, double tol,
-                   int max_iter, std::vector<int32_t> &iters, std::vector<double> &bnorm) {
+// Per-component means of the FP64 block Y into g->cmean.
+static void component_means(ss_graph *g, int k, const double *Y) {
     hipStream_t s = g->ctx->stream;
-    const int32_t n = (int32_t)g->n;
-    const int64_t nk = (int64_t)n * k;
-    const int nb = partial_blocks(n);
-    // b - mean(b)   (include/custom_cg.hpp:80-93, :131-132)
-    launch_col_partial<T, T>(s, n, k, rhs, rhs, 1, g->partial);
-    launch_col_mean(s, k, g->partial, nb, n, g->colbuf);
-    launch_project<T>(s, nk, k, rhs, g->colbuf, w.R, w.B, X);
-    int nb_rz = 0;
-    SS_TRY(precond<T>(g, k, w.R, w.W, w.Zp, g->partial2, &nb_rz));
-    launch_col_partial<T, T>(s, n, k, w.B, w.B, 0, g->partial);
-    launch_pcg_init(s, k, g->partial, nb, g->partial2, nb_rz, g->sc);     // beta = 0: P starts as Z
+    bt_tile_partial<double, double>(s, rtiles(g), k, Y, Y, nullptr, 1, g->partial);
+    bt_graph_sums(s, g->G * k, k, g->gtile, g->partial, g->gsum);
+    bt_small_comp_sums(s, g->nsmall, g->small_id, g->cptr, g->cvert, k, Y, g->csum);
+    bt_comp_means(s, g->ncomp, k, g->comp_graph, g->comp_off, g->cptr, g->gsum, g->csum, g->cmean);
+}
+
+// PCG on all k columns of all graphs: X ~ A^+ (rhs - component means).
+// Vectors in precision T, dots and scalars in FP64.  With mean_src the
+// component means of that FP64 block are removed (rhs itself in the FP64
+// solve); without it rhs is used as given.
+template <typename T>
+static int pcg_run(ss_graph *g, int k, const T *rhs, const double *mean_src, T *X, const PcgWork<T> &w, int rule,
+                   double tol, int max_iter, std::vector<int32_t> &iters) {
+    hipStream_t s = g->ctx->stream;
+    const Tiles rt = rtiles(g);
+    const int gk = g->G * k;
+    {
+        ProfScope ps(g, SS_KERNEL_VECTOR);
+        if (mean_src)
+            component_means(g, k, mean_src);
+        else
+            launch_fill_u32(s, (int64_t)2 * g->ncomp * k, reinterpret_cast<uint32_t *>(g->cmean), 0u);
+        bt_project<T, T>(s, rt, k, rhs, g->comp_of_row, g->cmean, w.R, w.B, X);
+    }
+    SS_TRY(precond<T>(g, k, w.R, w.W, w.Zp, g->partial2));
+    bt_tile_partial<T, T>(s, rt, k, w.B, w.B, nullptr, 0, g->partial);
+    bt_pcg_init(s, gk, k, g->gtile, g->partial, g->partial2, g->sc);   // beta = 0: P starts as Z
     for (int it = 0; it < max_iter; it++) {
         {
             ProfScope ps(g, SS_KERNEL_VECTOR);
-            launch_update_p<T>(s, nk, k, g->sc.beta, w.Zp, w.P);      // P = Z + beta P (beta = 0: P = Z)
+            bt_update_p<T>(s, rt, k, g->sc.beta, w.Zp, w.P);   // P = Z + beta P
         }
         {
             ProfScope ps(g, SS_KERNEL_SPMM);
-            launch_spmm_dot<T>(s, n, g->rowptr, g->col, avals<T>(g), k, w.P, w.Q, g->partial);   // Q = A P
+            bt_spmm_dot<T>(s, rt, g->rowptr, g->col, avals<T>(g), k, w.P, w.Q, g->partial);   // Q = A P, P.Q
         }
-        launch_pcg_alpha(s, k, g->partial, nb, g->sc, g->status);
+        bt_pcg_alpha(s, gk, k, g->gtile, g->partial, g->sc, g->status);
         {
             ProfScope ps(g, SS_KERNEL_VECTOR);
-            launch_update_xr_rr<T>(s, n, k, g->sc.alpha, w.P, w.Q, X, w.R, g->partial);
+            bt_update_xr_rr<T>(s, rt, k, g->sc.alpha, w.P, w.Q, X, w.R, g->partial);
         }
-        launch_pcg_check(s, k, g->partial, nb, rule, tol, g->sc, g->status);
+        bt_pcg_check(s, gk, k, g->gtile, g->partial, rule, tol, g->sc, g->status);
         int nactive = 0;
         SS_TRY(read_status(g, &nactive));
         if (nactive == 0) break;
-        SS_TRY(precond<T>(g, k, w.R, w.W, w.Zp, g->partial2, &nb_rz));
-        launch_pcg_beta(s, k, g->partial2, nb_rz, g->sc);
+        SS_TRY(precond<T>(g, k, w.R, w.W, w.Zp, g->partial2));
+        bt_pcg_beta(s, gk, k, g->gtile, g->partial2, g->sc);
     }
     SS_TRY(read_status(g, nullptr));
-    iters.resize(k);
-    bnorm.resize(k);
-    SS_HIP(hipMemcpyAsync(iters.data(), g->sc.iters, sizeof(int32_t) * k, hipMemcpyDeviceToHost, s));
-    SS_HIP(hipMemcpyAsync(bnorm.data(), g->sc.bnorm, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+    iters.resize(gk);
+    SS_HIP(hipMemcpyAsync(iters.data(), g->sc.iters, sizeof(int32_t) * gk, hipMemcpyDeviceToHost, s));
     SS_HIP(hipStreamSynchronize(s));
     return SS_OK;
 }
 
-// per-column ||M||^2 of an n x k FP64 block, to the host
-static int col_norms2(ss_graph *g, int k, const double *M, std::vector<double> &out) {
+// |v|^2 per (graph, column) of an FP64 block, on the host.
+static int block_norms2(ss_graph *g, int k, const double *V, std::vector<double> &out) {
     hipStream_t s = g->ctx->stream;
-    const int32_t n = (int32_t)g->n;
-    const int nb = partial_blocks(n);
-    launch_col_partial<double, double>(s, n, k, M, M, 0, g->partial);
-    launch_col_mean(s, k, g->partial, nb, 1, g->colbuf);
-    out.resize(k);
-    SS_HIP(hipMemcpyAsync(out.data(), g->colbuf, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+    const int gk = g->G * k;
+    bt_tile_partial<double, double>(s, rtiles(g), k, V, V, nullptr, 0, g->partial);
+    bt_graph_sums(s, gk, k, g->gtile, g->partial, g->gsum);
+    out.resize(gk);
+    SS_HIP(hipMemcpyAsync(out.data(), g->gsum, sizeof(double) * gk, hipMemcpyDeviceToHost, s));
+    SS_HIP(hipStreamSynchronize(s));
+    return SS_OK;
+}
+
+// FP64 true residual R = B - A X and its |R|^2 per (graph, column).
+static int residual_norms(ss_graph *g, int k, const double *X, const double *B, double *R, std::vector<double> &rn2) {
+    hipStream_t s = g->ctx->stream;
+    const int gk = g->G * k;
+    {
+        ProfScope ps(g, SS_KERNEL_SPMM);
+        bt_residual64(s, rtiles(g), g->rowptr, g->col, g->val, k, X, B, R, g->partial);
+    }
+    bt_graph_sums(s, gk, k, g->gtile, g->partial, g->gsum);
+    rn2.resize(gk);
+    SS_HIP(hipMemcpyAsync(rn2.data(), g->gsum, sizeof(double) * gk, hipMemcpyDeviceToHost, s));
     SS_HIP(hipStreamSynchronize(s));
     return SS_OK;
 }
 
 int graph_solve(ss_graph *g, int k, const ss_solve_opts &o, SolveStats &st) {
-    if (!g->has_factor) return fail(SS_ERR_INVALID, "graph has no factor (ss_graph_set_factor)");
+    if (!g->has_factor) return fail(SS_ERR_INVALID, "no factor uploaded");
     if (g->wk != k) return fail(SS_ERR_INVALID, "work not allocated for k=%d", k);
     if (o.max_iter < 0 || !(o.tol >= 0.0)) return fail(SS_ERR_INVALID, "bad solve options");
     SS_HIP(hipSetDevice(g->ctx->device));
     hipStream_t s = g->ctx->stream;
-    const int32_t n = (int32_t)g->n;
-    const int64_t nk = (int64_t)n * k;
-    std::vector<double> bnorm, rn2;
-    st.iters.assign(k, 0);
-    st.relres.assign(k, 0.0);
+    const int gk = g->G * k;
+    std::vector<double> bn2, rn2;
+    st.iters.assign(gk, 0);
+    st.relres.assign(gk, 0.0);
     st.refinements = 0;
     if (g->wprec == SS_PREC_FP64) {
-        SS_TRY(pcg_run<double>(g, k, g->Y64, g->X64, g->w64, o.stop_rule, o.tol, o.max_iter, st.iters, bnorm));
-        // true residual b - A x  (include/custom_cg.hpp:245-258)
-        {
-            ProfScope ps(g, SS_KERNEL_SPMM);
-            launch_residual64(s, n, g->rowptr, g->col, g->val, k, g->X64, g->w64.B, g->w64.Q);
-        }
-        SS_TRY(col_norms2(g, k, g->w64.Q, rn2));
+        SS_TRY(pcg_run<double>(g, k, g->Y64, g->Y64, g->X64, g->w64, o.stop_rule, o.tol, o.max_iter, st.iters));
+        SS_TRY(block_norms2(g, k, g->w64.B, bn2));
+        // true residual b - A x (include/custom_cg.hpp:245-258)
+        SS_TRY(residual_norms(g, k, g->X64, g->w64.B, g->w64.Q, rn2));
     } else {
         // FP64 outer refinement around an FP32 PCG.
-        const int nb = partial_blocks(n);
-        launch_col_partial<double, double>(s, n, k, g->Y64, g->Y64, 1, g->partial);
-        launch_col_mean(s, k, g->partial, nb, n, g->colbuf);
-        launch_project<double>(s, nk, k, g->Y64, g->colbuf, g->R64, g->B64, g->X64);
-        std::vector<double> bn2;
-        SS_TRY(col_norms2(g, k, g->B64, bn2));
-        bnorm.resize(k);
-        for (int j = 0; j < k; j++) bnorm[j] = std::sqrt(bn2[j]);
-        // Each FP32 sweep reduces the FP64 residual by ~1e-3 (well above the
-        // FP32 stagnation level); sweeps repeat until the FP64 stop rule holds.
+        const Tiles rt = rtiles(g);
+        component_means(g, k, g->Y64);
+        bt_project<double, double>(s, rt, k, g->Y64, g->comp_of_row, g->cmean, g->R64, g->B64, g->X64);
+        SS_TRY(block_norms2(g, k, g->B64, bn2));
+        // Each FP32 sweep cuts the FP64 residual by ~1e-3 (well above FP32's
+        // stagnation level); sweeps repeat until the FP64 stop rule holds.
         const double inner_tol = std::max(1e-3, o.stop_rule == SS_STOP_REL ? o.tol : 0.0);
         const int inner_max = std::min(o.max_iter, 300);
-        std::vector<int32_t> it_sweep;
-        std::vector<double> bn_inner;
+        std::vector<int32_t> it_sweep, keep(gk);
         rn2 = bn2;
         for (int sweep = 0; sweep < 16; sweep++) {
             bool all_done = true;
-            for (int j = 0; j < k; j++) {
-                double r = std::sqrt(rn2[j]);
-                bool done = o.stop_rule == SS_STOP_REL ? r <= o.tol * bnorm[j] : rn2[j] <= o.tol;
+            for (int i = 0; i < gk; i++) {
+                const double r = std::sqrt(rn2[i]);
+                const bool done = rn2[i] == 0.0 ||
+                                  (o.stop_rule == SS_STOP_REL ? r <= o.tol * std::sqrt(bn2[i]) : rn2[i] <= o.tol);
+                keep[i] = done ? 0 : 1;
                 if (!done) all_done = false;
             }
             if (all_done) break;
-            launch_convert<double, float>(s, nk, g->R64, g->Y32);
-            SS_TRY(pcg_run<float>(g, k, g->Y32, g->X32, g->w32, SS_STOP_REL, inner_tol, inner_max, it_sweep,
-                                  bn_inner));
-            for (int j = 0; j < k; j++) st.iters[j] += it_sweep[j];
-            launch_accumulate<float>(s, nk, g->X32, g->X64);
-            {
-                ProfScope ps(g, SS_KERNEL_SPMM);
-                launch_residual64(s, n, g->rowptr, g->col, g->val, k, g->X64, g->B64, g->R64);
-            }
-            SS_TRY(col_norms2(g, k, g->R64, rn2));
+            SS_HIP(hipMemcpyAsync(g->colkeep, keep.data(), sizeof(int) * gk, hipMemcpyHostToDevice, s));
+            bt_convert_masked(s, rt, k, g->R64, g->colkeep, g->Y32);
+            SS_TRY(pcg_run<float>(g, k, g->Y32, nullptr, g->X32, g->w32, SS_STOP_REL, inner_tol, inner_max,
+                                  it_sweep));
+            for (int i = 0; i < gk; i++) st.iters[i] += it_sweep[i];
+            bt_accumulate(s, rt, k, g->X32, g->X64);
+            SS_TRY(residual_norms(g, k, g->X64, g->B64, g->R64, rn2));
             st.refinements++;
         }
     }
-    st.max_iters = 0;
-    st.max_relres = 0.0;
-    for (int j = 0; j < k; j++) {
-        st.relres[j] = bnorm[j] > 0.0 ? std::sqrt(rn2[j]) / bnorm[j] : 0.0;
-        st.max_iters = std::max(st.max_iters, st.iters[j]);
-        st.max_relres = std::max(st.max_relres, st.relres[j]);
-    }
+    for (int i = 0; i < gk; i++) st.relres[i] = bn2[i] > 0.0 ? std::sqrt(rn2[i] / bn2[i]) : 0.0;
     return SS_OK;
 }
 
-int graph_sketch(ss_graph *g, int k, uint64_t seed, double scale, int uniform, int prec) {
+int graph_set_params(ss_graph *g, const std::vector<GraphParams> &prm) {
+    if ((int)prm.size() != g->G) return fail(SS_ERR_INVALID, "need %d parameter sets, got %zu", g->G, prm.size());
+    std::vector<uint64_t> seeds(g->G);
+    std::vector<double> scales(g->G), betas(g->G);
+    std::vector<uint32_t> keys((size_t)8 * g->G);
+    for (int i = 0; i < g->G; i++) {
+        seeds[i] = prm[i].sketch_seed;
+        scales[i] = prm[i].scale;
+        betas[i] = prm[i].beta;
+        const CoinKey ck = coin_key_from_u64(prm[i].sampling_seed);
+        for (int q = 0; q < 8; q++) keys[(size_t)8 * i + q] = ck.k[q];
+    }
+    SS_HIP(hipSetDevice(g->ctx->device));
+    hipStream_t s = g->ctx->stream;
+    SS_TRY(h2d(s, g->p_seeds, seeds.data(), seeds.size()));
+    SS_TRY(h2d(s, g->p_scales, scales.data(), scales.size()));
+    SS_TRY(h2d(s, g->p_betas, betas.data(), betas.size()));
+    SS_TRY(h2d(s, g->p_keys, keys.data(), keys.size()));
+    // the host vectors go out of scope: wait for the copies
+    SS_HIP(hipStreamSynchronize(s));
+    return SS_OK;
+}
+
+int graph_sketch(ss_graph *g, int k, int uniform, int prec) {
     SS_TRY(graph_ensure_work(g, k, prec));
     SS_HIP(hipSetDevice(g->ctx->device));
     ProfScope ps(g, SS_KERNEL_SKETCH);
-    launch_sketch<double>(g->ctx->stream, (int32_t)g->n, g->iptr, g->ieid, g->ib, k, seed, scale, uniform, 0,
-                          g->Y64);
+    bt_sketch(g->ctx->stream, rtiles(g), g->iptr, g->ieid, g->ib, k, g->p_seeds, g->p_scales, uniform, g->Y64);
     SS_HIP(hipGetLastError());
     return SS_OK;
 }
 
-int graph_sample(ss_graph *g, int k, double beta, uint64_t sampling_seed, const double *p_in_dev, bool write_d) {
+int graph_sample(ss_graph *g, int k, const double *p_in_dev, bool write_d) {
     SS_HIP(hipSetDevice(g->ctx->device));
     hipStream_t s = g->ctx->stream;
+    std::fill(g->kept_start.begin(), g->kept_start.end(), 0);
+    std::fill(g->kept_count.begin(), g->kept_count.end(), 0);
+    if (g->M == 0) return SS_OK;
+    if (!p_in_dev && g->wk != k) return fail(SS_ERR_INVALID, "no solution block for k=%d", k);
     if (g->samp_epoch >= (1u << 24) - 2) {
-        SS_HIP(hipMemsetAsync(g->tile_state, 0, sizeof(uint64_t) * (size_t)((g->m + 255) / 256 + 1), s));
+        SS_HIP(hipMemsetAsync(g->tile_state, 0, sizeof(uint64_t) * ((size_t)g->net + 1), s));
         g->samp_epoch = 0;
     }
     g->samp_epoch++;
     launch_fill_u32(s, 1, g->ticket, 0u);
-    ErSampleArgs<double> a;
-    a.m = g->m;
-    a.ntiles = (g->m + 255) / 256;
+    BatchSampleArgs a;
+    a.m = g->M;
+    a.et = Tiles{g->et_start, g->et_graph, g->net};
+    a.edge_off = g->edge_off_dev;
     a.eu = g->eu;
     a.ev = g->ev;
     a.ew = g->ew;
     a.Z = g->X64;
     a.k = k;
-    a.beta = beta;
-    a.ck = coin_key_from_u64(sampling_seed);
+    a.beta = g->p_betas;
+    a.keys = g->p_keys;
     a.p_in = p_in_dev;
     a.p_out = g->p;
     a.d_out = write_d ? g->dnorm : nullptr;
@@ -509,21 +780,25 @@ int graph_sample(ss_graph *g, int k, double beta, uint64_t sampling_seed, const 
     a.kw = g->kw;
     a.ticket = g->ticket;
     a.tile_state = g->tile_state;
+    a.tile_incl = g->tile_incl;
     a.epoch = g->samp_epoch;
-    a.kept_total = g->kept;
     a.err = g->status + 1;
     a.spin_limit = kSpinLimit;
-    if (g->m == 0) {
-        g->last_kept = 0;
-        return SS_OK;
-    }
     {
         ProfScope ps(g, SS_KERNEL_ER_SAMPLE);
-        launch_er_sample<double>(s, a);
+        bt_er_sample(s, a, g->net);
     }
     SS_HIP(hipGetLastError());
-    SS_HIP(hipMemcpyAsync(&g->last_kept, g->kept, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    std::vector<int64_t> incl((size_t)g->net);
+    SS_HIP(hipMemcpyAsync(incl.data(), g->tile_incl, sizeof(int64_t) * g->net, hipMemcpyDeviceToHost, s));
     SS_TRY(read_status(g, nullptr));
+    for (int i = 0; i < g->G; i++) {
+        const int32_t t0 = g->etile_off[i], t1 = g->etile_off[i + 1];
+        const int64_t start = t0 > 0 ? incl[t0 - 1] : 0;
+        const int64_t end = t1 > t0 ? incl[t1 - 1] : start;
+        g->kept_start[i] = start;
+        g->kept_count[i] = end - start;
+    }
     return SS_OK;
 }
 

@@ -402,7 +402,8 @@ int factor_schedules(const Factor &F, Schedule &fwd, Schedule &bwd, int32_t long
             }
     }
     auto pack = [&](Schedule &S, const std::vector<int32_t> &slot, const std::vector<int32_t> &lvl,
-                    const int32_t *ptr, const int32_t *col, const double *val, bool skip_diag) -> int {
+                    const int32_t *ptr, const int32_t *col, const double *val, bool skip_diag,
+                    bool by_level) -> int {
         S.ptr.assign((size_t)n + 1, 0);
         S.rows.resize((size_t)n);
         int64_t total = 0;
@@ -415,23 +416,33 @@ int factor_schedules(const Factor &F, Schedule &fwd, Schedule &bwd, int32_t long
         S.ptr[n] = (int32_t)total;
         S.col.resize((size_t)total);
         S.val.resize((size_t)total);
+        std::vector<std::pair<int32_t, double>> ent;
         int64_t p = 0;
         for (int64_t t = 0; t < n; t++) {
             const int32_t i = slot[t];
             S.rows[t] = F.order[i];
+            ent.clear();
             for (int32_t q = ptr[i]; q < ptr[i + 1]; q++) {
                 if (skip_diag && col[q] == i) continue;
                 if (lvl[col[q]] >= lvl[i])
                     return fail(SS_ERR_INVALID, "schedule: position %d depends on position %d of level >= its own", i, col[q]);
-                S.col[p] = F.order[col[q]];
-                S.val[p] = val[q];
+                ent.push_back({col[q], val[q]});
+            }
+            if (by_level)
+                std::stable_sort(ent.begin(), ent.end(), [&](const std::pair<int32_t, double> &a,
+                                                             const std::pair<int32_t, double> &b) {
+                    return lvl[a.first] < lvl[b.first];
+                });
+            for (const auto &e : ent) {
+                S.col[p] = F.order[e.first];
+                S.val[p] = e.second;
                 p++;
             }
         }
         return SS_OK;
     };
-    SS_TRY(pack(fwd, fslot, F.lvl_fwd, lcnt.data(), lcol.data(), lval.data(), false));
-    SS_TRY(pack(bwd, bslot, F.lvl_bwd, F.urowptr.data(), F.ucol.data(), F.uval.data(), true));
+    SS_TRY(pack(fwd, fslot, F.lvl_fwd, lcnt.data(), lcol.data(), lval.data(), false, true));
+    SS_TRY(pack(bwd, bslot, F.lvl_bwd, F.urowptr.data(), F.ucol.data(), F.uval.data(), true, false));
 
     // runs of small levels (one workgroup walks them)
     auto small = [&](const Schedule &S, int32_t l) { return S.level_ptr[l + 1] - S.level_ptr[l] <= small_rows; };
@@ -441,20 +452,38 @@ int factor_schedules(const Factor &F, Schedule &fwd, Schedule &bwd, int32_t long
     bwd.small_lo = 0;
     bwd.small_hi = 0;
     while (bwd.small_hi < bwd.depth && small(bwd, bwd.small_hi)) bwd.small_hi++;
-    // forward tail rows: entries from head levels first (1-based level of a
-    // vertex < small_lo + 1 means it is done before the tail starts)
+    // forward tail rows: the leading entries whose columns sit in head
+    // levels (1-based level <= small_lo) are done before the tail starts
     fwd.split.assign((size_t)n, 0);
     std::vector<int32_t> vlvl((size_t)n);
     for (int64_t p = 0; p < n; p++) vlvl[F.order[p]] = F.lvl_fwd[p];
     for (int64_t t = 0; t < n; t++) {
         const int32_t b = fwd.ptr[t], e = fwd.ptr[t + 1];
-        if (t < fwd.level_ptr[fwd.small_lo]) { fwd.split[t] = e - b; continue; }
-        std::vector<std::pair<int32_t, double>> ent;
-        for (int32_t q = b; q < e; q++) ent.push_back({fwd.col[q], fwd.val[q]});
-        auto mid = std::stable_partition(ent.begin(), ent.end(),
-                                         [&](const std::pair<int32_t, double> &x) { return vlvl[x.first] <= fwd.small_lo; });
-        fwd.split[t] = (int32_t)(mid - ent.begin());
-        for (int32_t q = b; q < e; q++) { fwd.col[q] = ent[q - b].first; fwd.val[q] = ent[q - b].second; }
+        int32_t q = b;
+        if (t >= fwd.level_ptr[fwd.small_lo])
+            while (q < e && vlvl[fwd.col[q]] <= fwd.small_lo) q++;
+        else
+            q = e;
+        fwd.split[t] = q - b;
+    }
+    // work units of the small runs
+    auto parts = [&](int32_t len) { return len <= long_row ? 1 : (len <= 16 * long_row ? 4 : 16); };
+    for (Schedule *S : {&fwd, &bwd}) {
+        S->unit_ptr.assign(1, 0);
+        S->unit_t.clear();
+        S->unit_s.clear();
+        for (int32_t l = S->small_lo; l < S->small_hi; l++) {
+            for (int want : {16, 4, 1})
+                for (int32_t t = S->level_ptr[l]; t < S->level_ptr[l + 1]; t++) {
+                    const int P = parts(S->ptr[t + 1] - S->ptr[t]);
+                    if (P != want) continue;
+                    for (int sp = 0; sp < P; sp++) {
+                        S->unit_t.push_back(t);
+                        S->unit_s.push_back(sp);
+                    }
+                }
+            S->unit_ptr.push_back((int32_t)S->unit_t.size());
+        }
     }
     return SS_OK;
 }

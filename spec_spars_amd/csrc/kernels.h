@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include "hash_dev.h"
+#include "solve_shape.h"
 
 namespace ss {
 
@@ -78,8 +79,7 @@ void launch_copy_f64(hipStream_t s, int64_t count, const double *a, double *b);
 // partials (partial buffers hold kMaxPartialBlocks * k doubles).
 static const int kMaxPartialBlocks = 4096;
 // Rows of one level, positions [t0, t1); the first nlong of them are long
-// rows processed by a whole block each.
-static const int kLongRow = 64;
+// rows (row_parts > 1) processed by a whole block each.
 template <typename T, bool BWD>
 void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nlong, int32_t t1, const int32_t *rows,
                            const int32_t *ptr, const int32_t *col, const T *val, const double *diag, int k,
@@ -90,10 +90,14 @@ void launch_spmm_dot(hipStream_t s, int32_t n, const int32_t *rowptr, const int3
 template <typename T>
 void launch_tri_tail_head(hipStream_t s, int32_t t0, int32_t t1, const int32_t *rows, const int32_t *ptr,
                           const int32_t *split, const int32_t *col, const T *val, int k, const T *in, T *out);
+// Small levels [l0, l1) walked by one workgroup.  Level l's work units are
+// [unit_ptr[l - l0], unit_ptr[l - l0 + 1]): unit u is part unit_s[u] of the
+// row at schedule position unit_t[u]; a row of P parts has P consecutive
+// units, 16-part rows first, then 4-part rows, then 1-part rows.
 template <typename T, bool BWD>
-void launch_tri_small(hipStream_t s, int32_t l0, int32_t l1, const int32_t *level_ptr, const int32_t *rows,
-                      const int32_t *ptr, const int32_t *split, const int32_t *col, const T *val, const double *diag,
-                      int k, const T *in, T *out);
+void launch_tri_small(hipStream_t s, int32_t l0, int32_t l1, const int32_t *unit_ptr, const int32_t *unit_t,
+                      const int32_t *unit_s, const int32_t *rows, const int32_t *ptr, const int32_t *split,
+                      const int32_t *col, const T *val, const double *diag, int k, const T *in, T *out);
 template <typename T>
 void launch_ydy_partial(hipStream_t s, int32_t n, int k, const T *Yv, const double *diag, double *partial);
 template <typename T>

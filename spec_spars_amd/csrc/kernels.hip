@@ -412,6 +412,22 @@ __global__ void k_fill_u32(int64_t count, uint32_t *p, uint32_t v) {
 //                         (division skipped where D_i == 0, as
 //                          include/custom_cg.hpp:227-233 does)
 // ---------------------------------------------------------------------------
+// Start value of row i: forward r_i; backward y_i / D_i (D_i == 0 keeps y_i).
+template <typename T, bool BWD>
+__device__ __forceinline__ T row_start(int i, int j, int k, double d, const T *in) {
+    T acc = in[(size_t)i * k + j];
+    if (BWD && d != 0.0) acc = (T)((double)acc / d);
+    return acc;
+}
+
+// Fold of the parts a block or workgroup left in sm[first .. first+P-1].
+template <typename T>
+__device__ __forceinline__ T fold_parts(const T (*sm)[64], int first, int P, int lane) {
+    T comb = sm[first][lane];
+    for (int s = 1; s < P; s++) comb += sm[first + s][lane];
+    return comb;
+}
+
 template <typename T, bool BWD>
 __global__ __launch_bounds__(256) void k_trisolve_level(int32_t t0, int32_t nlong, int32_t t1,
                                                         const int32_t *__restrict__ rows,
@@ -422,26 +438,20 @@ __global__ __launch_bounds__(256) void k_trisolve_level(int32_t t0, int32_t nlon
                                                         const T *__restrict__ in, T *__restrict__ out) {
     const int lane = SS_LANE, wv = SS_WAVE_IN_BLOCK;
     if ((int)blockIdx.x < nlong) {
-        // a long row: the block's 4 waves take every 4th entry, then combine
-        __shared__ T sm[4][64];
+        // a long row: its P parts go round the block's 4 waves, wave 0 folds
+        __shared__ T sm[16][64];
         const int t = t0 + blockIdx.x;
         const int i = wave_uniform(rows[t]);
         const int q0 = wave_uniform(ptr[t]), q1 = wave_uniform(ptr[t + 1]);
+        const int P = row_parts(q1 - q0);
+        const double d = BWD ? diag[i] : 1.0;
         for (int jb = 0; jb < k; jb += 64) {
             const int j = jb + lane;
-            T part = 0;
-            if (j < k) part = row_dot_sub<T>((T)0, q0 + wv, q1, 4, col, val, out, k, j);
-            sm[wv][lane] = part;
+            for (int sp = wv; sp < P; sp += 4)
+                sm[sp][lane] = j < k ? row_dot_sub<T>((T)0, q0 + sp, q1, P, col, val, out, k, j) : (T)0;
             __syncthreads();
-            if (wv == 0 && j < k) {
-                T acc = in[(size_t)i * k + j];
-                if (BWD) {
-                    const double d = diag[i];
-                    if (d != 0.0) acc = (T)((double)acc / d);
-                }
-                acc += ((sm[0][lane] + sm[1][lane]) + sm[2][lane]) + sm[3][lane];
-                out[(size_t)i * k + j] = acc;
-            }
+            if (wv == 0 && j < k)
+                out[(size_t)i * k + j] = row_start<T, BWD>(i, j, k, d, in) + fold_parts<T>(sm, 0, P, lane);
             __syncthreads();
         }
         return;
@@ -454,16 +464,16 @@ __global__ __launch_bounds__(256) void k_trisolve_level(int32_t t0, int32_t nlon
     for (int jb = 0; jb < k; jb += 64) {
         const int j = jb + lane;
         if (j >= k) break;
-        T acc = in[(size_t)i * k + j];
-        if (BWD && d != 0.0) acc = (T)((double)acc / d);
-        acc = row_dot_sub<T>(acc, q0, q1, 1, col, val, out, k, j);
-        out[(size_t)i * k + j] = acc;
+        out[(size_t)i * k + j] = row_dot_sub<T>(row_start<T, BWD>(i, j, k, d, in), q0, q1, 1, col, val, out, k, j);
     }
 }
 
-// Forward tail, step 1: for every row of the trailing run of small levels,
-// the contribution of the entries whose columns were finished by the head
-// levels (the first split[t] entries): out_i = in_i - sum.  Wave per row.
+// Forward tail, step 1: for every one-part row of the trailing run of small
+// levels, the running subtraction over the entries whose columns the head
+// levels finished (the first split[t]; a row's entries are sorted by the
+// level of their column, so these lead): out_i = r_i - those terms.  The
+// small-level kernel continues the same running subtraction, so the row's
+// bits equal a single pass.  Rows of several parts are left to it whole.
 template <typename T>
 __global__ __launch_bounds__(256) void k_tri_tail_head(int32_t t0, int32_t t1, const int32_t *__restrict__ rows,
                                                        const int32_t *__restrict__ ptr,
@@ -473,7 +483,8 @@ __global__ __launch_bounds__(256) void k_tri_tail_head(int32_t t0, int32_t t1, c
     const int t = wave_uniform(t0 + blockIdx.x * 4 + SS_WAVE_IN_BLOCK);
     if (t >= t1) return;
     const int i = wave_uniform(rows[t]);
-    const int q0 = wave_uniform(ptr[t]);
+    const int q0 = wave_uniform(ptr[t]), q1 = wave_uniform(ptr[t + 1]);
+    if (row_parts(q1 - q0) != 1) return;
     const int qs = q0 + wave_uniform(split[t]);
     for (int jb = 0; jb < k; jb += 64) {
         const int j = jb + SS_LANE;
@@ -482,23 +493,18 @@ __global__ __launch_bounds__(256) void k_tri_tail_head(int32_t t0, int32_t t1, c
     }
 }
 
-// A run of small levels [l0, l1) walked by one workgroup of 16 waves, a
+// A run of small levels [l0, l1) walked by one workgroup of 16 waves with a
 // workgroup barrier between levels (all waves share one CU, so the barrier's
-// workgroup-scope fence orders the row stores before the next level's loads).
-//   forward tail: out_i (head part from k_tri_tail_head) -= entries split..end
-//   backward head: out_i = in_i / D_i - all entries
+// workgroup-scope fence orders the row stores before the next level's
+// loads).  Each level's work units go out 16 at a time, one per wave; a
+// row's parts never straddle two groups of 16, and the wave holding part 0
+// folds them.  One-part rows: forward continues from the head part
+// k_tri_tail_head left in out (entries split..end), backward runs all
+// entries from y_i / D_i.
 template <typename T, bool BWD>
-__device__ __forceinline__ T small_start(int i, int j, int k, double d, const T *in, const T *out) {
-    if (BWD) {
-        T acc = in[(size_t)i * k + j];
-        if (d != 0.0) acc = (T)((double)acc / d);
-        return acc;
-    }
-    return out[(size_t)i * k + j];
-}
-
-template <typename T, bool BWD>
-__global__ __launch_bounds__(1024) void k_tri_small(int32_t l0, int32_t l1, const int32_t *__restrict__ level_ptr,
+__global__ __launch_bounds__(1024) void k_tri_small(int32_t l0, int32_t l1, const int32_t *__restrict__ unit_ptr,
+                                                    const int32_t *__restrict__ unit_t,
+                                                    const int32_t *__restrict__ unit_s,
                                                     const int32_t *__restrict__ rows,
                                                     const int32_t *__restrict__ ptr,
                                                     const int32_t *__restrict__ split,
@@ -508,44 +514,35 @@ __global__ __launch_bounds__(1024) void k_tri_small(int32_t l0, int32_t l1, cons
     __shared__ T sm[16][64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int l = l0; l < l1; l++) {
-        const int tb = level_ptr[l], te = level_ptr[l + 1];
-        const int r = te - tb;
-        if (r >= 16) {
-            // a wave per row
-            for (int t = tb + wv; t < te; t += 16) {
-                const int i = wave_uniform(rows[t]);
-                const int q0 = wave_uniform(ptr[t]), q1 = wave_uniform(ptr[t + 1]);
-                const int qs = BWD ? q0 : q0 + wave_uniform(split[t]);
-                const double d = BWD ? diag[i] : 1.0;
-                for (int jb = 0; jb < k; jb += 64) {
-                    const int j = jb + lane;
-                    if (j >= k) break;
-                    const T acc = small_start<T, BWD>(i, j, k, d, in, out);
-                    out[(size_t)i * k + j] = row_dot_sub<T>(acc, qs, q1, 1, col, val, out, k, j);
-                }
-            }
-        } else {
-            // g = 16 / r (rounded down to a power of two) waves per row, entries strided by g
-            int g = 16 / r;
-            g = 1 << (31 - __builtin_clz(g));
-            const int slot = wv / g, sub = wv - slot * g;
-            const int t = tb + slot;
-            const bool act = t < te;
+        const int u0 = unit_ptr[l - l0], u1 = unit_ptr[l - l0 + 1];
+        for (int ub = u0; ub < u1; ub += 16) {
+            const int u = ub + wv;
+            const bool act = u < u1;
+            const int t = act ? wave_uniform(unit_t[u]) : 0;
+            const int sp = act ? wave_uniform(unit_s[u]) : 0;
             const int i = act ? wave_uniform(rows[t]) : 0;
             const int q0 = act ? wave_uniform(ptr[t]) : 0, q1 = act ? wave_uniform(ptr[t + 1]) : 0;
-            const int qs = BWD ? q0 : (act ? q0 + wave_uniform(split[t]) : 0);
+            const int P = act ? row_parts(q1 - q0) : 1;
             const double d = (BWD && act) ? diag[i] : 1.0;
             for (int jb = 0; jb < k; jb += 64) {
                 const int j = jb + lane;
-                T part = 0;
-                if (act && j < k) part = row_dot_sub<T>((T)0, qs + sub, q1, g, col, val, out, k, j);
-                sm[wv][lane] = part;
-                __syncthreads();
-                if (act && sub == 0 && j < k) {
-                    T acc = small_start<T, BWD>(i, j, k, d, in, out);
-                    for (int u = 0; u < g; u++) acc += sm[wv + u][lane];
-                    out[(size_t)i * k + j] = acc;
+                if (P == 1) {
+                    if (act && j < k) {
+                        if (BWD) {
+                            out[(size_t)i * k + j] =
+                                row_dot_sub<T>(row_start<T, true>(i, j, k, d, in), q0, q1, 1, col, val, out, k, j);
+                        } else {
+                            const int qs = q0 + wave_uniform(split[t]);
+                            out[(size_t)i * k + j] =
+                                row_dot_sub<T>(out[(size_t)i * k + j], qs, q1, 1, col, val, out, k, j);
+                        }
+                    }
+                } else {
+                    sm[wv][lane] = (act && j < k) ? row_dot_sub<T>((T)0, q0 + sp, q1, P, col, val, out, k, j) : (T)0;
                 }
+                __syncthreads();
+                if (act && P > 1 && sp == 0 && j < k)
+                    out[(size_t)i * k + j] = row_start<T, BWD>(i, j, k, d, in) + fold_parts<T>(sm, wv, P, lane);
                 __syncthreads();
             }
         }
@@ -844,17 +841,17 @@ template void launch_tri_tail_head<float>(hipStream_t, int32_t, int32_t, const i
                                           const int32_t *, const int32_t *, const float *, int, const float *, float *);
 
 template <typename T, bool BWD>
-void launch_tri_small(hipStream_t s, int32_t l0, int32_t l1, const int32_t *level_ptr, const int32_t *rows,
-                      const int32_t *ptr, const int32_t *split, const int32_t *col, const T *val, const double *diag,
-                      int k, const T *in, T *out) {
+void launch_tri_small(hipStream_t s, int32_t l0, int32_t l1, const int32_t *unit_ptr, const int32_t *unit_t,
+                      const int32_t *unit_s, const int32_t *rows, const int32_t *ptr, const int32_t *split,
+                      const int32_t *col, const T *val, const double *diag, int k, const T *in, T *out) {
     if (l1 <= l0) return;
-    hipLaunchKernelGGL((k_tri_small<T, BWD>), dim3(1), dim3(1024), 0, s, l0, l1, level_ptr, rows, ptr, split, col, val,
-                       diag, k, in, out);
+    hipLaunchKernelGGL((k_tri_small<T, BWD>), dim3(1), dim3(1024), 0, s, l0, l1, unit_ptr, unit_t, unit_s, rows, ptr,
+                       split, col, val, diag, k, in, out);
 }
 #define SS_INST_SMALL(T, B)                                                                                       \
     template void launch_tri_small<T, B>(hipStream_t, int32_t, int32_t, const int32_t *, const int32_t *,          \
-                                         const int32_t *, const int32_t *, const int32_t *, const T *,              \
-                                         const double *, int, const T *, T *);
+                                         const int32_t *, const int32_t *, const int32_t *, const int32_t *,        \
+                                         const int32_t *, const T *, const double *, int, const T *, T *);
 SS_INST_SMALL(double, false)
 SS_INST_SMALL(double, true)
 SS_INST_SMALL(float, false)

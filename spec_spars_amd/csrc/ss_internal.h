@@ -73,6 +73,10 @@ struct Schedule {
     // the leading run.
     int32_t small_lo = 0, small_hi = 0;
     std::vector<int32_t> split;       // per position (forward only)
+    // Work units of the small-level run: level small_lo + i owns units
+    // [unit_ptr[i], unit_ptr[i+1]); unit u is part unit_s[u] of the row at
+    // position unit_t[u] (rows of 16 parts first, then 4, then 1).
+    std::vector<int32_t> unit_ptr, unit_t, unit_s;
     std::vector<int32_t> ptr;         // entries of position t: [ptr[t], ptr[t+1])
     std::vector<int32_t> col;
     std::vector<double> val;
@@ -82,7 +86,9 @@ int factor_build(int64_t n, const int32_t *colptr, const int32_t *rowidx, const 
                  uint64_t seed, int ordering, Factor &F);
 void factor_levels(Factor &F);
 // Level-ordered schedules in vertex ids; checks that every dependency sits
-// at a strictly lower level.
+// at a strictly lower level.  Forward rows list their entries sorted by the
+// level of the column (stable), so the entries finished before any level
+// lead the row.  long_row must be the kLongRow the solve kernels use.
 int factor_schedules(const Factor &F, Schedule &fwd, Schedule &bwd, int32_t long_row = 64,
                      int32_t small_rows = 16);
 

@@ -162,3 +162,22 @@ def path_with_shortcuts():
 def beta_for(n, epsilon=0.3, beta_constant=4):
     """beta = eps^-2 * beta_constant * ln(n)  (src/sparsifier.rs:264)."""
     return math.pow(epsilon, -2.0) * float(beta_constant) * math.log(float(n))
+
+
+def laplacian_csc(n, u, v, w):
+    """CSC Laplacian with the diagonal (symmetric, so also CSR) of canonical
+    edges: off-diagonals -w, diagonal = weighted degree summed in edge order
+    (u's and v's contributions interleaved as the edges come), rows sorted
+    within each column -- the same matrix the device graph holds."""
+    u = np.asarray(u, np.int64)
+    v = np.asarray(v, np.int64)
+    w = np.asarray(w, np.float64)
+    diag = np.zeros(n, np.float64)
+    np.add.at(diag, np.stack([u, v], 1).ravel(), np.repeat(w, 2))
+    rows = np.concatenate([v, u, np.arange(n)])
+    cols = np.concatenate([u, v, np.arange(n)])
+    vals = np.concatenate([-1.0 * w, -1.0 * w, diag])
+    order = np.lexsort((rows, cols))
+    colptr = np.zeros(n + 1, np.int32)
+    colptr[1:] = np.cumsum(np.bincount(cols, minlength=n))
+    return colptr, rows[order].astype(np.int32), vals[order]

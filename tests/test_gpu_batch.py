@@ -1,0 +1,276 @@
+"""Batches of uneven graphs on the GPU: each graph's results are bit-identical
+alone and in a mix, and match the oracle graph by graph."""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from conftest import components
+
+pytestmark = pytest.mark.gpu
+
+
+def _two_components():
+    """Two weighted grids side by side plus two isolated vertices."""
+    from spec_spars_amd import graphs
+    n1, u1, v1 = graphs.grid2d(6, 5)
+    n2, u2, v2 = graphs.grid2d(4, 4)
+    u = np.concatenate([u1, u2 + n1])
+    v = np.concatenate([v1, v2 + n1])
+    u, v, _ = graphs.canonical(u, v)
+    w = graphs.loguniform_weights(len(u), seed=5)
+    return n1 + n2 + 2, u, v, w
+
+
+def _mix():
+    from spec_spars_amd import graphs
+    out = []
+    n, u, v = graphs.grid2d(12, 9)
+    out.append((n, u, v, graphs.loguniform_weights(len(u), seed=1)))
+    n, u, v = graphs.rgg(500, seed=3)
+    out.append((n, u, v, np.ones(len(u))))
+    n, u, v = graphs.chung_lu(700, 2500, seed=2)
+    out.append((n, u, v, graphs.uniform_weights(len(u), seed=2)))
+    out.append(_two_components())
+    n = 50
+    out.append((n, np.arange(n - 1, dtype=np.int32), np.arange(1, n, dtype=np.int32), np.ones(n - 1)))
+    n, u, v = graphs.grid3d(7)
+    out.append((n, u, v, graphs.loguniform_weights(len(u), seed=4)))
+    return out
+
+
+def _factors(gs, seed=2, ordering=0):
+    import spec_spars_amd as ss
+    from spec_spars_amd import graphs
+    return [ss.Factor(*graphs.laplacian_csc(n, u, v, w), seed=seed + i, ordering=ordering)
+            for i, (n, u, v, w) in enumerate(gs)]
+
+
+def _run(ctx, gs, fs, idx, k, prec):
+    """Sparsify graphs gs[idx] as one batch; per-graph seeds follow the index."""
+    import spec_spars_amd as ss
+    from spec_spars_amd import graphs
+    B = ss.Batch(ctx, [gs[i] for i in idx])
+    B.set_factors([fs[i] for i in idx])
+    betas = [graphs.beta_for(gs[i][0], 0.3) for i in idx]
+    rep = B.sparsify(k, betas, sketch_seeds=[10 + i for i in idx], sampling_seeds=[20 + i for i in idx],
+                     solve=ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-8, max_iter=3000, precision=prec))
+    res = []
+    for j in range(len(idx)):
+        res.append({"Y": B.read_block(j, "Y"), "Z": B.read_block(j, "Z"), "p": B.read_probs(j),
+                    "kept": B.read_kept(j), "rep": rep[j]})
+    B.close()
+    return res
+
+
+def _laplacian(n, u, v, w):
+    from spec_spars_amd import graphs
+    cp, ri, va = graphs.laplacian_csc(n, u, v, w)
+    return sp.csc_matrix((va, ri, cp), shape=(n, n))
+
+
+def _component_projection(n, u, v, Y):
+    import scipy.sparse.csgraph as cg
+    A = sp.coo_matrix((np.ones(len(u)), (np.asarray(u), np.asarray(v))), shape=(n, n))
+    nc, lab = cg.connected_components(A, directed=False)
+    Yp = Y.copy()
+    for c in range(nc):
+        Yp[lab == c] -= Y[lab == c].mean(0)
+    return Yp
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_batch_alone_vs_mixed_bit_identical(ctx, prec):
+    gs = _mix()
+    fs = _factors(gs)
+    k = 8
+    mixed = _run(ctx, gs, fs, list(range(len(gs))), k, prec)
+    for i in range(len(gs)):
+        alone = _run(ctx, gs, fs, [i], k, prec)[0]
+        for key in ("Y", "Z", "p"):
+            assert np.array_equal(alone[key], mixed[i][key]), (i, key)
+        for a, b in zip(alone["kept"], mixed[i]["kept"]):
+            assert np.array_equal(a, b), i
+        assert alone["rep"]["kept"] == mixed[i]["rep"]["kept"]
+        assert alone["rep"]["max_iters"] == mixed[i]["rep"]["max_iters"]
+        assert alone["rep"]["max_relres"] == mixed[i]["rep"]["max_relres"]
+    # a reordered mix gives the same bits too
+    order = list(range(len(gs)))[::-1]
+    rev = _run(ctx, gs, fs, order, k, prec)
+    for j, i in enumerate(order):
+        assert np.array_equal(rev[j]["Z"], mixed[i]["Z"]) and np.array_equal(rev[j]["p"], mixed[i]["p"])
+
+
+def test_batch_matches_oracle_per_graph(ctx, oracle_mod):
+    import spec_spars_amd as ss
+    from spec_spars_amd import graphs
+    gs = _mix()
+    fs = _factors(gs)
+    k = 12
+    res = _run(ctx, gs, fs, list(range(len(gs))), k, ss.PREC_FP64)
+    for i, (n, u, v, w) in enumerate(gs):
+        r = res[i]
+        # (a) signs and incidence: bit-exact with the graph's own edge ids
+        Yo = oracle_mod.sketch(u, v, w, n, k, seed=10 + i, scale=1.0 / math.sqrt(k))
+        assert np.array_equal(r["Y"], Yo), i
+        # (b)+(c) every component's residual at the requested tolerance
+        L = _laplacian(n, u, v, w)
+        Yp = _component_projection(n, u, v, r["Y"])
+        res_norm = np.linalg.norm(Yp - L @ r["Z"], axis=0)
+        assert np.all(res_norm <= 1.01e-8 * np.linalg.norm(Yp, axis=0) + 1e-300), i
+        cp, ri, va = graphs.laplacian_csc(n, u, v, w)
+        uu, vv, lv = oracle_mod.upper_edges(n, cp, ri, va)
+        if components(n, u, v) == 1:
+            urp, uci, uva, dg = fs[i].export()
+            Zo, _, _ = oracle_mod.solve_lap(Yo, cp, ri, va, urp, uci, uva, dg, stop_rule=1, tol=1e-12,
+                                            max_iter=3000)
+            a, b = r["Z"] - r["Z"].mean(0), Zo - Zo.mean(0)
+            assert (np.linalg.norm(a - b, axis=0) / np.linalg.norm(b, axis=0)).max() < 1e-6, i
+            po = oracle_mod.probs(lv, oracle_mod.diff_norms(uu, vv, Zo), graphs.beta_for(n, 0.3), k)
+            both = (r["p"] < 1) & (po < 1)
+            assert np.all(np.abs(r["p"][both] - po[both]) <= 1e-4 * po[both]), i
+        # (d) the mask and weights given these probabilities are the oracle's
+        ref = oracle_mod.sample_reweight(n, uu, vv, lv, r["p"], oracle_mod.coins(20 + i, len(uu)))
+        ku, kv, kw = r["kept"]
+        assert np.array_equal(ref["ku"], ku) and np.array_equal(ref["kv"], kv) and np.array_equal(ref["kw"], kw), i
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_batch_solve_gaussian_rhs(ctx, prec):
+    import spec_spars_amd as ss
+    gs = _mix()
+    fs = _factors(gs, seed=7, ordering=ss.ORDER_ROUNDS)
+    B = ss.Batch(ctx, gs)
+    B.set_factors(fs)
+    rng = np.random.default_rng(0)
+    k = 16
+    rhs = []
+    for n, u, v, w in gs:
+        b = rng.standard_normal((n, k))
+        rhs.append(b - b.mean(0))
+    info = B.solve(rhs, ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-8, max_iter=3000, precision=prec))
+    assert info["relres"].max() <= 1e-8
+    for i, (n, u, v, w) in enumerate(gs):
+        Z = B.read_block(i, "Z")
+        Yp = _component_projection(n, u, v, rhs[i])
+        L = _laplacian(n, u, v, w)
+        rr = np.linalg.norm(Yp - L @ Z, axis=0) / np.linalg.norm(Yp, axis=0)
+        assert rr.max() <= 1.01e-8, i
+        assert np.all(info["iters"][i] > 0)
+    B.close()
+
+
+def test_single_graph_api_is_batch_of_one(ctx):
+    import spec_spars_amd as ss
+    from spec_spars_amd import graphs
+    gs = _mix()
+    n, u, v, w = gs[0]
+    f = _factors(gs)[0]
+    G = ss.Graph(ctx, n, u, v, w)
+    G.set_factor(f)
+    solve = ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-8, max_iter=3000)
+    G.sparsify(8, graphs.beta_for(n, 0.3), sketch_seed=10, sampling_seed=20, solve=solve)
+    one = _run(ctx, gs, [f], [0], 8, ss.PREC_FP64)[0]
+    assert np.array_equal(G.read_block("Z"), one["Z"])
+    assert np.array_equal(G.read_probs(), one["p"])
+    # the device Laplacian is the host one
+    cp, ri, va = G.laplacian()
+    hc, hr, hv = graphs.laplacian_csc(n, u, v, w)
+    assert np.array_equal(cp, hc) and np.array_equal(ri, hr) and np.array_equal(va, hv)
+
+
+def test_batch_argument_errors(ctx):
+    import spec_spars_amd as ss
+    gs = _mix()[:2]
+    fs = _factors(gs)
+    B = ss.Batch(ctx, gs)
+    with pytest.raises(ss.SSError):
+        B.set_factors(fs[:1])
+    B.set_factors(fs)
+    with pytest.raises(ss.SSError):
+        B.read_probs(2)
+    # per-graph k must agree: built by hand through the C ABI
+    from spec_spars_amd import _lib
+    o = (_lib.SparsifyOpts * 2)()
+    for i in range(2):
+        o[i].k = 8 + i
+        o[i].scale = 1.0
+        o[i].beta = 10.0
+        o[i].solve = ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-8)
+    assert _lib.load().ss_batch_sparsify(B._h, o, None) == _lib.SS_ERR_INVALID
+    B.close()
+
+
+def test_c1_grid64_pipeline(ctx, oracle_mod):
+    """2D 64x64 grid, unit weights, k=24 Rademacher columns scaled 1/sqrt(k),
+    PCG to 1e-8 relative residual, eps=0.3, seed 0, against the oracle."""
+    import spec_spars_amd as ss
+    from spec_spars_amd import graphs
+    n, u, v = graphs.grid2d(64, 64)
+    w = np.ones(len(u))
+    assert (n, len(u)) == (4096, 8064)
+    k = 24
+    G = ss.Graph(ctx, n, u, v, w)
+    cp, ri, va = G.laplacian()
+    f = ss.Factor(cp, ri, va, seed=0)
+    G.set_factor(f)
+    beta = graphs.beta_for(n, 0.3)
+    rep = G.sparsify(k, beta, scale=1.0 / math.sqrt(k), sketch_seed=0, sampling_seed=0,
+                     solve=ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-8, max_iter=3000))
+    assert rep["max_relres"] <= 1e-8
+    Y = G.read_block("Y")
+    Yo = oracle_mod.sketch(u, v, w, n, k, seed=0, scale=1.0 / math.sqrt(k))
+    assert np.array_equal(Y, Yo)
+    Zo, _, _ = oracle_mod.solve_lap(Yo, cp, ri, va, *f.export(), stop_rule=1, tol=1e-12, max_iter=3000)
+    Z = G.read_block("Z")
+    a, b = Z - Z.mean(0), Zo - Zo.mean(0)
+    assert (np.linalg.norm(a - b, axis=0) / np.linalg.norm(b, axis=0)).max() < 1e-5
+    uu, vv, lv = oracle_mod.upper_edges(n, cp, ri, va)
+    dn = oracle_mod.diff_norms(uu, vv, Zo)
+    lev_o = -lv * dn ** 2
+    lev = -lv * oracle_mod.diff_norms(uu, vv, Z) ** 2
+    assert np.all(np.abs(lev - lev_o) <= 1e-4 * lev_o)
+    p = G.read_probs()
+    ref = oracle_mod.sample_reweight(n, uu, vv, lv, p, oracle_mod.coins(0, len(uu)))
+    ku, kv, kw = G.read_kept()
+    assert np.array_equal(ref["ku"], ku) and np.array_equal(ref["kw"], kw)
+
+
+def test_spectral_quality_dense_graph(ctx):
+    """Complete graph on 700 vertices, eps = 0.3, unit-variance signs: about a
+    quarter of the edges are dropped and the generalized eigenvalues of
+    (L_H, L_G) and 64 Rayleigh quotients stay within 1 +- eps.  The kept
+    weights are w / sqrt(p) as in src/sparsifier.rs:655-657, so E[L_H] is
+    below L_G where p < 1: the bound holds while p stays near 1 (at eps = 0.5
+    on this graph the smallest eigenvalue is about 0.42)."""
+    import scipy.linalg as sl
+    import spec_spars_amd as ss
+    from spec_spars_amd import graphs
+    n = 700
+    iu, iv = np.triu_indices(n, 1)
+    u, v, _ = graphs.canonical(iu.astype(np.int32), iv.astype(np.int32))
+    w = graphs.uniform_weights(len(u), seed=3)
+    eps = 0.3
+    k = 64
+    G = ss.Graph(ctx, n, u, v, w)
+    cp, ri, va = G.laplacian()
+    G.set_factor(ss.Factor(cp, ri, va, seed=0))
+    rep = G.sparsify(k, graphs.beta_for(n, eps), scale=1.0, sketch_seed=1, sampling_seed=2,
+                     solve=ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-8, max_iter=3000))
+    ku, kv, kw = G.read_kept()
+    assert 0.5 * len(u) < rep["kept"] < 0.9 * len(u)
+    LG = _laplacian(n, u, v, w).toarray()
+    LH = np.zeros((n, n))
+    np.add.at(LH, (ku, kv), -kw)
+    np.add.at(LH, (kv, ku), -kw)
+    LH[np.diag_indices(n)] = -LH.sum(1)
+    # on the complement of the all-ones vector
+    Q = sl.null_space(np.ones((1, n)))
+    lam = sl.eigh(Q.T @ LH @ Q, Q.T @ LG @ Q, eigvals_only=True)
+    assert lam.min() >= 1 - eps and lam.max() <= 1 + eps
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal((n, 64))
+    x -= x.mean(0)
+    rq = np.einsum("ij,ij->j", x, LH @ x) / np.einsum("ij,ij->j", x, LG @ x)
+    assert rq.min() >= 1 - eps and rq.max() <= 1 + eps

@@ -193,3 +193,26 @@ def test_ordered_factor_preconditions_the_permuted_system():
     assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-7
     assert itp.max() <= itn.max() + 5
     assert f.info()["depth_fwd"] * 20 < fn.info()["depth_fwd"]
+
+
+def test_host_laplacian_matches_oracle():
+    """graphs.laplacian_csc (the Laplacian batches factor) is the oracle's
+    CSC Laplacian: same structure, same values (diagonal sums may differ in
+    the last bit only through summation order)."""
+    import oracle
+    from spec_spars_amd import graphs
+    oracle.load()
+    for n, u, v in (graphs.grid2d(9, 7), graphs.rgg(300, seed=1), graphs.chung_lu(400, 1500, seed=3)):
+        w = graphs.loguniform_weights(len(u), seed=2)
+        cp, ri, va = graphs.laplacian_csc(n, u, v, w)
+        # isolated vertices keep an explicit zero diagonal here (as on the
+        # device); the oracle drops explicit zeros
+        keep = va != 0.0
+        col = np.repeat(np.arange(n), np.diff(cp))[keep]
+        ri, va = ri[keep], va[keep]
+        cp = np.concatenate([[0], np.cumsum(np.bincount(col, minlength=n))]).astype(np.int32)
+        oc, orr, ov = oracle.laplacian(n, u, v, w)
+        assert np.array_equal(cp, oc) and np.array_equal(ri, orr)
+        assert np.allclose(va, ov, rtol=1e-14, atol=0)
+        off = ri != np.repeat(np.arange(n), np.diff(cp))
+        assert np.array_equal(va[off], ov[off])
