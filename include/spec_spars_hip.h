@@ -59,6 +59,7 @@ const char *ss_version(void);
 /* ---------------------------------------------------------------------- */
 /* Context: device + one HIP stream.                                       */
 int ss_context_create(int device, ss_context **out);
+/* Fails (SS_ERR_INVALID) while graphs created on the context still exist. */
 int ss_context_destroy(ss_context *ctx);
 /* Free / total device memory seen by this context (bytes). */
 int ss_context_mem_info(ss_context *ctx, size_t *free_b, size_t *total_b);
@@ -202,8 +203,10 @@ int ss_graph_sparsify(ss_graph *g, const ss_sparsify_opts *o,
                       ss_sparsify_report *rep);
 int ss_graph_read_kept(const ss_graph *g, int64_t cap, int32_t *ku,
                        int32_t *kv, double *kw, int64_t *kept);
-/* Sketch Y / solution Z of the last run as FP64 n x k (host). */
-int ss_graph_read_block(const ss_graph *g, int which /*0=Y,1=Z*/, double *out);
+/* Sketch Y / solution Z of the last run as FP64 n x k (host).  After an
+ * FP64 solve, which = 2 / 3 give the last preconditioner application's
+ * forward result L^{-1} r and its output L^{-T} D^+ L^{-1} r. */
+int ss_graph_read_block(const ss_graph *g, int which /*0=Y,1=Z,2,3*/, double *out);
 int ss_graph_read_probs(const ss_graph *g, double *p);
 
 /* ---------------------------------------------------------------------- */

@@ -4,6 +4,7 @@ Every call goes to libspecspars_hip.so; device work runs on the HIP device of
 the Context.  Host-only pieces (Factor) need no GPU.
 """
 import ctypes
+import weakref
 
 import numpy as np
 
@@ -81,6 +82,11 @@ class Factor:
     __del__ = close
 
 
+# blocks readable after a run: sketch, solution, and (FP64) the last
+# preconditioner application's forward result and output
+_BLOCKS = {"Y": 0, "Z": 1, "fwd": 2, "prec": 3}
+
+
 class Context:
     """A device and its HIP stream."""
 
@@ -89,9 +95,13 @@ class Context:
         h = ctypes.c_void_p()
         check(lib.ss_context_create(device, ctypes.byref(h)))
         self._h = h
+        self._children = weakref.WeakSet()   # graphs and batches on this stream
 
     def close(self):
+        """Destroy the context; graphs still open on it are closed first."""
         if getattr(self, "_h", None):
+            for child in list(self._children):
+                child.close()
             load().ss_context_destroy(self._h)
             self._h = None
 
@@ -173,6 +183,7 @@ class Graph:
                                      ctypes.byref(h)))
         self._h = h
         self.k = None
+        ctx._children.add(self)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -226,7 +237,7 @@ class Graph:
 
     def read_block(self, which):
         out = np.zeros((self.n, self.k), np.float64)
-        check(load().ss_graph_read_block(self._h, 0 if which == "Y" else 1, _p(out, ctypes.c_double)))
+        check(load().ss_graph_read_block(self._h, _BLOCKS[which], _p(out, ctypes.c_double)))
         return out
 
     def profile(self, enable=True):
@@ -274,6 +285,7 @@ class Batch:
                                      ctypes.byref(h)))
         self._h = h
         self.k = None
+        ctx._children.add(self)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -281,6 +293,11 @@ class Batch:
             self._h = None
 
     __del__ = close
+
+    def _index(self, i):
+        if not 0 <= i < self.count:
+            raise IndexError("graph %d outside the batch of %d" % (i, self.count))
+        return i
 
     def laplacian(self, i):
         """Host CSC Laplacian of graph i (built on the host from its edges)."""
@@ -328,6 +345,7 @@ class Batch:
                 "refinements": refin.value}
 
     def read_kept(self, i):
+        self._index(i)
         kept = ctypes.c_int64()
         cap = int(self.m[i])
         ku = np.zeros(max(cap, 1), np.int32)
@@ -339,11 +357,13 @@ class Batch:
         return ku[:c].copy(), kv[:c].copy(), kw[:c].copy()
 
     def read_block(self, i, which):
+        self._index(i)
         out = np.zeros((int(self.n[i]), self.k), np.float64)
-        check(load().ss_batch_read_block(self._h, i, 0 if which == "Y" else 1, _p(out, ctypes.c_double)))
+        check(load().ss_batch_read_block(self._h, i, _BLOCKS[which], _p(out, ctypes.c_double)))
         return out
 
     def read_probs(self, i):
+        self._index(i)
         p = np.zeros(int(self.m[i]), np.float64)
         check(load().ss_batch_read_probs(self._h, i, _p(p, ctypes.c_double)))
         return p

@@ -19,6 +19,11 @@ namespace ss {
 
 __device__ __forceinline__ int bt_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
+// Explicit fused multiply-adds: one rounding per term whatever the loop
+// layout (contraction only joins a multiply and an add in one basic block).
+__device__ __forceinline__ double bt_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float bt_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
 template <typename T>
 __device__ __forceinline__ T bt_dot_sub(T acc, int q0, int q1, const int32_t *__restrict__ col,
                                         const T *__restrict__ val, const T *X, int k, int j) {
@@ -31,9 +36,9 @@ __device__ __forceinline__ T bt_dot_sub(T acc, int q0, int q1, const int32_t *__
             v[u] = val[q + u];
         }
 #pragma unroll
-        for (int u = 0; u < 8; u++) acc -= v[u] * x[u];
+        for (int u = 0; u < 8; u++) acc = bt_fma(-v[u], x[u], acc);
     }
-    for (; q < q1; q++) acc -= val[q] * X[(size_t)col[q] * k + j];
+    for (; q < q1; q++) acc = bt_fma(-val[q], X[(size_t)col[q] * k + j], acc);
     return acc;
 }
 
@@ -95,11 +100,11 @@ __global__ __launch_bounds__(256) void kb_tile_partial(Tiles rt, int k, const TA
         if (j < k) {
             for (int r = r0 + BT_WAVE; r < r1; r += 4) {
                 const double a = (double)A[(size_t)r * k + j];
-                if (op == 0) acc += a * (double)B[(size_t)r * k + j];
+                if (op == 0) acc = bt_fma(a, (double)B[(size_t)r * k + j], acc);
                 else if (op == 1) acc += a;
                 else {
                     const double d = diag[r];
-                    acc += d != 0.0 ? a * (a / d) : a * a;
+                    acc = d != 0.0 ? bt_fma(a, a / d, acc) : bt_fma(a, a, acc);
                 }
             }
         }
@@ -260,7 +265,7 @@ __global__ __launch_bounds__(256) void kb_update_p(Tiles rt, int k, const double
         const T b = (T)beta[(size_t)g * k + j];
         for (int r = r0 + BT_WAVE; r < r1; r += 4) {
             const size_t o = (size_t)r * k + j;
-            P[o] = b == (T)0 ? Z[o] : Z[o] + b * P[o];
+            P[o] = b == (T)0 ? Z[o] : bt_fma(b, P[o], Z[o]);
         }
     }
 }
@@ -283,7 +288,7 @@ __global__ __launch_bounds__(256) void kb_spmm_dot(Tiles rt, const int32_t *__re
                 const T qv = -bt_dot_sub<T>((T)0, q0, q1, col, val, P, k, j);
                 const size_t o = (size_t)r * k + j;
                 Q[o] = qv;
-                acc += (double)P[o] * (double)qv;
+                acc = bt_fma((double)P[o], (double)qv, acc);
             }
         }
         bt_tile_store(acc, k, jb, partial, sm);
@@ -305,10 +310,10 @@ __global__ __launch_bounds__(256) void kb_update_xr_rr(Tiles rt, int k, const do
             const T a = (T)alpha[(size_t)g * k + j];
             for (int r = r0 + BT_WAVE; r < r1; r += 4) {
                 const size_t o = (size_t)r * k + j;
-                X[o] += a * P[o];
-                const T rv = R[o] - a * Q[o];
+                X[o] = bt_fma(a, P[o], X[o]);
+                const T rv = bt_fma(-a, Q[o], R[o]);
                 R[o] = rv;
-                acc += (double)rv * (double)rv;
+                acc = bt_fma((double)rv, (double)rv, acc);
             }
         }
         bt_tile_store(acc, k, jb, partial, sm);
@@ -333,7 +338,7 @@ __global__ __launch_bounds__(256) void kb_residual64(Tiles rt, const int32_t *__
                 const size_t o = (size_t)r * k + j;
                 const double v = bt_dot_sub<double>(B[o], q0, q1, col, val, X, k, j);
                 out[o] = v;
-                acc += v * v;
+                acc = bt_fma(v, v, acc);
             }
         }
         bt_tile_store(acc, k, jb, partial, sm);

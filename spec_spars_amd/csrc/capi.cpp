@@ -91,6 +91,8 @@ int ss_context_create(int device, ss_context **out) {
 
 int ss_context_destroy(ss_context *c) {
     if (!c) return SS_OK;
+    if (c->live_graphs > 0)
+        return fail(SS_ERR_INVALID, "context still has %d graph(s); destroy them first", c->live_graphs);
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     for (auto &ev : c->ev)
@@ -227,6 +229,7 @@ int ss_graph_destroy(ss_graph *g) {
     (void)hipSetDevice(g->ctx->device);
     (void)hipStreamSynchronize(g->ctx->stream);
     for (hipEvent_t e : g->ev_pool) (void)hipEventDestroy(e);
+    g->ctx->live_graphs--;
     delete g;
     return SS_OK;
 }
@@ -455,10 +458,12 @@ int ss_batch_read_kept(const ss_graph *g, int32_t i, int64_t cap, int32_t *ku, i
 static int read_block(const ss_graph *g, int32_t i, int which, double *out) {
     if (!out) return fail(SS_ERR_INVALID, "null argument");
     if (i < 0 || i >= g->G) return fail(SS_ERR_INVALID, "graph %d outside the batch of %d", i, g->G);
-    if (which != 0 && which != 1) return fail(SS_ERR_INVALID, "which must be 0 (Y) or 1 (Z)");
+    if (which < 0 || which > 3) return fail(SS_ERR_INVALID, "which must be 0 (Y), 1 (Z), 2 or 3");
     if (g->wk == 0) return fail(SS_ERR_INVALID, "nothing computed yet");
+    if (which >= 2 && g->wprec != SS_PREC_FP64)
+        return fail(SS_ERR_INVALID, "preconditioner blocks are readable after an FP64 solve only");
     SS_HIP(hipSetDevice(g->ctx->device));
-    const double *src = which == 0 ? g->Y64 : g->X64;
+    const double *src = which == 0 ? g->Y64 : which == 1 ? g->X64 : which == 2 ? g->w64.W : g->w64.Zp;
     const int64_t r0 = g->row_off[i], ni = g->row_off[i + 1] - r0;
     if (ni) SS_HIP(hipMemcpy(out, src + (size_t)r0 * g->wk, sizeof(double) * (size_t)ni * g->wk, hipMemcpyDeviceToHost));
     return SS_OK;

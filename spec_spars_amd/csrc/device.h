@@ -14,6 +14,7 @@ struct ss_context {
     hipStream_t stream = nullptr;
     hipEvent_t ev[8] = {};
     int *h_status = nullptr;      // pinned: {nactive, err}
+    int live_graphs = 0;          // graphs created on this context and not yet destroyed
 };
 
 namespace ss {
@@ -93,8 +94,7 @@ struct ss_graph {
     std::vector<int32_t> f_level_long, b_level_long; // long rows leading each level
     int32_t f_small_lo = 0, f_small_hi = 0, b_small_lo = 0, b_small_hi = 0;
     int32_t *f_split = nullptr;
-    int32_t *f_unit_ptr = nullptr, *f_unit_t = nullptr, *f_unit_s = nullptr;   // small-run work units
-    int32_t *b_unit_ptr = nullptr, *b_unit_t = nullptr, *b_unit_s = nullptr;
+    int32_t *f_lp_dev = nullptr, *b_lp_dev = nullptr;   // level_ptr on the device
     ss_schedule_info sched_info[2] = {};
     // status
     int *status = nullptr;           // {nactive, err}

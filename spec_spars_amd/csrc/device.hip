@@ -319,6 +319,7 @@ int batch_upload(ss_context *ctx, const std::vector<const HostGraph *> &graphs, 
     SS_HIP(hipStreamSynchronize(s));
     g->kept_start.assign((size_t)G, 0);
     g->kept_count.assign((size_t)G, 0);
+    ctx->live_graphs++;
     *out = g.release();
     return SS_OK;
 }
@@ -375,8 +376,14 @@ int batch_upload_factors(ss_graph *g, const std::vector<const Factor *> &factors
         Fp = &Fc;
     }
     const Factor &F = *Fp;
+    // each graph's own forward tail start fixes its rows' segments
+    std::vector<int32_t> tail_lo((size_t)F.n);
+    for (int i = 0; i < g->G; i++) {
+        const int32_t lo = forward_tail_start(*factors[i], kSmallRows);
+        for (int64_t p = g->row_off[i]; p < g->row_off[i + 1]; p++) tail_lo[p] = lo;
+    }
     Schedule fw, bw;
-    SS_TRY(factor_schedules(F, fw, bw, kLongRow));
+    SS_TRY(factor_schedules(F, fw, bw, &tail_lo, kLongRow, kSmallRows));
     // D per vertex (the factor keeps it per elimination position)
     std::vector<double> vdiag((size_t)F.n);
     for (int64_t p = 0; p < F.n; p++) vdiag[F.order[p]] = F.diag[p];
@@ -397,12 +404,8 @@ int batch_upload_factors(ss_graph *g, const std::vector<const Factor *> &factors
     SS_TRY(A.alloc(&g->b_valf, bnz));
     SS_TRY(A.alloc(&g->diag, n));
     SS_TRY(A.alloc(&g->f_split, n));
-    SS_TRY(A.alloc(&g->f_unit_ptr, fw.unit_ptr.size()));
-    SS_TRY(A.alloc(&g->f_unit_t, fw.unit_t.size()));
-    SS_TRY(A.alloc(&g->f_unit_s, fw.unit_s.size()));
-    SS_TRY(A.alloc(&g->b_unit_ptr, bw.unit_ptr.size()));
-    SS_TRY(A.alloc(&g->b_unit_t, bw.unit_t.size()));
-    SS_TRY(A.alloc(&g->b_unit_s, bw.unit_s.size()));
+    SS_TRY(A.alloc(&g->f_lp_dev, fw.level_ptr.size()));
+    SS_TRY(A.alloc(&g->b_lp_dev, bw.level_ptr.size()));
     hipStream_t s = g->ctx->stream;
     std::vector<float> fvf(fw.val.begin(), fw.val.end()), bvf(bw.val.begin(), bw.val.end());
     SS_TRY(h2d(s, g->f_rows, fw.rows.data(), n));
@@ -417,12 +420,8 @@ int batch_upload_factors(ss_graph *g, const std::vector<const Factor *> &factors
     SS_TRY(h2d(s, g->b_valf, bvf.data(), bnz));
     SS_TRY(h2d(s, g->diag, vdiag.data(), n));
     SS_TRY(h2d(s, g->f_split, fw.split.data(), n));
-    SS_TRY(h2d(s, g->f_unit_ptr, fw.unit_ptr.data(), fw.unit_ptr.size()));
-    SS_TRY(h2d(s, g->f_unit_t, fw.unit_t.data(), fw.unit_t.size()));
-    SS_TRY(h2d(s, g->f_unit_s, fw.unit_s.data(), fw.unit_s.size()));
-    SS_TRY(h2d(s, g->b_unit_ptr, bw.unit_ptr.data(), bw.unit_ptr.size()));
-    SS_TRY(h2d(s, g->b_unit_t, bw.unit_t.data(), bw.unit_t.size()));
-    SS_TRY(h2d(s, g->b_unit_s, bw.unit_s.data(), bw.unit_s.size()));
+    SS_TRY(h2d(s, g->f_lp_dev, fw.level_ptr.data(), fw.level_ptr.size()));
+    SS_TRY(h2d(s, g->b_lp_dev, bw.level_ptr.data(), bw.level_ptr.size()));
     SS_HIP(hipStreamSynchronize(s));
     g->f_level_ptr = fw.level_ptr;
     g->b_level_ptr = bw.level_ptr;
@@ -534,15 +533,16 @@ static int precond(ss_graph *g, int k, const T *R, T *W, T *Zp, double *rz_parti
         ProfScope ps(g, SS_KERNEL_TRI_FWD, g->f_small_lo);
         for (int32_t l = 0; l < g->f_small_lo; l++)
             launch_trisolve_level<T, false>(s, g->f_level_ptr[l], g->f_level_long[l], g->f_level_ptr[l + 1],
-                                            g->f_rows, g->f_ptr, g->f_col, fvals<T>(g), g->diag, k, R, W);
+                                            g->f_rows, g->f_ptr, g->f_split, g->f_col, fvals<T>(g), g->diag, k, R,
+                                            W);
     }
     {
         ProfScope ps(g, SS_KERNEL_TRI_FWD_SMALL, g->f_small_lo < g->f_small_hi ? 2 : 0);
         if (g->f_small_lo < g->f_small_hi) {
             launch_tri_tail_head<T>(s, g->f_level_ptr[g->f_small_lo], (int32_t)g->N, g->f_rows, g->f_ptr,
                                     g->f_split, g->f_col, fvals<T>(g), k, R, W);
-            launch_tri_small<T, false>(s, g->f_small_lo, g->f_small_hi, g->f_unit_ptr, g->f_unit_t, g->f_unit_s,
-                                       g->f_rows, g->f_ptr, g->f_split, g->f_col, fvals<T>(g), g->diag, k, R, W);
+            launch_tri_small<T, false>(s, g->f_small_lo, g->f_small_hi, g->f_lp_dev, g->f_rows, g->f_ptr,
+                                       g->f_split, g->f_col, fvals<T>(g), g->diag, k, R, W);
         }
     }
     {
@@ -551,15 +551,15 @@ static int precond(ss_graph *g, int k, const T *R, T *W, T *Zp, double *rz_parti
     }
     if (g->b_small_lo < g->b_small_hi) {
         ProfScope ps(g, SS_KERNEL_TRI_BWD_SMALL);
-        launch_tri_small<T, true>(s, g->b_small_lo, g->b_small_hi, g->b_unit_ptr, g->b_unit_t, g->b_unit_s,
-                                  g->b_rows, g->b_ptr, nullptr, g->b_col, bvals<T>(g), g->diag, k, W, Zp);
+        launch_tri_small<T, true>(s, g->b_small_lo, g->b_small_hi, g->b_lp_dev, g->b_rows, g->b_ptr, nullptr,
+                                  g->b_col, bvals<T>(g), g->diag, k, W, Zp);
     }
     {
         const int32_t nlev = (int32_t)g->b_level_ptr.size() - 1;
         ProfScope ps(g, SS_KERNEL_TRI_BWD, nlev - g->b_small_hi);
         for (int32_t l = g->b_small_hi; l < nlev; l++)
             launch_trisolve_level<T, true>(s, g->b_level_ptr[l], g->b_level_long[l], g->b_level_ptr[l + 1],
-                                           g->b_rows, g->b_ptr, g->b_col, bvals<T>(g), g->diag, k, W, Zp);
+                                           g->b_rows, g->b_ptr, nullptr, g->b_col, bvals<T>(g), g->diag, k, W, Zp);
     }
     SS_HIP(hipGetLastError());
     return SS_OK;

@@ -347,7 +347,16 @@ void factor_levels(Factor &F) {
 // pull form (the vertices it waits for).  The backward schedule holds the
 // rows of L^T, i.e. the off-diagonal part of U's rows.  Every dependency
 // must have a strictly lower level -- the sync-free solve relies on it.
-int factor_schedules(const Factor &F, Schedule &fwd, Schedule &bwd, int32_t long_row, int32_t small_rows) {
+int32_t forward_tail_start(const Factor &F, int32_t small_rows) {
+    std::vector<int64_t> cnt((size_t)F.depth_fwd + 1, 0);
+    for (int64_t p = 0; p < F.n; p++) cnt[F.lvl_fwd[p]]++;
+    int32_t lo = F.depth_fwd;   // 0-based levels [lo, depth) form the run
+    while (lo > 0 && cnt[lo] <= small_rows) lo--;
+    return lo;
+}
+
+int factor_schedules(const Factor &F, Schedule &fwd, Schedule &bwd, const std::vector<int32_t> *tail_lo,
+                     int32_t long_row, int32_t small_rows) {
     const int64_t n = F.n;
     // entries per position: forward = row of L (count of U rows naming it),
     // backward = U's off-diagonal row
@@ -452,39 +461,31 @@ int factor_schedules(const Factor &F, Schedule &fwd, Schedule &bwd, int32_t long
     bwd.small_lo = 0;
     bwd.small_hi = 0;
     while (bwd.small_hi < bwd.depth && small(bwd, bwd.small_hi)) bwd.small_hi++;
-    // forward tail rows: the leading entries whose columns sit in head
-    // levels (1-based level <= small_lo) are done before the tail starts
+    // forward segments: a row of its graph's tail run (1-based level > the
+    // graph's tail start lo) splits after the entries whose columns sit in
+    // levels <= lo; entries are sorted by column level, so these lead
+    const int32_t own_lo = tail_lo ? 0 : forward_tail_start(F, small_rows);
+    std::vector<int32_t> vlvl((size_t)n), vlo((size_t)n);
+    for (int64_t p = 0; p < n; p++) {
+        vlvl[F.order[p]] = F.lvl_fwd[p];
+        vlo[F.order[p]] = tail_lo ? (*tail_lo)[p] : own_lo;
+    }
     fwd.split.assign((size_t)n, 0);
-    std::vector<int32_t> vlvl((size_t)n);
-    for (int64_t p = 0; p < n; p++) vlvl[F.order[p]] = F.lvl_fwd[p];
     for (int64_t t = 0; t < n; t++) {
         const int32_t b = fwd.ptr[t], e = fwd.ptr[t + 1];
-        int32_t q = b;
-        if (t >= fwd.level_ptr[fwd.small_lo])
-            while (q < e && vlvl[fwd.col[q]] <= fwd.small_lo) q++;
-        else
-            q = e;
+        const int32_t r = fwd.rows[t];
+        int32_t q = e;
+        if (vlvl[r] > vlo[r]) {
+            q = b;
+            while (q < e && vlvl[fwd.col[q]] <= vlo[r]) q++;
+        }
         fwd.split[t] = q - b;
     }
-    // work units of the small runs
-    auto parts = [&](int32_t len) { return len <= long_row ? 1 : (len <= 16 * long_row ? 4 : 16); };
-    for (Schedule *S : {&fwd, &bwd}) {
-        S->unit_ptr.assign(1, 0);
-        S->unit_t.clear();
-        S->unit_s.clear();
-        for (int32_t l = S->small_lo; l < S->small_hi; l++) {
-            for (int want : {16, 4, 1})
-                for (int32_t t = S->level_ptr[l]; t < S->level_ptr[l + 1]; t++) {
-                    const int P = parts(S->ptr[t + 1] - S->ptr[t]);
-                    if (P != want) continue;
-                    for (int sp = 0; sp < P; sp++) {
-                        S->unit_t.push_back(t);
-                        S->unit_s.push_back(sp);
-                    }
-                }
-            S->unit_ptr.push_back((int32_t)S->unit_t.size());
-        }
-    }
+    // the small run only holds rows whose first segment is finished before
+    // it starts: a row of the run lies in its graph's own tail run
+    for (int64_t t = fwd.level_ptr[fwd.small_lo]; t < n; t++)
+        if (vlvl[fwd.rows[t]] <= vlo[fwd.rows[t]])
+            return fail(SS_ERR_INVALID, "schedule: small-level run starts before a graph's own tail");
     return SS_OK;
 }
 

@@ -78,26 +78,25 @@ void launch_copy_f64(hipStream_t s, int64_t count, const double *a, double *b);
 // Upper bound of the block count of every kernel that writes per-block
 // partials (partial buffers hold kMaxPartialBlocks * k doubles).
 static const int kMaxPartialBlocks = 4096;
-// Rows of one level, positions [t0, t1); the first nlong of them are long
-// rows (row_parts > 1) processed by a whole block each.
+// Rows of one level, positions [t0, t1); the first nlong of them are rows
+// of more than kLongRow entries (a block each).  Forward rows are split in
+// two segments at split[t] (backward: split unused, may be NULL).
 template <typename T, bool BWD>
 void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nlong, int32_t t1, const int32_t *rows,
-                           const int32_t *ptr, const int32_t *col, const T *val, const double *diag, int k,
-                           const T *in, T *out);
+                           const int32_t *ptr, const int32_t *split, const int32_t *col, const T *val,
+                           const double *diag, int k, const T *in, T *out);
 template <typename T>
 void launch_spmm_dot(hipStream_t s, int32_t n, const int32_t *rowptr, const int32_t *col, const T *val, int k,
                      const T *P, T *Q, double *partial);
 template <typename T>
 void launch_tri_tail_head(hipStream_t s, int32_t t0, int32_t t1, const int32_t *rows, const int32_t *ptr,
                           const int32_t *split, const int32_t *col, const T *val, int k, const T *in, T *out);
-// Small levels [l0, l1) walked by one workgroup.  Level l's work units are
-// [unit_ptr[l - l0], unit_ptr[l - l0 + 1]): unit u is part unit_s[u] of the
-// row at schedule position unit_t[u]; a row of P parts has P consecutive
-// units, 16-part rows first, then 4-part rows, then 1-part rows.
+// Small levels [l0, l1) (each of at most kSmallRows rows) walked by one
+// workgroup; level l holds positions [level_ptr[l], level_ptr[l+1]).
 template <typename T, bool BWD>
-void launch_tri_small(hipStream_t s, int32_t l0, int32_t l1, const int32_t *unit_ptr, const int32_t *unit_t,
-                      const int32_t *unit_s, const int32_t *rows, const int32_t *ptr, const int32_t *split,
-                      const int32_t *col, const T *val, const double *diag, int k, const T *in, T *out);
+void launch_tri_small(hipStream_t s, int32_t l0, int32_t l1, const int32_t *level_ptr, const int32_t *rows,
+                      const int32_t *ptr, const int32_t *split, const int32_t *col, const T *val, const double *diag,
+                      int k, const T *in, T *out);
 template <typename T>
 void launch_ydy_partial(hipStream_t s, int32_t n, int k, const T *Yv, const double *diag, double *partial);
 template <typename T>

@@ -17,6 +17,13 @@ namespace ss {
 
 __device__ __forceinline__ int wave_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
+// acc - v * x with one rounding.  Every accumulate of the solves is an
+// explicit fused multiply-add: the compiler only contracts a multiply and a
+// subtract that land in one basic block, so leaving it to contraction would
+// round some terms twice depending on how a loop was laid out.
+__device__ __forceinline__ double fnms(double v, double x, double acc) { return __builtin_fma(-v, x, acc); }
+__device__ __forceinline__ float fnms(float v, float x, float acc) { return __builtin_fmaf(-v, x, acc); }
+
 // acc -= sum_q val[q] * X[col[q]*k + j] over q in [q0, q1) with stride
 // `step`, 8 gathers in flight, terms subtracted in q order.
 template <typename T>
@@ -31,9 +38,9 @@ __device__ __forceinline__ T row_dot_sub(T acc, int q0, int q1, int step, const 
             v[u] = val[q + u * step];
         }
 #pragma unroll
-        for (int u = 0; u < 8; u++) acc -= v[u] * x[u];
+        for (int u = 0; u < 8; u++) acc = fnms(v[u], x[u], acc);
     }
-    for (; q < q1; q += step) acc -= val[q] * X[(size_t)col[q] * k + j];
+    for (; q < q1; q += step) acc = fnms(val[q], X[(size_t)col[q] * k + j], acc);
     return acc;
 }
 
@@ -420,39 +427,133 @@ __device__ __forceinline__ T row_start(int i, int j, int k, double d, const T *i
     return acc;
 }
 
-// Fold of the parts a block or workgroup left in sm[first .. first+P-1].
-template <typename T>
-__device__ __forceinline__ T fold_parts(const T (*sm)[64], int first, int P, int lane) {
-    T comb = sm[first][lane];
-    for (int s = 1; s < P; s++) comb += sm[first + s][lane];
-    return comb;
+// Pairwise tree fold of v[0..N) in place; the result lands in v[0].
+template <typename T, int N>
+__device__ __forceinline__ void tree_fold(T (&v)[N]) {
+#pragma unroll
+    for (int step = 1; step < N; step *= 2)
+#pragma unroll
+        for (int u = 0; u + step < N; u += 2 * step) v[u] = v[u] + v[u + step];
 }
 
+// Tree fold of parts [s0, s0 + NA) of segment [a, b) split in P parts (part
+// s: running subtraction from zero over q = a + s, a + s + P, ...).  NA
+// accumulators run side by side with 8 gathers in flight per round.
+template <typename T, int NA>
+__device__ __forceinline__ T seg_subtree(int a, int b, int P, int s0, const int32_t *__restrict__ col,
+                                         const T *__restrict__ val, const T *X, int k, int j) {
+    static_assert(NA >= 1 && NA <= 8, "at most 8 parts per wave");
+    constexpr int U = 8 / NA;                 // chunks per round
+    T acc[NA];
+#pragma unroll
+    for (int u = 0; u < NA; u++) acc[u] = (T)0;
+    for (int c = a + s0; c < b; c += U * P) {
+        T x[U][NA], v[U][NA];
+#pragma unroll
+        for (int r = 0; r < U; r++)
+#pragma unroll
+            for (int u = 0; u < NA; u++) {
+                const int q = c + r * P + u;
+                const bool in = q < b;
+                v[r][u] = in ? val[q] : (T)0;
+                x[r][u] = in ? X[(size_t)col[q] * k + j] : (T)0;
+            }
+#pragma unroll
+        for (int r = 0; r < U; r++)
+#pragma unroll
+            for (int u = 0; u < NA; u++)
+                if (c + r * P + u < b) acc[u] = fnms(v[r][u], x[r][u], acc[u]);
+    }
+    tree_fold<T, NA>(acc);
+    return acc[0];
+}
+
+// Tree fold of parts [s0, s0 + na) with na in {1, 2, 4, 8}.
+template <typename T>
+__device__ __forceinline__ T seg_subtree_n(int na, int a, int b, int P, int s0, const int32_t *__restrict__ col,
+                                           const T *__restrict__ val, const T *X, int k, int j) {
+    switch (na) {
+        case 1: return seg_subtree<T, 1>(a, b, P, s0, col, val, X, k, j);
+        case 2: return seg_subtree<T, 2>(a, b, P, s0, col, val, X, k, j);
+        case 4: return seg_subtree<T, 4>(a, b, P, s0, col, val, X, k, j);
+        default: return seg_subtree<T, 8>(a, b, P, s0, col, val, X, k, j);
+    }
+}
+
+// acc updated by segment [a, b), every part on this wave.
+template <typename T>
+__device__ __forceinline__ T seg_apply(T acc, int a, int b, const int32_t *__restrict__ col,
+                                       const T *__restrict__ val, const T *X, int k, int j) {
+    const int P = seg_parts(b - a);
+    if (P == 1) return row_dot_sub<T>(acc, a, b, 1, col, val, X, k, j);
+    return acc + (P == 4 ? seg_subtree<T, 4>(a, b, 4, 0, col, val, X, k, j)
+                         : seg_subtree<T, 8>(a, b, 8, 0, col, val, X, k, j));
+}
+
+// Tree fold of sm[first .. first + N) (column lane).
+template <typename T, int N>
+__device__ __forceinline__ T lds_tree_n(T (*sm)[64], int first, int lane) {
+    T v[N];
+#pragma unroll
+    for (int u = 0; u < N; u++) v[u] = sm[first + u][lane];
+    tree_fold<T, N>(v);
+    return v[0];
+}
+template <typename T>
+__device__ __forceinline__ T lds_tree(T (*sm)[64], int first, int n, int lane) {
+    switch (n) {
+        case 1: return sm[first][lane];
+        case 2: return lds_tree_n<T, 2>(sm, first, lane);
+        case 4: return lds_tree_n<T, 4>(sm, first, lane);
+        default: return lds_tree_n<T, 8>(sm, first, lane);
+    }
+}
+
+// acc updated by segment [a, b) on the g waves of a block or workgroup that
+// share the row (g a power of two; this wave is number sub of them, they
+// occupy sm[first .. first+g)); the result is valid on sub 0.  Every wave of
+// the block calls this (it holds barriers).
+template <typename T>
+__device__ __forceinline__ T seg_apply_shared(T acc, int a, int b, bool act, int g, int sub, int first, T (*sm)[64],
+                                              const int32_t *__restrict__ col, const T *__restrict__ val,
+                                              const T *X, int k, int j, int lane) {
+    const int P = act ? seg_parts(b - a) : 1;
+    const int gg = P < g ? P : g;             // waves that hold parts
+    if (act && P == 1 && sub == 0 && j < k) acc = row_dot_sub<T>(acc, a, b, 1, col, val, X, k, j);
+    if (act && P > 1 && sub < gg)
+        sm[first + sub][lane] = j < k ? seg_subtree_n<T>(P / gg, a, b, P, sub * (P / gg), col, val, X, k, j) : (T)0;
+    __syncthreads();
+    if (act && P > 1 && sub == 0) acc = acc + lds_tree<T>(sm, first, gg, lane);
+    __syncthreads();
+    return acc;
+}
+
+// One level of a triangular solve: schedule positions [t0, t1), the first
+// nlong of them rows of more than kLongRow entries (a block of 4 waves
+// each), the rest a wave each.  Forward rows are split at split[t].
 template <typename T, bool BWD>
 __global__ __launch_bounds__(256) void k_trisolve_level(int32_t t0, int32_t nlong, int32_t t1,
                                                         const int32_t *__restrict__ rows,
                                                         const int32_t *__restrict__ ptr,
+                                                        const int32_t *__restrict__ split,
                                                         const int32_t *__restrict__ col,
                                                         const T *__restrict__ val,
                                                         const double *__restrict__ diag, int k,
                                                         const T *__restrict__ in, T *__restrict__ out) {
     const int lane = SS_LANE, wv = SS_WAVE_IN_BLOCK;
     if ((int)blockIdx.x < nlong) {
-        // a long row: its P parts go round the block's 4 waves, wave 0 folds
-        __shared__ T sm[16][64];
+        __shared__ T sm[4][64];
         const int t = t0 + blockIdx.x;
         const int i = wave_uniform(rows[t]);
         const int q0 = wave_uniform(ptr[t]), q1 = wave_uniform(ptr[t + 1]);
-        const int P = row_parts(q1 - q0);
+        const int qs = BWD ? q1 : q0 + wave_uniform(split[t]);
         const double d = BWD ? diag[i] : 1.0;
         for (int jb = 0; jb < k; jb += 64) {
             const int j = jb + lane;
-            for (int sp = wv; sp < P; sp += 4)
-                sm[sp][lane] = j < k ? row_dot_sub<T>((T)0, q0 + sp, q1, P, col, val, out, k, j) : (T)0;
-            __syncthreads();
-            if (wv == 0 && j < k)
-                out[(size_t)i * k + j] = row_start<T, BWD>(i, j, k, d, in) + fold_parts<T>(sm, 0, P, lane);
-            __syncthreads();
+            T acc = (wv == 0 && j < k) ? row_start<T, BWD>(i, j, k, d, in) : (T)0;
+            acc = seg_apply_shared<T>(acc, q0, qs, true, 4, wv, 0, sm, col, val, out, k, j, lane);
+            if (!BWD) acc = seg_apply_shared<T>(acc, qs, q1, true, 4, wv, 0, sm, col, val, out, k, j, lane);
+            if (wv == 0 && j < k) out[(size_t)i * k + j] = acc;
         }
         return;
     }
@@ -460,20 +561,20 @@ __global__ __launch_bounds__(256) void k_trisolve_level(int32_t t0, int32_t nlon
     if (t >= t1) return;
     const int i = wave_uniform(rows[t]);
     const int q0 = wave_uniform(ptr[t]), q1 = wave_uniform(ptr[t + 1]);
+    const int qs = BWD ? q1 : q0 + wave_uniform(split[t]);
     const double d = BWD ? diag[i] : 1.0;
     for (int jb = 0; jb < k; jb += 64) {
         const int j = jb + lane;
         if (j >= k) break;
-        out[(size_t)i * k + j] = row_dot_sub<T>(row_start<T, BWD>(i, j, k, d, in), q0, q1, 1, col, val, out, k, j);
+        T acc = seg_apply<T>(row_start<T, BWD>(i, j, k, d, in), q0, qs, col, val, out, k, j);
+        if (!BWD) acc = seg_apply<T>(acc, qs, q1, col, val, out, k, j);
+        out[(size_t)i * k + j] = acc;
     }
 }
 
-// Forward tail, step 1: for every one-part row of the trailing run of small
-// levels, the running subtraction over the entries whose columns the head
-// levels finished (the first split[t]; a row's entries are sorted by the
-// level of their column, so these lead): out_i = r_i - those terms.  The
-// small-level kernel continues the same running subtraction, so the row's
-// bits equal a single pass.  Rows of several parts are left to it whole.
+// Forward tail, step 1: the first segment of every row of the trailing run
+// of small levels (its columns were all finished by the head levels):
+// out_i = r_i updated by entries [ptr[t], ptr[t] + split[t]).  Wave per row.
 template <typename T>
 __global__ __launch_bounds__(256) void k_tri_tail_head(int32_t t0, int32_t t1, const int32_t *__restrict__ rows,
                                                        const int32_t *__restrict__ ptr,
@@ -483,28 +584,24 @@ __global__ __launch_bounds__(256) void k_tri_tail_head(int32_t t0, int32_t t1, c
     const int t = wave_uniform(t0 + blockIdx.x * 4 + SS_WAVE_IN_BLOCK);
     if (t >= t1) return;
     const int i = wave_uniform(rows[t]);
-    const int q0 = wave_uniform(ptr[t]), q1 = wave_uniform(ptr[t + 1]);
-    if (row_parts(q1 - q0) != 1) return;
+    const int q0 = wave_uniform(ptr[t]);
     const int qs = q0 + wave_uniform(split[t]);
     for (int jb = 0; jb < k; jb += 64) {
         const int j = jb + SS_LANE;
         if (j >= k) break;
-        out[(size_t)i * k + j] = row_dot_sub<T>(in[(size_t)i * k + j], q0, qs, 1, col, val, out, k, j);
+        out[(size_t)i * k + j] = seg_apply<T>(in[(size_t)i * k + j], q0, qs, col, val, out, k, j);
     }
 }
 
 // A run of small levels [l0, l1) walked by one workgroup of 16 waves with a
 // workgroup barrier between levels (all waves share one CU, so the barrier's
 // workgroup-scope fence orders the row stores before the next level's
-// loads).  Each level's work units go out 16 at a time, one per wave; a
-// row's parts never straddle two groups of 16, and the wave holding part 0
-// folds them.  One-part rows: forward continues from the head part
-// k_tri_tail_head left in out (entries split..end), backward runs all
-// entries from y_i / D_i.
+// loads).  A level of r rows gives each row g = 16 / r waves (rounded down
+// to a power of two), which split the row's parts between them.  Forward:
+// the second segment, continuing from the first that k_tri_tail_head left
+// in out; backward: the whole row from y_i / D_i.
 template <typename T, bool BWD>
-__global__ __launch_bounds__(1024) void k_tri_small(int32_t l0, int32_t l1, const int32_t *__restrict__ unit_ptr,
-                                                    const int32_t *__restrict__ unit_t,
-                                                    const int32_t *__restrict__ unit_s,
+__global__ __launch_bounds__(1024) void k_tri_small(int32_t l0, int32_t l1, const int32_t *__restrict__ level_ptr,
                                                     const int32_t *__restrict__ rows,
                                                     const int32_t *__restrict__ ptr,
                                                     const int32_t *__restrict__ split,
@@ -514,37 +611,24 @@ __global__ __launch_bounds__(1024) void k_tri_small(int32_t l0, int32_t l1, cons
     __shared__ T sm[16][64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int l = l0; l < l1; l++) {
-        const int u0 = unit_ptr[l - l0], u1 = unit_ptr[l - l0 + 1];
-        for (int ub = u0; ub < u1; ub += 16) {
-            const int u = ub + wv;
-            const bool act = u < u1;
-            const int t = act ? wave_uniform(unit_t[u]) : 0;
-            const int sp = act ? wave_uniform(unit_s[u]) : 0;
-            const int i = act ? wave_uniform(rows[t]) : 0;
-            const int q0 = act ? wave_uniform(ptr[t]) : 0, q1 = act ? wave_uniform(ptr[t + 1]) : 0;
-            const int P = act ? row_parts(q1 - q0) : 1;
-            const double d = (BWD && act) ? diag[i] : 1.0;
-            for (int jb = 0; jb < k; jb += 64) {
-                const int j = jb + lane;
-                if (P == 1) {
-                    if (act && j < k) {
-                        if (BWD) {
-                            out[(size_t)i * k + j] =
-                                row_dot_sub<T>(row_start<T, true>(i, j, k, d, in), q0, q1, 1, col, val, out, k, j);
-                        } else {
-                            const int qs = q0 + wave_uniform(split[t]);
-                            out[(size_t)i * k + j] =
-                                row_dot_sub<T>(out[(size_t)i * k + j], qs, q1, 1, col, val, out, k, j);
-                        }
-                    }
-                } else {
-                    sm[wv][lane] = (act && j < k) ? row_dot_sub<T>((T)0, q0 + sp, q1, P, col, val, out, k, j) : (T)0;
-                }
-                __syncthreads();
-                if (act && P > 1 && sp == 0 && j < k)
-                    out[(size_t)i * k + j] = row_start<T, BWD>(i, j, k, d, in) + fold_parts<T>(sm, wv, P, lane);
-                __syncthreads();
-            }
+        const int tb = level_ptr[l], te = level_ptr[l + 1];
+        const int r = te - tb;                  // r <= kSmallRows == 16
+        if (r <= 0) continue;
+        int g = 16 / r;
+        g = 1 << (31 - __builtin_clz(g));
+        const int slot = wv / g, sub = wv - slot * g;
+        const int t = tb + slot;
+        const bool act = t < te;
+        const int i = act ? wave_uniform(rows[t]) : 0;
+        const int q0 = act ? wave_uniform(ptr[t]) : 0, q1 = act ? wave_uniform(ptr[t + 1]) : 0;
+        const int qs = BWD ? q0 : (act ? q0 + wave_uniform(split[t]) : 0);
+        const double d = (BWD && act) ? diag[i] : 1.0;
+        for (int jb = 0; jb < k; jb += 64) {
+            const int j = jb + lane;
+            T acc = (T)0;
+            if (act && sub == 0 && j < k) acc = BWD ? row_start<T, true>(i, j, k, d, in) : out[(size_t)i * k + j];
+            acc = seg_apply_shared<T>(acc, qs, q1, act, g, sub, slot * g, sm, col, val, out, k, j, lane);
+            if (act && sub == 0 && j < k) out[(size_t)i * k + j] = acc;
         }
         __syncthreads();
     }
@@ -841,17 +925,17 @@ template void launch_tri_tail_head<float>(hipStream_t, int32_t, int32_t, const i
                                           const int32_t *, const int32_t *, const float *, int, const float *, float *);
 
 template <typename T, bool BWD>
-void launch_tri_small(hipStream_t s, int32_t l0, int32_t l1, const int32_t *unit_ptr, const int32_t *unit_t,
-                      const int32_t *unit_s, const int32_t *rows, const int32_t *ptr, const int32_t *split,
-                      const int32_t *col, const T *val, const double *diag, int k, const T *in, T *out) {
+void launch_tri_small(hipStream_t s, int32_t l0, int32_t l1, const int32_t *level_ptr, const int32_t *rows,
+                      const int32_t *ptr, const int32_t *split, const int32_t *col, const T *val, const double *diag,
+                      int k, const T *in, T *out) {
     if (l1 <= l0) return;
-    hipLaunchKernelGGL((k_tri_small<T, BWD>), dim3(1), dim3(1024), 0, s, l0, l1, unit_ptr, unit_t, unit_s, rows, ptr,
-                       split, col, val, diag, k, in, out);
+    hipLaunchKernelGGL((k_tri_small<T, BWD>), dim3(1), dim3(1024), 0, s, l0, l1, level_ptr, rows, ptr, split, col, val,
+                       diag, k, in, out);
 }
 #define SS_INST_SMALL(T, B)                                                                                       \
     template void launch_tri_small<T, B>(hipStream_t, int32_t, int32_t, const int32_t *, const int32_t *,          \
-                                         const int32_t *, const int32_t *, const int32_t *, const int32_t *,        \
-                                         const int32_t *, const T *, const double *, int, const T *, T *);
+                                         const int32_t *, const int32_t *, const int32_t *, const T *,              \
+                                         const double *, int, const T *, T *);
 SS_INST_SMALL(double, false)
 SS_INST_SMALL(double, true)
 SS_INST_SMALL(float, false)
@@ -870,17 +954,17 @@ template void launch_update_xr_rr<float>(hipStream_t, int32_t, int, const double
 
 template <typename T, bool BWD>
 void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nlong, int32_t t1, const int32_t *rows,
-                           const int32_t *ptr, const int32_t *col, const T *val, const double *diag, int k,
-                           const T *in, T *out) {
+                           const int32_t *ptr, const int32_t *split, const int32_t *col, const T *val,
+                           const double *diag, int k, const T *in, T *out) {
     if (t1 <= t0) return;
     const unsigned grid = (unsigned)nlong + grid_for(t1 - t0 - nlong, 4);
-    hipLaunchKernelGGL((k_trisolve_level<T, BWD>), dim3(grid), dim3(256), 0, s, t0, nlong, t1, rows, ptr, col, val,
-                       diag, k, in, out);
+    hipLaunchKernelGGL((k_trisolve_level<T, BWD>), dim3(grid), dim3(256), 0, s, t0, nlong, t1, rows, ptr, split, col,
+                       val, diag, k, in, out);
 }
 #define SS_INST_TRI(T, B)                                                                                      \
     template void launch_trisolve_level<T, B>(hipStream_t, int32_t, int32_t, int32_t, const int32_t *,          \
-                                              const int32_t *, const int32_t *, const T *, const double *, int,  \
-                                              const T *, T *);
+                                              const int32_t *, const int32_t *, const int32_t *, const T *,      \
+                                              const double *, int, const T *, T *);
 SS_INST_TRI(double, false)
 SS_INST_TRI(double, true)
 SS_INST_TRI(float, false)

@@ -1,19 +1,31 @@
 // Summation shape of one row of a level-scheduled triangular solve.
 //
-// A row's entries are summed in P parts: entry q of a row whose entries
-// start at q0 goes to part (q - q0) mod P, each part is a running
-// subtraction from zero in entry order, the parts are added left to right
-// and the result is added to the row's start value.  P = 1 is a single
-// running subtraction from the start value.  P depends on the row's length
-// alone, so the per-level kernels and the single-workgroup small-level
-// kernels give a row the same bits whichever of them finishes it, and a
-// graph's solve does not change when other graphs share its launches.
+// A row's entries are sorted by the level of their column.  A forward row
+// of a graph's trailing run of small levels has two segments: the leading
+// entries whose columns sit in the graph's head levels, then the rest;
+// every other row is one segment.  The boundary depends on the graph alone
+// (its own small-level run), not on other graphs sharing the launches.
+//
+// Each segment [a, b) updates the running value acc (the row's start value
+// first): with P = seg_parts(b - a) == 1, acc -= val[q] * x[col[q]] for q in
+// order; otherwise part s (0 <= s < P) is a running subtraction from zero
+// over q = a + s, a + s + P, ..., the P parts are folded as a balanced
+// pairwise tree (fold(lo..hi) = fold(left half) + fold(right half)) and
+// acc += that fold.
+//
+// A shape that depends only on the row means any kernel may finish it --
+// a wave holding every part in registers, a block of 4 waves, or the waves
+// of the small-level kernel sharing a row -- with the same bits.
 #pragma once
 
 namespace ss {
 
-constexpr int kLongRow = 64;   // rows longer than this are summed in parts
+constexpr int kLongRow = 64;      // rows longer than this get a block of 4 waves per level launch
+constexpr int kSmallRows = 16;    // levels with at most this many rows go to the small-level kernel
 
-constexpr int row_parts(int len) { return len <= kLongRow ? 1 : (len <= 16 * kLongRow ? 4 : 16); }
+static_assert(kSmallRows == 16, "the small-level kernel runs 16 waves, at most one row each");
+
+constexpr int seg_parts(int len) { return len <= 8 ? 1 : (len <= 32 ? 4 : 8); }
+constexpr int kMaxParts = 8;
 
 }  // namespace ss

@@ -72,11 +72,10 @@ struct Schedule {
     // entries of earlier (head) levels first, split[t] of them.  Backward:
     // the leading run.
     int32_t small_lo = 0, small_hi = 0;
-    std::vector<int32_t> split;       // per position (forward only)
-    // Work units of the small-level run: level small_lo + i owns units
-    // [unit_ptr[i], unit_ptr[i+1]); unit u is part unit_s[u] of the row at
-    // position unit_t[u] (rows of 16 parts first, then 4, then 1).
-    std::vector<int32_t> unit_ptr, unit_t, unit_s;
+    // Forward rows: entries [ptr[t], ptr[t] + split[t]) form the first
+    // segment (the columns in the head levels of the row's graph), the rest
+    // the second; split[t] = row length outside the graph's own tail run.
+    std::vector<int32_t> split;
     std::vector<int32_t> ptr;         // entries of position t: [ptr[t], ptr[t+1])
     std::vector<int32_t> col;
     std::vector<double> val;
@@ -85,12 +84,17 @@ struct Schedule {
 int factor_build(int64_t n, const int32_t *colptr, const int32_t *rowidx, const double *vals,
                  uint64_t seed, int ordering, Factor &F);
 void factor_levels(Factor &F);
+// Small-level run of one graph's forward schedule: the first 0-based level
+// of its trailing run of levels with at most small_rows rows.
+int32_t forward_tail_start(const Factor &F, int32_t small_rows);
 // Level-ordered schedules in vertex ids; checks that every dependency sits
 // at a strictly lower level.  Forward rows list their entries sorted by the
-// level of the column (stable), so the entries finished before any level
-// lead the row.  long_row must be the kLongRow the solve kernels use.
-int factor_schedules(const Factor &F, Schedule &fwd, Schedule &bwd, int32_t long_row = 64,
-                     int32_t small_rows = 16);
+// level of the column (stable), so the entries of earlier levels lead.
+// tail_lo (per position, may be NULL = forward_tail_start(F) everywhere)
+// is the tail start of the position's own graph, which fixes the segments.
+// long_row and small_rows must be the kLongRow / kSmallRows of the kernels.
+int factor_schedules(const Factor &F, Schedule &fwd, Schedule &bwd, const std::vector<int32_t> *tail_lo,
+                     int32_t long_row, int32_t small_rows);
 
 // ---- host graph structure -----------------------------------------------
 // Canonical edges (u < v, sorted by (v, u)) -> full Laplacian CSR with the

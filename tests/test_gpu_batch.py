@@ -188,10 +188,12 @@ def test_batch_argument_errors(ctx):
     with pytest.raises(ss.SSError):
         B.set_factors(fs[:1])
     B.set_factors(fs)
-    with pytest.raises(ss.SSError):
+    from spec_spars_amd import _lib
+    dummy = np.zeros(8)
+    assert _lib.load().ss_batch_read_probs(B._h, 2, dummy.ctypes.data_as(_lib.c_f64p)) == _lib.SS_ERR_INVALID
+    with pytest.raises(IndexError):
         B.read_probs(2)
     # per-graph k must agree: built by hand through the C ABI
-    from spec_spars_amd import _lib
     o = (_lib.SparsifyOpts * 2)()
     for i in range(2):
         o[i].k = 8 + i
