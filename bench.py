@@ -136,6 +136,7 @@ def main():
     ap.add_argument("--ordering", choices=["natural", "random", "rounds"], default="rounds")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel event timing")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no autotuner)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (the multi-GPU "
@@ -168,6 +169,11 @@ def main():
     t_upload = time.perf_counter() - t0
     log("graph %.1fs factor %.1fs (nnz %d, depth %d/%d) upload %.1fs" %
         (t_graph, t_factor, finfo["nnz"], finfo["depth_fwd"], finfo["depth_bwd"], t_upload))
+    tune = ctx.tuning()
+    if not args.no_tune:
+        t0 = time.perf_counter()
+        tune = ctx.autotune(G, k, prec, reps=3)
+        log("autotune %.1fs: %s" % (time.perf_counter() - t0, tune))
     beta = graphs.beta_for(n, W["eps"])
     solve = ss.solve_opts(factor_seed=0, stop_rule=ss.STOP_REL, tol=1e-8, max_iter=2000, precision=prec,
                           ordering=ordering)
@@ -231,7 +237,7 @@ def main():
                    "ms_sketch": last["ms_sketch"], "ms_solve": last["ms_solve"], "ms_sample": last["ms_sample"],
                    "rhs_solves_per_s": k / (last["ms_solve"] / 1000.0),
                    "host_factor_s": t_factor, "host_graph_s": t_graph, "upload_s": t_upload,
-                   "copy_kernel_GBps": copy_gbs, "parallelism": "1 GPU",
+                   "copy_kernel_GBps": copy_gbs, "parallelism": "1 GPU", "tuning": tune,
                    "schedule_fwd": G.schedule_info(0), "schedule_bwd": G.schedule_info(1)},
         "roofline": roofline,
         "kernels": kernels,

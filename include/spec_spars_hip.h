@@ -273,6 +273,30 @@ typedef struct ss_schedule_info {
 int ss_graph_schedule_info(const ss_graph *g, int direction, ss_schedule_info *out);
 int ss_graph_profile_read(const ss_graph *g, ss_kernel_stat *out /* [SS_KERNEL_COUNT] */);
 
+/* ---------------------------------------------------------------------- */
+/* Launch shapes.  No reference counterpart.  tile_waves = wavefronts per
+ * block of the row-tile kernels (SpMM, vector updates, reductions);
+ * level_rows_per_wave = consecutive rows each wavefront of a per-level
+ * triangular-solve launch finishes.  A graph keeps the tile_waves in effect
+ * when it was created (its reductions are shaped by it, so compare results
+ * bit for bit only between graphs created under the same setting);
+ * level_rows_per_wave changes timing only. */
+typedef struct ss_tuning {
+    int32_t tile_waves;            /* 1, 2, 4, 8 or 16 (default 4)           */
+    int32_t level_rows_per_wave;   /* 1 .. 64 (default 1)                    */
+} ss_tuning;
+int ss_context_get_tuning(const ss_context *ctx, ss_tuning *out);
+int ss_context_set_tuning(ss_context *ctx, const ss_tuning *t);
+/* Autotuner: times the candidate shapes on graph g (factor set) for k
+ * columns at a precision -- one preconditioner application per
+ * rows-per-wave candidate, one SpMM + vector-update pass per tile-waves
+ * candidate, median of reps runs each -- and keeps the fastest in the
+ * context and in g.  ms_level / ms_tile (may be NULL) receive the best
+ * times. */
+int ss_context_autotune(ss_context *ctx, ss_graph *g, int32_t k, int32_t precision,
+                        int32_t reps, ss_tuning *best, double *ms_level,
+                        double *ms_tile);
+
 /* Achievable-bandwidth probe: a plain FP64 copy kernel over `bytes` bytes
  * (read + write), repeated reps times; *ms = average duration per copy. */
 int ss_bench_copy(ss_context *ctx, size_t bytes, int reps, double *ms);

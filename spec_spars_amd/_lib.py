@@ -52,6 +52,10 @@ class SparsifyOpts(ctypes.Structure):
                 ("beta", ctypes.c_double), ("solve", SolveOpts)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("tile_waves", ctypes.c_int32), ("level_rows_per_wave", ctypes.c_int32)]
+
+
 class SparsifyReport(ctypes.Structure):
     _fields_ = [("kept", ctypes.c_int64), ("max_iters", ctypes.c_int32),
                 ("refinements", ctypes.c_int32), ("max_relres", ctypes.c_double),
@@ -142,6 +146,10 @@ SIGNATURES = {
                                           c_f64p, c_i64p]),
     "ss_batch_read_block": (ctypes.c_int, [c_vp, ctypes.c_int32, ctypes.c_int, c_f64p]),
     "ss_batch_read_probs": (ctypes.c_int, [c_vp, ctypes.c_int32, c_f64p]),
+    "ss_context_get_tuning": (ctypes.c_int, [c_vp, ctypes.POINTER(Tuning)]),
+    "ss_context_set_tuning": (ctypes.c_int, [c_vp, ctypes.POINTER(Tuning)]),
+    "ss_context_autotune": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                           ctypes.POINTER(Tuning), c_f64p, c_f64p]),
     "ss_bench_copy": (ctypes.c_int, [c_vp, ctypes.c_size_t, ctypes.c_int, c_f64p]),
     # host layer (include/spec_spars_host.h)
     "ssh_params_default": (None, [ctypes.POINTER(HostParams), ctypes.c_int]),

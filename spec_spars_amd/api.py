@@ -112,6 +112,25 @@ class Context:
         check(load().ss_context_mem_info(self._h, ctypes.byref(f), ctypes.byref(t)))
         return f.value, t.value
 
+    def tuning(self):
+        t = _lib.Tuning()
+        check(load().ss_context_get_tuning(self._h, ctypes.byref(t)))
+        return {"tile_waves": t.tile_waves, "level_rows_per_wave": t.level_rows_per_wave}
+
+    def set_tuning(self, tile_waves=4, level_rows_per_wave=1):
+        t = _lib.Tuning(tile_waves, level_rows_per_wave)
+        check(load().ss_context_set_tuning(self._h, ctypes.byref(t)))
+
+    def autotune(self, graph, k, precision=_lib.PREC_FP64, reps=5):
+        """Time launch-shape candidates on graph (a Graph or Batch with its
+        factor set); the fastest are kept in this context and the graph."""
+        t = _lib.Tuning()
+        ml, mt = ctypes.c_double(), ctypes.c_double()
+        check(load().ss_context_autotune(self._h, graph._h, k, precision, reps, ctypes.byref(t),
+                                         ctypes.byref(ml), ctypes.byref(mt)))
+        return {"tile_waves": t.tile_waves, "level_rows_per_wave": t.level_rows_per_wave,
+                "ms_level": ml.value, "ms_tile": mt.value}
+
     def bench_copy(self, nbytes, reps=20):
         """Average ms of a plain FP64 copy kernel over nbytes (read + write)."""
         ms = ctypes.c_double()

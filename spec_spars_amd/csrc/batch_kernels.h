@@ -11,11 +11,16 @@ namespace ss {
 static const int kRowTile = 2048;    // rows per row tile
 static const int kEdgeTile = 256;    // edges per edge tile (one block)
 
+static const int kMaxTileWaves = 16;
+
 // Tiles of one kind: tile t covers [start[t], start[t+1]) of graph graph[t].
+// Row tiles are processed by a block of `waves` wavefronts (1..16) striding
+// the tile's rows; per-tile partials combine the waves in order.
 struct Tiles {
     const int32_t *start;
     const int32_t *graph;
     int count;
+    int waves;
 };
 
 struct BatchSampleArgs {

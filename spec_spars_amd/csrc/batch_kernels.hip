@@ -16,6 +16,7 @@ namespace ss {
 
 #define BT_LANE (threadIdx.x & 63)
 #define BT_WAVE (threadIdx.x >> 6)
+#define BT_NW ((int)(blockDim.x >> 6))   // waves per tile block
 
 __device__ __forceinline__ int bt_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
@@ -42,21 +43,24 @@ __device__ __forceinline__ T bt_dot_sub(T acc, int q0, int q1, const int32_t *__
     return acc;
 }
 
-// Per-tile column partial of f over the tile's rows; 4 waves stride the
-// rows, lanes take columns, waves combine in order.  Writes partial[t*k+j].
+// Per-tile column partial of f over the tile's rows; the block's waves
+// stride the rows, lanes take columns, waves combine in order.  Writes partial[t*k+j].
 __device__ __forceinline__ void bt_tile_store(double acc, int k, int jb, double *partial, double (*sm)[64]) {
     const int w = BT_WAVE, lane = BT_LANE, j = jb + lane;
     sm[w][lane] = acc;
     __syncthreads();
-    if (w == 0 && j < k)
-        partial[(size_t)blockIdx.x * k + j] = ((sm[0][lane] + sm[1][lane]) + sm[2][lane]) + sm[3][lane];
+    if (w == 0 && j < k) {
+        double s = sm[0][lane];
+        for (int u = 1; u < BT_NW; u++) s += sm[u][lane];
+        partial[(size_t)blockIdx.x * k + j] = s;
+    }
     __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
 // (a) sketch: block per row tile, wave per row, seed/scale of the tile's graph
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void kb_sketch(Tiles rt, const int32_t *__restrict__ iptr,
+__global__ __launch_bounds__(1024) void kb_sketch(Tiles rt, const int32_t *__restrict__ iptr,
                                                  const int32_t *__restrict__ ieid, const double *__restrict__ ib,
                                                  int k, const uint64_t *__restrict__ seeds,
                                                  const double *__restrict__ scales, int uniform, int variant,
@@ -67,7 +71,7 @@ __global__ __launch_bounds__(256) void kb_sketch(Tiles rt, const int32_t *__rest
     const uint64_t seed = seeds[g];
     const double scale = scales[g];
     const int r0 = rt.start[t], r1 = rt.start[t + 1];
-    for (int row = r0 + BT_WAVE; row < r1; row += 4) {
+    for (int row = r0 + BT_WAVE; row < r1; row += BT_NW) {
         const int r = bt_uniform(row);
         const int q0 = iptr[r], q1 = iptr[r + 1];
         for (int jb = 0; jb < k; jb += 64) {
@@ -88,17 +92,17 @@ __global__ __launch_bounds__(256) void kb_sketch(Tiles rt, const int32_t *__rest
 // (y^T D^+ y with the reference's skip of D == 0)
 // ---------------------------------------------------------------------------
 template <typename TA, typename TB>
-__global__ __launch_bounds__(256) void kb_tile_partial(Tiles rt, int k, const TA *__restrict__ A,
+__global__ __launch_bounds__(1024) void kb_tile_partial(Tiles rt, int k, const TA *__restrict__ A,
                                                        const TB *__restrict__ B, const double *__restrict__ diag,
                                                        int op, double *__restrict__ partial) {
-    __shared__ double sm[4][64];
+    __shared__ double sm[kMaxTileWaves][64];
     const int t = blockIdx.x;
     const int r0 = rt.start[t], r1 = rt.start[t + 1];
     for (int jb = 0; jb < k; jb += 64) {
         const int j = jb + BT_LANE;
         double acc = 0.0;
         if (j < k) {
-            for (int r = r0 + BT_WAVE; r < r1; r += 4) {
+            for (int r = r0 + BT_WAVE; r < r1; r += BT_NW) {
                 const double a = (double)A[(size_t)r * k + j];
                 if (op == 0) acc = bt_fma(a, (double)B[(size_t)r * k + j], acc);
                 else if (op == 1) acc += a;
@@ -231,13 +235,13 @@ __global__ __launch_bounds__(256) void kb_comp_means(int ncomp, int k, const int
 }
 
 template <typename TS, typename T>
-__global__ __launch_bounds__(256) void kb_project(Tiles rt, int k, const TS *__restrict__ Y,
+__global__ __launch_bounds__(1024) void kb_project(Tiles rt, int k, const TS *__restrict__ Y,
                                                   const int32_t *__restrict__ comp_of_row,
                                                   const double *__restrict__ cmean, T *__restrict__ R,
                                                   T *__restrict__ B, T *__restrict__ X) {
     const int t = blockIdx.x;
     const int r0 = rt.start[t], r1 = rt.start[t + 1];
-    for (int r = r0 + BT_WAVE; r < r1; r += 4) {
+    for (int r = r0 + BT_WAVE; r < r1; r += BT_NW) {
         const int c = comp_of_row[r];
         for (int jb = 0; jb < k; jb += 64) {
             const int j = jb + BT_LANE;
@@ -255,7 +259,7 @@ __global__ __launch_bounds__(256) void kb_project(Tiles rt, int k, const TS *__r
 // PCG vector kernels (block per row tile, per-(graph, column) scalars)
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void kb_update_p(Tiles rt, int k, const double *__restrict__ beta,
+__global__ __launch_bounds__(1024) void kb_update_p(Tiles rt, int k, const double *__restrict__ beta,
                                                    const T *__restrict__ Z, T *__restrict__ P) {
     const int t = blockIdx.x, g = rt.graph[t];
     const int r0 = rt.start[t], r1 = rt.start[t + 1];
@@ -263,7 +267,7 @@ __global__ __launch_bounds__(256) void kb_update_p(Tiles rt, int k, const double
         const int j = jb + BT_LANE;
         if (j >= k) break;
         const T b = (T)beta[(size_t)g * k + j];
-        for (int r = r0 + BT_WAVE; r < r1; r += 4) {
+        for (int r = r0 + BT_WAVE; r < r1; r += BT_NW) {
             const size_t o = (size_t)r * k + j;
             P[o] = b == (T)0 ? Z[o] : bt_fma(b, P[o], Z[o]);
         }
@@ -271,17 +275,17 @@ __global__ __launch_bounds__(256) void kb_update_p(Tiles rt, int k, const double
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void kb_spmm_dot(Tiles rt, const int32_t *__restrict__ rowptr,
+__global__ __launch_bounds__(1024) void kb_spmm_dot(Tiles rt, const int32_t *__restrict__ rowptr,
                                                    const int32_t *__restrict__ col, const T *__restrict__ val, int k,
                                                    const T *__restrict__ P, T *__restrict__ Q,
                                                    double *__restrict__ partial) {
-    __shared__ double sm[4][64];
+    __shared__ double sm[kMaxTileWaves][64];
     const int t = blockIdx.x;
     const int r0 = rt.start[t], r1 = rt.start[t + 1];
     for (int jb = 0; jb < k; jb += 64) {
         const int j = jb + BT_LANE;
         double acc = 0.0;
-        for (int row = r0 + BT_WAVE; row < r1; row += 4) {
+        for (int row = r0 + BT_WAVE; row < r1; row += BT_NW) {
             const int r = bt_uniform(row);
             const int q0 = rowptr[r], q1 = rowptr[r + 1];
             if (j < k) {
@@ -296,11 +300,11 @@ __global__ __launch_bounds__(256) void kb_spmm_dot(Tiles rt, const int32_t *__re
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void kb_update_xr_rr(Tiles rt, int k, const double *__restrict__ alpha,
+__global__ __launch_bounds__(1024) void kb_update_xr_rr(Tiles rt, int k, const double *__restrict__ alpha,
                                                        const T *__restrict__ P, const T *__restrict__ Q,
                                                        T *__restrict__ X, T *__restrict__ R,
                                                        double *__restrict__ partial) {
-    __shared__ double sm[4][64];
+    __shared__ double sm[kMaxTileWaves][64];
     const int t = blockIdx.x, g = rt.graph[t];
     const int r0 = rt.start[t], r1 = rt.start[t + 1];
     for (int jb = 0; jb < k; jb += 64) {
@@ -308,7 +312,7 @@ __global__ __launch_bounds__(256) void kb_update_xr_rr(Tiles rt, int k, const do
         double acc = 0.0;
         if (j < k) {
             const T a = (T)alpha[(size_t)g * k + j];
-            for (int r = r0 + BT_WAVE; r < r1; r += 4) {
+            for (int r = r0 + BT_WAVE; r < r1; r += BT_NW) {
                 const size_t o = (size_t)r * k + j;
                 X[o] = bt_fma(a, P[o], X[o]);
                 const T rv = bt_fma(-a, Q[o], R[o]);
@@ -321,17 +325,17 @@ __global__ __launch_bounds__(256) void kb_update_xr_rr(Tiles rt, int k, const do
 }
 
 // out = B - A X in FP64, with per-tile partials of |out|^2
-__global__ __launch_bounds__(256) void kb_residual64(Tiles rt, const int32_t *__restrict__ rowptr,
+__global__ __launch_bounds__(1024) void kb_residual64(Tiles rt, const int32_t *__restrict__ rowptr,
                                                      const int32_t *__restrict__ col, const double *__restrict__ val,
                                                      int k, const double *__restrict__ X, const double *__restrict__ B,
                                                      double *__restrict__ out, double *__restrict__ partial) {
-    __shared__ double sm[4][64];
+    __shared__ double sm[kMaxTileWaves][64];
     const int t = blockIdx.x;
     const int r0 = rt.start[t], r1 = rt.start[t + 1];
     for (int jb = 0; jb < k; jb += 64) {
         const int j = jb + BT_LANE;
         double acc = 0.0;
-        for (int row = r0 + BT_WAVE; row < r1; row += 4) {
+        for (int row = r0 + BT_WAVE; row < r1; row += BT_NW) {
             const int r = bt_uniform(row);
             const int q0 = rowptr[r], q1 = rowptr[r + 1];
             if (j < k) {
@@ -346,29 +350,30 @@ __global__ __launch_bounds__(256) void kb_residual64(Tiles rt, const int32_t *__
 }
 
 // FP64 -> FP32 with columns of converged (graph, column) pairs zeroed
-__global__ __launch_bounds__(256) void kb_convert_masked(Tiles rt, int k, const double *__restrict__ src,
+// (keep == NULL converts every column)
+__global__ __launch_bounds__(1024) void kb_convert_masked(Tiles rt, int k, const double *__restrict__ src,
                                                          const int *__restrict__ keep, float *__restrict__ dst) {
     const int t = blockIdx.x, g = rt.graph[t];
     const int r0 = rt.start[t], r1 = rt.start[t + 1];
     for (int jb = 0; jb < k; jb += 64) {
         const int j = jb + BT_LANE;
         if (j >= k) break;
-        const bool on = keep[(size_t)g * k + j] != 0;
-        for (int r = r0 + BT_WAVE; r < r1; r += 4) {
+        const bool on = !keep || keep[(size_t)g * k + j] != 0;
+        for (int r = r0 + BT_WAVE; r < r1; r += BT_NW) {
             const size_t o = (size_t)r * k + j;
             dst[o] = on ? (float)src[o] : 0.0f;
         }
     }
 }
 
-__global__ __launch_bounds__(256) void kb_accumulate(Tiles rt, int k, const float *__restrict__ d,
+__global__ __launch_bounds__(1024) void kb_accumulate(Tiles rt, int k, const float *__restrict__ d,
                                                      double *__restrict__ x) {
     const int t = blockIdx.x;
     const int r0 = rt.start[t], r1 = rt.start[t + 1];
     for (int jb = 0; jb < k; jb += 64) {
         const int j = jb + BT_LANE;
         if (j >= k) break;
-        for (int r = r0 + BT_WAVE; r < r1; r += 4) x[(size_t)r * k + j] += (double)d[(size_t)r * k + j];
+        for (int r = r0 + BT_WAVE; r < r1; r += BT_NW) x[(size_t)r * k + j] += (double)d[(size_t)r * k + j];
     }
 }
 
@@ -489,14 +494,14 @@ static inline unsigned bt_grid(int64_t items, int per) {
 
 void bt_sketch(hipStream_t s, Tiles rt, const int32_t *iptr, const int32_t *ieid, const double *ib, int k,
                const uint64_t *seeds, const double *scales, int uniform, double *Y) {
-    hipLaunchKernelGGL(kb_sketch, dim3(rt.count), dim3(256), 0, s, rt, iptr, ieid, ib, k, seeds, scales, uniform, 0,
+    hipLaunchKernelGGL(kb_sketch, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, iptr, ieid, ib, k, seeds, scales, uniform, 0,
                        Y);
 }
 
 template <typename TA, typename TB>
 void bt_tile_partial(hipStream_t s, Tiles rt, int k, const TA *A, const TB *B, const double *diag, int op,
                      double *partial) {
-    hipLaunchKernelGGL((kb_tile_partial<TA, TB>), dim3(rt.count), dim3(256), 0, s, rt, k, A, B, diag, op, partial);
+    hipLaunchKernelGGL((kb_tile_partial<TA, TB>), dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, k, A, B, diag, op, partial);
 }
 template void bt_tile_partial<double, double>(hipStream_t, Tiles, int, const double *, const double *,
                                               const double *, int, double *);
@@ -534,7 +539,7 @@ void bt_comp_means(hipStream_t s, int ncomp, int k, const int32_t *comp_graph, c
 template <typename TS, typename T>
 void bt_project(hipStream_t s, Tiles rt, int k, const TS *Y, const int32_t *comp_of_row, const double *cmean, T *R,
                 T *B, T *X) {
-    hipLaunchKernelGGL((kb_project<TS, T>), dim3(rt.count), dim3(256), 0, s, rt, k, Y, comp_of_row, cmean, R, B, X);
+    hipLaunchKernelGGL((kb_project<TS, T>), dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, k, Y, comp_of_row, cmean, R, B, X);
 }
 template void bt_project<double, double>(hipStream_t, Tiles, int, const double *, const int32_t *, const double *,
                                          double *, double *, double *);
@@ -542,14 +547,14 @@ template void bt_project<float, float>(hipStream_t, Tiles, int, const float *, c
                                        float *, float *, float *);
 template <typename T>
 void bt_update_p(hipStream_t s, Tiles rt, int k, const double *beta, const T *Z, T *P) {
-    hipLaunchKernelGGL(kb_update_p<T>, dim3(rt.count), dim3(256), 0, s, rt, k, beta, Z, P);
+    hipLaunchKernelGGL(kb_update_p<T>, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, k, beta, Z, P);
 }
 template void bt_update_p<double>(hipStream_t, Tiles, int, const double *, const double *, double *);
 template void bt_update_p<float>(hipStream_t, Tiles, int, const double *, const float *, float *);
 template <typename T>
 void bt_spmm_dot(hipStream_t s, Tiles rt, const int32_t *rowptr, const int32_t *col, const T *val, int k, const T *P,
                  T *Q, double *partial) {
-    hipLaunchKernelGGL(kb_spmm_dot<T>, dim3(rt.count), dim3(256), 0, s, rt, rowptr, col, val, k, P, Q, partial);
+    hipLaunchKernelGGL(kb_spmm_dot<T>, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, rowptr, col, val, k, P, Q, partial);
 }
 template void bt_spmm_dot<double>(hipStream_t, Tiles, const int32_t *, const int32_t *, const double *, int,
                                   const double *, double *, double *);
@@ -558,7 +563,7 @@ template void bt_spmm_dot<float>(hipStream_t, Tiles, const int32_t *, const int3
 template <typename T>
 void bt_update_xr_rr(hipStream_t s, Tiles rt, int k, const double *alpha, const T *P, const T *Q, T *X, T *R,
                      double *partial) {
-    hipLaunchKernelGGL(kb_update_xr_rr<T>, dim3(rt.count), dim3(256), 0, s, rt, k, alpha, P, Q, X, R, partial);
+    hipLaunchKernelGGL(kb_update_xr_rr<T>, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, k, alpha, P, Q, X, R, partial);
 }
 template void bt_update_xr_rr<double>(hipStream_t, Tiles, int, const double *, const double *, const double *,
                                       double *, double *, double *);
@@ -566,13 +571,13 @@ template void bt_update_xr_rr<float>(hipStream_t, Tiles, int, const double *, co
                                      float *, double *);
 void bt_residual64(hipStream_t s, Tiles rt, const int32_t *rowptr, const int32_t *col, const double *val, int k,
                    const double *X, const double *B, double *out, double *partial) {
-    hipLaunchKernelGGL(kb_residual64, dim3(rt.count), dim3(256), 0, s, rt, rowptr, col, val, k, X, B, out, partial);
+    hipLaunchKernelGGL(kb_residual64, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, rowptr, col, val, k, X, B, out, partial);
 }
 void bt_convert_masked(hipStream_t s, Tiles rt, int k, const double *src, const int *keep, float *dst) {
-    hipLaunchKernelGGL(kb_convert_masked, dim3(rt.count), dim3(256), 0, s, rt, k, src, keep, dst);
+    hipLaunchKernelGGL(kb_convert_masked, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, k, src, keep, dst);
 }
 void bt_accumulate(hipStream_t s, Tiles rt, int k, const float *d, double *x) {
-    hipLaunchKernelGGL(kb_accumulate, dim3(rt.count), dim3(256), 0, s, rt, k, d, x);
+    hipLaunchKernelGGL(kb_accumulate, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, k, d, x);
 }
 void bt_er_sample(hipStream_t s, const BatchSampleArgs &a, int ntiles) {
     hipLaunchKernelGGL(kb_er_sample, dim3(ntiles), dim3(256), 0, s, a);

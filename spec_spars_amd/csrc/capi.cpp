@@ -528,6 +528,50 @@ int ss_graph_profile_read(const ss_graph *g, ss_kernel_stat *out) {
     return SS_OK;
 }
 
+// ---- launch shapes ---------------------------------------------------------------
+static bool valid_tuning(const ss_tuning &t) {
+    const bool tw = t.tile_waves == 1 || t.tile_waves == 2 || t.tile_waves == 4 || t.tile_waves == 8 ||
+                    t.tile_waves == 16;
+    return tw && t.level_rows_per_wave >= 1 && t.level_rows_per_wave <= 64;
+}
+
+int ss_context_get_tuning(const ss_context *ctx, ss_tuning *out) {
+    if (!ctx || !out) return fail(SS_ERR_INVALID, "null argument");
+    out->tile_waves = ctx->tune.tile_waves;
+    out->level_rows_per_wave = ctx->tune.level_rows_per_wave;
+    return SS_OK;
+}
+
+int ss_context_set_tuning(ss_context *ctx, const ss_tuning *t) {
+    if (!ctx || !t) return fail(SS_ERR_INVALID, "null argument");
+    if (!valid_tuning(*t))
+        return fail(SS_ERR_INVALID, "tile_waves %d (1,2,4,8,16) / level_rows_per_wave %d (1..64) out of range",
+                    t->tile_waves, t->level_rows_per_wave);
+    ctx->tune.tile_waves = t->tile_waves;
+    ctx->tune.level_rows_per_wave = t->level_rows_per_wave;
+    return SS_OK;
+}
+
+int ss_context_autotune(ss_context *ctx, ss_graph *g, int32_t k, int32_t precision, int32_t reps, ss_tuning *best,
+                        double *ms_level, double *ms_tile) {
+    if (!ctx || !g) return fail(SS_ERR_INVALID, "null argument");
+    if (g->ctx != ctx) return fail(SS_ERR_INVALID, "graph belongs to another context");
+    SS_GUARD_BEGIN
+    ss_tuning_state b;
+    double ml = 0, mt = 0;
+    SS_TRY(graph_autotune(g, k, precision, reps, b, ml, mt));
+    ctx->tune = b;
+    g->tile_waves = b.tile_waves;
+    if (best) {
+        best->tile_waves = b.tile_waves;
+        best->level_rows_per_wave = b.level_rows_per_wave;
+    }
+    if (ms_level) *ms_level = ml;
+    if (ms_tile) *ms_tile = mt;
+    return SS_OK;
+    SS_GUARD_END
+}
+
 int ss_bench_copy(ss_context *ctx, size_t bytes, int reps, double *ms) {
     if (!ctx || !ms || bytes < 8 || reps < 1) return fail(SS_ERR_INVALID, "bad arguments");
     SS_GUARD_BEGIN

@@ -9,8 +9,15 @@
 #include "kernels.h"
 #include "ss_internal.h"
 
+// Launch shapes picked by ss_context_autotune (see ss_tuning in the C ABI).
+struct ss_tuning_state {
+    int tile_waves = 4;          // wavefronts per row-tile block (1, 2, 4, 8, 16)
+    int level_rows_per_wave = 1; // rows per wavefront in the per-level solve launches
+};
+
 struct ss_context {
     int device = 0;
+    ss_tuning_state tune;
     hipStream_t stream = nullptr;
     hipEvent_t ev[8] = {};
     int *h_status = nullptr;      // pinned: {nactive, err}
@@ -57,6 +64,7 @@ struct PcgWork {
 struct ss_graph {
     ss_context *ctx = nullptr;
     int G = 0;                       // graphs in the batch
+    int tile_waves = 4;              // fixed at creation: the reductions' shape depends on it
     int64_t N = 0, M = 0;            // total rows / edges
     std::vector<int64_t> row_off, edge_off;       // host, G+1
     std::vector<int32_t> rtile_off, etile_off;    // host, G+1: tile ranges per graph
@@ -173,5 +181,8 @@ struct GraphParams {
 };
 int graph_set_params(ss_graph *g, const std::vector<GraphParams> &prm);
 int graph_sketch(ss_graph *g, int k, int uniform, int prec);
+// Time launch-shape candidates on g (see ss_context_autotune).
+int graph_autotune(ss_graph *g, int k, int prec, int reps, ss_tuning_state &best, double &ms_level,
+                   double &ms_tile);
 int graph_sample(ss_graph *g, int k, const double *p_in_dev, bool write_d);
 }  // namespace ss

@@ -19,6 +19,7 @@
 #include <cstring>
 #include <memory>
 #include <numeric>
+#include <type_traits>
 
 #include "device.h"
 
@@ -153,6 +154,7 @@ int batch_upload(ss_context *ctx, const std::vector<const HostGraph *> &graphs, 
     std::unique_ptr<ss_graph> g(new ss_graph());
     g->ctx = ctx;
     g->G = G;
+    g->tile_waves = ctx->tune.tile_waves;
     g->row_off.assign((size_t)G + 1, 0);
     g->edge_off.assign((size_t)G + 1, 0);
     g->rtile_off.assign((size_t)G + 1, 0);
@@ -520,7 +522,7 @@ template <typename T> static const T *avals(const ss_graph *g);
 template <> const double *avals<double>(const ss_graph *g) { return g->val; }
 template <> const float *avals<float>(const ss_graph *g) { return g->valf; }
 
-static Tiles rtiles(const ss_graph *g) { return Tiles{g->rt_start, g->rt_graph, g->nrt}; }
+static Tiles rtiles(const ss_graph *g) { return Tiles{g->rt_start, g->rt_graph, g->nrt, g->tile_waves}; }
 
 // Zp = L^{-T} D^+ L^{-1} R (W holds the forward result y): one launch per
 // level plus the single-workgroup small-level runs.  rz_partial receives the
@@ -528,11 +530,12 @@ static Tiles rtiles(const ss_graph *g) { return Tiles{g->rt_start, g->rt_graph, 
 template <typename T>
 static int precond(ss_graph *g, int k, const T *R, T *W, T *Zp, double *rz_partial) {
     hipStream_t s = g->ctx->stream;
+    const int rpw = g->ctx->tune.level_rows_per_wave;
     {
         // the level launches are timed as one group over their count
         ProfScope ps(g, SS_KERNEL_TRI_FWD, g->f_small_lo);
         for (int32_t l = 0; l < g->f_small_lo; l++)
-            launch_trisolve_level<T, false>(s, g->f_level_ptr[l], g->f_level_long[l], g->f_level_ptr[l + 1],
+            launch_trisolve_level<T, false>(s, g->f_level_ptr[l], g->f_level_long[l], g->f_level_ptr[l + 1], rpw,
                                             g->f_rows, g->f_ptr, g->f_split, g->f_col, fvals<T>(g), g->diag, k, R,
                                             W);
     }
@@ -558,7 +561,7 @@ static int precond(ss_graph *g, int k, const T *R, T *W, T *Zp, double *rz_parti
         const int32_t nlev = (int32_t)g->b_level_ptr.size() - 1;
         ProfScope ps(g, SS_KERNEL_TRI_BWD, nlev - g->b_small_hi);
         for (int32_t l = g->b_small_hi; l < nlev; l++)
-            launch_trisolve_level<T, true>(s, g->b_level_ptr[l], g->b_level_long[l], g->b_level_ptr[l + 1],
+            launch_trisolve_level<T, true>(s, g->b_level_ptr[l], g->b_level_long[l], g->b_level_ptr[l + 1], rpw,
                                            g->b_rows, g->b_ptr, nullptr, g->b_col, bvals<T>(g), g->diag, k, W, Zp);
     }
     SS_HIP(hipGetLastError());
@@ -715,6 +718,93 @@ int graph_solve(ss_graph *g, int k, const ss_solve_opts &o, SolveStats &st) {
     return SS_OK;
 }
 
+// Median device time (ms) of reps runs of f on the graph's stream.
+template <typename F>
+static int time_median(ss_graph *g, int reps, F &&f, double &ms) {
+    ss_context *c = g->ctx;
+    std::vector<float> t;
+    SS_TRY(f());   // warm
+    for (int r = 0; r < reps; r++) {
+        SS_HIP(hipEventRecord(c->ev[6], c->stream));
+        SS_TRY(f());
+        SS_HIP(hipEventRecord(c->ev[7], c->stream));
+        SS_HIP(hipEventSynchronize(c->ev[7]));
+        float e = 0.f;
+        SS_HIP(hipEventElapsedTime(&e, c->ev[6], c->ev[7]));
+        t.push_back(e);
+    }
+    std::sort(t.begin(), t.end());
+    ms = t[t.size() / 2];
+    return SS_OK;
+}
+
+template <typename T>
+static int autotune_t(ss_graph *g, int k, const PcgWork<T> &w, int reps, ss_tuning_state &best, double &ms_level,
+                      double &ms_tile) {
+    hipStream_t s = g->ctx->stream;
+    const bool prof = g->prof;
+    g->prof = false;
+    // a realistic right-hand side: the sketch, projected (FP64) and converted
+    std::vector<GraphParams> prm((size_t)g->G);
+    for (auto &p : prm) p.scale = 1.0 / std::sqrt((double)k);
+    SS_TRY(graph_set_params(g, prm));
+    bt_sketch(s, rtiles(g), g->iptr, g->ieid, g->ib, k, g->p_seeds, g->p_scales, 0, g->Y64);
+    component_means(g, k, g->Y64);
+    if (std::is_same<T, double>::value) {
+        bt_project<double, double>(s, rtiles(g), k, g->Y64, g->comp_of_row, g->cmean, (double *)w.R, (double *)w.B,
+                                   g->X64);
+    } else {
+        bt_project<double, double>(s, rtiles(g), k, g->Y64, g->comp_of_row, g->cmean, g->R64, g->B64, g->X64);
+        bt_convert_masked(s, rtiles(g), k, g->R64, nullptr, (float *)w.R);
+    }
+    const ss_tuning_state orig = g->ctx->tune;
+    const int orig_waves = g->tile_waves;
+    ms_level = 1e30;
+    for (int rpw : {1, 2, 4, 8, 16}) {
+        g->ctx->tune.level_rows_per_wave = rpw;
+        double ms = 0;
+        SS_TRY(time_median(g, reps, [&]() { return precond<T>(g, k, w.R, w.W, w.Zp, g->partial2); }, ms));
+        if (ms < ms_level) {
+            ms_level = ms;
+            best.level_rows_per_wave = rpw;
+        }
+    }
+    ms_tile = 1e30;
+    launch_fill_u32(s, (int64_t)2 * g->G * k, reinterpret_cast<uint32_t *>(g->sc.beta), 0u);
+    launch_fill_u32(s, (int64_t)2 * g->G * k, reinterpret_cast<uint32_t *>(g->sc.alpha), 0u);
+    for (int tw : {2, 4, 8, 16}) {
+        g->tile_waves = tw;
+        const Tiles rt = rtiles(g);
+        double ms = 0;
+        SS_TRY(time_median(g, reps, [&]() {
+            bt_update_p<T>(s, rt, k, g->sc.beta, w.Zp, w.P);
+            bt_spmm_dot<T>(s, rt, g->rowptr, g->col, avals<T>(g), k, w.P, w.Q, g->partial);
+            bt_update_xr_rr<T>(s, rt, k, g->sc.alpha, w.P, w.Q, w.B, w.W, g->partial);
+            SS_HIP(hipGetLastError());
+            return SS_OK;
+        }, ms));
+        if (ms < ms_tile) {
+            ms_tile = ms;
+            best.tile_waves = tw;
+        }
+    }
+    g->tile_waves = orig_waves;
+    g->ctx->tune = orig;
+    g->prof = prof;
+    return SS_OK;
+}
+
+int graph_autotune(ss_graph *g, int k, int prec, int reps, ss_tuning_state &best, double &ms_level,
+                   double &ms_tile) {
+    if (!g->has_factor) return fail(SS_ERR_INVALID, "no factor uploaded");
+    if (reps < 1 || reps > 100) return fail(SS_ERR_INVALID, "reps=%d outside 1..100", reps);
+    SS_TRY(graph_ensure_work(g, k, prec));
+    SS_HIP(hipSetDevice(g->ctx->device));
+    best = g->ctx->tune;
+    if (prec == SS_PREC_FP64) return autotune_t<double>(g, k, g->w64, reps, best, ms_level, ms_tile);
+    return autotune_t<float>(g, k, g->w32, reps, best, ms_level, ms_tile);
+}
+
 int graph_set_params(ss_graph *g, const std::vector<GraphParams> &prm) {
     if ((int)prm.size() != g->G) return fail(SS_ERR_INVALID, "need %d parameter sets, got %zu", g->G, prm.size());
     std::vector<uint64_t> seeds(g->G);
@@ -762,7 +852,7 @@ int graph_sample(ss_graph *g, int k, const double *p_in_dev, bool write_d) {
     launch_fill_u32(s, 1, g->ticket, 0u);
     BatchSampleArgs a;
     a.m = g->M;
-    a.et = Tiles{g->et_start, g->et_graph, g->net};
+    a.et = Tiles{g->et_start, g->et_graph, g->net, 4};
     a.edge_off = g->edge_off_dev;
     a.eu = g->eu;
     a.ev = g->ev;

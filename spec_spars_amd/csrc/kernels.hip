@@ -530,9 +530,10 @@ __device__ __forceinline__ T seg_apply_shared(T acc, int a, int b, bool act, int
 
 // One level of a triangular solve: schedule positions [t0, t1), the first
 // nlong of them rows of more than kLongRow entries (a block of 4 waves
-// each), the rest a wave each.  Forward rows are split at split[t].
+// each), the rest rpw consecutive rows per wave.  Forward rows are split at
+// split[t].
 template <typename T, bool BWD>
-__global__ __launch_bounds__(256) void k_trisolve_level(int32_t t0, int32_t nlong, int32_t t1,
+__global__ __launch_bounds__(256) void k_trisolve_level(int32_t t0, int32_t nlong, int32_t t1, int rpw,
                                                         const int32_t *__restrict__ rows,
                                                         const int32_t *__restrict__ ptr,
                                                         const int32_t *__restrict__ split,
@@ -557,18 +558,21 @@ __global__ __launch_bounds__(256) void k_trisolve_level(int32_t t0, int32_t nlon
         }
         return;
     }
-    const int t = wave_uniform(t0 + nlong + ((int)blockIdx.x - nlong) * 4 + wv);
-    if (t >= t1) return;
-    const int i = wave_uniform(rows[t]);
-    const int q0 = wave_uniform(ptr[t]), q1 = wave_uniform(ptr[t + 1]);
-    const int qs = BWD ? q1 : q0 + wave_uniform(split[t]);
-    const double d = BWD ? diag[i] : 1.0;
-    for (int jb = 0; jb < k; jb += 64) {
-        const int j = jb + lane;
-        if (j >= k) break;
-        T acc = seg_apply<T>(row_start<T, BWD>(i, j, k, d, in), q0, qs, col, val, out, k, j);
-        if (!BWD) acc = seg_apply<T>(acc, qs, q1, col, val, out, k, j);
-        out[(size_t)i * k + j] = acc;
+    // rpw consecutive rows per wave
+    const int tb = wave_uniform(t0 + nlong + (((int)blockIdx.x - nlong) * 4 + wv) * rpw);
+    const int te = tb + rpw < t1 ? tb + rpw : t1;
+    for (int t = tb; t < te; t++) {
+        const int i = wave_uniform(rows[t]);
+        const int q0 = wave_uniform(ptr[t]), q1 = wave_uniform(ptr[t + 1]);
+        const int qs = BWD ? q1 : q0 + wave_uniform(split[t]);
+        const double d = BWD ? diag[i] : 1.0;
+        for (int jb = 0; jb < k; jb += 64) {
+            const int j = jb + lane;
+            if (j >= k) break;
+            T acc = seg_apply<T>(row_start<T, BWD>(i, j, k, d, in), q0, qs, col, val, out, k, j);
+            if (!BWD) acc = seg_apply<T>(acc, qs, q1, col, val, out, k, j);
+            out[(size_t)i * k + j] = acc;
+        }
     }
 }
 
@@ -953,16 +957,18 @@ template void launch_update_xr_rr<float>(hipStream_t, int32_t, int, const double
                                          float *, float *, double *);
 
 template <typename T, bool BWD>
-void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nlong, int32_t t1, const int32_t *rows,
+void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nlong, int32_t t1, int rpw, const int32_t *rows,
                            const int32_t *ptr, const int32_t *split, const int32_t *col, const T *val,
                            const double *diag, int k, const T *in, T *out) {
     if (t1 <= t0) return;
-    const unsigned grid = (unsigned)nlong + grid_for(t1 - t0 - nlong, 4);
-    hipLaunchKernelGGL((k_trisolve_level<T, BWD>), dim3(grid), dim3(256), 0, s, t0, nlong, t1, rows, ptr, split, col,
-                       val, diag, k, in, out);
+    if (rpw < 1) rpw = 1;
+    const int64_t short_rows = (int64_t)t1 - t0 - nlong;
+    const unsigned grid = (unsigned)nlong + (unsigned)((short_rows + 4LL * rpw - 1) / (4LL * rpw));
+    hipLaunchKernelGGL((k_trisolve_level<T, BWD>), dim3(grid), dim3(256), 0, s, t0, nlong, t1, rpw, rows, ptr, split,
+                       col, val, diag, k, in, out);
 }
 #define SS_INST_TRI(T, B)                                                                                      \
-    template void launch_trisolve_level<T, B>(hipStream_t, int32_t, int32_t, int32_t, const int32_t *,          \
+    template void launch_trisolve_level<T, B>(hipStream_t, int32_t, int32_t, int32_t, int, const int32_t *,     \
                                               const int32_t *, const int32_t *, const int32_t *, const T *,      \
                                               const double *, int, const T *, T *);
 SS_INST_TRI(double, false)

@@ -276,3 +276,37 @@ def test_spectral_quality_dense_graph(ctx):
     x -= x.mean(0)
     rq = np.einsum("ij,ij->j", x, LH @ x) / np.einsum("ij,ij->j", x, LG @ x)
     assert rq.min() >= 1 - eps and rq.max() <= 1 + eps
+
+
+def test_autotune_and_tuning_shapes(ctx, oracle_mod):
+    """The autotuner returns valid shapes; every shape gives solves within
+    tolerance of the oracle, and batches created under one shape stay
+    bit-identical alone and mixed."""
+    import spec_spars_amd as ss
+    from spec_spars_amd import graphs
+    gs = _mix()
+    fs = _factors(gs, ordering=ss.ORDER_ROUNDS)
+    saved = ctx.tuning()
+    try:
+        B = ss.Batch(ctx, gs)
+        B.set_factors(fs)
+        t = ctx.autotune(B, 8, ss.PREC_FP64, reps=2)
+        assert t["tile_waves"] in (1, 2, 4, 8, 16) and 1 <= t["level_rows_per_wave"] <= 64
+        assert t["ms_level"] > 0 and t["ms_tile"] > 0
+        B.close()
+        for tw, rpw in ((1, 1), (16, 8), (8, 3)):
+            ctx.set_tuning(tile_waves=tw, level_rows_per_wave=rpw)
+            mixed = _run(ctx, gs, fs, list(range(len(gs))), 8, ss.PREC_FP64)
+            alone = _run(ctx, gs, fs, [2], 8, ss.PREC_FP64)[0]
+            assert np.array_equal(alone["Z"], mixed[2]["Z"]) and np.array_equal(alone["p"], mixed[2]["p"])
+            n, u, v, w = gs[0]
+            cp, ri, va = graphs.laplacian_csc(n, u, v, w)
+            Yo = oracle_mod.sketch(u, v, w, n, 8, seed=10, scale=1.0 / math.sqrt(8))
+            assert np.array_equal(mixed[0]["Y"], Yo)
+            Zo, _, _ = oracle_mod.solve_lap(Yo, cp, ri, va, *fs[0].export(), stop_rule=1, tol=1e-12, max_iter=3000)
+            a, b = mixed[0]["Z"] - mixed[0]["Z"].mean(0), Zo - Zo.mean(0)
+            assert (np.linalg.norm(a - b, axis=0) / np.linalg.norm(b, axis=0)).max() < 1e-6
+        with pytest.raises(ss.SSError):
+            ctx.set_tuning(tile_waves=3)
+    finally:
+        ctx.set_tuning(**{k: saved[k] for k in ("tile_waves", "level_rows_per_wave")})
