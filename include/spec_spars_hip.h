@@ -280,7 +280,9 @@ int ss_graph_profile_read(const ss_graph *g, ss_kernel_stat *out /* [SS_KERNEL_C
  * triangular-solve launch finishes.  A graph keeps the tile_waves in effect
  * when it was created (its reductions are shaped by it, so compare results
  * bit for bit only between graphs created under the same setting);
- * level_rows_per_wave changes timing only. */
+ * level_rows_per_wave = consecutive short rows (at most 32 entries) each
+ * wavefront of a per-level triangular-solve launch finishes; it changes
+ * timing only. */
 typedef struct ss_tuning {
     int32_t tile_waves;            /* 1, 2, 4, 8 or 16 (default 4)           */
     int32_t level_rows_per_wave;   /* 1 .. 64 (default 1)                    */

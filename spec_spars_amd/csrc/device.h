@@ -12,7 +12,7 @@
 // Launch shapes picked by ss_context_autotune (see ss_tuning in the C ABI).
 struct ss_tuning_state {
     int tile_waves = 4;          // wavefronts per row-tile block (1, 2, 4, 8, 16)
-    int level_rows_per_wave = 1; // rows per wavefront in the per-level solve launches
+    int level_rows_per_wave = 1; // rows per wavefront of the short rows in the per-level solve launches
 };
 
 struct ss_context {
@@ -100,6 +100,7 @@ struct ss_graph {
     float *f_valf = nullptr, *b_valf = nullptr;
     std::vector<int32_t> f_level_ptr, b_level_ptr;   // host copies: launch bounds per level
     std::vector<int32_t> f_level_long, b_level_long; // long rows leading each level
+    std::vector<int32_t> f_level_waves, b_level_waves;   // wavefronts per block of each level launch
     int32_t f_small_lo = 0, f_small_hi = 0, b_small_lo = 0, b_small_hi = 0;
     int32_t *f_split = nullptr;
     int32_t *f_lp_dev = nullptr, *b_lp_dev = nullptr;   // level_ptr on the device

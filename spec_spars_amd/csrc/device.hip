@@ -429,6 +429,18 @@ int batch_upload_factors(ss_graph *g, const std::vector<const Factor *> &factors
     g->b_level_ptr = bw.level_ptr;
     g->f_level_long = fw.level_long;
     g->b_level_long = bw.level_long;
+    // a level's long rows get as many waves as their longest row has parts (4..16)
+    for (int dir = 0; dir < 2; dir++) {
+        const Schedule &S = dir == 0 ? fw : bw;
+        std::vector<int32_t> &out = dir == 0 ? g->f_level_waves : g->b_level_waves;
+        out.assign((size_t)S.depth, 4);
+        for (int32_t l = 0; l < S.depth; l++) {
+            int32_t maxlen = 0;
+            for (int64_t t = S.level_ptr[l]; t < S.level_ptr[l] + S.level_long[l]; t++)
+                maxlen = std::max(maxlen, S.ptr[t + 1] - S.ptr[t]);
+            out[l] = std::min(kLevelWaves, std::max(4, seg_parts(maxlen)));
+        }
+    }
     for (int dir = 0; dir < 2; dir++) {
         const Schedule &S = dir == 0 ? fw : bw;
         ss_schedule_info &I = g->sched_info[dir];
@@ -535,7 +547,8 @@ static int precond(ss_graph *g, int k, const T *R, T *W, T *Zp, double *rz_parti
         // the level launches are timed as one group over their count
         ProfScope ps(g, SS_KERNEL_TRI_FWD, g->f_small_lo);
         for (int32_t l = 0; l < g->f_small_lo; l++)
-            launch_trisolve_level<T, false>(s, g->f_level_ptr[l], g->f_level_long[l], g->f_level_ptr[l + 1], rpw,
+            launch_trisolve_level<T, false>(s, g->f_level_ptr[l], g->f_level_long[l], g->f_level_ptr[l + 1],
+                                            g->f_level_waves[l], rpw,
                                             g->f_rows, g->f_ptr, g->f_split, g->f_col, fvals<T>(g), g->diag, k, R,
                                             W);
     }
@@ -561,7 +574,8 @@ static int precond(ss_graph *g, int k, const T *R, T *W, T *Zp, double *rz_parti
         const int32_t nlev = (int32_t)g->b_level_ptr.size() - 1;
         ProfScope ps(g, SS_KERNEL_TRI_BWD, nlev - g->b_small_hi);
         for (int32_t l = g->b_small_hi; l < nlev; l++)
-            launch_trisolve_level<T, true>(s, g->b_level_ptr[l], g->b_level_long[l], g->b_level_ptr[l + 1], rpw,
+            launch_trisolve_level<T, true>(s, g->b_level_ptr[l], g->b_level_long[l], g->b_level_ptr[l + 1],
+                                           g->b_level_waves[l], rpw,
                                            g->b_rows, g->b_ptr, nullptr, g->b_col, bvals<T>(g), g->diag, k, W, Zp);
     }
     SS_HIP(hipGetLastError());
@@ -760,7 +774,7 @@ static int autotune_t(ss_graph *g, int k, const PcgWork<T> &w, int reps, ss_tuni
     const ss_tuning_state orig = g->ctx->tune;
     const int orig_waves = g->tile_waves;
     ms_level = 1e30;
-    for (int rpw : {1, 2, 4, 8, 16}) {
+    for (int rpw : {1, 2, 4}) {
         g->ctx->tune.level_rows_per_wave = rpw;
         double ms = 0;
         SS_TRY(time_median(g, reps, [&]() { return precond<T>(g, k, w.R, w.W, w.Zp, g->partial2); }, ms));

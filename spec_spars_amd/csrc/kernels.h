@@ -79,13 +79,13 @@ void launch_copy_f64(hipStream_t s, int64_t count, const double *a, double *b);
 // partials (partial buffers hold kMaxPartialBlocks * k doubles).
 static const int kMaxPartialBlocks = 4096;
 // Rows of one level, positions [t0, t1); the first nlong of them are rows
-// of more than kLongRow entries (a block each), the rest go rpw to a wave.
-// Forward rows are split in two segments at split[t] (backward: split
-// unused, may be NULL).
+// of more than kLongRow entries (a block of `waves` wavefronts each: 4, 8 or
+// 16), the rest go rpw to a wave.  Forward rows are split in two segments at
+// split[t] (backward: split unused, may be NULL).
 template <typename T, bool BWD>
-void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nlong, int32_t t1, int rpw, const int32_t *rows,
-                           const int32_t *ptr, const int32_t *split, const int32_t *col, const T *val,
-                           const double *diag, int k, const T *in, T *out);
+void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nlong, int32_t t1, int waves, int rpw,
+                           const int32_t *rows, const int32_t *ptr, const int32_t *split, const int32_t *col,
+                           const T *val, const double *diag, int k, const T *in, T *out);
 template <typename T>
 void launch_spmm_dot(hipStream_t s, int32_t n, const int32_t *rowptr, const int32_t *col, const T *val, int k,
                      const T *P, T *Q, double *partial);

@@ -468,26 +468,30 @@ __device__ __forceinline__ T seg_subtree(int a, int b, int P, int s0, const int3
     return acc[0];
 }
 
-// Tree fold of parts [s0, s0 + na) with na in {1, 2, 4, 8}.
-template <typename T>
+// Tree fold of parts [s0, s0 + na) with na in {1, 2, 4, 8, 16}, na <= MAXNA
+// (the bound keeps unused wide variants out of a kernel's registers); 16
+// parts fold as two subtrees of 8, the top of their tree.
+template <typename T, int MAXNA>
 __device__ __forceinline__ T seg_subtree_n(int na, int a, int b, int P, int s0, const int32_t *__restrict__ col,
                                            const T *__restrict__ val, const T *X, int k, int j) {
-    switch (na) {
-        case 1: return seg_subtree<T, 1>(a, b, P, s0, col, val, X, k, j);
-        case 2: return seg_subtree<T, 2>(a, b, P, s0, col, val, X, k, j);
-        case 4: return seg_subtree<T, 4>(a, b, P, s0, col, val, X, k, j);
-        default: return seg_subtree<T, 8>(a, b, P, s0, col, val, X, k, j);
-    }
+    if (na == 1 || MAXNA == 1) return seg_subtree<T, 1>(a, b, P, s0, col, val, X, k, j);
+    if (na == 2 || MAXNA == 2) return seg_subtree<T, (MAXNA < 2 ? MAXNA : 2)>(a, b, P, s0, col, val, X, k, j);
+    if (na == 4 || MAXNA == 4) return seg_subtree<T, (MAXNA < 4 ? MAXNA : 4)>(a, b, P, s0, col, val, X, k, j);
+    if (na == 8 || MAXNA == 8) return seg_subtree<T, (MAXNA < 8 ? MAXNA : 8)>(a, b, P, s0, col, val, X, k, j);
+    const T lo = seg_subtree<T, (MAXNA < 8 ? MAXNA : 8)>(a, b, P, s0, col, val, X, k, j);
+    const T hi = seg_subtree<T, (MAXNA < 8 ? MAXNA : 8)>(a, b, P, s0 + 8, col, val, X, k, j);
+    return lo + hi;
 }
 
-// acc updated by segment [a, b), every part on this wave.
-template <typename T>
+// acc updated by segment [a, b), every part on this wave; segments longer
+// than MAXLEN entries never reach here (MAXLEN bounds the part count).
+template <typename T, int MAXLEN>
 __device__ __forceinline__ T seg_apply(T acc, int a, int b, const int32_t *__restrict__ col,
                                        const T *__restrict__ val, const T *X, int k, int j) {
+    constexpr int PMAX = seg_parts(MAXLEN);
     const int P = seg_parts(b - a);
-    if (P == 1) return row_dot_sub<T>(acc, a, b, 1, col, val, X, k, j);
-    return acc + (P == 4 ? seg_subtree<T, 4>(a, b, 4, 0, col, val, X, k, j)
-                         : seg_subtree<T, 8>(a, b, 8, 0, col, val, X, k, j));
+    if (P == 1 || PMAX == 1) return row_dot_sub<T>(acc, a, b, 1, col, val, X, k, j);
+    return acc + seg_subtree_n<T, PMAX>(P, a, b, P, 0, col, val, X, k, j);
 }
 
 // Tree fold of sm[first .. first + N) (column lane).
@@ -505,7 +509,8 @@ __device__ __forceinline__ T lds_tree(T (*sm)[64], int first, int n, int lane) {
         case 1: return sm[first][lane];
         case 2: return lds_tree_n<T, 2>(sm, first, lane);
         case 4: return lds_tree_n<T, 4>(sm, first, lane);
-        default: return lds_tree_n<T, 8>(sm, first, lane);
+        case 8: return lds_tree_n<T, 8>(sm, first, lane);
+        default: return lds_tree_n<T, 16>(sm, first, lane);
     }
 }
 
@@ -513,7 +518,7 @@ __device__ __forceinline__ T lds_tree(T (*sm)[64], int first, int n, int lane) {
 // share the row (g a power of two; this wave is number sub of them, they
 // occupy sm[first .. first+g)); the result is valid on sub 0.  Every wave of
 // the block calls this (it holds barriers).
-template <typename T>
+template <typename T, int MAXNA>
 __device__ __forceinline__ T seg_apply_shared(T acc, int a, int b, bool act, int g, int sub, int first, T (*sm)[64],
                                               const int32_t *__restrict__ col, const T *__restrict__ val,
                                               const T *X, int k, int j, int lane) {
@@ -521,7 +526,8 @@ __device__ __forceinline__ T seg_apply_shared(T acc, int a, int b, bool act, int
     const int gg = P < g ? P : g;             // waves that hold parts
     if (act && P == 1 && sub == 0 && j < k) acc = row_dot_sub<T>(acc, a, b, 1, col, val, X, k, j);
     if (act && P > 1 && sub < gg)
-        sm[first + sub][lane] = j < k ? seg_subtree_n<T>(P / gg, a, b, P, sub * (P / gg), col, val, X, k, j) : (T)0;
+        sm[first + sub][lane] =
+            j < k ? seg_subtree_n<T, MAXNA>(P / gg, a, b, P, sub * (P / gg), col, val, X, k, j) : (T)0;
     __syncthreads();
     if (act && P > 1 && sub == 0) acc = acc + lds_tree<T>(sm, first, gg, lane);
     __syncthreads();
@@ -529,21 +535,21 @@ __device__ __forceinline__ T seg_apply_shared(T acc, int a, int b, bool act, int
 }
 
 // One level of a triangular solve: schedule positions [t0, t1), the first
-// nlong of them rows of more than kLongRow entries (a block of 4 waves
-// each), the rest rpw consecutive rows per wave.  Forward rows are split at
-// split[t].
+// nlong of them rows of more than kLongRow entries (a block each, its W =
+// blockDim / 64 waves sharing the row's parts), the rest rpw consecutive
+// rows per wave.  Forward rows are split at split[t].
 template <typename T, bool BWD>
-__global__ __launch_bounds__(256) void k_trisolve_level(int32_t t0, int32_t nlong, int32_t t1, int rpw,
-                                                        const int32_t *__restrict__ rows,
-                                                        const int32_t *__restrict__ ptr,
-                                                        const int32_t *__restrict__ split,
-                                                        const int32_t *__restrict__ col,
-                                                        const T *__restrict__ val,
-                                                        const double *__restrict__ diag, int k,
-                                                        const T *__restrict__ in, T *__restrict__ out) {
-    const int lane = SS_LANE, wv = SS_WAVE_IN_BLOCK;
+__global__ __launch_bounds__(64 * kLevelWaves) void k_trisolve_level(int32_t t0, int32_t nlong, int32_t t1, int rpw,
+                                                                     const int32_t *__restrict__ rows,
+                                                                     const int32_t *__restrict__ ptr,
+                                                                     const int32_t *__restrict__ split,
+                                                                     const int32_t *__restrict__ col,
+                                                                     const T *__restrict__ val,
+                                                                     const double *__restrict__ diag, int k,
+                                                                     const T *__restrict__ in, T *__restrict__ out) {
+    const int lane = SS_LANE, wv = SS_WAVE_IN_BLOCK, W = (int)(blockDim.x >> 6);
     if ((int)blockIdx.x < nlong) {
-        __shared__ T sm[4][64];
+        __shared__ T sm[kLevelWaves][64];
         const int t = t0 + blockIdx.x;
         const int i = wave_uniform(rows[t]);
         const int q0 = wave_uniform(ptr[t]), q1 = wave_uniform(ptr[t + 1]);
@@ -552,14 +558,15 @@ __global__ __launch_bounds__(256) void k_trisolve_level(int32_t t0, int32_t nlon
         for (int jb = 0; jb < k; jb += 64) {
             const int j = jb + lane;
             T acc = (wv == 0 && j < k) ? row_start<T, BWD>(i, j, k, d, in) : (T)0;
-            acc = seg_apply_shared<T>(acc, q0, qs, true, 4, wv, 0, sm, col, val, out, k, j, lane);
-            if (!BWD) acc = seg_apply_shared<T>(acc, qs, q1, true, 4, wv, 0, sm, col, val, out, k, j, lane);
+            acc = seg_apply_shared<T, kMaxParts / 4>(acc, q0, qs, true, W, wv, 0, sm, col, val, out, k, j, lane);
+            if (!BWD)
+                acc = seg_apply_shared<T, kMaxParts / 4>(acc, qs, q1, true, W, wv, 0, sm, col, val, out, k, j, lane);
             if (wv == 0 && j < k) out[(size_t)i * k + j] = acc;
         }
         return;
     }
     // rpw consecutive rows per wave
-    const int tb = wave_uniform(t0 + nlong + (((int)blockIdx.x - nlong) * 4 + wv) * rpw);
+    const int tb = wave_uniform(t0 + nlong + (((int)blockIdx.x - nlong) * W + wv) * rpw);
     const int te = tb + rpw < t1 ? tb + rpw : t1;
     for (int t = tb; t < te; t++) {
         const int i = wave_uniform(rows[t]);
@@ -569,8 +576,8 @@ __global__ __launch_bounds__(256) void k_trisolve_level(int32_t t0, int32_t nlon
         for (int jb = 0; jb < k; jb += 64) {
             const int j = jb + lane;
             if (j >= k) break;
-            T acc = seg_apply<T>(row_start<T, BWD>(i, j, k, d, in), q0, qs, col, val, out, k, j);
-            if (!BWD) acc = seg_apply<T>(acc, qs, q1, col, val, out, k, j);
+            T acc = seg_apply<T, kLongRow>(row_start<T, BWD>(i, j, k, d, in), q0, qs, col, val, out, k, j);
+            if (!BWD) acc = seg_apply<T, kLongRow>(acc, qs, q1, col, val, out, k, j);
             out[(size_t)i * k + j] = acc;
         }
     }
@@ -593,7 +600,7 @@ __global__ __launch_bounds__(256) void k_tri_tail_head(int32_t t0, int32_t t1, c
     for (int jb = 0; jb < k; jb += 64) {
         const int j = jb + SS_LANE;
         if (j >= k) break;
-        out[(size_t)i * k + j] = seg_apply<T>(in[(size_t)i * k + j], q0, qs, col, val, out, k, j);
+        out[(size_t)i * k + j] = seg_apply<T, (1 << 30)>(in[(size_t)i * k + j], q0, qs, col, val, out, k, j);
     }
 }
 
@@ -631,7 +638,7 @@ __global__ __launch_bounds__(1024) void k_tri_small(int32_t l0, int32_t l1, cons
             const int j = jb + lane;
             T acc = (T)0;
             if (act && sub == 0 && j < k) acc = BWD ? row_start<T, true>(i, j, k, d, in) : out[(size_t)i * k + j];
-            acc = seg_apply_shared<T>(acc, qs, q1, act, g, sub, slot * g, sm, col, val, out, k, j, lane);
+            acc = seg_apply_shared<T, kMaxParts>(acc, qs, q1, act, g, sub, slot * g, sm, col, val, out, k, j, lane);
             if (act && sub == 0 && j < k) out[(size_t)i * k + j] = acc;
         }
         __syncthreads();
@@ -957,18 +964,20 @@ template void launch_update_xr_rr<float>(hipStream_t, int32_t, int, const double
                                          float *, float *, double *);
 
 template <typename T, bool BWD>
-void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nlong, int32_t t1, int rpw, const int32_t *rows,
-                           const int32_t *ptr, const int32_t *split, const int32_t *col, const T *val,
-                           const double *diag, int k, const T *in, T *out) {
+void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nlong, int32_t t1, int waves, int rpw,
+                           const int32_t *rows, const int32_t *ptr, const int32_t *split, const int32_t *col,
+                           const T *val, const double *diag, int k, const T *in, T *out) {
     if (t1 <= t0) return;
     if (rpw < 1) rpw = 1;
+    if (waves < 4 || waves > kLevelWaves || (waves & (waves - 1))) waves = 4;
     const int64_t short_rows = (int64_t)t1 - t0 - nlong;
-    const unsigned grid = (unsigned)nlong + (unsigned)((short_rows + 4LL * rpw - 1) / (4LL * rpw));
-    hipLaunchKernelGGL((k_trisolve_level<T, BWD>), dim3(grid), dim3(256), 0, s, t0, nlong, t1, rpw, rows, ptr, split,
-                       col, val, diag, k, in, out);
+    const int64_t per_block = (int64_t)waves * rpw;
+    const unsigned grid = (unsigned)nlong + (unsigned)((short_rows + per_block - 1) / per_block);
+    hipLaunchKernelGGL((k_trisolve_level<T, BWD>), dim3(grid), dim3(64 * waves), 0, s, t0, nlong, t1, rpw, rows, ptr,
+                       split, col, val, diag, k, in, out);
 }
 #define SS_INST_TRI(T, B)                                                                                      \
-    template void launch_trisolve_level<T, B>(hipStream_t, int32_t, int32_t, int32_t, int, const int32_t *,     \
+    template void launch_trisolve_level<T, B>(hipStream_t, int32_t, int32_t, int32_t, int, int, const int32_t *,\
                                               const int32_t *, const int32_t *, const int32_t *, const T *,      \
                                               const double *, int, const T *, T *);
 SS_INST_TRI(double, false)

@@ -20,12 +20,13 @@
 
 namespace ss {
 
-constexpr int kLongRow = 64;      // rows longer than this get a block of 4 waves per level launch
+constexpr int kLongRow = 32;      // rows longer than this get a whole block per level launch
 constexpr int kSmallRows = 16;    // levels with at most this many rows go to the small-level kernel
 
 static_assert(kSmallRows == 16, "the small-level kernel runs 16 waves, at most one row each");
 
-constexpr int seg_parts(int len) { return len <= 8 ? 1 : (len <= 32 ? 4 : 8); }
-constexpr int kMaxParts = 8;
+constexpr int seg_parts(int len) { return len <= 8 ? 1 : (len <= 32 ? 4 : (len <= 128 ? 8 : 16)); }
+constexpr int kMaxParts = 16;
+constexpr int kLevelWaves = 16;   // wavefronts per block of a per-level launch (one long row, or 16 short rows)
 
 }  // namespace ss
