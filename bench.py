@@ -42,6 +42,22 @@ def workload(name):
         w = graphs.loguniform_weights(len(u), seed=0)
         return dict(name=name, desc="3D grid 64^3, weights 10^U(-3,3), k=48, FP64, PCG rel 1e-8, eps=0.3",
                     n=n, u=u, v=v, w=w, k=48, eps=0.3)
+    if name == "chunglu5m":
+        n, u, v = graphs.chung_lu(5_000_000, 50_000_000, gamma=2.2, seed=0)
+        w = graphs.uniform_weights(len(u), seed=0)
+        return dict(name=name, desc="Chung-Lu n=5e6 m=5e7 gamma=2.2, weights U[0.5,2], k=64, mixed precision, "
+                    "PCG rel 1e-8, eps=0.3", n=n, u=u, v=v, w=w, k=64, eps=0.3, precision="mixed")
+    if name == "rgg512":
+        import numpy as np
+        rng = np.random.default_rng(512)
+        sizes = np.round(10.0 ** rng.uniform(3.0, math.log10(2e5), 512)).astype(np.int64)
+        gs = []
+        for i, ni in enumerate(sizes):
+            _, u, v = graphs.rgg(int(ni), seed=1000 + i, mean_degree=8.0)
+            gs.append((int(ni), u, v, np.ones(len(u))))
+        return dict(name=name, desc="512 random geometric graphs, n log-uniform [1e3, 2e5], mean degree 8, "
+                    "unit weights, k=32 each, mixed precision, PCG rel 1e-8, eps=0.3", batch=gs, k=32, eps=0.3,
+                    precision="mixed", n=int(sizes.sum()), m=int(sum(len(g[1]) for g in gs)))
     if name == "grid2d64":
         n, u, v = graphs.grid2d(64, 64)
         import numpy as np
@@ -50,11 +66,10 @@ def workload(name):
     raise SystemExit("unknown workload " + name)
 
 
-def kernel_bytes(W, G, k, kept, s=8):
+def kernel_bytes(n, m, G, k, kept, s=8):
     """Algorithmic (compulsory) HBM bytes per launch of each kernel kind:
     every array counted once per pass, gathers beyond first touch not
-    counted."""
-    n, m = W["n"], len(W["u"])
+    counted.  n, m: total vertices / edges of the (batched) system."""
     nnzL = n + 2 * m
     fi, bi = G.schedule_info(0), G.schedule_info(1)
 
@@ -65,10 +80,22 @@ def kernel_bytes(W, G, k, kept, s=8):
         b = info["level_rows"] * (4 + 4 + 2 * k * s + extra_per_row) + info["level_nnz"] * (4 + s)
         return b / info["level_launches"]
 
-    def small(info, extra_per_row, passes):
+    def small(info, extra_per_row, passes, extra_launches):
+        # all runs of one application over their launches (runs, plus the
+        # forward trailing run's head-segment launch)
         if not info["small_levels"]:
             return None
-        return info["small_rows"] * (4 + 4 + passes * 2 * k * s + extra_per_row) + info["small_nnz"] * (4 + s)
+        total = info["small_rows"] * (4 + 4 + passes * 2 * k * s + extra_per_row) + info["small_nnz"] * (4 + s)
+        return total / (info["small_runs"] + extra_launches)
+
+    def core(info):
+        # dense cores, per launch: rows' outside entries (forward), the
+        # lower triangles once, every core row read and written once
+        if not info["core_rows"]:
+            return None
+        total = info["core_outside_nnz"] * (4 + s) + info["core_rows"] * (4 + 2 * k * s) + \
+            info["core_entries"] // 2 * s
+        return total / (1 + info["core_steps"])
 
     return {
         # sketch: incidence lists (edge id + signed sqrt(w)) and Y
@@ -77,8 +104,10 @@ def kernel_bytes(W, G, k, kept, s=8):
         "spmm": 4 * (n + 1) + nnzL * (4 + s) + 2 * n * k * s,
         "tri_fwd": per_level(fi, 0),
         "tri_bwd": per_level(bi, 8),
-        "tri_fwd_small": small(fi, 4, 2),
-        "tri_bwd_small": small(bi, 8, 1),
+        "tri_fwd_small": small(fi, 0, 1, 0),
+        "tri_bwd_small": small(bi, 8, 1, 0),
+        "core_fwd": core(fi),
+        "core_bwd": core(bi),
         # ER/sample: edges (u, v, w), Z once, p out, mask out, kept (u, v, w) out
         "er_sample": m * (4 + 4 + 8) + n * k * s + 8 * m + m + kept * 16,
         "vector": None,
@@ -126,13 +155,147 @@ def cpu_baseline(W, ordering_seed=0, columns=1):
     }
 
 
+def summarize(prof, kb, t_timed):
+    """Per-kernel averages and the roofline of the dominant kernel kind."""
+    kernels = {}
+    for name, st in prof.items():
+        if st["launches"]:
+            avg = st["ms_total"] / st["launches"]
+            kernels[name] = {"launches": st["launches"], "avg_ms": avg, "bytes_per_launch": kb[name],
+                             "GBps": (kb[name] / (avg * 1e-3) / 1e9) if kb[name] else None,
+                             "share": st["ms_total"] / (t_timed * 1e3)}
+    roofline = None
+    ranked = [x for x in kernels if kernels[x]["bytes_per_launch"]]
+    if ranked:
+        dom = max(ranked, key=lambda x: kernels[x]["avg_ms"] * kernels[x]["launches"])
+        kd = kernels[dom]
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": round(kd["GBps"], 1), "peak": PEAK_HBM_GBS,
+                    "unit": "GB/s", "frac": round(kd["GBps"] / PEAK_HBM_GBS, 4), "traffic": None,
+                    "bytes_per_launch": kd["bytes_per_launch"], "avg_launch_ms": kd["avg_ms"]}
+    return kernels, roofline
+
+
+def cpu_baseline_batch(W, count=4):
+    """The oracle on the first `count` graphs of the batch (every column,
+    natural-order factors, one thread): edges/s of that bounded sample."""
+    import numpy as np
+    import oracle
+    import spec_spars_amd as ss
+    from spec_spars_amd import graphs
+    k = W["k"]
+    t_total, m_total, iters = 0.0, 0, 0
+    for n, u, v, w in W["batch"][:count]:
+        cp, ri, va = oracle.laplacian(n, u, v, w)
+        f = ss.Factor(cp, ri, va, seed=0, ordering=ss.ORDER_NATURAL)
+        urp, uci, uva, dg = f.export()
+        t0 = time.perf_counter()
+        Y = oracle.sketch(u, v, w, n, k, seed=0, scale=1.0 / math.sqrt(k))
+        Z, it, rr = oracle.solve_lap(Y, cp, ri, va, urp, uci, uva, dg, stop_rule=1, tol=1e-8, max_iter=2000)
+        uu, vv, lv = oracle.upper_edges(n, cp, ri, va)
+        p = oracle.probs(lv, oracle.diff_norms(uu, vv, Z), graphs.beta_for(n, W["eps"]), k)
+        oracle.sample_reweight(n, uu, vv, lv, p, oracle.coins(0, len(uu)))
+        t_total += time.perf_counter() - t0
+        m_total += len(u)
+        iters = max(iters, int(it.max()))
+    return {"value": m_total / t_total, "unit": "edges/s", "cores": 1, "kind": "port",
+            "sample": "oracle on the first %d graphs of the batch (%d edges, all %d columns, natural-order "
+                      "factors, up to %d PCG iterations), one thread, %.1fs" % (count, m_total, k, iters, t_total),
+            "seconds_measured": t_total}
+
+
+def bench_batch(W, args):
+    """Many graphs as one resident batch: a step sparsifies all of them."""
+    from concurrent.futures import ThreadPoolExecutor
+    import spec_spars_amd as ss
+    from spec_spars_amd import graphs
+    gs, k = W["batch"], W["k"]
+    n, m = W["n"], W["m"]
+    prec = ss.PREC_FP64 if args.precision == "fp64" else ss.PREC_MIXED
+    ordering = {"natural": ss.ORDER_NATURAL, "random": ss.ORDER_RANDOM, "rounds": ss.ORDER_ROUNDS}[args.ordering]
+    log("workload %s: %d graphs, n=%d m=%d k=%d" % (W["name"], len(gs), n, m, k))
+    ctx = ss.Context(0)
+    t0 = time.perf_counter()
+    B = ss.Batch(ctx, gs)
+    t_graph = time.perf_counter() - t0
+    t0 = time.perf_counter()
+
+    def build(g):
+        return ss.Factor(*graphs.laplacian_csc(*g), seed=0, ordering=ordering)
+
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        fs = list(ex.map(build, gs))
+    t_factor = time.perf_counter() - t0
+    fnnz = sum(f.info()["nnz"] for f in fs)
+    t0 = time.perf_counter()
+    B.set_factors(fs)
+    t_upload = time.perf_counter() - t0
+    log("batch %.1fs factors %.1fs (nnz %d) upload %.1fs" % (t_graph, t_factor, fnnz, t_upload))
+    tune = ctx.tuning()
+    if not args.no_tune:
+        t0 = time.perf_counter()
+        tune = ctx.autotune(B, k, prec, reps=3)
+        log("autotune %.1fs: %s" % (time.perf_counter() - t0, tune))
+    betas = [graphs.beta_for(g[0], W["eps"]) for g in gs]
+    solve = ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-8, max_iter=2000, precision=prec)
+
+    def step():
+        return B.sparsify(k, betas, sketch_seeds=[0] * len(gs), sampling_seeds=[0] * len(gs), solve=solve)
+
+    for i in range(args.warmup):
+        reps = step()
+        log("warmup %d: %.1f ms (sketch %.1f solve %.1f sample %.1f) iters %d kept %d" %
+            (i, reps[0]["ms_total"], reps[0]["ms_sketch"], reps[0]["ms_solve"], reps[0]["ms_sample"],
+             max(r["max_iters"] for r in reps), sum(r["kept"] for r in reps)))
+    B.profile(not args.no_profile)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        reps = step()
+        log("step %d: %.1f ms" % (i, reps[0]["ms_total"]))
+    t_timed = time.perf_counter() - t0
+    prof = B.profile_read()
+    B.profile(False)
+    ms_per_step = 1000.0 * t_timed / args.steps
+    kept = sum(r["kept"] for r in reps)
+    kb = kernel_bytes(n, m, B, k, kept, s=8 if prec == ss.PREC_FP64 else 4)
+    kernels, roofline = summarize(prof, kb, t_timed)
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        cpu = cpu_baseline_batch(W)
+        log("cpu baseline %.3g edges/s (%s)" % (cpu["value"], cpu["sample"]))
+    r0 = reps[0]
+    out = {
+        "metric": METRIC, "value": m / (ms_per_step / 1000.0), "unit": "edges/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if prec == ss.PREC_FP64 else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": W["desc"], "graphs": len(gs), "n": n, "m": m, "k": k, "rhs": k * len(gs),
+                   "epsilon": W["eps"], "precision": args.precision, "ordering": args.ordering,
+                   "factor_nnz": fnnz, "pcg_iters": max(r["max_iters"] for r in reps),
+                   "refinements": r0["refinements"], "max_relres": max(r["max_relres"] for r in reps),
+                   "kept_edges": kept, "ms_sketch": r0["ms_sketch"], "ms_solve": r0["ms_solve"],
+                   "ms_sample": r0["ms_sample"], "rhs_solves_per_s": k * len(gs) / (r0["ms_solve"] / 1000.0),
+                   "host_factor_s": t_factor, "host_graph_s": t_graph, "upload_s": t_upload,
+                   "parallelism": "1 GPU", "tuning": tune,
+                   "schedule_fwd": B.schedule_info(0), "schedule_bwd": B.schedule_info(1)},
+        "roofline": roofline,
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    B.close()
+    ctx.close()
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="grid3d128")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default=None,
+                    help="default: the workload's (fp64 for the grids, mixed for chunglu5m and rgg512)")
     ap.add_argument("--ordering", choices=["natural", "random", "rounds"], default="rounds")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel event timing")
@@ -148,6 +311,10 @@ def main():
     from spec_spars_amd import graphs
 
     W = workload(args.workload)
+    if args.precision is None:
+        args.precision = W.get("precision", "fp64")
+    if "batch" in W:
+        return bench_batch(W, args)
     n, u, v, w, k = W["n"], W["u"], W["v"], W["w"], W["k"]
     m = len(u)
     prec = ss.PREC_FP64 if args.precision == "fp64" else ss.PREC_MIXED
@@ -199,22 +366,8 @@ def main():
     value = m / (ms_per_step / 1000.0)
     last = reps[-1]
 
-    kb = kernel_bytes(W, G, k, last["kept"], s=8 if prec == ss.PREC_FP64 else 4)
-    kernels = {}
-    for name, st in prof.items():
-        if st["launches"]:
-            avg = st["ms_total"] / st["launches"]
-            kernels[name] = {"launches": st["launches"], "avg_ms": avg, "bytes_per_launch": kb[name],
-                             "GBps": (kb[name] / (avg * 1e-3) / 1e9) if kb[name] else None,
-                             "share": st["ms_total"] / (t_timed * 1e3)}
-    roofline = None
-    ranked = [x for x in kernels if kernels[x]["bytes_per_launch"]]
-    if ranked:
-        dom = max(ranked, key=lambda x: kernels[x]["avg_ms"] * kernels[x]["launches"])
-        kd = kernels[dom]
-        roofline = {"bound": "hbm", "kernel": dom, "achieved": round(kd["GBps"], 1), "peak": PEAK_HBM_GBS,
-                    "unit": "GB/s", "frac": round(kd["GBps"] / PEAK_HBM_GBS, 4), "traffic": None,
-                    "bytes_per_launch": kd["bytes_per_launch"], "avg_launch_ms": kd["avg_ms"]}
+    kb = kernel_bytes(n, m, G, k, last["kept"], s=8 if prec == ss.PREC_FP64 else 4)
+    kernels, roofline = summarize(prof, kb, t_timed)
     copy_ms = ctx.bench_copy(1 << 30, reps=10)
     copy_gbs = 2.0 * (1 << 30) / (copy_ms * 1e-3) / 1e9
 

@@ -254,10 +254,12 @@ int ss_batch_read_probs(const ss_graph *g, int32_t i, double *p);
 #define SS_KERNEL_TRI_FWD 2     /* (c) one level of the forward solve        */
 #define SS_KERNEL_TRI_BWD 3     /* (c) one level of the backward solve       */
 #define SS_KERNEL_ER_SAMPLE 4   /* (d) ER + coin + sample + compaction       */
-#define SS_KERNEL_TRI_FWD_SMALL 5 /* forward small-level run (2 kernels)     */
-#define SS_KERNEL_TRI_BWD_SMALL 6 /* backward small-level run                */
+#define SS_KERNEL_TRI_FWD_SMALL 5 /* forward run of small levels (1 workgroup) */
+#define SS_KERNEL_TRI_BWD_SMALL 6 /* backward run of small levels            */
 #define SS_KERNEL_VECTOR 7      /* PCG vector updates and reductions         */
-#define SS_KERNEL_COUNT 8
+#define SS_KERNEL_CORE_FWD 8    /* dense cores, forward: outside entries + block steps */
+#define SS_KERNEL_CORE_BWD 9    /* dense cores, backward: y/D + block steps  */
+#define SS_KERNEL_COUNT 10
 typedef struct ss_kernel_stat {
     double ms_total;
     int64_t launches;
@@ -265,10 +267,16 @@ typedef struct ss_kernel_stat {
 int ss_graph_profile(ss_graph *g, int enable);
 /* Shape of the uploaded solve schedules: per direction (0 = forward,
  * 1 = backward) the level count, the rows and entries handled by per-level
- * launches and by the single-workgroup small-level run. */
+ * launches and by the single-workgroup runs of small levels (small_runs of
+ * them, small_levels levels in all), and the dense cores: core_rows rows of
+ * the graphs' trailing small levels, core_entries = sum of core_rows_g^2
+ * stored values, core_steps block steps, core_outside_nnz entries of core
+ * rows outside the cores (forward only). */
 typedef struct ss_schedule_info {
-    int32_t levels, level_launches, small_levels, reserved;
+    int32_t levels, level_launches, small_levels, small_runs;
     int64_t level_rows, level_nnz, small_rows, small_nnz;
+    int64_t core_rows, core_entries, core_outside_nnz;
+    int32_t core_steps, core_graphs;
 } ss_schedule_info;
 int ss_graph_schedule_info(const ss_graph *g, int direction, ss_schedule_info *out);
 int ss_graph_profile_read(const ss_graph *g, ss_kernel_stat *out /* [SS_KERNEL_COUNT] */);

@@ -74,14 +74,18 @@ class HostParams(ctypes.Structure):
                 ("max_iter", ctypes.c_int32), ("reserved", ctypes.c_int32), ("tol", ctypes.c_double)]
 
 
-KERNELS = ["sketch", "spmm", "tri_fwd", "tri_bwd", "er_sample", "tri_fwd_small", "tri_bwd_small", "vector"]
+KERNELS = ["sketch", "spmm", "tri_fwd", "tri_bwd", "er_sample", "tri_fwd_small", "tri_bwd_small", "vector",
+           "core_fwd", "core_bwd"]
 
 
 class ScheduleInfo(ctypes.Structure):
     _fields_ = [("levels", ctypes.c_int32), ("level_launches", ctypes.c_int32),
-                ("small_levels", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("small_levels", ctypes.c_int32), ("small_runs", ctypes.c_int32),
                 ("level_rows", ctypes.c_int64), ("level_nnz", ctypes.c_int64),
-                ("small_rows", ctypes.c_int64), ("small_nnz", ctypes.c_int64)]
+                ("small_rows", ctypes.c_int64), ("small_nnz", ctypes.c_int64),
+                ("core_rows", ctypes.c_int64), ("core_entries", ctypes.c_int64),
+                ("core_outside_nnz", ctypes.c_int64), ("core_steps", ctypes.c_int32),
+                ("core_graphs", ctypes.c_int32)]
 
 
 class KernelStat(ctypes.Structure):

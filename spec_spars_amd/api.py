@@ -271,7 +271,7 @@ class Graph:
     def schedule_info(self, direction):
         si = _lib.ScheduleInfo()
         check(load().ss_graph_schedule_info(self._h, direction, ctypes.byref(si)))
-        return {f: getattr(si, f) for f, _ in _lib.ScheduleInfo._fields_ if f != "reserved"}
+        return {f: getattr(si, f) for f, _ in _lib.ScheduleInfo._fields_}
 
     def read_probs(self):
         p = np.zeros(self.m, np.float64)
@@ -399,4 +399,4 @@ class Batch:
     def schedule_info(self, direction):
         si = _lib.ScheduleInfo()
         check(load().ss_graph_schedule_info(self._h, direction, ctypes.byref(si)))
-        return {f: getattr(si, f) for f, _ in _lib.ScheduleInfo._fields_ if f != "reserved"}
+        return {f: getattr(si, f) for f, _ in _lib.ScheduleInfo._fields_}

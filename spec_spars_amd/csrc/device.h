@@ -101,9 +101,17 @@ struct ss_graph {
     std::vector<int32_t> f_level_ptr, b_level_ptr;   // host copies: launch bounds per level
     std::vector<int32_t> f_level_long, b_level_long; // long rows leading each level
     std::vector<int32_t> f_level_waves, b_level_waves;   // wavefronts per block of each level launch
-    int32_t f_small_lo = 0, f_small_hi = 0, b_small_lo = 0, b_small_hi = 0;
-    int32_t *f_split = nullptr;
+    std::vector<std::pair<int32_t, int32_t>> f_runs, b_runs; // runs of small levels (single-workgroup launches)
     int32_t *f_lp_dev = nullptr, *b_lp_dev = nullptr;   // level_ptr on the device
+    // dense cores: per graph, the rows of its trailing small forward levels
+    // (at most kCoreMaxRows), solved by blocked substitution
+    int32_t f_core_t0 = 0, f_nrows = 0;   // forward positions [f_core_t0, f_nrows): core rows' outside entries
+    int core_max = 0, core_steps = 0;
+    int64_t core_total = 0, core_entries = 0;
+    int32_t *core_n = nullptr, *core_off = nullptr, *core_vert = nullptr;
+    int64_t *dense_off = nullptr;
+    double *Ld = nullptr, *Ltd = nullptr;   // L and L^T per core, FP64
+    float *Lf = nullptr, *Ltf = nullptr;    // FP32 copies
     ss_schedule_info sched_info[2] = {};
     // status
     int *status = nullptr;           // {nactive, err}

@@ -378,57 +378,112 @@ int batch_upload_factors(ss_graph *g, const std::vector<const Factor *> &factors
         Fp = &Fc;
     }
     const Factor &F = *Fp;
-    // each graph's own forward tail start fixes its rows' segments
-    std::vector<int32_t> tail_lo((size_t)F.n);
+    // each graph's dense core: the rows of its own trailing small forward levels
+    std::vector<char> core((size_t)F.n, 0);
+    std::vector<int32_t> core_n((size_t)g->G, 0), core_off((size_t)g->G + 1, 0), core_vert;
+    std::vector<int64_t> dense_off((size_t)g->G + 1, 0);
+    std::vector<int32_t> ci((size_t)F.n, -1);   // position -> index within its core
     for (int i = 0; i < g->G; i++) {
-        const int32_t lo = forward_tail_start(*factors[i], kSmallRows);
-        for (int64_t p = g->row_off[i]; p < g->row_off[i + 1]; p++) tail_lo[p] = lo;
+        const int32_t lo = forward_tail_start(*factors[i], kSmallRows, kCoreMaxRows);
+        for (int64_t p = g->row_off[i]; p < g->row_off[i + 1]; p++)
+            if (F.lvl_fwd[p] > lo) {
+                core[p] = 1;
+                ci[p] = core_n[i]++;
+                core_vert.push_back(F.order[p]);
+            }
+        core_off[i + 1] = core_off[i] + core_n[i];
+        dense_off[i + 1] = dense_off[i] + (int64_t)core_n[i] * core_n[i];
     }
     Schedule fw, bw;
-    SS_TRY(factor_schedules(F, fw, bw, &tail_lo, kLongRow, kSmallRows));
-    // D per vertex (the factor keeps it per elimination position)
+    SS_TRY(factor_schedules(F, core, fw, bw, kLongRow, kSmallRows));
+    // dense L and L^T of every core (position space: U row p lists q > p with L[q][p])
+    const int64_t dense_total = dense_off[g->G];
+    std::vector<double> Ld((size_t)dense_total, 0.0), Ltd((size_t)dense_total, 0.0);
+    {
+        int gi = 0;
+        for (int64_t p = 0; p < F.n; p++) {
+            while (p >= g->row_off[gi + 1]) gi++;
+            if (!core[p]) continue;
+            const int64_t c = core_n[gi], base = dense_off[gi];
+            for (int32_t t = F.urowptr[p]; t < F.urowptr[p + 1]; t++) {
+                const int32_t q = F.ucol[t];
+                if (q == p || !core[q]) continue;
+                Ld[(size_t)(base + (int64_t)ci[q] * c + ci[p])] = F.uval[t];
+                Ltd[(size_t)(base + (int64_t)ci[p] * c + ci[q])] = F.uval[t];
+            }
+        }
+    }
     std::vector<double> vdiag((size_t)F.n);
     for (int64_t p = 0; p < F.n; p++) vdiag[F.order[p]] = F.diag[p];
     g->arena_factor.release();
     g->has_factor = false;
     auto &A = g->arena_factor;
     const int64_t n = g->N;
+    const int64_t fn = fw.n, bn = bw.n;
     const int64_t fnz = (int64_t)fw.col.size(), bnz = (int64_t)bw.col.size();
-    SS_TRY(A.alloc(&g->f_rows, n));
-    SS_TRY(A.alloc(&g->f_ptr, n + 1));
+    SS_TRY(A.alloc(&g->f_rows, fn));
+    SS_TRY(A.alloc(&g->f_ptr, fn + 1));
     SS_TRY(A.alloc(&g->f_col, fnz));
     SS_TRY(A.alloc(&g->f_val, fnz));
     SS_TRY(A.alloc(&g->f_valf, fnz));
-    SS_TRY(A.alloc(&g->b_rows, n));
-    SS_TRY(A.alloc(&g->b_ptr, n + 1));
+    SS_TRY(A.alloc(&g->b_rows, bn));
+    SS_TRY(A.alloc(&g->b_ptr, bn + 1));
     SS_TRY(A.alloc(&g->b_col, bnz));
     SS_TRY(A.alloc(&g->b_val, bnz));
     SS_TRY(A.alloc(&g->b_valf, bnz));
     SS_TRY(A.alloc(&g->diag, n));
-    SS_TRY(A.alloc(&g->f_split, n));
     SS_TRY(A.alloc(&g->f_lp_dev, fw.level_ptr.size()));
     SS_TRY(A.alloc(&g->b_lp_dev, bw.level_ptr.size()));
+    SS_TRY(A.alloc(&g->core_n, core_n.size()));
+    SS_TRY(A.alloc(&g->core_off, core_off.size()));
+    SS_TRY(A.alloc(&g->core_vert, core_vert.size()));
+    SS_TRY(A.alloc(&g->dense_off, dense_off.size()));
+    SS_TRY(A.alloc(&g->Ld, dense_total));
+    SS_TRY(A.alloc(&g->Ltd, dense_total));
+    SS_TRY(A.alloc(&g->Lf, dense_total));
+    SS_TRY(A.alloc(&g->Ltf, dense_total));
     hipStream_t s = g->ctx->stream;
     std::vector<float> fvf(fw.val.begin(), fw.val.end()), bvf(bw.val.begin(), bw.val.end());
-    SS_TRY(h2d(s, g->f_rows, fw.rows.data(), n));
-    SS_TRY(h2d(s, g->f_ptr, fw.ptr.data(), n + 1));
+    std::vector<float> Lf(Ld.begin(), Ld.end()), Ltf(Ltd.begin(), Ltd.end());
+    SS_TRY(h2d(s, g->f_rows, fw.rows.data(), fn));
+    SS_TRY(h2d(s, g->f_ptr, fw.ptr.data(), fn + 1));
     SS_TRY(h2d(s, g->f_col, fw.col.data(), fnz));
     SS_TRY(h2d(s, g->f_val, fw.val.data(), fnz));
     SS_TRY(h2d(s, g->f_valf, fvf.data(), fnz));
-    SS_TRY(h2d(s, g->b_rows, bw.rows.data(), n));
-    SS_TRY(h2d(s, g->b_ptr, bw.ptr.data(), n + 1));
+    SS_TRY(h2d(s, g->b_rows, bw.rows.data(), bn));
+    SS_TRY(h2d(s, g->b_ptr, bw.ptr.data(), bn + 1));
     SS_TRY(h2d(s, g->b_col, bw.col.data(), bnz));
     SS_TRY(h2d(s, g->b_val, bw.val.data(), bnz));
     SS_TRY(h2d(s, g->b_valf, bvf.data(), bnz));
     SS_TRY(h2d(s, g->diag, vdiag.data(), n));
-    SS_TRY(h2d(s, g->f_split, fw.split.data(), n));
     SS_TRY(h2d(s, g->f_lp_dev, fw.level_ptr.data(), fw.level_ptr.size()));
     SS_TRY(h2d(s, g->b_lp_dev, bw.level_ptr.data(), bw.level_ptr.size()));
+    SS_TRY(h2d(s, g->core_n, core_n.data(), core_n.size()));
+    SS_TRY(h2d(s, g->core_off, core_off.data(), core_off.size()));
+    SS_TRY(h2d(s, g->core_vert, core_vert.data(), core_vert.size()));
+    SS_TRY(h2d(s, g->dense_off, dense_off.data(), dense_off.size()));
+    SS_TRY(h2d(s, g->Ld, Ld.data(), Ld.size()));
+    SS_TRY(h2d(s, g->Ltd, Ltd.data(), Ltd.size()));
+    SS_TRY(h2d(s, g->Lf, Lf.data(), Lf.size()));
+    SS_TRY(h2d(s, g->Ltf, Ltf.data(), Ltf.size()));
     SS_HIP(hipStreamSynchronize(s));
     g->f_level_ptr = fw.level_ptr;
     g->b_level_ptr = bw.level_ptr;
     g->f_level_long = fw.level_long;
     g->b_level_long = bw.level_long;
+    g->f_runs = fw.runs;
+    g->b_runs = bw.runs;
+    g->f_core_t0 = fw.core_t0;
+    g->f_nrows = (int32_t)fn;
+    g->core_max = 0;
+    int core_graphs = 0;
+    for (int i = 0; i < g->G; i++) {
+        g->core_max = std::max(g->core_max, core_n[i]);
+        core_graphs += core_n[i] > 0;
+    }
+    g->core_steps = (g->core_max + kCoreBlock - 1) / kCoreBlock;
+    g->core_total = (int64_t)core_vert.size();
+    g->core_entries = dense_total;
     // a level's long rows get as many waves as their longest row has parts (4..16)
     for (int dir = 0; dir < 2; dir++) {
         const Schedule &S = dir == 0 ? fw : bw;
@@ -446,20 +501,25 @@ int batch_upload_factors(ss_graph *g, const std::vector<const Factor *> &factors
         ss_schedule_info &I = g->sched_info[dir];
         I = ss_schedule_info{};
         I.levels = S.depth;
-        I.small_levels = S.small_hi - S.small_lo;
+        std::vector<char> in_run((size_t)S.depth, 0);
+        for (const auto &r : S.runs)
+            for (int32_t l = r.first; l < r.second; l++) in_run[l] = 1;
+        for (int32_t l = 0; l < S.depth; l++) I.small_levels += in_run[l];
         I.level_launches = S.depth - I.small_levels;
+        I.small_runs = (int32_t)S.runs.size();
         for (int32_t l = 0; l < S.depth; l++) {
-            const bool sm = l >= S.small_lo && l < S.small_hi;
+            const bool sm = in_run[l] != 0;
             const int64_t rows = S.level_ptr[l + 1] - S.level_ptr[l];
             const int64_t nz = S.ptr[S.level_ptr[l + 1]] - S.ptr[S.level_ptr[l]];
             (sm ? I.small_rows : I.level_rows) += rows;
             (sm ? I.small_nnz : I.level_nnz) += nz;
         }
+        I.core_rows = g->core_total;
+        I.core_entries = dense_total;
+        I.core_outside_nnz = dir == 0 ? (int64_t)(fw.ptr[fn] - fw.ptr[fw.core_t0]) : 0;
+        I.core_steps = g->core_steps;
+        I.core_graphs = core_graphs;
     }
-    g->f_small_lo = fw.small_lo;
-    g->f_small_hi = fw.small_hi;
-    g->b_small_lo = bw.small_lo;
-    g->b_small_hi = bw.small_hi;
     g->factor_nnz = F.nnz;
     g->depth_fwd = F.depth_fwd;
     g->depth_bwd = F.depth_bwd;
@@ -536,48 +596,73 @@ template <> const float *avals<float>(const ss_graph *g) { return g->valf; }
 
 static Tiles rtiles(const ss_graph *g) { return Tiles{g->rt_start, g->rt_graph, g->nrt, g->tile_waves}; }
 
-// Zp = L^{-T} D^+ L^{-1} R (W holds the forward result y): one launch per
-// level plus the single-workgroup small-level runs.  rz_partial receives the
-// per-tile partials of r . Zp = y^T D^+ y.
-template <typename T>
-static int precond(ss_graph *g, int k, const T *R, T *W, T *Zp, double *rz_partial) {
+template <typename T> static const T *core_L(const ss_graph *g, bool transposed);
+template <> const double *core_L<double>(const ss_graph *g, bool t) { return t ? g->Ltd : g->Ld; }
+template <> const float *core_L<float>(const ss_graph *g, bool t) { return t ? g->Ltf : g->Lf; }
+
+// One direction of the sparse part of the preconditioner: per-level launches
+// between the runs of small levels, one single-workgroup launch per run.
+template <typename T, bool BWD>
+static void solve_levels(ss_graph *g, int k, const T *in, T *out) {
     hipStream_t s = g->ctx->stream;
     const int rpw = g->ctx->tune.level_rows_per_wave;
-    {
-        // the level launches are timed as one group over their count
-        ProfScope ps(g, SS_KERNEL_TRI_FWD, g->f_small_lo);
-        for (int32_t l = 0; l < g->f_small_lo; l++)
-            launch_trisolve_level<T, false>(s, g->f_level_ptr[l], g->f_level_long[l], g->f_level_ptr[l + 1],
-                                            g->f_level_waves[l], rpw,
-                                            g->f_rows, g->f_ptr, g->f_split, g->f_col, fvals<T>(g), g->diag, k, R,
-                                            W);
-    }
-    {
-        ProfScope ps(g, SS_KERNEL_TRI_FWD_SMALL, g->f_small_lo < g->f_small_hi ? 2 : 0);
-        if (g->f_small_lo < g->f_small_hi) {
-            launch_tri_tail_head<T>(s, g->f_level_ptr[g->f_small_lo], (int32_t)g->N, g->f_rows, g->f_ptr,
-                                    g->f_split, g->f_col, fvals<T>(g), k, R, W);
-            launch_tri_small<T, false>(s, g->f_small_lo, g->f_small_hi, g->f_lp_dev, g->f_rows, g->f_ptr,
-                                       g->f_split, g->f_col, fvals<T>(g), g->diag, k, R, W);
+    const std::vector<int32_t> &lp = BWD ? g->b_level_ptr : g->f_level_ptr;
+    const std::vector<int32_t> &nlong = BWD ? g->b_level_long : g->f_level_long;
+    const std::vector<int32_t> &waves = BWD ? g->b_level_waves : g->f_level_waves;
+    const auto &runs = BWD ? g->b_runs : g->f_runs;
+    const int32_t *rows = BWD ? g->b_rows : g->f_rows, *ptr = BWD ? g->b_ptr : g->f_ptr;
+    const int32_t *col = BWD ? g->b_col : g->f_col, *lp_dev = BWD ? g->b_lp_dev : g->f_lp_dev;
+    const T *val = BWD ? bvals<T>(g) : fvals<T>(g);
+    const int32_t depth = (int32_t)lp.size() - 1;
+    size_t r = 0;
+    for (int32_t l = 0; l < depth;) {
+        if (r < runs.size() && runs[r].first == l) {
+            const int32_t e = runs[r++].second;
+            ProfScope ps(g, BWD ? SS_KERNEL_TRI_BWD_SMALL : SS_KERNEL_TRI_FWD_SMALL);
+            launch_tri_small<T, BWD>(s, l, e, lp_dev, rows, ptr, col, val, g->diag, k, in, out);
+            l = e;
+            continue;
         }
+        // the level launches up to the next run are timed as one group
+        const int32_t e = r < runs.size() ? runs[r].first : depth;
+        ProfScope ps(g, BWD ? SS_KERNEL_TRI_BWD : SS_KERNEL_TRI_FWD, e - l);
+        for (; l < e; l++)
+            launch_trisolve_level<T, BWD>(s, lp[l], nlong[l], lp[l + 1], waves[l], rpw, rows, ptr, col, val, g->diag,
+                                          k, in, out);
     }
+}
+
+// The dense cores of one direction: forward = their rows' entries outside
+// the cores, then the block steps; backward = y / D, then the block steps.
+template <typename T, bool BWD>
+static void solve_cores(ss_graph *g, int k, const T *in, T *out) {
+    if (g->core_total == 0) return;
+    hipStream_t s = g->ctx->stream;
+    ProfScope ps(g, BWD ? SS_KERNEL_CORE_BWD : SS_KERNEL_CORE_FWD, 1 + g->core_steps);
+    if (BWD)
+        launch_core_bwd_init<T>(s, (int32_t)g->core_total, g->core_vert, g->diag, k, in, out);
+    else
+        launch_core_head<T>(s, g->f_core_t0, g->f_nrows, g->f_rows, g->f_ptr, g->f_col, fvals<T>(g), k, in, out);
+    for (int st = 0; st < g->core_steps; st++)
+        launch_core_step<T, BWD>(s, st, g->G, g->core_max, g->core_n, g->core_off, g->dense_off, g->core_vert,
+                                 core_L<T>(g, BWD), k, out);
+}
+
+// Zp = L^{-T} D^+ L^{-1} R (W holds the forward result y).  rz_partial
+// receives the per-tile partials of r . Zp = y^T D^+ y.  Forward: the
+// sparse levels, then the dense cores (which only read finished rows);
+// backward: the cores first (their rows only need core rows), then the
+// sparse levels.
+template <typename T>
+static int precond(ss_graph *g, int k, const T *R, T *W, T *Zp, double *rz_partial) {
+    solve_levels<T, false>(g, k, R, W);
+    solve_cores<T, false>(g, k, R, W);
     {
         ProfScope ps(g, SS_KERNEL_VECTOR);
-        bt_tile_partial<T, T>(s, rtiles(g), k, W, W, g->diag, 2, rz_partial);
+        bt_tile_partial<T, T>(g->ctx->stream, rtiles(g), k, W, W, g->diag, 2, rz_partial);
     }
-    if (g->b_small_lo < g->b_small_hi) {
-        ProfScope ps(g, SS_KERNEL_TRI_BWD_SMALL);
-        launch_tri_small<T, true>(s, g->b_small_lo, g->b_small_hi, g->b_lp_dev, g->b_rows, g->b_ptr, nullptr,
-                                  g->b_col, bvals<T>(g), g->diag, k, W, Zp);
-    }
-    {
-        const int32_t nlev = (int32_t)g->b_level_ptr.size() - 1;
-        ProfScope ps(g, SS_KERNEL_TRI_BWD, nlev - g->b_small_hi);
-        for (int32_t l = g->b_small_hi; l < nlev; l++)
-            launch_trisolve_level<T, true>(s, g->b_level_ptr[l], g->b_level_long[l], g->b_level_ptr[l + 1],
-                                           g->b_level_waves[l], rpw,
-                                           g->b_rows, g->b_ptr, nullptr, g->b_col, bvals<T>(g), g->diag, k, W, Zp);
-    }
+    solve_cores<T, true>(g, k, W, Zp);
+    solve_levels<T, true>(g, k, W, Zp);
     SS_HIP(hipGetLastError());
     return SS_OK;
 }

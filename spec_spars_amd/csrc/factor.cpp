@@ -20,8 +20,11 @@
 // the value sort breaks ties by row.  The factor is therefore a pure
 // function of (Laplacian, seed), independent of scheduling.
 #include "ss_internal.h"
+#include "solve_shape.h"
 
 #include <algorithm>
+#include <cstdlib>
+#include <cstdio>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -174,7 +177,16 @@ static int factor_rounds(int64_t n, const int32_t *colptr, const int32_t *rowidx
     std::vector<Ent> work, col;
     std::vector<double> cum;
     rounds = 0;
+    // SPEC_SPARS_VERBOSE set: a progress line on stderr about every 20 s
+    const bool verbose = std::getenv("SPEC_SPARS_VERBOSE") != nullptr;
+    auto last_report = std::chrono::steady_clock::now();
     while (!live.empty()) {
+        if (verbose && std::chrono::steady_clock::now() - last_report > std::chrono::seconds(20)) {
+            last_report = std::chrono::steady_clock::now();
+            std::fprintf(stderr, "[factor] round %d: %zu of %lld vertices left, %zu factor entries\n", rounds,
+                         live.size(), (long long)n, crow.size());
+            std::fflush(stderr);
+        }
         const uint64_t rkey = mix64(seed ^ (0x94d049bb133111ebULL * (uint64_t)(rounds + 1)));
         for (int32_t v : live) {
             auto &a = adj[v];
@@ -347,55 +359,29 @@ void factor_levels(Factor &F) {
 // pull form (the vertices it waits for).  The backward schedule holds the
 // rows of L^T, i.e. the off-diagonal part of U's rows.  Every dependency
 // must have a strictly lower level -- the sync-free solve relies on it.
-int32_t forward_tail_start(const Factor &F, int32_t small_rows) {
+int32_t forward_tail_start(const Factor &F, int32_t small_rows, int64_t max_rows) {
     std::vector<int64_t> cnt((size_t)F.depth_fwd + 1, 0);
     for (int64_t p = 0; p < F.n; p++) cnt[F.lvl_fwd[p]]++;
     int32_t lo = F.depth_fwd;   // 0-based levels [lo, depth) form the run
-    while (lo > 0 && cnt[lo] <= small_rows) lo--;
+    int64_t rows = 0;
+    while (lo > 0 && cnt[lo] <= small_rows && rows + cnt[lo] <= max_rows) rows += cnt[lo--];
     return lo;
 }
 
-int factor_schedules(const Factor &F, Schedule &fwd, Schedule &bwd, const std::vector<int32_t> *tail_lo,
-                     int32_t long_row, int32_t small_rows) {
+int factor_schedules(const Factor &F, const std::vector<char> &core, Schedule &fwd, Schedule &bwd, int32_t long_row,
+                     int32_t small_rows) {
     const int64_t n = F.n;
-    // entries per position: forward = row of L (count of U rows naming it),
-    // backward = U's off-diagonal row
-    std::vector<int32_t> flen((size_t)n, 0), blen((size_t)n, 0);
+    if ((int64_t)core.size() != n) return fail(SS_ERR_INVALID, "schedule: core flags sized %zu, n=%lld", core.size(), (long long)n);
+    // strictly-lower rows of L (position space): transpose of U's off-diagonal;
+    // a core row keeps only its entries in columns outside the core
+    std::vector<int32_t> lcnt((size_t)n + 1, 0), blen((size_t)n, 0);
     for (int64_t i = 0; i < n; i++)
-        for (int32_t t = F.urowptr[i]; t < F.urowptr[i + 1]; t++)
-            if (F.ucol[t] != i) { flen[F.ucol[t]]++; blen[i]++; }
-    auto order = [&](const std::vector<int32_t> &lvl, const std::vector<int32_t> &len, Schedule &S,
-                     std::vector<int32_t> &pos_at) {
-        S.n = n;
-        S.depth = 0;
-        for (int64_t i = 0; i < n; i++) S.depth = std::max<int32_t>(S.depth, lvl[i]);
-        std::vector<int64_t> cnt((size_t)S.depth + 2, 0);
-        for (int64_t i = 0; i < n; i++) cnt[lvl[i]]++;
-        S.level_ptr.assign((size_t)S.depth + 1, 0);
-        int64_t acc = 0;
-        for (int32_t l = 1; l <= S.depth; l++) { S.level_ptr[l - 1] = (int32_t)acc; acc += cnt[l]; }
-        S.level_ptr[S.depth] = (int32_t)acc;
-        std::vector<int64_t> pos((size_t)S.depth + 1);
-        for (int32_t l = 1; l <= S.depth; l++) pos[l] = S.level_ptr[l - 1];
-        pos_at.resize((size_t)n);
-        for (int64_t i = 0; i < n; i++) pos_at[pos[lvl[i]]++] = (int32_t)i;   // schedule slot -> position
-        // within a level, long rows first (they get a block each)
-        S.level_long.assign((size_t)S.depth, 0);
-        for (int32_t l = 0; l < S.depth; l++) {
-            auto b = pos_at.begin() + S.level_ptr[l], e = pos_at.begin() + S.level_ptr[l + 1];
-            auto mid = std::stable_partition(b, e, [&](int32_t p) { return len[p] > long_row; });
-            S.level_long[l] = (int32_t)(mid - b);
+        for (int32_t t = F.urowptr[i]; t < F.urowptr[i + 1]; t++) {
+            const int32_t r = F.ucol[t];
+            if (r == i) continue;
+            blen[i]++;
+            if (!(core[r] && core[i])) lcnt[r + 1]++;
         }
-    };
-    std::vector<int32_t> fslot, bslot;
-    order(F.lvl_fwd, flen, fwd, fslot);
-    order(F.lvl_bwd, blen, bwd, bslot);
-
-    // strictly-lower rows of L (position space): transpose of U's off-diagonal
-    std::vector<int32_t> lcnt((size_t)n + 1, 0);
-    for (int64_t i = 0; i < n; i++)
-        for (int32_t t = F.urowptr[i]; t < F.urowptr[i + 1]; t++)
-            if (F.ucol[t] != i) lcnt[F.ucol[t] + 1]++;
     for (int64_t i = 0; i < n; i++) lcnt[i + 1] += lcnt[i];
     std::vector<int32_t> lcol((size_t)lcnt[n]);
     std::vector<double> lval((size_t)lcnt[n]);
@@ -403,89 +389,104 @@ int factor_schedules(const Factor &F, Schedule &fwd, Schedule &bwd, const std::v
         std::vector<int32_t> fillp(lcnt.begin(), lcnt.end() - 1);
         for (int64_t i = 0; i < n; i++)      // i ascending => each row's columns ascending
             for (int32_t t = F.urowptr[i]; t < F.urowptr[i + 1]; t++) {
-                int32_t r = F.ucol[t];
-                if (r == i) continue;
+                const int32_t r = F.ucol[t];
+                if (r == i || (core[r] && core[i])) continue;
                 lcol[fillp[r]] = (int32_t)i;
                 lval[fillp[r]] = F.uval[t];
                 fillp[r]++;
             }
     }
+    std::vector<int32_t> flen((size_t)n);
+    for (int64_t i = 0; i < n; i++) flen[i] = lcnt[i + 1] - lcnt[i];
+
+    // non-core positions by (level, position), levels renumbered densely;
+    // within a level, long rows first (they get a block each)
+    auto order = [&](const std::vector<int32_t> &lvl, const std::vector<int32_t> &len, Schedule &S,
+                     std::vector<int32_t> &slot) {
+        int32_t maxl = 0;
+        for (int64_t i = 0; i < n; i++)
+            if (!core[i]) maxl = std::max(maxl, lvl[i]);
+        std::vector<int64_t> cnt((size_t)maxl + 2, 0);
+        for (int64_t i = 0; i < n; i++)
+            if (!core[i]) cnt[lvl[i]]++;
+        std::vector<int32_t> dense((size_t)maxl + 2, -1);
+        S.depth = 0;
+        for (int32_t l = 1; l <= maxl; l++)
+            if (cnt[l]) dense[l] = S.depth++;
+        S.level_ptr.assign((size_t)S.depth + 1, 0);
+        for (int32_t l = 1; l <= maxl; l++)
+            if (cnt[l]) S.level_ptr[dense[l] + 1] = (int32_t)cnt[l];
+        for (int32_t l = 0; l < S.depth; l++) S.level_ptr[l + 1] += S.level_ptr[l];
+        std::vector<int64_t> pos(S.level_ptr.begin(), S.level_ptr.end());
+        slot.assign((size_t)S.level_ptr[S.depth], 0);
+        for (int64_t i = 0; i < n; i++)
+            if (!core[i]) slot[pos[dense[lvl[i]]]++] = (int32_t)i;
+        S.level_long.assign((size_t)S.depth, 0);
+        for (int32_t l = 0; l < S.depth; l++) {
+            auto b = slot.begin() + S.level_ptr[l], e = slot.begin() + S.level_ptr[l + 1];
+            auto mid = std::stable_partition(b, e, [&](int32_t p) { return len[p] > long_row; });
+            S.level_long[l] = (int32_t)(mid - b);
+        }
+    };
+    std::vector<int32_t> fslot, bslot;
+    order(F.lvl_fwd, flen, fwd, fslot);
+    order(F.lvl_bwd, blen, bwd, bslot);
+    // forward: the core rows follow the levels in elimination order (their
+    // entries outside the core; the rest is the dense core solve)
+    fwd.core_t0 = (int32_t)fslot.size();
+    for (int64_t i = 0; i < n; i++)
+        if (core[i]) fslot.push_back((int32_t)i);
+    bwd.core_t0 = (int32_t)bslot.size();
+
     auto pack = [&](Schedule &S, const std::vector<int32_t> &slot, const std::vector<int32_t> &lvl,
-                    const int32_t *ptr, const int32_t *col, const double *val, bool skip_diag,
-                    bool by_level) -> int {
-        S.ptr.assign((size_t)n + 1, 0);
-        S.rows.resize((size_t)n);
+                    const int32_t *ptr, const int32_t *col, const double *val, bool skip_diag) -> int {
+        const int64_t rows = (int64_t)slot.size();
+        S.n = rows;
+        S.ptr.assign((size_t)rows + 1, 0);
+        S.rows.resize((size_t)rows);
         int64_t total = 0;
-        for (int64_t t = 0; t < n; t++) {
+        for (int64_t t = 0; t < rows; t++) {
             const int32_t i = slot[t];
             S.ptr[t] = (int32_t)total;
             for (int32_t q = ptr[i]; q < ptr[i + 1]; q++)
                 if (!(skip_diag && col[q] == i)) total++;
         }
-        S.ptr[n] = (int32_t)total;
+        S.ptr[rows] = (int32_t)total;
         S.col.resize((size_t)total);
         S.val.resize((size_t)total);
-        std::vector<std::pair<int32_t, double>> ent;
         int64_t p = 0;
-        for (int64_t t = 0; t < n; t++) {
+        for (int64_t t = 0; t < rows; t++) {
             const int32_t i = slot[t];
             S.rows[t] = F.order[i];
-            ent.clear();
             for (int32_t q = ptr[i]; q < ptr[i + 1]; q++) {
                 if (skip_diag && col[q] == i) continue;
                 if (lvl[col[q]] >= lvl[i])
                     return fail(SS_ERR_INVALID, "schedule: position %d depends on position %d of level >= its own", i, col[q]);
-                ent.push_back({col[q], val[q]});
-            }
-            if (by_level)
-                std::stable_sort(ent.begin(), ent.end(), [&](const std::pair<int32_t, double> &a,
-                                                             const std::pair<int32_t, double> &b) {
-                    return lvl[a.first] < lvl[b.first];
-                });
-            for (const auto &e : ent) {
-                S.col[p] = F.order[e.first];
-                S.val[p] = e.second;
+                S.col[p] = F.order[col[q]];
+                S.val[p] = val[q];
                 p++;
             }
         }
         return SS_OK;
     };
-    SS_TRY(pack(fwd, fslot, F.lvl_fwd, lcnt.data(), lcol.data(), lval.data(), false, true));
-    SS_TRY(pack(bwd, bslot, F.lvl_bwd, F.urowptr.data(), F.ucol.data(), F.uval.data(), true, false));
+    SS_TRY(pack(fwd, fslot, F.lvl_fwd, lcnt.data(), lcol.data(), lval.data(), false));
+    SS_TRY(pack(bwd, bslot, F.lvl_bwd, F.urowptr.data(), F.ucol.data(), F.uval.data(), true));
 
-    // runs of small levels (one workgroup walks them)
-    auto small = [&](const Schedule &S, int32_t l) { return S.level_ptr[l + 1] - S.level_ptr[l] <= small_rows; };
-    fwd.small_hi = fwd.depth;
-    fwd.small_lo = fwd.depth;
-    while (fwd.small_lo > 0 && small(fwd, fwd.small_lo - 1)) fwd.small_lo--;
-    bwd.small_lo = 0;
-    bwd.small_hi = 0;
-    while (bwd.small_hi < bwd.depth && small(bwd, bwd.small_hi)) bwd.small_hi++;
-    // forward segments: a row of its graph's tail run (1-based level > the
-    // graph's tail start lo) splits after the entries whose columns sit in
-    // levels <= lo; entries are sorted by column level, so these lead
-    const int32_t own_lo = tail_lo ? 0 : forward_tail_start(F, small_rows);
-    std::vector<int32_t> vlvl((size_t)n), vlo((size_t)n);
-    for (int64_t p = 0; p < n; p++) {
-        vlvl[F.order[p]] = F.lvl_fwd[p];
-        vlo[F.order[p]] = tail_lo ? (*tail_lo)[p] : own_lo;
-    }
-    fwd.split.assign((size_t)n, 0);
-    for (int64_t t = 0; t < n; t++) {
-        const int32_t b = fwd.ptr[t], e = fwd.ptr[t + 1];
-        const int32_t r = fwd.rows[t];
-        int32_t q = e;
-        if (vlvl[r] > vlo[r]) {
-            q = b;
-            while (q < e && vlvl[fwd.col[q]] <= vlo[r]) q++;
+    // every maximal run of at least two small levels goes to one workgroup
+    for (Schedule *S : {&fwd, &bwd}) {
+        auto small = [&](int32_t l) {
+            return S->level_ptr[l + 1] - S->level_ptr[l] <= small_rows &&
+                   S->ptr[S->level_ptr[l + 1]] - S->ptr[S->level_ptr[l]] <= kSmallNnz;
+        };
+        S->runs.clear();
+        for (int32_t l = 0; l < S->depth;) {
+            if (!small(l)) { l++; continue; }
+            int32_t e = l;
+            while (e < S->depth && small(e)) e++;
+            if (e - l >= 2) S->runs.push_back({l, e});
+            l = e;
         }
-        fwd.split[t] = q - b;
     }
-    // the small run only holds rows whose first segment is finished before
-    // it starts: a row of the run lies in its graph's own tail run
-    for (int64_t t = fwd.level_ptr[fwd.small_lo]; t < n; t++)
-        if (vlvl[fwd.rows[t]] <= vlo[fwd.rows[t]])
-            return fail(SS_ERR_INVALID, "schedule: small-level run starts before a graph's own tail");
     return SS_OK;
 }
 

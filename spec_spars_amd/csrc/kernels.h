@@ -80,24 +80,36 @@ void launch_copy_f64(hipStream_t s, int64_t count, const double *a, double *b);
 static const int kMaxPartialBlocks = 4096;
 // Rows of one level, positions [t0, t1); the first nlong of them are rows
 // of more than kLongRow entries (a block of `waves` wavefronts each: 4, 8 or
-// 16), the rest go rpw to a wave.  Forward rows are split in two segments at
-// split[t] (backward: split unused, may be NULL).
+// 16), the rest go rpw to a wave.
 template <typename T, bool BWD>
 void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nlong, int32_t t1, int waves, int rpw,
-                           const int32_t *rows, const int32_t *ptr, const int32_t *split, const int32_t *col,
-                           const T *val, const double *diag, int k, const T *in, T *out);
+                           const int32_t *rows, const int32_t *ptr, const int32_t *col, const T *val,
+                           const double *diag, int k, const T *in, T *out);
 template <typename T>
 void launch_spmm_dot(hipStream_t s, int32_t n, const int32_t *rowptr, const int32_t *col, const T *val, int k,
                      const T *P, T *Q, double *partial);
-template <typename T>
-void launch_tri_tail_head(hipStream_t s, int32_t t0, int32_t t1, const int32_t *rows, const int32_t *ptr,
-                          const int32_t *split, const int32_t *col, const T *val, int k, const T *in, T *out);
 // Small levels [l0, l1) (each of at most kSmallRows rows) walked by one
 // workgroup; level l holds positions [level_ptr[l], level_ptr[l+1]).
 template <typename T, bool BWD>
 void launch_tri_small(hipStream_t s, int32_t l0, int32_t l1, const int32_t *level_ptr, const int32_t *rows,
-                      const int32_t *ptr, const int32_t *split, const int32_t *col, const T *val, const double *diag,
-                      int k, const T *in, T *out);
+                      const int32_t *ptr, const int32_t *col, const T *val, const double *diag, int k, const T *in,
+                      T *out);
+// Dense cores (per graph: core_n[g] rows, vertex ids vert[core_off[g] ..],
+// dense matrix at D + dense_off[g], row-major core_n x core_n).
+// Forward: launch_core_head (the entries outside the core, schedule
+// positions [t0, t1)), then launch_core_step<T, false> for steps 0 .. max
+// blocks - 1 with D = L.  Backward: launch_core_bwd_init, then
+// launch_core_step<T, true> for the same steps with D = L^T.
+template <typename T>
+void launch_core_head(hipStream_t s, int32_t t0, int32_t t1, const int32_t *rows, const int32_t *ptr,
+                      const int32_t *col, const T *val, int k, const T *in, T *out);
+template <typename T>
+void launch_core_bwd_init(hipStream_t s, int32_t count, const int32_t *vert, const double *diag, int k, const T *in,
+                          T *out);
+template <typename T, bool BWD>
+void launch_core_step(hipStream_t s, int step, int graphs, int max_rows, const int32_t *core_n,
+                      const int32_t *core_off, const int64_t *dense_off, const int32_t *vert, const T *D, int k,
+                      T *out);
 template <typename T>
 void launch_ydy_partial(hipStream_t s, int32_t n, int k, const T *Yv, const double *diag, double *partial);
 template <typename T>

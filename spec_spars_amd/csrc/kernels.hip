@@ -537,12 +537,11 @@ __device__ __forceinline__ T seg_apply_shared(T acc, int a, int b, bool act, int
 // One level of a triangular solve: schedule positions [t0, t1), the first
 // nlong of them rows of more than kLongRow entries (a block each, its W =
 // blockDim / 64 waves sharing the row's parts), the rest rpw consecutive
-// rows per wave.  Forward rows are split at split[t].
+// rows per wave.
 template <typename T, bool BWD>
 __global__ __launch_bounds__(64 * kLevelWaves) void k_trisolve_level(int32_t t0, int32_t nlong, int32_t t1, int rpw,
                                                                      const int32_t *__restrict__ rows,
                                                                      const int32_t *__restrict__ ptr,
-                                                                     const int32_t *__restrict__ split,
                                                                      const int32_t *__restrict__ col,
                                                                      const T *__restrict__ val,
                                                                      const double *__restrict__ diag, int k,
@@ -553,14 +552,11 @@ __global__ __launch_bounds__(64 * kLevelWaves) void k_trisolve_level(int32_t t0,
         const int t = t0 + blockIdx.x;
         const int i = wave_uniform(rows[t]);
         const int q0 = wave_uniform(ptr[t]), q1 = wave_uniform(ptr[t + 1]);
-        const int qs = BWD ? q1 : q0 + wave_uniform(split[t]);
         const double d = BWD ? diag[i] : 1.0;
         for (int jb = 0; jb < k; jb += 64) {
             const int j = jb + lane;
             T acc = (wv == 0 && j < k) ? row_start<T, BWD>(i, j, k, d, in) : (T)0;
-            acc = seg_apply_shared<T, kMaxParts / 4>(acc, q0, qs, true, W, wv, 0, sm, col, val, out, k, j, lane);
-            if (!BWD)
-                acc = seg_apply_shared<T, kMaxParts / 4>(acc, qs, q1, true, W, wv, 0, sm, col, val, out, k, j, lane);
+            acc = seg_apply_shared<T, kMaxParts / 4>(acc, q0, q1, true, W, wv, 0, sm, col, val, out, k, j, lane);
             if (wv == 0 && j < k) out[(size_t)i * k + j] = acc;
         }
         return;
@@ -571,36 +567,153 @@ __global__ __launch_bounds__(64 * kLevelWaves) void k_trisolve_level(int32_t t0,
     for (int t = tb; t < te; t++) {
         const int i = wave_uniform(rows[t]);
         const int q0 = wave_uniform(ptr[t]), q1 = wave_uniform(ptr[t + 1]);
-        const int qs = BWD ? q1 : q0 + wave_uniform(split[t]);
         const double d = BWD ? diag[i] : 1.0;
         for (int jb = 0; jb < k; jb += 64) {
             const int j = jb + lane;
             if (j >= k) break;
-            T acc = seg_apply<T, kLongRow>(row_start<T, BWD>(i, j, k, d, in), q0, qs, col, val, out, k, j);
-            if (!BWD) acc = seg_apply<T, kLongRow>(acc, qs, q1, col, val, out, k, j);
-            out[(size_t)i * k + j] = acc;
+            out[(size_t)i * k + j] =
+                seg_apply<T, kLongRow>(row_start<T, BWD>(i, j, k, d, in), q0, q1, col, val, out, k, j);
         }
     }
 }
 
-// Forward tail, step 1: the first segment of every row of the trailing run
-// of small levels (its columns were all finished by the head levels):
-// out_i = r_i updated by entries [ptr[t], ptr[t] + split[t]).  Wave per row.
+// Dense cores, forward step 0: every core row's entries outside the core
+// (schedule positions [t0, t1)): out_i = r_i updated by them.  Wave per row.
 template <typename T>
-__global__ __launch_bounds__(256) void k_tri_tail_head(int32_t t0, int32_t t1, const int32_t *__restrict__ rows,
-                                                       const int32_t *__restrict__ ptr,
-                                                       const int32_t *__restrict__ split,
-                                                       const int32_t *__restrict__ col, const T *__restrict__ val,
-                                                       int k, const T *__restrict__ in, T *__restrict__ out) {
+__global__ __launch_bounds__(256) void k_core_head(int32_t t0, int32_t t1, const int32_t *__restrict__ rows,
+                                                   const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
+                                                   const T *__restrict__ val, int k, const T *__restrict__ in,
+                                                   T *__restrict__ out) {
     const int t = wave_uniform(t0 + blockIdx.x * 4 + SS_WAVE_IN_BLOCK);
     if (t >= t1) return;
     const int i = wave_uniform(rows[t]);
-    const int q0 = wave_uniform(ptr[t]);
-    const int qs = q0 + wave_uniform(split[t]);
+    const int q0 = wave_uniform(ptr[t]), q1 = wave_uniform(ptr[t + 1]);
     for (int jb = 0; jb < k; jb += 64) {
         const int j = jb + SS_LANE;
         if (j >= k) break;
-        out[(size_t)i * k + j] = seg_apply<T, (1 << 30)>(in[(size_t)i * k + j], q0, qs, col, val, out, k, j);
+        out[(size_t)i * k + j] = seg_apply<T, (1 << 30)>(in[(size_t)i * k + j], q0, q1, col, val, out, k, j);
+    }
+}
+
+// Dense cores, backward step 0: z_i = y_i / D_i for every core row.
+template <typename T>
+__global__ __launch_bounds__(256) void k_core_bwd_init(int32_t count, const int32_t *__restrict__ vert,
+                                                       const double *__restrict__ diag, int k,
+                                                       const T *__restrict__ in, T *__restrict__ out) {
+    const int r = wave_uniform(blockIdx.x * 4 + SS_WAVE_IN_BLOCK);
+    if (r >= count) return;
+    const int i = wave_uniform(vert[r]);
+    const double d = diag[i];
+    for (int jb = 0; jb < k; jb += 64) {
+        const int j = jb + SS_LANE;
+        if (j < k) out[(size_t)i * k + j] = row_start<T, true>(i, j, k, d, in);
+    }
+}
+
+// One step of the blocked substitution over every graph's dense core
+// (c rows in elimination order, D = L row-major c x c strictly lower; the
+// backward solve reads D = L^T).  Forward step s finishes block s of each
+// graph: workgroup x == 0 applies block s-1 to the rows of block s, then
+// substitutes inside block s; workgroups x >= 1 apply block s-1 to the rows
+// of the later blocks, 64 rows each.  Backward runs the blocks from the
+// last.  Every core row is one running fused multiply-subtract chain over
+// its core columns -- ascending forward, descending backward, explicit
+// zeros included -- so its bits depend on its graph alone.
+template <typename T, bool BWD>
+__global__ __launch_bounds__(1024) void k_core_step(int s, const int32_t *__restrict__ core_n,
+                                                    const int32_t *__restrict__ core_off,
+                                                    const int64_t *__restrict__ dense_off,
+                                                    const int32_t *__restrict__ vert, const T *__restrict__ D, int k,
+                                                    T *out) {
+    constexpr int B = kCoreBlock;
+    __shared__ T sm_a[B][64];   // block rows being finished (column chunk)
+    __shared__ T sm_b[B][64];   // the block applied: finished rows, then the diagonal block of D
+    const int g = blockIdx.y, x = blockIdx.x;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int c = core_n[g];
+    const int nb = (c + B - 1) / B;
+    const int blk = BWD ? nb - 1 - s : s;       // block finished by this step
+    if (blk < 0 || blk >= nb) return;
+    const int prev = BWD ? blk + 1 : blk - 1;   // block applied by this step
+    const bool has_prev = prev >= 0 && prev < nb;
+    const int32_t *vg = vert + core_off[g];
+    const T *Dg = D + dense_off[g];
+    const int b0 = blk * B, b1 = b0 + B < c ? b0 + B : c;
+    const int p0 = prev * B, p1 = p0 + B < c ? p0 + B : c;
+    // rows this workgroup updates with the previous block
+    int r0, r1;
+    if (x == 0) {
+        r0 = b0;
+        r1 = b1;
+    } else if (BWD) {
+        r0 = (x - 1) * B;
+        r1 = r0 + B < b0 ? r0 + B : b0;
+    } else {
+        r0 = b1 + (x - 1) * B;
+        r1 = r0 + B < c ? r0 + B : c;
+    }
+    if (r0 >= r1) return;   // uniform over the workgroup
+    constexpr int RPW = B / 16;   // block rows per wave in the substitution (16 waves)
+    for (int jb = 0; jb < k; jb += 64) {
+        const int j = jb + lane;
+        // finished values of the previous block
+        if (has_prev)
+            for (int r = wv; r < p1 - p0; r += nw) sm_b[r][lane] = j < k ? out[(size_t)vg[p0 + r] * k + j] : (T)0;
+        __syncthreads();
+        // apply it to rows [r0, r1): ascending j forward, descending backward
+        for (int r = r0 + wv; r < r1; r += nw) {
+            const int i = vg[r];
+            T acc = j < k ? out[(size_t)i * k + j] : (T)0;
+            if (has_prev) {
+                const T *Drow = Dg + (size_t)r * c;
+                if (BWD) {
+                    for (int q = p1 - 1; q >= p0; q--) acc = fnms(Drow[q], sm_b[q - p0][lane], acc);
+                } else {
+                    for (int q = p0; q < p1; q++) acc = fnms(Drow[q], sm_b[q - p0][lane], acc);
+                }
+            }
+            if (x == 0)
+                sm_a[r - r0][lane] = acc;
+            else if (j < k)
+                out[(size_t)i * k + j] = acc;
+        }
+        if (x == 0) {
+            __syncthreads();
+            // the diagonal block of D into sm_b
+            for (int e = threadIdx.x; e < B * B; e += blockDim.x) {
+                const int a = e / B, b = e % B;
+                sm_b[a][b] = (a < b1 - b0 && b < b1 - b0) ? Dg[(size_t)(b0 + a) * c + b0 + b] : (T)0;
+            }
+            __syncthreads();
+            // substitution, column by column: wave w keeps rows w, w+16, ...
+            // in registers; once row a is final every later row takes its
+            // term, so each row still subtracts its columns in order
+            const int nr = b1 - b0;
+            T acc[RPW];
+#pragma unroll
+            for (int u = 0; u < RPW; u++) acc[u] = wv + 16 * u < nr ? sm_a[wv + 16 * u][lane] : (T)0;
+            for (int step = 0; step < nr; step++) {
+                const int a = BWD ? nr - 1 - step : step;
+                if ((a & 15) == wv) {
+                    T v = acc[0];
+#pragma unroll
+                    for (int u = 1; u < RPW; u++)
+                        if ((a >> 4) == u) v = acc[u];
+                    sm_a[a][lane] = v;
+                }
+                __syncthreads();
+                const T ya = sm_a[a][lane];
+#pragma unroll
+                for (int u = 0; u < RPW; u++) {
+                    const int r = wv + 16 * u;
+                    if (r < nr && (BWD ? r < a : r > a)) acc[u] = fnms(sm_b[r][a], ya, acc[u]);
+                }
+            }
+            __syncthreads();
+            for (int r = wv; r < nr; r += nw)
+                if (j < k) out[(size_t)vg[b0 + r] * k + j] = sm_a[r][lane];
+        }
+        __syncthreads();
     }
 }
 
@@ -608,14 +721,11 @@ __global__ __launch_bounds__(256) void k_tri_tail_head(int32_t t0, int32_t t1, c
 // workgroup barrier between levels (all waves share one CU, so the barrier's
 // workgroup-scope fence orders the row stores before the next level's
 // loads).  A level of r rows gives each row g = 16 / r waves (rounded down
-// to a power of two), which split the row's parts between them.  Forward:
-// the second segment, continuing from the first that k_tri_tail_head left
-// in out; backward: the whole row from y_i / D_i.
+// to a power of two), which split the row's parts between them.
 template <typename T, bool BWD>
 __global__ __launch_bounds__(1024) void k_tri_small(int32_t l0, int32_t l1, const int32_t *__restrict__ level_ptr,
                                                     const int32_t *__restrict__ rows,
                                                     const int32_t *__restrict__ ptr,
-                                                    const int32_t *__restrict__ split,
                                                     const int32_t *__restrict__ col, const T *__restrict__ val,
                                                     const double *__restrict__ diag, int k,
                                                     const T *__restrict__ in, T *out) {
@@ -632,13 +742,12 @@ __global__ __launch_bounds__(1024) void k_tri_small(int32_t l0, int32_t l1, cons
         const bool act = t < te;
         const int i = act ? wave_uniform(rows[t]) : 0;
         const int q0 = act ? wave_uniform(ptr[t]) : 0, q1 = act ? wave_uniform(ptr[t + 1]) : 0;
-        const int qs = BWD ? q0 : (act ? q0 + wave_uniform(split[t]) : 0);
         const double d = (BWD && act) ? diag[i] : 1.0;
         for (int jb = 0; jb < k; jb += 64) {
             const int j = jb + lane;
             T acc = (T)0;
-            if (act && sub == 0 && j < k) acc = BWD ? row_start<T, true>(i, j, k, d, in) : out[(size_t)i * k + j];
-            acc = seg_apply_shared<T, kMaxParts>(acc, qs, q1, act, g, sub, slot * g, sm, col, val, out, k, j, lane);
+            if (act && sub == 0 && j < k) acc = row_start<T, BWD>(i, j, k, d, in);
+            acc = seg_apply_shared<T, kMaxParts>(acc, q0, q1, act, g, sub, slot * g, sm, col, val, out, k, j, lane);
             if (act && sub == 0 && j < k) out[(size_t)i * k + j] = acc;
         }
         __syncthreads();
@@ -923,30 +1032,57 @@ template void launch_spmm_dot<float>(hipStream_t, int32_t, const int32_t *, cons
                                      const float *, float *, double *);
 
 template <typename T>
-void launch_tri_tail_head(hipStream_t s, int32_t t0, int32_t t1, const int32_t *rows, const int32_t *ptr,
-                          const int32_t *split, const int32_t *col, const T *val, int k, const T *in, T *out) {
+void launch_core_head(hipStream_t s, int32_t t0, int32_t t1, const int32_t *rows, const int32_t *ptr,
+                      const int32_t *col, const T *val, int k, const T *in, T *out) {
     if (t1 <= t0) return;
-    hipLaunchKernelGGL(k_tri_tail_head<T>, dim3(grid_for(t1 - t0, 4)), dim3(256), 0, s, t0, t1, rows, ptr, split, col,
-                       val, k, in, out);
+    hipLaunchKernelGGL(k_core_head<T>, dim3(grid_for(t1 - t0, 4)), dim3(256), 0, s, t0, t1, rows, ptr, col, val, k,
+                       in, out);
 }
-template void launch_tri_tail_head<double>(hipStream_t, int32_t, int32_t, const int32_t *, const int32_t *,
-                                           const int32_t *, const int32_t *, const double *, int, const double *,
-                                           double *);
-template void launch_tri_tail_head<float>(hipStream_t, int32_t, int32_t, const int32_t *, const int32_t *,
-                                          const int32_t *, const int32_t *, const float *, int, const float *, float *);
+template void launch_core_head<double>(hipStream_t, int32_t, int32_t, const int32_t *, const int32_t *,
+                                       const int32_t *, const double *, int, const double *, double *);
+template void launch_core_head<float>(hipStream_t, int32_t, int32_t, const int32_t *, const int32_t *,
+                                      const int32_t *, const float *, int, const float *, float *);
+
+template <typename T>
+void launch_core_bwd_init(hipStream_t s, int32_t count, const int32_t *vert, const double *diag, int k, const T *in,
+                          T *out) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_core_bwd_init<T>, dim3(grid_for(count, 4)), dim3(256), 0, s, count, vert, diag, k, in, out);
+}
+template void launch_core_bwd_init<double>(hipStream_t, int32_t, const int32_t *, const double *, int,
+                                           const double *, double *);
+template void launch_core_bwd_init<float>(hipStream_t, int32_t, const int32_t *, const double *, int, const float *,
+                                          float *);
+
+template <typename T, bool BWD>
+void launch_core_step(hipStream_t s, int step, int graphs, int max_rows, const int32_t *core_n,
+                      const int32_t *core_off, const int64_t *dense_off, const int32_t *vert, const T *D, int k,
+                      T *out) {
+    if (graphs <= 0 || max_rows <= 0) return;
+    const unsigned gx = 1 + (unsigned)((max_rows + kCoreBlock - 1) / kCoreBlock);
+    hipLaunchKernelGGL((k_core_step<T, BWD>), dim3(gx, graphs), dim3(1024), 0, s, step, core_n, core_off, dense_off,
+                       vert, D, k, out);
+}
+#define SS_INST_CORE(T, B)                                                                                    \
+    template void launch_core_step<T, B>(hipStream_t, int, int, int, const int32_t *, const int32_t *,         \
+                                         const int64_t *, const int32_t *, const T *, int, T *);
+SS_INST_CORE(double, false)
+SS_INST_CORE(double, true)
+SS_INST_CORE(float, false)
+SS_INST_CORE(float, true)
 
 template <typename T, bool BWD>
 void launch_tri_small(hipStream_t s, int32_t l0, int32_t l1, const int32_t *level_ptr, const int32_t *rows,
-                      const int32_t *ptr, const int32_t *split, const int32_t *col, const T *val, const double *diag,
-                      int k, const T *in, T *out) {
+                      const int32_t *ptr, const int32_t *col, const T *val, const double *diag, int k, const T *in,
+                      T *out) {
     if (l1 <= l0) return;
-    hipLaunchKernelGGL((k_tri_small<T, BWD>), dim3(1), dim3(1024), 0, s, l0, l1, level_ptr, rows, ptr, split, col, val,
-                       diag, k, in, out);
+    hipLaunchKernelGGL((k_tri_small<T, BWD>), dim3(1), dim3(1024), 0, s, l0, l1, level_ptr, rows, ptr, col, val, diag,
+                       k, in, out);
 }
 #define SS_INST_SMALL(T, B)                                                                                       \
     template void launch_tri_small<T, B>(hipStream_t, int32_t, int32_t, const int32_t *, const int32_t *,          \
-                                         const int32_t *, const int32_t *, const int32_t *, const T *,              \
-                                         const double *, int, const T *, T *);
+                                         const int32_t *, const int32_t *, const T *, const double *, int,          \
+                                         const T *, T *);
 SS_INST_SMALL(double, false)
 SS_INST_SMALL(double, true)
 SS_INST_SMALL(float, false)
@@ -965,8 +1101,8 @@ template void launch_update_xr_rr<float>(hipStream_t, int32_t, int, const double
 
 template <typename T, bool BWD>
 void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nlong, int32_t t1, int waves, int rpw,
-                           const int32_t *rows, const int32_t *ptr, const int32_t *split, const int32_t *col,
-                           const T *val, const double *diag, int k, const T *in, T *out) {
+                           const int32_t *rows, const int32_t *ptr, const int32_t *col, const T *val,
+                           const double *diag, int k, const T *in, T *out) {
     if (t1 <= t0) return;
     if (rpw < 1) rpw = 1;
     if (waves < 4 || waves > kLevelWaves || (waves & (waves - 1))) waves = 4;
@@ -974,12 +1110,12 @@ void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nlong, int32_t t1,
     const int64_t per_block = (int64_t)waves * rpw;
     const unsigned grid = (unsigned)nlong + (unsigned)((short_rows + per_block - 1) / per_block);
     hipLaunchKernelGGL((k_trisolve_level<T, BWD>), dim3(grid), dim3(64 * waves), 0, s, t0, nlong, t1, rpw, rows, ptr,
-                       split, col, val, diag, k, in, out);
+                       col, val, diag, k, in, out);
 }
 #define SS_INST_TRI(T, B)                                                                                      \
     template void launch_trisolve_level<T, B>(hipStream_t, int32_t, int32_t, int32_t, int, int, const int32_t *,\
-                                              const int32_t *, const int32_t *, const int32_t *, const T *,      \
-                                              const double *, int, const T *, T *);
+                                              const int32_t *, const int32_t *, const T *, const double *, int,  \
+                                              const T *, T *);
 SS_INST_TRI(double, false)
 SS_INST_TRI(double, true)
 SS_INST_TRI(float, false)

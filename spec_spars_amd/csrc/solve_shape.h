@@ -21,12 +21,15 @@
 namespace ss {
 
 constexpr int kLongRow = 32;      // rows longer than this get a whole block per level launch
-constexpr int kSmallRows = 16;    // levels with at most this many rows go to the small-level kernel
+constexpr int kSmallRows = 16;    // levels with at most this many rows ...
+constexpr int kSmallNnz = 2048;   // ... and entries go to the single-workgroup kernel
 
 static_assert(kSmallRows == 16, "the small-level kernel runs 16 waves, at most one row each");
 
 constexpr int seg_parts(int len) { return len <= 8 ? 1 : (len <= 32 ? 4 : (len <= 128 ? 8 : 16)); }
 constexpr int kMaxParts = 16;
+constexpr int kCoreBlock = 64;     // rows per block of the dense-core substitution
+constexpr int kCoreMaxRows = 16384; // largest dense core per graph (its trailing small levels beyond stay sparse)
 constexpr int kLevelWaves = 16;   // wavefronts per block of a per-level launch (one long row, or 16 short rows)
 
 }  // namespace ss

@@ -1,5 +1,6 @@
 // Internal declarations shared by the host C++ and the HIP sources.
 #pragma once
+#include <utility>
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
@@ -60,22 +61,19 @@ struct Factor {
     double ms_build = 0.0;
 };
 
-// Level-ordered pull-form CSR of one triangular solve.
+// Level-ordered pull-form CSR of one triangular solve over the rows outside
+// the dense cores.  Forward positions [core_t0, n) then hold the core rows
+// in elimination order with their entries outside the core.
 struct Schedule {
-    int64_t n = 0;
+    int64_t n = 0;                    // scheduled rows (positions)
     int32_t depth = 0;
-    std::vector<int32_t> rows;        // row processed at position t
-    std::vector<int32_t> level_ptr;   // positions of level l: [ptr[l-1], ptr[l])
+    int32_t core_t0 = 0;
+    std::vector<int32_t> rows;        // vertex finished at position t
+    std::vector<int32_t> level_ptr;   // positions of level l: [level_ptr[l], level_ptr[l+1])
     std::vector<int32_t> level_long;  // rows of level l with more than long_row entries (placed first)
-    // Small-level run handled by one workgroup: levels [small_lo, small_hi)
-    // (0-based).  Forward: the trailing run; each of its rows lists the
-    // entries of earlier (head) levels first, split[t] of them.  Backward:
-    // the leading run.
-    int32_t small_lo = 0, small_hi = 0;
-    // Forward rows: entries [ptr[t], ptr[t] + split[t]) form the first
-    // segment (the columns in the head levels of the row's graph), the rest
-    // the second; split[t] = row length outside the graph's own tail run.
-    std::vector<int32_t> split;
+    // Runs of consecutive small levels (at least two levels of at most
+    // small_rows rows each), each walked by one workgroup: [first, second).
+    std::vector<std::pair<int32_t, int32_t>> runs;
     std::vector<int32_t> ptr;         // entries of position t: [ptr[t], ptr[t+1])
     std::vector<int32_t> col;
     std::vector<double> val;
@@ -84,17 +82,15 @@ struct Schedule {
 int factor_build(int64_t n, const int32_t *colptr, const int32_t *rowidx, const double *vals,
                  uint64_t seed, int ordering, Factor &F);
 void factor_levels(Factor &F);
-// Small-level run of one graph's forward schedule: the first 0-based level
-// of its trailing run of levels with at most small_rows rows.
-int32_t forward_tail_start(const Factor &F, int32_t small_rows);
-// Level-ordered schedules in vertex ids; checks that every dependency sits
-// at a strictly lower level.  Forward rows list their entries sorted by the
-// level of the column (stable), so the entries of earlier levels lead.
-// tail_lo (per position, may be NULL = forward_tail_start(F) everywhere)
-// is the tail start of the position's own graph, which fixes the segments.
-// long_row and small_rows must be the kLongRow / kSmallRows of the kernels.
-int factor_schedules(const Factor &F, Schedule &fwd, Schedule &bwd, const std::vector<int32_t> *tail_lo,
-                     int32_t long_row, int32_t small_rows);
+// First 0-based forward level of a graph's dense core: the trailing run of
+// levels with at most small_rows rows each, cut to at most max_rows rows.
+int32_t forward_tail_start(const Factor &F, int32_t small_rows, int64_t max_rows);
+// Level-ordered schedules in vertex ids of the positions outside the dense
+// cores (core[p] != 0); checks that every dependency sits at a strictly
+// lower level.  long_row and small_rows must be the kernels' kLongRow /
+// kSmallRows.
+int factor_schedules(const Factor &F, const std::vector<char> &core, Schedule &fwd, Schedule &bwd, int32_t long_row,
+                     int32_t small_rows);
 
 // ---- host graph structure -----------------------------------------------
 // Canonical edges (u < v, sorted by (v, u)) -> full Laplacian CSR with the
