@@ -288,6 +288,66 @@ def bench_batch(W, args):
     return 0
 
 
+def serve_pool(count, seed=5):
+    """Resident graphs for the serving run: the three families mixed, n
+    log-uniform in [1e4, 2e6], m <= 2e7."""
+    import numpy as np
+    from spec_spars_amd import graphs
+    rng = np.random.default_rng(seed)
+    pool, desc = [], []
+    for i in range(count):
+        n = int(round(10.0 ** rng.uniform(4.0, math.log10(2e6))))
+        fam = ("grid3d", "chunglu", "rgg")[i % 3]
+        if fam == "grid3d":
+            side = max(3, int(round(n ** (1.0 / 3.0))))
+            nn, u, v = graphs.grid3d(side)
+            w = graphs.loguniform_weights(len(u), seed=100 + i)
+        elif fam == "chunglu":
+            nn, u, v = graphs.chung_lu(n, min(8 * n, 20_000_000), gamma=2.2, seed=100 + i)
+            w = graphs.uniform_weights(len(u), seed=100 + i)
+        else:
+            nn, u, v = graphs.rgg(n, seed=100 + i, mean_degree=8.0)
+            w = np.ones(len(u))
+        pool.append((nn, u, v, w))
+        desc.append("%s n=%d m=%d" % (fam, nn, len(u)))
+    return pool, desc
+
+
+def bench_serve(args):
+    """Poisson arrivals of sparsify / 16-RHS solve requests over resident
+    graphs, continuous batching by slots, admission by free device memory."""
+    import spec_spars_amd as ss
+    from spec_spars_amd.serve import Registry, Server, poisson_requests
+    prec = ss.PREC_FP64 if args.precision == "fp64" else ss.PREC_MIXED
+    t0 = time.perf_counter()
+    pool, desc = serve_pool(args.pool)
+    log("pool: " + "; ".join(desc))
+    reg = Registry(pool, threads=min(16, os.cpu_count() or 1))
+    t_build = time.perf_counter() - t0
+    log("registry built in %.1fs (%.1f GB resident)" %
+        (t_build, sum(G.resident_bytes() for G in reg.G) / 1e9))
+    reqs = poisson_requests(rate=args.rate, duration=args.duration, graphs=len(pool), seed=11, k=32, rhs=16)
+    log("%d requests over %.0fs" % (len(reqs), args.duration))
+    srv = Server(reg, slots=args.slots, precision=prec, reserve_bytes=4 << 30, keep_arrays=True)
+    out = srv.run(reqs, deadline=args.duration + 30.0)
+    log("served: %s" % out)
+    line = {
+        "metric": METRIC, "value": out["sparsified_edges_per_s"], "unit": "edges/s", "n_gpus": 1,
+        "steps": len(reqs), "warmup": 0, "ms_per_step": out["mean_ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if prec == ss.PREC_FP64 else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "serving: Poisson %.0f req/s for %.0fs over %d resident graphs (grid3d, Chung-Lu, "
+                               "RGG; n log-uniform [1e4, 2e6], m <= 2e7); half sparsify (k=32, eps in "
+                               "{0.2, 0.3, 0.5}), half 16-RHS solves; %d slots" %
+                               (args.rate, args.duration, len(pool), args.slots),
+                   "pool": desc, "precision": args.precision, "registry_build_s": t_build, **out},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    reg.close()
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -300,6 +360,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel event timing")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no autotuner)")
+    ap.add_argument("--rate", type=float, default=200.0, help="serve: Poisson arrivals per second")
+    ap.add_argument("--duration", type=float, default=60.0, help="serve: arrival window in seconds")
+    ap.add_argument("--pool", type=int, default=12, help="serve: resident graphs")
+    ap.add_argument("--slots", type=int, default=4, help="serve: requests on the device at once")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (the multi-GPU "
@@ -310,6 +374,10 @@ def main():
     import spec_spars_amd as ss
     from spec_spars_amd import graphs
 
+    if args.workload == "serve":
+        if args.precision is None:
+            args.precision = "mixed"
+        return bench_serve(args)
     W = workload(args.workload)
     if args.precision is None:
         args.precision = W.get("precision", "fp64")

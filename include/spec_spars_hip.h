@@ -171,6 +171,12 @@ int ss_graph_destroy(ss_graph *g);
 int ss_graph_bytes(int64_t n, int64_t m, int64_t factor_nnz, int32_t k,
                    int32_t precision, size_t *bytes);
 int ss_graph_set_factor(ss_graph *g, const ss_factor *f);
+/* Device bytes the graph holds now (structure, factor, work blocks), and
+ * the bytes a run with k columns at a precision would add (0 when the work
+ * blocks already allocated cover it): the admission test of a server.  No
+ * reference counterpart. */
+int ss_graph_resident_bytes(const ss_graph *g, size_t *bytes);
+int ss_graph_work_bytes(const ss_graph *g, int32_t k, int32_t precision, size_t *bytes);
 /* Laplacian of the graph as CSC with diagonal (host copy, sizes n+1 and
  * n+2m) -- the input ss_factor_build expects. */
 int ss_graph_laplacian(const ss_graph *g, int32_t *colptr, int32_t *rowidx,

@@ -127,6 +127,9 @@ SIGNATURES = {
                                       ctypes.c_int32, ctypes.c_int32,
                                       ctypes.POINTER(ctypes.c_size_t)]),
     "ss_graph_set_factor": (ctypes.c_int, [c_vp, c_vp]),
+    "ss_graph_resident_bytes": (ctypes.c_int, [c_vp, ctypes.POINTER(ctypes.c_size_t)]),
+    "ss_graph_work_bytes": (ctypes.c_int, [c_vp, ctypes.c_int32, ctypes.c_int32,
+                                           ctypes.POINTER(ctypes.c_size_t)]),
     "ss_graph_laplacian": (ctypes.c_int, [c_vp, c_i32p, c_i32p, c_f64p]),
     "ss_graph_sparsify": (ctypes.c_int, [c_vp, ctypes.POINTER(SparsifyOpts),
                                          ctypes.POINTER(SparsifyReport)]),

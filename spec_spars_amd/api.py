@@ -229,6 +229,31 @@ class Graph:
     def set_factor(self, factor):
         check(load().ss_graph_set_factor(self._h, factor._h))
 
+    def resident_bytes(self):
+        b = ctypes.c_size_t()
+        check(load().ss_graph_resident_bytes(self._h, ctypes.byref(b)))
+        return b.value
+
+    def work_bytes(self, k, precision=_lib.PREC_FP64):
+        b = ctypes.c_size_t()
+        check(load().ss_graph_work_bytes(self._h, k, precision, ctypes.byref(b)))
+        return b.value
+
+    def solve(self, rhs, solve=None):
+        """Solve L Z = rhs (n x k, component means removed) on the resident
+        graph and factor; returns (Z, info)."""
+        sv = solve if solve is not None else solve_opts(stop_rule=_lib.STOP_REL, tol=1e-8)
+        y = _f64(np.asarray(rhs).reshape(self.n, -1))
+        k = y.shape[1]
+        check(load().ss_batch_write_rhs(self._h, 0, k, sv.precision, _p(y, ctypes.c_double)))
+        iters = np.zeros(k, np.int32)
+        relres = np.zeros(k, np.float64)
+        refin = ctypes.c_int32()
+        check(load().ss_batch_solve(self._h, k, ctypes.byref(sv), _p(iters, ctypes.c_int32),
+                                    _p(relres, ctypes.c_double), ctypes.byref(refin)))
+        self.k = k
+        return self.read_block("Z"), {"iters": iters, "relres": relres, "refinements": refin.value}
+
     def sparsify(self, k, beta, scale=None, sketch_seed=0, sampling_seed=0, uniform=False, solve=None):
         o = _lib.SparsifyOpts()
         o.k = k

@@ -254,6 +254,20 @@ int ss_graph_bytes(int64_t n, int64_t m, int64_t factor_nnz, int32_t k, int32_t 
     return SS_OK;
 }
 
+int ss_graph_resident_bytes(const ss_graph *g, size_t *bytes) {
+    if (!g || !bytes) return fail(SS_ERR_INVALID, "null argument");
+    *bytes = g->arena_struct.bytes + g->arena_factor.bytes + g->arena_work.bytes + g->arena_params.bytes;
+    return SS_OK;
+}
+
+int ss_graph_work_bytes(const ss_graph *g, int32_t k, int32_t precision, size_t *bytes) {
+    if (!g || !bytes) return fail(SS_ERR_INVALID, "null argument");
+    if (k < 1 || k > 256 || (precision != SS_PREC_FP64 && precision != SS_PREC_MIXED))
+        return fail(SS_ERR_INVALID, "bad k=%d or precision %d", k, precision);
+    *bytes = graph_work_bytes(g, k, precision);
+    return SS_OK;
+}
+
 static int need_single(const ss_graph *g) {
     if (g->G != 1) return fail(SS_ERR_INVALID, "this call takes a single graph; use the ss_batch_* call for a batch of %d", g->G);
     return SS_OK;

@@ -116,7 +116,7 @@ struct ss_graph {
     // status
     int *status = nullptr;           // {nactive, err}
     // work (allocated for wk columns at wprec)
-    int wk = 0, wprec = -1;
+    int wk = 0, wcap = 0, wprec = -1;   // columns in use, columns allocated, precision
     double *Y64 = nullptr, *X64 = nullptr, *R64 = nullptr, *B64 = nullptr;
     float *Y32 = nullptr, *X32 = nullptr;
     ss::PcgWork<double> w64;
@@ -174,6 +174,8 @@ int batch_upload(ss_context *ctx, const std::vector<const HostGraph *> &graphs, 
 // Concatenate the graphs' factors into one schedule and upload it.
 int batch_upload_factors(ss_graph *g, const std::vector<const Factor *> &factors);
 int graph_ensure_work(ss_graph *g, int k, int prec);
+// Device bytes graph_ensure_work(g, k, prec) would allocate (0 if none).
+size_t graph_work_bytes(const ss_graph *g, int k, int prec);
 
 struct SolveStats {
     std::vector<int32_t> iters;      // [G*k]

@@ -527,20 +527,52 @@ int batch_upload_factors(ss_graph *g, const std::vector<const Factor *> &factors
     return SS_OK;
 }
 
+// Point the per-(graph, column) scalars at the scalar buffer for k columns.
+static void bind_scalars(ss_graph *g, int k) {
+    const size_t gk = (size_t)g->G * k;
+    g->sc.rz = g->sc_buf;
+    g->sc.alpha = g->sc_buf + gk;
+    g->sc.beta = g->sc_buf + 2 * gk;
+    g->sc.rr = g->sc_buf + 3 * gk;
+    g->sc.bnorm = g->sc_buf + 4 * gk;
+    g->sc.active = g->sc_int;
+    g->sc.iters = g->sc_int + gk;
+}
+
+size_t graph_work_bytes(const ss_graph *g, int k, int prec) {
+    if (g->wprec == prec && k <= g->wcap) return 0;
+    const int cap = g->wprec == prec ? std::max(k, g->wcap) : k;
+    const size_t nk = (size_t)g->N * (size_t)cap, gk = (size_t)g->G * cap;
+    size_t b = nk * 8 * 2;                                  // Y64, X64
+    b += prec == SS_PREC_FP64 ? nk * 8 * 6 : nk * (8 * 2 + 4 * 2 + 4 * 6);
+    b += (size_t)std::max(g->nrt, 1) * cap * 8 * 2 + gk * 8 + (size_t)g->ncomp * cap * 8 * 2;
+    b += gk * (5 * 8 + 2 * 4 + 4);
+    return b;
+}
+
+// Work blocks for k columns at a precision.  Blocks allocated for more
+// columns at the same precision are reused (the kernels take k as the row
+// stride), so a resident graph serving requests of different k keeps one
+// allocation.
 int graph_ensure_work(ss_graph *g, int k, int prec) {
     if (k < 1 || k > 256) return fail(SS_ERR_INVALID, "k=%d outside 1..256", k);
     if (prec != SS_PREC_FP64 && prec != SS_PREC_MIXED) return fail(SS_ERR_INVALID, "unknown precision %d", prec);
-    if (g->wk == k && g->wprec == prec) return SS_OK;
+    if (g->wprec == prec && k <= g->wcap) {
+        g->wk = k;
+        bind_scalars(g, k);
+        return SS_OK;
+    }
+    const int cap = g->wprec == prec ? std::max(k, g->wcap) : k;
     SS_HIP(hipSetDevice(g->ctx->device));
     g->arena_work.release();
-    g->wk = 0;
+    g->wk = g->wcap = 0;
     g->wprec = -1;
     g->Y64 = g->X64 = g->R64 = g->B64 = nullptr;
     g->Y32 = g->X32 = nullptr;
     g->w64 = PcgWork<double>();
     g->w32 = PcgWork<float>();
     auto &A = g->arena_work;
-    const size_t nk = (size_t)g->N * (size_t)k, gk = (size_t)g->G * k;
+    const size_t nk = (size_t)g->N * (size_t)cap, gk = (size_t)g->G * cap;
     SS_TRY(A.alloc(&g->Y64, nk));
     SS_TRY(A.alloc(&g->X64, nk));
     auto alloc_work = [&](auto &w) -> int {
@@ -561,23 +593,18 @@ int graph_ensure_work(ss_graph *g, int k, int prec) {
         SS_TRY(A.alloc(&g->X32, nk));
         SS_TRY(alloc_work(g->w32));
     }
-    SS_TRY(A.alloc(&g->partial, (size_t)std::max(g->nrt, 1) * k));
-    SS_TRY(A.alloc(&g->partial2, (size_t)std::max(g->nrt, 1) * k));
+    SS_TRY(A.alloc(&g->partial, (size_t)std::max(g->nrt, 1) * cap));
+    SS_TRY(A.alloc(&g->partial2, (size_t)std::max(g->nrt, 1) * cap));
     SS_TRY(A.alloc(&g->gsum, gk));
-    SS_TRY(A.alloc(&g->csum, (size_t)g->ncomp * k));
-    SS_TRY(A.alloc(&g->cmean, (size_t)g->ncomp * k));
+    SS_TRY(A.alloc(&g->csum, (size_t)g->ncomp * cap));
+    SS_TRY(A.alloc(&g->cmean, (size_t)g->ncomp * cap));
     SS_TRY(A.alloc(&g->sc_buf, 5 * gk));
     SS_TRY(A.alloc(&g->sc_int, 2 * gk));
     SS_TRY(A.alloc(&g->colkeep, gk));
-    g->sc.rz = g->sc_buf;
-    g->sc.alpha = g->sc_buf + gk;
-    g->sc.beta = g->sc_buf + 2 * gk;
-    g->sc.rr = g->sc_buf + 3 * gk;
-    g->sc.bnorm = g->sc_buf + 4 * gk;
-    g->sc.active = g->sc_int;
-    g->sc.iters = g->sc_int + gk;
     g->wk = k;
+    g->wcap = cap;
     g->wprec = prec;
+    bind_scalars(g, k);
     return SS_OK;
 }
 
