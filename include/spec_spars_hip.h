@@ -295,11 +295,14 @@ int ss_graph_profile_read(const ss_graph *g, ss_kernel_stat *out /* [SS_KERNEL_C
  * when it was created (its reductions are shaped by it, so compare results
  * bit for bit only between graphs created under the same setting);
  * level_rows_per_wave = consecutive short rows (at most 32 entries) each
- * wavefront of a per-level triangular-solve launch finishes; it changes
- * timing only. */
+ * wavefront of a per-level triangular-solve launch finishes; graph_iters =
+ * PCG iterations captured in one hipGraph and replayed between two reads of
+ * the active-column count (0 = launch every kernel directly; per-kernel
+ * profiling always launches directly).  Both change timing only. */
 typedef struct ss_tuning {
     int32_t tile_waves;            /* 1, 2, 4, 8 or 16 (default 4)           */
     int32_t level_rows_per_wave;   /* 1 .. 64 (default 1)                    */
+    int32_t graph_iters;           /* 0 .. 16 (default 0; the autotuner picks 0 or 4) */
 } ss_tuning;
 int ss_context_get_tuning(const ss_context *ctx, ss_tuning *out);
 int ss_context_set_tuning(ss_context *ctx, const ss_tuning *t);

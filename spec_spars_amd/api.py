@@ -115,10 +115,11 @@ class Context:
     def tuning(self):
         t = _lib.Tuning()
         check(load().ss_context_get_tuning(self._h, ctypes.byref(t)))
-        return {"tile_waves": t.tile_waves, "level_rows_per_wave": t.level_rows_per_wave}
+        return {"tile_waves": t.tile_waves, "level_rows_per_wave": t.level_rows_per_wave,
+                "graph_iters": t.graph_iters}
 
-    def set_tuning(self, tile_waves=4, level_rows_per_wave=1):
-        t = _lib.Tuning(tile_waves, level_rows_per_wave)
+    def set_tuning(self, tile_waves=4, level_rows_per_wave=1, graph_iters=0):
+        t = _lib.Tuning(tile_waves, level_rows_per_wave, graph_iters)
         check(load().ss_context_set_tuning(self._h, ctypes.byref(t)))
 
     def autotune(self, graph, k, precision=_lib.PREC_FP64, reps=5):
@@ -129,7 +130,7 @@ class Context:
         check(load().ss_context_autotune(self._h, graph._h, k, precision, reps, ctypes.byref(t),
                                          ctypes.byref(ml), ctypes.byref(mt)))
         return {"tile_waves": t.tile_waves, "level_rows_per_wave": t.level_rows_per_wave,
-                "ms_level": ml.value, "ms_tile": mt.value}
+                "graph_iters": t.graph_iters, "ms_level": ml.value, "ms_tile": mt.value}
 
     def bench_copy(self, nbytes, reps=20):
         """Average ms of a plain FP64 copy kernel over nbytes (read + write)."""

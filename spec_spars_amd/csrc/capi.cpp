@@ -546,23 +546,27 @@ int ss_graph_profile_read(const ss_graph *g, ss_kernel_stat *out) {
 static bool valid_tuning(const ss_tuning &t) {
     const bool tw = t.tile_waves == 1 || t.tile_waves == 2 || t.tile_waves == 4 || t.tile_waves == 8 ||
                     t.tile_waves == 16;
-    return tw && t.level_rows_per_wave >= 1 && t.level_rows_per_wave <= 64;
+    return tw && t.level_rows_per_wave >= 1 && t.level_rows_per_wave <= 64 && t.graph_iters >= 0 &&
+           t.graph_iters <= 16;
 }
 
 int ss_context_get_tuning(const ss_context *ctx, ss_tuning *out) {
     if (!ctx || !out) return fail(SS_ERR_INVALID, "null argument");
     out->tile_waves = ctx->tune.tile_waves;
     out->level_rows_per_wave = ctx->tune.level_rows_per_wave;
+    out->graph_iters = ctx->tune.graph_iters;
     return SS_OK;
 }
 
 int ss_context_set_tuning(ss_context *ctx, const ss_tuning *t) {
     if (!ctx || !t) return fail(SS_ERR_INVALID, "null argument");
     if (!valid_tuning(*t))
-        return fail(SS_ERR_INVALID, "tile_waves %d (1,2,4,8,16) / level_rows_per_wave %d (1..64) out of range",
-                    t->tile_waves, t->level_rows_per_wave);
+        return fail(SS_ERR_INVALID,
+                    "tile_waves %d (1,2,4,8,16) / level_rows_per_wave %d (1..64) / graph_iters %d (0..16) out of range",
+                    t->tile_waves, t->level_rows_per_wave, t->graph_iters);
     ctx->tune.tile_waves = t->tile_waves;
     ctx->tune.level_rows_per_wave = t->level_rows_per_wave;
+    ctx->tune.graph_iters = t->graph_iters;
     return SS_OK;
 }
 
@@ -579,6 +583,7 @@ int ss_context_autotune(ss_context *ctx, ss_graph *g, int32_t k, int32_t precisi
     if (best) {
         best->tile_waves = b.tile_waves;
         best->level_rows_per_wave = b.level_rows_per_wave;
+        best->graph_iters = b.graph_iters;
     }
     if (ms_level) *ms_level = ml;
     if (ms_tile) *ms_tile = mt;

@@ -13,6 +13,7 @@
 struct ss_tuning_state {
     int tile_waves = 4;          // wavefronts per row-tile block (1, 2, 4, 8, 16)
     int level_rows_per_wave = 1; // rows per wavefront of the short rows in the per-level solve launches
+    int graph_iters = 0;         // PCG iterations per captured hipGraph replay (0 = direct launches)
 };
 
 struct ss_context {
@@ -51,6 +52,29 @@ struct DevArena {
         bytes = 0;
     }
     ~DevArena() { release(); }
+};
+
+// What a captured PCG iteration graph bakes in; a different key recaptures.
+struct GraphKey {
+    int k = 0, rule = 0;
+    double tol = 0.0;
+    int iters = 0;
+    const void *X = nullptr, *R = nullptr;
+    uint64_t factor_epoch = 0;
+    int rpw = 0, tile_waves = 0;
+    bool operator==(const GraphKey &o) const {
+        return k == o.k && rule == o.rule && tol == o.tol && iters == o.iters && X == o.X && R == o.R &&
+               factor_epoch == o.factor_epoch && rpw == o.rpw && tile_waves == o.tile_waves;
+    }
+};
+struct GraphCache {
+    GraphKey key;
+    hipGraphExec_t exec = nullptr;
+    void reset() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+    }
+    ~GraphCache() { reset(); }
 };
 
 // Work blocks of one PCG run in precision T (N x k each).
@@ -92,6 +116,8 @@ struct ss_graph {
     std::vector<int32_t> h_comp_count;   // components per graph
     // factor schedules (of the concatenated factor)
     bool has_factor = false;
+    uint64_t factor_epoch = 0;             // bumped on every factor upload
+    ss::GraphCache gc64, gc32;             // captured PCG iterations (FP64 / FP32 work)
     int64_t factor_nnz = 0;
     int32_t depth_fwd = 0, depth_bwd = 0;
     int32_t *f_rows = nullptr, *f_ptr = nullptr, *f_col = nullptr;

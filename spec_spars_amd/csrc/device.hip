@@ -523,6 +523,9 @@ int batch_upload_factors(ss_graph *g, const std::vector<const Factor *> &factors
     g->factor_nnz = F.nnz;
     g->depth_fwd = F.depth_fwd;
     g->depth_bwd = F.depth_bwd;
+    g->factor_epoch++;
+    g->gc64.reset();
+    g->gc32.reset();
     g->has_factor = true;
     return SS_OK;
 }
@@ -564,6 +567,8 @@ int graph_ensure_work(ss_graph *g, int k, int prec) {
     }
     const int cap = g->wprec == prec ? std::max(k, g->wcap) : k;
     SS_HIP(hipSetDevice(g->ctx->device));
+    g->gc64.reset();
+    g->gc32.reset();
     g->arena_work.release();
     g->wk = g->wcap = 0;
     g->wprec = -1;
@@ -716,6 +721,40 @@ static void component_means(ss_graph *g, int k, const double *Y) {
     bt_comp_means(s, g->ncomp, k, g->comp_graph, g->comp_off, g->cptr, g->gsum, g->csum, g->cmean);
 }
 
+// One PCG iteration on every (graph, column): P = Z + beta P; Q = A P;
+// alpha; X += alpha P, R -= alpha Q; stop test.  With tail, also the next
+// preconditioner application and beta (what the next iteration needs).
+template <typename T>
+static int pcg_iteration(ss_graph *g, int k, T *X, const PcgWork<T> &w, int rule, double tol, bool tail) {
+    hipStream_t s = g->ctx->stream;
+    const Tiles rt = rtiles(g);
+    const int gk = g->G * k;
+    {
+        ProfScope ps(g, SS_KERNEL_VECTOR);
+        bt_update_p<T>(s, rt, k, g->sc.beta, w.Zp, w.P);   // P = Z + beta P
+    }
+    {
+        ProfScope ps(g, SS_KERNEL_SPMM);
+        bt_spmm_dot<T>(s, rt, g->rowptr, g->col, avals<T>(g), k, w.P, w.Q, g->partial);   // Q = A P, P.Q
+    }
+    bt_pcg_alpha(s, gk, k, g->gtile, g->partial, g->sc, g->status);
+    {
+        ProfScope ps(g, SS_KERNEL_VECTOR);
+        bt_update_xr_rr<T>(s, rt, k, g->sc.alpha, w.P, w.Q, X, w.R, g->partial);
+    }
+    bt_pcg_check(s, gk, k, g->gtile, g->partial, rule, tol, g->sc, g->status);
+    if (tail) {
+        SS_TRY(precond<T>(g, k, w.R, w.W, w.Zp, g->partial2));
+        bt_pcg_beta(s, gk, k, g->gtile, g->partial2, g->sc);
+    }
+    SS_HIP(hipGetLastError());
+    return SS_OK;
+}
+
+template <typename T> static GraphCache &graph_cache(ss_graph *g);
+template <> GraphCache &graph_cache<double>(ss_graph *g) { return g->gc64; }
+template <> GraphCache &graph_cache<float>(ss_graph *g) { return g->gc32; }
+
 // PCG on all k columns of all graphs: X ~ A^+ (rhs - component means).
 // Vectors in precision T, dots and scalars in FP64.  With mean_src the
 // component means of that FP64 block are removed (rhs itself in the FP64
@@ -737,21 +776,47 @@ static int pcg_run(ss_graph *g, int k, const T *rhs, const double *mean_src, T *
     SS_TRY(precond<T>(g, k, w.R, w.W, w.Zp, g->partial2));
     bt_tile_partial<T, T>(s, rt, k, w.B, w.B, nullptr, 0, g->partial);
     bt_pcg_init(s, gk, k, g->gtile, g->partial, g->partial2, g->sc);   // beta = 0: P starts as Z
-    for (int it = 0; it < max_iter; it++) {
-        {
-            ProfScope ps(g, SS_KERNEL_VECTOR);
-            bt_update_p<T>(s, rt, k, g->sc.beta, w.Zp, w.P);   // P = Z + beta P
+    const int m = g->prof ? 0 : g->ctx->tune.graph_iters;
+    int it = 0;
+    if (m > 0) {
+        // whole iterations replayed from a captured graph, the host reading
+        // the active count after each replay; an iteration after every
+        // column converged changes nothing (alpha = 0 freezes X and R)
+        GraphCache &gc = graph_cache<T>(g);
+        const GraphKey key{k, rule, tol, m, (const void *)X, (const void *)w.R, g->factor_epoch,
+                           g->ctx->tune.level_rows_per_wave, g->tile_waves};
+        if (!gc.exec || !(gc.key == key)) {
+            gc.reset();
+            int rc = SS_OK;
+            SS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+            for (int r = 0; r < m && rc == SS_OK; r++) rc = pcg_iteration<T>(g, k, X, w, rule, tol, true);
+            hipGraph_t graph = nullptr;
+            const hipError_t e = hipStreamEndCapture(s, &graph);
+            if (rc != SS_OK || e != hipSuccess || !graph) {
+                if (graph) (void)hipGraphDestroy(graph);
+                (void)hipGetLastError();
+                return rc != SS_OK ? rc : fail(SS_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+            }
+            const hipError_t ei = hipGraphInstantiate(&gc.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ei != hipSuccess) {
+                gc.exec = nullptr;
+                return fail(SS_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(ei));
+            }
+            gc.key = key;
         }
-        {
-            ProfScope ps(g, SS_KERNEL_SPMM);
-            bt_spmm_dot<T>(s, rt, g->rowptr, g->col, avals<T>(g), k, w.P, w.Q, g->partial);   // Q = A P, P.Q
+        for (; it + m <= max_iter; it += m) {
+            SS_HIP(hipGraphLaunch(gc.exec, s));
+            int nactive = 0;
+            SS_TRY(read_status(g, &nactive));
+            if (nactive == 0) {
+                it = max_iter;
+                break;
+            }
         }
-        bt_pcg_alpha(s, gk, k, g->gtile, g->partial, g->sc, g->status);
-        {
-            ProfScope ps(g, SS_KERNEL_VECTOR);
-            bt_update_xr_rr<T>(s, rt, k, g->sc.alpha, w.P, w.Q, X, w.R, g->partial);
-        }
-        bt_pcg_check(s, gk, k, g->gtile, g->partial, rule, tol, g->sc, g->status);
+    }
+    for (; it < max_iter; it++) {
+        SS_TRY(pcg_iteration<T>(g, k, X, w, rule, tol, false));
         int nactive = 0;
         SS_TRY(read_status(g, &nactive));
         if (nactive == 0) break;
@@ -913,6 +978,40 @@ static int autotune_t(ss_graph *g, int k, const PcgWork<T> &w, int reps, ss_tuni
             ms_tile = ms;
             best.tile_waves = tw;
         }
+    }
+    // direct launches against graph replay, 4 PCG iterations each (the
+    // state need not be meaningful for timing, only valid)
+    {
+        const Tiles rt = rtiles(g);
+        const int gk = g->G * k;
+        T *X = std::is_same<T, double>::value ? (T *)g->X64 : (T *)g->X32;
+        bt_project<T, T>(s, rt, k, w.R, g->comp_of_row, g->cmean, w.R, w.B, X);
+        SS_TRY(precond<T>(g, k, w.R, w.W, w.Zp, g->partial2));
+        bt_tile_partial<T, T>(s, rt, k, w.B, w.B, nullptr, 0, g->partial);
+        bt_pcg_init(s, gk, k, g->gtile, g->partial, g->partial2, g->sc);
+        constexpr int M = 4;
+        double ms_direct = 0, ms_graph = 1e30;
+        SS_TRY(time_median(g, reps, [&]() {
+            for (int r = 0; r < M; r++) SS_TRY(pcg_iteration<T>(g, k, X, w, SS_STOP_REL, 0.0, true));
+            return SS_OK;
+        }, ms_direct));
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        int rc = SS_OK;
+        SS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+        for (int r = 0; r < M && rc == SS_OK; r++) rc = pcg_iteration<T>(g, k, X, w, SS_STOP_REL, 0.0, true);
+        const hipError_t e = hipStreamEndCapture(s, &graph);
+        if (rc == SS_OK && e == hipSuccess && graph && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess) {
+            rc = time_median(g, reps, [&]() {
+                SS_HIP(hipGraphLaunch(exec, s));
+                return SS_OK;
+            }, ms_graph);
+        }
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
+        SS_TRY(rc);
+        best.graph_iters = ms_graph < 0.97 * ms_direct ? M : 0;
     }
     g->tile_waves = orig_waves;
     g->ctx->tune = orig;

@@ -309,4 +309,36 @@ def test_autotune_and_tuning_shapes(ctx, oracle_mod):
         with pytest.raises(ss.SSError):
             ctx.set_tuning(tile_waves=3)
     finally:
-        ctx.set_tuning(**{k: saved[k] for k in ("tile_waves", "level_rows_per_wave")})
+        ctx.set_tuning(**{k: saved[k] for k in ("tile_waves", "level_rows_per_wave", "graph_iters")})
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_graph_replay_equals_direct_launches(ctx, prec):
+    """PCG iterations replayed from captured hipGraphs give the same bits as
+    direct launches, over repeated solves with eager work between them and
+    a reused capture."""
+    import spec_spars_amd as ss
+    gs = _mix()
+    fs = _factors(gs, ordering=ss.ORDER_ROUNDS)
+    saved = ctx.tuning()
+    try:
+        ctx.set_tuning(tile_waves=saved["tile_waves"], level_rows_per_wave=1, graph_iters=0)
+        direct = _run(ctx, gs, fs, list(range(len(gs))), 8, prec)
+        for gi in (1, 3):
+            ctx.set_tuning(tile_waves=saved["tile_waves"], level_rows_per_wave=1, graph_iters=gi)
+            B = ss.Batch(ctx, gs)
+            B.set_factors(fs)
+            for rep in range(3):   # the second and third solves replay the cached graph
+                from spec_spars_amd import graphs
+                betas = [graphs.beta_for(g[0], 0.3) for g in gs]
+                reps = B.sparsify(8, betas, sketch_seeds=[10 + i for i in range(len(gs))],
+                                  sampling_seeds=[20 + i for i in range(len(gs))],
+                                  solve=ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-8, max_iter=3000,
+                                                      precision=prec))
+                for i in range(len(gs)):
+                    assert np.array_equal(B.read_block(i, "Z"), direct[i]["Z"]), (gi, rep, i)
+                    assert np.array_equal(B.read_probs(i), direct[i]["p"])
+                    assert reps[i]["max_iters"] == direct[i]["rep"]["max_iters"]
+            B.close()
+    finally:
+        ctx.set_tuning(**{k: saved[k] for k in ("tile_waves", "level_rows_per_wave", "graph_iters")})
