@@ -353,6 +353,42 @@ def bench_serve(args):
     return 0
 
 
+def probe(args):
+    """A short fixed program for hardware-counter runs: the workload's
+    graph and factor, one sparsify pass capped at 3 PCG iterations with the
+    default tuned shapes, then one 1 GiB FP64 copy for calibration."""
+    import spec_spars_amd as ss
+    from spec_spars_amd import graphs
+    W = workload(args.workload)
+    n, u, v, w, k = W["n"], W["u"], W["v"], W["w"], W["k"]
+    ctx = ss.Context(0)
+    G = ss.Graph(ctx, n, u, v, w)
+    cp, ri, va = G.laplacian()
+    G.set_factor(ss.Factor(cp, ri, va, seed=0, ordering=ss.ORDER_ROUNDS))
+    ctx.set_tuning(tile_waves=16, level_rows_per_wave=1, graph_iters=0, xcd_slices=0 if args.no_xcd else 1)
+    prec = ss.PREC_FP64 if (args.precision or W.get("precision", "fp64")) == "fp64" else ss.PREC_MIXED
+    solve = ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-8, max_iter=3, precision=prec)
+    solve.fail_relres = 1e300
+    rep = G.sparsify(k, graphs.beta_for(n, W["eps"]), scale=1.0 / math.sqrt(k), solve=solve)
+    log("probe pass %.1f ms" % rep["ms_total"])
+    log("copy %.3f ms per GiB copy" % ctx.bench_copy(1 << 30, reps=1))
+    G.close()
+    ctx.close()
+    return 0
+
+
+def measured_traffic(workload, precision, kind):
+    """HBM bytes per launch of a kernel kind from the committed counter
+    summary (profiles/), when one exists for this workload."""
+    path = os.path.join(ROOT, "profiles", "trisolve_%s_%s_pmc.md" % (workload, precision))
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch %s:" % kind):
+            return int(line.split(":")[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -365,6 +401,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel event timing")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no autotuner)")
+    ap.add_argument("--probe", action="store_true",
+                    help="one short sparsify pass (3 PCG iterations) and a 1 GiB copy, for counter runs")
+    ap.add_argument("--no-xcd", action="store_true", help="probe: round-robin rows over the XCDs")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: Poisson arrivals per second")
     ap.add_argument("--duration", type=float, default=60.0, help="serve: arrival window in seconds")
     ap.add_argument("--pool", type=int, default=12, help="serve: resident graphs")
@@ -379,6 +418,8 @@ def main():
     import spec_spars_amd as ss
     from spec_spars_amd import graphs
 
+    if args.probe:
+        return probe(args)
     if args.workload == "serve":
         if args.precision is None:
             args.precision = "mixed"
@@ -449,6 +490,8 @@ def main():
 
     kb = kernel_bytes(n, m, G, k, last["kept"], s=8 if prec == ss.PREC_FP64 else 4)
     kernels, roofline = summarize(prof, kb, t_prof)
+    if roofline:
+        roofline["traffic"] = measured_traffic(W["name"], args.precision, roofline["kernel"])
     copy_ms = ctx.bench_copy(1 << 30, reps=10)
     copy_gbs = 2.0 * (1 << 30) / (copy_ms * 1e-3) / 1e9
 

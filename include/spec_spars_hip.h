@@ -303,6 +303,9 @@ typedef struct ss_tuning {
     int32_t tile_waves;            /* 1, 2, 4, 8 or 16 (default 4)           */
     int32_t level_rows_per_wave;   /* 1 .. 64 (default 1)                    */
     int32_t graph_iters;           /* 0 .. 16 (default 0; the autotuner picks 0 or 4) */
+    int32_t xcd_slices;            /* 1: each of the 8 XCDs takes one contiguous
+                                      slice of a level's rows; 0: round robin
+                                      (default; the autotuner picks)          */
 } ss_tuning;
 int ss_context_get_tuning(const ss_context *ctx, ss_tuning *out);
 int ss_context_set_tuning(ss_context *ctx, const ss_tuning *t);

@@ -54,7 +54,7 @@ class SparsifyOpts(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("tile_waves", ctypes.c_int32), ("level_rows_per_wave", ctypes.c_int32),
-                ("graph_iters", ctypes.c_int32)]
+                ("graph_iters", ctypes.c_int32), ("xcd_slices", ctypes.c_int32)]
 
 
 class SparsifyReport(ctypes.Structure):

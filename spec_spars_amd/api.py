@@ -116,10 +116,10 @@ class Context:
         t = _lib.Tuning()
         check(load().ss_context_get_tuning(self._h, ctypes.byref(t)))
         return {"tile_waves": t.tile_waves, "level_rows_per_wave": t.level_rows_per_wave,
-                "graph_iters": t.graph_iters}
+                "graph_iters": t.graph_iters, "xcd_slices": t.xcd_slices}
 
-    def set_tuning(self, tile_waves=4, level_rows_per_wave=1, graph_iters=0):
-        t = _lib.Tuning(tile_waves, level_rows_per_wave, graph_iters)
+    def set_tuning(self, tile_waves=4, level_rows_per_wave=1, graph_iters=0, xcd_slices=1):
+        t = _lib.Tuning(tile_waves, level_rows_per_wave, graph_iters, xcd_slices)
         check(load().ss_context_set_tuning(self._h, ctypes.byref(t)))
 
     def autotune(self, graph, k, precision=_lib.PREC_FP64, reps=5):
@@ -130,7 +130,8 @@ class Context:
         check(load().ss_context_autotune(self._h, graph._h, k, precision, reps, ctypes.byref(t),
                                          ctypes.byref(ml), ctypes.byref(mt)))
         return {"tile_waves": t.tile_waves, "level_rows_per_wave": t.level_rows_per_wave,
-                "graph_iters": t.graph_iters, "ms_level": ml.value, "ms_tile": mt.value}
+                "graph_iters": t.graph_iters, "xcd_slices": t.xcd_slices, "ms_level": ml.value,
+                "ms_tile": mt.value}
 
     def bench_copy(self, nbytes, reps=20):
         """Average ms of a plain FP64 copy kernel over nbytes (read + write)."""

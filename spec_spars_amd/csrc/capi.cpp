@@ -547,7 +547,7 @@ static bool valid_tuning(const ss_tuning &t) {
     const bool tw = t.tile_waves == 1 || t.tile_waves == 2 || t.tile_waves == 4 || t.tile_waves == 8 ||
                     t.tile_waves == 16;
     return tw && t.level_rows_per_wave >= 1 && t.level_rows_per_wave <= 64 && t.graph_iters >= 0 &&
-           t.graph_iters <= 16;
+           t.graph_iters <= 16 && (t.xcd_slices == 0 || t.xcd_slices == 1);
 }
 
 int ss_context_get_tuning(const ss_context *ctx, ss_tuning *out) {
@@ -555,6 +555,7 @@ int ss_context_get_tuning(const ss_context *ctx, ss_tuning *out) {
     out->tile_waves = ctx->tune.tile_waves;
     out->level_rows_per_wave = ctx->tune.level_rows_per_wave;
     out->graph_iters = ctx->tune.graph_iters;
+    out->xcd_slices = ctx->tune.xcd_slices;
     return SS_OK;
 }
 
@@ -567,6 +568,7 @@ int ss_context_set_tuning(ss_context *ctx, const ss_tuning *t) {
     ctx->tune.tile_waves = t->tile_waves;
     ctx->tune.level_rows_per_wave = t->level_rows_per_wave;
     ctx->tune.graph_iters = t->graph_iters;
+    ctx->tune.xcd_slices = t->xcd_slices;
     return SS_OK;
 }
 
@@ -584,6 +586,7 @@ int ss_context_autotune(ss_context *ctx, ss_graph *g, int32_t k, int32_t precisi
         best->tile_waves = b.tile_waves;
         best->level_rows_per_wave = b.level_rows_per_wave;
         best->graph_iters = b.graph_iters;
+        best->xcd_slices = b.xcd_slices;
     }
     if (ms_level) *ms_level = ml;
     if (ms_tile) *ms_tile = mt;

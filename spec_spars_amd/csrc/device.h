@@ -14,6 +14,7 @@ struct ss_tuning_state {
     int tile_waves = 4;          // wavefronts per row-tile block (1, 2, 4, 8, 16)
     int level_rows_per_wave = 1; // rows per wavefront of the short rows in the per-level solve launches
     int graph_iters = 0;         // PCG iterations per captured hipGraph replay (0 = direct launches)
+    int xcd_slices = 0;          // per-level launches: each XCD takes one contiguous slice of a level's rows
 };
 
 struct ss_context {

@@ -659,8 +659,8 @@ static void solve_levels(ss_graph *g, int k, const T *in, T *out) {
         const int32_t e = r < runs.size() ? runs[r].first : depth;
         ProfScope ps(g, BWD ? SS_KERNEL_TRI_BWD : SS_KERNEL_TRI_FWD, e - l);
         for (; l < e; l++)
-            launch_trisolve_level<T, BWD>(s, lp[l], nlong[l], lp[l + 1], waves[l], rpw, rows, ptr, col, val, g->diag,
-                                          k, in, out);
+            launch_trisolve_level<T, BWD>(s, lp[l], nlong[l], lp[l + 1], waves[l], rpw, g->ctx->tune.xcd_slices, rows,
+                                          ptr, col, val, g->diag, k, in, out);
     }
 }
 
@@ -951,15 +951,18 @@ static int autotune_t(ss_graph *g, int k, const PcgWork<T> &w, int reps, ss_tuni
     const ss_tuning_state orig = g->ctx->tune;
     const int orig_waves = g->tile_waves;
     ms_level = 1e30;
-    for (int rpw : {1, 2, 4}) {
-        g->ctx->tune.level_rows_per_wave = rpw;
-        double ms = 0;
-        SS_TRY(time_median(g, reps, [&]() { return precond<T>(g, k, w.R, w.W, w.Zp, g->partial2); }, ms));
-        if (ms < ms_level) {
-            ms_level = ms;
-            best.level_rows_per_wave = rpw;
+    for (int xs : {0, 1})
+        for (int rpw : {1, 2, 4}) {
+            g->ctx->tune.level_rows_per_wave = rpw;
+            g->ctx->tune.xcd_slices = xs;
+            double ms = 0;
+            SS_TRY(time_median(g, reps, [&]() { return precond<T>(g, k, w.R, w.W, w.Zp, g->partial2); }, ms));
+            if (ms < ms_level) {
+                ms_level = ms;
+                best.level_rows_per_wave = rpw;
+                best.xcd_slices = xs;
+            }
         }
-    }
     ms_tile = 1e30;
     launch_fill_u32(s, (int64_t)2 * g->G * k, reinterpret_cast<uint32_t *>(g->sc.beta), 0u);
     launch_fill_u32(s, (int64_t)2 * g->G * k, reinterpret_cast<uint32_t *>(g->sc.alpha), 0u);

@@ -81,8 +81,9 @@ static const int kMaxPartialBlocks = 4096;
 // Rows of one level, positions [t0, t1); the first nlong of them are rows
 // of more than kLongRow entries (a block of `waves` wavefronts each: 4, 8 or
 // 16), the rest go rpw to a wave.
+// xcd != 0: each of the 8 XCDs takes one contiguous slice of the short rows.
 template <typename T, bool BWD>
-void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nlong, int32_t t1, int waves, int rpw,
+void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nlong, int32_t t1, int waves, int rpw, int xcd,
                            const int32_t *rows, const int32_t *ptr, const int32_t *col, const T *val,
                            const double *diag, int k, const T *in, T *out);
 template <typename T>

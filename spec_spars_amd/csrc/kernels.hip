@@ -540,6 +540,7 @@ __device__ __forceinline__ T seg_apply_shared(T acc, int a, int b, bool act, int
 // rows per wave.
 template <typename T, bool BWD>
 __global__ __launch_bounds__(64 * kLevelWaves) void k_trisolve_level(int32_t t0, int32_t nlong, int32_t t1, int rpw,
+                                                                     int xcd_slices,
                                                                      const int32_t *__restrict__ rows,
                                                                      const int32_t *__restrict__ ptr,
                                                                      const int32_t *__restrict__ col,
@@ -561,8 +562,14 @@ __global__ __launch_bounds__(64 * kLevelWaves) void k_trisolve_level(int32_t t0,
         }
         return;
     }
-    // rpw consecutive rows per wave
-    const int tb = wave_uniform(t0 + nlong + (((int)blockIdx.x - nlong) * W + wv) * rpw);
+    // rpw consecutive rows per wave.  Workgroups go to the 8 XCDs round
+    // robin; block b works on chunk (b % 8) * per + b / 8, so each XCD owns
+    // one contiguous slice of the level's rows (with rows in elimination
+    // order: one region of the graph, the same region at every level)
+    const int nb = (int)gridDim.x - nlong, b = (int)blockIdx.x - nlong;
+    const int per = nb / kXcds;
+    const int chunk = xcd_slices ? (b % kXcds) * per + b / kXcds : b;
+    const int tb = wave_uniform(t0 + nlong + (chunk * W + wv) * rpw);
     const int te = tb + rpw < t1 ? tb + rpw : t1;
     for (int t = tb; t < te; t++) {
         const int i = wave_uniform(rows[t]);
@@ -1100,7 +1107,7 @@ template void launch_update_xr_rr<float>(hipStream_t, int32_t, int, const double
                                          float *, float *, double *);
 
 template <typename T, bool BWD>
-void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nlong, int32_t t1, int waves, int rpw,
+void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nlong, int32_t t1, int waves, int rpw, int xcd,
                            const int32_t *rows, const int32_t *ptr, const int32_t *col, const T *val,
                            const double *diag, int k, const T *in, T *out) {
     if (t1 <= t0) return;
@@ -1108,12 +1115,18 @@ void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nlong, int32_t t1,
     if (waves < 4 || waves > kLevelWaves || (waves & (waves - 1))) waves = 4;
     const int64_t short_rows = (int64_t)t1 - t0 - nlong;
     const int64_t per_block = (int64_t)waves * rpw;
-    const unsigned grid = (unsigned)nlong + (unsigned)((short_rows + per_block - 1) / per_block);
-    hipLaunchKernelGGL((k_trisolve_level<T, BWD>), dim3(grid), dim3(64 * waves), 0, s, t0, nlong, t1, rpw, rows, ptr,
-                       col, val, diag, k, in, out);
+    int64_t blocks = (short_rows + per_block - 1) / per_block;
+    // XCD slices: a whole number of blocks per XCD (blocks past the rows
+    // return at once), only when every XCD gets several
+    const int xcd_slices = xcd && blocks >= 4 * kXcds;
+    if (xcd_slices) blocks = (blocks + kXcds - 1) / kXcds * kXcds;
+    const unsigned grid = (unsigned)nlong + (unsigned)blocks;
+    hipLaunchKernelGGL((k_trisolve_level<T, BWD>), dim3(grid), dim3(64 * waves), 0, s, t0, nlong, t1, rpw, xcd_slices,
+                       rows, ptr, col, val, diag, k, in, out);
 }
 #define SS_INST_TRI(T, B)                                                                                      \
-    template void launch_trisolve_level<T, B>(hipStream_t, int32_t, int32_t, int32_t, int, int, const int32_t *,\
+    template void launch_trisolve_level<T, B>(hipStream_t, int32_t, int32_t, int32_t, int, int, int,            \
+                                              const int32_t *,                                                   \
                                               const int32_t *, const int32_t *, const T *, const double *, int,  \
                                               const T *, T *);
 SS_INST_TRI(double, false)

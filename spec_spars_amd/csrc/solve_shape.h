@@ -30,6 +30,7 @@ constexpr int seg_parts(int len) { return len <= 8 ? 1 : (len <= 32 ? 4 : (len <
 constexpr int kMaxParts = 16;
 constexpr int kCoreBlock = 64;     // rows per block of the dense-core substitution
 constexpr int kCoreMaxRows = 16384; // largest dense core per graph (its trailing small levels beyond stay sparse)
-constexpr int kLevelWaves = 16;   // wavefronts per block of a per-level launch (one long row, or 16 short rows)
+constexpr int kLevelWaves = 16;
+constexpr int kXcds = 8;           // MI355X: workgroups are dealt to the 8 XCDs round robin   // wavefronts per block of a per-level launch (one long row, or 16 short rows)
 
 }  // namespace ss

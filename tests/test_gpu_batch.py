@@ -309,7 +309,7 @@ def test_autotune_and_tuning_shapes(ctx, oracle_mod):
         with pytest.raises(ss.SSError):
             ctx.set_tuning(tile_waves=3)
     finally:
-        ctx.set_tuning(**{k: saved[k] for k in ("tile_waves", "level_rows_per_wave", "graph_iters")})
+        ctx.set_tuning(**{k: saved[k] for k in ("tile_waves", "level_rows_per_wave", "graph_iters", "xcd_slices")})
 
 
 @pytest.mark.parametrize("prec", [0, 1])
@@ -341,4 +341,4 @@ def test_graph_replay_equals_direct_launches(ctx, prec):
                     assert reps[i]["max_iters"] == direct[i]["rep"]["max_iters"]
             B.close()
     finally:
-        ctx.set_tuning(**{k: saved[k] for k in ("tile_waves", "level_rows_per_wave", "graph_iters")})
+        ctx.set_tuning(**{k: saved[k] for k in ("tile_waves", "level_rows_per_wave", "graph_iters", "xcd_slices")})
