@@ -127,6 +127,7 @@ struct ss_graph {
     float *f_valf = nullptr, *b_valf = nullptr;
     std::vector<int32_t> f_level_ptr, b_level_ptr;   // host copies: launch bounds per level
     std::vector<int32_t> f_level_long, b_level_long; // long rows leading each level
+    std::vector<int32_t> f_level_med, b_level_med;   // then rows taking one wave each
     std::vector<int32_t> f_level_waves, b_level_waves;   // wavefronts per block of each level launch
     std::vector<std::pair<int32_t, int32_t>> f_runs, b_runs; // runs of small levels (single-workgroup launches)
     int32_t *f_lp_dev = nullptr, *b_lp_dev = nullptr;   // level_ptr on the device

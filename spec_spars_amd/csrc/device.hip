@@ -471,6 +471,8 @@ int batch_upload_factors(ss_graph *g, const std::vector<const Factor *> &factors
     g->b_level_ptr = bw.level_ptr;
     g->f_level_long = fw.level_long;
     g->b_level_long = bw.level_long;
+    g->f_level_med = fw.level_med;
+    g->b_level_med = bw.level_med;
     g->f_runs = fw.runs;
     g->b_runs = bw.runs;
     g->f_core_t0 = fw.core_t0;
@@ -640,6 +642,7 @@ static void solve_levels(ss_graph *g, int k, const T *in, T *out) {
     const int rpw = g->ctx->tune.level_rows_per_wave;
     const std::vector<int32_t> &lp = BWD ? g->b_level_ptr : g->f_level_ptr;
     const std::vector<int32_t> &nlong = BWD ? g->b_level_long : g->f_level_long;
+    const std::vector<int32_t> &nmed = BWD ? g->b_level_med : g->f_level_med;
     const std::vector<int32_t> &waves = BWD ? g->b_level_waves : g->f_level_waves;
     const auto &runs = BWD ? g->b_runs : g->f_runs;
     const int32_t *rows = BWD ? g->b_rows : g->f_rows, *ptr = BWD ? g->b_ptr : g->f_ptr;
@@ -659,8 +662,8 @@ static void solve_levels(ss_graph *g, int k, const T *in, T *out) {
         const int32_t e = r < runs.size() ? runs[r].first : depth;
         ProfScope ps(g, BWD ? SS_KERNEL_TRI_BWD : SS_KERNEL_TRI_FWD, e - l);
         for (; l < e; l++)
-            launch_trisolve_level<T, BWD>(s, lp[l], nlong[l], lp[l + 1], waves[l], rpw, g->ctx->tune.xcd_slices, rows,
-                                          ptr, col, val, g->diag, k, in, out);
+            launch_trisolve_level<T, BWD>(s, lp[l], nlong[l], nmed[l], lp[l + 1], waves[l], rpw,
+                                          g->ctx->tune.xcd_slices, rows, ptr, col, val, g->diag, k, in, out);
     }
 }
 

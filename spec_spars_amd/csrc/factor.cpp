@@ -421,11 +421,16 @@ int factor_schedules(const Factor &F, const std::vector<char> &core, Schedule &f
         slot.assign((size_t)S.level_ptr[S.depth], 0);
         for (int64_t i = 0; i < n; i++)
             if (!core[i]) slot[pos[dense[lvl[i]]]++] = (int32_t)i;
+        // then the rows taking one wave each, then the tiny rows
         S.level_long.assign((size_t)S.depth, 0);
+        S.level_med.assign((size_t)S.depth, 0);
         for (int32_t l = 0; l < S.depth; l++) {
             auto b = slot.begin() + S.level_ptr[l], e = slot.begin() + S.level_ptr[l + 1];
-            auto mid = std::stable_partition(b, e, [&](int32_t p) { return len[p] > long_row; });
+            const int32_t wave_max = e - b >= kWaveLevelRows ? std::max(long_row, (int32_t)kWaveRowMax) : long_row;
+            auto mid = std::stable_partition(b, e, [&](int32_t p) { return len[p] > wave_max; });
+            auto tiny = std::stable_partition(mid, e, [&](int32_t p) { return len[p] > kTinyRow; });
             S.level_long[l] = (int32_t)(mid - b);
+            S.level_med[l] = (int32_t)(tiny - mid);
         }
     };
     std::vector<int32_t> fslot, bslot;

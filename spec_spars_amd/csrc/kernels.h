@@ -78,12 +78,13 @@ void launch_copy_f64(hipStream_t s, int64_t count, const double *a, double *b);
 // Upper bound of the block count of every kernel that writes per-block
 // partials (partial buffers hold kMaxPartialBlocks * k doubles).
 static const int kMaxPartialBlocks = 4096;
-// Rows of one level, positions [t0, t1); the first nlong of them are rows
-// of more than kLongRow entries (a block of `waves` wavefronts each: 4, 8 or
-// 16), the rest go rpw to a wave.
-// xcd != 0: each of the 8 XCDs takes one contiguous slice of the short rows.
+// Rows of one level, positions [t0, t1): the first nlong of them take a
+// block of `waves` wavefronts each (4, 8 or 16), the next nmed one wave
+// each, the rest (at most kTinyRow entries each) go rpw to a wave.
+// xcd != 0: each of the 8 XCDs takes one contiguous slice of the wave rows.
 template <typename T, bool BWD>
-void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nlong, int32_t t1, int waves, int rpw, int xcd,
+void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nlong, int32_t nmed, int32_t t1, int waves, int rpw,
+                           int xcd,
                            const int32_t *rows, const int32_t *ptr, const int32_t *col, const T *val,
                            const double *diag, int k, const T *in, T *out);
 template <typename T>

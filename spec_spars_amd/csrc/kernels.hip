@@ -534,13 +534,133 @@ __device__ __forceinline__ T seg_apply_shared(T acc, int a, int b, bool act, int
     return acc;
 }
 
-// One level of a triangular solve: schedule positions [t0, t1), the first
-// nlong of them rows of more than kLongRow entries (a block each, its W =
-// blockDim / 64 waves sharing the row's parts), the rest rpw consecutive
-// rows per wave.
+// Lane l's value of v (l uniform; the lane need not be active).
+__device__ __forceinline__ int lane_get(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ float lane_get(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double lane_get(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Parts of a row whose entries sit one per lane (entry e in lane e & 63 of
+// c0/v0 below 64, of c1/v1 from 64): gathered 16 at a time, entry e into
+// part e % P.  Lanes past the row's end hold zero values, and their terms
+// (zero times zero) leave a part exact.
+template <typename T, int P>
+__device__ __forceinline__ T wave_parts(int len, int c0, int c1, T v0, T v1, const T *X, int k, int jj) {
+    T part[P];
+#pragma unroll
+    for (int u = 0; u < P; u++) part[u] = (T)0;
+    for (int e0 = 0; e0 < len; e0 += 16) {
+        const bool hi = e0 >= 64;
+        const int cs = hi ? c1 : c0;
+        const T vs = hi ? v1 : v0;
+        T x[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int c = lane_get(cs, (e0 + u) & 63);
+            x[u] = e0 + u < len ? X[(size_t)c * k + jj] : (T)0;
+        }
+#pragma unroll
+        for (int u = 0; u < 16; u++) part[u % P] = fnms(lane_get(vs, (e0 + u) & 63), x[u], part[u % P]);
+    }
+    tree_fold<T, P>(part);
+    return part[0];
+}
+
+// One row of at most kWaveRowMax entries on one wave: the row's entries are
+// loaded one per lane, so a single dependent load separates the row record
+// from all of its gathers.
 template <typename T, bool BWD>
-__global__ __launch_bounds__(64 * kLevelWaves) void k_trisolve_level(int32_t t0, int32_t nlong, int32_t t1, int rpw,
-                                                                     int xcd_slices,
+__device__ __forceinline__ void wave_row(int t, const int32_t *__restrict__ rows, const int32_t *__restrict__ ptr,
+                                         const int32_t *__restrict__ col, const T *__restrict__ val,
+                                         const double *__restrict__ diag, int k, const T *__restrict__ in, T *out,
+                                         int lane) {
+    const int i = wave_uniform(rows[t]);
+    const int q0 = wave_uniform(ptr[t]), len = wave_uniform(ptr[t + 1]) - q0;
+    const int c0 = lane < len ? col[q0 + lane] : 0;
+    const T v0 = lane < len ? val[q0 + lane] : (T)0;
+    const int c1 = 64 + lane < len ? col[q0 + 64 + lane] : 0;
+    const T v1 = 64 + lane < len ? val[q0 + 64 + lane] : (T)0;
+    const double d = BWD ? diag[i] : 1.0;
+    const int P = seg_parts(len);
+    for (int jb = 0; jb < k; jb += 64) {
+        const int j = jb + lane, jj = j < k ? j : k - 1;
+        T acc = row_start<T, BWD>(i, jj, k, d, in);
+        if (P == 1) {
+            for (int e = 0; e < len; e++) acc = fnms(lane_get(v0, e), out[(size_t)lane_get(c0, e) * k + jj], acc);
+        } else if (P == 4) {
+            acc = acc + wave_parts<T, 4>(len, c0, c1, v0, v1, out, k, jj);
+        } else {
+            acc = acc + wave_parts<T, 8>(len, c0, c1, v0, v1, out, k, jj);
+        }
+        if (j < k) out[(size_t)i * k + j] = acc;
+    }
+}
+
+// Up to kTinyGroup consecutive rows of at most kTinyRow entries each on one
+// wave (positions [tb, tb + cnt)): their row records, then all of their
+// entries (contiguous, at most 32) load one per lane, every gather of the
+// group is in flight before the first row finishes, and the stores come last.
+template <typename T, bool BWD>
+__device__ __forceinline__ void wave_tiny_rows(int tb, int cnt, const int32_t *__restrict__ rows,
+                                               const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
+                                               const T *__restrict__ val, const double *__restrict__ diag, int k,
+                                               const T *__restrict__ in, T *out, int lane) {
+    constexpr int R = kTinyGroup, E = kTinyRow;
+    const int li = lane < cnt ? rows[tb + lane] : 0;
+    const int lp = lane <= cnt ? ptr[tb + lane] : 0;
+    const int base = lane_get(lp, 0);
+    const int ne = lane_get(lp, cnt) - base;   // <= R * E
+    const int lc = lane < ne ? col[base + lane] : 0;
+    const T lv = lane < ne ? val[base + lane] : (T)0;
+    const double ld = (BWD && lane < cnt) ? diag[li] : 1.0;
+    int ir[R], ar[R], nr[R];
+    double dr[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const bool on = r < cnt;
+        ir[r] = on ? lane_get(li, r) : 0;
+        ar[r] = on ? lane_get(lp, r) - base : 0;
+        nr[r] = on ? lane_get(lp, r + 1) - lane_get(lp, r) : 0;
+        dr[r] = on ? lane_get(ld, r) : 1.0;
+    }
+    for (int jb = 0; jb < k; jb += 64) {
+        const int j = jb + lane, jj = j < k ? j : k - 1;
+        T st[R], x[R][E];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            st[r] = r < cnt ? row_start<T, BWD>(ir[r], jj, k, dr[r], in) : (T)0;
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const int c = lane_get(lc, (ar[r] + e) & 63);
+                x[r][e] = e < nr[r] ? out[(size_t)c * k + jj] : (T)0;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            T acc = st[r];
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const T v = e < nr[r] ? lane_get(lv, (ar[r] + e) & 63) : (T)0;
+                acc = fnms(v, x[r][e], acc);
+            }
+            if (r < cnt && j < k) out[(size_t)ir[r] * k + j] = acc;
+        }
+    }
+}
+
+// One level of a triangular solve: schedule positions [t0, t1); the first
+// nlong of them take a block each (its W = blockDim / 64 waves share the
+// row's parts), the next nmed one wave each, the rest (tiny rows) rpw
+// consecutive rows per wave.
+template <typename T, bool BWD>
+__global__ __launch_bounds__(64 * kLevelWaves) void k_trisolve_level(int32_t t0, int32_t nlong, int32_t nmed,
+                                                                     int32_t t1, int rpw, int xcd_slices,
                                                                      const int32_t *__restrict__ rows,
                                                                      const int32_t *__restrict__ ptr,
                                                                      const int32_t *__restrict__ col,
@@ -562,25 +682,26 @@ __global__ __launch_bounds__(64 * kLevelWaves) void k_trisolve_level(int32_t t0,
         }
         return;
     }
-    // rpw consecutive rows per wave.  Workgroups go to the 8 XCDs round
-    // robin; block b works on chunk (b % 8) * per + b / 8, so each XCD owns
-    // one contiguous slice of the level's rows (with rows in elimination
-    // order: one region of the graph, the same region at every level)
+    // wave work: nmed single rows, then groups of rpw tiny rows.  Workgroups
+    // go to the 8 XCDs round robin; with xcd_slices block b works on chunk
+    // (b % 8) * per + b / 8, so each XCD owns one contiguous slice of the
+    // level's rows (with rows in elimination order: one region of the
+    // graph, the same region at every level)
     const int nb = (int)gridDim.x - nlong, b = (int)blockIdx.x - nlong;
     const int per = nb / kXcds;
     const int chunk = xcd_slices ? (b % kXcds) * per + b / kXcds : b;
-    const int tb = wave_uniform(t0 + nlong + (chunk * W + wv) * rpw);
-    const int te = tb + rpw < t1 ? tb + rpw : t1;
-    for (int t = tb; t < te; t++) {
-        const int i = wave_uniform(rows[t]);
-        const int q0 = wave_uniform(ptr[t]), q1 = wave_uniform(ptr[t + 1]);
-        const double d = BWD ? diag[i] : 1.0;
-        for (int jb = 0; jb < k; jb += 64) {
-            const int j = jb + lane;
-            if (j >= k) break;
-            out[(size_t)i * k + j] =
-                seg_apply<T, kLongRow>(row_start<T, BWD>(i, j, k, d, in), q0, q1, col, val, out, k, j);
-        }
+    const int w = wave_uniform(chunk * W + wv);
+    if (w < nmed) {
+        wave_row<T, BWD>(t0 + nlong + w, rows, ptr, col, val, diag, k, in, out, lane);
+        return;
+    }
+    const int tb = t0 + nlong + nmed + (w - nmed) * rpw;
+    for (int g0 = 0; g0 < rpw; g0 += kTinyGroup) {
+        const int t = tb + g0;
+        if (t >= t1) break;
+        int cnt = rpw - g0 < kTinyGroup ? rpw - g0 : kTinyGroup;
+        if (t + cnt > t1) cnt = t1 - t;
+        wave_tiny_rows<T, BWD>(t, cnt, rows, ptr, col, val, diag, k, in, out, lane);
     }
 }
 
@@ -617,26 +738,58 @@ __global__ __launch_bounds__(256) void k_core_bwd_init(int32_t count, const int3
     }
 }
 
+// Substitution inside one dense-core block by a single wave: rows [0, nr)
+// of sa (this lane's column) updated by the strictly triangular block sb
+// (sb[r][a] = entry of row r, column a).  All 64 rows stay in registers;
+// column by column, once row a is final every later row (earlier row
+// backward) takes its term, so each row subtracts its columns in order.
+// Rows past nr start at zero and their entries of sb are zero, so the
+// terms they add to a real row are exact zeros.
+// Rows of a dense-core block held by each of k_core_step's 4 waves.
+constexpr int kCoreRows = kCoreBlock / 4;
+
+// Substitution inside one wave's rows of a dense-core block, in registers:
+// acc[u] is row base + u; sd[r][a] is the block entry of row r, column a.
+// Column by column, once row a is final every later row (earlier row
+// backward) of the wave takes its term.
+template <typename T, bool BWD>
+__device__ __forceinline__ void core_sub_solve(T (&acc)[kCoreRows], const T (*sd)[kCoreBlock], int base) {
+#pragma unroll
+    for (int s = 0; s < kCoreRows; s++) {
+        const int a = BWD ? kCoreRows - 1 - s : s;
+        const T v = acc[a];
+#pragma unroll
+        for (int u = 0; u < kCoreRows; u++)
+            if (BWD ? u < a : u > a) acc[u] = fnms(sd[base + u][base + a], v, acc[u]);
+    }
+}
+
 // One step of the blocked substitution over every graph's dense core
 // (c rows in elimination order, D = L row-major c x c strictly lower; the
 // backward solve reads D = L^T).  Forward step s finishes block s of each
 // graph: workgroup x == 0 applies block s-1 to the rows of block s, then
 // substitutes inside block s; workgroups x >= 1 apply block s-1 to the rows
 // of the later blocks, 64 rows each.  Backward runs the blocks from the
-// last.  Every core row is one running fused multiply-subtract chain over
-// its core columns -- ascending forward, descending backward, explicit
-// zeros included -- so its bits depend on its graph alone.
+// last.  Each of the 4 waves keeps 16 rows in registers.  Inside block s
+// the waves finish their rows in turn (a barrier each): the wave whose turn
+// it is substitutes among its own rows, then every wave still waiting takes
+// the finished rows' terms.  Every core row is one running fused
+// multiply-subtract chain over its core columns -- ascending forward,
+// descending backward, explicit zeros included -- so its bits depend on its
+// graph alone.  Rows past the core's end start at zero and have zero
+// entries, so the terms they add to real rows are exact zeros.
 template <typename T, bool BWD>
-__global__ __launch_bounds__(1024) void k_core_step(int s, const int32_t *__restrict__ core_n,
+__global__ __launch_bounds__(256) void k_core_step(int s, const int32_t *__restrict__ core_n,
                                                     const int32_t *__restrict__ core_off,
                                                     const int64_t *__restrict__ dense_off,
                                                     const int32_t *__restrict__ vert, const T *__restrict__ D, int k,
                                                     T *out) {
-    constexpr int B = kCoreBlock;
-    __shared__ T sm_a[B][64];   // block rows being finished (column chunk)
-    __shared__ T sm_b[B][64];   // the block applied: finished rows, then the diagonal block of D
+    constexpr int B = kCoreBlock, RW = kCoreRows;
+    static_assert(RW * 4 == B, "4 waves cover a block");
+    __shared__ T sm_z[B][64];   // finished values (column chunk): the previous block, then this block's rows
+    __shared__ T sm_d[B][B];    // entries of D: this workgroup's rows x the previous block, then the diagonal block
     const int g = blockIdx.y, x = blockIdx.x;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int c = core_n[g];
     const int nb = (c + B - 1) / B;
     const int blk = BWD ? nb - 1 - s : s;       // block finished by this step
@@ -647,6 +800,7 @@ __global__ __launch_bounds__(1024) void k_core_step(int s, const int32_t *__rest
     const T *Dg = D + dense_off[g];
     const int b0 = blk * B, b1 = b0 + B < c ? b0 + B : c;
     const int p0 = prev * B, p1 = p0 + B < c ? p0 + B : c;
+    const int np = has_prev ? p1 - p0 : 0;
     // rows this workgroup updates with the previous block
     int r0, r1;
     if (x == 0) {
@@ -660,67 +814,59 @@ __global__ __launch_bounds__(1024) void k_core_step(int s, const int32_t *__rest
         r1 = r0 + B < c ? r0 + B : c;
     }
     if (r0 >= r1) return;   // uniform over the workgroup
-    constexpr int RPW = B / 16;   // block rows per wave in the substitution (16 waves)
+    const int nr = r1 - r0;
     for (int jb = 0; jb < k; jb += 64) {
         const int j = jb + lane;
-        // finished values of the previous block
-        if (has_prev)
-            for (int r = wv; r < p1 - p0; r += nw) sm_b[r][lane] = j < k ? out[(size_t)vg[p0 + r] * k + j] : (T)0;
+        for (int r = wv; r < np; r += 4) sm_z[r][lane] = j < k ? out[(size_t)vg[p0 + r] * k + j] : (T)0;
+        for (int e = threadIdx.x; e < B * B; e += blockDim.x) {
+            const int a = e / B, b = e % B;
+            sm_d[a][b] = (a < nr && b < np) ? Dg[(size_t)(r0 + a) * c + p0 + b] : (T)0;
+        }
         __syncthreads();
-        // apply it to rows [r0, r1): ascending j forward, descending backward
-        for (int r = r0 + wv; r < r1; r += nw) {
-            const int i = vg[r];
-            T acc = j < k ? out[(size_t)i * k + j] : (T)0;
-            if (has_prev) {
-                const T *Drow = Dg + (size_t)r * c;
-                if (BWD) {
-                    for (int q = p1 - 1; q >= p0; q--) acc = fnms(Drow[q], sm_b[q - p0][lane], acc);
-                } else {
-                    for (int q = p0; q < p1; q++) acc = fnms(Drow[q], sm_b[q - p0][lane], acc);
-                }
-            }
-            if (x == 0)
-                sm_a[r - r0][lane] = acc;
-            else if (j < k)
-                out[(size_t)i * k + j] = acc;
+        T acc[RW];
+#pragma unroll
+        for (int u = 0; u < RW; u++) {
+            const int r = wv * RW + u;
+            acc[u] = (r < nr && j < k) ? out[(size_t)vg[r0 + r] * k + j] : (T)0;
+        }
+        // the previous block: ascending columns forward, descending backward
+        for (int qq = 0; qq < np; qq++) {
+            const int q = BWD ? np - 1 - qq : qq;
+            const T v = sm_z[q][lane];
+#pragma unroll
+            for (int u = 0; u < RW; u++) acc[u] = fnms(sm_d[wv * RW + u][q], v, acc[u]);
         }
         if (x == 0) {
-            __syncthreads();
-            // the diagonal block of D into sm_b
+            __syncthreads();   // the panel's reads of sm_z and sm_d are done
             for (int e = threadIdx.x; e < B * B; e += blockDim.x) {
                 const int a = e / B, b = e % B;
-                sm_b[a][b] = (a < b1 - b0 && b < b1 - b0) ? Dg[(size_t)(b0 + a) * c + b0 + b] : (T)0;
+                sm_d[a][b] = (a < nr && b < nr) ? Dg[(size_t)(b0 + a) * c + b0 + b] : (T)0;
             }
             __syncthreads();
-            // substitution, column by column: wave w keeps rows w, w+16, ...
-            // in registers; once row a is final every later row takes its
-            // term, so each row still subtracts its columns in order
-            const int nr = b1 - b0;
-            T acc[RPW];
+            for (int st = 0; st < 4; st++) {
+                const int w = BWD ? 3 - st : st;   // the wave whose rows finish now
+                if (wv == w) {
+                    core_sub_solve<T, BWD>(acc, sm_d, w * RW);
 #pragma unroll
-            for (int u = 0; u < RPW; u++) acc[u] = wv + 16 * u < nr ? sm_a[wv + 16 * u][lane] : (T)0;
-            for (int step = 0; step < nr; step++) {
-                const int a = BWD ? nr - 1 - step : step;
-                if ((a & 15) == wv) {
-                    T v = acc[0];
-#pragma unroll
-                    for (int u = 1; u < RPW; u++)
-                        if ((a >> 4) == u) v = acc[u];
-                    sm_a[a][lane] = v;
+                    for (int u = 0; u < RW; u++) sm_z[w * RW + u][lane] = acc[u];
                 }
                 __syncthreads();
-                const T ya = sm_a[a][lane];
+                if (BWD ? wv < w : wv > w) {
+                    for (int aa = 0; aa < RW; aa++) {
+                        const int a = w * RW + (BWD ? RW - 1 - aa : aa);
+                        const T v = sm_z[a][lane];
 #pragma unroll
-                for (int u = 0; u < RPW; u++) {
-                    const int r = wv + 16 * u;
-                    if (r < nr && (BWD ? r < a : r > a)) acc[u] = fnms(sm_b[r][a], ya, acc[u]);
+                        for (int u = 0; u < RW; u++) acc[u] = fnms(sm_d[wv * RW + u][a], v, acc[u]);
+                    }
                 }
             }
-            __syncthreads();
-            for (int r = wv; r < nr; r += nw)
-                if (j < k) out[(size_t)vg[b0 + r] * k + j] = sm_a[r][lane];
         }
-        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < RW; u++) {
+            const int r = wv * RW + u;
+            if (r < nr && j < k) out[(size_t)vg[r0 + r] * k + j] = acc[u];
+        }
+        __syncthreads();   // the next column chunk reuses the shared arrays
     }
 }
 
@@ -1067,7 +1213,7 @@ void launch_core_step(hipStream_t s, int step, int graphs, int max_rows, const i
                       T *out) {
     if (graphs <= 0 || max_rows <= 0) return;
     const unsigned gx = 1 + (unsigned)((max_rows + kCoreBlock - 1) / kCoreBlock);
-    hipLaunchKernelGGL((k_core_step<T, BWD>), dim3(gx, graphs), dim3(1024), 0, s, step, core_n, core_off, dense_off,
+    hipLaunchKernelGGL((k_core_step<T, BWD>), dim3(gx, graphs), dim3(256), 0, s, step, core_n, core_off, dense_off,
                        vert, D, k, out);
 }
 #define SS_INST_CORE(T, B)                                                                                    \
@@ -1107,25 +1253,25 @@ template void launch_update_xr_rr<float>(hipStream_t, int32_t, int, const double
                                          float *, float *, double *);
 
 template <typename T, bool BWD>
-void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nlong, int32_t t1, int waves, int rpw, int xcd,
-                           const int32_t *rows, const int32_t *ptr, const int32_t *col, const T *val,
+void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nlong, int32_t nmed, int32_t t1, int waves, int rpw,
+                           int xcd, const int32_t *rows, const int32_t *ptr, const int32_t *col, const T *val,
                            const double *diag, int k, const T *in, T *out) {
     if (t1 <= t0) return;
     if (rpw < 1) rpw = 1;
     if (waves < 4 || waves > kLevelWaves || (waves & (waves - 1))) waves = 4;
-    const int64_t short_rows = (int64_t)t1 - t0 - nlong;
-    const int64_t per_block = (int64_t)waves * rpw;
-    int64_t blocks = (short_rows + per_block - 1) / per_block;
+    const int64_t tiny_rows = (int64_t)t1 - t0 - nlong - nmed;
+    const int64_t wave_work = (int64_t)nmed + (tiny_rows + rpw - 1) / rpw;
+    int64_t blocks = (wave_work + waves - 1) / waves;
     // XCD slices: a whole number of blocks per XCD (blocks past the rows
     // return at once), only when every XCD gets several
     const int xcd_slices = xcd && blocks >= 4 * kXcds;
     if (xcd_slices) blocks = (blocks + kXcds - 1) / kXcds * kXcds;
     const unsigned grid = (unsigned)nlong + (unsigned)blocks;
-    hipLaunchKernelGGL((k_trisolve_level<T, BWD>), dim3(grid), dim3(64 * waves), 0, s, t0, nlong, t1, rpw, xcd_slices,
-                       rows, ptr, col, val, diag, k, in, out);
+    hipLaunchKernelGGL((k_trisolve_level<T, BWD>), dim3(grid), dim3(64 * waves), 0, s, t0, nlong, nmed, t1, rpw,
+                       xcd_slices, rows, ptr, col, val, diag, k, in, out);
 }
 #define SS_INST_TRI(T, B)                                                                                      \
-    template void launch_trisolve_level<T, B>(hipStream_t, int32_t, int32_t, int32_t, int, int, int,            \
+    template void launch_trisolve_level<T, B>(hipStream_t, int32_t, int32_t, int32_t, int32_t, int, int, int,   \
                                               const int32_t *,                                                   \
                                               const int32_t *, const int32_t *, const T *, const double *, int,  \
                                               const T *, T *);

@@ -71,6 +71,7 @@ struct Schedule {
     std::vector<int32_t> rows;        // vertex finished at position t
     std::vector<int32_t> level_ptr;   // positions of level l: [level_ptr[l], level_ptr[l+1])
     std::vector<int32_t> level_long;  // rows of level l with more than long_row entries (placed first)
+    std::vector<int32_t> level_med;   // then rows taking one wave each (more than kTinyRow entries); tiny rows last
     // Runs of consecutive small levels (at least two levels of at most
     // small_rows rows each), each walked by one workgroup: [first, second).
     std::vector<std::pair<int32_t, int32_t>> runs;

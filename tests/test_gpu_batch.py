@@ -102,6 +102,50 @@ def test_batch_alone_vs_mixed_bit_identical(ctx, prec):
         assert np.array_equal(rev[j]["Z"], mixed[i]["Z"]) and np.array_equal(rev[j]["p"], mixed[i]["p"])
 
 
+def _hubs(nh=100, seed=7):
+    """Hubs on a ring, each with 35..119 leaves: the leaves go in the first
+    elimination round, so each hub's forward row holds 35..120 entries in
+    the second and third levels."""
+    from spec_spars_amd import graphs
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(35, 120, nh)
+    u, v = [], []
+    leaf = nh
+    for h in range(nh):
+        u.append(h)
+        v.append((h + 1) % nh)
+        for _ in range(sizes[h]):
+            u.append(h)
+            v.append(leaf)
+            leaf += 1
+    u, v, _ = graphs.canonical(np.array(u, np.int32), np.array(v, np.int32))
+    return leaf, u, v, graphs.uniform_weights(len(u), seed=seed)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_wave_rows_and_blocks_bit_identical(ctx, prec):
+    """A hub row of 35..120 entries is finished by a block of waves when its
+    level is small (the hub graph alone) and by one wave when a large grid
+    shares the level (the mix); k = 72 takes two column chunks.  The bits
+    agree, and every graph's residual meets the tolerance."""
+    import spec_spars_amd as ss
+    from spec_spars_amd import graphs
+    n, u, v = graphs.grid2d(200, 200)
+    gs = [(n, u, v, graphs.loguniform_weights(len(u), seed=3)), _hubs()]
+    fs = _factors(gs, seed=9, ordering=ss.ORDER_ROUNDS)
+    k = 72
+    mixed = _run(ctx, gs, fs, [0, 1], k, prec)
+    for i in range(2):
+        alone = _run(ctx, gs, fs, [i], k, prec)[0]
+        for key in ("Y", "Z", "p"):
+            assert np.array_equal(alone[key], mixed[i][key]), (i, key)
+        n, u, v, w = gs[i]
+        L = _laplacian(n, u, v, w)
+        Yp = _component_projection(n, u, v, mixed[i]["Y"])
+        rr = np.linalg.norm(Yp - L @ mixed[i]["Z"], axis=0) / np.linalg.norm(Yp, axis=0)
+        assert rr.max() <= 1.01e-8, i
+
+
 def test_batch_matches_oracle_per_graph(ctx, oracle_mod):
     import spec_spars_amd as ss
     from spec_spars_amd import graphs
