@@ -274,25 +274,86 @@ __global__ __launch_bounds__(1024) void kb_update_p(Tiles rt, int k, const doubl
     }
 }
 
+// Lane l's value of v (l uniform; the lane need not be active).
+__device__ __forceinline__ int bt_lane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ float bt_lane(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double bt_lane(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Q = A P with per-tile partials of P . Q.  Wave w of the tile takes rows
+// r0 + w, r0 + w + NW, ... in order, kSpmmGroup of them at a time: when no
+// row of the group has more than kSpmmShort entries, each row's entries load
+// one per lane and every gather of the group is in flight together; longer
+// rows go one at a time.  Every row is one running subtraction from zero
+// over its entries in order and the wave's dot takes its rows in order, so
+// the grouping does not change a bit.
+constexpr int kSpmmGroup = 4;
+constexpr int kSpmmShort = 8;
 template <typename T>
 __global__ __launch_bounds__(1024) void kb_spmm_dot(Tiles rt, const int32_t *__restrict__ rowptr,
                                                    const int32_t *__restrict__ col, const T *__restrict__ val, int k,
                                                    const T *__restrict__ P, T *__restrict__ Q,
                                                    double *__restrict__ partial) {
+    constexpr int G = kSpmmGroup, E = kSpmmShort;
     __shared__ double sm[kMaxTileWaves][64];
-    const int t = blockIdx.x;
+    const int t = blockIdx.x, NW = BT_NW, lane = BT_LANE;
     const int r0 = rt.start[t], r1 = rt.start[t + 1];
     for (int jb = 0; jb < k; jb += 64) {
-        const int j = jb + BT_LANE;
+        const int j = jb + lane, jj = j < k ? j : k - 1;
         double acc = 0.0;
-        for (int row = r0 + BT_WAVE; row < r1; row += BT_NW) {
-            const int r = bt_uniform(row);
-            const int q0 = rowptr[r], q1 = rowptr[r + 1];
-            if (j < k) {
-                const T qv = -bt_dot_sub<T>((T)0, q0, q1, col, val, P, k, j);
-                const size_t o = (size_t)r * k + j;
-                Q[o] = qv;
-                acc = bt_fma((double)P[o], (double)qv, acc);
+        for (int rb = bt_uniform(r0 + BT_WAVE); rb < r1; rb += G * NW) {
+            int rr[G], q0[G], len[G];
+            int maxlen = 0;
+#pragma unroll
+            for (int u = 0; u < G; u++) {
+                const int r = rb + u * NW;
+                const bool on = r < r1;
+                rr[u] = on ? r : rb;
+                q0[u] = on ? rowptr[r] : 0;
+                len[u] = on ? rowptr[r + 1] - q0[u] : 0;
+                maxlen = len[u] > maxlen ? len[u] : maxlen;
+            }
+            if (maxlen <= E) {
+                int c[G];
+                T v[G], own[G], x[G][E];
+#pragma unroll
+                for (int u = 0; u < G; u++) {
+                    c[u] = lane < len[u] ? col[q0[u] + lane] : 0;
+                    v[u] = lane < len[u] ? val[q0[u] + lane] : (T)0;
+                }
+#pragma unroll
+                for (int u = 0; u < G; u++) {
+                    own[u] = P[(size_t)rr[u] * k + jj];
+#pragma unroll
+                    for (int e = 0; e < E; e++) x[u][e] = e < len[u] ? P[(size_t)bt_lane(c[u], e) * k + jj] : (T)0;
+                }
+#pragma unroll
+                for (int u = 0; u < G; u++) {
+                    T a = (T)0;
+#pragma unroll
+                    for (int e = 0; e < E; e++) a = bt_fma(-bt_lane(v[u], e), x[u][e], a);
+                    if (rb + u * NW < r1 && j < k) {
+                        Q[(size_t)rr[u] * k + j] = -a;
+                        acc = bt_fma((double)own[u], (double)(-a), acc);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < G; u++) {
+                    if (rb + u * NW >= r1) break;
+                    if (j < k) {
+                        const T qv = -bt_dot_sub<T>((T)0, q0[u], q0[u] + len[u], col, val, P, k, j);
+                        const size_t o = (size_t)rr[u] * k + j;
+                        Q[o] = qv;
+                        acc = bt_fma((double)P[o], (double)qv, acc);
+                    }
+                }
             }
         }
         bt_tile_store(acc, k, jb, partial, sm);

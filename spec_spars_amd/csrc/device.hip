@@ -421,13 +421,15 @@ int batch_upload_factors(ss_graph *g, const std::vector<const Factor *> &factors
     const int64_t n = g->N;
     const int64_t fn = fw.n, bn = bw.n;
     const int64_t fnz = (int64_t)fw.col.size(), bnz = (int64_t)bw.col.size();
-    SS_TRY(A.alloc(&g->f_rows, fn));
-    SS_TRY(A.alloc(&g->f_ptr, fn + 1));
+    // the row and pointer arrays carry kTinyGroup entries of padding (a
+    // group of tiny rows reads its records without bounds checks)
+    SS_TRY(A.alloc(&g->f_rows, fn + kTinyGroup));
+    SS_TRY(A.alloc(&g->f_ptr, fn + 1 + kTinyGroup));
     SS_TRY(A.alloc(&g->f_col, fnz));
     SS_TRY(A.alloc(&g->f_val, fnz));
     SS_TRY(A.alloc(&g->f_valf, fnz));
-    SS_TRY(A.alloc(&g->b_rows, bn));
-    SS_TRY(A.alloc(&g->b_ptr, bn + 1));
+    SS_TRY(A.alloc(&g->b_rows, bn + kTinyGroup));
+    SS_TRY(A.alloc(&g->b_ptr, bn + 1 + kTinyGroup));
     SS_TRY(A.alloc(&g->b_col, bnz));
     SS_TRY(A.alloc(&g->b_val, bnz));
     SS_TRY(A.alloc(&g->b_valf, bnz));
@@ -445,13 +447,18 @@ int batch_upload_factors(ss_graph *g, const std::vector<const Factor *> &factors
     hipStream_t s = g->ctx->stream;
     std::vector<float> fvf(fw.val.begin(), fw.val.end()), bvf(bw.val.begin(), bw.val.end());
     std::vector<float> Lf(Ld.begin(), Ld.end()), Ltf(Ltd.begin(), Ltd.end());
-    SS_TRY(h2d(s, g->f_rows, fw.rows.data(), fn));
-    SS_TRY(h2d(s, g->f_ptr, fw.ptr.data(), fn + 1));
+    std::vector<int32_t> frows(fw.rows), fptr(fw.ptr), brows(bw.rows), bptr(bw.ptr);
+    frows.resize((size_t)fn + kTinyGroup, 0);
+    fptr.resize((size_t)fn + 1 + kTinyGroup, fw.ptr[fn]);
+    brows.resize((size_t)bn + kTinyGroup, 0);
+    bptr.resize((size_t)bn + 1 + kTinyGroup, bw.ptr[bn]);
+    SS_TRY(h2d(s, g->f_rows, frows.data(), fn + kTinyGroup));
+    SS_TRY(h2d(s, g->f_ptr, fptr.data(), fn + 1 + kTinyGroup));
     SS_TRY(h2d(s, g->f_col, fw.col.data(), fnz));
     SS_TRY(h2d(s, g->f_val, fw.val.data(), fnz));
     SS_TRY(h2d(s, g->f_valf, fvf.data(), fnz));
-    SS_TRY(h2d(s, g->b_rows, bw.rows.data(), bn));
-    SS_TRY(h2d(s, g->b_ptr, bw.ptr.data(), bn + 1));
+    SS_TRY(h2d(s, g->b_rows, brows.data(), bn + kTinyGroup));
+    SS_TRY(h2d(s, g->b_ptr, bptr.data(), bn + 1 + kTinyGroup));
     SS_TRY(h2d(s, g->b_col, bw.col.data(), bnz));
     SS_TRY(h2d(s, g->b_val, bw.val.data(), bnz));
     SS_TRY(h2d(s, g->b_valf, bvf.data(), bnz));

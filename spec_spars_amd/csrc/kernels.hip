@@ -602,33 +602,49 @@ __device__ __forceinline__ void wave_row(int t, const int32_t *__restrict__ rows
     }
 }
 
-// Up to kTinyGroup consecutive rows of at most kTinyRow entries each on one
-// wave (positions [tb, tb + cnt)): their row records, then all of their
-// entries (contiguous, at most 32) load one per lane, every gather of the
-// group is in flight before the first row finishes, and the stores come last.
-template <typename T, bool BWD>
+// Up to R <= kTinyGroup consecutive rows of at most kTinyRow entries each on
+// one wave (positions [tb, tb + cnt)): their row records load as scalars (the
+// schedule arrays carry kTinyGroup entries of padding), their entries
+// (contiguous, at most 32) one per lane, every gather of the group is in
+// flight before the first row finishes, and the stores come last.
+template <typename T, bool BWD, int R>
 __device__ __forceinline__ void wave_tiny_rows(int tb, int cnt, const int32_t *__restrict__ rows,
                                                const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
                                                const T *__restrict__ val, const double *__restrict__ diag, int k,
                                                const T *__restrict__ in, T *out, int lane) {
-    constexpr int R = kTinyGroup, E = kTinyRow;
-    const int li = lane < cnt ? rows[tb + lane] : 0;
-    const int lp = lane <= cnt ? ptr[tb + lane] : 0;
-    const int base = lane_get(lp, 0);
-    const int ne = lane_get(lp, cnt) - base;   // <= R * E
-    const int lc = lane < ne ? col[base + lane] : 0;
-    const T lv = lane < ne ? val[base + lane] : (T)0;
-    const double ld = (BWD && lane < cnt) ? diag[li] : 1.0;
-    int ir[R], ar[R], nr[R];
+    static_assert(R >= 1 && R <= kTinyGroup, "at most kTinyGroup tiny rows at a time");
+    constexpr int E = kTinyRow;
+    int ir[R], pr[R + 1];
+#pragma unroll
+    for (int r = 0; r < R; r++) ir[r] = rows[tb + r];
+#pragma unroll
+    for (int r = 0; r <= R; r++) pr[r] = ptr[tb + r];
+    const int base = pr[0];
+    const int ne = ptr[tb + cnt] - base;   // <= R * E
+    int ar[R], nr[R];
     double dr[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
-        const bool on = r < cnt;
-        ir[r] = on ? lane_get(li, r) : 0;
-        ar[r] = on ? lane_get(lp, r) - base : 0;
-        nr[r] = on ? lane_get(lp, r + 1) - lane_get(lp, r) : 0;
-        dr[r] = on ? lane_get(ld, r) : 1.0;
+        ar[r] = pr[r] - base;
+        nr[r] = r < cnt ? pr[r + 1] - pr[r] : 0;
+        dr[r] = BWD ? diag[ir[r]] : 1.0;
     }
+    if (ne == 0) {
+        // no entries in the group: every row is its start value
+        for (int jb = 0; jb < k; jb += 64) {
+            const int j = jb + lane;
+            if (j >= k) break;
+            T st[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) st[r] = r < cnt ? row_start<T, BWD>(ir[r], j, k, dr[r], in) : (T)0;
+#pragma unroll
+            for (int r = 0; r < R; r++)
+                if (r < cnt) out[(size_t)ir[r] * k + j] = st[r];
+        }
+        return;
+    }
+    const int lc = lane < ne ? col[base + lane] : 0;
+    const T lv = lane < ne ? val[base + lane] : (T)0;
     for (int jb = 0; jb < k; jb += 64) {
         const int j = jb + lane, jj = j < k ? j : k - 1;
         T st[R], x[R][E];
@@ -657,8 +673,9 @@ __device__ __forceinline__ void wave_tiny_rows(int tb, int cnt, const int32_t *_
 // One level of a triangular solve: schedule positions [t0, t1); the first
 // nlong of them take a block each (its W = blockDim / 64 waves share the
 // row's parts), the next nmed one wave each, the rest (tiny rows) rpw
-// consecutive rows per wave.
-template <typename T, bool BWD>
+// consecutive rows per wave, TG at a time (the group size sets how many
+// gathers a wave holds in flight, and so its registers).
+template <typename T, bool BWD, int TG>
 __global__ __launch_bounds__(64 * kLevelWaves) void k_trisolve_level(int32_t t0, int32_t nlong, int32_t nmed,
                                                                      int32_t t1, int rpw, int xcd_slices,
                                                                      const int32_t *__restrict__ rows,
@@ -696,12 +713,12 @@ __global__ __launch_bounds__(64 * kLevelWaves) void k_trisolve_level(int32_t t0,
         return;
     }
     const int tb = t0 + nlong + nmed + (w - nmed) * rpw;
-    for (int g0 = 0; g0 < rpw; g0 += kTinyGroup) {
+    for (int g0 = 0; g0 < rpw; g0 += TG) {
         const int t = tb + g0;
         if (t >= t1) break;
-        int cnt = rpw - g0 < kTinyGroup ? rpw - g0 : kTinyGroup;
+        int cnt = rpw - g0 < TG ? rpw - g0 : TG;
         if (t + cnt > t1) cnt = t1 - t;
-        wave_tiny_rows<T, BWD>(t, cnt, rows, ptr, col, val, diag, k, in, out, lane);
+        wave_tiny_rows<T, BWD, TG>(t, cnt, rows, ptr, col, val, diag, k, in, out, lane);
     }
 }
 
@@ -1267,8 +1284,15 @@ void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nlong, int32_t nme
     const int xcd_slices = xcd && blocks >= 4 * kXcds;
     if (xcd_slices) blocks = (blocks + kXcds - 1) / kXcds * kXcds;
     const unsigned grid = (unsigned)nlong + (unsigned)blocks;
-    hipLaunchKernelGGL((k_trisolve_level<T, BWD>), dim3(grid), dim3(64 * waves), 0, s, t0, nlong, nmed, t1, rpw,
-                       xcd_slices, rows, ptr, col, val, diag, k, in, out);
+    if (rpw == 1)
+        hipLaunchKernelGGL((k_trisolve_level<T, BWD, 1>), dim3(grid), dim3(64 * waves), 0, s, t0, nlong, nmed, t1,
+                           rpw, xcd_slices, rows, ptr, col, val, diag, k, in, out);
+    else if (rpw == 2)
+        hipLaunchKernelGGL((k_trisolve_level<T, BWD, 2>), dim3(grid), dim3(64 * waves), 0, s, t0, nlong, nmed, t1,
+                           rpw, xcd_slices, rows, ptr, col, val, diag, k, in, out);
+    else
+        hipLaunchKernelGGL((k_trisolve_level<T, BWD, kTinyGroup>), dim3(grid), dim3(64 * waves), 0, s, t0, nlong,
+                           nmed, t1, rpw, xcd_slices, rows, ptr, col, val, diag, k, in, out);
 }
 #define SS_INST_TRI(T, B)                                                                                      \
     template void launch_trisolve_level<T, B>(hipStream_t, int32_t, int32_t, int32_t, int32_t, int, int, int,   \
