@@ -65,14 +65,18 @@ void bt_comp_means(hipStream_t s, int ncomp, int k, const int32_t *comp_graph, c
 template <typename TS, typename T>
 void bt_project(hipStream_t s, Tiles rt, int k, const TS *Y, const int32_t *comp_of_row, const double *cmean, T *R,
                 T *B, T *X);
+// P = Z + beta P, with the previous iteration's X += alpha P folded in.
 template <typename T>
-void bt_update_p(hipStream_t s, Tiles rt, int k, const double *beta, const T *Z, T *P);
+void bt_update_px(hipStream_t s, Tiles rt, int k, const double *alpha, const double *beta, const T *Z, T *P, T *X);
+// X += alpha P (the last iteration's solution update).
+template <typename T>
+void bt_update_x(hipStream_t s, Tiles rt, int k, const double *alpha, const T *P, T *X);
 template <typename T>
 void bt_spmm_dot(hipStream_t s, Tiles rt, const int32_t *rowptr, const int32_t *col, const T *val, int k, const T *P,
                  T *Q, double *partial);
+// R -= alpha Q, per-tile partials of |R|^2.
 template <typename T>
-void bt_update_xr_rr(hipStream_t s, Tiles rt, int k, const double *alpha, const T *P, const T *Q, T *X, T *R,
-                     double *partial);
+void bt_update_r_rr(hipStream_t s, Tiles rt, int k, const double *alpha, const T *Q, T *R, double *partial);
 void bt_residual64(hipStream_t s, Tiles rt, const int32_t *rowptr, const int32_t *col, const double *val, int k,
                    const double *X, const double *B, double *out, double *partial);
 void bt_convert_masked(hipStream_t s, Tiles rt, int k, const double *src, const int *keep, float *dst);

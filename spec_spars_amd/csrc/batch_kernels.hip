@@ -258,18 +258,48 @@ __global__ __launch_bounds__(1024) void kb_project(Tiles rt, int k, const TS *__
 // ---------------------------------------------------------------------------
 // PCG vector kernels (block per row tile, per-(graph, column) scalars)
 // ---------------------------------------------------------------------------
+// P = Z + beta P, and the previous iteration's X += alpha P with the old P
+// (the solution update rides along with the search-direction update, so
+// P is read once for both).  A column with alpha == 0 keeps X untouched
+// (on the first iteration P holds no direction yet).
 template <typename T>
-__global__ __launch_bounds__(1024) void kb_update_p(Tiles rt, int k, const double *__restrict__ beta,
-                                                   const T *__restrict__ Z, T *__restrict__ P) {
+__global__ __launch_bounds__(1024) void kb_update_px(Tiles rt, int k, const double *__restrict__ alpha,
+                                                    const double *__restrict__ beta, const T *__restrict__ Z,
+                                                    T *__restrict__ P, T *__restrict__ X) {
     const int t = blockIdx.x, g = rt.graph[t];
     const int r0 = rt.start[t], r1 = rt.start[t + 1];
     for (int jb = 0; jb < k; jb += 64) {
         const int j = jb + BT_LANE;
         if (j >= k) break;
+        const T a = (T)alpha[(size_t)g * k + j];
         const T b = (T)beta[(size_t)g * k + j];
         for (int r = r0 + BT_WAVE; r < r1; r += BT_NW) {
             const size_t o = (size_t)r * k + j;
-            P[o] = b == (T)0 ? Z[o] : bt_fma(b, P[o], Z[o]);
+            if (a != (T)0) {
+                const T p = P[o];
+                X[o] = bt_fma(a, p, X[o]);
+                P[o] = b == (T)0 ? Z[o] : bt_fma(b, p, Z[o]);
+            } else {
+                P[o] = b == (T)0 ? Z[o] : bt_fma(b, P[o], Z[o]);
+            }
+        }
+    }
+}
+
+// X += alpha P: the last iteration's solution update.
+template <typename T>
+__global__ __launch_bounds__(1024) void kb_update_x(Tiles rt, int k, const double *__restrict__ alpha,
+                                                   const T *__restrict__ P, T *__restrict__ X) {
+    const int t = blockIdx.x, g = rt.graph[t];
+    const int r0 = rt.start[t], r1 = rt.start[t + 1];
+    for (int jb = 0; jb < k; jb += 64) {
+        const int j = jb + BT_LANE;
+        if (j >= k) break;
+        const T a = (T)alpha[(size_t)g * k + j];
+        if (a == (T)0) continue;
+        for (int r = r0 + BT_WAVE; r < r1; r += BT_NW) {
+            const size_t o = (size_t)r * k + j;
+            X[o] = bt_fma(a, P[o], X[o]);
         }
     }
 }
@@ -360,11 +390,11 @@ __global__ __launch_bounds__(1024) void kb_spmm_dot(Tiles rt, const int32_t *__r
     }
 }
 
+// R -= alpha Q with per-tile partials of |R|^2.
 template <typename T>
-__global__ __launch_bounds__(1024) void kb_update_xr_rr(Tiles rt, int k, const double *__restrict__ alpha,
-                                                       const T *__restrict__ P, const T *__restrict__ Q,
-                                                       T *__restrict__ X, T *__restrict__ R,
-                                                       double *__restrict__ partial) {
+__global__ __launch_bounds__(1024) void kb_update_r_rr(Tiles rt, int k, const double *__restrict__ alpha,
+                                                      const T *__restrict__ Q, T *__restrict__ R,
+                                                      double *__restrict__ partial) {
     __shared__ double sm[kMaxTileWaves][64];
     const int t = blockIdx.x, g = rt.graph[t];
     const int r0 = rt.start[t], r1 = rt.start[t + 1];
@@ -375,7 +405,6 @@ __global__ __launch_bounds__(1024) void kb_update_xr_rr(Tiles rt, int k, const d
             const T a = (T)alpha[(size_t)g * k + j];
             for (int r = r0 + BT_WAVE; r < r1; r += BT_NW) {
                 const size_t o = (size_t)r * k + j;
-                X[o] = bt_fma(a, P[o], X[o]);
                 const T rv = bt_fma(-a, Q[o], R[o]);
                 R[o] = rv;
                 acc = bt_fma((double)rv, (double)rv, acc);
@@ -607,11 +636,19 @@ template void bt_project<double, double>(hipStream_t, Tiles, int, const double *
 template void bt_project<float, float>(hipStream_t, Tiles, int, const float *, const int32_t *, const double *,
                                        float *, float *, float *);
 template <typename T>
-void bt_update_p(hipStream_t s, Tiles rt, int k, const double *beta, const T *Z, T *P) {
-    hipLaunchKernelGGL(kb_update_p<T>, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, k, beta, Z, P);
+void bt_update_px(hipStream_t s, Tiles rt, int k, const double *alpha, const double *beta, const T *Z, T *P, T *X) {
+    hipLaunchKernelGGL(kb_update_px<T>, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, k, alpha, beta, Z, P, X);
 }
-template void bt_update_p<double>(hipStream_t, Tiles, int, const double *, const double *, double *);
-template void bt_update_p<float>(hipStream_t, Tiles, int, const double *, const float *, float *);
+template void bt_update_px<double>(hipStream_t, Tiles, int, const double *, const double *, const double *, double *,
+                                   double *);
+template void bt_update_px<float>(hipStream_t, Tiles, int, const double *, const double *, const float *, float *,
+                                  float *);
+template <typename T>
+void bt_update_x(hipStream_t s, Tiles rt, int k, const double *alpha, const T *P, T *X) {
+    hipLaunchKernelGGL(kb_update_x<T>, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, k, alpha, P, X);
+}
+template void bt_update_x<double>(hipStream_t, Tiles, int, const double *, const double *, double *);
+template void bt_update_x<float>(hipStream_t, Tiles, int, const double *, const float *, float *);
 template <typename T>
 void bt_spmm_dot(hipStream_t s, Tiles rt, const int32_t *rowptr, const int32_t *col, const T *val, int k, const T *P,
                  T *Q, double *partial) {
@@ -622,14 +659,11 @@ template void bt_spmm_dot<double>(hipStream_t, Tiles, const int32_t *, const int
 template void bt_spmm_dot<float>(hipStream_t, Tiles, const int32_t *, const int32_t *, const float *, int,
                                  const float *, float *, double *);
 template <typename T>
-void bt_update_xr_rr(hipStream_t s, Tiles rt, int k, const double *alpha, const T *P, const T *Q, T *X, T *R,
-                     double *partial) {
-    hipLaunchKernelGGL(kb_update_xr_rr<T>, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, k, alpha, P, Q, X, R, partial);
+void bt_update_r_rr(hipStream_t s, Tiles rt, int k, const double *alpha, const T *Q, T *R, double *partial) {
+    hipLaunchKernelGGL(kb_update_r_rr<T>, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, k, alpha, Q, R, partial);
 }
-template void bt_update_xr_rr<double>(hipStream_t, Tiles, int, const double *, const double *, const double *,
-                                      double *, double *, double *);
-template void bt_update_xr_rr<float>(hipStream_t, Tiles, int, const double *, const float *, const float *, float *,
-                                     float *, double *);
+template void bt_update_r_rr<double>(hipStream_t, Tiles, int, const double *, const double *, double *, double *);
+template void bt_update_r_rr<float>(hipStream_t, Tiles, int, const double *, const float *, float *, double *);
 void bt_residual64(hipStream_t s, Tiles rt, const int32_t *rowptr, const int32_t *col, const double *val, int k,
                    const double *X, const double *B, double *out, double *partial) {
     hipLaunchKernelGGL(kb_residual64, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, rowptr, col, val, k, X, B, out, partial);

@@ -731,9 +731,10 @@ static void component_means(ss_graph *g, int k, const double *Y) {
     bt_comp_means(s, g->ncomp, k, g->comp_graph, g->comp_off, g->cptr, g->gsum, g->csum, g->cmean);
 }
 
-// One PCG iteration on every (graph, column): P = Z + beta P; Q = A P;
-// alpha; X += alpha P, R -= alpha Q; stop test.  With tail, also the next
-// preconditioner application and beta (what the next iteration needs).
+// One PCG iteration on every (graph, column): P = Z + beta P together with
+// the previous iteration's X += alpha P; Q = A P; alpha; R -= alpha Q; stop
+// test.  With tail, also the next preconditioner application and beta
+// (what the next iteration needs).  The solve ends with the last X update.
 template <typename T>
 static int pcg_iteration(ss_graph *g, int k, T *X, const PcgWork<T> &w, int rule, double tol, bool tail) {
     hipStream_t s = g->ctx->stream;
@@ -741,7 +742,8 @@ static int pcg_iteration(ss_graph *g, int k, T *X, const PcgWork<T> &w, int rule
     const int gk = g->G * k;
     {
         ProfScope ps(g, SS_KERNEL_VECTOR);
-        bt_update_p<T>(s, rt, k, g->sc.beta, w.Zp, w.P);   // P = Z + beta P
+        // P = Z + beta P; X += alpha P of the previous iteration
+        bt_update_px<T>(s, rt, k, g->sc.alpha, g->sc.beta, w.Zp, w.P, X);
     }
     {
         ProfScope ps(g, SS_KERNEL_SPMM);
@@ -750,7 +752,7 @@ static int pcg_iteration(ss_graph *g, int k, T *X, const PcgWork<T> &w, int rule
     bt_pcg_alpha(s, gk, k, g->gtile, g->partial, g->sc, g->status);
     {
         ProfScope ps(g, SS_KERNEL_VECTOR);
-        bt_update_xr_rr<T>(s, rt, k, g->sc.alpha, w.P, w.Q, X, w.R, g->partial);
+        bt_update_r_rr<T>(s, rt, k, g->sc.alpha, w.Q, w.R, g->partial);
     }
     bt_pcg_check(s, gk, k, g->gtile, g->partial, rule, tol, g->sc, g->status);
     if (tail) {
@@ -832,6 +834,12 @@ static int pcg_run(ss_graph *g, int k, const T *rhs, const double *mean_src, T *
         if (nactive == 0) break;
         SS_TRY(precond<T>(g, k, w.R, w.W, w.Zp, g->partial2));
         bt_pcg_beta(s, gk, k, g->gtile, g->partial2, g->sc);
+    }
+    {
+        // the last iteration's X += alpha P (alpha is zero for the columns
+        // that stopped earlier: their last update rode with update_px)
+        ProfScope ps(g, SS_KERNEL_VECTOR);
+        bt_update_x<T>(s, rt, k, g->sc.alpha, w.P, X);
     }
     SS_TRY(read_status(g, nullptr));
     iters.resize(gk);
@@ -981,9 +989,9 @@ static int autotune_t(ss_graph *g, int k, const PcgWork<T> &w, int reps, ss_tuni
         const Tiles rt = rtiles(g);
         double ms = 0;
         SS_TRY(time_median(g, reps, [&]() {
-            bt_update_p<T>(s, rt, k, g->sc.beta, w.Zp, w.P);
+            bt_update_px<T>(s, rt, k, g->sc.alpha, g->sc.beta, w.Zp, w.P, w.B);
             bt_spmm_dot<T>(s, rt, g->rowptr, g->col, avals<T>(g), k, w.P, w.Q, g->partial);
-            bt_update_xr_rr<T>(s, rt, k, g->sc.alpha, w.P, w.Q, w.B, w.W, g->partial);
+            bt_update_r_rr<T>(s, rt, k, g->sc.alpha, w.Q, w.W, g->partial);
             SS_HIP(hipGetLastError());
             return SS_OK;
         }, ms));

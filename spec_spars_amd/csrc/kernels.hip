@@ -572,6 +572,40 @@ __device__ __forceinline__ T wave_parts(int len, int c0, int c1, T v0, T v1, con
     return part[0];
 }
 
+// Tree fold of parts [s0, s0 + NA) of segment [a, b) split in P parts, on
+// one wave: slot l of a 64-slot round holds entry a + s0 + l % NA + P (l / NA)
+// (so part s0 + u takes its entries in order), the slots' entries load one
+// per lane, then gather 16 at a time.  Slots past the segment's end hold
+// zero values, whose terms leave a part exact.
+template <typename T, int NA>
+__device__ __forceinline__ T wave_subtree(int a, int b, int P, int s0, const int32_t *__restrict__ col,
+                                          const T *__restrict__ val, const T *X, int k, int jj, int lane) {
+    static_assert(NA == 1 || NA == 2 || NA == 4, "parts per wave");
+    T part[NA];
+#pragma unroll
+    for (int u = 0; u < NA; u++) part[u] = (T)0;
+    const int a0 = a + s0;
+    for (int base = 0; a0 + P * (base / NA) < b; base += 64) {
+        const int l = base + lane;
+        const int q = a0 + l % NA + P * (l / NA);
+        const int c = q < b ? col[q] : 0;
+        const T v = q < b ? val[q] : (T)0;
+        for (int e0 = 0; e0 < 64 && a0 + P * ((base + e0) / NA) < b; e0 += 16) {
+            T x[16];
+#pragma unroll
+            for (int u = 0; u < 16; u++) {
+                const int cu = lane_get(c, e0 + u);
+                const int lu = base + e0 + u;
+                x[u] = a0 + lu % NA + P * (lu / NA) < b ? X[(size_t)cu * k + jj] : (T)0;
+            }
+#pragma unroll
+            for (int u = 0; u < 16; u++) part[u % NA] = fnms(lane_get(v, e0 + u), x[u], part[u % NA]);
+        }
+    }
+    tree_fold<T, NA>(part);
+    return part[0];
+}
+
 // One row of at most kWaveRowMax entries on one wave: the row's entries are
 // loaded one per lane, so a single dependent load separates the row record
 // from all of its gathers.
@@ -686,16 +720,32 @@ __global__ __launch_bounds__(64 * kLevelWaves) void k_trisolve_level(int32_t t0,
                                                                      const T *__restrict__ in, T *__restrict__ out) {
     const int lane = SS_LANE, wv = SS_WAVE_IN_BLOCK, W = (int)(blockDim.x >> 6);
     if ((int)blockIdx.x < nlong) {
+        // a long row: gg of the W waves hold P / gg consecutive parts each
+        // (subtrees of the row's part tree), wave 0 folds them
         __shared__ T sm[kLevelWaves][64];
         const int t = t0 + blockIdx.x;
         const int i = wave_uniform(rows[t]);
         const int q0 = wave_uniform(ptr[t]), q1 = wave_uniform(ptr[t + 1]);
         const double d = BWD ? diag[i] : 1.0;
+        const int P = seg_parts(q1 - q0);
+        const int gg = P < W ? P : W, na = P / gg;
         for (int jb = 0; jb < k; jb += 64) {
-            const int j = jb + lane;
-            T acc = (wv == 0 && j < k) ? row_start<T, BWD>(i, j, k, d, in) : (T)0;
-            acc = seg_apply_shared<T, kMaxParts / 4>(acc, q0, q1, true, W, wv, 0, sm, col, val, out, k, j, lane);
-            if (wv == 0 && j < k) out[(size_t)i * k + j] = acc;
+            const int j = jb + lane, jj = j < k ? j : k - 1;
+            T acc = wv == 0 ? row_start<T, BWD>(i, jj, k, d, in) : (T)0;
+            if (P == 1) {
+                if (wv == 0) acc = row_dot_sub<T>(acc, q0, q1, 1, col, val, out, k, jj);
+            } else if (wv < gg) {
+                const int s0 = wv * na;
+                sm[wv][lane] = na == 1   ? wave_subtree<T, 1>(q0, q1, P, s0, col, val, out, k, jj, lane)
+                               : na == 2 ? wave_subtree<T, 2>(q0, q1, P, s0, col, val, out, k, jj, lane)
+                                         : wave_subtree<T, 4>(q0, q1, P, s0, col, val, out, k, jj, lane);
+            }
+            __syncthreads();
+            if (wv == 0) {
+                if (P > 1) acc = acc + lds_tree<T>(sm, 0, gg, lane);
+                if (j < k) out[(size_t)i * k + j] = acc;
+            }
+            __syncthreads();
         }
         return;
     }
