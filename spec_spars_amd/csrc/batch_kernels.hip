@@ -374,14 +374,30 @@ __global__ __launch_bounds__(1024) void kb_spmm_dot(Tiles rt, const int32_t *__r
                     }
                 }
             } else {
+                // one row at a time: 64 entries at a time one per lane, 16
+                // gathers in flight (zero-valued slots past the row's end add
+                // exact zeros)
 #pragma unroll
                 for (int u = 0; u < G; u++) {
                     if (rb + u * NW >= r1) break;
+                    T a = (T)0;
+                    for (int base = 0; base < len[u]; base += 64) {
+                        const int n = len[u] - base;
+                        const int cc = lane < n ? col[q0[u] + base + lane] : 0;
+                        const T vv = lane < n ? val[q0[u] + base + lane] : (T)0;
+                        for (int e0 = 0; e0 < 64 && e0 < n; e0 += 16) {
+                            T x[16];
+#pragma unroll
+                            for (int e = 0; e < 16; e++)
+                                x[e] = e0 + e < n ? P[(size_t)bt_lane(cc, e0 + e) * k + jj] : (T)0;
+#pragma unroll
+                            for (int e = 0; e < 16; e++) a = bt_fma(-bt_lane(vv, e0 + e), x[e], a);
+                        }
+                    }
                     if (j < k) {
-                        const T qv = -bt_dot_sub<T>((T)0, q0[u], q0[u] + len[u], col, val, P, k, j);
                         const size_t o = (size_t)rr[u] * k + j;
-                        Q[o] = qv;
-                        acc = bt_fma((double)P[o], (double)qv, acc);
+                        Q[o] = -a;
+                        acc = bt_fma((double)P[o], (double)(-a), acc);
                     }
                 }
             }

@@ -684,7 +684,8 @@ static void solve_cores(ss_graph *g, int k, const T *in, T *out) {
     if (BWD)
         launch_core_bwd_init<T>(s, (int32_t)g->core_total, g->core_vert, g->diag, k, in, out);
     else
-        launch_core_head<T>(s, g->f_core_t0, g->f_nrows, g->f_rows, g->f_ptr, g->f_col, fvals<T>(g), k, in, out);
+        launch_core_head<T>(s, g->f_core_t0, g->f_nrows, g->f_rows, g->f_ptr, g->f_col, fvals<T>(g), g->diag, k, in,
+                            out);
     for (int st = 0; st < g->core_steps; st++)
         launch_core_step<T, BWD>(s, st, g->G, g->core_max, g->core_n, g->core_off, g->dense_off, g->core_vert,
                                  core_L<T>(g, BWD), k, out);

@@ -104,7 +104,7 @@ void launch_tri_small(hipStream_t s, int32_t l0, int32_t l1, const int32_t *leve
 // launch_core_step<T, true> for the same steps with D = L^T.
 template <typename T>
 void launch_core_head(hipStream_t s, int32_t t0, int32_t t1, const int32_t *rows, const int32_t *ptr,
-                      const int32_t *col, const T *val, int k, const T *in, T *out);
+                      const int32_t *col, const T *val, const double *diag, int k, const T *in, T *out);
 template <typename T>
 void launch_core_bwd_init(hipStream_t s, int32_t count, const int32_t *vert, const double *diag, int k, const T *in,
                           T *out);

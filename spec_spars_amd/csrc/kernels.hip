@@ -436,64 +436,6 @@ __device__ __forceinline__ void tree_fold(T (&v)[N]) {
         for (int u = 0; u + step < N; u += 2 * step) v[u] = v[u] + v[u + step];
 }
 
-// Tree fold of parts [s0, s0 + NA) of segment [a, b) split in P parts (part
-// s: running subtraction from zero over q = a + s, a + s + P, ...).  NA
-// accumulators run side by side with 8 gathers in flight per round.
-template <typename T, int NA>
-__device__ __forceinline__ T seg_subtree(int a, int b, int P, int s0, const int32_t *__restrict__ col,
-                                         const T *__restrict__ val, const T *X, int k, int j) {
-    static_assert(NA >= 1 && NA <= 8, "at most 8 parts per wave");
-    constexpr int U = 8 / NA;                 // chunks per round
-    T acc[NA];
-#pragma unroll
-    for (int u = 0; u < NA; u++) acc[u] = (T)0;
-    for (int c = a + s0; c < b; c += U * P) {
-        T x[U][NA], v[U][NA];
-#pragma unroll
-        for (int r = 0; r < U; r++)
-#pragma unroll
-            for (int u = 0; u < NA; u++) {
-                const int q = c + r * P + u;
-                const bool in = q < b;
-                v[r][u] = in ? val[q] : (T)0;
-                x[r][u] = in ? X[(size_t)col[q] * k + j] : (T)0;
-            }
-#pragma unroll
-        for (int r = 0; r < U; r++)
-#pragma unroll
-            for (int u = 0; u < NA; u++)
-                if (c + r * P + u < b) acc[u] = fnms(v[r][u], x[r][u], acc[u]);
-    }
-    tree_fold<T, NA>(acc);
-    return acc[0];
-}
-
-// Tree fold of parts [s0, s0 + na) with na in {1, 2, 4, 8, 16}, na <= MAXNA
-// (the bound keeps unused wide variants out of a kernel's registers); 16
-// parts fold as two subtrees of 8, the top of their tree.
-template <typename T, int MAXNA>
-__device__ __forceinline__ T seg_subtree_n(int na, int a, int b, int P, int s0, const int32_t *__restrict__ col,
-                                           const T *__restrict__ val, const T *X, int k, int j) {
-    if (na == 1 || MAXNA == 1) return seg_subtree<T, 1>(a, b, P, s0, col, val, X, k, j);
-    if (na == 2 || MAXNA == 2) return seg_subtree<T, (MAXNA < 2 ? MAXNA : 2)>(a, b, P, s0, col, val, X, k, j);
-    if (na == 4 || MAXNA == 4) return seg_subtree<T, (MAXNA < 4 ? MAXNA : 4)>(a, b, P, s0, col, val, X, k, j);
-    if (na == 8 || MAXNA == 8) return seg_subtree<T, (MAXNA < 8 ? MAXNA : 8)>(a, b, P, s0, col, val, X, k, j);
-    const T lo = seg_subtree<T, (MAXNA < 8 ? MAXNA : 8)>(a, b, P, s0, col, val, X, k, j);
-    const T hi = seg_subtree<T, (MAXNA < 8 ? MAXNA : 8)>(a, b, P, s0 + 8, col, val, X, k, j);
-    return lo + hi;
-}
-
-// acc updated by segment [a, b), every part on this wave; segments longer
-// than MAXLEN entries never reach here (MAXLEN bounds the part count).
-template <typename T, int MAXLEN>
-__device__ __forceinline__ T seg_apply(T acc, int a, int b, const int32_t *__restrict__ col,
-                                       const T *__restrict__ val, const T *X, int k, int j) {
-    constexpr int PMAX = seg_parts(MAXLEN);
-    const int P = seg_parts(b - a);
-    if (P == 1 || PMAX == 1) return row_dot_sub<T>(acc, a, b, 1, col, val, X, k, j);
-    return acc + seg_subtree_n<T, PMAX>(P, a, b, P, 0, col, val, X, k, j);
-}
-
 // Tree fold of sm[first .. first + N) (column lane).
 template <typename T, int N>
 __device__ __forceinline__ T lds_tree_n(T (*sm)[64], int first, int lane) {
@@ -512,26 +454,6 @@ __device__ __forceinline__ T lds_tree(T (*sm)[64], int first, int n, int lane) {
         case 8: return lds_tree_n<T, 8>(sm, first, lane);
         default: return lds_tree_n<T, 16>(sm, first, lane);
     }
-}
-
-// acc updated by segment [a, b) on the g waves of a block or workgroup that
-// share the row (g a power of two; this wave is number sub of them, they
-// occupy sm[first .. first+g)); the result is valid on sub 0.  Every wave of
-// the block calls this (it holds barriers).
-template <typename T, int MAXNA>
-__device__ __forceinline__ T seg_apply_shared(T acc, int a, int b, bool act, int g, int sub, int first, T (*sm)[64],
-                                              const int32_t *__restrict__ col, const T *__restrict__ val,
-                                              const T *X, int k, int j, int lane) {
-    const int P = act ? seg_parts(b - a) : 1;
-    const int gg = P < g ? P : g;             // waves that hold parts
-    if (act && P == 1 && sub == 0 && j < k) acc = row_dot_sub<T>(acc, a, b, 1, col, val, X, k, j);
-    if (act && P > 1 && sub < gg)
-        sm[first + sub][lane] =
-            j < k ? seg_subtree_n<T, MAXNA>(P / gg, a, b, P, sub * (P / gg), col, val, X, k, j) : (T)0;
-    __syncthreads();
-    if (act && P > 1 && sub == 0) acc = acc + lds_tree<T>(sm, first, gg, lane);
-    __syncthreads();
-    return acc;
 }
 
 // Lane l's value of v (l uniform; the lane need not be active).
@@ -580,7 +502,7 @@ __device__ __forceinline__ T wave_parts(int len, int c0, int c1, T v0, T v1, con
 template <typename T, int NA>
 __device__ __forceinline__ T wave_subtree(int a, int b, int P, int s0, const int32_t *__restrict__ col,
                                           const T *__restrict__ val, const T *X, int k, int jj, int lane) {
-    static_assert(NA == 1 || NA == 2 || NA == 4, "parts per wave");
+    static_assert(NA == 1 || NA == 2 || NA == 4 || NA == 8 || NA == 16, "parts per wave");
     T part[NA];
 #pragma unroll
     for (int u = 0; u < NA; u++) part[u] = (T)0;
@@ -604,6 +526,18 @@ __device__ __forceinline__ T wave_subtree(int a, int b, int P, int s0, const int
     }
     tree_fold<T, NA>(part);
     return part[0];
+}
+
+// na in {1, 2, 4, 8, 16}, na <= MAXNA (the bound keeps wider variants out
+// of a kernel's registers).
+template <typename T, int MAXNA>
+__device__ __forceinline__ T wave_subtree_n(int na, int a, int b, int P, int s0, const int32_t *__restrict__ col,
+                                            const T *__restrict__ val, const T *X, int k, int jj, int lane) {
+    if (na == 1 || MAXNA == 1) return wave_subtree<T, 1>(a, b, P, s0, col, val, X, k, jj, lane);
+    if (na == 2 || MAXNA == 2) return wave_subtree<T, (MAXNA < 2 ? MAXNA : 2)>(a, b, P, s0, col, val, X, k, jj, lane);
+    if (na == 4 || MAXNA == 4) return wave_subtree<T, (MAXNA < 4 ? MAXNA : 4)>(a, b, P, s0, col, val, X, k, jj, lane);
+    if (na == 8 || MAXNA == 8) return wave_subtree<T, (MAXNA < 8 ? MAXNA : 8)>(a, b, P, s0, col, val, X, k, jj, lane);
+    return wave_subtree<T, MAXNA>(a, b, P, s0, col, val, X, k, jj, lane);
 }
 
 // One row of at most kWaveRowMax entries on one wave: the row's entries are
@@ -736,9 +670,7 @@ __global__ __launch_bounds__(64 * kLevelWaves) void k_trisolve_level(int32_t t0,
                 if (wv == 0) acc = row_dot_sub<T>(acc, q0, q1, 1, col, val, out, k, jj);
             } else if (wv < gg) {
                 const int s0 = wv * na;
-                sm[wv][lane] = na == 1   ? wave_subtree<T, 1>(q0, q1, P, s0, col, val, out, k, jj, lane)
-                               : na == 2 ? wave_subtree<T, 2>(q0, q1, P, s0, col, val, out, k, jj, lane)
-                                         : wave_subtree<T, 4>(q0, q1, P, s0, col, val, out, k, jj, lane);
+                sm[wv][lane] = wave_subtree_n<T, kMaxParts / 4>(na, q0, q1, P, s0, col, val, out, k, jj, lane);
             }
             __syncthreads();
             if (wv == 0) {
@@ -769,24 +701,6 @@ __global__ __launch_bounds__(64 * kLevelWaves) void k_trisolve_level(int32_t t0,
         int cnt = rpw - g0 < TG ? rpw - g0 : TG;
         if (t + cnt > t1) cnt = t1 - t;
         wave_tiny_rows<T, BWD, TG>(t, cnt, rows, ptr, col, val, diag, k, in, out, lane);
-    }
-}
-
-// Dense cores, forward step 0: every core row's entries outside the core
-// (schedule positions [t0, t1)): out_i = r_i updated by them.  Wave per row.
-template <typename T>
-__global__ __launch_bounds__(256) void k_core_head(int32_t t0, int32_t t1, const int32_t *__restrict__ rows,
-                                                   const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
-                                                   const T *__restrict__ val, int k, const T *__restrict__ in,
-                                                   T *__restrict__ out) {
-    const int t = wave_uniform(t0 + blockIdx.x * 4 + SS_WAVE_IN_BLOCK);
-    if (t >= t1) return;
-    const int i = wave_uniform(rows[t]);
-    const int q0 = wave_uniform(ptr[t]), q1 = wave_uniform(ptr[t + 1]);
-    for (int jb = 0; jb < k; jb += 64) {
-        const int j = jb + SS_LANE;
-        if (j >= k) break;
-        out[(size_t)i * k + j] = seg_apply<T, (1 << 30)>(in[(size_t)i * k + j], q0, q1, col, val, out, k, j);
     }
 }
 
@@ -882,12 +796,39 @@ __global__ __launch_bounds__(256) void k_core_step(int s, const int32_t *__restr
     }
     if (r0 >= r1) return;   // uniform over the workgroup
     const int nr = r1 - r0;
+    constexpr int DPT = B * B / 256;   // entries of a 64 x 64 block per thread
     for (int jb = 0; jb < k; jb += 64) {
         const int j = jb + lane;
-        for (int r = wv; r < np; r += 4) sm_z[r][lane] = j < k ? out[(size_t)vg[p0 + r] * k + j] : (T)0;
-        for (int e = threadIdx.x; e < B * B; e += blockDim.x) {
-            const int a = e / B, b = e % B;
-            sm_d[a][b] = (a < nr && b < np) ? Dg[(size_t)(r0 + a) * c + p0 + b] : (T)0;
+        // every load of the previous block's values, the D panel and (for
+        // the workgroup that substitutes) the diagonal block is in flight
+        // before the first shared store
+        T dd[DPT];
+        if (x == 0) {
+#pragma unroll
+            for (int u = 0; u < DPT; u++) {
+                const int e = (int)threadIdx.x + 256 * u, a = e / B, b = e % B;
+                dd[u] = (a < nr && b < nr) ? Dg[(size_t)(b0 + a) * c + b0 + b] : (T)0;
+            }
+        }
+        {
+            T zv[RW], dv[DPT];
+#pragma unroll
+            for (int u = 0; u < RW; u++) {
+                const int r = wv + 4 * u;
+                zv[u] = (r < np && j < k) ? out[(size_t)vg[p0 + r] * k + j] : (T)0;
+            }
+#pragma unroll
+            for (int u = 0; u < DPT; u++) {
+                const int e = (int)threadIdx.x + 256 * u, a = e / B, b = e % B;
+                dv[u] = (a < nr && b < np) ? Dg[(size_t)(r0 + a) * c + p0 + b] : (T)0;
+            }
+#pragma unroll
+            for (int u = 0; u < RW; u++) sm_z[wv + 4 * u][lane] = zv[u];
+#pragma unroll
+            for (int u = 0; u < DPT; u++) {
+                const int e = (int)threadIdx.x + 256 * u;
+                sm_d[e / B][e % B] = dv[u];
+            }
         }
         __syncthreads();
         T acc[RW];
@@ -897,6 +838,7 @@ __global__ __launch_bounds__(256) void k_core_step(int s, const int32_t *__restr
             acc[u] = (r < nr && j < k) ? out[(size_t)vg[r0 + r] * k + j] : (T)0;
         }
         // the previous block: ascending columns forward, descending backward
+#pragma unroll 4
         for (int qq = 0; qq < np; qq++) {
             const int q = BWD ? np - 1 - qq : qq;
             const T v = sm_z[q][lane];
@@ -905,9 +847,10 @@ __global__ __launch_bounds__(256) void k_core_step(int s, const int32_t *__restr
         }
         if (x == 0) {
             __syncthreads();   // the panel's reads of sm_z and sm_d are done
-            for (int e = threadIdx.x; e < B * B; e += blockDim.x) {
-                const int a = e / B, b = e % B;
-                sm_d[a][b] = (a < nr && b < nr) ? Dg[(size_t)(b0 + a) * c + b0 + b] : (T)0;
+#pragma unroll
+            for (int u = 0; u < DPT; u++) {
+                const int e = (int)threadIdx.x + 256 * u;
+                sm_d[e / B][e % B] = dd[u];
             }
             __syncthreads();
             for (int st = 0; st < 4; st++) {
@@ -919,6 +862,7 @@ __global__ __launch_bounds__(256) void k_core_step(int s, const int32_t *__restr
                 }
                 __syncthreads();
                 if (BWD ? wv < w : wv > w) {
+#pragma unroll
                     for (int aa = 0; aa < RW; aa++) {
                         const int a = w * RW + (BWD ? RW - 1 - aa : aa);
                         const T v = sm_z[a][lane];
@@ -963,12 +907,20 @@ __global__ __launch_bounds__(1024) void k_tri_small(int32_t l0, int32_t l1, cons
         const int i = act ? wave_uniform(rows[t]) : 0;
         const int q0 = act ? wave_uniform(ptr[t]) : 0, q1 = act ? wave_uniform(ptr[t + 1]) : 0;
         const double d = (BWD && act) ? diag[i] : 1.0;
+        const int P = act ? seg_parts(q1 - q0) : 1;
+        const int gg = P < g ? P : g;             // waves that hold parts, P / gg each
         for (int jb = 0; jb < k; jb += 64) {
-            const int j = jb + lane;
+            const int j = jb + lane, jj = j < k ? j : k - 1;
             T acc = (T)0;
-            if (act && sub == 0 && j < k) acc = row_start<T, BWD>(i, j, k, d, in);
-            acc = seg_apply_shared<T, kMaxParts>(acc, q0, q1, act, g, sub, slot * g, sm, col, val, out, k, j, lane);
+            if (act && sub == 0) acc = row_start<T, BWD>(i, jj, k, d, in);
+            if (act && P == 1 && sub == 0) acc = row_dot_sub<T>(acc, q0, q1, 1, col, val, out, k, jj);
+            if (act && P > 1 && sub < gg)
+                sm[slot * g + sub][lane] = wave_subtree_n<T, kMaxParts>(P / gg, q0, q1, P, sub * (P / gg), col, val,
+                                                                       out, k, jj, lane);
+            __syncthreads();
+            if (act && P > 1 && sub == 0) acc = acc + lds_tree<T>(sm, slot * g, gg, lane);
             if (act && sub == 0 && j < k) out[(size_t)i * k + j] = acc;
+            __syncthreads();
         }
         __syncthreads();
     }
@@ -1253,15 +1205,16 @@ template void launch_spmm_dot<float>(hipStream_t, int32_t, const int32_t *, cons
 
 template <typename T>
 void launch_core_head(hipStream_t s, int32_t t0, int32_t t1, const int32_t *rows, const int32_t *ptr,
-                      const int32_t *col, const T *val, int k, const T *in, T *out) {
-    if (t1 <= t0) return;
-    hipLaunchKernelGGL(k_core_head<T>, dim3(grid_for(t1 - t0, 4)), dim3(256), 0, s, t0, t1, rows, ptr, col, val, k,
-                       in, out);
+                      const int32_t *col, const T *val, const double *diag, int k, const T *in, T *out) {
+    // every core row takes a block of 16 waves, as a long row of a level
+    launch_trisolve_level<T, false>(s, t0, t1 - t0, 0, t1, kLevelWaves, 1, 0, rows, ptr, col, val, diag, k, in,
+                                    out);
 }
 template void launch_core_head<double>(hipStream_t, int32_t, int32_t, const int32_t *, const int32_t *,
-                                       const int32_t *, const double *, int, const double *, double *);
+                                       const int32_t *, const double *, const double *, int, const double *,
+                                       double *);
 template void launch_core_head<float>(hipStream_t, int32_t, int32_t, const int32_t *, const int32_t *,
-                                      const int32_t *, const float *, int, const float *, float *);
+                                      const int32_t *, const float *, const double *, int, const float *, float *);
 
 template <typename T>
 void launch_core_bwd_init(hipStream_t s, int32_t count, const int32_t *vert, const double *diag, int k, const T *in,
