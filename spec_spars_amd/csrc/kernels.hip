@@ -920,9 +920,8 @@ __global__ __launch_bounds__(1024) void k_tri_small(int32_t l0, int32_t l1, cons
             __syncthreads();
             if (act && P > 1 && sub == 0) acc = acc + lds_tree<T>(sm, slot * g, gg, lane);
             if (act && sub == 0 && j < k) out[(size_t)i * k + j] = acc;
-            __syncthreads();
+            __syncthreads();   // also orders this level's stores before the next level's loads
         }
-        __syncthreads();
     }
 }
 
