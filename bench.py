@@ -356,7 +356,8 @@ def bench_serve(args):
 def probe(args):
     """A short fixed program for hardware-counter runs: the workload's
     graph and factor, one sparsify pass capped at 3 PCG iterations with the
-    default tuned shapes, then one 1 GiB FP64 copy for calibration."""
+    shapes the autotuner picks on the 128^3 grid (16 waves per row tile, two
+    tiny rows per wave), then one 1 GiB FP64 copy for calibration."""
     import spec_spars_amd as ss
     from spec_spars_amd import graphs
     W = workload(args.workload)
@@ -365,7 +366,8 @@ def probe(args):
     G = ss.Graph(ctx, n, u, v, w)
     cp, ri, va = G.laplacian()
     G.set_factor(ss.Factor(cp, ri, va, seed=0, ordering=ss.ORDER_ROUNDS))
-    ctx.set_tuning(tile_waves=16, level_rows_per_wave=1, graph_iters=0, xcd_slices=0 if args.no_xcd else 1)
+    ctx.set_tuning(tile_waves=16, level_rows_per_wave=args.probe_rpw, graph_iters=0,
+                   xcd_slices=0 if args.no_xcd else 1)
     prec = ss.PREC_FP64 if (args.precision or W.get("precision", "fp64")) == "fp64" else ss.PREC_MIXED
     solve = ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-8, max_iter=3, precision=prec)
     solve.fail_relres = 1e300
@@ -404,6 +406,7 @@ def main():
     ap.add_argument("--probe", action="store_true",
                     help="one short sparsify pass (3 PCG iterations) and a 1 GiB copy, for counter runs")
     ap.add_argument("--no-xcd", action="store_true", help="probe: round-robin rows over the XCDs")
+    ap.add_argument("--probe-rpw", type=int, default=2, help="probe: tiny rows per wave of the level launches")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: Poisson arrivals per second")
     ap.add_argument("--duration", type=float, default=60.0, help="serve: arrival window in seconds")
     ap.add_argument("--pool", type=int, default=12, help="serve: resident graphs")
