@@ -20,6 +20,11 @@ namespace ss {
 
 __device__ __forceinline__ int bt_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
+// Rows a wave of a streaming tile kernel handles per pass (rows w, w + NW,
+// ... of the tile, their loads in flight together; reductions still take
+// the rows in order).
+constexpr int kBtRows = 4;
+
 // Explicit fused multiply-adds: one rounding per term whatever the loop
 // layout (contraction only joins a multiply and an add in one basic block).
 __device__ __forceinline__ double bt_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
@@ -102,13 +107,22 @@ __global__ __launch_bounds__(1024) void kb_tile_partial(Tiles rt, int k, const T
         const int j = jb + BT_LANE;
         double acc = 0.0;
         if (j < k) {
-            for (int r = r0 + BT_WAVE; r < r1; r += BT_NW) {
-                const double a = (double)A[(size_t)r * k + j];
-                if (op == 0) acc = bt_fma(a, (double)B[(size_t)r * k + j], acc);
-                else if (op == 1) acc += a;
-                else {
-                    const double d = diag[r];
-                    acc = d != 0.0 ? bt_fma(a, a / d, acc) : bt_fma(a, a, acc);
+            // kBtRows rows per pass (their loads in flight together), added in row order
+            for (int rb = r0 + BT_WAVE; rb < r1; rb += kBtRows * BT_NW) {
+                double a[kBtRows], b[kBtRows];
+#pragma unroll
+                for (int u = 0; u < kBtRows; u++) {
+                    const int r = rb + u * BT_NW;
+                    const bool on = r < r1;
+                    a[u] = on ? (double)A[(size_t)r * k + j] : 0.0;
+                    b[u] = on && op == 0 ? (double)B[(size_t)r * k + j] : (on && op == 2 ? diag[r] : 0.0);
+                }
+#pragma unroll
+                for (int u = 0; u < kBtRows; u++) {
+                    if (rb + u * BT_NW >= r1) break;
+                    if (op == 0) acc = bt_fma(a[u], b[u], acc);
+                    else if (op == 1) acc += a[u];
+                    else acc = b[u] != 0.0 ? bt_fma(a[u], a[u] / b[u], acc) : bt_fma(a[u], a[u], acc);
                 }
             }
         }
@@ -273,14 +287,24 @@ __global__ __launch_bounds__(1024) void kb_update_px(Tiles rt, int k, const doub
         if (j >= k) break;
         const T a = (T)alpha[(size_t)g * k + j];
         const T b = (T)beta[(size_t)g * k + j];
-        for (int r = r0 + BT_WAVE; r < r1; r += BT_NW) {
-            const size_t o = (size_t)r * k + j;
-            if (a != (T)0) {
-                const T p = P[o];
-                X[o] = bt_fma(a, p, X[o]);
-                P[o] = b == (T)0 ? Z[o] : bt_fma(b, p, Z[o]);
-            } else {
-                P[o] = b == (T)0 ? Z[o] : bt_fma(b, P[o], Z[o]);
+        for (int rb = r0 + BT_WAVE; rb < r1; rb += kBtRows * BT_NW) {
+            T p[kBtRows], z[kBtRows], x[kBtRows];
+#pragma unroll
+            for (int u = 0; u < kBtRows; u++) {
+                const int r = rb + u * BT_NW;
+                const bool on = r < r1;
+                const size_t o = (size_t)r * k + j;
+                p[u] = on && (a != (T)0 || b != (T)0) ? P[o] : (T)0;
+                z[u] = on ? Z[o] : (T)0;
+                x[u] = on && a != (T)0 ? X[o] : (T)0;
+            }
+#pragma unroll
+            for (int u = 0; u < kBtRows; u++) {
+                const int r = rb + u * BT_NW;
+                if (r >= r1) break;
+                const size_t o = (size_t)r * k + j;
+                if (a != (T)0) X[o] = bt_fma(a, p[u], x[u]);
+                P[o] = b == (T)0 ? z[u] : bt_fma(b, p[u], z[u]);
             }
         }
     }
@@ -419,11 +443,23 @@ __global__ __launch_bounds__(1024) void kb_update_r_rr(Tiles rt, int k, const do
         double acc = 0.0;
         if (j < k) {
             const T a = (T)alpha[(size_t)g * k + j];
-            for (int r = r0 + BT_WAVE; r < r1; r += BT_NW) {
-                const size_t o = (size_t)r * k + j;
-                const T rv = bt_fma(-a, Q[o], R[o]);
-                R[o] = rv;
-                acc = bt_fma((double)rv, (double)rv, acc);
+            for (int rb = r0 + BT_WAVE; rb < r1; rb += kBtRows * BT_NW) {
+                T q[kBtRows], rv[kBtRows];
+#pragma unroll
+                for (int u = 0; u < kBtRows; u++) {
+                    const int r = rb + u * BT_NW;
+                    const bool on = r < r1;
+                    q[u] = on ? Q[(size_t)r * k + j] : (T)0;
+                    rv[u] = on ? R[(size_t)r * k + j] : (T)0;
+                }
+#pragma unroll
+                for (int u = 0; u < kBtRows; u++) {
+                    const int r = rb + u * BT_NW;
+                    if (r >= r1) break;
+                    const T v = bt_fma(-a, q[u], rv[u]);
+                    R[(size_t)r * k + j] = v;
+                    acc = bt_fma((double)v, (double)v, acc);
+                }
             }
         }
         bt_tile_store(acc, k, jb, partial, sm);

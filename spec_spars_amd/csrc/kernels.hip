@@ -1174,7 +1174,7 @@ template void launch_accumulate<float>(hipStream_t, int64_t, const float *, doub
 template void launch_accumulate<double>(hipStream_t, int64_t, const double *, double *);
 
 void launch_copy_f64(hipStream_t s, int64_t count, const double *a, double *b) {
-    hipLaunchKernelGGL(k_copy_f64, dim3(grid_for(count, 256, 8192)), dim3(256), 0, s, count, a, b);
+    hipLaunchKernelGGL(k_copy_f64, dim3(grid_for(count, 256, 1 << 20)), dim3(256), 0, s, count, a, b);
 }
 
 void launch_fill_u32(hipStream_t s, int64_t count, uint32_t *p, uint32_t v) {
