@@ -48,36 +48,8 @@ struct ErSampleArgs {
 template <typename OutT>
 void launch_sketch(hipStream_t s, int32_t n, const int32_t *iptr, const int32_t *ieid, const double *ib, int k,
                    uint64_t seed, double scale, int uniform, int variant, OutT *Y);
-template <typename T>
-void launch_spmm(hipStream_t s, int32_t n, const int32_t *rowptr, const int32_t *col, const T *val, int k,
-                 const T *P, T *out);
-void launch_residual64(hipStream_t s, int32_t n, const int32_t *rowptr, const int32_t *col, const double *val,
-                       int k, const double *X, const double *B, double *out);
-int partial_blocks(int64_t n);
-template <typename TA, typename TB>
-void launch_col_partial(hipStream_t s, int32_t n, int k, const TA *A, const TB *B, int op, double *partial);
-void launch_pcg_init(hipStream_t s, int k, const double *pbb, int nb_bb, const double *prz, int nb_rz,
-                     PcgScalars sc);
-void launch_pcg_alpha(hipStream_t s, int k, const double *ppq, int nb, PcgScalars sc, int *nactive);
-void launch_pcg_check(hipStream_t s, int k, const double *prr, int nb, int rule, double tol, PcgScalars sc,
-                      int *nactive);
-void launch_pcg_beta(hipStream_t s, int k, const double *prz, int nb, PcgScalars sc);
-void launch_col_mean(hipStream_t s, int k, const double *partial, int nb, int64_t n, double *mean);
-template <typename T>
-void launch_project(hipStream_t s, int64_t nk, int k, const T *Y, const double *mean, T *R, T *B, T *X);
-template <typename T>
-void launch_update_xr(hipStream_t s, int64_t nk, int k, const double *alpha, const T *P, const T *Q, T *X, T *R);
-template <typename T>
-void launch_update_p(hipStream_t s, int64_t nk, int k, const double *beta, const T *Z, T *P);
-template <typename TS, typename TD>
-void launch_convert(hipStream_t s, int64_t count, const TS *src, TD *dst);
-template <typename T>
-void launch_accumulate(hipStream_t s, int64_t count, const T *d, double *x);
 void launch_fill_u32(hipStream_t s, int64_t count, uint32_t *p, uint32_t v);
 void launch_copy_f64(hipStream_t s, int64_t count, const double *a, double *b);
-// Upper bound of the block count of every kernel that writes per-block
-// partials (partial buffers hold kMaxPartialBlocks * k doubles).
-static const int kMaxPartialBlocks = 4096;
 // Rows of one level, positions [t0, t1): the first nlong of them take a
 // block of `waves` wavefronts each (4, 8 or 16), the next nmed one wave
 // each, the rest (at most kTinyRow entries each) go rpw to a wave.
@@ -87,9 +59,6 @@ void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nlong, int32_t nme
                            int xcd,
                            const int32_t *rows, const int32_t *ptr, const int32_t *col, const T *val,
                            const double *diag, int k, const T *in, T *out);
-template <typename T>
-void launch_spmm_dot(hipStream_t s, int32_t n, const int32_t *rowptr, const int32_t *col, const T *val, int k,
-                     const T *P, T *Q, double *partial);
 // Small levels [l0, l1) (each of at most kSmallRows rows) walked by one
 // workgroup; level l holds positions [level_ptr[l], level_ptr[l+1]).
 template <typename T, bool BWD>
@@ -112,14 +81,6 @@ template <typename T, bool BWD>
 void launch_core_step(hipStream_t s, int step, int graphs, int max_rows, const int32_t *core_n,
                       const int32_t *core_off, const int64_t *dense_off, const int32_t *vert, const T *D, int k,
                       T *out);
-template <typename T>
-void launch_ydy_partial(hipStream_t s, int32_t n, int k, const T *Yv, const double *diag, double *partial);
-template <typename T>
-void launch_spmm_pq(hipStream_t s, int32_t n, const int32_t *rowptr, const int32_t *col, const T *val, int k,
-                    const double *beta, const T *Z, const T *Pold, T *Pnew, T *Q, double *partial);
-template <typename T>
-void launch_update_xr_rr(hipStream_t s, int32_t n, int k, const double *alpha, const T *P, const T *Q, T *X, T *R,
-                         double *partial);
 template <typename TZ>
 void launch_er_sample(hipStream_t s, const ErSampleArgs<TZ> &a);
 

@@ -1,10 +1,11 @@
-// gfx950 kernels of the sparsification hot path.
+// gfx950 kernels of the sparsification hot path: the single-graph sketch
+// and sampling kernels behind ss_sketch / ss_er_sample, and the
+// preconditioner's triangular solves (level launches, small-level runs,
+// dense cores) shared by every resident graph and batch.
 //
-// Dense n x k blocks are row-major (node i's k values contiguous); every
-// row kernel maps one 64-lane wavefront to one node and its lanes to the k
-// columns, so a node's k-vector moves as one coalesced 64*s-byte access.
-// Reductions over nodes are per-block partials in a fixed order followed by
-// a one-block fixed-order sum: results do not depend on timing.
+// Dense n x k blocks are row-major (node i's k values contiguous); a row's
+// k values sit on the 64 lanes of a wavefront (64-column chunks for larger
+// k), so a node's k-vector moves as one coalesced 64*s-byte access.
 #include <hip/hip_runtime.h>
 
 #include "hash_dev.h"
@@ -76,326 +77,6 @@ __global__ __launch_bounds__(256) void k_sketch(int32_t n, const int32_t *__rest
     }
 }
 
-// ---------------------------------------------------------------------------
-// (b) Laplacian SpMM  T = A * P  (A full symmetric CSR with diagonal).
-// ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(256) void k_spmm(int32_t n, const int32_t *__restrict__ rowptr,
-                                              const int32_t *__restrict__ col,
-                                              const T *__restrict__ val, int k,
-                                              const T *__restrict__ P, T *__restrict__ out) {
-    const int row = wave_uniform(blockIdx.x * 4 + SS_WAVE_IN_BLOCK);
-    if (row >= n) return;
-    const int q0 = rowptr[row], q1 = rowptr[row + 1];
-    for (int jb = 0; jb < k; jb += 64) {
-        const int j = jb + SS_LANE;
-        if (j >= k) break;
-        T acc = 0;
-        for (int q = q0; q < q1; q++) acc += val[q] * P[(size_t)col[q] * k + j];
-        out[(size_t)row * k + j] = acc;
-    }
-}
-
-// Fused PCG step 1: P_new = Z + beta P_old (beta == 0 takes Z alone),
-// Q = A P_new gathered as A Z + beta A P_old row by row, and per-block
-// partials of P_new . Q.  A fixed grid walks rows wave by wave so the
-// partials stay deterministic.
-template <typename T>
-__global__ __launch_bounds__(256) void k_spmm_pq(int32_t n, const int32_t *__restrict__ rowptr,
-                                                 const int32_t *__restrict__ col, const T *__restrict__ val,
-                                                 int k, const double *__restrict__ beta,
-                                                 const T *__restrict__ Z, const T *__restrict__ Pold,
-                                                 T *__restrict__ Pnew, T *__restrict__ Q,
-                                                 double *__restrict__ partial) {
-    __shared__ double sm[4][64];
-    const int w = SS_WAVE_IN_BLOCK, lane = SS_LANE;
-    for (int jb = 0; jb < k; jb += 64) {
-        const int j = jb + lane;
-        const bool on = j < k;
-        const T b = on ? (T)beta[j] : (T)0;
-        const bool useb = b != (T)0;
-        double acc = 0.0;
-        for (int row = blockIdx.x * 4 + w; row < n; row += gridDim.x * 4) {
-            const int r = wave_uniform(row);
-            const int q0 = rowptr[r], q1 = rowptr[r + 1];
-            if (on) {
-                T qv = 0;
-                for (int q = q0; q < q1; q++) {
-                    const size_t o = (size_t)col[q] * k + j;
-                    T pc = Z[o];
-                    if (useb) pc += b * Pold[o];
-                    qv += val[q] * pc;
-                }
-                const size_t o = (size_t)r * k + j;
-                T pn = Z[o];
-                if (useb) pn += b * Pold[o];
-                Pnew[o] = pn;
-                Q[o] = qv;
-                acc += (double)pn * (double)qv;
-            }
-        }
-        sm[w][lane] = acc;
-        __syncthreads();
-        if (w == 0 && on)
-            partial[(size_t)blockIdx.x * k + j] = ((sm[0][lane] + sm[1][lane]) + sm[2][lane]) + sm[3][lane];
-        __syncthreads();
-    }
-}
-
-// PCG: Q = A P and per-block partials of P . Q (P[i] is read with the row).
-template <typename T>
-__global__ __launch_bounds__(256) void k_spmm_dot(int32_t n, const int32_t *__restrict__ rowptr,
-                                                  const int32_t *__restrict__ col, const T *__restrict__ val, int k,
-                                                  const T *__restrict__ P, T *__restrict__ Q,
-                                                  double *__restrict__ partial) {
-    __shared__ double sm[4][64];
-    const int w = SS_WAVE_IN_BLOCK, lane = SS_LANE;
-    for (int jb = 0; jb < k; jb += 64) {
-        const int j = jb + lane;
-        const bool on = j < k;
-        double acc = 0.0;
-        for (int row = blockIdx.x * 4 + w; row < n; row += gridDim.x * 4) {
-            const int r = wave_uniform(row);
-            const int q0 = rowptr[r], q1 = rowptr[r + 1];
-            if (on) {
-                T qv = 0;
-                qv = -row_dot_sub<T>(qv, q0, q1, 1, col, val, P, k, j);
-                const size_t o = (size_t)r * k + j;
-                Q[o] = qv;
-                acc += (double)P[o] * (double)qv;
-            }
-        }
-        sm[w][lane] = acc;
-        __syncthreads();
-        if (w == 0 && on)
-            partial[(size_t)blockIdx.x * k + j] = ((sm[0][lane] + sm[1][lane]) + sm[2][lane]) + sm[3][lane];
-        __syncthreads();
-    }
-}
-
-// Fused PCG step 2: X += alpha P, R -= alpha Q, per-block partials of R . R.
-template <typename T>
-__global__ __launch_bounds__(256) void k_update_xr_rr(int32_t n, int k, const double *__restrict__ alpha,
-                                                      const T *__restrict__ P, const T *__restrict__ Q,
-                                                      T *__restrict__ X, T *__restrict__ R,
-                                                      double *__restrict__ partial) {
-    __shared__ double sm[4][64];
-    const int w = SS_WAVE_IN_BLOCK, lane = SS_LANE;
-    for (int jb = 0; jb < k; jb += 64) {
-        const int j = jb + lane;
-        const bool on = j < k;
-        const T a = on ? (T)alpha[j] : (T)0;
-        double acc = 0.0;
-        if (on) {
-            for (int row = blockIdx.x * 4 + w; row < n; row += gridDim.x * 4) {
-                const size_t o = (size_t)row * k + j;
-                X[o] += a * P[o];
-                const T r = R[o] - a * Q[o];
-                R[o] = r;
-                acc += (double)r * (double)r;
-            }
-        }
-        sm[w][lane] = acc;
-        __syncthreads();
-        if (w == 0 && on)
-            partial[(size_t)blockIdx.x * k + j] = ((sm[0][lane] + sm[1][lane]) + sm[2][lane]) + sm[3][lane];
-        __syncthreads();
-    }
-}
-
-// FP64 residual  out = B - A * X  (iterative refinement's true residual).
-template <typename T>
-__global__ __launch_bounds__(256) void k_residual64(int32_t n, const int32_t *__restrict__ rowptr,
-                                                    const int32_t *__restrict__ col,
-                                                    const double *__restrict__ val, int k,
-                                                    const double *__restrict__ X,
-                                                    const double *__restrict__ B,
-                                                    double *__restrict__ out) {
-    const int row = wave_uniform(blockIdx.x * 4 + SS_WAVE_IN_BLOCK);
-    if (row >= n) return;
-    const int q0 = rowptr[row], q1 = rowptr[row + 1];
-    for (int jb = 0; jb < k; jb += 64) {
-        const int j = jb + SS_LANE;
-        if (j >= k) break;
-        double acc = 0;
-        for (int q = q0; q < q1; q++) acc += val[q] * X[(size_t)col[q] * k + j];
-        out[(size_t)row * k + j] = B[(size_t)row * k + j] - acc;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Column reductions: partial[b*k + j] = sum over this block's rows of
-// f(A[i,j], B[i,j]); op 0: A*B, op 1: A (column sums).
-// ---------------------------------------------------------------------------
-template <typename TA, typename TB>
-__global__ __launch_bounds__(256) void k_col_partial(int32_t n, int k, const TA *__restrict__ A,
-                                                     const TB *__restrict__ B, int op,
-                                                     double *__restrict__ partial) {
-    __shared__ double sm[4][64];
-    const int w = SS_WAVE_IN_BLOCK, lane = SS_LANE;
-    for (int jb = 0; jb < k; jb += 64) {
-        const int j = jb + lane;
-        double acc = 0.0;
-        if (j < k) {
-            for (int64_t r = (int64_t)blockIdx.x * 4 + w; r < n; r += (int64_t)gridDim.x * 4) {
-                const double a = (double)A[(size_t)r * k + j];
-                acc += op == 0 ? a * (double)B[(size_t)r * k + j] : a;
-            }
-        }
-        sm[w][lane] = acc;
-        __syncthreads();
-        if (w == 0 && j < k)
-            partial[(size_t)blockIdx.x * k + j] = ((sm[0][lane] + sm[1][lane]) + sm[2][lane]) + sm[3][lane];
-        __syncthreads();
-    }
-}
-
-// Column j's sum of nb block partials, by block j of the launch: thread t
-// adds partials t, t+256, ... in order, then a fixed-shape tree over the
-// 256 threads.  Deterministic, and ~nb/256 dependent loads per thread.
-__device__ __forceinline__ double block_sum_partials(const double *partial, int nb, int k, int j, double *sm) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) s += partial[(size_t)b * k + j];
-    sm[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
-        __syncthreads();
-    }
-    const double r = sm[0];
-    __syncthreads();
-    return r;
-}
-
-// ---------------------------------------------------------------------------
-// PCG scalar steps: one block of 256 threads per column j (grid = k).
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_pcg_init(int k, const double *partial_bb, int nb_bb,
-                                                  const double *partial_rz, int nb_rz, PcgScalars s) {
-    __shared__ double sm[256];
-    const int j = blockIdx.x;
-    const double bb = block_sum_partials(partial_bb, nb_bb, k, j, sm);
-    const double rz = block_sum_partials(partial_rz, nb_rz, k, j, sm);
-    if (threadIdx.x == 0) {
-        s.bnorm[j] = sqrt(bb);
-        s.rz[j] = rz;
-        s.active[j] = bb > 0.0 ? 1 : 0;
-        s.iters[j] = 0;
-        s.alpha[j] = 0.0;
-        s.beta[j] = 0.0;
-    }
-}
-
-__global__ __launch_bounds__(256) void k_pcg_alpha(int k, const double *partial_pq, int nb, PcgScalars s,
-                                                   int *nactive) {
-    __shared__ double sm[256];
-    const int j = blockIdx.x;
-    const double pq = block_sum_partials(partial_pq, nb, k, j, sm);
-    if (threadIdx.x == 0) {
-        double a = 0.0;
-        if (s.active[j]) {
-            if (pq > 0.0) a = s.rz[j] / pq;
-            else s.active[j] = 0;
-        }
-        s.alpha[j] = a;
-        if (j == 0) *nactive = 0;      // k_pcg_check counts afresh
-    }
-}
-
-// After x += alpha p, r -= alpha q: count the update, apply the stop rule.
-__global__ __launch_bounds__(256) void k_pcg_check(int k, const double *partial_rr, int nb, int rule, double tol,
-                                                   PcgScalars s, int *nactive) {
-    __shared__ double sm[256];
-    const int j = blockIdx.x;
-    const double rr = block_sum_partials(partial_rr, nb, k, j, sm);
-    if (threadIdx.x == 0) {
-        s.rr[j] = rr;
-        if (s.active[j]) {
-            s.iters[j] += 1;
-            const double rn = sqrt(rr);
-            const bool stop = rule == 0 ? (rn <= sqrt(tol)) : (rn <= tol * s.bnorm[j]);
-            if (stop) s.active[j] = 0;
-        }
-        if (s.active[j]) atomicAdd(nactive, 1);
-    }
-}
-
-__global__ __launch_bounds__(256) void k_pcg_beta(int k, const double *partial_rz, int nb, PcgScalars s) {
-    __shared__ double sm[256];
-    const int j = blockIdx.x;
-    const double rzn = block_sum_partials(partial_rz, nb, k, j, sm);
-    if (threadIdx.x == 0) {
-        double b = 0.0;
-        if (s.active[j] && s.rz[j] != 0.0) b = rzn / s.rz[j];
-        s.beta[j] = b;
-        s.rz[j] = rzn;
-    }
-}
-
-// mean[j] = colsum[j] / n   (n = 1 gives plain sums)
-__global__ __launch_bounds__(256) void k_col_mean(int k, const double *partial, int nb, int64_t n, double *mean) {
-    __shared__ double sm[256];
-    const int j = blockIdx.x;
-    const double t = block_sum_partials(partial, nb, k, j, sm);
-    if (threadIdx.x == 0) mean[j] = t / (double)n;
-}
-
-// ---------------------------------------------------------------------------
-// Elementwise PCG updates (grid-stride over n*k).
-// ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(256) void k_project(int64_t nk, int k, const T *__restrict__ Y,
-                                                 const double *__restrict__ mean,
-                                                 T *__restrict__ R, T *__restrict__ B,
-                                                 T *__restrict__ X) {
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nk;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const int j = (int)(t % k);
-        const T r = (T)((double)Y[t] - mean[j]);
-        R[t] = r;
-        B[t] = r;
-        X[t] = 0;
-    }
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_update_xr(int64_t nk, int k, const double *__restrict__ alpha,
-                                                   const T *__restrict__ P, const T *__restrict__ Q,
-                                                   T *__restrict__ X, T *__restrict__ R) {
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nk;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const T a = (T)alpha[t % k];
-        X[t] += a * P[t];
-        R[t] -= a * Q[t];
-    }
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_update_p(int64_t nk, int k, const double *__restrict__ beta,
-                                                  const T *__restrict__ Z, T *__restrict__ P) {
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nk;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const T b = (T)beta[t % k];
-        P[t] = b == (T)0 ? Z[t] : Z[t] + b * P[t];
-    }
-}
-
-template <typename TS, typename TD>
-__global__ __launch_bounds__(256) void k_convert(int64_t count, const TS *__restrict__ s, TD *__restrict__ d) {
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < count;
-         t += (int64_t)gridDim.x * blockDim.x)
-        d[t] = (TD)s[t];
-}
-
-// X64 += D (refinement accumulation)
-template <typename T>
-__global__ __launch_bounds__(256) void k_accumulate(int64_t count, const T *__restrict__ d,
-                                                    double *__restrict__ x) {
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < count;
-         t += (int64_t)gridDim.x * blockDim.x)
-        x[t] += (double)d[t];
-}
-
 __global__ __launch_bounds__(256) void k_copy_f64(int64_t count, const double *__restrict__ a,
                                                   double *__restrict__ b) {
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < count;
@@ -413,7 +94,7 @@ __global__ void k_fill_u32(int64_t count, uint32_t *p, uint32_t v) {
 // (c) level-scheduled triangular solves: one launch per level.
 // Schedule positions [t0, t1) hold the rows of one level (sorted by level,
 // so every row they read was finished by an earlier launch on the stream);
-// one wavefront per row, lanes over the k columns.
+// lanes over the k columns, a row's arithmetic fixed by solve_shape.h.
 //   forward  (BWD=false): y_i = r_i - sum_{j<i} L[i,j] y_j
 //   backward (BWD=true):  z_i = y_i / D_i - sum_{j>i} L[j,i] z_j
 //                         (division skipped where D_i == 0, as
@@ -925,31 +606,6 @@ __global__ __launch_bounds__(1024) void k_tri_small(int32_t l0, int32_t l1, cons
     }
 }
 
-// Per-block partials of y^T D^+ y (= r . M^{-1} r) from the forward result y.
-template <typename T>
-__global__ __launch_bounds__(256) void k_ydy_partial(int32_t n, int k, const T *__restrict__ Yv,
-                                                     const double *__restrict__ diag,
-                                                     double *__restrict__ partial) {
-    __shared__ double sm[4][64];
-    const int w = SS_WAVE_IN_BLOCK, lane = SS_LANE;
-    for (int jb = 0; jb < k; jb += 64) {
-        const int j = jb + lane;
-        double acc = 0.0;
-        if (j < k) {
-            for (int64_t r = (int64_t)blockIdx.x * 4 + w; r < n; r += (int64_t)gridDim.x * 4) {
-                const double y = (double)Yv[(size_t)r * k + j];
-                const double d = diag[r];
-                acc += d != 0.0 ? y * (y / d) : y * y;
-            }
-        }
-        sm[w][lane] = acc;
-        __syncthreads();
-        if (w == 0 && j < k)
-            partial[(size_t)blockIdx.x * k + j] = ((sm[0][lane] + sm[1][lane]) + sm[2][lane]) + sm[3][lane];
-        __syncthreads();
-    }
-}
-
 // ---------------------------------------------------------------------------
 // (d) fused effective resistance + probability + coin + sample + compaction.
 // One thread per edge (canonical order), 256 edges per tile.  Tiles take a
@@ -1085,94 +741,6 @@ template void launch_sketch<double>(hipStream_t, int32_t, const int32_t *, const
 template void launch_sketch<float>(hipStream_t, int32_t, const int32_t *, const int32_t *, const double *,
                                    int, uint64_t, double, int, int, float *);
 
-template <typename T>
-void launch_spmm(hipStream_t s, int32_t n, const int32_t *rowptr, const int32_t *col, const T *val, int k,
-                 const T *P, T *out) {
-    hipLaunchKernelGGL(k_spmm<T>, dim3(grid_for(n, 4)), dim3(256), 0, s, n, rowptr, col, val, k, P, out);
-}
-template void launch_spmm<double>(hipStream_t, int32_t, const int32_t *, const int32_t *, const double *, int,
-                                  const double *, double *);
-template void launch_spmm<float>(hipStream_t, int32_t, const int32_t *, const int32_t *, const float *, int,
-                                 const float *, float *);
-
-void launch_residual64(hipStream_t s, int32_t n, const int32_t *rowptr, const int32_t *col, const double *val,
-                       int k, const double *X, const double *B, double *out) {
-    hipLaunchKernelGGL(k_residual64<double>, dim3(grid_for(n, 4)), dim3(256), 0, s, n, rowptr, col, val, k, X,
-                       B, out);
-}
-
-int partial_blocks(int64_t n) { return (int)grid_for(n, 64, 1024); }
-
-template <typename TA, typename TB>
-void launch_col_partial(hipStream_t s, int32_t n, int k, const TA *A, const TB *B, int op, double *partial) {
-    hipLaunchKernelGGL((k_col_partial<TA, TB>), dim3(partial_blocks(n)), dim3(256), 0, s, n, k, A, B, op,
-                       partial);
-}
-template void launch_col_partial<double, double>(hipStream_t, int32_t, int, const double *, const double *, int,
-                                                 double *);
-template void launch_col_partial<float, float>(hipStream_t, int32_t, int, const float *, const float *, int,
-                                               double *);
-
-void launch_pcg_init(hipStream_t s, int k, const double *pbb, int nb_bb, const double *prz, int nb_rz,
-                     PcgScalars sc) {
-    hipLaunchKernelGGL(k_pcg_init, dim3(k), dim3(256), 0, s, k, pbb, nb_bb, prz, nb_rz, sc);
-}
-void launch_pcg_alpha(hipStream_t s, int k, const double *ppq, int nb, PcgScalars sc, int *nactive) {
-    hipLaunchKernelGGL(k_pcg_alpha, dim3(k), dim3(256), 0, s, k, ppq, nb, sc, nactive);
-}
-void launch_pcg_check(hipStream_t s, int k, const double *prr, int nb, int rule, double tol, PcgScalars sc,
-                      int *nactive) {
-    hipLaunchKernelGGL(k_pcg_check, dim3(k), dim3(256), 0, s, k, prr, nb, rule, tol, sc, nactive);
-}
-void launch_pcg_beta(hipStream_t s, int k, const double *prz, int nb, PcgScalars sc) {
-    hipLaunchKernelGGL(k_pcg_beta, dim3(k), dim3(256), 0, s, k, prz, nb, sc);
-}
-void launch_col_mean(hipStream_t s, int k, const double *partial, int nb, int64_t n, double *mean) {
-    hipLaunchKernelGGL(k_col_mean, dim3(k), dim3(256), 0, s, k, partial, nb, n, mean);
-}
-
-template <typename T>
-void launch_project(hipStream_t s, int64_t nk, int k, const T *Y, const double *mean, T *R, T *B, T *X) {
-    hipLaunchKernelGGL(k_project<T>, dim3(grid_for(nk, 256, kStreamGrid)), dim3(256), 0, s, nk, k, Y, mean, R, B,
-                       X);
-}
-template void launch_project<double>(hipStream_t, int64_t, int, const double *, const double *, double *,
-                                     double *, double *);
-template void launch_project<float>(hipStream_t, int64_t, int, const float *, const double *, float *, float *,
-                                    float *);
-
-template <typename T>
-void launch_update_xr(hipStream_t s, int64_t nk, int k, const double *alpha, const T *P, const T *Q, T *X, T *R) {
-    hipLaunchKernelGGL(k_update_xr<T>, dim3(grid_for(nk, 256, kStreamGrid)), dim3(256), 0, s, nk, k, alpha, P, Q,
-                       X, R);
-}
-template void launch_update_xr<double>(hipStream_t, int64_t, int, const double *, const double *,
-                                       const double *, double *, double *);
-template void launch_update_xr<float>(hipStream_t, int64_t, int, const double *, const float *, const float *,
-                                      float *, float *);
-
-template <typename T>
-void launch_update_p(hipStream_t s, int64_t nk, int k, const double *beta, const T *Z, T *P) {
-    hipLaunchKernelGGL(k_update_p<T>, dim3(grid_for(nk, 256, kStreamGrid)), dim3(256), 0, s, nk, k, beta, Z, P);
-}
-template void launch_update_p<double>(hipStream_t, int64_t, int, const double *, const double *, double *);
-template void launch_update_p<float>(hipStream_t, int64_t, int, const double *, const float *, float *);
-
-template <typename TS, typename TD>
-void launch_convert(hipStream_t s, int64_t count, const TS *src, TD *dst) {
-    hipLaunchKernelGGL((k_convert<TS, TD>), dim3(grid_for(count, 256, kStreamGrid)), dim3(256), 0, s, count, src,
-                       dst);
-}
-template void launch_convert<double, float>(hipStream_t, int64_t, const double *, float *);
-template void launch_convert<float, double>(hipStream_t, int64_t, const float *, double *);
-
-template <typename T>
-void launch_accumulate(hipStream_t s, int64_t count, const T *d, double *x) {
-    hipLaunchKernelGGL(k_accumulate<T>, dim3(grid_for(count, 256, kStreamGrid)), dim3(256), 0, s, count, d, x);
-}
-template void launch_accumulate<float>(hipStream_t, int64_t, const float *, double *);
-template void launch_accumulate<double>(hipStream_t, int64_t, const double *, double *);
-
 void launch_copy_f64(hipStream_t s, int64_t count, const double *a, double *b) {
     hipLaunchKernelGGL(k_copy_f64, dim3(grid_for(count, 256, 1 << 20)), dim3(256), 0, s, count, a, b);
 }
@@ -1180,27 +748,6 @@ void launch_copy_f64(hipStream_t s, int64_t count, const double *a, double *b) {
 void launch_fill_u32(hipStream_t s, int64_t count, uint32_t *p, uint32_t v) {
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(count, 256, kStreamGrid)), dim3(256), 0, s, count, p, v);
 }
-
-template <typename T>
-void launch_spmm_pq(hipStream_t s, int32_t n, const int32_t *rowptr, const int32_t *col, const T *val, int k,
-                    const double *beta, const T *Z, const T *Pold, T *Pnew, T *Q, double *partial) {
-    hipLaunchKernelGGL(k_spmm_pq<T>, dim3(partial_blocks(n)), dim3(256), 0, s, n, rowptr, col, val, k, beta, Z,
-                       Pold, Pnew, Q, partial);
-}
-template void launch_spmm_pq<double>(hipStream_t, int32_t, const int32_t *, const int32_t *, const double *, int,
-                                     const double *, const double *, const double *, double *, double *, double *);
-template void launch_spmm_pq<float>(hipStream_t, int32_t, const int32_t *, const int32_t *, const float *, int,
-                                    const double *, const float *, const float *, float *, float *, double *);
-
-template <typename T>
-void launch_spmm_dot(hipStream_t s, int32_t n, const int32_t *rowptr, const int32_t *col, const T *val, int k,
-                     const T *P, T *Q, double *partial) {
-    hipLaunchKernelGGL(k_spmm_dot<T>, dim3(partial_blocks(n)), dim3(256), 0, s, n, rowptr, col, val, k, P, Q, partial);
-}
-template void launch_spmm_dot<double>(hipStream_t, int32_t, const int32_t *, const int32_t *, const double *, int,
-                                      const double *, double *, double *);
-template void launch_spmm_dot<float>(hipStream_t, int32_t, const int32_t *, const int32_t *, const float *, int,
-                                     const float *, float *, double *);
 
 template <typename T>
 void launch_core_head(hipStream_t s, int32_t t0, int32_t t1, const int32_t *rows, const int32_t *ptr,
@@ -1260,17 +807,6 @@ SS_INST_SMALL(double, true)
 SS_INST_SMALL(float, false)
 SS_INST_SMALL(float, true)
 
-template <typename T>
-void launch_update_xr_rr(hipStream_t s, int32_t n, int k, const double *alpha, const T *P, const T *Q, T *X, T *R,
-                         double *partial) {
-    hipLaunchKernelGGL(k_update_xr_rr<T>, dim3(partial_blocks(n)), dim3(256), 0, s, n, k, alpha, P, Q, X, R,
-                       partial);
-}
-template void launch_update_xr_rr<double>(hipStream_t, int32_t, int, const double *, const double *, const double *,
-                                          double *, double *, double *);
-template void launch_update_xr_rr<float>(hipStream_t, int32_t, int, const double *, const float *, const float *,
-                                         float *, float *, double *);
-
 template <typename T, bool BWD>
 void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nlong, int32_t nmed, int32_t t1, int waves, int rpw,
                            int xcd, const int32_t *rows, const int32_t *ptr, const int32_t *col, const T *val,
@@ -1306,12 +842,6 @@ SS_INST_TRI(double, true)
 SS_INST_TRI(float, false)
 SS_INST_TRI(float, true)
 
-template <typename T>
-void launch_ydy_partial(hipStream_t s, int32_t n, int k, const T *Yv, const double *diag, double *partial) {
-    hipLaunchKernelGGL(k_ydy_partial<T>, dim3(partial_blocks(n)), dim3(256), 0, s, n, k, Yv, diag, partial);
-}
-template void launch_ydy_partial<double>(hipStream_t, int32_t, int, const double *, const double *, double *);
-template void launch_ydy_partial<float>(hipStream_t, int32_t, int, const float *, const double *, double *);
 
 template <typename TZ>
 void launch_er_sample(hipStream_t s, const ErSampleArgs<TZ> &a) {
