@@ -65,17 +65,23 @@ __device__ __forceinline__ void bt_tile_store(double acc, int k, int jb, double 
 // ---------------------------------------------------------------------------
 // (a) sketch: block per row tile, wave per row, seed/scale of the tile's graph
 // ---------------------------------------------------------------------------
+constexpr int kSketchSplit = 128;   // blocks per row tile in the sketch
 __global__ __launch_bounds__(1024) void kb_sketch(Tiles rt, const int32_t *__restrict__ iptr,
                                                  const int32_t *__restrict__ ieid, const double *__restrict__ ib,
                                                  int k, const uint64_t *__restrict__ seeds,
                                                  const double *__restrict__ scales, int uniform, int variant,
                                                  double *__restrict__ Y) {
 #pragma clang fp contract(off)
-    const int t = blockIdx.x;
+    // kSketchSplit blocks per row tile, one row per wave: a row's cost is
+    // its degree, and the rows of a power-law graph's hubs often sit
+    // together, so rows spread over as many waves as there are rows
+    const int t = blockIdx.x / kSketchSplit, sb = blockIdx.x % kSketchSplit;
     const int g = rt.graph[t];
     const uint64_t seed = seeds[g];
     const double scale = scales[g];
-    const int r0 = rt.start[t], r1 = rt.start[t + 1];
+    const int per = (rt.start[t + 1] - rt.start[t] + kSketchSplit - 1) / kSketchSplit;
+    const int r0 = rt.start[t] + sb * per;
+    const int r1 = r0 + per < rt.start[t + 1] ? r0 + per : rt.start[t + 1];
     for (int row = r0 + BT_WAVE; row < r1; row += BT_NW) {
         const int r = bt_uniform(row);
         const int q0 = iptr[r], q1 = iptr[r + 1];
@@ -636,7 +642,7 @@ static inline unsigned bt_grid(int64_t items, int per) {
 
 void bt_sketch(hipStream_t s, Tiles rt, const int32_t *iptr, const int32_t *ieid, const double *ib, int k,
                const uint64_t *seeds, const double *scales, int uniform, double *Y) {
-    hipLaunchKernelGGL(kb_sketch, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, iptr, ieid, ib, k, seeds, scales, uniform, 0,
+    hipLaunchKernelGGL(kb_sketch, dim3(rt.count * kSketchSplit), dim3(64 * rt.waves), 0, s, rt, iptr, ieid, ib, k, seeds, scales, uniform, 0,
                        Y);
 }
 
