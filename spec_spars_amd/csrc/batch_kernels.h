@@ -9,6 +9,8 @@
 namespace ss {
 
 static const int kRowTile = 2048;    // rows per row tile
+static const int kSpmmTileNnz = 32768;  // SpMM tiles: matrix entries per tile, about
+static const int kSpmmMaxSplit = 64;    // SpMM tiles per row tile, at most
 static const int kEdgeTile = 256;    // edges per edge tile (one block)
 
 static const int kMaxTileWaves = 16;

@@ -98,6 +98,13 @@ struct ss_graph {
     // tiles
     int32_t *rt_start = nullptr, *rt_graph = nullptr, *et_start = nullptr, *et_graph = nullptr;
     int32_t *gtile = nullptr;        // device copy of rtile_off
+    // SpMM tiles: each row tile split into up to kSpmmMaxSplit tiles so that
+    // none holds much more than kSpmmTileNnz matrix entries (a power-law
+    // graph's hub rows sit in a few row tiles); per graph ranges on the host
+    // (stile_off) and the device (sgtile)
+    std::vector<int32_t> stile_off;
+    int nst = 0;
+    int32_t *st_start = nullptr, *st_graph = nullptr, *sgtile = nullptr;
     int32_t *edge_off_dev = nullptr; // device copy of edge_off (int32)
     // canonical edges (global vertex ids)
     int32_t *eu = nullptr, *ev = nullptr;

@@ -251,6 +251,27 @@ int batch_upload(ss_context *ctx, const std::vector<const HostGraph *> &graphs, 
     }
     comp_off[G] = (int32_t)comp_graph.size();
     edge_off32[G] = (int32_t)M;
+    // SpMM tiles: row tile t of nnz_t entries becomes ceil(nnz_t /
+    // kSpmmTileNnz) tiles (at most kSpmmMaxSplit) of equal row counts
+    std::vector<int32_t> st_start, st_graph;
+    g->stile_off.assign((size_t)G + 1, 0);
+    for (int i = 0; i < G; i++) {
+        g->stile_off[i] = (int32_t)st_start.size();
+        for (int32_t t = g->rtile_off[i]; t < g->rtile_off[i + 1]; t++) {
+            const int32_t a = rt_start[t], b = rt_start[t + 1];
+            const int64_t nnz_t = (int64_t)rowptr[b] - rowptr[a];
+            const int64_t split = std::min<int64_t>(kSpmmMaxSplit, std::max<int64_t>(1, (nnz_t + kSpmmTileNnz - 1) / kSpmmTileNnz));
+            const int32_t per = (int32_t)((b - a + split - 1) / split);
+            for (int32_t r = a; r < b; r += per) {
+                st_start.push_back(r);
+                st_graph.push_back(i);
+            }
+        }
+    }
+    g->stile_off[G] = (int32_t)st_start.size();
+    g->nst = (int)st_start.size();
+    st_start.push_back((int32_t)N);
+    if (st_graph.empty()) st_graph.push_back(0);
     g->ncomp = (int)comp_graph.size();
     g->nsmall = (int)small_id.size();
     std::vector<float> valf(val.begin(), val.end());
@@ -261,6 +282,9 @@ int batch_upload(ss_context *ctx, const std::vector<const HostGraph *> &graphs, 
     SS_TRY(A.alloc(&g->et_start, et_start.size()));
     SS_TRY(A.alloc(&g->et_graph, et_graph.size()));
     SS_TRY(A.alloc(&g->gtile, g->rtile_off.size()));
+    SS_TRY(A.alloc(&g->st_start, st_start.size()));
+    SS_TRY(A.alloc(&g->st_graph, st_graph.size()));
+    SS_TRY(A.alloc(&g->sgtile, g->stile_off.size()));
     SS_TRY(A.alloc(&g->edge_off_dev, edge_off32.size()));
     SS_TRY(A.alloc(&g->eu, M));
     SS_TRY(A.alloc(&g->ev, M));
@@ -299,6 +323,9 @@ int batch_upload(ss_context *ctx, const std::vector<const HostGraph *> &graphs, 
     SS_TRY(h2d(s, g->et_start, et_start.data(), et_start.size()));
     SS_TRY(h2d(s, g->et_graph, et_graph.data(), et_graph.size()));
     SS_TRY(h2d(s, g->gtile, g->rtile_off.data(), g->rtile_off.size()));
+    SS_TRY(h2d(s, g->st_start, st_start.data(), st_start.size()));
+    SS_TRY(h2d(s, g->st_graph, st_graph.data(), st_graph.size()));
+    SS_TRY(h2d(s, g->sgtile, g->stile_off.data(), g->stile_off.size()));
     SS_TRY(h2d(s, g->edge_off_dev, edge_off32.data(), edge_off32.size()));
     SS_TRY(h2d(s, g->eu, eu.data(), M));
     SS_TRY(h2d(s, g->ev, ev.data(), M));
@@ -557,7 +584,8 @@ size_t graph_work_bytes(const ss_graph *g, int k, int prec) {
     const size_t nk = (size_t)g->N * (size_t)cap, gk = (size_t)g->G * cap;
     size_t b = nk * 8 * 2;                                  // Y64, X64
     b += prec == SS_PREC_FP64 ? nk * 8 * 6 : nk * (8 * 2 + 4 * 2 + 4 * 6);
-    b += (size_t)std::max(g->nrt, 1) * cap * 8 * 2 + gk * 8 + (size_t)g->ncomp * cap * 8 * 2;
+    b += ((size_t)std::max(std::max(g->nrt, g->nst), 1) + (size_t)std::max(g->nrt, 1)) * cap * 8 + gk * 8 +
+         (size_t)g->ncomp * cap * 8 * 2;
     b += gk * (5 * 8 + 2 * 4 + 4);
     return b;
 }
@@ -607,7 +635,7 @@ int graph_ensure_work(ss_graph *g, int k, int prec) {
         SS_TRY(A.alloc(&g->X32, nk));
         SS_TRY(alloc_work(g->w32));
     }
-    SS_TRY(A.alloc(&g->partial, (size_t)std::max(g->nrt, 1) * cap));
+    SS_TRY(A.alloc(&g->partial, (size_t)std::max(std::max(g->nrt, g->nst), 1) * cap));
     SS_TRY(A.alloc(&g->partial2, (size_t)std::max(g->nrt, 1) * cap));
     SS_TRY(A.alloc(&g->gsum, gk));
     SS_TRY(A.alloc(&g->csum, (size_t)g->ncomp * cap));
@@ -636,6 +664,7 @@ template <> const double *avals<double>(const ss_graph *g) { return g->val; }
 template <> const float *avals<float>(const ss_graph *g) { return g->valf; }
 
 static Tiles rtiles(const ss_graph *g) { return Tiles{g->rt_start, g->rt_graph, g->nrt, g->tile_waves}; }
+static Tiles stiles(const ss_graph *g) { return Tiles{g->st_start, g->st_graph, g->nst, g->tile_waves}; }
 
 template <typename T> static const T *core_L(const ss_graph *g, bool transposed);
 template <> const double *core_L<double>(const ss_graph *g, bool t) { return t ? g->Ltd : g->Ld; }
@@ -748,9 +777,9 @@ static int pcg_iteration(ss_graph *g, int k, T *X, const PcgWork<T> &w, int rule
     }
     {
         ProfScope ps(g, SS_KERNEL_SPMM);
-        bt_spmm_dot<T>(s, rt, g->rowptr, g->col, avals<T>(g), k, w.P, w.Q, g->partial);   // Q = A P, P.Q
+        bt_spmm_dot<T>(s, stiles(g), g->rowptr, g->col, avals<T>(g), k, w.P, w.Q, g->partial);   // Q = A P, P.Q
     }
-    bt_pcg_alpha(s, gk, k, g->gtile, g->partial, g->sc, g->status);
+    bt_pcg_alpha(s, gk, k, g->sgtile, g->partial, g->sc, g->status);
     {
         ProfScope ps(g, SS_KERNEL_VECTOR);
         bt_update_r_rr<T>(s, rt, k, g->sc.alpha, w.Q, w.R, g->partial);
@@ -867,9 +896,9 @@ static int residual_norms(ss_graph *g, int k, const double *X, const double *B, 
     const int gk = g->G * k;
     {
         ProfScope ps(g, SS_KERNEL_SPMM);
-        bt_residual64(s, rtiles(g), g->rowptr, g->col, g->val, k, X, B, R, g->partial);
+        bt_residual64(s, stiles(g), g->rowptr, g->col, g->val, k, X, B, R, g->partial);
     }
-    bt_graph_sums(s, gk, k, g->gtile, g->partial, g->gsum);
+    bt_graph_sums(s, gk, k, g->sgtile, g->partial, g->gsum);
     rn2.resize(gk);
     SS_HIP(hipMemcpyAsync(rn2.data(), g->gsum, sizeof(double) * gk, hipMemcpyDeviceToHost, s));
     SS_HIP(hipStreamSynchronize(s));
@@ -991,7 +1020,7 @@ static int autotune_t(ss_graph *g, int k, const PcgWork<T> &w, int reps, ss_tuni
         double ms = 0;
         SS_TRY(time_median(g, reps, [&]() {
             bt_update_px<T>(s, rt, k, g->sc.alpha, g->sc.beta, w.Zp, w.P, w.B);
-            bt_spmm_dot<T>(s, rt, g->rowptr, g->col, avals<T>(g), k, w.P, w.Q, g->partial);
+            bt_spmm_dot<T>(s, stiles(g), g->rowptr, g->col, avals<T>(g), k, w.P, w.Q, g->partial);
             bt_update_r_rr<T>(s, rt, k, g->sc.alpha, w.Q, w.W, g->partial);
             SS_HIP(hipGetLastError());
             return SS_OK;
