@@ -8,7 +8,17 @@
 
 namespace ss {
 
-static const int kRowTile = 2048;    // rows per row tile
+static const int kRowTile = 2048;    // rows per row tile, at most
+static const int kMinRowTile = 64;   // rows per row tile, at least
+// Rows per row tile of a graph of n rows: about n / 256 (a power of two in
+// [kMinRowTile, kRowTile]), so a small graph still spreads its streaming and
+// reduction kernels over many workgroups.  A function of n alone: a graph's
+// tiles, and so its reductions' shape, are the same alone and in a batch.
+inline int64_t row_tile_rows(int64_t n) {
+    int64_t r = kMinRowTile;
+    while (r < kRowTile && r * 256 < n) r *= 2;
+    return r;
+}
 static const int kSpmmTileNnz = 32768;  // SpMM tiles: matrix entries per tile, about
 static const int kSpmmMaxSplit = 64;    // SpMM tiles per row tile, at most
 static const int kEdgeTile = 256;    // edges per edge tile (one block)

@@ -2,7 +2,7 @@
 //
 // A batch packs G graphs into one block-diagonal system: graph g owns rows
 // [row_off[g], row_off[g+1]) and edges [edge_off[g], edge_off[g+1]).  Rows
-// are cut into tiles of kRowTile rows and edges into tiles of kEdgeTile
+// are cut into tiles of row_tile_rows(n) rows and edges into tiles of kEdgeTile
 // edges, never straddling two graphs; tile t knows its first row/edge and
 // its graph.  Every reduction is a per-tile partial followed by a per
 // (graph, column) sum over that graph's tiles in a fixed shape, so a graph's

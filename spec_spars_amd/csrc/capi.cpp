@@ -237,7 +237,7 @@ int ss_graph_destroy(ss_graph *g) {
 int ss_graph_bytes(int64_t n, int64_t m, int64_t factor_nnz, int32_t k, int32_t precision, size_t *bytes) {
     if (!bytes || n <= 0 || m < 0 || k < 1) return fail(SS_ERR_INVALID, "bad arguments");
     const size_t N = (size_t)n, M = (size_t)m, nnzL = N + 2 * M, F = (size_t)std::max<int64_t>(factor_nnz, n);
-    const size_t rt = (N + kRowTile - 1) / kRowTile + 1, et = (M + kEdgeTile - 1) / kEdgeTile + 1;
+    const size_t rt = (N + kMinRowTile - 1) / kMinRowTile + 1, et = (M + kEdgeTile - 1) / kEdgeTile + 1;
     size_t b = 0;
     b += M * (4 + 4 + 8);                      // canonical edges
     b += (N + 1) * 4 + nnzL * (4 + 8 + 4);     // Laplacian CSR (FP64 + FP32 values)

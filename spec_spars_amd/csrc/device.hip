@@ -162,7 +162,8 @@ int batch_upload(ss_context *ctx, const std::vector<const HostGraph *> &graphs, 
     for (int i = 0; i < G; i++) {
         g->row_off[i + 1] = g->row_off[i] + graphs[i]->n;
         g->edge_off[i + 1] = g->edge_off[i] + graphs[i]->m;
-        const int64_t rt = g->rtile_off[i] + (graphs[i]->n + kRowTile - 1) / kRowTile;
+        const int64_t tr = row_tile_rows(graphs[i]->n);
+        const int64_t rt = g->rtile_off[i] + (graphs[i]->n + tr - 1) / tr;
         const int64_t et = g->etile_off[i] + (graphs[i]->m + kEdgeTile - 1) / kEdgeTile;
         if (rt >= INT32_MAX || et >= INT32_MAX) return fail(SS_ERR_CAPACITY, "batch: too many tiles");
         g->rtile_off[i + 1] = (int32_t)rt;
@@ -181,7 +182,7 @@ int batch_upload(ss_context *ctx, const std::vector<const HostGraph *> &graphs, 
         et_start((size_t)g->net + 1), et_graph((size_t)std::max(g->net, 1));
     for (int i = 0; i < G; i++) {
         for (int32_t t = g->rtile_off[i]; t < g->rtile_off[i + 1]; t++) {
-            rt_start[t] = (int32_t)(g->row_off[i] + (int64_t)(t - g->rtile_off[i]) * kRowTile);
+            rt_start[t] = (int32_t)(g->row_off[i] + (int64_t)(t - g->rtile_off[i]) * row_tile_rows(graphs[i]->n));
             rt_graph[t] = i;
         }
         for (int32_t t = g->etile_off[i]; t < g->etile_off[i + 1]; t++) {
