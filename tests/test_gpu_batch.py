@@ -146,6 +146,32 @@ def test_wave_rows_and_blocks_bit_identical(ctx, prec):
         assert rr.max() <= 1.01e-8, i
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_hub_tiles_bit_identical(ctx, oracle_mod, prec):
+    """A power-law graph whose first row tile holds 58K matrix entries (its
+    SpMM tile is split) and whose hub rows make the sketch's heaviest rows,
+    alone and mixed with a grid of another tile size: the same bits, and the
+    residual at the tolerance."""
+    import spec_spars_amd as ss
+    from spec_spars_amd import graphs
+    n, u, v = graphs.chung_lu(20000, 200000, gamma=2.2, seed=3)
+    m, gu, gv = graphs.grid2d(40, 30)
+    gs = [(n, u, v, graphs.uniform_weights(len(u), seed=3)), (m, gu, gv, graphs.loguniform_weights(len(gu), seed=6))]
+    fs = _factors(gs, seed=11, ordering=ss.ORDER_ROUNDS)
+    mixed = _run(ctx, gs, fs, [0, 1], 16, prec)
+    for i in range(2):
+        alone = _run(ctx, gs, fs, [i], 16, prec)[0]
+        for key in ("Y", "Z", "p"):
+            assert np.array_equal(alone[key], mixed[i][key]), (i, key)
+        nn, uu, vv, ww = gs[i]
+        Yo = oracle_mod.sketch(uu, vv, ww, nn, 16, seed=10 + i, scale=1.0 / math.sqrt(16))
+        assert np.array_equal(mixed[i]["Y"], Yo), i
+        L = _laplacian(nn, uu, vv, ww)
+        Yp = _component_projection(nn, uu, vv, mixed[i]["Y"])
+        rr = np.linalg.norm(Yp - L @ mixed[i]["Z"], axis=0) / np.linalg.norm(Yp, axis=0)
+        assert rr.max() <= 1.01e-8, i
+
+
 def test_batch_matches_oracle_per_graph(ctx, oracle_mod):
     import spec_spars_amd as ss
     from spec_spars_amd import graphs
