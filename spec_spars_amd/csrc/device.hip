@@ -521,7 +521,8 @@ int batch_upload_factors(ss_graph *g, const std::vector<const Factor *> &factors
     g->core_steps = (g->core_max + kCoreBlock - 1) / kCoreBlock;
     g->core_total = (int64_t)core_vert.size();
     g->core_entries = dense_total;
-    // a level's long rows get as many waves as their longest row has parts (4..16)
+    // a level's long rows get as many waves as their longest row has parts
+    // (4..16); 0 marks a level without entries
     for (int dir = 0; dir < 2; dir++) {
         const Schedule &S = dir == 0 ? fw : bw;
         std::vector<int32_t> &out = dir == 0 ? g->f_level_waves : g->b_level_waves;
@@ -531,6 +532,7 @@ int batch_upload_factors(ss_graph *g, const std::vector<const Factor *> &factors
             for (int64_t t = S.level_ptr[l]; t < S.level_ptr[l] + S.level_long[l]; t++)
                 maxlen = std::max(maxlen, S.ptr[t + 1] - S.ptr[t]);
             out[l] = std::min(kLevelWaves, std::max(4, seg_parts(maxlen)));
+            if (S.ptr[S.level_ptr[l + 1]] == S.ptr[S.level_ptr[l]]) out[l] = 0;   // no entries: a copy launch
         }
     }
     for (int dir = 0; dir < 2; dir++) {

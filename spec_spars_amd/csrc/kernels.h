@@ -53,6 +53,7 @@ void launch_copy_f64(hipStream_t s, int64_t count, const double *a, double *b);
 // Rows of one level, positions [t0, t1): the first nlong of them take a
 // block of `waves` wavefronts each (4, 8 or 16), the next nmed one wave
 // each, the rest (at most kTinyRow entries each) go rpw to a wave.
+// waves == 0: no row of the level has entries (a flat copy of start values).
 // xcd != 0: each of the 8 XCDs takes one contiguous slice of the wave rows.
 template <typename T, bool BWD>
 void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nlong, int32_t nmed, int32_t t1, int waves, int rpw,

@@ -319,6 +319,21 @@ __device__ __forceinline__ void wave_tiny_rows(int tb, int cnt, const int32_t *_
     }
 }
 
+// A level whose rows have no entries (the first forward level: the rows
+// that depend on nothing): every row is its start value.  One thread per
+// (row, column), a flat streaming copy.
+template <typename T, bool BWD>
+__global__ __launch_bounds__(256) void k_level_copy(int32_t t0, int64_t count, const int32_t *__restrict__ rows,
+                                                    const double *__restrict__ diag, int k,
+                                                    const T *__restrict__ in, T *__restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= count) return;
+    const int64_t t = e / k;
+    const int j = (int)(e - t * k);
+    const int i = rows[t0 + t];
+    out[(size_t)i * k + j] = row_start<T, BWD>(i, j, k, BWD ? diag[i] : 1.0, in);
+}
+
 // One level of a triangular solve: schedule positions [t0, t1); the first
 // nlong of them take a block each (its W = blockDim / 64 waves share the
 // row's parts), the next nmed one wave each, the rest (tiny rows) rpw
@@ -812,6 +827,12 @@ void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nlong, int32_t nme
                            int xcd, const int32_t *rows, const int32_t *ptr, const int32_t *col, const T *val,
                            const double *diag, int k, const T *in, T *out) {
     if (t1 <= t0) return;
+    if (waves == 0) {   // a level of rows without entries
+        const int64_t count = (int64_t)(t1 - t0) * k;
+        hipLaunchKernelGGL((k_level_copy<T, BWD>), dim3(grid_for(count, 256)), dim3(256), 0, s, t0, count, rows, diag,
+                           k, in, out);
+        return;
+    }
     if (rpw < 1) rpw = 1;
     if (waves < 4 || waves > kLevelWaves || (waves & (waves - 1))) waves = 4;
     const int64_t tiny_rows = (int64_t)t1 - t0 - nlong - nmed;
