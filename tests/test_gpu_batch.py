@@ -155,6 +155,9 @@ def test_hub_tiles_bit_identical(ctx, oracle_mod, prec):
     import spec_spars_amd as ss
     from spec_spars_amd import graphs
     n, u, v = graphs.chung_lu(20000, 200000, gamma=2.2, seed=3)
+    # a path through every vertex makes it connected (the oracle solves
+    # connected graphs)
+    u, v, _ = graphs.canonical(np.concatenate([u, np.arange(n - 1)]), np.concatenate([v, np.arange(1, n)]))
     m, gu, gv = graphs.grid2d(40, 30)
     gs = [(n, u, v, graphs.uniform_weights(len(u), seed=3)), (m, gu, gv, graphs.loguniform_weights(len(gu), seed=6))]
     fs = _factors(gs, seed=11, ordering=ss.ORDER_ROUNDS)
@@ -170,6 +173,16 @@ def test_hub_tiles_bit_identical(ctx, oracle_mod, prec):
         Yp = _component_projection(nn, uu, vv, mixed[i]["Y"])
         rr = np.linalg.norm(Yp - L @ mixed[i]["Z"], axis=0) / np.linalg.norm(Yp, axis=0)
         assert rr.max() <= 1.01e-8, i
+        if i == 0 and prec == ss.PREC_MIXED:
+            # mixed precision: leverage scores within 1e-4 of the FP64 oracle
+            assert components(nn, uu, vv) == 1
+            cp, ri, va = graphs.laplacian_csc(nn, uu, vv, ww)
+            Zo, _, _ = oracle_mod.solve_lap(Yo, cp, ri, va, *fs[0].export(), stop_rule=1, tol=1e-10, max_iter=3000)
+            eu, ev, lv = oracle_mod.upper_edges(nn, cp, ri, va)
+            po = oracle_mod.probs(lv, oracle_mod.diff_norms(eu, ev, Zo), graphs.beta_for(nn, 0.3), 16)
+            both = (mixed[i]["p"] < 1) & (po < 1)
+            assert both.sum() > 1000
+            assert np.all(np.abs(mixed[i]["p"][both] - po[both]) <= 1e-4 * po[both])
 
 
 def test_batch_matches_oracle_per_graph(ctx, oracle_mod):
