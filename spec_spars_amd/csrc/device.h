@@ -135,6 +135,11 @@ struct ss_graph {
     std::vector<int32_t> f_level_ptr, b_level_ptr;   // host copies: launch bounds per level
     std::vector<int32_t> f_level_long, b_level_long; // long rows leading each level
     std::vector<int32_t> f_level_med, b_level_med;   // then rows taking one wave each
+    std::vector<int32_t> f_level_huge, b_level_huge; // the first long rows: more than kWgRowMax entries
+    std::vector<int32_t> f_level_hm, b_level_hm;     // workgroups per such row (the level's most)
+    int32_t core_huge = 0;                           // the same for the forward core rows
+    double *huge_scratch = nullptr;   // per long row of more than kWgRowMax entries: kMaxWg subtrees x kMaxK values
+    int *huge_counter = nullptr;      // per such row: workgroups arrived (zero between launches)
     std::vector<int32_t> f_level_waves, b_level_waves;   // wavefronts per block of each level launch
     std::vector<std::pair<int32_t, int32_t>> f_runs, b_runs; // runs of small levels (single-workgroup launches)
     int32_t *f_lp_dev = nullptr, *b_lp_dev = nullptr;   // level_ptr on the device

@@ -521,20 +521,38 @@ int batch_upload_factors(ss_graph *g, const std::vector<const Factor *> &factors
     g->core_steps = (g->core_max + kCoreBlock - 1) / kCoreBlock;
     g->core_total = (int64_t)core_vert.size();
     g->core_entries = dense_total;
-    // a level's long rows get as many waves as their longest row has parts
-    // (4..16); 0 marks a level without entries
+    // a level's long rows of up to kWgRowMax entries get a block of as many
+    // waves as their longest row has parts (4..16; 0 marks a level without
+    // entries); its rows of more than kWgRowMax entries get as many
+    // workgroups each as the longest of them needs, folding through scratch
+    // slots and counters sized for the most such rows one launch holds
+    int64_t nhuge = 1;
+    auto max_len = [](const Schedule &S, int64_t t0, int64_t t1) {
+        int32_t m = 0;
+        for (int64_t t = t0; t < t1; t++) m = std::max(m, S.ptr[t + 1] - S.ptr[t]);
+        return m;
+    };
     for (int dir = 0; dir < 2; dir++) {
         const Schedule &S = dir == 0 ? fw : bw;
         std::vector<int32_t> &out = dir == 0 ? g->f_level_waves : g->b_level_waves;
+        std::vector<int32_t> &hm = dir == 0 ? g->f_level_hm : g->b_level_hm;
         out.assign((size_t)S.depth, 4);
+        hm.assign((size_t)S.depth, 1);
         for (int32_t l = 0; l < S.depth; l++) {
-            int32_t maxlen = 0;
-            for (int64_t t = S.level_ptr[l]; t < S.level_ptr[l] + S.level_long[l]; t++)
-                maxlen = std::max(maxlen, S.ptr[t + 1] - S.ptr[t]);
-            out[l] = std::min(kLevelWaves, std::max(4, seg_parts(maxlen)));
+            const int64_t t0 = S.level_ptr[l], th = t0 + S.level_huge[l];
+            out[l] = std::min(kLevelWaves, std::max(4, seg_parts(max_len(S, th, t0 + S.level_long[l]))));
+            hm[l] = row_wgs(max_len(S, t0, th));
+            nhuge = std::max<int64_t>(nhuge, S.level_huge[l]);
             if (S.ptr[S.level_ptr[l + 1]] == S.ptr[S.level_ptr[l]]) out[l] = 0;   // no entries: a copy launch
         }
     }
+    g->f_level_huge = fw.level_huge;
+    g->b_level_huge = bw.level_huge;
+    g->core_huge = fw.core_huge;
+    SS_TRY(A.alloc(&g->huge_counter, (size_t)nhuge));
+    SS_TRY(A.alloc(&g->huge_scratch, (size_t)nhuge * kMaxWg * kMaxK));
+    SS_HIP(hipMemsetAsync(g->huge_counter, 0, sizeof(int) * (size_t)nhuge, s));
+    SS_HIP(hipStreamSynchronize(s));
     for (int dir = 0; dir < 2; dir++) {
         const Schedule &S = dir == 0 ? fw : bw;
         ss_schedule_info &I = g->sched_info[dir];
@@ -598,7 +616,7 @@ size_t graph_work_bytes(const ss_graph *g, int k, int prec) {
 // stride), so a resident graph serving requests of different k keeps one
 // allocation.
 int graph_ensure_work(ss_graph *g, int k, int prec) {
-    if (k < 1 || k > 256) return fail(SS_ERR_INVALID, "k=%d outside 1..256", k);
+    if (k < 1 || k > kMaxK) return fail(SS_ERR_INVALID, "k=%d outside 1..%d", k, kMaxK);
     if (prec != SS_PREC_FP64 && prec != SS_PREC_MIXED) return fail(SS_ERR_INVALID, "unknown precision %d", prec);
     if (g->wprec == prec && k <= g->wcap) {
         g->wk = k;
@@ -682,6 +700,8 @@ static void solve_levels(ss_graph *g, int k, const T *in, T *out) {
     const std::vector<int32_t> &lp = BWD ? g->b_level_ptr : g->f_level_ptr;
     const std::vector<int32_t> &nlong = BWD ? g->b_level_long : g->f_level_long;
     const std::vector<int32_t> &nmed = BWD ? g->b_level_med : g->f_level_med;
+    const std::vector<int32_t> &nhuge = BWD ? g->b_level_huge : g->f_level_huge;
+    const std::vector<int32_t> &hms = BWD ? g->b_level_hm : g->f_level_hm;
     const std::vector<int32_t> &waves = BWD ? g->b_level_waves : g->f_level_waves;
     const auto &runs = BWD ? g->b_runs : g->f_runs;
     const int32_t *rows = BWD ? g->b_rows : g->f_rows, *ptr = BWD ? g->b_ptr : g->f_ptr;
@@ -701,8 +721,10 @@ static void solve_levels(ss_graph *g, int k, const T *in, T *out) {
         const int32_t e = r < runs.size() ? runs[r].first : depth;
         ProfScope ps(g, BWD ? SS_KERNEL_TRI_BWD : SS_KERNEL_TRI_FWD, e - l);
         for (; l < e; l++)
-            launch_trisolve_level<T, BWD>(s, lp[l], nlong[l], nmed[l], lp[l + 1], waves[l], rpw,
-                                          g->ctx->tune.xcd_slices, rows, ptr, col, val, g->diag, k, in, out);
+            launch_trisolve_level<T, BWD>(s, lp[l], nhuge[l], hms[l], nlong[l] - nhuge[l], nmed[l], lp[l + 1],
+                                          waves[l], rpw,
+                                          g->ctx->tune.xcd_slices, rows, ptr, col, val, g->diag, k, in, out,
+                                          reinterpret_cast<T *>(g->huge_scratch), g->huge_counter);
     }
 }
 
@@ -716,8 +738,8 @@ static void solve_cores(ss_graph *g, int k, const T *in, T *out) {
     if (BWD)
         launch_core_bwd_init<T>(s, (int32_t)g->core_total, g->core_vert, g->diag, k, in, out);
     else
-        launch_core_head<T>(s, g->f_core_t0, g->f_nrows, g->f_rows, g->f_ptr, g->f_col, fvals<T>(g), g->diag, k, in,
-                            out);
+        launch_core_head<T>(s, g->f_core_t0, g->core_huge, g->f_nrows, g->f_rows, g->f_ptr, g->f_col, fvals<T>(g),
+                            g->diag, k, in, out);
     for (int st = 0; st < g->core_steps; st++)
         launch_core_step<T, BWD>(s, st, g->G, g->core_max, g->core_n, g->core_off, g->dense_off, g->core_vert,
                                  core_L<T>(g, BWD), k, out);

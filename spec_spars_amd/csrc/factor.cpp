@@ -421,15 +421,19 @@ int factor_schedules(const Factor &F, const std::vector<char> &core, Schedule &f
         slot.assign((size_t)S.level_ptr[S.depth], 0);
         for (int64_t i = 0; i < n; i++)
             if (!core[i]) slot[pos[dense[lvl[i]]]++] = (int32_t)i;
-        // then the rows taking one wave each, then the tiny rows
+        // then the rows taking one wave each, then the tiny rows; the long
+        // rows of more than kWgRowMax entries (several workgroups each) lead
         S.level_long.assign((size_t)S.depth, 0);
+        S.level_huge.assign((size_t)S.depth, 0);
         S.level_med.assign((size_t)S.depth, 0);
         for (int32_t l = 0; l < S.depth; l++) {
             auto b = slot.begin() + S.level_ptr[l], e = slot.begin() + S.level_ptr[l + 1];
             const int32_t wave_max = e - b >= kWaveLevelRows ? std::max(long_row, (int32_t)kWaveRowMax) : long_row;
             auto mid = std::stable_partition(b, e, [&](int32_t p) { return len[p] > wave_max; });
             auto tiny = std::stable_partition(mid, e, [&](int32_t p) { return len[p] > kTinyRow; });
+            auto huge = std::stable_partition(b, mid, [&](int32_t p) { return len[p] > kWgRowMax; });
             S.level_long[l] = (int32_t)(mid - b);
+            S.level_huge[l] = (int32_t)(huge - b);
             S.level_med[l] = (int32_t)(tiny - mid);
         }
     };
@@ -441,6 +445,9 @@ int factor_schedules(const Factor &F, const std::vector<char> &core, Schedule &f
     fwd.core_t0 = (int32_t)fslot.size();
     for (int64_t i = 0; i < n; i++)
         if (core[i]) fslot.push_back((int32_t)i);
+    fwd.core_huge = (int32_t)(std::stable_partition(fslot.begin() + fwd.core_t0, fslot.end(),
+                                                    [&](int32_t p) { return flen[p] > kWgRowMax; }) -
+                              (fslot.begin() + fwd.core_t0));
     bwd.core_t0 = (int32_t)bslot.size();
 
     auto pack = [&](Schedule &S, const std::vector<int32_t> &slot, const std::vector<int32_t> &lvl,
@@ -479,9 +486,15 @@ int factor_schedules(const Factor &F, const std::vector<char> &core, Schedule &f
 
     // every maximal run of at least two small levels goes to one workgroup
     for (Schedule *S : {&fwd, &bwd}) {
+        // (and no row of more than kWgRowMax entries: the workgroup folds at
+        // most kWgParts parts of a row)
         auto small = [&](int32_t l) {
-            return S->level_ptr[l + 1] - S->level_ptr[l] <= small_rows &&
-                   S->ptr[S->level_ptr[l + 1]] - S->ptr[S->level_ptr[l]] <= kSmallNnz;
+            if (S->level_ptr[l + 1] - S->level_ptr[l] > small_rows ||
+                S->ptr[S->level_ptr[l + 1]] - S->ptr[S->level_ptr[l]] > kSmallNnz)
+                return false;
+            for (int32_t t = S->level_ptr[l]; t < S->level_ptr[l + 1]; t++)
+                if (S->ptr[t + 1] - S->ptr[t] > kWgRowMax) return false;
+            return true;
         };
         S->runs.clear();
         for (int32_t l = 0; l < S->depth;) {

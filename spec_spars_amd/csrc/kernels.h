@@ -50,16 +50,19 @@ void launch_sketch(hipStream_t s, int32_t n, const int32_t *iptr, const int32_t 
                    uint64_t seed, double scale, int uniform, int variant, OutT *Y);
 void launch_fill_u32(hipStream_t s, int64_t count, uint32_t *p, uint32_t v);
 void launch_copy_f64(hipStream_t s, int64_t count, const double *a, double *b);
-// Rows of one level, positions [t0, t1): the first nlong of them take a
-// block of `waves` wavefronts each (4, 8 or 16), the next nmed one wave
-// each, the rest (at most kTinyRow entries each) go rpw to a wave.
+// Rows of one level, positions [t0, t1): the first nhuge of them have more
+// than kWgRowMax entries and take hm workgroups each, folding their parts
+// across workgroups through scratch (nhuge x kMaxWg x k values) and one
+// zeroed counter per row; the next nlong take a block of `waves`
+// wavefronts each (4, 8 or 16), the next nmed one wave each, the rest (at
+// most kTinyRow entries each) go rpw to a wave.
 // waves == 0: no row of the level has entries (a flat copy of start values).
 // xcd != 0: each of the 8 XCDs takes one contiguous slice of the wave rows.
 template <typename T, bool BWD>
-void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nlong, int32_t nmed, int32_t t1, int waves, int rpw,
-                           int xcd,
-                           const int32_t *rows, const int32_t *ptr, const int32_t *col, const T *val,
-                           const double *diag, int k, const T *in, T *out);
+void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nhuge, int32_t hm, int32_t nlong, int32_t nmed,
+                           int32_t t1, int waves, int rpw, int xcd, const int32_t *rows, const int32_t *ptr,
+                           const int32_t *col, const T *val, const double *diag, int k, const T *in, T *out,
+                           T *scratch, int *counter);
 // Small levels [l0, l1) (each of at most kSmallRows rows) walked by one
 // workgroup; level l holds positions [level_ptr[l], level_ptr[l+1]).
 template <typename T, bool BWD>
@@ -69,11 +72,11 @@ void launch_tri_small(hipStream_t s, int32_t l0, int32_t l1, const int32_t *leve
 // Dense cores (per graph: core_n[g] rows, vertex ids vert[core_off[g] ..],
 // dense matrix at D + dense_off[g], row-major core_n x core_n).
 // Forward: launch_core_head (the entries outside the core, schedule
-// positions [t0, t1)), then launch_core_step<T, false> for steps 0 .. max
+// positions [t0, t1), the first nhuge of more than kWgRowMax entries), then launch_core_step<T, false> for steps 0 .. max
 // blocks - 1 with D = L.  Backward: launch_core_bwd_init, then
 // launch_core_step<T, true> for the same steps with D = L^T.
 template <typename T>
-void launch_core_head(hipStream_t s, int32_t t0, int32_t t1, const int32_t *rows, const int32_t *ptr,
+void launch_core_head(hipStream_t s, int32_t t0, int32_t nhuge, int32_t t1, const int32_t *rows, const int32_t *ptr,
                       const int32_t *col, const T *val, const double *diag, int k, const T *in, T *out);
 template <typename T>
 void launch_core_bwd_init(hipStream_t s, int32_t count, const int32_t *vert, const double *diag, int k, const T *in,

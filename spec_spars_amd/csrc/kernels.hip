@@ -334,6 +334,90 @@ __global__ __launch_bounds__(256) void k_level_copy(int32_t t0, int64_t count, c
     out[(size_t)i * k + j] = row_start<T, BWD>(i, j, k, BWD ? diag[i] : 1.0, in);
 }
 
+// Rows of more than kWgRowMax entries (positions [t0, t0 + count)), split
+// in P = 64 or 128 parts folded as one adjacent-pair tree (tree_fold's), so
+// any split of the parts into aligned power-of-two groups gives the same
+// bits.  hm == 1: one workgroup per row, wave w folding parts [w P/16,
+// (w+1) P/16).  hm > 1: H = P / 16 workgroups per row (hm launched per
+// row), workgroup h folding parts [16h, 16h + 16) with one wave per part and
+// storing that subtree of every column chunk in its scratch slot; the last
+// of the row's workgroups to arrive (one device-scope counter per row, reset
+// by that workgroup) folds the H subtrees, adds them to the start value and
+// stores the row.
+template <typename T, bool BWD>
+__global__ __launch_bounds__(64 * kLevelWaves) void k_trisolve_huge(int32_t t0, int32_t hm,
+                                                                    const int32_t *__restrict__ rows,
+                                                                    const int32_t *__restrict__ ptr,
+                                                                    const int32_t *__restrict__ col,
+                                                                    const T *__restrict__ val,
+                                                                    const double *__restrict__ diag, int k,
+                                                                    const T *__restrict__ in, T *out, T *scratch,
+                                                                    int *counter) {
+    __shared__ T sm[kWgParts][64];
+    const int lane = SS_LANE, wv = SS_WAVE_IN_BLOCK;
+    const int slot = (int)blockIdx.x / hm, h = (int)blockIdx.x - slot * hm;
+    const int t = t0 + slot;
+    const int i = wave_uniform(rows[t]);
+    const int q0 = wave_uniform(ptr[t]), q1 = wave_uniform(ptr[t + 1]);
+    const int P = seg_parts(q1 - q0);
+    const int H = row_wgs(q1 - q0);
+    const double d = BWD ? diag[i] : 1.0;
+    if (hm == 1) {
+        const int na = P / kWgParts;
+        for (int jb = 0; jb < k; jb += 64) {
+            const int j = jb + lane, jj = j < k ? j : k - 1;
+            sm[wv][lane] = wave_subtree_n<T, kMaxWg>(na, q0, q1, P, wv * na, col, val, out, k, jj, lane);
+            __syncthreads();
+            if (wv == 0 && j < k) {
+                const T acc = row_start<T, BWD>(i, jj, k, d, in);
+                out[(size_t)i * k + j] = acc + lds_tree_n<T, kWgParts>(sm, 0, lane);
+            }
+            __syncthreads();
+        }
+        return;
+    }
+    if (h >= H) return;   // uniform over the workgroup
+    T *sc = scratch + ((size_t)slot * kMaxWg + h) * k;
+    for (int jb = 0; jb < k; jb += 64) {
+        const int j = jb + lane, jj = j < k ? j : k - 1;
+        sm[wv][lane] = wave_subtree<T, 1>(q0, q1, P, h * kWgParts + wv, col, val, out, k, jj, lane);
+        __syncthreads();
+        if (wv == 0 && j < k) sc[j] = lds_tree_n<T, kWgParts>(sm, 0, lane);
+        __syncthreads();
+    }
+    if (wv != 0) return;
+    // release this workgroup's subtrees before its arrival; only the last
+    // arrival acquires the others'
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    int old = 0;
+    if (lane == 0) old = __hip_atomic_fetch_add(counter + slot, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    old = __shfl(old, 0, 64);
+    if (old != H - 1) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const T *rs = scratch + (size_t)slot * kMaxWg * k;
+    for (int jb = 0; jb < k; jb += 64) {
+        const int j = jb + lane, jj = j < k ? j : k - 1;
+        T acc = row_start<T, BWD>(i, jj, k, d, in);
+        T fold;
+        if (H == 4) {
+            T v[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) v[u] = rs[(size_t)u * k + jj];
+            tree_fold<T, 4>(v);
+            fold = v[0];
+        } else {
+            T v[kMaxWg];
+#pragma unroll
+            for (int u = 0; u < kMaxWg; u++) v[u] = rs[(size_t)u * k + jj];
+            tree_fold<T, kMaxWg>(v);
+            fold = v[0];
+        }
+        acc = acc + fold;
+        if (j < k) out[(size_t)i * k + j] = acc;
+    }
+    if (lane == 0) counter[slot] = 0;
+}
+
 // One level of a triangular solve: schedule positions [t0, t1); the first
 // nlong of them take a block each (its W = blockDim / 64 waves share the
 // row's parts), the next nmed one wave each, the rest (tiny rows) rpw
@@ -366,7 +450,7 @@ __global__ __launch_bounds__(64 * kLevelWaves) void k_trisolve_level(int32_t t0,
                 if (wv == 0) acc = row_dot_sub<T>(acc, q0, q1, 1, col, val, out, k, jj);
             } else if (wv < gg) {
                 const int s0 = wv * na;
-                sm[wv][lane] = wave_subtree_n<T, kMaxParts / 4>(na, q0, q1, P, s0, col, val, out, k, jj, lane);
+                sm[wv][lane] = wave_subtree_n<T, kWgParts / 4>(na, q0, q1, P, s0, col, val, out, k, jj, lane);
             }
             __syncthreads();
             if (wv == 0) {
@@ -611,7 +695,7 @@ __global__ __launch_bounds__(1024) void k_tri_small(int32_t l0, int32_t l1, cons
             if (act && sub == 0) acc = row_start<T, BWD>(i, jj, k, d, in);
             if (act && P == 1 && sub == 0) acc = row_dot_sub<T>(acc, q0, q1, 1, col, val, out, k, jj);
             if (act && P > 1 && sub < gg)
-                sm[slot * g + sub][lane] = wave_subtree_n<T, kMaxParts>(P / gg, q0, q1, P, sub * (P / gg), col, val,
+                sm[slot * g + sub][lane] = wave_subtree_n<T, kWgParts>(P / gg, q0, q1, P, sub * (P / gg), col, val,
                                                                        out, k, jj, lane);
             __syncthreads();
             if (act && P > 1 && sub == 0) acc = acc + lds_tree<T>(sm, slot * g, gg, lane);
@@ -765,16 +849,17 @@ void launch_fill_u32(hipStream_t s, int64_t count, uint32_t *p, uint32_t v) {
 }
 
 template <typename T>
-void launch_core_head(hipStream_t s, int32_t t0, int32_t t1, const int32_t *rows, const int32_t *ptr,
+void launch_core_head(hipStream_t s, int32_t t0, int32_t nhuge, int32_t t1, const int32_t *rows, const int32_t *ptr,
                       const int32_t *col, const T *val, const double *diag, int k, const T *in, T *out) {
-    // every core row takes a block of 16 waves, as a long row of a level
-    launch_trisolve_level<T, false>(s, t0, t1 - t0, 0, t1, kLevelWaves, 1, 0, rows, ptr, col, val, diag, k, in,
-                                    out);
+    // every core row as a long row of a level, a workgroup each (the core
+    // rows are many): the rows of more than kWgRowMax entries first
+    launch_trisolve_level<T, false>(s, t0, nhuge, 1, t1 - t0 - nhuge, 0, t1, kLevelWaves, 1, 0, rows, ptr, col, val,
+                                    diag, k, in, out, nullptr, nullptr);
 }
-template void launch_core_head<double>(hipStream_t, int32_t, int32_t, const int32_t *, const int32_t *,
+template void launch_core_head<double>(hipStream_t, int32_t, int32_t, int32_t, const int32_t *, const int32_t *,
                                        const int32_t *, const double *, const double *, int, const double *,
                                        double *);
-template void launch_core_head<float>(hipStream_t, int32_t, int32_t, const int32_t *, const int32_t *,
+template void launch_core_head<float>(hipStream_t, int32_t, int32_t, int32_t, const int32_t *, const int32_t *,
                                       const int32_t *, const float *, const double *, int, const float *, float *);
 
 template <typename T>
@@ -823,9 +908,15 @@ SS_INST_SMALL(float, false)
 SS_INST_SMALL(float, true)
 
 template <typename T, bool BWD>
-void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nlong, int32_t nmed, int32_t t1, int waves, int rpw,
-                           int xcd, const int32_t *rows, const int32_t *ptr, const int32_t *col, const T *val,
-                           const double *diag, int k, const T *in, T *out) {
+void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nhuge, int32_t hm, int32_t nlong, int32_t nmed,
+                           int32_t t1, int waves, int rpw, int xcd, const int32_t *rows, const int32_t *ptr,
+                           const int32_t *col, const T *val, const double *diag, int k, const T *in, T *out,
+                           T *scratch, int *counter) {
+    if (nhuge > 0) {
+        hipLaunchKernelGGL((k_trisolve_huge<T, BWD>), dim3((unsigned)nhuge * (unsigned)hm), dim3(64 * kWgParts), 0, s,
+                           t0, hm, rows, ptr, col, val, diag, k, in, out, scratch, counter);
+        t0 += nhuge;
+    }
     if (t1 <= t0) return;
     if (waves == 0) {   // a level of rows without entries
         const int64_t count = (int64_t)(t1 - t0) * k;
@@ -854,10 +945,9 @@ void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nlong, int32_t nme
                            nmed, t1, rpw, xcd_slices, rows, ptr, col, val, diag, k, in, out);
 }
 #define SS_INST_TRI(T, B)                                                                                      \
-    template void launch_trisolve_level<T, B>(hipStream_t, int32_t, int32_t, int32_t, int32_t, int, int, int,   \
-                                              const int32_t *,                                                   \
-                                              const int32_t *, const int32_t *, const T *, const double *, int,  \
-                                              const T *, T *);
+    template void launch_trisolve_level<T, B>(hipStream_t, int32_t, int32_t, int32_t, int32_t, int32_t, int32_t,   \
+                                              int, int, int, const int32_t *, const int32_t *, const int32_t *,  \
+                                              const T *, const double *, int, const T *, T *, T *, int *);
 SS_INST_TRI(double, false)
 SS_INST_TRI(double, true)
 SS_INST_TRI(float, false)

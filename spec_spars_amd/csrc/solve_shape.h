@@ -10,14 +10,24 @@
 //
 // A shape that depends only on the row means any kernel may finish it --
 // one wave holding every part in registers, a block of waves sharing the
-// parts, the small-level kernel -- with the same bits, whatever other rows
-// or graphs share the launch.
+// parts, several workgroups each folding 16 parts (rows of more than 1024
+// entries), the small-level kernel -- with the same bits, whatever other
+// rows or graphs share the launch.
 #pragma once
 
 namespace ss {
 
-constexpr int seg_parts(int len) { return len <= 8 ? 1 : (len <= 32 ? 4 : (len <= 128 ? 8 : 16)); }
-constexpr int kMaxParts = 16;
+constexpr int seg_parts(int len) {
+    return len <= 8 ? 1 : (len <= 32 ? 4 : (len <= 128 ? 8 : (len <= 1024 ? 16 : (len <= 4096 ? 64 : 128))));
+}
+constexpr int kWgParts = 16;      // parts one workgroup folds (16 waves, a part each at most)
+constexpr int kWgRowMax = 1024;   // the longest row of kWgParts parts
+constexpr int kMaxParts = 128;    // rows past kWgRowMax: kMaxParts / kWgParts workgroups at most
+constexpr int kMaxWg = kMaxParts / kWgParts;   // workgroups of one row, at most
+// Workgroups that fold a row of len entries.
+constexpr int row_wgs(int len) { return seg_parts(len) > kWgParts ? seg_parts(len) / kWgParts : 1; }
+static_assert(row_wgs(1 << 30) == kMaxWg, "the longest rows take kMaxWg workgroups");
+static_assert(seg_parts(kWgRowMax) == kWgParts && seg_parts(kWgRowMax + 1) > kWgParts, "one workgroup per row up to kWgRowMax");
 
 // Row classes of a per-level launch.  In a level of at least kWaveLevelRows
 // rows, rows of up to kWaveRowMax entries take one wave each; in smaller
@@ -31,6 +41,8 @@ constexpr int kTinyRow = 8;
 constexpr int kTinyGroup = 4;
 static_assert(seg_parts(kTinyRow) == 1 && seg_parts(kTinyRow + 1) > 1, "tiny rows are the one-part rows");
 static_assert(seg_parts(kWaveRowMax) <= 8 && kWaveRowMax <= 128, "a wave row has at most 8 parts and 2 entries per lane");
+
+constexpr int kMaxK = 256;        // sketch columns / right-hand sides per call, at most
 
 constexpr int kSmallRows = 16;    // levels with at most this many rows ...
 constexpr int kSmallNnz = 2048;   // ... and entries go to the single-workgroup kernel

@@ -68,9 +68,11 @@ struct Schedule {
     int64_t n = 0;                    // scheduled rows (positions)
     int32_t depth = 0;
     int32_t core_t0 = 0;
+    int32_t core_huge = 0;            // forward core rows of more than kWgRowMax entries (placed first)
     std::vector<int32_t> rows;        // vertex finished at position t
     std::vector<int32_t> level_ptr;   // positions of level l: [level_ptr[l], level_ptr[l+1])
     std::vector<int32_t> level_long;  // rows of level l with more than long_row entries (placed first)
+    std::vector<int32_t> level_huge;  // the first of them: rows of more than kWgRowMax entries
     std::vector<int32_t> level_med;   // then rows taking one wave each (more than kTinyRow entries); tiny rows last
     // Runs of consecutive small levels (at least two levels of at most
     // small_rows rows each), each walked by one workgroup: [first, second).

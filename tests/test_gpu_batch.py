@@ -102,13 +102,13 @@ def test_batch_alone_vs_mixed_bit_identical(ctx, prec):
         assert np.array_equal(rev[j]["Z"], mixed[i]["Z"]) and np.array_equal(rev[j]["p"], mixed[i]["p"])
 
 
-def _hubs(nh=100, seed=7):
-    """Hubs on a ring, each with 35..119 leaves: the leaves go in the first
-    elimination round, so each hub's forward row holds 35..120 entries in
-    the second and third levels."""
+def _hubs(nh=100, seed=7, lo=35, hi=120):
+    """Hubs on a ring, each with lo..hi-1 leaves (35..119 by default): the
+    leaves go in the first elimination round, so each hub's forward row
+    holds as many entries, plus one, in the second and third levels."""
     from spec_spars_amd import graphs
     rng = np.random.default_rng(seed)
-    sizes = rng.integers(35, 120, nh)
+    sizes = rng.integers(lo, hi, nh)
     u, v = [], []
     leaf = nh
     for h in range(nh):
@@ -139,6 +139,33 @@ def test_wave_rows_and_blocks_bit_identical(ctx, prec):
         alone = _run(ctx, gs, fs, [i], k, prec)[0]
         for key in ("Y", "Z", "p"):
             assert np.array_equal(alone[key], mixed[i][key]), (i, key)
+        n, u, v, w = gs[i]
+        L = _laplacian(n, u, v, w)
+        Yp = _component_projection(n, u, v, mixed[i]["Y"])
+        rr = np.linalg.norm(Yp - L @ mixed[i]["Z"], axis=0) / np.linalg.norm(Yp, axis=0)
+        assert rr.max() <= 1.01e-8, i
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_multi_workgroup_rows_bit_identical(ctx, prec):
+    """Hub rows of 1100..9000 entries (more than 1024: 4 or 8 workgroups
+    per row, folding through scratch and an arrival counter) alone and
+    mixed with a grid, k = 72 (two column chunks): the same bits, every
+    residual at the tolerance, and a second solve on the same resident
+    scratch gives the same bits again."""
+    import spec_spars_amd as ss
+    from spec_spars_amd import graphs
+    n, u, v = graphs.grid2d(200, 200)
+    gs = [(n, u, v, graphs.loguniform_weights(len(u), seed=3)), _hubs(nh=12, seed=5, lo=1100, hi=9000)]
+    fs = _factors(gs, seed=9, ordering=ss.ORDER_ROUNDS)
+    k = 72
+    mixed = _run(ctx, gs, fs, [0, 1], k, prec)
+    again = _run(ctx, gs, fs, [0, 1], k, prec)
+    for i in range(2):
+        alone = _run(ctx, gs, fs, [i], k, prec)[0]
+        for key in ("Y", "Z", "p"):
+            assert np.array_equal(alone[key], mixed[i][key]), (i, key)
+            assert np.array_equal(again[i][key], mixed[i][key]), (i, key)
         n, u, v, w = gs[i]
         L = _laplacian(n, u, v, w)
         Yp = _component_projection(n, u, v, mixed[i]["Y"])
