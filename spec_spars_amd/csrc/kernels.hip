@@ -666,6 +666,14 @@ __global__ __launch_bounds__(256) void k_core_step(int s, const int32_t *__restr
 // workgroup-scope fence orders the row stores before the next level's
 // loads).  A level of r rows gives each row g = 16 / r waves (rounded down
 // to a power of two), which split the row's parts between them.
+//
+// The schedule of the run (level bounds, row records, entries) does not
+// depend on the solve, so it is staged in LDS a chunk of levels at a time:
+// three dependent loads per chunk instead of per level, leaving each level
+// one dependent load (its gathers) before its barrier.  A chunk is the most
+// levels whose rows and entries fit the stage; one small level always fits.
+constexpr int kStageLevels = 256, kStageRows = 512, kStageNnz = 2 * kSmallNnz;
+static_assert(kStageRows >= kSmallRows && kStageNnz >= kSmallNnz, "one small level fits the stage");
 template <typename T, bool BWD>
 __global__ __launch_bounds__(1024) void k_tri_small(int32_t l0, int32_t l1, const int32_t *__restrict__ level_ptr,
                                                     const int32_t *__restrict__ rows,
@@ -674,34 +682,65 @@ __global__ __launch_bounds__(1024) void k_tri_small(int32_t l0, int32_t l1, cons
                                                     const double *__restrict__ diag, int k,
                                                     const T *__restrict__ in, T *out) {
     __shared__ T sm[16][64];
+    __shared__ int32_t s_lp[kStageLevels + 1];
+    __shared__ int32_t s_rows[kStageRows];
+    __shared__ int32_t s_ptr[kStageRows + 1];
+    __shared__ int32_t s_col[kStageNnz];
+    __shared__ T s_val[kStageNnz];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int l = l0; l < l1; l++) {
-        const int tb = level_ptr[l], te = level_ptr[l + 1];
-        const int r = te - tb;                  // r <= kSmallRows == 16
-        if (r <= 0) continue;
-        int g = 16 / r;
-        g = 1 << (31 - __builtin_clz(g));
-        const int slot = wv / g, sub = wv - slot * g;
-        const int t = tb + slot;
-        const bool act = t < te;
-        const int i = act ? wave_uniform(rows[t]) : 0;
-        const int q0 = act ? wave_uniform(ptr[t]) : 0, q1 = act ? wave_uniform(ptr[t + 1]) : 0;
-        const double d = (BWD && act) ? diag[i] : 1.0;
-        const int P = act ? seg_parts(q1 - q0) : 1;
-        const int gg = P < g ? P : g;             // waves that hold parts, P / gg each
-        for (int jb = 0; jb < k; jb += 64) {
-            const int j = jb + lane, jj = j < k ? j : k - 1;
-            T acc = (T)0;
-            if (act && sub == 0) acc = row_start<T, BWD>(i, jj, k, d, in);
-            if (act && P == 1 && sub == 0) acc = row_dot_sub<T>(acc, q0, q1, 1, col, val, out, k, jj);
-            if (act && P > 1 && sub < gg)
-                sm[slot * g + sub][lane] = wave_subtree_n<T, kWgParts>(P / gg, q0, q1, P, sub * (P / gg), col, val,
-                                                                       out, k, jj, lane);
-            __syncthreads();
-            if (act && P > 1 && sub == 0) acc = acc + lds_tree<T>(sm, slot * g, gg, lane);
-            if (act && sub == 0 && j < k) out[(size_t)i * k + j] = acc;
-            __syncthreads();   // also orders this level's stores before the next level's loads
+    for (int c0 = l0; c0 < l1;) {
+        // level bounds, then row records, then entries of the chunk
+        const int nl = l1 - c0 < kStageLevels ? l1 - c0 : kStageLevels;
+        for (int u = threadIdx.x; u <= nl; u += blockDim.x) s_lp[u] = level_ptr[c0 + u];
+        __syncthreads();
+        const int tb = s_lp[0];
+        const int np = s_lp[nl] - tb < kStageRows ? s_lp[nl] - tb : kStageRows;
+        for (int u = threadIdx.x; u <= np; u += blockDim.x) {
+            s_ptr[u] = ptr[tb + u];
+            if (u < np) s_rows[u] = rows[tb + u];
         }
+        __syncthreads();
+        const int eb = s_ptr[0];
+        // levels 1 .. nc end inside the stage (a prefix: bounds only grow)
+        const int u = (int)threadIdx.x + 1;
+        const bool fits = u <= nl && s_lp[u] - tb <= np && s_ptr[s_lp[u] - tb] - eb <= kStageNnz;
+        const int nc = __syncthreads_count(fits);
+        if (nc == 0) return;   // not reached: every level of a run is small, and one small level fits
+        const int ne = s_ptr[s_lp[nc] - tb] - eb;
+        for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+            s_col[e] = col[eb + e];
+            s_val[e] = val[eb + e];
+        }
+        __syncthreads();
+        for (int l = 0; l < nc; l++) {
+            const int pb = s_lp[l] - tb, pe = s_lp[l + 1] - tb;
+            const int r = pe - pb;                  // r <= kSmallRows == 16
+            if (r <= 0) continue;
+            int g = 16 / r;
+            g = 1 << (31 - __builtin_clz(g));
+            const int slot = wv / g, sub = wv - slot * g;
+            const int pt = pb + slot;
+            const bool act = pt < pe;
+            const int i = act ? s_rows[pt] : 0;
+            const int q0 = act ? s_ptr[pt] - eb : 0, q1 = act ? s_ptr[pt + 1] - eb : 0;
+            const double d = (BWD && act) ? diag[i] : 1.0;
+            const int P = act ? seg_parts(q1 - q0) : 1;
+            const int gg = P < g ? P : g;             // waves that hold parts, P / gg each
+            for (int jb = 0; jb < k; jb += 64) {
+                const int j = jb + lane, jj = j < k ? j : k - 1;
+                T acc = (T)0;
+                if (act && sub == 0) acc = row_start<T, BWD>(i, jj, k, d, in);
+                if (act && P == 1 && sub == 0) acc = row_dot_sub<T>(acc, q0, q1, 1, s_col, s_val, out, k, jj);
+                if (act && P > 1 && sub < gg)
+                    sm[slot * g + sub][lane] = wave_subtree_n<T, kWgParts>(P / gg, q0, q1, P, sub * (P / gg), s_col,
+                                                                           s_val, out, k, jj, lane);
+                __syncthreads();
+                if (act && P > 1 && sub == 0) acc = acc + lds_tree<T>(sm, slot * g, gg, lane);
+                if (act && sub == 0 && j < k) out[(size_t)i * k + j] = acc;
+                __syncthreads();   // also orders this level's stores before the next level's loads
+            }
+        }
+        c0 += nc;
     }
 }
 
