@@ -579,9 +579,15 @@ __global__ __launch_bounds__(256) void k_core_step(int s, const int32_t *__restr
     constexpr int DPT = B * B / 256;   // entries of a 64 x 64 block per thread
     for (int jb = 0; jb < k; jb += 64) {
         const int j = jb + lane;
-        // every load of the previous block's values, the D panel and (for
-        // the workgroup that substitutes) the diagonal block is in flight
-        // before the first shared store
+        // every load of this workgroup's rows, the previous block's values,
+        // the D panel and (for the workgroup that substitutes) the diagonal
+        // block is in flight before the first shared store
+        T acc[RW];
+#pragma unroll
+        for (int u = 0; u < RW; u++) {
+            const int r = wv * RW + u;
+            acc[u] = (r < nr && j < k) ? out[(size_t)vg[r0 + r] * k + j] : (T)0;
+        }
         T dd[DPT];
         if (x == 0) {
 #pragma unroll
@@ -611,12 +617,6 @@ __global__ __launch_bounds__(256) void k_core_step(int s, const int32_t *__restr
             }
         }
         __syncthreads();
-        T acc[RW];
-#pragma unroll
-        for (int u = 0; u < RW; u++) {
-            const int r = wv * RW + u;
-            acc[u] = (r < nr && j < k) ? out[(size_t)vg[r0 + r] * k + j] : (T)0;
-        }
         // the previous block: ascending columns forward, descending backward
 #pragma unroll 4
         for (int qq = 0; qq < np; qq++) {
