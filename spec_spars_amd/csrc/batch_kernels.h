@@ -19,8 +19,8 @@ inline int64_t row_tile_rows(int64_t n) {
     while (r < kRowTile && r * 256 < n) r *= 2;
     return r;
 }
-static const int kSpmmTileNnz = 32768;  // SpMM tiles: matrix entries per tile, about
-static const int kSpmmMaxSplit = 64;    // SpMM tiles per row tile, at most
+static const int kSpmmTileNnz = 8192;   // SpMM tiles: matrix entries per tile, about
+static const int kSpmmMaxSplit = 256;   // SpMM tiles per row tile, at most
 static const int kEdgeTile = 256;    // edges per edge tile (one block)
 
 static const int kMaxTileWaves = 16;
