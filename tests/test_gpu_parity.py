@@ -274,3 +274,50 @@ def test_graph_sparsify_ordered_mixed(ctx, oracle_mod):
     ku, kv, kw = G.read_kept()
     ref = oracle_mod.sample_reweight(n, uu, vv, lv, p, oracle_mod.coins(4, len(uu)))
     assert np.array_equal(ref["ku"], ku) and np.array_equal(ref["kw"], kw)
+
+
+def test_rounds_factor_solve_matches_permuted_oracle(ctx, oracle_mod):
+    """The whole preconditioner on a power-law graph, checked against the
+    oracle's PCG with the same rounds-order factor: the oracle runs in the
+    factor's position space (L and Y permuted by the factor's order), so both
+    apply the same M^-1, and the iteration counts and solutions agree.  The
+    graph (one large component and isolated vertices, the right-hand side
+    projected per component) has a backward solve that starts with a run of
+    more than 256 small levels (several LDS stages of the small-level
+    kernel), and hub rows of more than 1,024 entries."""
+    import scipy.sparse as sp
+    import scipy.sparse.csgraph as cg
+    import spec_spars_amd as ss
+    from spec_spars_amd import graphs
+    n, u, v = graphs.chung_lu(300000, 3000000, gamma=2.2, seed=0)
+    w = graphs.uniform_weights(len(u), seed=0)
+    G = ss.Graph(ctx, n, u, v, w)
+    cp, ri, va = G.laplacian()
+    F = ss.Factor(cp, ri, va, seed=0, ordering=ss.ORDER_ROUNDS)
+    G.set_factor(F)
+    bi = G.schedule_info(1)
+    assert bi["small_runs"] >= 1 and bi["small_levels"] > 256, bi
+    nc, lab = cg.connected_components(sp.coo_matrix((np.ones(len(u)), (u, v)), shape=(n, n)), directed=False)
+    cnt = np.bincount(lab, minlength=nc).astype(np.float64)
+
+    def project(X):
+        X = X.copy()
+        for j in range(X.shape[1]):
+            X[:, j] -= (np.bincount(lab, weights=X[:, j], minlength=nc) / cnt)[lab]
+        return X
+
+    Y = project(oracle_mod.sketch(u, v, w, n, 4, seed=1, scale=0.5))
+    Z, info = G.solve(Y, ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-10, max_iter=300, precision=ss.PREC_FP64))
+    G.close()
+    order, _ = F.order()
+    L = sp.csc_matrix((va, ri, cp), shape=(n, n))
+    Lp = L[order][:, order].tocsc()
+    Lp.sort_indices()
+    Zp, it, rr = oracle_mod.solve_lap(Y[order], Lp.indptr.astype(np.int32), Lp.indices.astype(np.int32), Lp.data,
+                                      *F.export(), stop_rule=1, tol=1e-10, max_iter=300)
+    assert rr.max() <= 1e-10
+    assert np.abs(info["iters"] - it).max() <= 1, (info["iters"], it)
+    Zo = np.empty_like(Zp)
+    Zo[order] = Zp
+    a, b = project(Z), project(Zo)
+    assert (np.linalg.norm(a - b, axis=0) / np.linalg.norm(b, axis=0)).max() < 1e-7
