@@ -139,8 +139,17 @@ __global__ __launch_bounds__(1024) void kb_tile_partial(Tiles rt, int k, const T
 // Per (graph, column) sum of the graph's tile partials by one 64-lane block:
 // lane l adds tiles l, l+64, ... in order, then a fixed butterfly.
 __device__ __forceinline__ double bt_graph_sum(const double *partial, const int32_t *gtile, int g, int k, int j) {
+    // lane l adds tiles l, l + 64, ... in order; 8 of its loads in flight
     double s = 0.0;
-    for (int t = gtile[g] + BT_LANE; t < gtile[g + 1]; t += 64) s += partial[(size_t)t * k + j];
+    const int t1 = gtile[g + 1];
+    for (int t = gtile[g] + BT_LANE; t < t1; t += 64 * 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = t + 64 * u < t1 ? partial[(size_t)(t + 64 * u) * k + j] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            if (t + 64 * u < t1) s += v[u];
+    }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     return s;
@@ -281,37 +290,39 @@ __global__ __launch_bounds__(1024) void kb_project(Tiles rt, int k, const TS *__
 // P = Z + beta P, and the previous iteration's X += alpha P with the old P
 // (the solution update rides along with the search-direction update, so
 // P is read once for both).  A column with alpha == 0 keeps X untouched
-// (on the first iteration P holds no direction yet).
+// (on the first iteration P holds no direction yet).  Thread i of the
+// tile's block works on column j = i % k of rows i / k, i / k + RP, ...
+// (RP = blockDim / k rows per pass), so a pass moves RP whole rows as one
+// contiguous run and no lane idles when k is below 64.
 template <typename T>
 __global__ __launch_bounds__(1024) void kb_update_px(Tiles rt, int k, const double *__restrict__ alpha,
                                                     const double *__restrict__ beta, const T *__restrict__ Z,
                                                     T *__restrict__ P, T *__restrict__ X) {
     const int t = blockIdx.x, g = rt.graph[t];
     const int r0 = rt.start[t], r1 = rt.start[t + 1];
-    for (int jb = 0; jb < k; jb += 64) {
-        const int j = jb + BT_LANE;
-        if (j >= k) break;
-        const T a = (T)alpha[(size_t)g * k + j];
-        const T b = (T)beta[(size_t)g * k + j];
-        for (int rb = r0 + BT_WAVE; rb < r1; rb += kBtRows * BT_NW) {
-            T p[kBtRows], z[kBtRows], x[kBtRows];
+    const int RP = (int)blockDim.x / k, i = (int)threadIdx.x;
+    if (i >= RP * k) return;
+    const int j = i % k, ro = i / k;
+    const T a = (T)alpha[(size_t)g * k + j];
+    const T b = (T)beta[(size_t)g * k + j];
+    for (int rb = r0 + ro; rb < r1; rb += kBtRows * RP) {
+        T p[kBtRows], z[kBtRows], x[kBtRows];
 #pragma unroll
-            for (int u = 0; u < kBtRows; u++) {
-                const int r = rb + u * BT_NW;
-                const bool on = r < r1;
-                const size_t o = (size_t)r * k + j;
-                p[u] = on && (a != (T)0 || b != (T)0) ? P[o] : (T)0;
-                z[u] = on ? Z[o] : (T)0;
-                x[u] = on && a != (T)0 ? X[o] : (T)0;
-            }
+        for (int u = 0; u < kBtRows; u++) {
+            const int r = rb + u * RP;
+            const bool on = r < r1;
+            const size_t o = (size_t)r * k + j;
+            p[u] = on && (a != (T)0 || b != (T)0) ? P[o] : (T)0;
+            z[u] = on ? Z[o] : (T)0;
+            x[u] = on && a != (T)0 ? X[o] : (T)0;
+        }
 #pragma unroll
-            for (int u = 0; u < kBtRows; u++) {
-                const int r = rb + u * BT_NW;
-                if (r >= r1) break;
-                const size_t o = (size_t)r * k + j;
-                if (a != (T)0) X[o] = bt_fma(a, p[u], x[u]);
-                P[o] = b == (T)0 ? z[u] : bt_fma(b, p[u], z[u]);
-            }
+        for (int u = 0; u < kBtRows; u++) {
+            const int r = rb + u * RP;
+            if (r >= r1) break;
+            const size_t o = (size_t)r * k + j;
+            if (a != (T)0) X[o] = bt_fma(a, p[u], x[u]);
+            P[o] = b == (T)0 ? z[u] : bt_fma(b, p[u], z[u]);
         }
     }
 }
@@ -436,39 +447,47 @@ __global__ __launch_bounds__(1024) void kb_spmm_dot(Tiles rt, const int32_t *__r
     }
 }
 
-// R -= alpha Q with per-tile partials of |R|^2.
+// R -= alpha Q with per-tile partials of |R|^2, in the same row-block
+// mapping: each thread sums its column over its rows in order, then the
+// column's RP threads are summed in row-offset order.
 template <typename T>
 __global__ __launch_bounds__(1024) void kb_update_r_rr(Tiles rt, int k, const double *__restrict__ alpha,
                                                       const T *__restrict__ Q, T *__restrict__ R,
                                                       double *__restrict__ partial) {
-    __shared__ double sm[kMaxTileWaves][64];
+    __shared__ double sm[1024];
     const int t = blockIdx.x, g = rt.graph[t];
     const int r0 = rt.start[t], r1 = rt.start[t + 1];
-    for (int jb = 0; jb < k; jb += 64) {
-        const int j = jb + BT_LANE;
-        double acc = 0.0;
-        if (j < k) {
-            const T a = (T)alpha[(size_t)g * k + j];
-            for (int rb = r0 + BT_WAVE; rb < r1; rb += kBtRows * BT_NW) {
-                T q[kBtRows], rv[kBtRows];
+    const int RP = (int)blockDim.x / k, i = (int)threadIdx.x;
+    double acc = 0.0;
+    if (i < RP * k) {
+        const int j = i % k, ro = i / k;
+        const T a = (T)alpha[(size_t)g * k + j];
+        for (int rb = r0 + ro; rb < r1; rb += kBtRows * RP) {
+            T q[kBtRows], rv[kBtRows];
 #pragma unroll
-                for (int u = 0; u < kBtRows; u++) {
-                    const int r = rb + u * BT_NW;
-                    const bool on = r < r1;
-                    q[u] = on ? Q[(size_t)r * k + j] : (T)0;
-                    rv[u] = on ? R[(size_t)r * k + j] : (T)0;
-                }
+            for (int u = 0; u < kBtRows; u++) {
+                const int r = rb + u * RP;
+                const bool on = r < r1;
+                q[u] = on ? Q[(size_t)r * k + j] : (T)0;
+                rv[u] = on ? R[(size_t)r * k + j] : (T)0;
+            }
 #pragma unroll
-                for (int u = 0; u < kBtRows; u++) {
-                    const int r = rb + u * BT_NW;
-                    if (r >= r1) break;
-                    const T v = bt_fma(-a, q[u], rv[u]);
-                    R[(size_t)r * k + j] = v;
-                    acc = bt_fma((double)v, (double)v, acc);
-                }
+            for (int u = 0; u < kBtRows; u++) {
+                const int r = rb + u * RP;
+                if (r >= r1) break;
+                const T v = bt_fma(-a, q[u], rv[u]);
+                R[(size_t)r * k + j] = v;
+                acc = bt_fma((double)v, (double)v, acc);
             }
         }
-        bt_tile_store(acc, k, jb, partial, sm);
+    }
+    // column j of the tile: its RP threads' sums in row-offset order
+    sm[i] = acc;
+    __syncthreads();
+    if (i < k) {
+        double sum = sm[i];
+        for (int q = 1; q < RP; q++) sum += sm[q * k + i];
+        partial[(size_t)t * k + i] = sum;
     }
 }
 
@@ -693,9 +712,17 @@ template void bt_project<double, double>(hipStream_t, Tiles, int, const double *
                                          double *, double *, double *);
 template void bt_project<float, float>(hipStream_t, Tiles, int, const float *, const int32_t *, const double *,
                                        float *, float *, float *);
+// Row-block launches (update_px, update_r_rr): thread i of a tile block
+// works on column i % k of every (blockDim / k)-th row, so at least k
+// threads are needed.
+static int rowblock_threads(const Tiles &rt, int k) {
+    const int w = 64 * rt.waves, need = (k + 63) / 64 * 64;
+    return w > need ? w : need;
+}
 template <typename T>
 void bt_update_px(hipStream_t s, Tiles rt, int k, const double *alpha, const double *beta, const T *Z, T *P, T *X) {
-    hipLaunchKernelGGL(kb_update_px<T>, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, k, alpha, beta, Z, P, X);
+    hipLaunchKernelGGL(kb_update_px<T>, dim3(rt.count), dim3(rowblock_threads(rt, k)), 0, s, rt, k, alpha, beta, Z, P,
+                       X);
 }
 template void bt_update_px<double>(hipStream_t, Tiles, int, const double *, const double *, const double *, double *,
                                    double *);
@@ -718,7 +745,8 @@ template void bt_spmm_dot<float>(hipStream_t, Tiles, const int32_t *, const int3
                                  const float *, float *, double *);
 template <typename T>
 void bt_update_r_rr(hipStream_t s, Tiles rt, int k, const double *alpha, const T *Q, T *R, double *partial) {
-    hipLaunchKernelGGL(kb_update_r_rr<T>, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, k, alpha, Q, R, partial);
+    hipLaunchKernelGGL(kb_update_r_rr<T>, dim3(rt.count), dim3(rowblock_threads(rt, k)), 0, s, rt, k, alpha, Q, R,
+                       partial);
 }
 template void bt_update_r_rr<double>(hipStream_t, Tiles, int, const double *, const double *, double *, double *);
 template void bt_update_r_rr<float>(hipStream_t, Tiles, int, const double *, const float *, float *, double *);
