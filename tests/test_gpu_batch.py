@@ -151,21 +151,32 @@ def test_multi_workgroup_rows_bit_identical(ctx, prec):
     """Hub rows of 1100..9000 entries (more than 1024: 4 or 8 workgroups
     per row, folding through scratch and an arrival counter) alone and
     mixed with a grid, k = 72 (two column chunks): the same bits, every
-    residual at the tolerance, and a second solve on the same resident
-    scratch gives the same bits again."""
+    residual at the tolerance, a second solve gives the same bits again,
+    and so do PCG iterations replayed from captured hipGraphs (the counters
+    are reset by the kernel itself, inside the graph)."""
     import spec_spars_amd as ss
     from spec_spars_amd import graphs
     n, u, v = graphs.grid2d(200, 200)
     gs = [(n, u, v, graphs.loguniform_weights(len(u), seed=3)), _hubs(nh=12, seed=5, lo=1100, hi=9000)]
     fs = _factors(gs, seed=9, ordering=ss.ORDER_ROUNDS)
     k = 72
-    mixed = _run(ctx, gs, fs, [0, 1], k, prec)
-    again = _run(ctx, gs, fs, [0, 1], k, prec)
+    saved = ctx.tuning()
+    try:
+        ctx.set_tuning(tile_waves=saved["tile_waves"], level_rows_per_wave=saved["level_rows_per_wave"],
+                       graph_iters=0)
+        mixed = _run(ctx, gs, fs, [0, 1], k, prec)
+        again = _run(ctx, gs, fs, [0, 1], k, prec)
+        ctx.set_tuning(tile_waves=saved["tile_waves"], level_rows_per_wave=saved["level_rows_per_wave"],
+                       graph_iters=2)
+        replayed = _run(ctx, gs, fs, [0, 1], k, prec)
+    finally:
+        ctx.set_tuning(**{x: saved[x] for x in ("tile_waves", "level_rows_per_wave", "graph_iters", "xcd_slices")})
     for i in range(2):
         alone = _run(ctx, gs, fs, [i], k, prec)[0]
         for key in ("Y", "Z", "p"):
             assert np.array_equal(alone[key], mixed[i][key]), (i, key)
             assert np.array_equal(again[i][key], mixed[i][key]), (i, key)
+            assert np.array_equal(replayed[i][key], mixed[i][key]), (i, key)
         n, u, v, w = gs[i]
         L = _laplacian(n, u, v, w)
         Yp = _component_projection(n, u, v, mixed[i]["Y"])
