@@ -284,7 +284,9 @@ def test_rounds_factor_solve_matches_permuted_oracle(ctx, oracle_mod):
     graph (one large component and isolated vertices, the right-hand side
     projected per component) has a backward solve that starts with a run of
     more than 256 small levels (several LDS stages of the small-level
-    kernel), and hub rows of more than 1,024 entries."""
+    kernel), and hub rows of more than 1,024 entries.  A mixed-precision
+    solve of the same system gives leverage scores within 1e-4 of the
+    oracle's."""
     import scipy.sparse as sp
     import scipy.sparse.csgraph as cg
     import spec_spars_amd as ss
@@ -308,6 +310,7 @@ def test_rounds_factor_solve_matches_permuted_oracle(ctx, oracle_mod):
 
     Y = project(oracle_mod.sketch(u, v, w, n, 4, seed=1, scale=0.5))
     Z, info = G.solve(Y, ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-10, max_iter=300, precision=ss.PREC_FP64))
+    Zm, _ = G.solve(Y, ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-10, max_iter=300, precision=ss.PREC_MIXED))
     G.close()
     order, _ = F.order()
     L = sp.csc_matrix((va, ri, cp), shape=(n, n))
@@ -321,3 +324,12 @@ def test_rounds_factor_solve_matches_permuted_oracle(ctx, oracle_mod):
     Zo[order] = Zp
     a, b = project(Z), project(Zo)
     assert (np.linalg.norm(a - b, axis=0) / np.linalg.norm(b, axis=0)).max() < 1e-7
+    # mixed precision (FP32 solves, FP64 reductions and refinement):
+    # leverage scores within 1e-4 of the FP64 oracle's
+    eu, ev, lv = oracle_mod.upper_edges(n, cp, ri, va)
+    beta = graphs.beta_for(n, 0.3)
+    po = oracle_mod.probs(lv, oracle_mod.diff_norms(eu, ev, Zo), beta, 4)
+    pm = oracle_mod.probs(lv, oracle_mod.diff_norms(eu, ev, Zm), beta, 4)
+    both = (pm < 1) & (po < 1)
+    assert both.sum() > 100000
+    assert np.all(np.abs(pm[both] - po[both]) <= 1e-4 * po[both])
