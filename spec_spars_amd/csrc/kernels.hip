@@ -320,18 +320,30 @@ __device__ __forceinline__ void wave_tiny_rows(int tb, int cnt, const int32_t *_
 }
 
 // A level whose rows have no entries (the first forward level: the rows
-// that depend on nothing): every row is its start value.  One thread per
-// (row, column), a flat streaming copy.
+// that depend on nothing): every row is its start value.  Thread i of a
+// block copies column i % k of rows i / k, i / k + 256 / k, ..., four rows
+// in flight, so a block pass moves whole rows as one contiguous run.
 template <typename T, bool BWD>
-__global__ __launch_bounds__(256) void k_level_copy(int32_t t0, int64_t count, const int32_t *__restrict__ rows,
+__global__ __launch_bounds__(256) void k_level_copy(int32_t t0, int32_t nrows, const int32_t *__restrict__ rows,
                                                     const double *__restrict__ diag, int k,
                                                     const T *__restrict__ in, T *__restrict__ out) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= count) return;
-    const int64_t t = e / k;
-    const int j = (int)(e - t * k);
-    const int i = rows[t0 + t];
-    out[(size_t)i * k + j] = row_start<T, BWD>(i, j, k, BWD ? diag[i] : 1.0, in);
+    // thread i copies column i % k of every (256 / k)-th row, 4 rows in flight
+    const int RP = (int)blockDim.x / k, i = (int)threadIdx.x;
+    if (i >= RP * k) return;
+    const int j = i % k, ro = i / k;
+    const int64_t step = (int64_t)gridDim.x * RP * 4;
+    for (int64_t tb = (int64_t)blockIdx.x * RP * 4 + ro; tb < nrows; tb += step) {
+        int r[4];
+        T v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) r[u] = tb + u * RP < nrows ? rows[t0 + tb + u * RP] : 0;
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            v[u] = tb + u * RP < nrows ? row_start<T, BWD>(r[u], j, k, BWD ? diag[r[u]] : 1.0, in) : (T)0;
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (tb + u * RP < nrows) out[(size_t)r[u] * k + j] = v[u];
+    }
 }
 
 // Rows of more than kWgRowMax entries (positions [t0, t0 + count)), split
@@ -958,9 +970,9 @@ void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nhuge, int32_t hm,
     }
     if (t1 <= t0) return;
     if (waves == 0) {   // a level of rows without entries
-        const int64_t count = (int64_t)(t1 - t0) * k;
-        hipLaunchKernelGGL((k_level_copy<T, BWD>), dim3(grid_for(count, 256)), dim3(256), 0, s, t0, count, rows, diag,
-                           k, in, out);
+        const int32_t nrows = t1 - t0;
+        hipLaunchKernelGGL((k_level_copy<T, BWD>), dim3(grid_for(nrows, 4 * (256 / k), 1 << 16)), dim3(256), 0, s, t0,
+                           nrows, rows, diag, k, in, out);
         return;
     }
     if (rpw < 1) rpw = 1;
