@@ -333,3 +333,22 @@ def test_rounds_factor_solve_matches_permuted_oracle(ctx, oracle_mod):
     both = (pm < 1) & (po < 1)
     assert both.sum() > 100000
     assert np.all(np.abs(pm[both] - po[both]) <= 1e-4 * po[both])
+
+
+@pytest.mark.parametrize("k", [130, 256])
+def test_wide_blocks_match_oracle(ctx, oracle_mod, k):
+    """k above 128 (one row per pass in the row-block kernels, several
+    64-column chunks in the solves) up to the maximum, 256: the sketch is
+    bit-exact and the FP64 solve matches the oracle; k = 257 is refused."""
+    import spec_spars_amd as ss
+    n, u, v, w = _graph("grid")
+    Y = ctx.sketch(u, v, w, n, k, seed=2, scale=1.0 / math.sqrt(k))
+    assert np.array_equal(Y, oracle_mod.sketch(u, v, w, n, k, seed=2, scale=1.0 / math.sqrt(k)))
+    cp, ri, va, Zo, _ = _oracle_solution(oracle_mod, n, u, v, w, Y, 3, 1e-12)
+    for ordering in (ss.ORDER_NATURAL, ss.ORDER_ROUNDS):
+        Z, info = ctx.solve_lap(Y, cp, ri, va, ss.solve_opts(factor_seed=3, stop_rule=ss.STOP_REL, tol=1e-10,
+                                                             max_iter=2000, ordering=ordering))
+        assert info["max_relres"] < 1e-9
+        assert _rel_centered(Z, Zo).max() < 1e-7
+    with pytest.raises(ss.SSError):
+        ctx.solve_lap(np.zeros((n, 257)), cp, ri, va, ss.solve_opts(factor_seed=3))
