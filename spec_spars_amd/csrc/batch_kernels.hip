@@ -447,47 +447,41 @@ __global__ __launch_bounds__(1024) void kb_spmm_dot(Tiles rt, const int32_t *__r
     }
 }
 
-// R -= alpha Q with per-tile partials of |R|^2, in the same row-block
-// mapping: each thread sums its column over its rows in order, then the
-// column's RP threads are summed in row-offset order.
+// R -= alpha Q with per-tile partials of |R|^2.  Lanes are columns and the
+// waves split the rows, so a column's sum order does not depend on k (a
+// request's columns give the same bits alone or merged with others').
 template <typename T>
 __global__ __launch_bounds__(1024) void kb_update_r_rr(Tiles rt, int k, const double *__restrict__ alpha,
                                                       const T *__restrict__ Q, T *__restrict__ R,
                                                       double *__restrict__ partial) {
-    __shared__ double sm[1024];
+    __shared__ double sm[kMaxTileWaves][64];
     const int t = blockIdx.x, g = rt.graph[t];
     const int r0 = rt.start[t], r1 = rt.start[t + 1];
-    const int RP = (int)blockDim.x / k, i = (int)threadIdx.x;
-    double acc = 0.0;
-    if (i < RP * k) {
-        const int j = i % k, ro = i / k;
-        const T a = (T)alpha[(size_t)g * k + j];
-        for (int rb = r0 + ro; rb < r1; rb += kBtRows * RP) {
-            T q[kBtRows], rv[kBtRows];
+    for (int jb = 0; jb < k; jb += 64) {
+        const int j = jb + BT_LANE;
+        double acc = 0.0;
+        if (j < k) {
+            const T a = (T)alpha[(size_t)g * k + j];
+            for (int rb = r0 + BT_WAVE; rb < r1; rb += kBtRows * BT_NW) {
+                T q[kBtRows], rv[kBtRows];
 #pragma unroll
-            for (int u = 0; u < kBtRows; u++) {
-                const int r = rb + u * RP;
-                const bool on = r < r1;
-                q[u] = on ? Q[(size_t)r * k + j] : (T)0;
-                rv[u] = on ? R[(size_t)r * k + j] : (T)0;
-            }
+                for (int u = 0; u < kBtRows; u++) {
+                    const int r = rb + u * BT_NW;
+                    const bool on = r < r1;
+                    q[u] = on ? Q[(size_t)r * k + j] : (T)0;
+                    rv[u] = on ? R[(size_t)r * k + j] : (T)0;
+                }
 #pragma unroll
-            for (int u = 0; u < kBtRows; u++) {
-                const int r = rb + u * RP;
-                if (r >= r1) break;
-                const T v = bt_fma(-a, q[u], rv[u]);
-                R[(size_t)r * k + j] = v;
-                acc = bt_fma((double)v, (double)v, acc);
+                for (int u = 0; u < kBtRows; u++) {
+                    const int r = rb + u * BT_NW;
+                    if (r >= r1) break;
+                    const T v = bt_fma(-a, q[u], rv[u]);
+                    R[(size_t)r * k + j] = v;
+                    acc = bt_fma((double)v, (double)v, acc);
+                }
             }
         }
-    }
-    // column j of the tile: its RP threads' sums in row-offset order
-    sm[i] = acc;
-    __syncthreads();
-    if (i < k) {
-        double sum = sm[i];
-        for (int q = 1; q < RP; q++) sum += sm[q * k + i];
-        partial[(size_t)t * k + i] = sum;
+        bt_tile_store(acc, k, jb, partial, sm);
     }
 }
 
@@ -712,9 +706,9 @@ template void bt_project<double, double>(hipStream_t, Tiles, int, const double *
                                          double *, double *, double *);
 template void bt_project<float, float>(hipStream_t, Tiles, int, const float *, const int32_t *, const double *,
                                        float *, float *, float *);
-// Row-block launches (update_px, update_r_rr): thread i of a tile block
-// works on column i % k of every (blockDim / k)-th row, so at least k
-// threads are needed.
+// Row-block launches (update_px: no reduction, so the mapping leaves every
+// value's bits alone): thread i of a tile block works on column i % k of
+// every (blockDim / k)-th row, so at least k threads are needed.
 static int rowblock_threads(const Tiles &rt, int k) {
     const int w = 64 * rt.waves, need = (k + 63) / 64 * 64;
     return w > need ? w : need;
@@ -745,8 +739,7 @@ template void bt_spmm_dot<float>(hipStream_t, Tiles, const int32_t *, const int3
                                  const float *, float *, double *);
 template <typename T>
 void bt_update_r_rr(hipStream_t s, Tiles rt, int k, const double *alpha, const T *Q, T *R, double *partial) {
-    hipLaunchKernelGGL(kb_update_r_rr<T>, dim3(rt.count), dim3(rowblock_threads(rt, k)), 0, s, rt, k, alpha, Q, R,
-                       partial);
+    hipLaunchKernelGGL(kb_update_r_rr<T>, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, k, alpha, Q, R, partial);
 }
 template void bt_update_r_rr<double>(hipStream_t, Tiles, int, const double *, const double *, double *, double *);
 template void bt_update_r_rr<float>(hipStream_t, Tiles, int, const double *, const float *, float *, double *);
