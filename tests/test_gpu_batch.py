@@ -463,3 +463,29 @@ def test_graph_replay_equals_direct_launches(ctx, prec):
             B.close()
     finally:
         ctx.set_tuning(**{k: saved[k] for k in ("tile_waves", "level_rows_per_wave", "graph_iters", "xcd_slices")})
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_columns_independent_of_k(ctx, prec):
+    """A column's result does not depend on how many other columns share the
+    call: every per-column reduction (the tile partials, |R|^2, the SpMM's
+    P.Q) keeps an order that is fixed by the graph alone, so solving 8
+    right-hand sides, or the same 8 inside a call with 40, gives the same
+    bits (what merged serving requests rely on)."""
+    import spec_spars_amd as ss
+    from spec_spars_amd import graphs
+    n, u, v = graphs.rgg(3000, seed=5)
+    w = graphs.loguniform_weights(len(u), seed=5)
+    G = ss.Graph(ctx, n, u, v, w)
+    try:
+        cp, ri, va = G.laplacian()
+        G.set_factor(ss.Factor(cp, ri, va, seed=3, ordering=ss.ORDER_ROUNDS))
+        rng = np.random.default_rng(4)
+        Y = rng.standard_normal((n, 40))
+        opts = ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-8, max_iter=2000, precision=prec)
+        Z8, i8 = G.solve(Y[:, :8].copy(), opts)
+        Z40, i40 = G.solve(Y, opts)
+        assert np.array_equal(i8["iters"], i40["iters"][:8])
+        assert np.array_equal(Z8, Z40[:, :8])
+    finally:
+        G.close()
