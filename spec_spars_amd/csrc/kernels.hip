@@ -562,7 +562,7 @@ __global__ __launch_bounds__(256) void k_core_step(int s, const int32_t *__restr
     __shared__ T sm_z[B][64];   // finished values (column chunk): the previous block, then this block's rows
     __shared__ T sm_d[B][B];    // entries of D: this workgroup's rows x the previous block, then the diagonal block
     const int g = blockIdx.y, x = blockIdx.x;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = wave_uniform(threadIdx.x >> 6);
     const int c = core_n[g];
     const int nb = (c + B - 1) / B;
     const int blk = BWD ? nb - 1 - s : s;       // block finished by this step
@@ -589,6 +589,16 @@ __global__ __launch_bounds__(256) void k_core_step(int s, const int32_t *__restr
     if (r0 >= r1) return;   // uniform over the workgroup
     const int nr = r1 - r0;
     constexpr int DPT = B * B / 256;   // entries of a 64 x 64 block per thread
+    // vertex ids of this wave's rows and of its rows of the previous block,
+    // all loaded together up front: the value loads below then wait on one
+    // batch instead of one id at a time
+    int vr[RW], vp[RW];
+#pragma unroll
+    for (int u = 0; u < RW; u++) {
+        const int r = wv * RW + u, q = wv + 4 * u;
+        vr[u] = r < nr ? vg[r0 + r] : 0;
+        vp[u] = q < np ? vg[p0 + q] : 0;
+    }
     for (int jb = 0; jb < k; jb += 64) {
         const int j = jb + lane;
         // every load of this workgroup's rows, the previous block's values,
@@ -598,7 +608,7 @@ __global__ __launch_bounds__(256) void k_core_step(int s, const int32_t *__restr
 #pragma unroll
         for (int u = 0; u < RW; u++) {
             const int r = wv * RW + u;
-            acc[u] = (r < nr && j < k) ? out[(size_t)vg[r0 + r] * k + j] : (T)0;
+            acc[u] = (r < nr && j < k) ? out[(size_t)vr[u] * k + j] : (T)0;
         }
         T dd[DPT];
         if (x == 0) {
@@ -613,7 +623,7 @@ __global__ __launch_bounds__(256) void k_core_step(int s, const int32_t *__restr
 #pragma unroll
             for (int u = 0; u < RW; u++) {
                 const int r = wv + 4 * u;
-                zv[u] = (r < np && j < k) ? out[(size_t)vg[p0 + r] * k + j] : (T)0;
+                zv[u] = (r < np && j < k) ? out[(size_t)vp[u] * k + j] : (T)0;
             }
 #pragma unroll
             for (int u = 0; u < DPT; u++) {
@@ -667,7 +677,7 @@ __global__ __launch_bounds__(256) void k_core_step(int s, const int32_t *__restr
 #pragma unroll
         for (int u = 0; u < RW; u++) {
             const int r = wv * RW + u;
-            if (r < nr && j < k) out[(size_t)vg[r0 + r] * k + j] = acc[u];
+            if (r < nr && j < k) out[(size_t)vr[u] * k + j] = acc[u];
         }
         __syncthreads();   // the next column chunk reuses the shared arrays
     }
