@@ -381,13 +381,18 @@ __global__ __launch_bounds__(1024) void kb_spmm_dot(Tiles rt, const int32_t *__r
         for (int rb = bt_uniform(r0 + BT_WAVE); rb < r1; rb += G * NW) {
             int rr[G], q0[G], len[G];
             int maxlen = 0;
+            // the group's row records: lane u < G loads row u's (rows past
+            // the tile read row rb's) in one vector load, read back per lane
+            const int lu = BT_LANE < G ? BT_LANE : 0;
+            const int ru = rb + lu * NW < r1 ? rb + lu * NW : rb;
+            const int va = rowptr[ru], vb = rowptr[ru + 1];
 #pragma unroll
             for (int u = 0; u < G; u++) {
-                const int r = rb + u * NW;
-                const bool on = r < r1;
-                rr[u] = on ? r : rb;
-                q0[u] = on ? rowptr[r] : 0;
-                len[u] = on ? rowptr[r + 1] - q0[u] : 0;
+                const bool on = rb + u * NW < r1;
+                rr[u] = on ? rb + u * NW : rb;
+                const int a = bt_lane(va, u), b = bt_lane(vb, u);
+                q0[u] = on ? a : 0;
+                len[u] = on ? b - a : 0;
                 maxlen = len[u] > maxlen ? len[u] : maxlen;
             }
             if (maxlen <= E) {
