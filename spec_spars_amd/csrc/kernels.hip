@@ -180,10 +180,11 @@ __device__ __forceinline__ T wave_parts(int len, int c0, int c1, T v0, T v1, con
 // (so part s0 + u takes its entries in order), the slots' entries load one
 // per lane, then gather 16 at a time.  Slots past the segment's end hold
 // zero values, whose terms leave a part exact.
-template <typename T, int NA>
+template <typename T, int NA, int GW = 16>
 __device__ __forceinline__ T wave_subtree(int a, int b, int P, int s0, const int32_t *__restrict__ col,
                                           const T *__restrict__ val, const T *X, int k, int jj, int lane) {
     static_assert(NA == 1 || NA == 2 || NA == 4 || NA == 8 || NA == 16, "parts per wave");
+    static_assert((GW == 8 || GW == 16) && NA <= GW, "gathers in flight: a whole number of slots per part");
     T part[NA];
 #pragma unroll
     for (int u = 0; u < NA; u++) part[u] = (T)0;
@@ -193,16 +194,16 @@ __device__ __forceinline__ T wave_subtree(int a, int b, int P, int s0, const int
         const int q = a0 + l % NA + P * (l / NA);
         const int c = q < b ? col[q] : 0;
         const T v = q < b ? val[q] : (T)0;
-        for (int e0 = 0; e0 < 64 && a0 + P * ((base + e0) / NA) < b; e0 += 16) {
-            T x[16];
+        for (int e0 = 0; e0 < 64 && a0 + P * ((base + e0) / NA) < b; e0 += GW) {
+            T x[GW];
 #pragma unroll
-            for (int u = 0; u < 16; u++) {
+            for (int u = 0; u < GW; u++) {
                 const int cu = lane_get(c, e0 + u);
                 const int lu = base + e0 + u;
                 x[u] = a0 + lu % NA + P * (lu / NA) < b ? X[(size_t)cu * k + jj] : (T)0;
             }
 #pragma unroll
-            for (int u = 0; u < 16; u++) part[u % NA] = fnms(lane_get(v, e0 + u), x[u], part[u % NA]);
+            for (int u = 0; u < GW; u++) part[u % NA] = fnms(lane_get(v, e0 + u), x[u], part[u % NA]);
         }
     }
     tree_fold<T, NA>(part);
@@ -378,7 +379,18 @@ __global__ __launch_bounds__(64 * kLevelWaves) void k_trisolve_huge(int32_t t0, 
         const int na = P / kWgParts;
         for (int jb = 0; jb < k; jb += 64) {
             const int j = jb + lane, jj = j < k ? j : k - 1;
-            sm[wv][lane] = wave_subtree_n<T, kMaxWg>(na, q0, q1, P, wv * na, col, val, out, k, jj, lane);
+            if constexpr (sizeof(T) == 8) {
+                // FP64: 8 parts as two subtrees of 4 (the same tree), 8
+                // gathers in flight, which keeps the kernel within the 128
+                // registers a 1,024-thread workgroup leaves each lane
+                if (na == 4)
+                    sm[wv][lane] = wave_subtree<T, 4, 8>(q0, q1, P, wv * 4, col, val, out, k, jj, lane);
+                else
+                    sm[wv][lane] = wave_subtree<T, 4, 8>(q0, q1, P, wv * 8, col, val, out, k, jj, lane) +
+                                   wave_subtree<T, 4, 8>(q0, q1, P, wv * 8 + 4, col, val, out, k, jj, lane);
+            } else {
+                sm[wv][lane] = wave_subtree_n<T, kMaxWg>(na, q0, q1, P, wv * na, col, val, out, k, jj, lane);
+            }
             __syncthreads();
             if (wv == 0 && j < k) {
                 const T acc = row_start<T, BWD>(i, jj, k, d, in);
@@ -392,7 +404,8 @@ __global__ __launch_bounds__(64 * kLevelWaves) void k_trisolve_huge(int32_t t0, 
     T *sc = scratch + ((size_t)slot * kMaxWg + h) * k;
     for (int jb = 0; jb < k; jb += 64) {
         const int j = jb + lane, jj = j < k ? j : k - 1;
-        sm[wv][lane] = wave_subtree<T, 1>(q0, q1, P, h * kWgParts + wv, col, val, out, k, jj, lane);
+        sm[wv][lane] = wave_subtree<T, 1, sizeof(T) == 8 ? 8 : 16>(q0, q1, P, h * kWgParts + wv, col, val, out, k,
+                                                                 jj, lane);
         __syncthreads();
         if (wv == 0 && j < k) sc[j] = lds_tree_n<T, kWgParts>(sm, 0, lane);
         __syncthreads();
