@@ -531,8 +531,10 @@ __global__ __launch_bounds__(256) void k_core_bwd_init(int32_t count, const int3
 // backward) takes its term, so each row subtracts its columns in order.
 // Rows past nr start at zero and their entries of sb are zero, so the
 // terms they add to a real row are exact zeros.
-// Rows of a dense-core block held by each of k_core_step's 4 waves.
-constexpr int kCoreRows = kCoreBlock / 4;
+// Waves of a k_core_step workgroup, and the rows of a dense-core block each
+// holds in registers.
+constexpr int kCoreWaves = 8;
+constexpr int kCoreRows = kCoreBlock / kCoreWaves;
 
 // Substitution inside one wave's rows of a dense-core block, in registers:
 // acc[u] is row base + u; sd[r][a] is the block entry of row r, column a.
@@ -556,7 +558,7 @@ __device__ __forceinline__ void core_sub_solve(T (&acc)[kCoreRows], const T (*sd
 // graph: workgroup x == 0 applies block s-1 to the rows of block s, then
 // substitutes inside block s; workgroups x >= 1 apply block s-1 to the rows
 // of the later blocks, 64 rows each.  Backward runs the blocks from the
-// last.  Each of the 4 waves keeps 16 rows in registers.  Inside block s
+// last.  Each of the 8 waves keeps 8 rows in registers.  Inside block s
 // the waves finish their rows in turn (a barrier each): the wave whose turn
 // it is substitutes among its own rows, then every wave still waiting takes
 // the finished rows' terms.  Every core row is one running fused
@@ -565,13 +567,13 @@ __device__ __forceinline__ void core_sub_solve(T (&acc)[kCoreRows], const T (*sd
 // graph alone.  Rows past the core's end start at zero and have zero
 // entries, so the terms they add to real rows are exact zeros.
 template <typename T, bool BWD>
-__global__ __launch_bounds__(256) void k_core_step(int s, const int32_t *__restrict__ core_n,
+__global__ __launch_bounds__(64 * kCoreWaves) void k_core_step(int s, const int32_t *__restrict__ core_n,
                                                     const int32_t *__restrict__ core_off,
                                                     const int64_t *__restrict__ dense_off,
                                                     const int32_t *__restrict__ vert, const T *__restrict__ D, int k,
                                                     T *out) {
     constexpr int B = kCoreBlock, RW = kCoreRows;
-    static_assert(RW * 4 == B, "4 waves cover a block");
+    static_assert(RW * kCoreWaves == B, "the waves cover a block");
     __shared__ T sm_z[B][64];   // finished values (column chunk): the previous block, then this block's rows
     __shared__ T sm_d[B][B];    // entries of D: this workgroup's rows x the previous block, then the diagonal block
     const int g = blockIdx.y, x = blockIdx.x;
@@ -601,14 +603,14 @@ __global__ __launch_bounds__(256) void k_core_step(int s, const int32_t *__restr
     }
     if (r0 >= r1) return;   // uniform over the workgroup
     const int nr = r1 - r0;
-    constexpr int DPT = B * B / 256;   // entries of a 64 x 64 block per thread
+    constexpr int NT = 64 * kCoreWaves, DPT = B * B / NT;   // entries of a 64 x 64 block per thread
     // vertex ids of this wave's rows and of its rows of the previous block,
     // all loaded together up front: the value loads below then wait on one
     // batch instead of one id at a time
     int vr[RW], vp[RW];
 #pragma unroll
     for (int u = 0; u < RW; u++) {
-        const int r = wv * RW + u, q = wv + 4 * u;
+        const int r = wv * RW + u, q = wv + kCoreWaves * u;
         vr[u] = r < nr ? vg[r0 + r] : 0;
         vp[u] = q < np ? vg[p0 + q] : 0;
     }
@@ -627,7 +629,7 @@ __global__ __launch_bounds__(256) void k_core_step(int s, const int32_t *__restr
         if (x == 0) {
 #pragma unroll
             for (int u = 0; u < DPT; u++) {
-                const int e = (int)threadIdx.x + 256 * u, a = e / B, b = e % B;
+                const int e = (int)threadIdx.x + NT * u, a = e / B, b = e % B;
                 dd[u] = (a < nr && b < nr) ? Dg[(size_t)(b0 + a) * c + b0 + b] : (T)0;
             }
         }
@@ -635,19 +637,19 @@ __global__ __launch_bounds__(256) void k_core_step(int s, const int32_t *__restr
             T zv[RW], dv[DPT];
 #pragma unroll
             for (int u = 0; u < RW; u++) {
-                const int r = wv + 4 * u;
+                const int r = wv + kCoreWaves * u;
                 zv[u] = (r < np && j < k) ? out[(size_t)vp[u] * k + j] : (T)0;
             }
 #pragma unroll
             for (int u = 0; u < DPT; u++) {
-                const int e = (int)threadIdx.x + 256 * u, a = e / B, b = e % B;
+                const int e = (int)threadIdx.x + NT * u, a = e / B, b = e % B;
                 dv[u] = (a < nr && b < np) ? Dg[(size_t)(r0 + a) * c + p0 + b] : (T)0;
             }
 #pragma unroll
-            for (int u = 0; u < RW; u++) sm_z[wv + 4 * u][lane] = zv[u];
+            for (int u = 0; u < RW; u++) sm_z[wv + kCoreWaves * u][lane] = zv[u];
 #pragma unroll
             for (int u = 0; u < DPT; u++) {
-                const int e = (int)threadIdx.x + 256 * u;
+                const int e = (int)threadIdx.x + NT * u;
                 sm_d[e / B][e % B] = dv[u];
             }
         }
@@ -664,12 +666,12 @@ __global__ __launch_bounds__(256) void k_core_step(int s, const int32_t *__restr
             __syncthreads();   // the panel's reads of sm_z and sm_d are done
 #pragma unroll
             for (int u = 0; u < DPT; u++) {
-                const int e = (int)threadIdx.x + 256 * u;
+                const int e = (int)threadIdx.x + NT * u;
                 sm_d[e / B][e % B] = dd[u];
             }
             __syncthreads();
-            for (int st = 0; st < 4; st++) {
-                const int w = BWD ? 3 - st : st;   // the wave whose rows finish now
+            for (int st = 0; st < kCoreWaves; st++) {
+                const int w = BWD ? kCoreWaves - 1 - st : st;   // the wave whose rows finish now
                 if (wv == w) {
                     core_sub_solve<T, BWD>(acc, sm_d, w * RW);
 #pragma unroll
@@ -953,7 +955,7 @@ void launch_core_step(hipStream_t s, int step, int graphs, int max_rows, const i
                       T *out) {
     if (graphs <= 0 || max_rows <= 0) return;
     const unsigned gx = 1 + (unsigned)((max_rows + kCoreBlock - 1) / kCoreBlock);
-    hipLaunchKernelGGL((k_core_step<T, BWD>), dim3(gx, graphs), dim3(256), 0, s, step, core_n, core_off, dense_off,
+    hipLaunchKernelGGL((k_core_step<T, BWD>), dim3(gx, graphs), dim3(64 * kCoreWaves), 0, s, step, core_n, core_off, dense_off,
                        vert, D, k, out);
 }
 #define SS_INST_CORE(T, B)                                                                                    \
