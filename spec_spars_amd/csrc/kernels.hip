@@ -533,7 +533,7 @@ __global__ __launch_bounds__(256) void k_core_bwd_init(int32_t count, const int3
 // terms they add to a real row are exact zeros.
 // Waves of a k_core_step workgroup, and the rows of a dense-core block each
 // holds in registers.
-constexpr int kCoreWaves = 8;
+constexpr int kCoreWaves = 16;
 constexpr int kCoreRows = kCoreBlock / kCoreWaves;
 
 // Substitution inside one wave's rows of a dense-core block, in registers:
@@ -558,7 +558,7 @@ __device__ __forceinline__ void core_sub_solve(T (&acc)[kCoreRows], const T (*sd
 // graph: workgroup x == 0 applies block s-1 to the rows of block s, then
 // substitutes inside block s; workgroups x >= 1 apply block s-1 to the rows
 // of the later blocks, 64 rows each.  Backward runs the blocks from the
-// last.  Each of the 8 waves keeps 8 rows in registers.  Inside block s
+// last.  Each of the 16 waves keeps 4 rows in registers.  Inside block s
 // the waves finish their rows in turn (a barrier each): the wave whose turn
 // it is substitutes among its own rows, then every wave still waiting takes
 // the finished rows' terms.  Every core row is one running fused
