@@ -62,6 +62,24 @@ __device__ __forceinline__ void bt_tile_store(double acc, int k, int jb, double 
     __syncthreads();
 }
 
+// The same for a 32-column chunk held by both half-waves (lanes l and l + 32
+// hold column jb + l of two row sets): the column's value is the sum over
+// waves, in order, of the two halves' sums.
+__device__ __forceinline__ void bt_tile_store32(double acc, int k, int jb, double *partial, double (*sm)[64]) {
+    const int w = BT_WAVE, lane = BT_LANE, j = jb + (lane & 31);
+    sm[w][lane] = acc;
+    __syncthreads();
+    if (w == 0 && lane < 32 && j < k) {
+        double s = 0.0;
+        for (int u = 0; u < BT_NW; u++) {
+            s += sm[u][lane];
+            s += sm[u][lane + 32];
+        }
+        partial[(size_t)blockIdx.x * k + j] = s;
+    }
+    __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
 // (a) sketch: block per row tile, wave per row, seed/scale of the tile's graph
 // ---------------------------------------------------------------------------
@@ -100,7 +118,8 @@ __global__ __launch_bounds__(1024) void kb_sketch(Tiles rt, const int32_t *__res
 
 // ---------------------------------------------------------------------------
 // per-tile column partials: op 0 sum A*B, op 1 sum A, op 2 sum y (y / D)
-// (y^T D^+ y with the reference's skip of D == 0)
+// (y^T D^+ y with the reference's skip of D == 0), in the row split of
+// update_r_rr (32-column chunks over both half-waves)
 // ---------------------------------------------------------------------------
 template <typename TA, typename TB>
 __global__ __launch_bounds__(1024) void kb_tile_partial(Tiles rt, int k, const TA *__restrict__ A,
@@ -109,30 +128,32 @@ __global__ __launch_bounds__(1024) void kb_tile_partial(Tiles rt, int k, const T
     __shared__ double sm[kMaxTileWaves][64];
     const int t = blockIdx.x;
     const int r0 = rt.start[t], r1 = rt.start[t + 1];
-    for (int jb = 0; jb < k; jb += 64) {
-        const int j = jb + BT_LANE;
+    // 32-column chunks; half-wave h of wave w takes rows 2 (w + NW m) + h
+    const int half = BT_LANE >> 5, NW2 = 2 * BT_NW;
+    for (int jb = 0; jb < k; jb += 32) {
+        const int j = jb + (BT_LANE & 31);
         double acc = 0.0;
         if (j < k) {
             // kBtRows rows per pass (their loads in flight together), added in row order
-            for (int rb = r0 + BT_WAVE; rb < r1; rb += kBtRows * BT_NW) {
+            for (int rb = r0 + 2 * BT_WAVE + half; rb < r1; rb += kBtRows * NW2) {
                 double a[kBtRows], b[kBtRows];
 #pragma unroll
                 for (int u = 0; u < kBtRows; u++) {
-                    const int r = rb + u * BT_NW;
+                    const int r = rb + u * NW2;
                     const bool on = r < r1;
                     a[u] = on ? (double)A[(size_t)r * k + j] : 0.0;
                     b[u] = on && op == 0 ? (double)B[(size_t)r * k + j] : (on && op == 2 ? diag[r] : 0.0);
                 }
 #pragma unroll
                 for (int u = 0; u < kBtRows; u++) {
-                    if (rb + u * BT_NW >= r1) break;
+                    if (rb + u * NW2 >= r1) break;
                     if (op == 0) acc = bt_fma(a[u], b[u], acc);
                     else if (op == 1) acc += a[u];
                     else acc = b[u] != 0.0 ? bt_fma(a[u], a[u] / b[u], acc) : bt_fma(a[u], a[u], acc);
                 }
             }
         }
-        bt_tile_store(acc, k, jb, partial, sm);
+        bt_tile_store32(acc, k, jb, partial, sm);
     }
 }
 
@@ -452,9 +473,11 @@ __global__ __launch_bounds__(1024) void kb_spmm_dot(Tiles rt, const int32_t *__r
     }
 }
 
-// R -= alpha Q with per-tile partials of |R|^2.  Lanes are columns and the
-// waves split the rows, so a column's sum order does not depend on k (a
-// request's columns give the same bits alone or merged with others').
+// R -= alpha Q with per-tile partials of |R|^2.  Columns go in chunks of
+// 32, each held by both half-waves, and the half-waves and waves split the
+// rows, so every lane is busy for k = 32 and a column's sum order does not
+// depend on k (a request's columns give the same bits alone or merged with
+// others').
 template <typename T>
 __global__ __launch_bounds__(1024) void kb_update_r_rr(Tiles rt, int k, const double *__restrict__ alpha,
                                                       const T *__restrict__ Q, T *__restrict__ R,
@@ -462,23 +485,25 @@ __global__ __launch_bounds__(1024) void kb_update_r_rr(Tiles rt, int k, const do
     __shared__ double sm[kMaxTileWaves][64];
     const int t = blockIdx.x, g = rt.graph[t];
     const int r0 = rt.start[t], r1 = rt.start[t + 1];
-    for (int jb = 0; jb < k; jb += 64) {
-        const int j = jb + BT_LANE;
+    // 32-column chunks; half-wave h of wave w takes rows 2 (w + NW m) + h
+    const int half = BT_LANE >> 5, NW2 = 2 * BT_NW;
+    for (int jb = 0; jb < k; jb += 32) {
+        const int j = jb + (BT_LANE & 31);
         double acc = 0.0;
         if (j < k) {
             const T a = (T)alpha[(size_t)g * k + j];
-            for (int rb = r0 + BT_WAVE; rb < r1; rb += kBtRows * BT_NW) {
+            for (int rb = r0 + 2 * BT_WAVE + half; rb < r1; rb += kBtRows * NW2) {
                 T q[kBtRows], rv[kBtRows];
 #pragma unroll
                 for (int u = 0; u < kBtRows; u++) {
-                    const int r = rb + u * BT_NW;
+                    const int r = rb + u * NW2;
                     const bool on = r < r1;
                     q[u] = on ? Q[(size_t)r * k + j] : (T)0;
                     rv[u] = on ? R[(size_t)r * k + j] : (T)0;
                 }
 #pragma unroll
                 for (int u = 0; u < kBtRows; u++) {
-                    const int r = rb + u * BT_NW;
+                    const int r = rb + u * NW2;
                     if (r >= r1) break;
                     const T v = bt_fma(-a, q[u], rv[u]);
                     R[(size_t)r * k + j] = v;
@@ -486,7 +511,7 @@ __global__ __launch_bounds__(1024) void kb_update_r_rr(Tiles rt, int k, const do
                 }
             }
         }
-        bt_tile_store(acc, k, jb, partial, sm);
+        bt_tile_store32(acc, k, jb, partial, sm);
     }
 }
 
