@@ -392,7 +392,9 @@ __global__ __launch_bounds__(1024) void kb_spmm_dot(Tiles rt, const int32_t *__r
                                                    const int32_t *__restrict__ col, const T *__restrict__ val, int k,
                                                    const T *__restrict__ P, T *__restrict__ Q,
                                                    double *__restrict__ partial) {
-    constexpr int G = kSpmmGroup, E = kSpmmShort;
+    // FP32 rows of up to 16 entries take the all-gathers-in-flight path (FP64
+    // at 16 would not fit the registers)
+    constexpr int G = kSpmmGroup, E = sizeof(T) == 4 ? 2 * kSpmmShort : kSpmmShort;
     __shared__ double sm[kMaxTileWaves][64];
     const int t = blockIdx.x, NW = BT_NW, lane = BT_LANE;
     const int r0 = rt.start[t], r1 = rt.start[t + 1];
