@@ -524,6 +524,8 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    G.close()
+    ctx.close()
     return 0
 
 

@@ -137,7 +137,11 @@ typedef struct ss_solve_report {
     double ms_solve;        /* out: device solve time                        */
 } ss_solve_report;
 
-void ss_solve_opts_default(ss_solve_opts *o);   /* reference settings */
+/* The reference's stop rule (REF_ABS, tol 2e-7, 400 iterations, fail at a
+ * relative residual of 1e-2), FP64, factor seed 0 and SS_ORDER_ROUNDS: the
+ * rounds order keeps the level-scheduled solves shallow (the natural order
+ * gives a DAG of depth ~n/2 on grids, one launch per level). */
+void ss_solve_opts_default(ss_solve_opts *o);
 int ss_solve_lap(ss_context *ctx, const double *Y, int64_t n, int32_t k,
                  const int32_t *colptr, const int32_t *rowidx,
                  const double *vals, int64_t nnz, const ss_solve_opts *opts,
@@ -302,7 +306,10 @@ int ss_graph_profile_read(const ss_graph *g, ss_kernel_stat *out /* [SS_KERNEL_C
 typedef struct ss_tuning {
     int32_t tile_waves;            /* 1, 2, 4, 8 or 16 (default 4)           */
     int32_t level_rows_per_wave;   /* 1 .. 64 (default 1)                    */
-    int32_t graph_iters;           /* 0 .. 16 (default 0; the autotuner picks 0 or 4) */
+    int32_t graph_iters;           /* 0 .. 16 (default 0, which the autotuner
+                                      keeps: replay never beat direct launches);
+                                      iterations whose capture would exceed
+                                      1,024 kernel nodes launch directly      */
     int32_t xcd_slices;            /* 1: each of the 8 XCDs takes one contiguous
                                       slice of a level's rows; 0: round robin
                                       (default; the autotuner picks)          */

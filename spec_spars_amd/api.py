@@ -4,6 +4,7 @@ Every call goes to libspecspars_hip.so; device work runs on the HIP device of
 the Context.  Host-only pieces (Factor) need no GPU.
 """
 import ctypes
+import sys
 import weakref
 
 import numpy as np
@@ -20,7 +21,7 @@ def _p(a, ct):
 
 
 def solve_opts(factor_seed=0, stop_rule=_lib.STOP_REF_ABS, max_iter=400, tol=None,
-               fail_relres=1e-2, precision=_lib.PREC_FP64, ordering=_lib.ORDER_NATURAL):
+               fail_relres=1e-2, precision=_lib.PREC_FP64, ordering=_lib.ORDER_ROUNDS):
     o = _lib.SolveOpts()
     load().ss_solve_opts_default(ctypes.byref(o))
     o.factor_seed = factor_seed
@@ -36,7 +37,7 @@ def solve_opts(factor_seed=0, stop_rule=_lib.STOP_REF_ABS, max_iter=400, tol=Non
 class Factor:
     """Host approximate-Cholesky factor of a Laplacian given as CSC."""
 
-    def __init__(self, colptr, rowidx, vals, seed=0, ordering=_lib.ORDER_NATURAL):
+    def __init__(self, colptr, rowidx, vals, seed=0, ordering=_lib.ORDER_ROUNDS):
         lib = load()
         self._cp, self._ri, self._v = _i32(colptr), _i32(rowidx), _f64(vals)
         self.n = len(self._cp) - 1
@@ -79,7 +80,11 @@ class Factor:
             load().ss_factor_destroy(self._h)
             self._h = None
 
-    __del__ = close
+    def __del__(self):
+        # at interpreter exit the HIP runtime may already be torn down:
+        # leave the memory to the process exit instead of calling into it
+        if not sys.is_finalizing():
+            self.close()
 
 
 # blocks readable after a run: sketch, solution, and (FP64) the last
@@ -105,7 +110,11 @@ class Context:
             load().ss_context_destroy(self._h)
             self._h = None
 
-    __del__ = close
+    def __del__(self):
+        # at interpreter exit the HIP runtime may already be torn down:
+        # leave the memory to the process exit instead of calling into it
+        if not sys.is_finalizing():
+            self.close()
 
     def mem_info(self):
         f, t = ctypes.c_size_t(), ctypes.c_size_t()
@@ -211,7 +220,11 @@ class Graph:
             load().ss_graph_destroy(self._h)
             self._h = None
 
-    __del__ = close
+    def __del__(self):
+        # at interpreter exit the HIP runtime may already be torn down:
+        # leave the memory to the process exit instead of calling into it
+        if not sys.is_finalizing():
+            self.close()
 
     @staticmethod
     def bytes_needed(n, m, factor_nnz, k, precision):
@@ -338,7 +351,11 @@ class Batch:
             load().ss_graph_destroy(self._h)
             self._h = None
 
-    __del__ = close
+    def __del__(self):
+        # at interpreter exit the HIP runtime may already be torn down:
+        # leave the memory to the process exit instead of calling into it
+        if not sys.is_finalizing():
+            self.close()
 
     def _index(self, i):
         if not 0 <= i < self.count:

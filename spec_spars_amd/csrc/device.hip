@@ -818,6 +818,15 @@ static int pcg_iteration(ss_graph *g, int k, T *X, const PcgWork<T> &w, int rule
     return SS_OK;
 }
 
+// Kernel launches of one PCG iteration with its preconditioner application
+// (an upper bound: per-level launches of both directions, small-level runs,
+// dense-core steps, vector updates and reductions).
+static int64_t launches_per_iteration(const ss_graph *g) {
+    const int64_t levels = (int64_t)g->f_level_ptr.size() + (int64_t)g->b_level_ptr.size();
+    return levels + 2 * (int64_t)(g->core_steps + 1) + 16;
+}
+constexpr int64_t kGraphMaxNodes = 1024;
+
 template <typename T> static GraphCache &graph_cache(ss_graph *g);
 template <> GraphCache &graph_cache<double>(ss_graph *g) { return g->gc64; }
 template <> GraphCache &graph_cache<float>(ss_graph *g) { return g->gc32; }
@@ -843,7 +852,12 @@ static int pcg_run(ss_graph *g, int k, const T *rhs, const double *mean_src, T *
     SS_TRY(precond<T>(g, k, w.R, w.W, w.Zp, g->partial2));
     bt_tile_partial<T, T>(s, rt, k, w.B, w.B, nullptr, 0, g->partial);
     bt_pcg_init(s, gk, k, g->gtile, g->partial, g->partial2, g->sc);   // beta = 0: P starts as Z
-    const int m = g->prof ? 0 : g->ctx->tune.graph_iters;
+    // captured iterations only for schedules of at most kGraphMaxNodes
+    // launches per capture: a deep factor (thousands of level launches per
+    // iteration) is launched directly
+    const int m = g->prof || g->ctx->tune.graph_iters * launches_per_iteration(g) > kGraphMaxNodes
+                      ? 0
+                      : g->ctx->tune.graph_iters;
     int it = 0;
     if (m > 0) {
         // whole iterations replayed from a captured graph, the host reading
@@ -1055,40 +1069,10 @@ static int autotune_t(ss_graph *g, int k, const PcgWork<T> &w, int reps, ss_tuni
             best.tile_waves = tw;
         }
     }
-    // direct launches against graph replay, 4 PCG iterations each (the
-    // state need not be meaningful for timing, only valid)
-    {
-        const Tiles rt = rtiles(g);
-        const int gk = g->G * k;
-        T *X = std::is_same<T, double>::value ? (T *)g->X64 : (T *)g->X32;
-        bt_project<T, T>(s, rt, k, w.R, g->comp_of_row, g->cmean, w.R, w.B, X);
-        SS_TRY(precond<T>(g, k, w.R, w.W, w.Zp, g->partial2));
-        bt_tile_partial<T, T>(s, rt, k, w.B, w.B, nullptr, 0, g->partial);
-        bt_pcg_init(s, gk, k, g->gtile, g->partial, g->partial2, g->sc);
-        constexpr int M = 4;
-        double ms_direct = 0, ms_graph = 1e30;
-        SS_TRY(time_median(g, reps, [&]() {
-            for (int r = 0; r < M; r++) SS_TRY(pcg_iteration<T>(g, k, X, w, SS_STOP_REL, 0.0, true));
-            return SS_OK;
-        }, ms_direct));
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        int rc = SS_OK;
-        SS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
-        for (int r = 0; r < M && rc == SS_OK; r++) rc = pcg_iteration<T>(g, k, X, w, SS_STOP_REL, 0.0, true);
-        const hipError_t e = hipStreamEndCapture(s, &graph);
-        if (rc == SS_OK && e == hipSuccess && graph && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess) {
-            rc = time_median(g, reps, [&]() {
-                SS_HIP(hipGraphLaunch(exec, s));
-                return SS_OK;
-            }, ms_graph);
-        }
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        (void)hipGetLastError();
-        SS_TRY(rc);
-        best.graph_iters = ms_graph < 0.97 * ms_direct ? M : 0;
-    }
+    // hipGraph replay is not a candidate: it never beat direct launches
+    // (the solves are kernel-bound), and a capture of a deep factor holds
+    // thousands of nodes (see pcg_run's node cap)
+    best.graph_iters = 0;
     g->tile_waves = orig_waves;
     g->ctx->tune = orig;
     g->prof = prof;

@@ -3,6 +3,7 @@ reference's SparsifierParameters / Sparsifier interface (src/sparsifier.rs)
 with the GPU path underneath.  Method names and argument meanings follow the
 Rust code; its panics surface as SSError."""
 import ctypes
+import sys
 
 import numpy as np
 
@@ -56,7 +57,11 @@ class Sparsifier:
             load().ssh_sparsifier_destroy(self._h)
             self._h = None
 
-    __del__ = close
+    def __del__(self):
+        # at interpreter exit the HIP runtime may already be torn down:
+        # leave the memory to the process exit instead of calling into it
+        if not sys.is_finalizing():
+            self.close()
 
     def insert(self, v1, v2, value):
         self.insert_many([v1], [v2], [value])

@@ -336,7 +336,7 @@ def test_c1_grid64_pipeline(ctx, oracle_mod):
     k = 24
     G = ss.Graph(ctx, n, u, v, w)
     cp, ri, va = G.laplacian()
-    f = ss.Factor(cp, ri, va, seed=0)
+    f = ss.Factor(cp, ri, va, seed=0, ordering=ss.ORDER_NATURAL)
     G.set_factor(f)
     beta = graphs.beta_for(n, 0.3)
     rep = G.sparsify(k, beta, scale=1.0 / math.sqrt(k), sketch_seed=0, sampling_seed=0,

@@ -58,7 +58,7 @@ def test_sketch_empty_and_isolated(ctx, oracle_mod):
 def _oracle_solution(oracle_mod, n, u, v, w, Y, seed, tol):
     import spec_spars_amd as ss
     cp, ri, va = oracle_mod.laplacian(n, u, v, w)
-    f = ss.Factor(cp, ri, va, seed=seed)
+    f = ss.Factor(cp, ri, va, seed=seed, ordering=ss.ORDER_NATURAL)
     urp, uci, uva, dg = f.export()
     Zo, it, rr = oracle_mod.solve_lap(Y, cp, ri, va, urp, uci, uva, dg, stop_rule=1, tol=tol, max_iter=2000)
     return cp, ri, va, Zo, it
@@ -102,9 +102,9 @@ def test_solve_reference_stop_rule(ctx, oracle_mod):
     n, u, v, w = _graph("grid")
     Y = ctx.sketch(u, v, w, n, 8, seed=2, scale=math.sqrt(3.0))
     cp, ri, va = oracle_mod.laplacian(n, u, v, w)
-    Z, info = ctx.solve_lap(Y, cp, ri, va, ss.solve_opts(factor_seed=1))     # ||r||^2 <= 2e-7, 400 its
+    Z, info = ctx.solve_lap(Y, cp, ri, va, ss.solve_opts(factor_seed=1, ordering=ss.ORDER_NATURAL))  # ||r||^2 <= 2e-7, 400 its
     assert info["max_relres"] < 1e-2
-    f = ss.Factor(cp, ri, va, seed=1)
+    f = ss.Factor(cp, ri, va, seed=1, ordering=ss.ORDER_NATURAL)
     urp, uci, uva, dg = f.export()
     Zo, it, rr = oracle_mod.solve_lap(Y, cp, ri, va, urp, uci, uva, dg, stop_rule=0, tol=2e-7)
     assert abs(int(info["iters"].max()) - int(it.max())) <= 2
@@ -201,7 +201,7 @@ def test_graph_sparsify_pipeline(ctx, oracle_mod):
     k = 24
     G = ss.Graph(ctx, n, u, v, w)
     cp, ri, va = G.laplacian()
-    f = ss.Factor(cp, ri, va, seed=2)
+    f = ss.Factor(cp, ri, va, seed=2, ordering=ss.ORDER_NATURAL)
     G.set_factor(f)
     beta = graphs.beta_for(n, 0.3)
     rep = G.sparsify(k, beta, scale=1.0 / math.sqrt(k), sketch_seed=0, sampling_seed=0,
@@ -262,7 +262,7 @@ def test_graph_sparsify_ordered_mixed(ctx, oracle_mod):
     assert all(prof[x]["launches"] > 0 for x in ("sketch", "spmm", "tri_fwd", "tri_bwd", "er_sample"))
     assert rep["max_relres"] <= 1e-8
     Yo = oracle_mod.sketch(u, v, w, n, k, seed=3, scale=1.0 / math.sqrt(k))
-    f = ss.Factor(cp, ri, va, seed=2)
+    f = ss.Factor(cp, ri, va, seed=2, ordering=ss.ORDER_NATURAL)
     Zo, _, _ = oracle_mod.solve_lap(Yo, cp, ri, va, *f.export(), stop_rule=1, tol=1e-12, max_iter=2000)
     Z = G.read_block("Z")
     assert _rel_centered(Z, Zo).max() < 1e-6

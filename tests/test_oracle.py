@@ -148,7 +148,7 @@ def test_pcg_matches_exact_solve(oracle_mod, name):
     g = golden(name)
     n = int(g["n"])
     cp, ri, va = oracle_mod.laplacian(n, g["u"], g["v"], g["w"])
-    f = ss.Factor(cp, ri, va, seed=4)
+    f = ss.Factor(cp, ri, va, seed=4, ordering=ss.ORDER_NATURAL)
     urp, uci, uva, dg = f.export()
     Z, it, rr = oracle_mod.solve_lap(g["Y"], cp, ri, va, urp, uci, uva, dg, stop_rule=1, tol=1e-11)
     assert components(n, g["u"], g["v"]) == 1
