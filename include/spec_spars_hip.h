@@ -79,6 +79,14 @@ int ss_factor_build(int64_t n, const int32_t *colptr, const int32_t *rowidx,
 int ss_factor_build_ordered(int64_t n, const int32_t *colptr, const int32_t *rowidx,
                             const double *vals, uint64_t seed, int32_t ordering,
                             ss_factor **out);
+/* Host threads the factorization uses (0 = default: SPEC_SPARS_THREADS, else
+ * the hardware threads capped at 32, the reference's OpenMP team size,
+ * src/example.cc:672).  The factor does not depend on it.  Process-wide. */
+int ss_factor_set_threads(int32_t threads);
+/* Rows of the dense tail (SS_ORDER_ROUNDS): positions [n - tail, n) were
+ * eliminated one at a time once the remaining graph was nearly dense; the
+ * device applies that block of L as a dense inverse.  0 for other orders. */
+int ss_factor_tail(const ss_factor *f, int64_t *tail);
 /* Elimination order: order[p] = vertex eliminated at position p (identity
  * for the natural order); rounds = number of rounds (SS_ORDER_ROUNDS). */
 int ss_factor_order(const ss_factor *f, int32_t *order, int32_t *rounds);
@@ -267,8 +275,10 @@ int ss_batch_read_probs(const ss_graph *g, int32_t i, double *p);
 #define SS_KERNEL_TRI_FWD_SMALL 5 /* forward run of small levels (1 workgroup) */
 #define SS_KERNEL_TRI_BWD_SMALL 6 /* backward run of small levels            */
 #define SS_KERNEL_VECTOR 7      /* PCG vector updates and reductions         */
-#define SS_KERNEL_CORE_FWD 8    /* dense cores, forward: outside entries + block steps */
-#define SS_KERNEL_CORE_BWD 9    /* dense cores, backward: y/D + block steps  */
+#define SS_KERNEL_CORE_FWD 8    /* dense cores, forward: outside entries, gather,
+                                   inverse product, reduction               */
+#define SS_KERNEL_CORE_BWD 9    /* dense cores, backward: gather of y/D,
+                                   transposed inverse product, reduction    */
 #define SS_KERNEL_COUNT 10
 typedef struct ss_kernel_stat {
     double ms_total;
@@ -278,10 +288,13 @@ int ss_graph_profile(ss_graph *g, int enable);
 /* Shape of the uploaded solve schedules: per direction (0 = forward,
  * 1 = backward) the level count, the rows and entries handled by per-level
  * launches and by the single-workgroup runs of small levels (small_runs of
- * them, small_levels levels in all), and the dense cores: core_rows rows of
- * the graphs' trailing small levels, core_entries = sum of core_rows_g^2
- * stored values, core_steps block steps, core_outside_nnz entries of core
- * rows outside the cores (forward only). */
+ * them, small_levels levels in all), and the dense cores -- per graph the
+ * last rows of its elimination order (the factor's dense tail, else its
+ * trailing small levels), applied as the inverse of their block of L:
+ * core_rows rows, core_entries = sum of padded core_rows_g^2 stored values
+ * of the inverses, core_steps = product pieces per direction (64 rows times
+ * at most 2,048 inner rows each), core_outside_nnz entries of core rows
+ * outside the cores (forward only). */
 typedef struct ss_schedule_info {
     int32_t levels, level_launches, small_levels, small_runs;
     int64_t level_rows, level_nnz, small_rows, small_nnz;

@@ -11,6 +11,6 @@ from ._lib import (SSError, PREC_FP64, PREC_MIXED, STOP_REF_ABS, STOP_REL,  # no
 
 _lib.load()
 
-from .api import Batch, Context, Factor, Graph, solve_opts  # noqa: E402,F401
+from .api import Batch, Context, Factor, Graph, set_factor_threads, solve_opts  # noqa: E402,F401
 from . import graphs  # noqa: E402,F401
 from .sparsifier import SparsifierParameters, Sparsifier  # noqa: E402,F401

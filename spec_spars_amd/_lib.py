@@ -106,6 +106,8 @@ SIGNATURES = {
     "ss_factor_build_ordered": (ctypes.c_int, [ctypes.c_int64, c_i32p, c_i32p, c_f64p, ctypes.c_uint64,
                                                ctypes.c_int32, ctypes.POINTER(c_vp)]),
     "ss_factor_order": (ctypes.c_int, [c_vp, c_i32p, c_i32p]),
+    "ss_factor_set_threads": (ctypes.c_int, [ctypes.c_int32]),
+    "ss_factor_tail": (ctypes.c_int, [c_vp, c_i64p]),
     "ss_factor_info": (ctypes.c_int, [c_vp, c_i64p, c_i64p, c_i32p, c_i32p]),
     "ss_factor_export": (ctypes.c_int, [c_vp, c_i32p, c_i32p, c_f64p, c_f64p]),
     "ss_factor_levels": (ctypes.c_int, [c_vp, c_i32p, c_i32p]),

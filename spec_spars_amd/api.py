@@ -34,6 +34,12 @@ def solve_opts(factor_seed=0, stop_rule=_lib.STOP_REF_ABS, max_iter=400, tol=Non
     return o
 
 
+def set_factor_threads(threads=0):
+    """Host threads of the factorization (0 = default); the factor does not
+    depend on it."""
+    check(load().ss_factor_set_threads(int(threads)))
+
+
 class Factor:
     """Host approximate-Cholesky factor of a Laplacian given as CSC."""
 
@@ -51,6 +57,12 @@ class Factor:
         r = ctypes.c_int32()
         check(load().ss_factor_order(self._h, _p(o, ctypes.c_int32), ctypes.byref(r)))
         return o, r.value
+
+    def tail(self):
+        """Rows of the dense tail (rounds order): the last positions."""
+        t = ctypes.c_int64()
+        check(load().ss_factor_tail(self._h, ctypes.byref(t)))
+        return t.value
 
     def info(self):
         n, nnz = ctypes.c_int64(), ctypes.c_int64()

@@ -12,7 +12,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libspecspars_hip.so")
 
-HIP_SOURCES = ["kernels.hip", "batch_kernels.hip", "device.hip"]
+HIP_SOURCES = ["kernels.hip", "core_kernels.hip", "batch_kernels.hip", "device.hip"]
 CPP_SOURCES = ["factor.cpp", "graph.cpp", "capi.cpp", "sparsifier.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -37,17 +37,20 @@ def _run(cmd):
 
 def build_library(verbose=False):
     """Compile every HIP/C++ source for gfx950 and link the C-ABI library."""
+    from concurrent.futures import ThreadPoolExecutor
     objdir = os.path.join(PKG, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
+    objs, cmds = [], []
     for src in HIP_SOURCES + CPP_SOURCES:
         obj = os.path.join(objdir, src + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
-        cmd = [HIPCC, "-c"] + lang + [os.path.join(CSRC, src), "-o", obj] + COMMON + DEVICE_FLAGS
-        out = _run(cmd)
-        if verbose and out.strip():
-            print(out)
+        cmds.append([HIPCC, "-c"] + lang + [os.path.join(CSRC, src), "-o", obj] + COMMON + DEVICE_FLAGS)
         objs.append(obj)
+    # one hipcc per source, at most 8 at a time
+    with ThreadPoolExecutor(max_workers=min(8, len(cmds))) as ex:
+        for out in ex.map(_run, cmds):
+            if verbose and out.strip():
+                print(out)
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-o", tmp] + objs + DEVICE_FLAGS + ["-fPIC"])
     os.replace(tmp, LIB)

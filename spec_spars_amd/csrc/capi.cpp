@@ -160,6 +160,18 @@ int ss_factor_export(const ss_factor *f, int32_t *urowptr, int32_t *ucol, double
     return SS_OK;
 }
 
+int ss_factor_set_threads(int32_t threads) {
+    if (threads < 0 || threads > 256) return fail(SS_ERR_INVALID, "threads=%d outside 0..256", threads);
+    factor_set_threads(threads);
+    return SS_OK;
+}
+
+int ss_factor_tail(const ss_factor *f, int64_t *tail) {
+    if (!f || !tail) return fail(SS_ERR_INVALID, "null argument");
+    *tail = f->F.tail;
+    return SS_OK;
+}
+
 int ss_factor_order(const ss_factor *f, int32_t *order, int32_t *rounds) {
     if (!f) return fail(SS_ERR_INVALID, "null factor");
     if (order) memcpy(order, f->F.order.data(), sizeof(int32_t) * f->F.order.size());
@@ -259,7 +271,8 @@ int ss_graph_bytes(int64_t n, int64_t m, int64_t factor_nnz, int32_t k, int32_t 
 
 int ss_graph_resident_bytes(const ss_graph *g, size_t *bytes) {
     if (!g || !bytes) return fail(SS_ERR_INVALID, "null argument");
-    *bytes = g->arena_struct.bytes + g->arena_factor.bytes + g->arena_work.bytes + g->arena_params.bytes;
+    *bytes = g->arena_struct.bytes + g->arena_factor.bytes + g->arena_work.bytes + g->arena_params.bytes +
+             g->arena_core_work.bytes;
     return SS_OK;
 }
 

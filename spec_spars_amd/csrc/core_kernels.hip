@@ -1,0 +1,283 @@
+// Dense cores of the preconditioner: per graph, the last c rows of its
+// elimination order (a suffix, so the rows outside never depend on them in
+// the forward solve and the core rows never depend on the rows outside in
+// the backward solve) form the unit-lower block L_cc of the factor.  The
+// block is inverted once on the device when the factor is uploaded,
+// M = L_cc^{-1}, and each preconditioner application multiplies by it:
+//   forward   y_c = M   (r_c - L_co y_o)      (the head launch subtracts L_co y_o)
+//   backward  z_c = M^T (D_c^+ y_c)
+// so a core costs one dense product per direction instead of one step per
+// 64 rows.  L = I - N with N >= 0 strictly lower, so M = I + N + N^2 + ...
+// has entries in [0, 1] and the inversion only ever adds terms of one sign.
+//
+// Layout per graph g: the core is padded to cp = a multiple of kCT rows
+// (padding rows: identity in M, zero right-hand side); M is row-major
+// cp x cp at dense_off[g]; the gathered right-hand sides Tc are rows
+// [tc_off[g], tc_off[g] + cp) of a (sum cp) x k block.  Every sum runs in an
+// order fixed by the graph alone, so a graph's bits do not depend on the
+// batch it is in.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace ss {
+
+constexpr int kCT = kCoreTile;   // tile rows and columns
+constexpr int kCK = 32;          // inner (m) depth staged in LDS per pass
+
+// L_cc, strictly lower part, FP64: dense[pos[e]] = val[e].
+__global__ __launch_bounds__(256) void k_core_scatter(int64_t count, const int64_t *__restrict__ pos,
+                                                      const double *__restrict__ val, double *__restrict__ dense) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < count;
+         e += (int64_t)gridDim.x * blockDim.x)
+        dense[pos[e]] = val[e];
+}
+
+// Inverse of every diagonal tile b of one graph's L_cc (unit lower):
+// thread j substitutes column j of the inverse, x_i = -sum_{j<=m<i} L[i][m] x_m.
+__global__ __launch_bounds__(kCT) void k_core_inv_diag(int cp, const double *__restrict__ L, double *__restrict__ M) {
+    __shared__ double ls[kCT][kCT + 1];
+    __shared__ double xs[kCT][kCT + 1];
+    const int b = blockIdx.x, j = threadIdx.x;
+    const size_t o = (size_t)b * kCT;
+    for (int i = 0; i < kCT; i++) ls[i][j] = j < i ? L[(o + i) * cp + o + j] : 0.0;
+    __syncthreads();
+    for (int i = 0; i < kCT; i++) {
+        double x;
+        if (i < j) {
+            x = 0.0;
+        } else if (i == j) {
+            x = 1.0;
+        } else {
+            double s = 0.0;
+            for (int m = j; m < i; m++) s = __builtin_fma(ls[i][m], xs[m][j], s);
+            x = -s;
+        }
+        xs[i][j] = x;
+    }
+    for (int i = 0; i < kCT; i++) M[(o + i) * cp + o + j] = xs[i][j];
+}
+
+// One block row of the inverse: for every tile column jt < it,
+//   M_{it,jt} = -Minv_{it,it} * sum_{jt <= mt < it} L_{it,mt} M_{mt,jt}
+// (block forward substitution of L M = I, column tiles independent).  A
+// workgroup of 256 threads per jt, 4 x 4 outputs per thread.
+__global__ __launch_bounds__(256) void k_core_inv_row(int it, int cp, const double *__restrict__ L,
+                                                      double *__restrict__ M) {
+    __shared__ double as[kCT][kCK + 1];     // L rows x m (or the diagonal inverse)
+    __shared__ double ts[kCT][kCT + 1];     // M m-rows (first kCK rows), then the sum tile
+    const int jt = blockIdx.x, t = threadIdx.x;
+    const int tr = t >> 4, tc = t & 15;
+    double acc[4][4] = {};
+    const size_t r0 = (size_t)it * kCT, c0 = (size_t)jt * kCT;
+    for (size_t m0 = c0; m0 < r0; m0 += kCK) {
+        for (int e = t; e < kCT * kCK; e += 256) {
+            const int r = e / kCK, mm = e % kCK;
+            as[r][mm] = L[(r0 + r) * cp + m0 + mm];
+        }
+        for (int e = t; e < kCK * kCT; e += 256) {
+            const int mm = e / kCT, c = e % kCT;
+            ts[mm][c] = M[(m0 + mm) * cp + c0 + c];
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int mm = 0; mm < kCK; mm++) {
+            double a[4], bb[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) a[u] = as[tr * 4 + u][mm];
+#pragma unroll
+            for (int v = 0; v < 4; v++) bb[v] = ts[mm][tc * 4 + v];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) acc[u][v] = __builtin_fma(a[u], bb[v], acc[u][v]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) ts[tr * 4 + u][tc * 4 + v] = acc[u][v];
+    double out[4][4] = {};
+    for (int m0 = 0; m0 < kCT; m0 += kCK) {
+        __syncthreads();
+        for (int e = t; e < kCT * kCK; e += 256) {
+            const int r = e / kCK, mm = e % kCK;
+            as[r][mm] = M[(r0 + r) * cp + r0 + m0 + mm];
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int mm = 0; mm < kCK; mm++) {
+            double a[4], bb[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) a[u] = as[tr * 4 + u][mm];
+#pragma unroll
+            for (int v = 0; v < 4; v++) bb[v] = ts[m0 + mm][tc * 4 + v];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) out[u][v] = __builtin_fma(a[u], bb[v], out[u][v]);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) M[(r0 + tr * 4 + u) * cp + c0 + tc * 4 + v] = -out[u][v];
+}
+
+__global__ __launch_bounds__(256) void k_to_f32(int64_t count, const double *__restrict__ a, float *__restrict__ b) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < count;
+         e += (int64_t)gridDim.x * blockDim.x)
+        b[e] = (float)a[e];
+}
+
+// Tc[r] = forward: out[vert[r]] (the head's r_c - L_co y_o);
+//         backward: in[vert[r]] / D (D == 0 keeps the value, as the level
+//         solves do); zero on padding rows (vert < 0).
+template <typename T, bool BWD>
+__global__ __launch_bounds__(256) void k_core_gather(int64_t rows, const int32_t *__restrict__ vert,
+                                                     const double *__restrict__ diag, int k,
+                                                     const T *__restrict__ src, T *__restrict__ tc) {
+    const int64_t total = rows * k;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = e / k;
+        const int j = (int)(e - r * k);
+        const int v = vert[r];
+        T x = (T)0;
+        if (v >= 0) {
+            x = src[(size_t)v * k + j];
+            if (BWD) {
+                const double d = diag[v];
+                if (d != 0.0) x = (T)((double)x / d);
+            }
+        }
+        tc[e] = x;
+    }
+}
+
+// One piece of a core product: rows [r0, r0 + kCT) of graph g, inner range
+// [m0, m1) -> partial tile part[piece] (kCT x k).  Forward A = M, backward
+// A = M^T.  256 threads, 4 x 4 outputs each, per 64-column chunk of k.
+template <typename T, bool BWD>
+__global__ __launch_bounds__(256) void k_core_apply(const CorePiece *__restrict__ pieces,
+                                                    const int32_t *__restrict__ cpad,
+                                                    const int64_t *__restrict__ dense_off,
+                                                    const int64_t *__restrict__ tc_off, const T *__restrict__ M,
+                                                    int k, const T *__restrict__ tc, T *__restrict__ part) {
+    __shared__ T as[kCK][kCT + 4];
+    __shared__ T bs[kCK][kCT + 4];
+    const CorePiece pc = pieces[blockIdx.x];
+    const int cp = cpad[pc.graph];
+    const T *Mg = M + dense_off[pc.graph];
+    const T *tg = tc + (size_t)tc_off[pc.graph] * k;
+    const int t = threadIdx.x, tr = t >> 4, tc4 = (t & 15) * 4;
+    T *po = part + (size_t)blockIdx.x * kCT * k;
+    for (int jb = 0; jb < k; jb += kCT) {
+        T acc[4][4] = {};
+        for (int m0 = pc.m0; m0 < pc.m1; m0 += kCK) {
+            if (BWD) {
+                for (int e = t; e < kCK * kCT; e += 256) {
+                    const int mm = e / kCT, r = e % kCT;
+                    as[mm][r] = Mg[(size_t)(m0 + mm) * cp + pc.r0 + r];
+                }
+            } else {
+                for (int e = t; e < kCT * kCK; e += 256) {
+                    const int r = e / kCK, mm = e % kCK;
+                    as[mm][r] = Mg[(size_t)(pc.r0 + r) * cp + m0 + mm];
+                }
+            }
+            for (int e = t; e < kCK * kCT; e += 256) {
+                const int mm = e / kCT, c = e % kCT;
+                bs[mm][c] = jb + c < k ? tg[(size_t)(m0 + mm) * k + jb + c] : (T)0;
+            }
+            __syncthreads();
+#pragma unroll 8
+            for (int mm = 0; mm < kCK; mm++) {
+                T a[4], b[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) a[u] = as[mm][tr * 4 + u];
+#pragma unroll
+                for (int v = 0; v < 4; v++) b[v] = bs[mm][tc4 + v];
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) acc[u][v] = __builtin_fma(a[u], b[v], acc[u][v]);
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                if (jb + tc4 + v < k) po[(size_t)(tr * 4 + u) * k + jb + tc4 + v] = acc[u][v];
+    }
+}
+
+// out[vert[r]] = sum of the row's pieces, in piece order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_core_reduce(int32_t tiles, const int32_t *__restrict__ tile_piece,
+                                                     const int32_t *__restrict__ tile_row,
+                                                     const int32_t *__restrict__ vert, int k,
+                                                     const T *__restrict__ part, T *__restrict__ out) {
+    const int64_t total = (int64_t)tiles * kCT * k;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t tr = e / ((int64_t)kCT * k);
+        const int rem = (int)(e - tr * kCT * k);   // row-in-tile * k + column
+        const int v = vert[tile_row[tr] + rem / k];
+        if (v < 0) continue;
+        const int p0 = tile_piece[tr], p1 = tile_piece[tr + 1];
+        T s = part[(size_t)p0 * kCT * k + rem];
+        for (int p = p0 + 1; p < p1; p++) s = s + part[(size_t)p * kCT * k + rem];
+        out[(size_t)v * k + rem % k] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline unsigned stream_grid(int64_t items) {
+    int64_t g = (items + 255) / 256;
+    if (g < 1) g = 1;
+    if (g > 4096) g = 4096;
+    return (unsigned)g;
+}
+
+void launch_core_scatter(hipStream_t s, int64_t count, const int64_t *pos, const double *val, double *dense) {
+    if (count > 0)
+        hipLaunchKernelGGL(k_core_scatter, dim3(stream_grid(count)), dim3(256), 0, s, count, pos, val, dense);
+}
+
+void launch_core_invert(hipStream_t s, int cp, const double *L, double *M) {
+    const int nb = cp / kCT;
+    if (nb <= 0) return;
+    hipLaunchKernelGGL(k_core_inv_diag, dim3(nb), dim3(kCT), 0, s, cp, L, M);
+    for (int it = 1; it < nb; it++)
+        hipLaunchKernelGGL(k_core_inv_row, dim3(it), dim3(256), 0, s, it, cp, L, M);
+}
+
+void launch_to_f32(hipStream_t s, int64_t count, const double *a, float *b) {
+    if (count > 0) hipLaunchKernelGGL(k_to_f32, dim3(stream_grid(count)), dim3(256), 0, s, count, a, b);
+}
+
+template <typename T, bool BWD>
+void launch_core_apply(hipStream_t s, const CoreApply &c, const T *M, int k, const double *diag, const T *src,
+                       T *tc, T *part, T *out) {
+    if (c.pieces <= 0) return;
+    hipLaunchKernelGGL((k_core_gather<T, BWD>), dim3(stream_grid(c.rows * k)), dim3(256), 0, s, c.rows, c.vert, diag,
+                       k, src, tc);
+    hipLaunchKernelGGL((k_core_apply<T, BWD>), dim3((unsigned)c.pieces), dim3(256), 0, s,
+                       BWD ? c.piece_bwd : c.piece_fwd, c.cpad, c.dense_off, c.tc_off, M, k, tc, part);
+    hipLaunchKernelGGL(k_core_reduce<T>, dim3(stream_grid((int64_t)c.tiles * kCT * k)), dim3(256), 0, s, c.tiles,
+                       BWD ? c.tile_piece_bwd : c.tile_piece_fwd, c.tile_row, c.vert, k, part, out);
+}
+#define SS_INST_CORE_APPLY(T, B)                                                                             \
+    template void launch_core_apply<T, B>(hipStream_t, const CoreApply &, const T *, int, const double *,     \
+                                          const T *, T *, T *, T *);
+SS_INST_CORE_APPLY(double, false)
+SS_INST_CORE_APPLY(double, true)
+SS_INST_CORE_APPLY(float, false)
+SS_INST_CORE_APPLY(float, true)
+
+}  // namespace ss

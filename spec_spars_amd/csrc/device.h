@@ -143,15 +143,19 @@ struct ss_graph {
     std::vector<int32_t> f_level_waves, b_level_waves;   // wavefronts per block of each level launch
     std::vector<std::pair<int32_t, int32_t>> f_runs, b_runs; // runs of small levels (single-workgroup launches)
     int32_t *f_lp_dev = nullptr, *b_lp_dev = nullptr;   // level_ptr on the device
-    // dense cores: per graph, the rows of its trailing small forward levels
-    // (at most kCoreMaxRows), solved by blocked substitution
+    // dense cores: per graph, the last rows of its elimination order (its
+    // factor's dense tail, else its trailing small forward levels), applied
+    // as the inverse of their block of L (core_kernels.hip)
     int32_t f_core_t0 = 0, f_nrows = 0;   // forward positions [f_core_t0, f_nrows): core rows' outside entries
-    int core_max = 0, core_steps = 0;
-    int64_t core_total = 0, core_entries = 0;
-    int32_t *core_n = nullptr, *core_off = nullptr, *core_vert = nullptr;
-    int64_t *dense_off = nullptr;
-    double *Ld = nullptr, *Ltd = nullptr;   // L and L^T per core, FP64
-    float *Lf = nullptr, *Ltf = nullptr;    // FP32 copies
+    int core_max = 0;                     // largest core (rows)
+    int64_t core_total = 0;               // core rows of the batch
+    int64_t core_entries = 0;             // entries of the padded inverses (sum of cp^2)
+    ss::CoreApply core;                   // device arrays of the core products
+    double *Md = nullptr;                 // inverses, FP64 and FP32
+    float *Mf = nullptr;
+    ss::DevArena arena_core_work;         // gathered right-hand sides and product pieces
+    int core_wcap = 0, core_wprec = -1;   // columns / precision they are sized for
+    void *core_tc = nullptr, *core_part = nullptr;
     ss_schedule_info sched_info[2] = {};
     // status
     int *status = nullptr;           // {nactive, err}

@@ -392,6 +392,81 @@ static void factor_concat(const std::vector<const Factor *> &fs, const std::vect
     F.nnz = q;
 }
 
+// Rows of a factor's dense core: its dense tail (rounds order), else the
+// last positions whose forward levels belong to its trailing run of small
+// levels (at most kCoreMaxRows).  Always a suffix of the elimination order.
+static int64_t core_rows_of(const Factor &f) {
+    if (f.tail > 0) return std::min<int64_t>(f.tail, kTailMaxRows);
+    const int32_t lo = forward_tail_start(f, kSmallRows, kCoreMaxRows);
+    int64_t c = 0;
+    while (c < f.n && c < kCoreMaxRows && f.lvl_fwd[(size_t)(f.n - 1 - c)] > lo) c++;
+    return c;
+}
+
+// Invert every graph's core block of L on the device: its strictly lower
+// entries (the entries of the core columns, whose rows are all core rows)
+// are scattered into a zeroed FP64 cp x cp block, inverted tile row by tile
+// row, and copied to FP32.
+static int core_invert(ss_graph *g, const Factor &F, const std::vector<int32_t> &core_c,
+                       const std::vector<int32_t> &cpad, const std::vector<int64_t> &dense_off) {
+    const int64_t total = dense_off[g->G];
+    if (total == 0) return SS_OK;
+    hipStream_t s = g->ctx->stream;
+    std::vector<int64_t> pos;
+    std::vector<double> val;
+    for (int i = 0; i < g->G; i++) {
+        const int64_t c = core_c[i];
+        if (c == 0) continue;
+        const int64_t start = g->row_off[i + 1] - c;   // first core position
+        for (int64_t p = start; p < g->row_off[i + 1]; p++)
+            for (int32_t t = F.urowptr[p]; t < F.urowptr[p + 1]; t++) {
+                const int64_t q = F.ucol[t];
+                if (q == p) continue;
+                if (q < start || q >= g->row_off[i + 1])
+                    return fail(SS_ERR_INVALID, "core: column %lld has row %lld outside its core", (long long)p,
+                                (long long)q);
+                pos.push_back(dense_off[i] + (q - start) * cpad[i] + (p - start));
+                val.push_back(F.uval[t]);
+            }
+    }
+    DevArena tmp;
+    double *L = nullptr, *dval = nullptr;
+    int64_t *dpos = nullptr;
+    SS_TRY(tmp.alloc(&L, (size_t)total));
+    SS_TRY(tmp.alloc(&dpos, pos.size()));
+    SS_TRY(tmp.alloc(&dval, val.size()));
+    SS_HIP(hipMemsetAsync(L, 0, sizeof(double) * (size_t)total, s));
+    SS_HIP(hipMemsetAsync(g->Md, 0, sizeof(double) * (size_t)total, s));
+    SS_TRY(h2d(s, dpos, pos.data(), pos.size()));
+    SS_TRY(h2d(s, dval, val.data(), val.size()));
+    launch_core_scatter(s, (int64_t)pos.size(), dpos, dval, L);
+    for (int i = 0; i < g->G; i++)
+        if (cpad[i] > 0) launch_core_invert(s, cpad[i], L + dense_off[i], g->Md + dense_off[i]);
+    launch_to_f32(s, total, g->Md, g->Mf);
+    SS_HIP(hipGetLastError());
+    SS_HIP(hipStreamSynchronize(s));   // the host vectors and tmp are freed on return
+    return SS_OK;
+}
+
+// Gathered right-hand sides and product pieces of the dense cores for k
+// columns at a precision.
+static int core_ensure_work(ss_graph *g, int k, int prec) {
+    if (g->core.pieces == 0) return SS_OK;
+    if (g->core_wprec == prec && k <= g->core_wcap) return SS_OK;
+    g->arena_core_work.release();
+    g->core_wcap = 0;
+    g->core_wprec = -1;
+    const size_t es = prec == SS_PREC_FP64 ? 8 : 4;
+    unsigned char *tc = nullptr, *part = nullptr;
+    SS_TRY(g->arena_core_work.alloc(&tc, (size_t)g->core.rows * k * es));
+    SS_TRY(g->arena_core_work.alloc(&part, (size_t)g->core.pieces * kCoreTile * k * es));
+    g->core_tc = tc;
+    g->core_part = part;
+    g->core_wcap = k;
+    g->core_wprec = prec;
+    return SS_OK;
+}
+
 int batch_upload_factors(ss_graph *g, const std::vector<const Factor *> &factors) {
     if ((int)factors.size() != g->G) return fail(SS_ERR_INVALID, "need %d factors, got %zu", g->G, factors.size());
     for (int i = 0; i < g->G; i++)
@@ -406,41 +481,41 @@ int batch_upload_factors(ss_graph *g, const std::vector<const Factor *> &factors
         Fp = &Fc;
     }
     const Factor &F = *Fp;
-    // each graph's dense core: the rows of its own trailing small forward levels
+    // each graph's dense core: the last core_rows_of(factor) positions of
+    // its elimination order
     std::vector<char> core((size_t)F.n, 0);
-    std::vector<int32_t> core_n((size_t)g->G, 0), core_off((size_t)g->G + 1, 0), core_vert;
-    std::vector<int64_t> dense_off((size_t)g->G + 1, 0);
-    std::vector<int32_t> ci((size_t)F.n, -1);   // position -> index within its core
+    std::vector<int32_t> core_c((size_t)g->G, 0), cpad((size_t)g->G, 0), core_vert;
+    std::vector<int64_t> dense_off((size_t)g->G + 1, 0), tc_off((size_t)g->G + 1, 0);
     for (int i = 0; i < g->G; i++) {
-        const int32_t lo = forward_tail_start(*factors[i], kSmallRows, kCoreMaxRows);
-        for (int64_t p = g->row_off[i]; p < g->row_off[i + 1]; p++)
-            if (F.lvl_fwd[p] > lo) {
-                core[p] = 1;
-                ci[p] = core_n[i]++;
-                core_vert.push_back(F.order[p]);
-            }
-        core_off[i + 1] = core_off[i] + core_n[i];
-        dense_off[i + 1] = dense_off[i] + (int64_t)core_n[i] * core_n[i];
+        const int64_t ni = g->row_off[i + 1] - g->row_off[i];
+        const int64_t c = core_rows_of(*factors[i]);
+        core_c[i] = (int32_t)c;
+        cpad[i] = (int32_t)((c + kCoreTile - 1) / kCoreTile * kCoreTile);
+        for (int64_t p = g->row_off[i] + ni - c; p < g->row_off[i + 1]; p++) {
+            core[p] = 1;
+            core_vert.push_back(F.order[p]);
+        }
+        for (int64_t r = c; r < cpad[i]; r++) core_vert.push_back(-1);
+        dense_off[i + 1] = dense_off[i] + (int64_t)cpad[i] * cpad[i];
+        tc_off[i + 1] = tc_off[i] + cpad[i];
     }
     Schedule fw, bw;
     SS_TRY(factor_schedules(F, core, fw, bw, kLongRow, kSmallRows));
-    // dense L and L^T of every core (position space: U row p lists q > p with L[q][p])
-    const int64_t dense_total = dense_off[g->G];
-    std::vector<double> Ld((size_t)dense_total, 0.0), Ltd((size_t)dense_total, 0.0);
-    {
-        int gi = 0;
-        for (int64_t p = 0; p < F.n; p++) {
-            while (p >= g->row_off[gi + 1]) gi++;
-            if (!core[p]) continue;
-            const int64_t c = core_n[gi], base = dense_off[gi];
-            for (int32_t t = F.urowptr[p]; t < F.urowptr[p + 1]; t++) {
-                const int32_t q = F.ucol[t];
-                if (q == p || !core[q]) continue;
-                Ld[(size_t)(base + (int64_t)ci[q] * c + ci[p])] = F.uval[t];
-                Ltd[(size_t)(base + (int64_t)ci[p] * c + ci[q])] = F.uval[t];
-            }
+    // product pieces: per row tile of each core, inner ranges of at most
+    // kCoreSeg rows -- forward [0, r0 + kCoreTile), backward [r0, cp)
+    std::vector<CorePiece> pf, pb;
+    std::vector<int32_t> tile_row, tpf(1, 0), tpb(1, 0);
+    for (int i = 0; i < g->G; i++) {
+        for (int32_t r0 = 0; r0 < cpad[i]; r0 += kCoreTile) {
+            for (int32_t m0 = 0; m0 < r0 + kCoreTile; m0 += kCoreSeg)
+                pf.push_back({i, r0, m0, std::min(m0 + kCoreSeg, r0 + kCoreTile)});
+            for (int32_t m0 = r0; m0 < cpad[i]; m0 += kCoreSeg) pb.push_back({i, r0, m0, std::min(m0 + kCoreSeg, cpad[i])});
+            tile_row.push_back((int32_t)(tc_off[i] + r0));
+            tpf.push_back((int32_t)pf.size());
+            tpb.push_back((int32_t)pb.size());
         }
     }
+    if (pf.size() != pb.size()) return fail(SS_ERR_INVALID, "core pieces: %zu forward, %zu backward", pf.size(), pb.size());
     std::vector<double> vdiag((size_t)F.n);
     for (int64_t p = 0; p < F.n; p++) vdiag[F.order[p]] = F.diag[p];
     g->arena_factor.release();
@@ -464,17 +539,23 @@ int batch_upload_factors(ss_graph *g, const std::vector<const Factor *> &factors
     SS_TRY(A.alloc(&g->diag, n));
     SS_TRY(A.alloc(&g->f_lp_dev, fw.level_ptr.size()));
     SS_TRY(A.alloc(&g->b_lp_dev, bw.level_ptr.size()));
-    SS_TRY(A.alloc(&g->core_n, core_n.size()));
-    SS_TRY(A.alloc(&g->core_off, core_off.size()));
-    SS_TRY(A.alloc(&g->core_vert, core_vert.size()));
-    SS_TRY(A.alloc(&g->dense_off, dense_off.size()));
-    SS_TRY(A.alloc(&g->Ld, dense_total));
-    SS_TRY(A.alloc(&g->Ltd, dense_total));
-    SS_TRY(A.alloc(&g->Lf, dense_total));
-    SS_TRY(A.alloc(&g->Ltf, dense_total));
+    const int64_t dense_total = dense_off[g->G];
+    int32_t *d_vert = nullptr, *d_cpad = nullptr, *d_tile_row = nullptr, *d_tpf = nullptr, *d_tpb = nullptr;
+    int64_t *d_dense_off = nullptr, *d_tc_off = nullptr;
+    CorePiece *d_pf = nullptr, *d_pb = nullptr;
+    SS_TRY(A.alloc(&d_vert, core_vert.size()));
+    SS_TRY(A.alloc(&d_cpad, cpad.size()));
+    SS_TRY(A.alloc(&d_dense_off, dense_off.size()));
+    SS_TRY(A.alloc(&d_tc_off, tc_off.size()));
+    SS_TRY(A.alloc(&d_pf, pf.size()));
+    SS_TRY(A.alloc(&d_pb, pb.size()));
+    SS_TRY(A.alloc(&d_tile_row, tile_row.size()));
+    SS_TRY(A.alloc(&d_tpf, tpf.size()));
+    SS_TRY(A.alloc(&d_tpb, tpb.size()));
+    SS_TRY(A.alloc(&g->Md, dense_total));
+    SS_TRY(A.alloc(&g->Mf, dense_total));
     hipStream_t s = g->ctx->stream;
     std::vector<float> fvf(fw.val.begin(), fw.val.end()), bvf(bw.val.begin(), bw.val.end());
-    std::vector<float> Lf(Ld.begin(), Ld.end()), Ltf(Ltd.begin(), Ltd.end());
     std::vector<int32_t> frows(fw.rows), fptr(fw.ptr), brows(bw.rows), bptr(bw.ptr);
     frows.resize((size_t)fn + kTinyGroup, 0);
     fptr.resize((size_t)fn + 1 + kTinyGroup, fw.ptr[fn]);
@@ -493,15 +574,32 @@ int batch_upload_factors(ss_graph *g, const std::vector<const Factor *> &factors
     SS_TRY(h2d(s, g->diag, vdiag.data(), n));
     SS_TRY(h2d(s, g->f_lp_dev, fw.level_ptr.data(), fw.level_ptr.size()));
     SS_TRY(h2d(s, g->b_lp_dev, bw.level_ptr.data(), bw.level_ptr.size()));
-    SS_TRY(h2d(s, g->core_n, core_n.data(), core_n.size()));
-    SS_TRY(h2d(s, g->core_off, core_off.data(), core_off.size()));
-    SS_TRY(h2d(s, g->core_vert, core_vert.data(), core_vert.size()));
-    SS_TRY(h2d(s, g->dense_off, dense_off.data(), dense_off.size()));
-    SS_TRY(h2d(s, g->Ld, Ld.data(), Ld.size()));
-    SS_TRY(h2d(s, g->Ltd, Ltd.data(), Ltd.size()));
-    SS_TRY(h2d(s, g->Lf, Lf.data(), Lf.size()));
-    SS_TRY(h2d(s, g->Ltf, Ltf.data(), Ltf.size()));
+    SS_TRY(h2d(s, d_vert, core_vert.data(), core_vert.size()));
+    SS_TRY(h2d(s, d_cpad, cpad.data(), cpad.size()));
+    SS_TRY(h2d(s, d_dense_off, dense_off.data(), dense_off.size()));
+    SS_TRY(h2d(s, d_tc_off, tc_off.data(), tc_off.size()));
+    SS_TRY(h2d(s, d_pf, pf.data(), pf.size()));
+    SS_TRY(h2d(s, d_pb, pb.data(), pb.size()));
+    SS_TRY(h2d(s, d_tile_row, tile_row.data(), tile_row.size()));
+    SS_TRY(h2d(s, d_tpf, tpf.data(), tpf.size()));
+    SS_TRY(h2d(s, d_tpb, tpb.data(), tpb.size()));
+    SS_TRY(core_invert(g, F, core_c, cpad, dense_off));
     SS_HIP(hipStreamSynchronize(s));
+    g->core.rows = tc_off[g->G];
+    g->core.pieces = (int32_t)pf.size();
+    g->core.tiles = (int32_t)tile_row.size();
+    g->core.vert = d_vert;
+    g->core.cpad = d_cpad;
+    g->core.dense_off = d_dense_off;
+    g->core.tc_off = d_tc_off;
+    g->core.piece_fwd = d_pf;
+    g->core.piece_bwd = d_pb;
+    g->core.tile_row = d_tile_row;
+    g->core.tile_piece_fwd = d_tpf;
+    g->core.tile_piece_bwd = d_tpb;
+    g->core_wcap = 0;
+    g->core_wprec = -1;
+    g->arena_core_work.release();
     g->f_level_ptr = fw.level_ptr;
     g->b_level_ptr = bw.level_ptr;
     g->f_level_long = fw.level_long;
@@ -513,13 +611,13 @@ int batch_upload_factors(ss_graph *g, const std::vector<const Factor *> &factors
     g->f_core_t0 = fw.core_t0;
     g->f_nrows = (int32_t)fn;
     g->core_max = 0;
+    g->core_total = 0;
     int core_graphs = 0;
     for (int i = 0; i < g->G; i++) {
-        g->core_max = std::max(g->core_max, core_n[i]);
-        core_graphs += core_n[i] > 0;
+        g->core_max = std::max(g->core_max, core_c[i]);
+        g->core_total += core_c[i];
+        core_graphs += core_c[i] > 0;
     }
-    g->core_steps = (g->core_max + kCoreBlock - 1) / kCoreBlock;
-    g->core_total = (int64_t)core_vert.size();
     g->core_entries = dense_total;
     // a level's long rows of up to kWgRowMax entries get a block of as many
     // waves as their longest row has parts (4..16; 0 marks a level without
@@ -574,7 +672,7 @@ int batch_upload_factors(ss_graph *g, const std::vector<const Factor *> &factors
         I.core_rows = g->core_total;
         I.core_entries = dense_total;
         I.core_outside_nnz = dir == 0 ? (int64_t)(fw.ptr[fn] - fw.ptr[fw.core_t0]) : 0;
-        I.core_steps = g->core_steps;
+        I.core_steps = g->core.pieces;
         I.core_graphs = core_graphs;
     }
     g->factor_nnz = F.nnz;
@@ -608,6 +706,8 @@ size_t graph_work_bytes(const ss_graph *g, int k, int prec) {
     b += ((size_t)std::max(std::max(g->nrt, g->nst), 1) + (size_t)std::max(g->nrt, 1)) * cap * 8 + gk * 8 +
          (size_t)g->ncomp * cap * 8 * 2;
     b += gk * (5 * 8 + 2 * 4 + 4);
+    // the dense cores' gathered block and product pieces
+    b += ((size_t)g->core.rows + (size_t)g->core.pieces * kCoreTile) * cap * (prec == SS_PREC_FP64 ? 8 : 4);
     return b;
 }
 
@@ -687,9 +787,9 @@ template <> const float *avals<float>(const ss_graph *g) { return g->valf; }
 static Tiles rtiles(const ss_graph *g) { return Tiles{g->rt_start, g->rt_graph, g->nrt, g->tile_waves}; }
 static Tiles stiles(const ss_graph *g) { return Tiles{g->st_start, g->st_graph, g->nst, g->tile_waves}; }
 
-template <typename T> static const T *core_L(const ss_graph *g, bool transposed);
-template <> const double *core_L<double>(const ss_graph *g, bool t) { return t ? g->Ltd : g->Ld; }
-template <> const float *core_L<float>(const ss_graph *g, bool t) { return t ? g->Ltf : g->Lf; }
+template <typename T> static const T *core_M(const ss_graph *g);
+template <> const double *core_M<double>(const ss_graph *g) { return g->Md; }
+template <> const float *core_M<float>(const ss_graph *g) { return g->Mf; }
 
 // One direction of the sparse part of the preconditioner: per-level launches
 // between the runs of small levels, one single-workgroup launch per run.
@@ -729,20 +829,17 @@ static void solve_levels(ss_graph *g, int k, const T *in, T *out) {
 }
 
 // The dense cores of one direction: forward = their rows' entries outside
-// the cores, then the block steps; backward = y / D, then the block steps.
+// the cores (the head), then y_c = M (that); backward = z_c = M^T (D^+ y_c).
 template <typename T, bool BWD>
 static void solve_cores(ss_graph *g, int k, const T *in, T *out) {
     if (g->core_total == 0) return;
     hipStream_t s = g->ctx->stream;
-    ProfScope ps(g, BWD ? SS_KERNEL_CORE_BWD : SS_KERNEL_CORE_FWD, 1 + g->core_steps);
-    if (BWD)
-        launch_core_bwd_init<T>(s, (int32_t)g->core_total, g->core_vert, g->diag, k, in, out);
-    else
+    ProfScope ps(g, BWD ? SS_KERNEL_CORE_BWD : SS_KERNEL_CORE_FWD, BWD ? 3 : 4);
+    if (!BWD)
         launch_core_head<T>(s, g->f_core_t0, g->core_huge, g->f_nrows, g->f_rows, g->f_ptr, g->f_col, fvals<T>(g),
                             g->diag, k, in, out);
-    for (int st = 0; st < g->core_steps; st++)
-        launch_core_step<T, BWD>(s, st, g->G, g->core_max, g->core_n, g->core_off, g->dense_off, g->core_vert,
-                                 core_L<T>(g, BWD), k, out);
+    launch_core_apply<T, BWD>(s, g->core, core_M<T>(g), k, g->diag, BWD ? in : out, static_cast<T *>(g->core_tc),
+                              static_cast<T *>(g->core_part), out);
 }
 
 // Zp = L^{-T} D^+ L^{-1} R (W holds the forward result y).  rz_partial
@@ -823,7 +920,7 @@ static int pcg_iteration(ss_graph *g, int k, T *X, const PcgWork<T> &w, int rule
 // dense-core steps, vector updates and reductions).
 static int64_t launches_per_iteration(const ss_graph *g) {
     const int64_t levels = (int64_t)g->f_level_ptr.size() + (int64_t)g->b_level_ptr.size();
-    return levels + 2 * (int64_t)(g->core_steps + 1) + 16;
+    return levels + 8 + 16;
 }
 constexpr int64_t kGraphMaxNodes = 1024;
 
@@ -949,6 +1046,7 @@ int graph_solve(ss_graph *g, int k, const ss_solve_opts &o, SolveStats &st) {
     if (g->wk != k) return fail(SS_ERR_INVALID, "work not allocated for k=%d", k);
     if (o.max_iter < 0 || !(o.tol >= 0.0)) return fail(SS_ERR_INVALID, "bad solve options");
     SS_HIP(hipSetDevice(g->ctx->device));
+    SS_TRY(core_ensure_work(g, k, g->wprec));
     hipStream_t s = g->ctx->stream;
     const int gk = g->G * k;
     std::vector<double> bn2, rn2;
@@ -1085,6 +1183,7 @@ int graph_autotune(ss_graph *g, int k, int prec, int reps, ss_tuning_state &best
     if (reps < 1 || reps > 100) return fail(SS_ERR_INVALID, "reps=%d outside 1..100", reps);
     SS_TRY(graph_ensure_work(g, k, prec));
     SS_HIP(hipSetDevice(g->ctx->device));
+    SS_TRY(core_ensure_work(g, k, prec));
     best = g->ctx->tune;
     if (prec == SS_PREC_FP64) return autotune_t<double>(g, k, g->w64, reps, best, ms_level, ms_tile);
     return autotune_t<float>(g, k, g->w32, reps, best, ms_level, ms_tile);

@@ -21,6 +21,7 @@
 // function of (Laplacian, seed), independent of scheduling.
 #include "ss_internal.h"
 #include "solve_shape.h"
+#include "host_pool.h"
 
 #include <algorithm>
 #include <cstdlib>
@@ -151,20 +152,42 @@ static int factor_natural(int64_t n, const int32_t *colptr, const int32_t *rowid
 // vertices never depend on each other, so every round is one level of both
 // triangular solves; low degree first keeps fill (and PCG iterations) at or
 // below the natural order's.
+//
+// Dense tail: once at most kTailMaxRows vertices are live and a round takes
+// fewer than 1/kTailFraction of them (the remaining graph is nearly dense:
+// the hubs of a power-law graph, a grid's last separators), the rest are
+// eliminated one at a time in (live degree, random) order.  They form the
+// trailing block of the factor, which the device applies as a dense
+// inverse instead of level by level.
+//
+// Parallel and deterministic: every step of a round either works on
+// disjoint data per vertex (adjacency compaction and keys, eliminations of
+// the independent set) or is a fixed sequential rule (candidate sort, the
+// greedy pick).  Fill edges are bucketed by (producing thread, owner of the
+// target vertex) and appended by the owner; the order inside an adjacency
+// list never matters (eliminate() sorts by (row, value) and merges equal
+// rows), so the factor is the same for every thread count.
 static int factor_rounds(int64_t n, const int32_t *colptr, const int32_t *rowidx, const double *vals,
-                         uint64_t seed, std::vector<int32_t> &order, std::vector<int64_t> &vstart,
+                         uint64_t seed, int threads, std::vector<int32_t> &order, std::vector<int64_t> &vstart,
                          std::vector<int32_t> &crow, std::vector<double> &cval, std::vector<double> &vdiag,
-                         int32_t &rounds) {
+                         int32_t &rounds, int64_t &tail_rows) {
+    for (int64_t t = 0; t < colptr[n]; t++)
+        if (rowidx[t] < 0 || rowidx[t] >= n) return fail(SS_ERR_INVALID, "factor: row index %d out of range", rowidx[t]);
+    HostPool pool(threads);
+    const int T = pool.size();
     std::vector<std::vector<Ent>> adj((size_t)n);
-    for (int64_t c = 0; c < n; c++)
-        for (int32_t t = colptr[c]; t < colptr[c + 1]; t++) {
-            int32_t r = rowidx[t];
-            if (r < 0 || r >= n) return fail(SS_ERR_INVALID, "factor: row index %d out of range", r);
-            if (r != c) adj[c].push_back({r, std::fabs(vals[t])});
+    pool.run([&](int t) {
+        int64_t lo, hi;
+        HostPool::range(n, T, t, lo, hi);
+        for (int64_t c = lo; c < hi; c++) {
+            adj[c].reserve((size_t)(colptr[c + 1] - colptr[c]));
+            for (int32_t q = colptr[c]; q < colptr[c + 1]; q++)
+                if (rowidx[q] != c) adj[c].push_back({rowidx[q], std::fabs(vals[q])});
         }
+    });
     std::vector<char> elim((size_t)n, 0);
     std::vector<uint64_t> key((size_t)n);
-    std::vector<int32_t> live((size_t)n), sel, next, cand;
+    std::vector<int32_t> live((size_t)n), sel, cand;
     std::vector<int32_t> blocked((size_t)n, 0);   // == round + 1: a neighbour was picked
     for (int64_t i = 0; i < n; i++) live[i] = (int32_t)i;
     order.clear();
@@ -174,12 +197,31 @@ static int factor_rounds(int64_t n, const int32_t *colptr, const int32_t *rowidx
     crow.clear();
     cval.clear();
     vdiag.assign((size_t)n, 0.0);
-    std::vector<Ent> work, col;
-    std::vector<double> cum;
     rounds = 0;
+    tail_rows = 0;
+    // per thread: scratch of eliminate(), the columns it produced (in sel
+    // order), fill edges bucketed by owner thread, candidate lists
+    struct Fill3 {
+        int32_t target, other;
+        double val;
+    };
+    struct PerThread {
+        std::vector<Ent> work, col, cols;
+        std::vector<double> cum;
+        std::vector<int64_t> ccount;
+        std::vector<std::vector<Fill3>> fills;
+        std::vector<int32_t> cand, next;
+        uint64_t mindeg = ~0ULL, sumdeg = 0;
+    };
+    std::vector<PerThread> pt((size_t)T);
+    for (auto &x : pt) x.fills.resize((size_t)T);
+    auto owner = [T](int32_t v) { return (int)(((uint32_t)v >> 4) % (uint32_t)T); };
     // SPEC_SPARS_VERBOSE set: a progress line on stderr about every 20 s
-    const bool verbose = std::getenv("SPEC_SPARS_VERBOSE") != nullptr;
+    const char *venv = std::getenv("SPEC_SPARS_VERBOSE");
+    const bool verbose = venv != nullptr, trace = venv && std::atoi(venv) >= 2;
     auto last_report = std::chrono::steady_clock::now();
+    const auto t_start = last_report;
+    int64_t last_sel = (int64_t)n;
     while (!live.empty()) {
         if (verbose && std::chrono::steady_clock::now() - last_report > std::chrono::seconds(20)) {
             last_report = std::chrono::steady_clock::now();
@@ -187,48 +229,166 @@ static int factor_rounds(int64_t n, const int32_t *colptr, const int32_t *rowidx
                          live.size(), (long long)n, crow.size());
             std::fflush(stderr);
         }
+        const int64_t L = (int64_t)live.size();
+        const int active = L >= 4096 ? T : 1;
         const uint64_t rkey = mix64(seed ^ (0x94d049bb133111ebULL * (uint64_t)(rounds + 1)));
-        for (int32_t v : live) {
-            auto &a = adj[v];
-            size_t w = 0;
-            for (const Ent &e : a)
-                if (!elim[e.row]) a[w++] = e;
-            a.resize(w);
-            const uint64_t deg = std::min<uint64_t>(w, 0xffffff);
-            key[v] = (deg << 40) | (mix64(rkey ^ (uint64_t)v) & ((1ULL << 40) - 1));
+        // 1. drop eliminated neighbours, key = (live degree, random)
+        pool.run([&](int t) {
+            int64_t lo, hi;
+            HostPool::range(L, active, t, lo, hi);
+            uint64_t md = ~0ULL, sd = 0;
+            for (int64_t x = lo; x < hi; x++) {
+                const int32_t v = live[x];
+                auto &a = adj[v];
+                size_t w = 0;
+                for (const Ent &e : a)
+                    if (!elim[e.row]) a[w++] = e;
+                a.resize(w);
+                const uint64_t deg = std::min<uint64_t>(w, 0xffffff);
+                key[v] = (deg << 40) | (mix64(rkey ^ (uint64_t)v) & ((1ULL << 40) - 1));
+                md = std::min(md, deg);
+                sd += deg;
+            }
+            pt[t].mindeg = md;
+            pt[t].sumdeg = sd;
+        }, active);
+        uint64_t mindeg = ~0ULL, sumdeg = 0;
+        for (int t = 0; t < active; t++) {
+            mindeg = std::min(mindeg, pt[t].mindeg);
+            sumdeg += pt[t].sumdeg;
         }
-        // greedy maximal independent set in key order, restricted to
-        // vertices of degree <= 2x the round's minimum (min-degree flavour)
-        cand.assign(live.begin(), live.end());
+        if (trace && L <= 2 * kTailMaxRows)
+            std::fprintf(stderr, "[factor] round %d live %lld last_sel %lld mindeg %llu avgdeg %.1f\n", rounds,
+                         (long long)L, (long long)last_sel, (unsigned long long)mindeg, (double)sumdeg / (double)L);
+        // dense enough: the live graph's mean degree is at least 1/kTailDensity of its size
+        if (L <= kTailMaxRows && last_sel * kTailFraction < L && sumdeg * kTailDensity >= (uint64_t)(L * L)) {
+            // dense tail: the rest one at a time in key order
+            const auto t_tail = std::chrono::steady_clock::now();
+            if (verbose) {
+                std::fprintf(stderr, "[factor] rounds done: %.1f s\n",
+                             std::chrono::duration<double>(t_tail - t_start).count());
+                std::fflush(stderr);
+            }
+            cand.assign(live.begin(), live.end());
+            std::sort(cand.begin(), cand.end(),
+                      [&](int32_t x, int32_t y) { return key[x] != key[y] ? key[x] < key[y] : x < y; });
+            PerThread &P = pt[0];
+            for (int32_t v : cand) {
+                order.push_back(v);
+                P.work.clear();
+                for (const Ent &e : adj[v])
+                    if (!elim[e.row]) P.work.push_back(e);
+                ColumnRng rng(seed, v);
+                vdiag[v] = eliminate(P.work, P.cum, rng, P.col, [&](int32_t a, int32_t b, double fv) {
+                    adj[a].push_back({b, fv});
+                    adj[b].push_back({a, fv});
+                });
+                vstart[v] = (int64_t)crow.size();
+                vcount[v] = (int64_t)P.col.size();
+                for (const Ent &e : P.col) { crow.push_back(e.row); cval.push_back(e.val); }
+                elim[v] = 1;
+                std::vector<Ent>().swap(adj[v]);
+                rounds++;
+            }
+            tail_rows = (int64_t)cand.size();
+            live.clear();
+            if (verbose) {
+                std::fprintf(stderr, "[factor] dense tail of %lld vertices from round %d: %.1f s\n",
+                             (long long)tail_rows, rounds - (int32_t)tail_rows,
+                             std::chrono::duration<double>(std::chrono::steady_clock::now() - t_tail).count());
+                std::fflush(stderr);
+            }
+            break;
+        }
+        // 2. candidates of degree <= 2x the minimum, in key order
+        const uint64_t capdeg = std::max<uint64_t>(2 * mindeg, 2);
+        pool.run([&](int t) {
+            int64_t lo, hi;
+            HostPool::range(L, active, t, lo, hi);
+            pt[t].cand.clear();
+            for (int64_t x = lo; x < hi; x++)
+                if ((key[live[x]] >> 40) <= capdeg) pt[t].cand.push_back(live[x]);
+        }, active);
+        cand.clear();
+        for (int t = 0; t < active; t++) cand.insert(cand.end(), pt[t].cand.begin(), pt[t].cand.end());
         std::sort(cand.begin(), cand.end(),
                   [&](int32_t x, int32_t y) { return key[x] != key[y] ? key[x] < key[y] : x < y; });
-        const uint64_t capdeg = std::max<uint64_t>(2 * (key[cand[0]] >> 40), 2);
+        // 3. greedy maximal independent set (sequential, in key order)
         sel.clear();
         for (int32_t v : cand) {
-            if ((key[v] >> 40) > capdeg) break;
             if (blocked[v] == rounds + 1) continue;
             sel.push_back(v);
             for (const Ent &e : adj[v]) blocked[e.row] = rounds + 1;
         }
         std::sort(sel.begin(), sel.end());
-        for (int32_t v : sel) {
-            order.push_back(v);
-            work.assign(adj[v].begin(), adj[v].end());
-            ColumnRng rng(seed, v);
-            vdiag[v] = eliminate(work, cum, rng, col, [&](int32_t a, int32_t b, double fv) {
-                adj[a].push_back({b, fv});
-                adj[b].push_back({a, fv});
-            });
-            vstart[v] = (int64_t)crow.size();
-            vcount[v] = (int64_t)col.size();
-            for (const Ent &e : col) { crow.push_back(e.row); cval.push_back(e.val); }
-            elim[v] = 1;
-            std::vector<Ent>().swap(adj[v]);
+        last_sel = (int64_t)sel.size();
+        // 4. eliminate the set: thread t takes a contiguous range of sel
+        const int64_t S = (int64_t)sel.size();
+        const int sact = S >= 256 ? T : 1;
+        pool.run([&](int t) {
+            PerThread &P = pt[t];
+            int64_t lo, hi;
+            HostPool::range(S, sact, t, lo, hi);
+            P.cols.clear();
+            P.ccount.clear();
+            for (auto &f : P.fills) f.clear();
+            for (int64_t x = lo; x < hi; x++) {
+                const int32_t v = sel[x];
+                P.work.assign(adj[v].begin(), adj[v].end());
+                ColumnRng rng(seed, v);
+                vdiag[v] = eliminate(P.work, P.cum, rng, P.col, [&](int32_t a, int32_t b, double fv) {
+                    P.fills[owner(a)].push_back({a, b, fv});
+                    P.fills[owner(b)].push_back({b, a, fv});
+                });
+                P.ccount.push_back((int64_t)P.col.size());
+                P.cols.insert(P.cols.end(), P.col.begin(), P.col.end());
+                elim[v] = 1;
+                std::vector<Ent>().swap(adj[v]);
+            }
+        }, sact);
+        // columns in sel order
+        {
+            int64_t base = (int64_t)crow.size(), total = 0;
+            for (int t = 0; t < sact; t++) total += (int64_t)pt[t].cols.size();
+            crow.resize((size_t)(base + total));
+            cval.resize((size_t)(base + total));
+            for (int t = 0; t < sact; t++) {
+                int64_t lo, hi;
+                HostPool::range(S, sact, t, lo, hi);
+                int64_t off = 0;
+                for (int64_t x = lo; x < hi; x++) {
+                    const int32_t v = sel[x];
+                    order.push_back(v);
+                    vstart[v] = base + off;
+                    vcount[v] = pt[t].ccount[x - lo];
+                    off += vcount[v];
+                }
+                for (int64_t q = 0; q < (int64_t)pt[t].cols.size(); q++) {
+                    crow[(size_t)(base + q)] = pt[t].cols[q].row;
+                    cval[(size_t)(base + q)] = pt[t].cols[q].val;
+                }
+                base += (int64_t)pt[t].cols.size();
+            }
         }
-        next.clear();
-        for (int32_t v : live)
-            if (!elim[v]) next.push_back(v);
-        live.swap(next);
+        // 5. fill edges, appended by the owner of their target
+        pool.run([&](int o) {
+            for (int t = 0; t < sact; t++)
+                for (const Fill3 &f : pt[t].fills[o]) adj[f.target].push_back({f.other, f.val});
+        });
+        // 6. the live set shrinks (order kept)
+        pool.run([&](int t) {
+            int64_t lo, hi;
+            HostPool::range(L, active, t, lo, hi);
+            pt[t].next.clear();
+            for (int64_t x = lo; x < hi; x++)
+                if (!elim[live[x]]) pt[t].next.push_back(live[x]);
+        }, active);
+        int64_t w = 0;
+        for (int t = 0; t < active; t++) {
+            std::copy(pt[t].next.begin(), pt[t].next.end(), live.begin() + w);
+            w += (int64_t)pt[t].next.size();
+        }
+        live.resize((size_t)w);
         rounds++;
     }
     // vstart/vcount are per vertex; pack them as [start, start+count)
@@ -238,6 +398,11 @@ static int factor_rounds(int64_t n, const int32_t *colptr, const int32_t *rowidx
     return SS_OK;
 }
 
+static int g_factor_threads = 0;   // 0: HostPool::default_threads()
+
+void factor_set_threads(int threads) { g_factor_threads = threads < 0 ? 0 : threads; }
+int factor_threads() { return g_factor_threads > 0 ? g_factor_threads : HostPool::default_threads(); }
+
 int factor_build(int64_t n, const int32_t *colptr, const int32_t *rowidx, const double *vals, uint64_t seed,
                  int ordering, Factor &F) {
     auto t0 = std::chrono::steady_clock::now();
@@ -246,6 +411,7 @@ int factor_build(int64_t n, const int32_t *colptr, const int32_t *rowidx, const 
     F.n = n;
     F.ordering = ordering;
     F.rounds = 0;
+    F.tail = 0;
     std::vector<int32_t> rank((size_t)n);
     // per-position column of L in position ids: start, rows, values
     std::vector<int64_t> pstart((size_t)n + 1, 0);
@@ -255,7 +421,8 @@ int factor_build(int64_t n, const int32_t *colptr, const int32_t *rowidx, const 
         std::vector<int64_t> vsc;
         std::vector<int32_t> crow;
         std::vector<double> cval, vdiag;
-        SS_TRY(factor_rounds(n, colptr, rowidx, vals, seed, F.order, vsc, crow, cval, vdiag, F.rounds));
+        SS_TRY(factor_rounds(n, colptr, rowidx, vals, seed, factor_threads(), F.order, vsc, crow, cval, vdiag,
+                             F.rounds, F.tail));
         for (int64_t p = 0; p < n; p++) rank[F.order[p]] = (int32_t)p;
         F.diag.resize((size_t)n);
         prow.reserve(crow.size());
@@ -437,9 +604,21 @@ int factor_schedules(const Factor &F, const std::vector<char> &core, Schedule &f
             S.level_med[l] = (int32_t)(tiny - mid);
         }
     };
+    // backward levels of the rows outside the cores, whose rows (solved
+    // first, densely) count as level 0
+    std::vector<int32_t> blvl((size_t)n, 0);
+    for (int64_t i = n - 1; i >= 0; i--) {
+        if (core[i]) continue;
+        int32_t l = 1;
+        for (int32_t t = F.urowptr[i]; t < F.urowptr[i + 1]; t++) {
+            const int32_t c = F.ucol[t];
+            if (c != i && !core[c] && blvl[c] + 1 > l) l = blvl[c] + 1;
+        }
+        blvl[i] = l;
+    }
     std::vector<int32_t> fslot, bslot;
     order(F.lvl_fwd, flen, fwd, fslot);
-    order(F.lvl_bwd, blen, bwd, bslot);
+    order(blvl, blen, bwd, bslot);
     // forward: the core rows follow the levels in elimination order (their
     // entries outside the core; the rest is the dense core solve)
     fwd.core_t0 = (int32_t)fslot.size();
@@ -482,7 +661,7 @@ int factor_schedules(const Factor &F, const std::vector<char> &core, Schedule &f
         return SS_OK;
     };
     SS_TRY(pack(fwd, fslot, F.lvl_fwd, lcnt.data(), lcol.data(), lval.data(), false));
-    SS_TRY(pack(bwd, bslot, F.lvl_bwd, F.urowptr.data(), F.ucol.data(), F.uval.data(), true));
+    SS_TRY(pack(bwd, bslot, blvl, F.urowptr.data(), F.ucol.data(), F.uval.data(), true));
 
     // every maximal run of at least two small levels goes to one workgroup
     for (Schedule *S : {&fwd, &bwd}) {

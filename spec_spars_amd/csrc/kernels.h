@@ -69,22 +69,40 @@ template <typename T, bool BWD>
 void launch_tri_small(hipStream_t s, int32_t l0, int32_t l1, const int32_t *level_ptr, const int32_t *rows,
                       const int32_t *ptr, const int32_t *col, const T *val, const double *diag, int k, const T *in,
                       T *out);
-// Dense cores (per graph: core_n[g] rows, vertex ids vert[core_off[g] ..],
-// dense matrix at D + dense_off[g], row-major core_n x core_n).
-// Forward: launch_core_head (the entries outside the core, schedule
-// positions [t0, t1), the first nhuge of more than kWgRowMax entries), then launch_core_step<T, false> for steps 0 .. max
-// blocks - 1 with D = L.  Backward: launch_core_bwd_init, then
-// launch_core_step<T, true> for the same steps with D = L^T.
+// Dense cores (per graph, the last rows of its elimination order; see
+// core_kernels.hip).  Forward: launch_core_head (the core rows' entries
+// outside the core, schedule positions [t0, t1), the first nhuge of more
+// than kWgRowMax entries, writing r_c - L_co y_o to the core rows), then
+// launch_core_apply<T, false>.  Backward: launch_core_apply<T, true> first.
 template <typename T>
 void launch_core_head(hipStream_t s, int32_t t0, int32_t nhuge, int32_t t1, const int32_t *rows, const int32_t *ptr,
                       const int32_t *col, const T *val, const double *diag, int k, const T *in, T *out);
-template <typename T>
-void launch_core_bwd_init(hipStream_t s, int32_t count, const int32_t *vert, const double *diag, int k, const T *in,
-                          T *out);
+// A piece of a core product: rows [r0, r0 + kCoreTile) of graph `graph`'s
+// core times inner rows [m0, m1) (at most kCoreSeg of them).
+struct CorePiece {
+    int32_t graph, r0, m0, m1;
+};
+// Device arrays of the dense cores of a batch.
+struct CoreApply {
+    int64_t rows = 0;                  // sum of padded core sizes (rows of the gathered block)
+    int32_t pieces = 0, tiles = 0;     // pieces per direction; row tiles
+    const int32_t *vert = nullptr;     // [rows]: vertex of each padded core row, -1 on padding
+    const int32_t *cpad = nullptr;     // [G]: padded core size
+    const int64_t *dense_off = nullptr;// [G]: offset of the graph's cp x cp inverse
+    const int64_t *tc_off = nullptr;   // [G]: first gathered row of the graph
+    const CorePiece *piece_fwd = nullptr, *piece_bwd = nullptr;   // [pieces], grouped by tile
+    const int32_t *tile_row = nullptr;          // [tiles]: first gathered row of the tile
+    const int32_t *tile_piece_fwd = nullptr, *tile_piece_bwd = nullptr;   // [tiles + 1]
+};
+void launch_core_scatter(hipStream_t s, int64_t count, const int64_t *pos, const double *val, double *dense);
+// M = L^{-1} of one unit-lower cp x cp block (strictly lower part in L).
+void launch_core_invert(hipStream_t s, int cp, const double *L, double *M);
+void launch_to_f32(hipStream_t s, int64_t count, const double *a, float *b);
+// Forward: out[core rows] = M (out[core rows]); backward: out[core rows] =
+// M^T (D^+ src[core rows]).  tc: rows x k scratch; part: pieces x kCoreTile x k.
 template <typename T, bool BWD>
-void launch_core_step(hipStream_t s, int step, int graphs, int max_rows, const int32_t *core_n,
-                      const int32_t *core_off, const int64_t *dense_off, const int32_t *vert, const T *D, int k,
-                      T *out);
+void launch_core_apply(hipStream_t s, const CoreApply &c, const T *M, int k, const double *diag, const T *src,
+                       T *tc, T *part, T *out);
 template <typename TZ>
 void launch_er_sample(hipStream_t s, const ErSampleArgs<TZ> &a);
 

@@ -509,195 +509,6 @@ __global__ __launch_bounds__(64 * kLevelWaves) void k_trisolve_level(int32_t t0,
     }
 }
 
-// Dense cores, backward step 0: z_i = y_i / D_i for every core row.
-template <typename T>
-__global__ __launch_bounds__(256) void k_core_bwd_init(int32_t count, const int32_t *__restrict__ vert,
-                                                       const double *__restrict__ diag, int k,
-                                                       const T *__restrict__ in, T *__restrict__ out) {
-    const int r = wave_uniform(blockIdx.x * 4 + SS_WAVE_IN_BLOCK);
-    if (r >= count) return;
-    const int i = wave_uniform(vert[r]);
-    const double d = diag[i];
-    for (int jb = 0; jb < k; jb += 64) {
-        const int j = jb + SS_LANE;
-        if (j < k) out[(size_t)i * k + j] = row_start<T, true>(i, j, k, d, in);
-    }
-}
-
-// Substitution inside one dense-core block by a single wave: rows [0, nr)
-// of sa (this lane's column) updated by the strictly triangular block sb
-// (sb[r][a] = entry of row r, column a).  All 64 rows stay in registers;
-// column by column, once row a is final every later row (earlier row
-// backward) takes its term, so each row subtracts its columns in order.
-// Rows past nr start at zero and their entries of sb are zero, so the
-// terms they add to a real row are exact zeros.
-// Waves of a k_core_step workgroup, and the rows of a dense-core block each
-// holds in registers.
-constexpr int kCoreWaves = 16;
-constexpr int kCoreRows = kCoreBlock / kCoreWaves;
-
-// Substitution inside one wave's rows of a dense-core block, in registers:
-// acc[u] is row base + u; sd[r][a] is the block entry of row r, column a.
-// Column by column, once row a is final every later row (earlier row
-// backward) of the wave takes its term.
-template <typename T, bool BWD>
-__device__ __forceinline__ void core_sub_solve(T (&acc)[kCoreRows], const T (*sd)[kCoreBlock], int base) {
-#pragma unroll
-    for (int s = 0; s < kCoreRows; s++) {
-        const int a = BWD ? kCoreRows - 1 - s : s;
-        const T v = acc[a];
-#pragma unroll
-        for (int u = 0; u < kCoreRows; u++)
-            if (BWD ? u < a : u > a) acc[u] = fnms(sd[base + u][base + a], v, acc[u]);
-    }
-}
-
-// One step of the blocked substitution over every graph's dense core
-// (c rows in elimination order, D = L row-major c x c strictly lower; the
-// backward solve reads D = L^T).  Forward step s finishes block s of each
-// graph: workgroup x == 0 applies block s-1 to the rows of block s, then
-// substitutes inside block s; workgroups x >= 1 apply block s-1 to the rows
-// of the later blocks, 64 rows each.  Backward runs the blocks from the
-// last.  Each of the 16 waves keeps 4 rows in registers.  Inside block s
-// the waves finish their rows in turn (a barrier each): the wave whose turn
-// it is substitutes among its own rows, then every wave still waiting takes
-// the finished rows' terms.  Every core row is one running fused
-// multiply-subtract chain over its core columns -- ascending forward,
-// descending backward, explicit zeros included -- so its bits depend on its
-// graph alone.  Rows past the core's end start at zero and have zero
-// entries, so the terms they add to real rows are exact zeros.
-template <typename T, bool BWD>
-__global__ __launch_bounds__(64 * kCoreWaves) void k_core_step(int s, const int32_t *__restrict__ core_n,
-                                                    const int32_t *__restrict__ core_off,
-                                                    const int64_t *__restrict__ dense_off,
-                                                    const int32_t *__restrict__ vert, const T *__restrict__ D, int k,
-                                                    T *out) {
-    constexpr int B = kCoreBlock, RW = kCoreRows;
-    static_assert(RW * kCoreWaves == B, "the waves cover a block");
-    __shared__ T sm_z[B][64];   // finished values (column chunk): the previous block, then this block's rows
-    __shared__ T sm_d[B][B];    // entries of D: this workgroup's rows x the previous block, then the diagonal block
-    const int g = blockIdx.y, x = blockIdx.x;
-    const int lane = threadIdx.x & 63, wv = wave_uniform(threadIdx.x >> 6);
-    const int c = core_n[g];
-    const int nb = (c + B - 1) / B;
-    const int blk = BWD ? nb - 1 - s : s;       // block finished by this step
-    if (blk < 0 || blk >= nb) return;
-    const int prev = BWD ? blk + 1 : blk - 1;   // block applied by this step
-    const bool has_prev = prev >= 0 && prev < nb;
-    const int32_t *vg = vert + core_off[g];
-    const T *Dg = D + dense_off[g];
-    const int b0 = blk * B, b1 = b0 + B < c ? b0 + B : c;
-    const int p0 = prev * B, p1 = p0 + B < c ? p0 + B : c;
-    const int np = has_prev ? p1 - p0 : 0;
-    // rows this workgroup updates with the previous block
-    int r0, r1;
-    if (x == 0) {
-        r0 = b0;
-        r1 = b1;
-    } else if (BWD) {
-        r0 = (x - 1) * B;
-        r1 = r0 + B < b0 ? r0 + B : b0;
-    } else {
-        r0 = b1 + (x - 1) * B;
-        r1 = r0 + B < c ? r0 + B : c;
-    }
-    if (r0 >= r1) return;   // uniform over the workgroup
-    const int nr = r1 - r0;
-    constexpr int NT = 64 * kCoreWaves, DPT = B * B / NT;   // entries of a 64 x 64 block per thread
-    // vertex ids of this wave's rows and of its rows of the previous block,
-    // all loaded together up front: the value loads below then wait on one
-    // batch instead of one id at a time
-    int vr[RW], vp[RW];
-#pragma unroll
-    for (int u = 0; u < RW; u++) {
-        const int r = wv * RW + u, q = wv + kCoreWaves * u;
-        vr[u] = r < nr ? vg[r0 + r] : 0;
-        vp[u] = q < np ? vg[p0 + q] : 0;
-    }
-    for (int jb = 0; jb < k; jb += 64) {
-        const int j = jb + lane;
-        // every load of this workgroup's rows, the previous block's values,
-        // the D panel and (for the workgroup that substitutes) the diagonal
-        // block is in flight before the first shared store
-        T acc[RW];
-#pragma unroll
-        for (int u = 0; u < RW; u++) {
-            const int r = wv * RW + u;
-            acc[u] = (r < nr && j < k) ? out[(size_t)vr[u] * k + j] : (T)0;
-        }
-        T dd[DPT];
-        if (x == 0) {
-#pragma unroll
-            for (int u = 0; u < DPT; u++) {
-                const int e = (int)threadIdx.x + NT * u, a = e / B, b = e % B;
-                dd[u] = (a < nr && b < nr) ? Dg[(size_t)(b0 + a) * c + b0 + b] : (T)0;
-            }
-        }
-        {
-            T zv[RW], dv[DPT];
-#pragma unroll
-            for (int u = 0; u < RW; u++) {
-                const int r = wv + kCoreWaves * u;
-                zv[u] = (r < np && j < k) ? out[(size_t)vp[u] * k + j] : (T)0;
-            }
-#pragma unroll
-            for (int u = 0; u < DPT; u++) {
-                const int e = (int)threadIdx.x + NT * u, a = e / B, b = e % B;
-                dv[u] = (a < nr && b < np) ? Dg[(size_t)(r0 + a) * c + p0 + b] : (T)0;
-            }
-#pragma unroll
-            for (int u = 0; u < RW; u++) sm_z[wv + kCoreWaves * u][lane] = zv[u];
-#pragma unroll
-            for (int u = 0; u < DPT; u++) {
-                const int e = (int)threadIdx.x + NT * u;
-                sm_d[e / B][e % B] = dv[u];
-            }
-        }
-        __syncthreads();
-        // the previous block: ascending columns forward, descending backward
-#pragma unroll 4
-        for (int qq = 0; qq < np; qq++) {
-            const int q = BWD ? np - 1 - qq : qq;
-            const T v = sm_z[q][lane];
-#pragma unroll
-            for (int u = 0; u < RW; u++) acc[u] = fnms(sm_d[wv * RW + u][q], v, acc[u]);
-        }
-        if (x == 0) {
-            __syncthreads();   // the panel's reads of sm_z and sm_d are done
-#pragma unroll
-            for (int u = 0; u < DPT; u++) {
-                const int e = (int)threadIdx.x + NT * u;
-                sm_d[e / B][e % B] = dd[u];
-            }
-            __syncthreads();
-            for (int st = 0; st < kCoreWaves; st++) {
-                const int w = BWD ? kCoreWaves - 1 - st : st;   // the wave whose rows finish now
-                if (wv == w) {
-                    core_sub_solve<T, BWD>(acc, sm_d, w * RW);
-#pragma unroll
-                    for (int u = 0; u < RW; u++) sm_z[w * RW + u][lane] = acc[u];
-                }
-                __syncthreads();
-                if (BWD ? wv < w : wv > w) {
-#pragma unroll
-                    for (int aa = 0; aa < RW; aa++) {
-                        const int a = w * RW + (BWD ? RW - 1 - aa : aa);
-                        const T v = sm_z[a][lane];
-#pragma unroll
-                        for (int u = 0; u < RW; u++) acc[u] = fnms(sm_d[wv * RW + u][a], v, acc[u]);
-                    }
-                }
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < RW; u++) {
-            const int r = wv * RW + u;
-            if (r < nr && j < k) out[(size_t)vr[u] * k + j] = acc[u];
-        }
-        __syncthreads();   // the next column chunk reuses the shared arrays
-    }
-}
-
 // A run of small levels [l0, l1) walked by one workgroup of 16 waves with a
 // workgroup barrier between levels (all waves share one CU, so the barrier's
 // workgroup-scope fence orders the row stores before the next level's
@@ -937,34 +748,6 @@ template void launch_core_head<double>(hipStream_t, int32_t, int32_t, int32_t, c
                                        double *);
 template void launch_core_head<float>(hipStream_t, int32_t, int32_t, int32_t, const int32_t *, const int32_t *,
                                       const int32_t *, const float *, const double *, int, const float *, float *);
-
-template <typename T>
-void launch_core_bwd_init(hipStream_t s, int32_t count, const int32_t *vert, const double *diag, int k, const T *in,
-                          T *out) {
-    if (count <= 0) return;
-    hipLaunchKernelGGL(k_core_bwd_init<T>, dim3(grid_for(count, 4)), dim3(256), 0, s, count, vert, diag, k, in, out);
-}
-template void launch_core_bwd_init<double>(hipStream_t, int32_t, const int32_t *, const double *, int,
-                                           const double *, double *);
-template void launch_core_bwd_init<float>(hipStream_t, int32_t, const int32_t *, const double *, int, const float *,
-                                          float *);
-
-template <typename T, bool BWD>
-void launch_core_step(hipStream_t s, int step, int graphs, int max_rows, const int32_t *core_n,
-                      const int32_t *core_off, const int64_t *dense_off, const int32_t *vert, const T *D, int k,
-                      T *out) {
-    if (graphs <= 0 || max_rows <= 0) return;
-    const unsigned gx = 1 + (unsigned)((max_rows + kCoreBlock - 1) / kCoreBlock);
-    hipLaunchKernelGGL((k_core_step<T, BWD>), dim3(gx, graphs), dim3(64 * kCoreWaves), 0, s, step, core_n, core_off, dense_off,
-                       vert, D, k, out);
-}
-#define SS_INST_CORE(T, B)                                                                                    \
-    template void launch_core_step<T, B>(hipStream_t, int, int, int, const int32_t *, const int32_t *,         \
-                                         const int64_t *, const int32_t *, const T *, int, T *);
-SS_INST_CORE(double, false)
-SS_INST_CORE(double, true)
-SS_INST_CORE(float, false)
-SS_INST_CORE(float, true)
 
 template <typename T, bool BWD>
 void launch_tri_small(hipStream_t s, int32_t l0, int32_t l1, const int32_t *level_ptr, const int32_t *rows,

@@ -48,8 +48,10 @@ constexpr int kSmallRows = 16;    // levels with at most this many rows ...
 constexpr int kSmallNnz = 2048;   // ... and entries go to the single-workgroup kernel
 static_assert(kSmallRows == 16, "the small-level kernel runs 16 waves, at most one row each");
 
-constexpr int kCoreBlock = 64;      // rows per block of the dense-core substitution
-constexpr int kCoreMaxRows = 16384; // largest dense core per graph (its trailing small levels beyond stay sparse)
+constexpr int kCoreTile = 64;       // rows / columns of a dense-core tile
+constexpr int kCoreSeg = 2048;      // inner rows of one dense-core product piece, at most
+constexpr int kCoreMaxRows = 4096;  // largest dense core per graph taken from trailing small levels
+                                    // (a factor's dense tail can be larger)
 constexpr int kLevelWaves = 16;     // wavefronts per block of a per-level launch (at most)
 constexpr int kXcds = 8;            // MI355X: workgroups are dealt to the 8 XCDs round robin
 

@@ -50,7 +50,8 @@ struct Factor {
     int64_t n = 0;
     int64_t nnz = 0;                  // incl. unit diagonal
     int32_t ordering = ORDER_NATURAL;
-    int32_t rounds = 0;               // elimination rounds (ORDER_ROUNDS)
+    int32_t rounds = 0;               // elimination rounds (ORDER_ROUNDS; a dense-tail vertex is a round)
+    int64_t tail = 0;                 // ORDER_ROUNDS: positions [n - tail, n) are the dense tail
     std::vector<int32_t> order;       // position -> vertex
     std::vector<int32_t> urowptr;     // reference precond_M layout
     std::vector<int32_t> ucol;
@@ -82,8 +83,18 @@ struct Schedule {
     std::vector<double> val;
 };
 
+// Rounds order: the dense tail starts once at most kTailMaxRows vertices
+// are live, a round took fewer than 1/kTailFraction of them and their mean
+// live degree is at least 1/kTailDensity of their count.
+constexpr int64_t kTailMaxRows = 32768;
+constexpr int64_t kTailFraction = 64;
+constexpr uint64_t kTailDensity = 16;
+
 int factor_build(int64_t n, const int32_t *colptr, const int32_t *rowidx, const double *vals,
                  uint64_t seed, int ordering, Factor &F);
+// Host threads of the factorization (0 = default); results do not depend on it.
+void factor_set_threads(int threads);
+int factor_threads();
 void factor_levels(Factor &F);
 // First 0-based forward level of a graph's dense core: the trailing run of
 // levels with at most small_rows rows each, cut to at most max_rows rows.

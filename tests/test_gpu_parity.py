@@ -282,11 +282,10 @@ def test_rounds_factor_solve_matches_permuted_oracle(ctx, oracle_mod):
     factor's position space (L and Y permuted by the factor's order), so both
     apply the same M^-1, and the iteration counts and solutions agree.  The
     graph (one large component and isolated vertices, the right-hand side
-    projected per component) has a backward solve that starts with a run of
-    more than 256 small levels (several LDS stages of the small-level
-    kernel), and hub rows of more than 1,024 entries.  A mixed-precision
-    solve of the same system gives leverage scores within 1e-4 of the
-    oracle's."""
+    projected per component) has a dense tail of thousands of hub vertices,
+    applied as the inverse of its block of L, and hub rows of more than
+    1,024 entries.  A mixed-precision solve of the same system gives
+    leverage scores within 1e-4 of the oracle's."""
     import scipy.sparse as sp
     import scipy.sparse.csgraph as cg
     import spec_spars_amd as ss
@@ -298,7 +297,7 @@ def test_rounds_factor_solve_matches_permuted_oracle(ctx, oracle_mod):
     F = ss.Factor(cp, ri, va, seed=0, ordering=ss.ORDER_ROUNDS)
     G.set_factor(F)
     bi = G.schedule_info(1)
-    assert bi["small_runs"] >= 1 and bi["small_levels"] > 256, bi
+    assert F.tail() > 1000 and bi["core_rows"] == F.tail(), (F.tail(), bi)
     nc, lab = cg.connected_components(sp.coo_matrix((np.ones(len(u)), (u, v)), shape=(n, n)), directed=False)
     cnt = np.bincount(lab, minlength=nc).astype(np.float64)
 
@@ -333,6 +332,32 @@ def test_rounds_factor_solve_matches_permuted_oracle(ctx, oracle_mod):
     both = (pm < 1) & (po < 1)
     assert both.sum() > 100000
     assert np.all(np.abs(pm[both] - po[both]) <= 1e-4 * po[both])
+
+
+def test_small_level_runs_match_oracle(ctx, oracle_mod):
+    """Natural order on a 120 x 120 grid: the factor's solve DAG is nearly a
+    chain, so outside the dense core (at most the last 4,096 positions) its levels
+    are runs of small levels walked by one workgroup (more than 256 levels:
+    several LDS stages), in both directions.  FP64 solution within 1e-7 of
+    the oracle's with the same factor."""
+    import spec_spars_amd as ss
+    from spec_spars_amd import graphs
+    n, u, v = graphs.grid2d(120, 120)
+    w = graphs.loguniform_weights(len(u), seed=5, lo=0.1, hi=10.0)
+    G = ss.Graph(ctx, n, u, v, w)
+    cp, ri, va = G.laplacian()
+    F = ss.Factor(cp, ri, va, seed=3, ordering=ss.ORDER_NATURAL)
+    G.set_factor(F)
+    for d in (0, 1):
+        info = G.schedule_info(d)
+        assert info["small_runs"] >= 1 and info["small_levels"] > 256 and 3000 < info["core_rows"] <= 4096, info
+    Y = oracle_mod.sketch(u, v, w, n, 6, seed=2, scale=1.0)
+    Z, rep = G.solve(Y, ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-11, max_iter=3000))
+    G.close()
+    Zo, it, rr = oracle_mod.solve_lap(Y, cp, ri, va, *F.export(), stop_rule=1, tol=1e-11, max_iter=3000)
+    assert np.abs(rep["iters"] - it).max() <= 1
+    a, b = Z - Z.mean(0), Zo - Zo.mean(0)
+    assert (np.linalg.norm(a - b, axis=0) / np.linalg.norm(b, axis=0)).max() < 1e-7
 
 
 @pytest.mark.parametrize("k", [130, 256])
