@@ -221,6 +221,21 @@ int ss_graph_sparsify(ss_graph *g, const ss_sparsify_opts *o,
                       ss_sparsify_report *rep);
 int ss_graph_read_kept(const ss_graph *g, int64_t cap, int32_t *ku,
                        int32_t *kv, double *kw, int64_t *kept);
+/* Several sparsify requests on one graph in one pass (no reference
+ * counterpart: a served form merging requests that arrive together).  The
+ * requests' sketch columns sit side by side (request r's column j hashes
+ * index j with r's sketch seed and scale), one multi-RHS solve takes every
+ * column (each keeps its own PCG scalars and stop test), then one sampling
+ * pass per request reads its own columns with its own beta and sampling
+ * seed.  Each request's kept edges are bit-identical to ss_graph_sparsify
+ * with its options alone.  uniform and the solve options must agree; the
+ * requests' k add up to at most 256.  rep[r] gets request r's kept count,
+ * iterations and residual; the phase times are the pass's. */
+int ss_graph_sparsify_many(ss_graph *g, int32_t count, const ss_sparsify_opts *opts,
+                           ss_sparsify_report *rep);
+/* Request r's kept edges of the last ss_graph_sparsify_many call. */
+int ss_graph_read_kept_many(const ss_graph *g, int32_t r, int64_t cap, int32_t *ku,
+                            int32_t *kv, double *kw, int64_t *kept);
 /* Sketch Y / solution Z of the last run as FP64 n x k (host).  After an
  * FP64 solve, which = 2 / 3 give the last preconditioner application's
  * forward result L^{-1} r and its output L^{-T} D^+ L^{-1} r. */

@@ -34,6 +34,10 @@ def load():
                                               ctypes.c_int, ctypes.c_int]),
         "or_sketch": (None, [ctypes.c_int64, _i32p, _i32p, _f64p, ctypes.c_int64, ctypes.c_int32, ctypes.c_uint64,
                              ctypes.c_double, ctypes.c_int, ctypes.c_int, _f64p]),
+        "or_sketch_edges": (None, [ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), _i32p, _i32p, _f64p,
+                                   ctypes.c_int64, ctypes.c_int32, ctypes.c_uint64, ctypes.c_double, ctypes.c_int,
+                                   ctypes.c_int, _f64p]),
+        "or_permute_csc": (None, [ctypes.c_int64, _i32p, _i32p, _f64p, _i32p, _i32p, _i32p, _f64p]),
         "or_laplacian": (ctypes.c_int64, [ctypes.c_int64, ctypes.c_int64, _i32p, _i32p, _f64p, _i32p, _i32p, _f64p]),
         "or_upper_edges": (ctypes.c_int64, [ctypes.c_int64, _i32p, _i32p, _f64p, _i32p, _i32p, _f64p]),
         "or_levels_fwd": (None, [ctypes.c_int64, _i32p, _i32p, _i32p]),
@@ -86,6 +90,36 @@ def sketch(v1, v2, w, n, k, seed=0, scale=1.0, uniform=False, variant=0):
     load().or_sketch(len(w), _p(v1, ctypes.c_int32), _p(v2, ctypes.c_int32), _p(w, ctypes.c_double), n, k, seed,
                      scale, int(uniform), variant, _p(Y, ctypes.c_double))
     return Y
+
+
+def sketch_edges(eid, v1, v2, w, n, k, seed=0, scale=1.0, uniform=False, variant=0, Y=None):
+    """Sketch of the edges with ids eid (ascending; v1, v2, w are theirs),
+    added into Y (n x k, zeros if None).  Rows whose incident edges are all
+    in the subset get their full sketch rows."""
+    eid = _c(eid, np.int64)
+    v1, v2, w = _c(v1, np.int32), _c(v2, np.int32), _c(w, np.float64)
+    if Y is None:
+        Y = np.zeros((n, k), np.float64)
+    load().or_sketch_edges(len(eid), eid.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), _p(v1, ctypes.c_int32),
+                           _p(v2, ctypes.c_int32), _p(w, ctypes.c_double), n, k, seed, scale, int(uniform),
+                           variant, _p(Y, ctypes.c_double))
+    return Y
+
+
+def permute_csc(cp, ri, va, order):
+    """The CSC matrix in the position space of an elimination order
+    (position p holds vertex order[p]): P A P^T, rows sorted."""
+    n = len(cp) - 1
+    cp, ri, va = _c(cp, np.int32), _c(ri, np.int32), _c(va, np.float64)
+    rank = np.empty(n, np.int32)
+    rank[np.asarray(order)] = np.arange(n, dtype=np.int32)
+    pc = np.zeros(n + 1, np.int32)
+    pr = np.zeros(len(ri), np.int32)
+    pv = np.zeros(len(va), np.float64)
+    load().or_permute_csc(n, _p(cp, ctypes.c_int32), _p(ri, ctypes.c_int32), _p(va, ctypes.c_double),
+                          _p(rank, ctypes.c_int32), _p(pc, ctypes.c_int32), _p(pr, ctypes.c_int32),
+                          _p(pv, ctypes.c_double))
+    return pc, pr, pv
 
 
 def laplacian(n, v1, v2, w):

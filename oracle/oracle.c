@@ -111,6 +111,69 @@ void or_sketch(int64_t m, const int32_t *v1, const int32_t *v2, const double *w,
     free(q);
 }
 
+/* The same sum over a subset of the edges: edge eid[t] = (v1[t], v2[t],
+ * w[t]) for t in [0, count), ids ascending, added into Y (not cleared).  A
+ * row that every incident edge of is in the subset gets exactly its
+ * or_sketch value; the reference splits the edges into blocks of n the same
+ * way, one thread each (src/sparsifier.rs:509-529).                      */
+void or_sketch_edges(int64_t count, const int64_t *eid, const int32_t *v1, const int32_t *v2,
+                     const double *w, int64_t n, int32_t k, uint64_t seed, double scale,
+                     int uniform, int variant, double *Y) {
+    (void)n;
+    double *q = (double *)malloc(sizeof(double) * (size_t)k);
+    for (int64_t t = 0; t < count; t++) {
+        const uint64_t e = (uint64_t)eid[t];
+        for (int32_t j = 0; j < k; j++)
+            q[j] = or_sketch_value(e, (uint64_t)j, seed, scale, uniform, variant);
+        double nv = sqrt(-1.0 * (-w[t])) * -1.0;   /* value at v1 */
+        int32_t rows[2];
+        double vals[2];
+        if (v1[t] < v2[t]) {
+            rows[0] = v1[t]; vals[0] = nv;
+            rows[1] = v2[t]; vals[1] = -1.0 * nv;
+        } else {
+            rows[0] = v2[t]; vals[0] = -1.0 * nv;
+            rows[1] = v1[t]; vals[1] = nv;
+        }
+        for (int r = 0; r < 2; r++) {
+            double *yr = Y + (size_t)rows[r] * (size_t)k;
+            for (int32_t j = 0; j < k; j++) yr[j] += q[j] * vals[r];
+        }
+    }
+    free(q);
+}
+
+/* P A P^T of a CSC matrix for the permutation rank (new index of old index
+ * i is rank[i]), rows sorted within each column: the Laplacian in a
+ * factor's position space.  Output arrays sized n + 1 and nnz.            */
+void or_permute_csc(int64_t n, const int32_t *colptr, const int32_t *rowidx, const double *vals,
+                    const int32_t *rank, int32_t *pcolptr, int32_t *prowidx, double *pvals) {
+    pcolptr[0] = 0;
+    for (int64_t c = 0; c < n; c++) pcolptr[rank[c] + 1] = colptr[c + 1] - colptr[c];
+    for (int64_t i = 0; i < n; i++) pcolptr[i + 1] += pcolptr[i];
+    for (int64_t c = 0; c < n; c++) {
+        const int32_t o = pcolptr[rank[c]];
+        const int32_t len = colptr[c + 1] - colptr[c];
+        for (int32_t t = 0; t < len; t++) {
+            prowidx[o + t] = rank[rowidx[colptr[c] + t]];
+            pvals[o + t] = vals[colptr[c] + t];
+        }
+        /* insertion sort: a Laplacian column is short except at hubs */
+        for (int32_t a = 1; a < len; a++) {
+            const int32_t r = prowidx[o + a];
+            const double x = pvals[o + a];
+            int32_t b = a - 1;
+            while (b >= 0 && prowidx[o + b] > r) {
+                prowidx[o + b + 1] = prowidx[o + b];
+                pvals[o + b + 1] = pvals[o + b];
+                b--;
+            }
+            prowidx[o + b + 1] = r;
+            pvals[o + b + 1] = x;
+        }
+    }
+}
+
 /* ======================================================================
  * Laplacian (src/sparsifier.rs:48-87, :693-709): each edge contributes -w
  * at (v1,v2) and (v2,v1); diag[v] accumulates w in insertion order; the

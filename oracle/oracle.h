@@ -42,6 +42,12 @@ void or_sketch(int64_t m, const int32_t *v1, const int32_t *v2, const double *w,
                int64_t n, int32_t k, uint64_t seed, double scale, int uniform,
                int variant, double *Y);
 
+void or_sketch_edges(int64_t count, const int64_t *eid, const int32_t *v1, const int32_t *v2,
+                     const double *w, int64_t n, int32_t k, uint64_t seed, double scale,
+                     int uniform, int variant, double *Y);
+void or_permute_csc(int64_t n, const int32_t *colptr, const int32_t *rowidx, const double *vals,
+                    const int32_t *rank, int32_t *pcolptr, int32_t *prowidx, double *pvals);
+
 /* ---- Laplacian assembly (CSC with diagonal; symmetric so CSC == CSR) --- */
 /* Returns nnz written (<= n + 2m) or -1 on a bad vertex. */
 int64_t or_laplacian(int64_t n, int64_t m, const int32_t *v1, const int32_t *v2,

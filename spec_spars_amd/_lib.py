@@ -137,6 +137,10 @@ SIGNATURES = {
     "ss_graph_sparsify": (ctypes.c_int, [c_vp, ctypes.POINTER(SparsifyOpts),
                                          ctypes.POINTER(SparsifyReport)]),
     "ss_graph_read_kept": (ctypes.c_int, [c_vp, ctypes.c_int64, c_i32p, c_i32p, c_f64p, c_i64p]),
+    "ss_graph_sparsify_many": (ctypes.c_int, [c_vp, ctypes.c_int32, ctypes.POINTER(SparsifyOpts),
+                                              ctypes.POINTER(SparsifyReport)]),
+    "ss_graph_read_kept_many": (ctypes.c_int, [c_vp, ctypes.c_int32, ctypes.c_int64, c_i32p, c_i32p, c_f64p,
+                                               c_i64p]),
     "ss_graph_read_block": (ctypes.c_int, [c_vp, ctypes.c_int, c_f64p]),
     "ss_graph_read_probs": (ctypes.c_int, [c_vp, c_f64p]),
     "ss_graph_profile": (ctypes.c_int, [c_vp, ctypes.c_int]),

@@ -295,6 +295,39 @@ class Graph:
         self.k = k
         return {f: getattr(rep, f) for f, _ in _lib.SparsifyReport._fields_}
 
+    def sparsify_many(self, requests, solve=None, uniform=False):
+        """Several sparsify requests in one pass: requests is a list of dicts
+        with k, beta and optionally scale (1/sqrt(k)), sketch_seed,
+        sampling_seed.  Returns one report per request; read_kept_many(r)
+        gives request r's kept edges."""
+        R = len(requests)
+        arr = (_lib.SparsifyOpts * R)()
+        sv = solve if solve is not None else solve_opts(stop_rule=_lib.STOP_REL, tol=1e-8)
+        for i, q in enumerate(requests):
+            o = arr[i]
+            o.k = int(q["k"])
+            o.uniform = int(bool(uniform))
+            o.scale = float(q.get("scale", 1.0 / np.sqrt(q["k"])))
+            o.sketch_seed = int(q.get("sketch_seed", 0))
+            o.sampling_seed = int(q.get("sampling_seed", 0))
+            o.beta = float(q["beta"])
+            o.solve = sv
+        reps = (_lib.SparsifyReport * R)()
+        check(load().ss_graph_sparsify_many(self._h, R, arr, reps))
+        self.k = sum(int(q["k"]) for q in requests)
+        return [{f: getattr(r, f) for f, _ in _lib.SparsifyReport._fields_} for r in reps]
+
+    def read_kept_many(self, r):
+        kept = ctypes.c_int64()
+        cap = self.m
+        ku = np.zeros(max(cap, 1), np.int32)
+        kv = np.zeros(max(cap, 1), np.int32)
+        kw = np.zeros(max(cap, 1), np.float64)
+        check(load().ss_graph_read_kept_many(self._h, r, cap, _p(ku, ctypes.c_int32), _p(kv, ctypes.c_int32),
+                                             _p(kw, ctypes.c_double), ctypes.byref(kept)))
+        c = kept.value
+        return ku[:c].copy(), kv[:c].copy(), kw[:c].copy()
+
     def read_kept(self):
         kept = ctypes.c_int64()
         cap = self.m

@@ -41,8 +41,9 @@ struct BatchSampleArgs {
     const int32_t *edge_off;   // per graph: first global edge
     const int32_t *eu, *ev;
     const double *ew;
-    const double *Z;
-    int k;
+    const double *Z;           // the first column sampled
+    int k;                     // columns sampled
+    int ldz;                   // row stride of Z
     const double *beta;        // per graph
     const uint32_t *keys;      // per graph: 8 ChaCha key words
     const double *p_in;
@@ -58,8 +59,15 @@ struct BatchSampleArgs {
     int spin_limit;
 };
 
+// Per (graph, column) sketch parameters, indexed g * k + j: the sign
+// hash's seed and column index, and the scale.
+struct SketchCols {
+    const uint64_t *seed;
+    const double *scale;
+    const int32_t *j;
+};
 void bt_sketch(hipStream_t s, Tiles rt, const int32_t *iptr, const int32_t *ieid, const double *ib, int k,
-               const uint64_t *seeds, const double *scales, int uniform, double *Y);
+               SketchCols cols, int uniform, double *Y);
 template <typename TA, typename TB>
 void bt_tile_partial(hipStream_t s, Tiles rt, int k, const TA *A, const TB *B, const double *diag, int op,
                      double *partial);

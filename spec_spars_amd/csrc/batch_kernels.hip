@@ -86,8 +86,7 @@ __device__ __forceinline__ void bt_tile_store32(double acc, int k, int jb, doubl
 constexpr int kSketchSplit = 128;   // blocks per row tile in the sketch
 __global__ __launch_bounds__(1024) void kb_sketch(Tiles rt, const int32_t *__restrict__ iptr,
                                                  const int32_t *__restrict__ ieid, const double *__restrict__ ib,
-                                                 int k, const uint64_t *__restrict__ seeds,
-                                                 const double *__restrict__ scales, int uniform, int variant,
+                                                 int k, SketchCols cols, int uniform, int variant,
                                                  double *__restrict__ Y) {
 #pragma clang fp contract(off)
     // kSketchSplit blocks per row tile, one row per wave: a row's cost is
@@ -95,8 +94,6 @@ __global__ __launch_bounds__(1024) void kb_sketch(Tiles rt, const int32_t *__res
     // together, so rows spread over as many waves as there are rows
     const int t = blockIdx.x / kSketchSplit, sb = blockIdx.x % kSketchSplit;
     const int g = rt.graph[t];
-    const uint64_t seed = seeds[g];
-    const double scale = scales[g];
     const int per = (rt.start[t + 1] - rt.start[t] + kSketchSplit - 1) / kSketchSplit;
     const int r0 = rt.start[t] + sb * per;
     const int r1 = r0 + per < rt.start[t + 1] ? r0 + per : rt.start[t + 1];
@@ -104,10 +101,14 @@ __global__ __launch_bounds__(1024) void kb_sketch(Tiles rt, const int32_t *__res
         const int r = bt_uniform(row);
         const int q0 = iptr[r], q1 = iptr[r + 1];
         for (int jb = 0; jb < k; jb += 64) {
-            const int j = jb + BT_LANE;
+            const int j = jb + BT_LANE, jc = g * k + (j < k ? j : k - 1);
+            // column j's seed, scale and the column index its signs hash
+            const uint64_t seed = cols.seed[jc];
+            const double scale = cols.scale[jc];
+            const uint64_t jh = (uint64_t)cols.j[jc];
             double acc = 0.0;
             for (int q = q0; q < q1; q++) {
-                const double sv = sign_value((uint64_t)ieid[q], (uint64_t)j, seed, uniform, variant) * scale;
+                const double sv = sign_value((uint64_t)ieid[q], jh, seed, uniform, variant) * scale;
                 const double term = sv * ib[q];
                 acc = acc + term;
             }
@@ -578,36 +579,61 @@ __global__ __launch_bounds__(1024) void kb_accumulate(Tiles rt, int k, const flo
 #define BT_FLAG_SHIFT 38
 #define BT_VALUE_MASK ((1ull << 38) - 1)
 
+constexpr int kErCols = 16;   // Z columns staged per pass of the ER kernel
+
 __global__ __launch_bounds__(256) void kb_er_sample(BatchSampleArgs a) {
 #pragma clang fp contract(off)
     __shared__ uint32_t s_tile;
     __shared__ int s_wcnt[4];
     __shared__ int64_t s_excl;
+    __shared__ double s_t[4][64][kErCols + 1];
     if (threadIdx.x == 0) s_tile = atomicAdd(a.ticket, 1u);
     __syncthreads();
     const uint32_t tile = s_tile;
     const int g = a.et.graph[tile];
     const int64_t e0 = a.et.start[tile], e1 = a.et.start[tile + 1];
     const int64_t e = e0 + threadIdx.x;
+    const int lane = BT_LANE, wv = BT_WAVE;
     bool keep = false;
     int32_t eu = 0, ev = 0;
     double kwv = 0.0;
     if (e < e1) {
         eu = a.eu[e];
         ev = a.ev[e];
+    }
+    // s = ||Z_u - Z_v||^2 summed in column order.  The rows are gathered
+    // coalesced: in each pass a wave stages (Z_u - Z_v) of kErCols columns
+    // for its 64 edges, four edges per load (16 lanes each, one 128-byte
+    // run per row), then every lane sums its own edge's terms in order.
+    double s = 0.0;
+    if (!a.p_in) {
+        const int sub = lane >> 4, cl = lane & (kErCols - 1);
+        for (int jb = 0; jb < a.k; jb += kErCols) {
+            const int j = jb + cl;
+#pragma unroll
+            for (int i0 = 0; i0 < 64; i0 += 4) {
+                const int i = i0 + sub;
+                const int u = __shfl(eu, i, 64), v = __shfl(ev, i, 64);
+                double t = 0.0;
+                if (j < a.k) t = a.Z[(size_t)u * a.ldz + j] - a.Z[(size_t)v * a.ldz + j];
+                s_t[wv][i][cl] = t;
+            }
+            __syncthreads();
+            const int nj = a.k - jb < kErCols ? a.k - jb : kErCols;
+            for (int c = 0; c < nj; c++) {
+                const double t = s_t[wv][lane][c];
+                s = s + t * t;
+            }
+            __syncthreads();
+        }
+    }
+    if (e < e1) {
         const double w = a.ew[e];
         const double lval = -1.0 * w;
         double p;
         if (a.p_in) {
             p = a.p_in[e];
         } else {
-            const double *zu = a.Z + (size_t)eu * a.k;
-            const double *zv = a.Z + (size_t)ev * a.k;
-            double s = 0.0;
-            for (int j = 0; j < a.k; j++) {
-                const double t = zu[j] - zv[j];
-                s = s + t * t;
-            }
             const double d = sqrt(s);
             if (a.d_out) a.d_out[e] = d;
             const double d2 = d * d;
@@ -627,7 +653,6 @@ __global__ __launch_bounds__(256) void kb_er_sample(BatchSampleArgs a) {
             kwv = -1.0 * (lval + change);
         }
     }
-    const int lane = BT_LANE, wv = BT_WAVE;
     const uint64_t bal = __ballot(keep);
     const int wpre = __popcll(bal & ((1ull << lane) - 1ull));
     if (lane == 0) s_wcnt[wv] = __popcll(bal);
@@ -686,9 +711,9 @@ static inline unsigned bt_grid(int64_t items, int per) {
 }
 
 void bt_sketch(hipStream_t s, Tiles rt, const int32_t *iptr, const int32_t *ieid, const double *ib, int k,
-               const uint64_t *seeds, const double *scales, int uniform, double *Y) {
-    hipLaunchKernelGGL(kb_sketch, dim3(rt.count * kSketchSplit), dim3(64 * rt.waves), 0, s, rt, iptr, ieid, ib, k, seeds, scales, uniform, 0,
-                       Y);
+               SketchCols cols, int uniform, double *Y) {
+    hipLaunchKernelGGL(kb_sketch, dim3(rt.count * kSketchSplit), dim3(64 * rt.waves), 0, s, rt, iptr, ieid, ib, k,
+                       cols, uniform, 0, Y);
 }
 
 template <typename TA, typename TB>

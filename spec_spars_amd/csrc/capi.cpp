@@ -272,7 +272,7 @@ int ss_graph_bytes(int64_t n, int64_t m, int64_t factor_nnz, int32_t k, int32_t 
 int ss_graph_resident_bytes(const ss_graph *g, size_t *bytes) {
     if (!g || !bytes) return fail(SS_ERR_INVALID, "null argument");
     *bytes = g->arena_struct.bytes + g->arena_factor.bytes + g->arena_work.bytes + g->arena_params.bytes +
-             g->arena_core_work.bytes;
+             g->arena_core_work.bytes + g->arena_many.bytes;
     return SS_OK;
 }
 
@@ -354,7 +354,7 @@ static int batch_sparsify(ss_graph *g, const ss_sparsify_opts *opts, ss_sparsify
     SS_HIP(hipSetDevice(c->device));
     hipStream_t s = c->stream;
     SS_TRY(graph_ensure_work(g, k, o0.solve.precision));
-    SS_TRY(graph_set_params(g, prm));
+    SS_TRY(graph_set_params(g, prm, k));
     SS_HIP(hipEventRecord(c->ev[0], s));
     SS_TRY(graph_sketch(g, k, o0.uniform, o0.solve.precision));
     SS_HIP(hipEventRecord(c->ev[1], s));
@@ -403,6 +403,122 @@ int ss_graph_sparsify(ss_graph *g, const ss_sparsify_opts *o, ss_sparsify_report
     SS_GUARD_BEGIN
     return batch_sparsify(g, o, rep);
     SS_GUARD_END
+}
+
+// Several sparsify requests on one graph in one pass (see the header).
+static int graph_sparsify_many(ss_graph *g, int32_t count, const ss_sparsify_opts *opts, ss_sparsify_report *rep) {
+    if (!g->has_factor) return fail(SS_ERR_INVALID, "no factor set (ss_graph_set_factor)");
+    if (count < 1) return fail(SS_ERR_INVALID, "count=%d", count);
+    const ss_sparsify_opts &o0 = opts[0];
+    int K = 0;
+    std::vector<uint64_t> seeds;
+    std::vector<double> scales;
+    std::vector<int32_t> js, off((size_t)count);
+    for (int r = 0; r < count; r++) {
+        const ss_sparsify_opts &o = opts[r];
+        if (o.uniform != o0.uniform || !same_solve(o.solve, o0.solve))
+            return fail(SS_ERR_INVALID, "request %d: uniform and solve options must agree across the requests", r);
+        if (o.k < 1 || K + o.k > kMaxK)
+            return fail(SS_ERR_INVALID, "request %d: k=%d (the requests' columns add up to at most %d)", r, o.k, kMaxK);
+        if (!(o.beta > 0.0) || !std::isfinite(o.beta) || !std::isfinite(o.scale))
+            return fail(SS_ERR_INVALID, "request %d: bad beta %g or scale %g", r, o.beta, o.scale);
+        off[r] = K;
+        for (int j = 0; j < o.k; j++) {
+            seeds.push_back(o.sketch_seed);
+            scales.push_back(o.scale);
+            js.push_back(j);
+        }
+        K += o.k;
+    }
+    ss_context *c = g->ctx;
+    SS_HIP(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    SS_TRY(graph_ensure_work(g, K, o0.solve.precision));
+    if (g->many_cap < count) {
+        g->arena_many.release();
+        g->many_cap = 0;
+        const size_t cap = (size_t)count * (size_t)std::max<int64_t>(g->M, 1);
+        SS_TRY(g->arena_many.alloc(&g->many_ku, cap));
+        SS_TRY(g->arena_many.alloc(&g->many_kv, cap));
+        SS_TRY(g->arena_many.alloc(&g->many_kw, cap));
+        g->many_cap = count;
+    }
+    g->many_kept.assign((size_t)count, 0);
+    SS_TRY(graph_set_columns(g, K, seeds, scales, js));
+    SS_HIP(hipEventRecord(c->ev[0], s));
+    SS_TRY(graph_sketch(g, K, o0.uniform, o0.solve.precision));
+    SS_HIP(hipEventRecord(c->ev[1], s));
+    SolveStats st;
+    SS_TRY(graph_solve(g, K, o0.solve, st));
+    SS_HIP(hipEventRecord(c->ev[2], s));
+    const size_t M = (size_t)std::max<int64_t>(g->M, 1);
+    for (int r = 0; r < count; r++) {
+        SS_TRY(graph_set_sampling(g, 0, opts[r].beta, opts[r].sampling_seed));
+        std::vector<int64_t> ks, kc;
+        SS_TRY(graph_sample_cols(g, opts[r].k, off[r], K, nullptr, false, g->many_ku + r * M, g->many_kv + r * M,
+                                 g->many_kw + r * M, ks, kc));
+        g->many_kept[r] = kc[0];
+    }
+    SS_HIP(hipEventRecord(c->ev[3], s));
+    SS_HIP(hipEventSynchronize(c->ev[3]));
+    SS_TRY(prof_resolve(g));
+    g->last_k = K;
+    double worst = 0.0;
+    int worst_r = 0;
+    for (int r = 0; r < count; r++) {
+        int32_t mi = 0;
+        double mr = 0.0;
+        for (int j = off[r]; j < off[r] + opts[r].k; j++) {
+            mi = std::max(mi, st.iters[j]);
+            mr = std::max(mr, st.relres[j]);
+        }
+        if (mr > worst) {
+            worst = mr;
+            worst_r = r;
+        }
+        if (rep) {
+            ss_sparsify_report &x = rep[r];
+            x.kept = g->many_kept[r];
+            x.max_iters = mi;
+            x.refinements = st.refinements;
+            x.max_relres = mr;
+            x.ms_sketch = elapsed(c->ev[0], c->ev[1]);
+            x.ms_solve = elapsed(c->ev[1], c->ev[2]);
+            x.ms_sample = elapsed(c->ev[2], c->ev[3]);
+            x.ms_total = elapsed(c->ev[0], c->ev[3]);
+        }
+    }
+    if (worst >= o0.solve.fail_relres)
+        return fail(SS_ERR_NOT_CONVERGED, "solve did not converge: request %d has relative residual %.3e >= %.3e",
+                    worst_r, worst, o0.solve.fail_relres);
+    return SS_OK;
+}
+
+int ss_graph_sparsify_many(ss_graph *g, int32_t count, const ss_sparsify_opts *opts, ss_sparsify_report *rep) {
+    if (!g || !opts) return fail(SS_ERR_INVALID, "null argument");
+    SS_TRY(need_single(g));
+    SS_GUARD_BEGIN
+    return graph_sparsify_many(g, count, opts, rep);
+    SS_GUARD_END
+}
+
+int ss_graph_read_kept_many(const ss_graph *g, int32_t r, int64_t cap, int32_t *ku, int32_t *kv, double *kw,
+                            int64_t *kept) {
+    if (!g) return fail(SS_ERR_INVALID, "null graph");
+    SS_TRY(need_single(g));
+    if (r < 0 || r >= (int32_t)g->many_kept.size())
+        return fail(SS_ERR_INVALID, "request %d outside the last ss_graph_sparsify_many call (%zu requests)", r,
+                    g->many_kept.size());
+    const int64_t cnt = g->many_kept[r];
+    if (kept) *kept = cnt;
+    if (cnt > cap) return fail(SS_ERR_CAPACITY, "capacity %lld < kept %lld", (long long)cap, (long long)cnt);
+    if (cnt == 0) return SS_OK;
+    SS_HIP(hipSetDevice(g->ctx->device));
+    const size_t o = (size_t)r * (size_t)std::max<int64_t>(g->M, 1);
+    if (ku) SS_HIP(hipMemcpy(ku, g->many_ku + o, sizeof(int32_t) * cnt, hipMemcpyDeviceToHost));
+    if (kv) SS_HIP(hipMemcpy(kv, g->many_kv + o, sizeof(int32_t) * cnt, hipMemcpyDeviceToHost));
+    if (kw) SS_HIP(hipMemcpy(kw, g->many_kw + o, sizeof(double) * cnt, hipMemcpyDeviceToHost));
+    return SS_OK;
 }
 
 int ss_batch_sparsify(ss_graph *g, const ss_sparsify_opts *opts, ss_sparsify_report *rep) {

@@ -171,8 +171,10 @@ struct ss_graph {
     int *sc_int = nullptr, *colkeep = nullptr;
     ss::PcgScalars sc{};
     // per-call parameters (per graph)
-    uint64_t *p_seeds = nullptr;
-    double *p_scales = nullptr, *p_betas = nullptr;
+    uint64_t *p_colseed = nullptr;   // per (graph, column), G x kMaxK: sketch seed,
+    double *p_colscale = nullptr;    // scale,
+    int32_t *p_colj = nullptr;       // and the column index the signs hash
+    double *p_betas = nullptr;
     uint32_t *p_keys = nullptr;
     // sampling
     double *p = nullptr, *dnorm = nullptr;
@@ -184,6 +186,13 @@ struct ss_graph {
     uint32_t *ticket = nullptr;
     uint32_t samp_epoch = 0;
     std::vector<int64_t> kept_start, kept_count;   // per graph, last sample
+    // several requests sampled from one run (ss_graph_sparsify_many):
+    // request r's kept edges at many_k*[r * M ..], many_kept[r] of them
+    ss::DevArena arena_many;
+    int many_cap = 0;
+    int32_t *many_ku = nullptr, *many_kv = nullptr;
+    double *many_kw = nullptr;
+    std::vector<int64_t> many_kept;
     // per-kernel HIP-event timing (ss_graph_profile)
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
@@ -234,10 +243,22 @@ struct GraphParams {
     uint64_t sketch_seed = 0, sampling_seed = 0;
     double scale = 1.0, beta = 1.0;
 };
-int graph_set_params(ss_graph *g, const std::vector<GraphParams> &prm);
+// Per-graph parameters for k columns (column j of graph i hashes index j
+// with graph i's sketch seed and scale).
+int graph_set_params(ss_graph *g, const std::vector<GraphParams> &prm, int k);
+// A batch of one with several requests side by side: column c hashes
+// index js[c] with seeds[c] and scale scales[c].
+int graph_set_columns(ss_graph *g, int k, const std::vector<uint64_t> &seeds, const std::vector<double> &scales,
+                      const std::vector<int32_t> &js);
+// Sampling parameters of graph i: beta and the coin stream's seed.
+int graph_set_sampling(ss_graph *g, int i, double beta, uint64_t sampling_seed);
 int graph_sketch(ss_graph *g, int k, int uniform, int prec);
 // Time launch-shape candidates on g (see ss_context_autotune).
 int graph_autotune(ss_graph *g, int k, int prec, int reps, ss_tuning_state &best, double &ms_level,
                    double &ms_tile);
 int graph_sample(ss_graph *g, int k, const double *p_in_dev, bool write_d);
+// Sampling from columns [zoff, zoff + k) of the n x ldz solution block into
+// the given kept arrays (capacity M); per graph start and count.
+int graph_sample_cols(ss_graph *g, int k, int zoff, int ldz, const double *p_in_dev, bool write_d, int32_t *ku,
+                      int32_t *kv, double *kw, std::vector<int64_t> &kept_start, std::vector<int64_t> &kept_count);
 }  // namespace ss

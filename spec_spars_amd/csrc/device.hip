@@ -314,8 +314,9 @@ int batch_upload(ss_context *ctx, const std::vector<const HostGraph *> &graphs, 
     SS_TRY(A.alloc(&g->tile_incl, (size_t)g->net + 1));
     SS_TRY(A.alloc(&g->ticket, 1));
     auto &P = g->arena_params;
-    SS_TRY(P.alloc(&g->p_seeds, G));
-    SS_TRY(P.alloc(&g->p_scales, G));
+    SS_TRY(P.alloc(&g->p_colseed, (size_t)G * kMaxK));
+    SS_TRY(P.alloc(&g->p_colscale, (size_t)G * kMaxK));
+    SS_TRY(P.alloc(&g->p_colj, (size_t)G * kMaxK));
     SS_TRY(P.alloc(&g->p_betas, G));
     SS_TRY(P.alloc(&g->p_keys, (size_t)8 * G));
     hipStream_t s = ctx->stream;
@@ -784,6 +785,7 @@ template <typename T> static const T *avals(const ss_graph *g);
 template <> const double *avals<double>(const ss_graph *g) { return g->val; }
 template <> const float *avals<float>(const ss_graph *g) { return g->valf; }
 
+static SketchCols sketch_cols(const ss_graph *g) { return SketchCols{g->p_colseed, g->p_colscale, g->p_colj}; }
 static Tiles rtiles(const ss_graph *g) { return Tiles{g->rt_start, g->rt_graph, g->nrt, g->tile_waves}; }
 static Tiles stiles(const ss_graph *g) { return Tiles{g->st_start, g->st_graph, g->nst, g->tile_waves}; }
 
@@ -1123,8 +1125,8 @@ static int autotune_t(ss_graph *g, int k, const PcgWork<T> &w, int reps, ss_tuni
     // a realistic right-hand side: the sketch, projected (FP64) and converted
     std::vector<GraphParams> prm((size_t)g->G);
     for (auto &p : prm) p.scale = 1.0 / std::sqrt((double)k);
-    SS_TRY(graph_set_params(g, prm));
-    bt_sketch(s, rtiles(g), g->iptr, g->ieid, g->ib, k, g->p_seeds, g->p_scales, 0, g->Y64);
+    SS_TRY(graph_set_params(g, prm, k));
+    bt_sketch(s, rtiles(g), g->iptr, g->ieid, g->ib, k, sketch_cols(g), 0, g->Y64);
     component_means(g, k, g->Y64);
     if (std::is_same<T, double>::value) {
         bt_project<double, double>(s, rtiles(g), k, g->Y64, g->comp_of_row, g->cmean, (double *)w.R, (double *)w.B,
@@ -1189,25 +1191,57 @@ int graph_autotune(ss_graph *g, int k, int prec, int reps, ss_tuning_state &best
     return autotune_t<float>(g, k, g->w32, reps, best, ms_level, ms_tile);
 }
 
-int graph_set_params(ss_graph *g, const std::vector<GraphParams> &prm) {
+int graph_set_params(ss_graph *g, const std::vector<GraphParams> &prm, int k) {
     if ((int)prm.size() != g->G) return fail(SS_ERR_INVALID, "need %d parameter sets, got %zu", g->G, prm.size());
-    std::vector<uint64_t> seeds(g->G);
-    std::vector<double> scales(g->G), betas(g->G);
+    if (k < 1 || k > kMaxK) return fail(SS_ERR_INVALID, "k=%d outside 1..%d", k, kMaxK);
+    const size_t gk = (size_t)g->G * k;
+    std::vector<uint64_t> seeds(gk);
+    std::vector<double> scales(gk), betas(g->G);
+    std::vector<int32_t> js(gk);
     std::vector<uint32_t> keys((size_t)8 * g->G);
     for (int i = 0; i < g->G; i++) {
-        seeds[i] = prm[i].sketch_seed;
-        scales[i] = prm[i].scale;
+        for (int j = 0; j < k; j++) {
+            seeds[(size_t)i * k + j] = prm[i].sketch_seed;
+            scales[(size_t)i * k + j] = prm[i].scale;
+            js[(size_t)i * k + j] = j;
+        }
         betas[i] = prm[i].beta;
         const CoinKey ck = coin_key_from_u64(prm[i].sampling_seed);
         for (int q = 0; q < 8; q++) keys[(size_t)8 * i + q] = ck.k[q];
     }
     SS_HIP(hipSetDevice(g->ctx->device));
     hipStream_t s = g->ctx->stream;
-    SS_TRY(h2d(s, g->p_seeds, seeds.data(), seeds.size()));
-    SS_TRY(h2d(s, g->p_scales, scales.data(), scales.size()));
+    SS_TRY(h2d(s, g->p_colseed, seeds.data(), gk));
+    SS_TRY(h2d(s, g->p_colscale, scales.data(), gk));
+    SS_TRY(h2d(s, g->p_colj, js.data(), gk));
     SS_TRY(h2d(s, g->p_betas, betas.data(), betas.size()));
     SS_TRY(h2d(s, g->p_keys, keys.data(), keys.size()));
     // the host vectors go out of scope: wait for the copies
+    SS_HIP(hipStreamSynchronize(s));
+    return SS_OK;
+}
+
+int graph_set_columns(ss_graph *g, int k, const std::vector<uint64_t> &seeds, const std::vector<double> &scales,
+                      const std::vector<int32_t> &js) {
+    if (g->G != 1) return fail(SS_ERR_INVALID, "per-column parameters need a batch of one");
+    if (k < 1 || k > kMaxK || (int)seeds.size() != k || (int)scales.size() != k || (int)js.size() != k)
+        return fail(SS_ERR_INVALID, "per-column parameters: k=%d", k);
+    SS_HIP(hipSetDevice(g->ctx->device));
+    hipStream_t s = g->ctx->stream;
+    SS_TRY(h2d(s, g->p_colseed, seeds.data(), (size_t)k));
+    SS_TRY(h2d(s, g->p_colscale, scales.data(), (size_t)k));
+    SS_TRY(h2d(s, g->p_colj, js.data(), (size_t)k));
+    SS_HIP(hipStreamSynchronize(s));
+    return SS_OK;
+}
+
+int graph_set_sampling(ss_graph *g, int i, double beta, uint64_t sampling_seed) {
+    if (i < 0 || i >= g->G) return fail(SS_ERR_INVALID, "graph %d outside the batch of %d", i, g->G);
+    const CoinKey ck = coin_key_from_u64(sampling_seed);
+    SS_HIP(hipSetDevice(g->ctx->device));
+    hipStream_t s = g->ctx->stream;
+    SS_TRY(h2d(s, g->p_betas + i, &beta, 1));
+    SS_TRY(h2d(s, g->p_keys + (size_t)8 * i, ck.k, 8));
     SS_HIP(hipStreamSynchronize(s));
     return SS_OK;
 }
@@ -1216,18 +1250,24 @@ int graph_sketch(ss_graph *g, int k, int uniform, int prec) {
     SS_TRY(graph_ensure_work(g, k, prec));
     SS_HIP(hipSetDevice(g->ctx->device));
     ProfScope ps(g, SS_KERNEL_SKETCH);
-    bt_sketch(g->ctx->stream, rtiles(g), g->iptr, g->ieid, g->ib, k, g->p_seeds, g->p_scales, uniform, g->Y64);
+    bt_sketch(g->ctx->stream, rtiles(g), g->iptr, g->ieid, g->ib, k, sketch_cols(g), uniform, g->Y64);
     SS_HIP(hipGetLastError());
     return SS_OK;
 }
 
 int graph_sample(ss_graph *g, int k, const double *p_in_dev, bool write_d) {
+    return graph_sample_cols(g, k, 0, k, p_in_dev, write_d, g->ku, g->kv, g->kw, g->kept_start, g->kept_count);
+}
+
+int graph_sample_cols(ss_graph *g, int k, int zoff, int ldz, const double *p_in_dev, bool write_d, int32_t *ku,
+                      int32_t *kv, double *kw, std::vector<int64_t> &kept_start, std::vector<int64_t> &kept_count) {
     SS_HIP(hipSetDevice(g->ctx->device));
     hipStream_t s = g->ctx->stream;
-    std::fill(g->kept_start.begin(), g->kept_start.end(), 0);
-    std::fill(g->kept_count.begin(), g->kept_count.end(), 0);
+    kept_start.assign((size_t)g->G, 0);
+    kept_count.assign((size_t)g->G, 0);
     if (g->M == 0) return SS_OK;
-    if (!p_in_dev && g->wk != k) return fail(SS_ERR_INVALID, "no solution block for k=%d", k);
+    if (!p_in_dev && (g->wk != ldz || zoff < 0 || k < 1 || zoff + k > ldz))
+        return fail(SS_ERR_INVALID, "no solution columns [%d, %d) in a block of %d", zoff, zoff + k, g->wk);
     if (g->samp_epoch >= (1u << 24) - 2) {
         SS_HIP(hipMemsetAsync(g->tile_state, 0, sizeof(uint64_t) * ((size_t)g->net + 1), s));
         g->samp_epoch = 0;
@@ -1241,17 +1281,18 @@ int graph_sample(ss_graph *g, int k, const double *p_in_dev, bool write_d) {
     a.eu = g->eu;
     a.ev = g->ev;
     a.ew = g->ew;
-    a.Z = g->X64;
+    a.Z = g->X64 + zoff;
     a.k = k;
+    a.ldz = ldz;
     a.beta = g->p_betas;
     a.keys = g->p_keys;
     a.p_in = p_in_dev;
     a.p_out = g->p;
     a.d_out = write_d ? g->dnorm : nullptr;
     a.mask = g->mask;
-    a.ku = g->ku;
-    a.kv = g->kv;
-    a.kw = g->kw;
+    a.ku = ku;
+    a.kv = kv;
+    a.kw = kw;
     a.ticket = g->ticket;
     a.tile_state = g->tile_state;
     a.tile_incl = g->tile_incl;
@@ -1270,8 +1311,8 @@ int graph_sample(ss_graph *g, int k, const double *p_in_dev, bool write_d) {
         const int32_t t0 = g->etile_off[i], t1 = g->etile_off[i + 1];
         const int64_t start = t0 > 0 ? incl[t0 - 1] : 0;
         const int64_t end = t1 > t0 ? incl[t1 - 1] : start;
-        g->kept_start[i] = start;
-        g->kept_count[i] = end - start;
+        kept_start[i] = start;
+        kept_count[i] = end - start;
     }
     return SS_OK;
 }

@@ -29,6 +29,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <sys/mman.h>
 
 namespace ss {
 
@@ -55,6 +56,31 @@ struct Ent {
     double val;
 };
 
+// A large zero-initialised array from anonymous memory (pages fault in as
+// they are first touched, by whichever thread touches them), backed by huge
+// pages where the system allows: the dense tail's random updates then miss
+// the TLB far less often.
+template <typename T>
+class ZeroedBuffer {
+public:
+    explicit ZeroedBuffer(size_t n) : bytes_(n * sizeof(T)) {
+        void *p = mmap(nullptr, bytes_ ? bytes_ : 1, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS, -1, 0);
+        if (p == MAP_FAILED) return;
+        (void)madvise(p, bytes_, MADV_HUGEPAGE);
+        p_ = static_cast<T *>(p);
+    }
+    ~ZeroedBuffer() {
+        if (p_) munmap(p_, bytes_ ? bytes_ : 1);
+    }
+    ZeroedBuffer(const ZeroedBuffer &) = delete;
+    ZeroedBuffer &operator=(const ZeroedBuffer &) = delete;
+    T *data() const { return p_; }
+
+private:
+    size_t bytes_;
+    T *p_ = nullptr;
+};
+
 struct Fill {
     int32_t row;
     int32_t next;   // index into the fill pool, -1 ends the list
@@ -64,23 +90,16 @@ struct Fill {
 }  // namespace
 
 // Eliminate one vertex whose live lower neighbours (row, |L_rc| + fill) are
-// in `work`: merge equal rows, sort by value, sample one fill edge per entry
-// but the last, scale.  Fill edges go to emit(a, b, value); the column of L
-// (rows ascending, values -v/tot) to `col`.  Returns D = tot (0 if empty).
+// in `work`, distinct rows in ascending order: sort by value, sample one
+// fill edge per entry but the last, scale.  Fill edges go to emit(a, b,
+// value); the column of L (rows ascending, values -v/tot) to `col`.
+// Returns D = tot (0 if empty).
 template <typename Emit>
-static double eliminate(std::vector<Ent> &work, std::vector<double> &cum, ColumnRng &rng, std::vector<Ent> &col,
-                        Emit emit) {
-    col.clear();
-    if (work.empty()) return 0.0;
-    std::sort(work.begin(), work.end(), [](const Ent &a, const Ent &b) {
-        return a.row != b.row ? a.row < b.row : a.val < b.val;
-    });
-    size_t cnt = 0;
-    for (size_t i = 0; i < work.size(); i++) {
-        if (cnt > 0 && work[cnt - 1].row == work[i].row) work[cnt - 1].val += work[i].val;
-        else work[cnt++] = work[i];
-    }
-    work.resize(cnt);
+static double eliminate_merged(std::vector<Ent> &work, std::vector<double> &cum, ColumnRng &rng,
+                               std::vector<Ent> &col, Emit emit) {
+    const size_t cnt = work.size();
+    col.assign(work.begin(), work.end());   // rows ascending
+    if (cnt == 0) return 0.0;
     std::sort(work.begin(), work.end(), [](const Ent &a, const Ent &b) {
         return a.val != b.val ? a.val < b.val : a.row < b.row;
     });
@@ -101,9 +120,25 @@ static double eliminate(std::vector<Ent> &work, std::vector<double> &cum, Column
         const size_t j = lo < cnt ? lo : cnt - 1;
         emit(work[i].row, work[j].row, work[i].val * rest / tot);
     }
-    for (size_t i = 0; i < cnt; i++) col.push_back({work[i].row, -(work[i].val / tot)});
-    std::sort(col.begin(), col.end(), [](const Ent &a, const Ent &b) { return a.row < b.row; });
+    for (Ent &e : col) e.val = -(e.val / tot);
     return tot;
+}
+
+// The same for a work list that may repeat rows: equal rows are merged
+// first, their values summed in ascending order.
+template <typename Emit>
+static double eliminate(std::vector<Ent> &work, std::vector<double> &cum, ColumnRng &rng, std::vector<Ent> &col,
+                        Emit emit) {
+    std::sort(work.begin(), work.end(), [](const Ent &a, const Ent &b) {
+        return a.row != b.row ? a.row < b.row : a.val < b.val;
+    });
+    size_t cnt = 0;
+    for (size_t i = 0; i < work.size(); i++) {
+        if (cnt > 0 && work[cnt - 1].row == work[i].row) work[cnt - 1].val += work[i].val;
+        else work[cnt++] = work[i];
+    }
+    work.resize(cnt);
+    return eliminate_merged(work, cum, rng, col, emit);
 }
 
 // Natural order on the given CSC (positions == indices of the CSC).
@@ -269,25 +304,82 @@ static int factor_rounds(int64_t n, const int32_t *colptr, const int32_t *rowidx
                              std::chrono::duration<double>(t_tail - t_start).count());
                 std::fflush(stderr);
             }
+            // tail ids in vertex order; every live list sorted by (row,
+            // value) and merged once, so the sequential eliminations below
+            // see lists in an order independent of the thread count
+            std::vector<int32_t> tid((size_t)n, -1);
+            for (int64_t x = 0; x < L; x++) tid[live[x]] = (int32_t)x;   // live is ascending
+            pool.run([&](int t) {
+                int64_t lo, hi;
+                HostPool::range(L, active, t, lo, hi);
+                for (int64_t x = lo; x < hi; x++) {
+                    auto &a = adj[live[x]];
+                    std::sort(a.begin(), a.end(), [](const Ent &p, const Ent &q) {
+                        return p.row != q.row ? p.row < q.row : p.val < q.val;
+                    });
+                    size_t c = 0;
+                    for (size_t i = 0; i < a.size(); i++) {
+                        if (c > 0 && a[c - 1].row == a[i].row) a[c - 1].val += a[i].val;
+                        else a[c++] = a[i];
+                    }
+                    a.resize(c);
+                }
+            }, active);
             cand.assign(live.begin(), live.end());
             std::sort(cand.begin(), cand.end(),
                       [&](int32_t x, int32_t y) { return key[x] != key[y] ? key[x] < key[y] : x < y; });
             PerThread &P = pt[0];
+            // the tail graph as a dense symmetric matrix over tail ids (in
+            // vertex order): its live entries accumulate fills in
+            // elimination order, and a row scan lists a vertex's live
+            // neighbours with rows ascending
+            ZeroedBuffer<double> W((size_t)L * (size_t)L);
+            if (!W.data()) return fail(SS_ERR_NOMEM, "factor: dense tail of %lld vertices", (long long)L);
+            pool.run([&](int t) {
+                int64_t lo, hi;
+                HostPool::range(L, active, t, lo, hi);
+                for (int64_t x = lo; x < hi; x++) {
+                    double *row = W.data() + (size_t)x * L;
+                    for (const Ent &e : adj[live[x]]) row[tid[e.row]] = e.val;
+                    std::vector<Ent>().swap(adj[live[x]]);
+                }
+            }, active);
+            std::vector<char> tdone((size_t)L, 0);
+            struct TailFill {
+                int32_t a, b;   // tail ids
+                double val;
+            };
+            std::vector<TailFill> tf;
             for (int32_t v : cand) {
                 order.push_back(v);
+                const int32_t tv = tid[v];
+                const double *row = W.data() + (size_t)tv * L;
                 P.work.clear();
-                for (const Ent &e : adj[v])
-                    if (!elim[e.row]) P.work.push_back(e);
+                for (int64_t t = 0; t < L; t++)
+                    if (row[t] != 0.0 && !tdone[t]) P.work.push_back({live[t], row[t]});
                 ColumnRng rng(seed, v);
-                vdiag[v] = eliminate(P.work, P.cum, rng, P.col, [&](int32_t a, int32_t b, double fv) {
-                    adj[a].push_back({b, fv});
-                    adj[b].push_back({a, fv});
+                tf.clear();
+                vdiag[v] = eliminate_merged(P.work, P.cum, rng, P.col, [&](int32_t a, int32_t b, double fv) {
+                    tf.push_back({tid[a], tid[b], fv});
                 });
+                // one elimination adds to each entry of W at most once (its
+                // fill pairs are distinct), so the additions may go in any
+                // order: threads take the rows of their range
+                const int fact = tf.size() >= 2048 ? T : 1;
+                pool.run([&](int t) {
+                    int64_t lo, hi;
+                    HostPool::range(L, fact, t, lo, hi);
+                    double *w = W.data();
+                    for (const TailFill &f : tf) {
+                        if (f.a >= lo && f.a < hi) w[(size_t)f.a * L + f.b] += f.val;
+                        if (f.b >= lo && f.b < hi) w[(size_t)f.b * L + f.a] += f.val;
+                    }
+                }, fact);
                 vstart[v] = (int64_t)crow.size();
                 vcount[v] = (int64_t)P.col.size();
                 for (const Ent &e : P.col) { crow.push_back(e.row); cval.push_back(e.val); }
                 elim[v] = 1;
-                std::vector<Ent>().swap(adj[v]);
+                tdone[tv] = 1;
                 rounds++;
             }
             tail_rows = (int64_t)cand.size();
@@ -423,20 +515,54 @@ int factor_build(int64_t n, const int32_t *colptr, const int32_t *rowidx, const 
         std::vector<double> cval, vdiag;
         SS_TRY(factor_rounds(n, colptr, rowidx, vals, seed, factor_threads(), F.order, vsc, crow, cval, vdiag,
                              F.rounds, F.tail));
-        for (int64_t p = 0; p < n; p++) rank[F.order[p]] = (int32_t)p;
-        F.diag.resize((size_t)n);
-        prow.reserve(crow.size());
-        pval.reserve(cval.size());
-        std::vector<Ent> tmp;
-        for (int64_t p = 0; p < n; p++) {
-            const int32_t v = F.order[p];
-            F.diag[p] = vdiag[v];
-            tmp.clear();
-            for (int64_t q = vsc[2 * v]; q < vsc[2 * v] + vsc[2 * v + 1]; q++) tmp.push_back({rank[crow[q]], cval[q]});
-            std::sort(tmp.begin(), tmp.end(), [](const Ent &a, const Ent &b) { return a.row < b.row; });
-            for (const Ent &e : tmp) { prow.push_back(e.row); pval.push_back(e.val); }
-            pstart[p + 1] = (int64_t)prow.size();
+        const auto t_elim = std::chrono::steady_clock::now();
+        if (std::getenv("SPEC_SPARS_VERBOSE")) {
+            std::fprintf(stderr, "[factor] eliminated at %.1f s\n", std::chrono::duration<double>(t_elim - t0).count());
+            std::fflush(stderr);
         }
+        for (int64_t p = 0; p < n; p++) rank[F.order[p]] = (int32_t)p;
+        // U rows (columns of L in position ids, diagonal first, rows
+        // ascending) assembled in parallel over positions
+        F.diag.resize((size_t)n);
+        F.urowptr.assign((size_t)n + 1, 0);
+        int64_t tot = 0;
+        for (int64_t p = 0; p < n; p++) {
+            F.urowptr[p] = (int32_t)std::min<int64_t>(tot, INT32_MAX);
+            tot += 1 + vsc[2 * (size_t)F.order[p] + 1];
+        }
+        if (tot > (int64_t)INT32_MAX) return fail(SS_ERR_CAPACITY, "factor: nnz %lld exceeds int32", (long long)tot);
+        F.urowptr[n] = (int32_t)tot;
+        F.ucol.resize((size_t)tot);
+        F.uval.resize((size_t)tot);
+        HostPool pool(factor_threads());
+        pool.run([&](int t) {
+            int64_t lo, hi;
+            HostPool::range(n, pool.size(), t, lo, hi);
+            std::vector<Ent> tmp;
+            for (int64_t p = lo; p < hi; p++) {
+                const int32_t v = F.order[p];
+                F.diag[p] = vdiag[v];
+                tmp.clear();
+                for (int64_t q = vsc[2 * v]; q < vsc[2 * v] + vsc[2 * v + 1]; q++) tmp.push_back({rank[crow[q]], cval[q]});
+                std::sort(tmp.begin(), tmp.end(), [](const Ent &a, const Ent &b) { return a.row < b.row; });
+                int64_t o = F.urowptr[p];
+                F.ucol[o] = (int32_t)p;
+                F.uval[o] = 1.0;
+                for (const Ent &e : tmp) { o++; F.ucol[o] = e.row; F.uval[o] = e.val; }
+            }
+        });
+        F.nnz = tot;
+        const auto t_asm = std::chrono::steady_clock::now();
+        factor_levels(F);
+        F.ms_build = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (std::getenv("SPEC_SPARS_VERBOSE")) {
+            std::fprintf(stderr, "[factor] assembled at %.1f s, levels %.1f s, total %.1f s\n",
+                         std::chrono::duration<double>(t_asm - t0).count(),
+                         std::chrono::duration<double>(std::chrono::steady_clock::now() - t_asm).count(),
+                         F.ms_build / 1000.0);
+            std::fflush(stderr);
+        }
+        return SS_OK;
     } else {
         F.order.resize((size_t)n);
         for (int64_t i = 0; i < n; i++) F.order[i] = (int32_t)i;

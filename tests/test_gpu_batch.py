@@ -489,3 +489,33 @@ def test_columns_independent_of_k(ctx, prec):
         assert np.array_equal(Z8, Z40[:, :8])
     finally:
         G.close()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_sparsify_many_matches_alone(ctx, prec):
+    """Several sparsify requests merged into one pass on one graph (their
+    sketch columns side by side, one multi-RHS solve, one sampling pass
+    each) give every request the kept edges it gets alone, bit for bit,
+    in FP64 and in mixed precision, on a graph with a dense tail."""
+    import spec_spars_amd as ss
+    from spec_spars_amd import graphs
+    n, u, v = graphs.chung_lu(20000, 200000, gamma=2.2, seed=7)
+    w = graphs.uniform_weights(len(u), seed=7)
+    G = ss.Graph(ctx, n, u, v, w)
+    F = ss.Factor(*G.laplacian(), seed=1, ordering=ss.ORDER_ROUNDS)
+    assert F.tail() > 0
+    G.set_factor(F)
+    solve = ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-8, max_iter=2000, precision=prec)
+    reqs = [dict(k=16, beta=graphs.beta_for(n, 0.3), sketch_seed=1, sampling_seed=2),
+            dict(k=24, beta=graphs.beta_for(n, 0.5), sketch_seed=3, sampling_seed=4, scale=0.3),
+            dict(k=8, beta=graphs.beta_for(n, 0.2), sketch_seed=5, sampling_seed=6)]
+    reps = G.sparsify_many(reqs, solve=solve)
+    many = [G.read_kept_many(r) for r in range(len(reqs))]
+    for r, q in enumerate(reqs):
+        rep = G.sparsify(q["k"], q["beta"], scale=q.get("scale", 1.0 / math.sqrt(q["k"])),
+                         sketch_seed=q["sketch_seed"], sampling_seed=q["sampling_seed"], solve=solve)
+        ku, kv, kw = G.read_kept()
+        assert rep["kept"] == reps[r]["kept"] == len(ku) and rep["max_iters"] == reps[r]["max_iters"]
+        assert np.array_equal(ku, many[r][0]) and np.array_equal(kv, many[r][1])
+        assert np.array_equal(kw, many[r][2])
+    G.close()

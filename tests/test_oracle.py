@@ -220,3 +220,25 @@ def test_sampling_verify_ratios(oracle_mod):
         ge = set(zip(uu.tolist(), vv.tolist()))
         assert all((a, b) in ge for a, b in zip(r["ku"].tolist(), r["kv"].tolist()))
         assert np.allclose(r["kw"], r["kw"] * 0 + np.sqrt(2.0) * w[r["mask"].astype(bool)], rtol=1e-15)
+
+
+def test_sketch_edges_and_permute_csc(oracle_mod):
+    """The subset sketch gives the full sketch rows of every row whose edges
+    are all in the subset (and the full sketch for all edges); the CSC
+    permutation equals scipy's P A P^T."""
+    import scipy.sparse as sp
+    from spec_spars_amd import graphs
+    n, u, v = graphs.chung_lu(3000, 20000, seed=1)
+    w = graphs.uniform_weights(len(u), seed=1)
+    Y = oracle_mod.sketch(u, v, w, n, 16, seed=3, scale=0.25)
+    rows = np.array([5, 17, 1000, 2999])
+    sel = np.nonzero(np.isin(u, rows) | np.isin(v, rows))[0]
+    Ys = oracle_mod.sketch_edges(sel, u[sel], v[sel], w[sel], n, 16, seed=3, scale=0.25)
+    assert np.array_equal(Ys[rows], Y[rows])
+    assert np.array_equal(oracle_mod.sketch_edges(np.arange(len(u)), u, v, w, n, 16, seed=3, scale=0.25), Y)
+    cp, ri, va = oracle_mod.laplacian(n, u, v, w)
+    order = np.random.default_rng(0).permutation(n)
+    pc, pr, pv = oracle_mod.permute_csc(cp, ri, va, order)
+    Lp = sp.csc_matrix((va, ri, cp), shape=(n, n))[order][:, order].tocsc()
+    Lp.sort_indices()
+    assert np.array_equal(Lp.indptr, pc) and np.array_equal(Lp.indices, pr) and np.array_equal(Lp.data, pv)
