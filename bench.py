@@ -1,16 +1,28 @@
 #!/usr/bin/env python3
 """Benchmark of the sparsification hot path on one MI355X.
 
-Workload (default): the 3D 128^3 grid (n = 2,097,152, m = 6,242,304), weights
-10^U(-3,3) in canonical edge order, k = 48 Rademacher sketch columns scaled
-1/sqrt(k), FP64 SpMM / triangular solves, PCG to a 1e-8 relative residual,
-eps = 0.3 sampling, sketch and sampling seed 0.  A step is one pass of the
-hot path over the resident graph: sketch -> multi-RHS PCG -> fused
-ER/coin/sample/compaction (the host factor is built and uploaded once, before
-timing).  value = input edges / step time.
+Default workload: the Chung-Lu power-law graph n = 5,000,000, m = 50,000,000
+(degree exponent 2.2, weights U[0.5, 2]), k = 64 Rademacher sketch columns
+scaled 1/sqrt(k), PCG to a 1e-8 relative residual, eps = 0.3.  A step is one
+pass of the hot path over the resident graph (sketch -> multi-RHS PCG ->
+fused ER/coin/sample/compaction); the host factor is built and uploaded once
+before timing.  Legs of one run:
 
-Prints one JSON line (rank 0).  --gpus above 1 is refused: this build runs on
-one GPU.
+  plain   FP64 SpMM / triangular solves, one request per pass (k = 64).
+  served  the mixed-precision served form: R requests (different eps and
+          seeds) merged into one pass -- their sketch columns side by side,
+          one multi-RHS solve with FP32 SpMM / triangular solves and FP64
+          reductions and refinement, one sampling pass each.  Every request's
+          kept edges equal what it gets alone (tested bit for bit).
+  value = served edges/s = R * m / served step time.
+
+Also reported: the plain method column by column (the reference's structure:
+k single-RHS solves, FP64) on a timed sample of columns, leverage scores of
+the mixed solve against the FP64 solve of the same sketch, a bounded serving
+window (Poisson arrivals, p50/p99 latency), the dominant kernel's roofline
+and a CPU baseline on the host cores.
+
+Prints one JSON line.  --gpus above 1 is refused: this build runs on one GPU.
 """
 import argparse
 import json
@@ -32,23 +44,23 @@ def log(*a):
 
 def workload(name):
     from spec_spars_amd import graphs
+    import numpy as np
+    if name == "chunglu5m":
+        n, u, v = graphs.chung_lu(5_000_000, 50_000_000, gamma=2.2, seed=0)
+        w = graphs.uniform_weights(len(u), seed=0)
+        return dict(name=name, desc="Chung-Lu n=5e6 m=5e7 gamma=2.2, weights U[0.5,2], k=64, PCG rel 1e-8, eps=0.3",
+                    n=n, u=u, v=v, w=w, k=64, eps=0.3, precision="mixed")
     if name == "grid3d128":
         n, u, v = graphs.grid3d(128)
         w = graphs.loguniform_weights(len(u), seed=0)
         return dict(name=name, desc="3D grid 128^3, weights 10^U(-3,3), k=48, FP64, PCG rel 1e-8, eps=0.3",
-                    n=n, u=u, v=v, w=w, k=48, eps=0.3)
+                    n=n, u=u, v=v, w=w, k=48, eps=0.3, precision="fp64")
     if name == "grid3d64":
         n, u, v = graphs.grid3d(64)
         w = graphs.loguniform_weights(len(u), seed=0)
         return dict(name=name, desc="3D grid 64^3, weights 10^U(-3,3), k=48, FP64, PCG rel 1e-8, eps=0.3",
-                    n=n, u=u, v=v, w=w, k=48, eps=0.3)
-    if name == "chunglu5m":
-        n, u, v = graphs.chung_lu(5_000_000, 50_000_000, gamma=2.2, seed=0)
-        w = graphs.uniform_weights(len(u), seed=0)
-        return dict(name=name, desc="Chung-Lu n=5e6 m=5e7 gamma=2.2, weights U[0.5,2], k=64, mixed precision, "
-                    "PCG rel 1e-8, eps=0.3", n=n, u=u, v=v, w=w, k=64, eps=0.3, precision="mixed")
+                    n=n, u=u, v=v, w=w, k=48, eps=0.3, precision="fp64")
     if name == "rgg512":
-        import numpy as np
         rng = np.random.default_rng(512)
         sizes = np.round(10.0 ** rng.uniform(3.0, math.log10(2e5), 512)).astype(np.int64)
         gs = []
@@ -60,16 +72,16 @@ def workload(name):
                     precision="mixed", n=int(sizes.sum()), m=int(sum(len(g[1]) for g in gs)))
     if name == "grid2d64":
         n, u, v = graphs.grid2d(64, 64)
-        import numpy as np
         return dict(name=name, desc="2D grid 64x64, unit weights, k=24, FP64, PCG rel 1e-8, eps=0.3",
-                    n=n, u=u, v=v, w=np.ones(len(u)), k=24, eps=0.3)
+                    n=n, u=u, v=v, w=np.ones(len(u)), k=24, eps=0.3, precision="fp64")
     raise SystemExit("unknown workload " + name)
 
 
 def kernel_bytes(n, m, G, k, kept, s=8):
     """Algorithmic (compulsory) HBM bytes per launch of each kernel kind:
     every array counted once per pass, gathers beyond first touch not
-    counted.  n, m: total vertices / edges of the (batched) system."""
+    counted.  n, m: total vertices / edges of the (batched) system; k the
+    columns of the pass; s the bytes of the solve's values."""
     nnzL = n + 2 * m
     fi, bi = G.schedule_info(0), G.schedule_info(1)
 
@@ -80,78 +92,36 @@ def kernel_bytes(n, m, G, k, kept, s=8):
         b = info["level_rows"] * (4 + 4 + 2 * k * s + extra_per_row) + info["level_nnz"] * (4 + s)
         return b / info["level_launches"]
 
-    def small(info, extra_per_row, passes, extra_launches):
-        # all runs of one application over their launches (runs, plus the
-        # forward trailing run's head-segment launch)
+    def small(info, extra_per_row):
         if not info["small_levels"]:
             return None
-        total = info["small_rows"] * (4 + 4 + passes * 2 * k * s + extra_per_row) + info["small_nnz"] * (4 + s)
-        return total / (info["small_runs"] + extra_launches)
+        total = info["small_rows"] * (4 + 4 + 2 * k * s + extra_per_row) + info["small_nnz"] * (4 + s)
+        return total / info["small_runs"]
 
-    def core(info):
-        # dense cores, per launch: rows' outside entries (forward), the
-        # lower triangles once, every core row read and written once
+    def core(info, launches):
+        # dense cores, one direction: the inverse's triangle once, the core
+        # rows read (gathered) and written once, the forward head's entries
+        # outside the core; over the direction's launches
         if not info["core_rows"]:
             return None
-        total = info["core_outside_nnz"] * (4 + s) + info["core_rows"] * (4 + 2 * k * s) + \
-            info["core_entries"] // 2 * s
-        return total / (1 + info["core_steps"])
+        total = info["core_entries"] // 2 * s + info["core_rows"] * 2 * k * s + \
+            info["core_outside_nnz"] * (4 + s)
+        return total / launches
 
     return {
-        # sketch: incidence lists (edge id + signed sqrt(w)) and Y
-        "sketch": 4 * (n + 1) + 2 * m * (4 + 8) + n * k * s,
+        # sketch: incidence lists (edge id + signed sqrt(w)) and Y (FP64)
+        "sketch": 4 * (n + 1) + 2 * m * (4 + 8) + n * k * 8,
         # SpMM: rowptr, (col, val) per nonzero, read P, write Q
         "spmm": 4 * (n + 1) + nnzL * (4 + s) + 2 * n * k * s,
         "tri_fwd": per_level(fi, 0),
         "tri_bwd": per_level(bi, 8),
-        "tri_fwd_small": small(fi, 0, 1, 0),
-        "tri_bwd_small": small(bi, 8, 1, 0),
-        "core_fwd": core(fi),
-        "core_bwd": core(bi),
-        # ER/sample: edges (u, v, w), Z once, p out, mask out, kept (u, v, w) out
-        "er_sample": m * (4 + 4 + 8) + n * k * s + 8 * m + m + kept * 16,
+        "tri_fwd_small": small(fi, 0),
+        "tri_bwd_small": small(bi, 8),
+        "core_fwd": core(fi, 4),
+        "core_bwd": core(bi, 3),
+        # ER/sample: edges (u, v, w), Z (FP64) once, p out, mask out, kept (u, v, w) out
+        "er_sample": m * (4 + 4 + 8) + n * k * 8 + 8 * m + m + kept * 16,
         "vector": None,
-    }
-
-
-def cpu_baseline(W, ordering_seed=0, columns=1):
-    """The oracle (CPU restatement of the reference path), one thread, on a
-    bounded sample: full sketch, `columns` of the k PCG columns (scaled to k),
-    full ER / probabilities / coins / sampling, natural-order factor."""
-    import numpy as np
-    import oracle
-    import spec_spars_amd as ss
-    from spec_spars_amd import graphs
-    n, u, v, w, k = W["n"], W["u"], W["v"], W["w"], W["k"]
-    m = len(u)
-    cp, ri, va = oracle.laplacian(n, u, v, w)
-    f = ss.Factor(cp, ri, va, seed=ordering_seed, ordering=ss.ORDER_NATURAL)
-    urp, uci, uva, dg = f.export()
-    t0 = time.perf_counter()
-    Y = oracle.sketch(u, v, w, n, k, seed=0, scale=1.0 / math.sqrt(k))
-    t_sketch = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    Zc, it, rr = oracle.solve_lap(Y[:, :columns], cp, ri, va, urp, uci, uva, dg, stop_rule=1, tol=1e-8,
-                                  max_iter=1000)
-    t_col = (time.perf_counter() - t0) / columns
-    # ER over all edges needs a full Z: use the sampled columns tiled to k
-    Z = np.tile(Zc, (1, (k + columns - 1) // columns))[:, :k]
-    uu, vv, lv = oracle.upper_edges(n, cp, ri, va)
-    t0 = time.perf_counter()
-    d = oracle.diff_norms(uu, vv, Z)
-    p = oracle.probs(lv, d, graphs.beta_for(n, W["eps"]), k)
-    c = oracle.coins(0, m)
-    r = oracle.sample_reweight(n, uu, vv, lv, p, c)
-    t_er = time.perf_counter() - t0
-    total = t_sketch + k * t_col + t_er
-    return {
-        "value": m / total, "unit": "edges/s", "cores": 1, "kind": "port",
-        "sample": ("oracle on the full workload graph: full sketch (%.1fs), %d of %d PCG columns "
-                   "(%.1fs each, %d iterations, natural-order factor, rel 1e-8) scaled to %d, full "
-                   "ER/probabilities/coins/sampling (%.1fs); one thread" %
-                   (t_sketch, columns, k, t_col, int(it.max()), k, t_er)),
-        "seconds_measured": t_sketch + columns * t_col + t_er,
-        "kept_sample": int(r["kept"]),
     }
 
 
@@ -175,10 +145,331 @@ def summarize(prof, kb, t_timed):
     return kernels, roofline
 
 
+def measured_traffic(workload, precision, kind):
+    """HBM bytes per launch of a kernel kind, from the committed counter
+    summary profiles/sparsify_kernels_<workload>_<precision>_pmc.md
+    (FETCH_SIZE and WRITE_SIZE per dispatch, calibrated on a copy kernel),
+    when one exists."""
+    path = os.path.join(ROOT, "profiles", "sparsify_kernels_%s_%s_pmc.md" % (workload, precision))
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch %s:" % kind):
+            return int(line.split(":")[1])
+    return None
+
+
+def host_info():
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(),
+            "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
+
+
+def cpu_threads():
+    """Host threads for the CPU legs: the process's CPU share, at most 16
+    (the GPU box's share for one GPU)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return max(1, min(16, n))
+
+
+def cpu_baseline(W, F, cp, ri, va, t_factor, pcg_iters, columns=None, er_edges=5_000_000, sketch_edges=2_000_000,
+                 pcg_iters_timed=2):
+    """The CPU restatement of the reference path (the oracle, plain C) on the
+    host cores, on a bounded sample of the same workload:
+      sketch   sketch_edges edges timed on one thread, scaled to m and
+               divided by the reference's ceil(m/n) sketch threads
+               (src/sparsifier.rs:509-529);
+      PCG      `columns` single-RHS solves (the product's factor in its
+               position space), one per thread, pcg_iters_timed iterations
+               each, scaled to pcg_iters iterations (the FP64 solve's count
+               to rel 1e-8 on the GPU) and to k columns over those threads
+               (the reference solves the k columns one after another,
+               src/example.cc:552-569);
+      ER, probabilities, coins, sampling: er_edges edges on one thread
+               (the reference's loops are sequential), scaled to m.
+    The factorization is the product's host factor (timed in the GPU leg)."""
+    import numpy as np
+    import oracle
+    from concurrent.futures import ThreadPoolExecutor
+    from spec_spars_amd import graphs
+    n, u, v, w, k = W["n"], W["u"], W["v"], W["w"], W["k"]
+    m = len(u)
+    T = cpu_threads()
+    columns = columns or T
+    # sketch sample
+    se = min(sketch_edges, m)
+    t0 = time.perf_counter()
+    oracle.sketch_edges(np.arange(se), u[:se], v[:se], w[:se], n, k, seed=0, scale=1.0 / math.sqrt(k))
+    t_sk1 = time.perf_counter() - t0
+    sk_threads = int(math.ceil(m / n))
+    t_sketch = t_sk1 * (m / se) / sk_threads
+    # PCG: columns single-RHS solves in parallel, position space
+    order, _ = F.order()
+    pc, pr, pv = oracle.permute_csc(cp, ri, va, order)
+    urp, uci, uva, dg = F.export()
+    rng = np.random.default_rng(0)
+    Y = rng.standard_normal((n, columns))
+    Y -= Y.mean(0)
+    Yp = np.ascontiguousarray(Y[order])
+
+    def one(j):
+        return oracle.solve_lap(Yp[:, j:j + 1], pc, pr, pv, urp, uci, uva, dg, stop_rule=1, tol=1e-30,
+                                max_iter=pcg_iters_timed)
+
+    log("cpu baseline: sketch sample %.1fs; %d PCG columns x %d iterations on %d threads ..." %
+        (t_sk1, columns, pcg_iters_timed, T))
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=T) as ex:
+        res = list(ex.map(one, range(columns)))
+    t_cols = time.perf_counter() - t0
+    # the solve takes pcg_iters iterations (the FP64 GPU solve's count on
+    # the same system): scale the timed iterations to it and to k columns
+    t_pcg = t_cols * (pcg_iters / pcg_iters_timed) * (k / columns)
+    iters = pcg_iters
+    # ER / probabilities / coins / sampling sample (canonical order: the
+    # first er_edges edges, their endpoints' rows of a solution block)
+    log("cpu baseline: PCG sample %.1fs; ER/sampling sample ..." % t_cols)
+    ee = min(er_edges, m)
+    Z = np.zeros((n, k))
+    zr = rng.standard_normal((n, 1)) * 1e-3
+    Z[:, :] = zr
+    uu, vv, lv = u[:ee], v[:ee], -1.0 * w[:ee]
+    t0 = time.perf_counter()
+    d = oracle.diff_norms(uu, vv, Z)
+    p = oracle.probs(lv, d, graphs.beta_for(n, W["eps"]), k)
+    c = oracle.coins(0, ee)
+    oracle.sample_reweight(n, uu, vv, lv, p, c)
+    t_er = (time.perf_counter() - t0) * (m / ee)
+    total = t_sketch + t_pcg + t_er
+    return {
+        "value": m / total, "unit": "edges/s", "cores": T, "kind": "port",
+        "value_end_to_end": m / (total + t_factor),
+        "sample": ("oracle (plain C restatement of the reference path) on the host: sketch %d of %d edges on one "
+                   "thread (%.2fs) scaled to m over the reference's ceil(m/n)=%d sketch threads; %d single-RHS "
+                   "FP64 PCG solves (this build's rounds-order factor) of %d iterations each on %d threads in %.1fs, "
+                   "scaled to %d iterations (the FP64 solve to rel 1e-8) and k=%d columns; "
+                   "ER/probabilities/coins/sampling on %d edges on one thread, scaled to m; factorization = this "
+                   "build's host factor (%.1fs, shared with the GPU path)" %
+                   (se, m, t_sk1, sk_threads, columns, pcg_iters_timed, T, t_cols, iters, k, ee, t_factor)),
+        "seconds": {"sketch": t_sketch, "pcg": t_pcg, "er_sample": t_er, "factor": t_factor},
+        "threads": {"pcg": T, "sketch": sk_threads, "factor": None},
+        "host": host_info(),
+    }
+
+
+def serve_pool(count, seed=5, n_max=2e6, m_max=20_000_000):
+    """Resident graphs for serving: the three families mixed, n log-uniform
+    in [1e4, n_max], m <= m_max."""
+    import numpy as np
+    from spec_spars_amd import graphs
+    rng = np.random.default_rng(seed)
+    pool, desc = [], []
+    for i in range(count):
+        n = int(round(10.0 ** rng.uniform(4.0, math.log10(n_max))))
+        fam = ("grid3d", "chunglu", "rgg")[i % 3]
+        if fam == "grid3d":
+            side = max(3, int(round(n ** (1.0 / 3.0))))
+            nn, u, v = graphs.grid3d(side)
+            w = graphs.loguniform_weights(len(u), seed=100 + i)
+        elif fam == "chunglu":
+            nn, u, v = graphs.chung_lu(n, min(8 * n, m_max), gamma=2.2, seed=100 + i)
+            w = graphs.uniform_weights(len(u), seed=100 + i)
+        else:
+            nn, u, v = graphs.rgg(n, seed=100 + i, mean_degree=8.0)
+            w = np.ones(len(u))
+        pool.append((nn, u, v, w))
+        desc.append("%s n=%d m=%d" % (fam, nn, len(u)))
+    return pool, desc
+
+
+def serve_window(rate, duration, pool_size, n_max, prec, slots=4, seed=11):
+    """Poisson arrivals over a resident pool; returns the server summary."""
+    from spec_spars_amd.serve import Registry, Server, poisson_requests
+    t0 = time.perf_counter()
+    pool, desc = serve_pool(pool_size, n_max=n_max)
+    reg = Registry(pool, threads=4)
+    t_build = time.perf_counter() - t0
+    log("serve pool (%.1fs): %s" % (t_build, "; ".join(desc)))
+    reqs = poisson_requests(rate=rate, duration=duration, graphs=len(pool), seed=seed, k=32, rhs=16)
+    srv = Server(reg, slots=slots, precision=prec, reserve_bytes=4 << 30, keep_arrays=False)
+    out = srv.run(reqs, deadline=duration + 30.0)
+    reg.close()
+    out.update({"rate": rate, "window_s": duration, "pool": desc, "registry_build_s": t_build, "slots": slots})
+    return out
+
+
+def step_loop(fn, warmup, steps, tag):
+    for i in range(warmup):
+        r = fn()
+        log("%s warmup %d: %.1f ms" % (tag, i, r[0]["ms_total"] if isinstance(r, list) else r["ms_total"]))
+    t0 = time.perf_counter()
+    last = None
+    for i in range(steps):
+        last = fn()
+        log("%s step %d: %.1f ms" % (tag, i, last[0]["ms_total"] if isinstance(last, list) else last["ms_total"]))
+    return (time.perf_counter() - t0) / steps, last
+
+
+def bench_single(W, args):
+    """One resident graph: plain FP64 leg, served mixed leg (merged requests),
+    column-wise sample, leverage check, serving window, CPU baseline."""
+    import numpy as np
+    import spec_spars_amd as ss
+    from spec_spars_amd import graphs
+    n, u, v, w, k = W["n"], W["u"], W["v"], W["w"], W["k"]
+    m = len(u)
+    eps = W["eps"]
+    beta = graphs.beta_for(n, eps)
+    log("workload %s n=%d m=%d k=%d" % (W["name"], n, m, k))
+    ctx = ss.Context(0)
+    G = None
+    try:
+        t0 = time.perf_counter()
+        G = ss.Graph(ctx, n, u, v, w)
+        cp, ri, va = G.laplacian()
+        t_graph = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        F = ss.Factor(cp, ri, va, seed=0, ordering=ss.ORDER_ROUNDS)
+        t_factor = time.perf_counter() - t0
+        finfo = F.info()
+        _, rounds = F.order()
+        t0 = time.perf_counter()
+        G.set_factor(F)
+        t_upload = time.perf_counter() - t0
+        log("graph %.1fs factor %.1fs (nnz %d, tail %d, rounds %d) upload %.1fs" %
+            (t_graph, t_factor, finfo["nnz"], F.tail(), rounds, t_upload))
+
+        # ---- plain: FP64, one request per pass
+        P64 = ss.PREC_FP64
+        tune64 = ctx.autotune(G, k, P64, reps=3) if not args.no_tune else ctx.tuning()
+        sv64 = ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-8, max_iter=2000, precision=P64)
+        t_plain, rep_plain = step_loop(
+            lambda: G.sparsify(k, beta, scale=1.0 / math.sqrt(k), sketch_seed=0, sampling_seed=0, solve=sv64),
+            args.warmup, args.steps, "plain fp64")
+        p64 = G.read_probs()
+        plain_eps = m / t_plain
+
+        # ---- plain, column by column (the reference's structure): a sample
+        # of single-RHS FP64 solves, scaled to k
+        rhs = G.read_block("Y")[:, :args.col_sample]
+        t0 = time.perf_counter()
+        for j in range(rhs.shape[1]):
+            _, info1 = G.solve(rhs[:, j:j + 1], sv64)
+        t_col = (time.perf_counter() - t0) / rhs.shape[1]
+        colwise_step = rep_plain["ms_sketch"] / 1e3 + k * t_col + rep_plain["ms_sample"] / 1e3
+        log("column-wise: %.1f ms per single-RHS solve (%d iterations)" % (t_col * 1e3, int(info1["iters"].max())))
+
+        # ---- served: R requests merged, mixed precision
+        PM = ss.PREC_MIXED if args.precision != "fp64" else ss.PREC_FP64
+        R = args.requests
+        eps_list = [eps, 0.2, 0.5, eps, 0.2, 0.5][:R]
+        reqs = [dict(k=k, beta=graphs.beta_for(n, e), sketch_seed=i, sampling_seed=i) for i, e in enumerate(eps_list)]
+        K = R * k
+        tuneM = ctx.autotune(G, K, PM, reps=3) if not args.no_tune else ctx.tuning()
+        svM = ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-8, max_iter=2000, precision=PM)
+        t_served, rep_served = step_loop(lambda: G.sparsify_many(reqs, solve=svM), args.warmup, args.steps,
+                                         "served x%d" % R)
+        value = R * m / t_served
+        prof, t_prof = {}, 1.0
+        if not args.no_profile:
+            G.profile(True)
+            t0 = time.perf_counter()
+            G.sparsify_many(reqs, solve=svM)
+            t_prof = time.perf_counter() - t0
+            prof = G.profile_read()
+            G.profile(False)
+        kept = sum(r["kept"] for r in rep_served)
+        kb = kernel_bytes(n, m, G, K, kept, s=8 if PM == ss.PREC_FP64 else 4)
+        kernels, roofline = summarize(prof, kb, t_prof)
+        # er_sample runs once per request: its bytes are per request's k
+        if "er_sample" in kernels:
+            ke = kernels["er_sample"]
+            ke["bytes_per_launch"] = kernel_bytes(n, m, G, k, kept // R, s=4)["er_sample"]
+            ke["GBps"] = ke["bytes_per_launch"] / (ke["avg_ms"] * 1e-3) / 1e9
+        if roofline:
+            roofline["traffic"] = measured_traffic(W["name"], "mixed" if PM == ss.PREC_MIXED else "fp64",
+                                                   roofline["kernel"])
+        copy_ms = ctx.bench_copy(1 << 30, reps=10)
+        copy_gbs = 2.0 * (1 << 30) / (copy_ms * 1e-3) / 1e9
+
+        # ---- leverage scores of the mixed solve vs the FP64 solve (same
+        # sketch: request 0's parameters are the plain leg's)
+        rep_m = G.sparsify(k, beta, scale=1.0 / math.sqrt(k), sketch_seed=0, sampling_seed=0, solve=svM)
+        pm = G.read_probs()
+        both = (pm < 1.0) & (p64 < 1.0)
+        lev_err = float(np.max(np.abs(pm[both] - p64[both]) / p64[both])) if both.any() else 0.0
+        log("leverage error mixed vs FP64: %.3e over %d edges with p < 1" % (lev_err, int(both.sum())))
+        sched = {"fwd": G.schedule_info(0), "bwd": G.schedule_info(1)}
+    finally:
+        if G is not None:
+            G.close()
+        ctx.close()
+
+    serve = None
+    if not args.no_serve:
+        log("serving window: %.0f req/s for %.0fs" % (args.rate, args.serve_seconds))
+        serve = serve_window(args.rate, args.serve_seconds, args.serve_pool, args.serve_nmax, PM)
+        log("served: p50 %.1f ms p99 %.1f ms, %d/%d completed" %
+            (serve["p50_ms"], serve["p99_ms"], serve["completed"], serve["requests"]))
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(W, F, cp, ri, va, t_factor, pcg_iters=rep_plain["max_iters"])
+        log("cpu baseline %.3g edges/s (%s)" % (cpu["value"], cpu["sample"]))
+
+    r0 = rep_served[0]
+    out = {
+        "metric": METRIC, "value": value, "unit": "edges/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": t_served * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64" if PM == ss.PREC_MIXED else "f64", "data": "synthetic",
+        "config": {"workload": W["desc"], "n": n, "m": m, "k": k, "epsilon": eps,
+                   "served": {"requests_per_pass": R, "epsilons": eps_list, "columns": K,
+                              "precision": "mixed" if PM == ss.PREC_MIXED else "fp64", "ms_per_pass": t_served * 1e3,
+                              "edges_per_s": value, "kept_edges": [r["kept"] for r in rep_served],
+                              "pcg_iters": r0["max_iters"], "refinements": r0["refinements"],
+                              "max_relres": max(r["max_relres"] for r in rep_served),
+                              "ms_sketch": r0["ms_sketch"], "ms_solve": r0["ms_solve"], "ms_sample": r0["ms_sample"],
+                              "rhs_solves_per_s": K / (r0["ms_solve"] / 1000.0), "tuning": tuneM},
+                   "plain": {"precision": "fp64", "ms_per_pass": t_plain * 1e3, "edges_per_s": plain_eps,
+                             "pcg_iters": rep_plain["max_iters"], "max_relres": rep_plain["max_relres"],
+                             "kept_edges": rep_plain["kept"], "ms_sketch": rep_plain["ms_sketch"],
+                             "ms_solve": rep_plain["ms_solve"], "ms_sample": rep_plain["ms_sample"],
+                             "rhs_solves_per_s": k / (rep_plain["ms_solve"] / 1000.0), "tuning": tune64},
+                   "plain_columnwise": {"precision": "fp64", "columns_timed": int(rhs.shape[1]),
+                                        "ms_per_column_solve": t_col * 1e3, "ms_per_pass_est": colwise_step * 1e3,
+                                        "edges_per_s": m / colwise_step},
+                   "served_vs_plain": value / plain_eps,
+                   "served_vs_plain_columnwise": value / (m / colwise_step),
+                   "leverage_vs_fp64": {"max_rel_err": lev_err, "edges_compared": int(both.sum()),
+                                        "mixed_iters": rep_m["max_iters"], "mixed_relres": rep_m["max_relres"]},
+                   "parity_unpinned": ["CityHash64WithSeed HashLen0to16 at len 16 (fasthash-sys version unknown)",
+                                       "rand 0.8.5 StdRng seed_from_u64 + gen_range f64 coin stream"],
+                   "ordering": "rounds", "factor_nnz": finfo["nnz"], "factor_tail_rows": F.tail(),
+                   "depth_fwd": finfo["depth_fwd"], "depth_bwd": finfo["depth_bwd"], "rounds": rounds,
+                   "host_factor_s": t_factor, "host_graph_s": t_graph, "upload_s": t_upload,
+                   "factor_threads": int(os.environ.get("SPEC_SPARS_THREADS", "0")) or min(os.cpu_count() or 1, 32),
+                   "edges_per_s_end_to_end": R * m / (t_served + t_factor + t_upload),
+                   "copy_kernel_GBps": copy_gbs, "parallelism": "1 GPU", "schedule": sched,
+                   "serving": serve},
+        "roofline": roofline,
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def cpu_baseline_batch(W, count=4):
     """The oracle on the first `count` graphs of the batch (every column,
     natural-order factors, one thread): edges/s of that bounded sample."""
-    import numpy as np
     import oracle
     import spec_spars_amd as ss
     from spec_spars_amd import graphs
@@ -200,7 +491,7 @@ def cpu_baseline_batch(W, count=4):
     return {"value": m_total / t_total, "unit": "edges/s", "cores": 1, "kind": "port",
             "sample": "oracle on the first %d graphs of the batch (%d edges, all %d columns, natural-order "
                       "factors, up to %d PCG iterations), one thread, %.1fs" % (count, m_total, k, iters, t_total),
-            "seconds_measured": t_total}
+            "host": host_info()}
 
 
 def bench_batch(W, args):
@@ -211,58 +502,52 @@ def bench_batch(W, args):
     gs, k = W["batch"], W["k"]
     n, m = W["n"], W["m"]
     prec = ss.PREC_FP64 if args.precision == "fp64" else ss.PREC_MIXED
-    ordering = {"natural": ss.ORDER_NATURAL, "random": ss.ORDER_RANDOM, "rounds": ss.ORDER_ROUNDS}[args.ordering]
     log("workload %s: %d graphs, n=%d m=%d k=%d" % (W["name"], len(gs), n, m, k))
     ctx = ss.Context(0)
-    t0 = time.perf_counter()
-    B = ss.Batch(ctx, gs)
-    t_graph = time.perf_counter() - t0
-    t0 = time.perf_counter()
-
-    def build(g):
-        return ss.Factor(*graphs.laplacian_csc(*g), seed=0, ordering=ordering)
-
-    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
-        fs = list(ex.map(build, gs))
-    t_factor = time.perf_counter() - t0
-    fnnz = sum(f.info()["nnz"] for f in fs)
-    t0 = time.perf_counter()
-    B.set_factors(fs)
-    t_upload = time.perf_counter() - t0
-    log("batch %.1fs factors %.1fs (nnz %d) upload %.1fs" % (t_graph, t_factor, fnnz, t_upload))
-    tune = ctx.tuning()
-    if not args.no_tune:
+    B = None
+    try:
         t0 = time.perf_counter()
-        tune = ctx.autotune(B, k, prec, reps=3)
-        log("autotune %.1fs: %s" % (time.perf_counter() - t0, tune))
-    betas = [graphs.beta_for(g[0], W["eps"]) for g in gs]
-    solve = ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-8, max_iter=2000, precision=prec)
-
-    def step():
-        return B.sparsify(k, betas, sketch_seeds=[0] * len(gs), sampling_seeds=[0] * len(gs), solve=solve)
-
-    for i in range(args.warmup):
-        reps = step()
-        log("warmup %d: %.1f ms (sketch %.1f solve %.1f sample %.1f) iters %d kept %d" %
-            (i, reps[0]["ms_total"], reps[0]["ms_sketch"], reps[0]["ms_solve"], reps[0]["ms_sample"],
-             max(r["max_iters"] for r in reps), sum(r["kept"] for r in reps)))
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        reps = step()
-        log("step %d: %.1f ms" % (i, reps[0]["ms_total"]))
-    t_timed = time.perf_counter() - t0
-    prof, t_prof = {}, 1.0
-    if not args.no_profile:
-        B.profile(True)
+        B = ss.Batch(ctx, gs)
+        t_graph = time.perf_counter() - t0
         t0 = time.perf_counter()
-        step()
-        t_prof = time.perf_counter() - t0
-        prof = B.profile_read()
-        B.profile(False)
-    ms_per_step = 1000.0 * t_timed / args.steps
-    kept = sum(r["kept"] for r in reps)
-    kb = kernel_bytes(n, m, B, k, kept, s=8 if prec == ss.PREC_FP64 else 4)
-    kernels, roofline = summarize(prof, kb, t_prof)
+        ss.set_factor_threads(1)   # one graph per thread
+
+        def build(g):
+            return ss.Factor(*graphs.laplacian_csc(*g), seed=0, ordering=ss.ORDER_ROUNDS)
+
+        with ThreadPoolExecutor(max_workers=cpu_threads()) as ex:
+            fs = list(ex.map(build, gs))
+        ss.set_factor_threads(0)
+        t_factor = time.perf_counter() - t0
+        fnnz = sum(f.info()["nnz"] for f in fs)
+        t0 = time.perf_counter()
+        B.set_factors(fs)
+        t_upload = time.perf_counter() - t0
+        log("batch %.1fs factors %.1fs (nnz %d) upload %.1fs" % (t_graph, t_factor, fnnz, t_upload))
+        tune = ctx.autotune(B, k, prec, reps=3) if not args.no_tune else ctx.tuning()
+        betas = [graphs.beta_for(g[0], W["eps"]) for g in gs]
+        solve = ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-8, max_iter=2000, precision=prec)
+        t_step, reps = step_loop(lambda: B.sparsify(k, betas, sketch_seeds=[0] * len(gs),
+                                                    sampling_seeds=[0] * len(gs), solve=solve),
+                                 args.warmup, args.steps, "batch")
+        prof, t_prof = {}, 1.0
+        if not args.no_profile:
+            B.profile(True)
+            t0 = time.perf_counter()
+            B.sparsify(k, betas, sketch_seeds=[0] * len(gs), sampling_seeds=[0] * len(gs), solve=solve)
+            t_prof = time.perf_counter() - t0
+            prof = B.profile_read()
+            B.profile(False)
+        kept = sum(r["kept"] for r in reps)
+        kb = kernel_bytes(n, m, B, k, kept, s=8 if prec == ss.PREC_FP64 else 4)
+        kernels, roofline = summarize(prof, kb, t_prof)
+        if roofline:
+            roofline["traffic"] = measured_traffic(W["name"], args.precision, roofline["kernel"])
+        sched = {"fwd": B.schedule_info(0), "bwd": B.schedule_info(1)}
+    finally:
+        if B is not None:
+            B.close()
+        ctx.close()
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -270,156 +555,108 @@ def bench_batch(W, args):
         log("cpu baseline %.3g edges/s (%s)" % (cpu["value"], cpu["sample"]))
     r0 = reps[0]
     out = {
-        "metric": METRIC, "value": m / (ms_per_step / 1000.0), "unit": "edges/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "metric": METRIC, "value": m / t_step, "unit": "edges/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_step * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64" if prec == ss.PREC_FP64 else "f32+f64",
         "data": "synthetic",
         "config": {"workload": W["desc"], "graphs": len(gs), "n": n, "m": m, "k": k, "rhs": k * len(gs),
-                   "epsilon": W["eps"], "precision": args.precision, "ordering": args.ordering,
+                   "epsilon": W["eps"], "precision": args.precision, "ordering": "rounds",
                    "factor_nnz": fnnz, "pcg_iters": max(r["max_iters"] for r in reps),
                    "refinements": r0["refinements"], "max_relres": max(r["max_relres"] for r in reps),
                    "kept_edges": kept, "ms_sketch": r0["ms_sketch"], "ms_solve": r0["ms_solve"],
                    "ms_sample": r0["ms_sample"], "rhs_solves_per_s": k * len(gs) / (r0["ms_solve"] / 1000.0),
                    "host_factor_s": t_factor, "host_graph_s": t_graph, "upload_s": t_upload,
-                   "parallelism": "1 GPU", "tuning": tune,
-                   "schedule_fwd": B.schedule_info(0), "schedule_bwd": B.schedule_info(1)},
+                   "parallelism": "1 GPU", "tuning": tune, "schedule": sched},
         "roofline": roofline,
         "kernels": kernels,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
-    B.close()
-    ctx.close()
     return 0
-
-
-def serve_pool(count, seed=5):
-    """Resident graphs for the serving run: the three families mixed, n
-    log-uniform in [1e4, 2e6], m <= 2e7."""
-    import numpy as np
-    from spec_spars_amd import graphs
-    rng = np.random.default_rng(seed)
-    pool, desc = [], []
-    for i in range(count):
-        n = int(round(10.0 ** rng.uniform(4.0, math.log10(2e6))))
-        fam = ("grid3d", "chunglu", "rgg")[i % 3]
-        if fam == "grid3d":
-            side = max(3, int(round(n ** (1.0 / 3.0))))
-            nn, u, v = graphs.grid3d(side)
-            w = graphs.loguniform_weights(len(u), seed=100 + i)
-        elif fam == "chunglu":
-            nn, u, v = graphs.chung_lu(n, min(8 * n, 20_000_000), gamma=2.2, seed=100 + i)
-            w = graphs.uniform_weights(len(u), seed=100 + i)
-        else:
-            nn, u, v = graphs.rgg(n, seed=100 + i, mean_degree=8.0)
-            w = np.ones(len(u))
-        pool.append((nn, u, v, w))
-        desc.append("%s n=%d m=%d" % (fam, nn, len(u)))
-    return pool, desc
 
 
 def bench_serve(args):
     """Poisson arrivals of sparsify / 16-RHS solve requests over resident
     graphs, continuous batching by slots, admission by free device memory."""
     import spec_spars_amd as ss
-    from spec_spars_amd.serve import Registry, Server, poisson_requests
     prec = ss.PREC_FP64 if args.precision == "fp64" else ss.PREC_MIXED
-    t0 = time.perf_counter()
-    pool, desc = serve_pool(args.pool)
-    log("pool: " + "; ".join(desc))
-    reg = Registry(pool, threads=min(16, os.cpu_count() or 1))
-    t_build = time.perf_counter() - t0
-    log("registry built in %.1fs (%.1f GB resident)" %
-        (t_build, sum(G.resident_bytes() for G in reg.G) / 1e9))
-    reqs = poisson_requests(rate=args.rate, duration=args.duration, graphs=len(pool), seed=11, k=32, rhs=16)
-    log("%d requests over %.0fs" % (len(reqs), args.duration))
-    srv = Server(reg, slots=args.slots, precision=prec, reserve_bytes=4 << 30, keep_arrays=True)
-    out = srv.run(reqs, deadline=args.duration + 30.0)
-    log("served: %s" % out)
+    out = serve_window(args.rate, args.duration, args.pool, 2e6, prec, slots=args.slots)
+    log("served: %s" % {k: v for k, v in out.items() if k != "pool"})
     line = {
         "metric": METRIC, "value": out["sparsified_edges_per_s"], "unit": "edges/s", "n_gpus": 1,
-        "steps": len(reqs), "warmup": 0, "ms_per_step": out["mean_ms"], "higher_is_better": True,
+        "steps": out["requests"], "warmup": 0, "ms_per_step": out["mean_ms"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64" if prec == ss.PREC_FP64 else "f32+f64",
         "data": "synthetic",
         "config": {"workload": "serving: Poisson %.0f req/s for %.0fs over %d resident graphs (grid3d, Chung-Lu, "
                                "RGG; n log-uniform [1e4, 2e6], m <= 2e7); half sparsify (k=32, eps in "
                                "{0.2, 0.3, 0.5}), half 16-RHS solves; %d slots" %
-                               (args.rate, args.duration, len(pool), args.slots),
-                   "pool": desc, "precision": args.precision, "registry_build_s": t_build, **out},
+                               (args.rate, args.duration, args.pool, args.slots),
+                   "precision": args.precision, **out},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
-    reg.close()
     return 0
 
 
 def probe(args):
-    """A short fixed program for hardware-counter runs: the workload's
-    graph and factor, one sparsify pass capped at 3 PCG iterations with the
-    shapes the autotuner picks on the 128^3 grid (16 waves per row tile, two
-    tiny rows per wave), then one 1 GiB FP64 copy for calibration."""
+    """A short fixed program for hardware-counter runs: the workload's graph
+    and factor, the served form's merged pass (or one plain pass) capped at
+    3 PCG iterations, then one 1 GiB FP64 copy for calibration."""
     import spec_spars_amd as ss
     from spec_spars_amd import graphs
     W = workload(args.workload)
     n, u, v, w, k = W["n"], W["u"], W["v"], W["w"], W["k"]
     ctx = ss.Context(0)
     G = ss.Graph(ctx, n, u, v, w)
-    cp, ri, va = G.laplacian()
-    G.set_factor(ss.Factor(cp, ri, va, seed=0, ordering=ss.ORDER_ROUNDS))
-    ctx.set_tuning(tile_waves=16, level_rows_per_wave=args.probe_rpw, graph_iters=0,
-                   xcd_slices=0 if args.no_xcd else 1)
-    prec = ss.PREC_FP64 if (args.precision or W.get("precision", "fp64")) == "fp64" else ss.PREC_MIXED
-    solve = ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-8, max_iter=3, precision=prec)
-    solve.fail_relres = 1e300
-    rep = G.sparsify(k, graphs.beta_for(n, W["eps"]), scale=1.0 / math.sqrt(k), solve=solve)
-    log("probe pass %.1f ms" % rep["ms_total"])
-    log("copy %.3f ms per GiB copy" % ctx.bench_copy(1 << 30, reps=1))
-    G.close()
-    ctx.close()
+    try:
+        cp, ri, va = G.laplacian()
+        G.set_factor(ss.Factor(cp, ri, va, seed=0, ordering=ss.ORDER_ROUNDS))
+        log("probe: factor uploaded")
+        prec = ss.PREC_FP64 if (args.precision or W["precision"]) == "fp64" else ss.PREC_MIXED
+        solve = ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-8, max_iter=3, precision=prec)
+        solve.fail_relres = 1e300
+        if args.requests > 1:
+            reqs = [dict(k=k, beta=graphs.beta_for(n, W["eps"]), sketch_seed=i, sampling_seed=i)
+                    for i in range(args.requests)]
+            rep = G.sparsify_many(reqs, solve=solve)[0]
+        else:
+            rep = G.sparsify(k, graphs.beta_for(n, W["eps"]), scale=1.0 / math.sqrt(k), solve=solve)
+        log("probe pass %.1f ms" % rep["ms_total"])
+        log("copy %.3f ms per GiB copy" % ctx.bench_copy(1 << 30, reps=1))
+    finally:
+        G.close()
+        ctx.close()
     return 0
-
-
-def measured_traffic(workload, precision, kind):
-    """HBM bytes per launch of a kernel kind from the committed counter
-    summary (profiles/), when one exists for this workload."""
-    path = os.path.join(ROOT, "profiles", "trisolve_%s_%s_pmc.md" % (workload, precision))
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        if line.startswith("traffic_bytes_per_launch %s:" % kind):
-            return int(line.split(":")[1])
-    return None
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="grid3d128")
+    ap.add_argument("--workload", default="chunglu5m",
+                    help="chunglu5m (default), grid3d128, grid3d64, grid2d64, rgg512 (batch), serve")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default=None,
-                    help="default: the workload's (fp64 for the grids, mixed for chunglu5m and rgg512)")
-    ap.add_argument("--ordering", choices=["natural", "random", "rounds"], default="rounds")
+                    help="served leg / batch precision (default: the workload's)")
+    ap.add_argument("--requests", type=int, default=4, help="requests merged per served pass")
+    ap.add_argument("--col-sample", type=int, default=2, help="single-RHS solves timed for the column-wise plain leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel event timing")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no autotuner)")
-    ap.add_argument("--probe", action="store_true",
-                    help="one short sparsify pass (3 PCG iterations) and a 1 GiB copy, for counter runs")
-    ap.add_argument("--no-xcd", action="store_true", help="probe: round-robin rows over the XCDs")
-    ap.add_argument("--probe-rpw", type=int, default=2, help="probe: tiny rows per wave of the level launches")
-    ap.add_argument("--rate", type=float, default=200.0, help="serve: Poisson arrivals per second")
-    ap.add_argument("--duration", type=float, default=60.0, help="serve: arrival window in seconds")
-    ap.add_argument("--pool", type=int, default=12, help="serve: resident graphs")
-    ap.add_argument("--slots", type=int, default=4, help="serve: requests on the device at once")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving window")
+    ap.add_argument("--serve-seconds", type=float, default=10.0, help="serving window length")
+    ap.add_argument("--serve-pool", type=int, default=6, help="graphs in the serving window's pool")
+    ap.add_argument("--serve-nmax", type=float, default=2e5, help="largest graph of the serving window's pool")
+    ap.add_argument("--probe", action="store_true", help="short pass for counter runs")
+    ap.add_argument("--rate", type=float, default=200.0, help="serving: Poisson arrivals per second")
+    ap.add_argument("--duration", type=float, default=60.0, help="serve workload: arrival window in seconds")
+    ap.add_argument("--pool", type=int, default=12, help="serve workload: resident graphs")
+    ap.add_argument("--slots", type=int, default=4, help="serve workload: requests on the device at once")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (the multi-GPU "
               "sharding design is documented, not built)" % args.gpus, file=sys.stderr)
         return 2
-
-    import numpy as np
-    import spec_spars_amd as ss
-    from spec_spars_amd import graphs
 
     if args.probe:
         return probe(args)
@@ -429,104 +666,10 @@ def main():
         return bench_serve(args)
     W = workload(args.workload)
     if args.precision is None:
-        args.precision = W.get("precision", "fp64")
+        args.precision = W["precision"]
     if "batch" in W:
         return bench_batch(W, args)
-    n, u, v, w, k = W["n"], W["u"], W["v"], W["w"], W["k"]
-    m = len(u)
-    prec = ss.PREC_FP64 if args.precision == "fp64" else ss.PREC_MIXED
-    ordering = {"natural": ss.ORDER_NATURAL, "random": ss.ORDER_RANDOM, "rounds": ss.ORDER_ROUNDS}[args.ordering]
-    log("workload %s n=%d m=%d k=%d" % (W["name"], n, m, k))
-
-    ctx = ss.Context(0)
-    t0 = time.perf_counter()
-    G = ss.Graph(ctx, n, u, v, w)
-    cp, ri, va = G.laplacian()
-    t_graph = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    F = ss.Factor(cp, ri, va, seed=0, ordering=ordering)
-    t_factor = time.perf_counter() - t0
-    finfo = F.info()
-    _, rounds = F.order()
-    t0 = time.perf_counter()
-    G.set_factor(F)
-    t_upload = time.perf_counter() - t0
-    log("graph %.1fs factor %.1fs (nnz %d, depth %d/%d) upload %.1fs" %
-        (t_graph, t_factor, finfo["nnz"], finfo["depth_fwd"], finfo["depth_bwd"], t_upload))
-    tune = ctx.tuning()
-    if not args.no_tune:
-        t0 = time.perf_counter()
-        tune = ctx.autotune(G, k, prec, reps=3)
-        log("autotune %.1fs: %s" % (time.perf_counter() - t0, tune))
-    beta = graphs.beta_for(n, W["eps"])
-    solve = ss.solve_opts(factor_seed=0, stop_rule=ss.STOP_REL, tol=1e-8, max_iter=2000, precision=prec,
-                          ordering=ordering)
-
-    def step():
-        return G.sparsify(k, beta, scale=1.0 / math.sqrt(k), sketch_seed=0, sampling_seed=0, solve=solve)
-
-    for i in range(args.warmup):
-        rep = step()
-        log("warmup %d: %.1f ms (sketch %.1f solve %.1f sample %.1f) iters %d relres %.2e kept %d" %
-            (i, rep["ms_total"], rep["ms_sketch"], rep["ms_solve"], rep["ms_sample"], rep["max_iters"],
-             rep["max_relres"], rep["kept"]))
-    reps = []
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        reps.append(step())
-        log("step %d: %.1f ms" % (i, reps[-1]["ms_total"]))
-    t_timed = time.perf_counter() - t0
-    # per-kernel times from one more step with HIP events around every
-    # launch group (direct launches; the timed steps replay graphs)
-    prof = {}
-    t_prof = 1.0
-    if not args.no_profile:
-        G.profile(True)
-        t0 = time.perf_counter()
-        step()
-        t_prof = time.perf_counter() - t0
-        prof = G.profile_read()
-        G.profile(False)
-    ms_per_step = 1000.0 * t_timed / args.steps
-    value = m / (ms_per_step / 1000.0)
-    last = reps[-1]
-
-    kb = kernel_bytes(n, m, G, k, last["kept"], s=8 if prec == ss.PREC_FP64 else 4)
-    kernels, roofline = summarize(prof, kb, t_prof)
-    if roofline:
-        roofline["traffic"] = measured_traffic(W["name"], args.precision, roofline["kernel"])
-    copy_ms = ctx.bench_copy(1 << 30, reps=10)
-    copy_gbs = 2.0 * (1 << 30) / (copy_ms * 1e-3) / 1e9
-
-    cpu = None
-    if not args.no_cpu:
-        log("cpu baseline ...")
-        cpu = cpu_baseline(W)
-        log("cpu baseline %.3g edges/s (%s)" % (cpu["value"], cpu["sample"]))
-
-    out = {
-        "metric": METRIC, "value": value, "unit": "edges/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64" if prec == ss.PREC_FP64 else "f32+f64",
-        "data": "synthetic",
-        "config": {"workload": W["desc"], "n": n, "m": m, "k": k, "epsilon": W["eps"],
-                   "precision": args.precision, "ordering": args.ordering, "factor_nnz": finfo["nnz"],
-                   "depth_fwd": finfo["depth_fwd"], "depth_bwd": finfo["depth_bwd"], "rounds": rounds,
-                   "pcg_iters": last["max_iters"], "refinements": last["refinements"],
-                   "max_relres": last["max_relres"], "kept_edges": last["kept"],
-                   "ms_sketch": last["ms_sketch"], "ms_solve": last["ms_solve"], "ms_sample": last["ms_sample"],
-                   "rhs_solves_per_s": k / (last["ms_solve"] / 1000.0),
-                   "host_factor_s": t_factor, "host_graph_s": t_graph, "upload_s": t_upload,
-                   "copy_kernel_GBps": copy_gbs, "parallelism": "1 GPU", "tuning": tune,
-                   "schedule_fwd": G.schedule_info(0), "schedule_bwd": G.schedule_info(1)},
-        "roofline": roofline,
-        "kernels": kernels,
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(out), flush=True)
-    G.close()
-    ctx.close()
-    return 0
+    return bench_single(W, args)
 
 
 if __name__ == "__main__":

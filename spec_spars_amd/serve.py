@@ -155,10 +155,22 @@ class Registry:
         """Run requests that share a graph (blocking).  Solve requests run as
         one multi-RHS solve over their concatenated right-hand sides (every
         column keeps its own PCG scalars, mask and reductions, so its result
-        is the one it gets alone); returns one result per request."""
-        if len(group) == 1 or any(r.op != "solve" for r in group):
+        is the one it gets alone); sparsify requests as one merged pass (their
+        sketch columns side by side, one solve, a sampling pass each, each
+        result bit-identical to the request alone).  Returns one result per
+        request."""
+        if len(group) == 1 or len({r.op for r in group}) > 1:
             return [self.execute(r, precision, tol, keep_arrays) for r in group]
         G = self.G[group[0].graph]
+        if group[0].op == "sparsify":
+            from . import graphs as gen
+            n = self.specs[group[0].graph][0]
+            opts = self.ss.solve_opts(stop_rule=self.ss.STOP_REL, tol=tol, max_iter=2000, precision=precision)
+            reps = G.sparsify_many([dict(k=r.k, beta=gen.beta_for(n, r.eps), sketch_seed=r.seed,
+                                         sampling_seed=r.seed) for r in group], solve=opts)
+            return [{"kept": rep["kept"], "edges": G.m, "arrays": G.read_kept_many(i) if keep_arrays else None,
+                     "iters": rep["max_iters"], "relres": rep["max_relres"], "group": len(group)}
+                    for i, rep in enumerate(reps)]
         b = np.concatenate([self.rhs(r.graph, r.k, r.seed) for r in group], axis=1)
         opts = self.ss.solve_opts(stop_rule=self.ss.STOP_REL, tol=tol, max_iter=2000, precision=precision)
         Z, info = G.solve(b, opts)
@@ -189,26 +201,29 @@ class Registry:
 
 class Server:
     """Slot dispatcher over a Registry (or any executor with free_bytes,
-    need and execute_group).  A started solve request takes the queued solve
-    requests of the same graph with it, up to max_rhs right-hand sides."""
+    need and execute_group).  A started request takes the queued requests of
+    the same kind on the same graph with it: solves up to max_rhs
+    right-hand sides, sparsifies up to max_cols sketch columns."""
 
-    def __init__(self, registry, slots=4, precision=None, reserve_bytes=1 << 30, keep_arrays=True, max_rhs=64):
+    def __init__(self, registry, slots=4, precision=None, reserve_bytes=1 << 30, keep_arrays=True, max_rhs=64,
+                 max_cols=256):
         self.reg = registry
         self.slots = slots
         self.precision = precision
         self.keep_arrays = keep_arrays
         self.max_rhs = max_rhs
+        self.max_cols = max_cols
         self.sched = Scheduler(slots, reserve_bytes)
 
     def _group(self, r, queue):
-        """r plus the queued solve requests on its graph that fit (removed
-        from the queue)."""
+        """r plus the queued requests of the same kind on its graph that fit
+        (removed from the queue): solves up to max_rhs right-hand sides,
+        sparsifies up to max_cols sketch columns."""
         group = [r]
-        if r.op != "solve":
-            return group
+        cap = self.max_rhs if r.op == "solve" else self.max_cols
         total, keep = r.k, []
         for q in queue:
-            if q.op == "solve" and q.graph == r.graph and total + q.k <= self.max_rhs:
+            if q.op == r.op and q.graph == r.graph and total + q.k <= cap:
                 group.append(q)
                 total += q.k
             else:

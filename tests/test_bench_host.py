@@ -26,7 +26,8 @@ def test_unknown_workload_is_refused():
 
 
 def test_traffic_figures_are_read_from_the_committed_counter_summary():
-    """The roofline's traffic comes from profiles/, one line per kernel kind."""
+    """The roofline's traffic comes from the committed counter summary under
+    profiles/, one line per kernel kind."""
     sys.path.insert(0, ROOT)
     import bench
     t_fwd = bench.measured_traffic("grid3d128", "fp64", "tri_fwd")

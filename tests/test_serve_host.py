@@ -91,10 +91,31 @@ def test_server_deadline_drops_and_failures_are_contained():
             return super().execute(r, precision, keep_arrays)
 
     reg = Failing([0.05])
-    reqs = [Request(i, 0, "sparsify", 0.0, k=4) for i in range(40)]
+    # requests of 64 columns: four go in one merged pass (256 columns); the
+    # failing request fails its whole pass, the server goes on
+    reqs = [Request(i, 0, "sparsify", 0.0, k=64) for i in range(40)]
     out = Server(reg, slots=2, reserve_bytes=0).run(reqs, deadline=0.2)
-    assert out["failed"] == 1
-    assert out["dropped"] > 0 and out["completed"] + out["failed"] + out["dropped"] == 40
+    assert out["failed"] == 4
+    assert [r.rid for r in reqs if r.error] == [0, 1, 2, 3]
+    assert out["completed"] > 0 and out["dropped"] > 0
+    assert out["completed"] + out["failed"] + out["dropped"] == 40
+
+
+def test_server_merges_sparsifies_on_one_graph():
+    class Rec(_FakeRegistry):
+        groups = []
+
+        def execute_group(self, group, precision, keep_arrays=True):
+            Rec.groups.append([(r.rid, r.op) for r in group])
+            return super().execute_group(group, precision, keep_arrays)
+
+    reg = Rec([0.05])
+    reqs = [Request(0, 0, "solve", 0.0, k=16)] + [Request(i, 0, "sparsify", 0.001 * i, k=64) for i in range(1, 7)]
+    out = Server(reg, slots=1, reserve_bytes=0, max_cols=256).run(reqs)
+    assert out["completed"] == 7
+    # while the solve runs the sparsifies queue, then go four (256 columns) and two at a time
+    assert [len(g) for g in Rec.groups] == [1, 4, 2]
+    assert all(op == "sparsify" for g in Rec.groups[1:] for _, op in g)
 
 
 def test_poisson_requests_shape():
