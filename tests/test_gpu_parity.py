@@ -18,7 +18,7 @@ def _graph(kind, seed=0):
         n, u, v = graphs.grid3d(10)
         w = graphs.loguniform_weights(len(u), seed=seed)
     elif kind == "rgg":
-        n, u, v = graphs.rgg(900, seed=seed)
+        n, u, v = graphs.rgg(900, seed=seed, mean_degree=10.0)   # connected for seed 0
         w = np.ones(len(u))
     else:
         n, u, v = graphs.chung_lu(1500, 6000, seed=seed)
