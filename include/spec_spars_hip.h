@@ -130,6 +130,8 @@ typedef struct ss_solve_opts {
                                1e-2 in the reference (custom_cg.hpp:254)     */
     int32_t precision;      /* SS_PREC_FP64 or SS_PREC_MIXED                 */
     int32_t ordering;       /* SS_ORDER_* of the factor ss_solve_lap builds  */
+    double inner_tol;       /* SS_PREC_MIXED: relative residual reduction each
+                               FP32 PCG sweep asks for (0 = 1e-3)            */
 } ss_solve_opts;
 
 typedef struct ss_solve_report {

@@ -329,7 +329,7 @@ static float elapsed(hipEvent_t a, hipEvent_t b) {
 
 static bool same_solve(const ss_solve_opts &a, const ss_solve_opts &b) {
     return a.stop_rule == b.stop_rule && a.max_iter == b.max_iter && a.tol == b.tol && a.precision == b.precision &&
-           a.fail_relres == b.fail_relres;
+           a.fail_relres == b.fail_relres && a.inner_tol == b.inner_tol;
 }
 
 // Sketch -> solve -> sample over the whole batch (opts has G entries).

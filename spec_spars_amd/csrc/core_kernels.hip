@@ -156,25 +156,54 @@ __global__ __launch_bounds__(256) void k_core_gather(int64_t rows, const int32_t
     }
 }
 
+// One 16 x 16 x 4 matrix-core step: acc += A (16 x 4) B (4 x 16), lane l
+// holding A[l % 16][l / 16] and B[l / 16][l % 16]; acc[i] is the block's
+// column l % 16, row acc_row<T>(l / 16, i): 4 (l / 16) + i in FP32,
+// (l / 16) + 4 i in FP64 (the two instructions deal their outputs out
+// differently; checked on the device).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ f32x4 mfma16(float a, float b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f64x4 mfma16(double a, double b, f64x4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+template <typename T> __device__ __forceinline__ int acc_row(int lk, int i);
+template <> __device__ __forceinline__ int acc_row<float>(int lk, int i) { return 4 * lk + i; }
+template <> __device__ __forceinline__ int acc_row<double>(int lk, int i) { return lk + 4 * i; }
+template <typename T> struct Acc4;
+template <> struct Acc4<float> { typedef f32x4 type; };
+template <> struct Acc4<double> { typedef f64x4 type; };
+
 // One piece of a core product: rows [r0, r0 + kCT) of graph g, inner range
 // [m0, m1) -> partial tile part[piece] (kCT x k).  Forward A = M, backward
-// A = M^T.  256 threads, 4 x 4 outputs each, per 64-column chunk of k.
+// A = M^T.  Four waves per 64 x 64 output tile (a 64-column chunk of k),
+// each a 32 x 32 quarter as 2 x 2 blocks of 16 x 16 on the matrix cores;
+// the inner dimension is staged in LDS kCK rows at a time.
 template <typename T, bool BWD>
 __global__ __launch_bounds__(256) void k_core_apply(const CorePiece *__restrict__ pieces,
                                                     const int32_t *__restrict__ cpad,
                                                     const int64_t *__restrict__ dense_off,
                                                     const int64_t *__restrict__ tc_off, const T *__restrict__ M,
                                                     int k, const T *__restrict__ tc, T *__restrict__ part) {
-    __shared__ T as[kCK][kCT + 4];
-    __shared__ T bs[kCK][kCT + 4];
+    typedef typename Acc4<T>::type AccT;
+    __shared__ T as[kCK][kCT + 4];   // as[m][r]: A(r, m)
+    __shared__ T bs[kCK][kCT + 4];   // bs[m][c]: Tc(m, c)
     const CorePiece pc = pieces[blockIdx.x];
     const int cp = cpad[pc.graph];
     const T *Mg = M + dense_off[pc.graph];
     const T *tg = tc + (size_t)tc_off[pc.graph] * k;
-    const int t = threadIdx.x, tr = t >> 4, tc4 = (t & 15) * 4;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;   // this wave's quarter
+    const int li = lane & 15, lk = lane >> 4;
     T *po = part + (size_t)blockIdx.x * kCT * k;
     for (int jb = 0; jb < k; jb += kCT) {
-        T acc[4][4] = {};
+        AccT acc[2][2];
+#pragma unroll
+        for (int x = 0; x < 2; x++)
+#pragma unroll
+            for (int y = 0; y < 2; y++) acc[x][y] = AccT{0, 0, 0, 0};
         for (int m0 = pc.m0; m0 < pc.m1; m0 += kCK) {
             if (BWD) {
                 for (int e = t; e < kCK * kCT; e += 256) {
@@ -192,25 +221,30 @@ __global__ __launch_bounds__(256) void k_core_apply(const CorePiece *__restrict_
                 bs[mm][c] = jb + c < k ? tg[(size_t)(m0 + mm) * k + jb + c] : (T)0;
             }
             __syncthreads();
-#pragma unroll 8
-            for (int mm = 0; mm < kCK; mm++) {
-                T a[4], b[4];
 #pragma unroll
-                for (int u = 0; u < 4; u++) a[u] = as[mm][tr * 4 + u];
+            for (int m4 = 0; m4 < kCK; m4 += 4) {
+                T a[2], b[2];
 #pragma unroll
-                for (int v = 0; v < 4; v++) b[v] = bs[mm][tc4 + v];
+                for (int x = 0; x < 2; x++) a[x] = as[m4 + lk][wr + 16 * x + li];
 #pragma unroll
-                for (int u = 0; u < 4; u++)
+                for (int y = 0; y < 2; y++) b[y] = bs[m4 + lk][wc + 16 * y + li];
 #pragma unroll
-                    for (int v = 0; v < 4; v++) acc[u][v] = __builtin_fma(a[u], b[v], acc[u][v]);
+                for (int x = 0; x < 2; x++)
+#pragma unroll
+                    for (int y = 0; y < 2; y++) acc[x][y] = mfma16(a[x], b[y], acc[x][y]);
             }
             __syncthreads();
         }
 #pragma unroll
-        for (int u = 0; u < 4; u++)
+        for (int x = 0; x < 2; x++)
 #pragma unroll
-            for (int v = 0; v < 4; v++)
-                if (jb + tc4 + v < k) po[(size_t)(tr * 4 + u) * k + jb + tc4 + v] = acc[u][v];
+            for (int y = 0; y < 2; y++) {
+                const int c = jb + wc + 16 * y + li;
+                if (c < k)
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+                        po[(size_t)(wr + 16 * x + acc_row<T>(lk, i)) * k + c] = acc[x][y][i];
+            }
     }
 }
 

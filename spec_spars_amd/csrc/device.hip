@@ -1068,7 +1068,8 @@ int graph_solve(ss_graph *g, int k, const ss_solve_opts &o, SolveStats &st) {
         SS_TRY(block_norms2(g, k, g->B64, bn2));
         // Each FP32 sweep cuts the FP64 residual by ~1e-3 (well above FP32's
         // stagnation level); sweeps repeat until the FP64 stop rule holds.
-        const double inner_tol = std::max(1e-3, o.stop_rule == SS_STOP_REL ? o.tol : 0.0);
+        if (!(o.inner_tol >= 0.0 && o.inner_tol < 1.0)) return fail(SS_ERR_INVALID, "inner_tol %g outside [0, 1)", o.inner_tol);
+        const double inner_tol = std::max(o.inner_tol > 0.0 ? o.inner_tol : 1e-3, o.stop_rule == SS_STOP_REL ? o.tol : 0.0);
         const int inner_max = std::min(o.max_iter, 300);
         std::vector<int32_t> it_sweep, keep(gk);
         rn2 = bn2;

@@ -79,7 +79,7 @@ def test_solve_fp64_matches_oracle(ctx, oracle_mod, kind):
     Y = ctx.sketch(u, v, w, n, k, seed=2, scale=1.0 / math.sqrt(k))
     cp, ri, va, Zo, it = _oracle_solution(oracle_mod, n, u, v, w, Y, 3, 1e-12)
     Z, info = ctx.solve_lap(Y, cp, ri, va, ss.solve_opts(factor_seed=3, stop_rule=ss.STOP_REL, tol=1e-10,
-                                                         max_iter=2000))
+                                                         max_iter=2000, ordering=ss.ORDER_NATURAL))
     assert info["max_relres"] < 1e-9
     assert _rel_centered(Z, Zo).max() < 1e-7
     # same factor, same method, same stop rule: iteration counts agree to within a couple
