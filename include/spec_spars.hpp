@@ -32,7 +32,7 @@ struct SparsifierParameters {
     // solve settings (the reference hard-codes these in C++:
     // include/custom_cg.hpp:150-159, src/example.cc:99-100)
     uint64_t factor_seed = 0;
-    int32_t ordering = SS_ORDER_NATURAL;
+    int32_t ordering = SS_ORDER_ROUNDS;   // shallow solves (the natural order is a chain)
     int32_t precision = SS_PREC_FP64;
     int32_t stop_rule = SS_STOP_REF_ABS;
     double tol = 2e-7;
@@ -121,6 +121,33 @@ private:
     int reweight_from_mask(const std::vector<int32_t> &u, const std::vector<int32_t> &v,
                            const std::vector<double> &lval, const std::vector<double> &p,
                            const std::vector<uint8_t> &mask, Triplet &reweightings) const;
+};
+
+// A MatrixMarket matrix as CSC (rows sorted in each column).
+struct MtxMatrix {
+    int32_t rows = 0, cols = 0;
+    std::vector<int32_t> colptr{0};
+    std::vector<int32_t> rowidx;
+    std::vector<double> vals;
+};
+// read_mtx (src/utils.rs:149-154): coordinate real / integer / pattern,
+// general / symmetric (mirrored) / skew-symmetric; duplicates summed.
+int read_mtx(const std::string &path, MtxMatrix &out);
+// write_mtx (src/utils.rs:189-191): coordinate real general, storage order.
+int write_mtx(const std::string &path, int32_t rows, int32_t cols, const int32_t *colptr, const int32_t *rowidx,
+              const double *vals);
+
+// src/stream.rs:8-64
+struct InputStream {
+    CscMatrix input_matrix;       // the file, diagonal zeroed
+    int32_t num_nodes = 0;
+    std::string dataset_name;
+    int32_t diag_zeros = 0;       // diagonal positions the file does not store
+    // InputStream::new (src/stream.rs:18-37)
+    int open(const std::string &filename, const std::string &name);
+    // run_stream (src/stream.rs:41-64) into a sparsifier built for
+    // num_nodes vertices; writes s.params.output_file when writeout and not test
+    int run_stream(Sparsifier &s, bool test, bool writeout, SparsificationStats *stats) const;
 };
 
 }  // namespace spec_spars

@@ -65,6 +65,34 @@ int ssh_sparsifier_info(const ssh_sparsifier *s, double *beta, int32_t *jl_dim, 
 /* jl_sketch_colwise_flat of new_entries (src/sparsifier.rs:532-539), n x jl_dim row-major */
 int ssh_sparsifier_sketch(const ssh_sparsifier *s, double *Y);
 
+/* ---- MatrixMarket files and the stream driver (src/utils.rs:149-191,
+ * src/stream.rs:18-64) ---------------------------------------------------- */
+typedef struct ssh_matrix ssh_matrix;
+typedef struct ssh_stream ssh_stream;
+/* read_mtx: a coordinate file (real / integer / pattern values; general,
+ * symmetric or skew-symmetric storage, mirrored) as CSC, rows sorted in each
+ * column, duplicates summed, explicit zeros kept. */
+int ssh_mtx_read(const char *path, ssh_matrix **out);
+int ssh_matrix_info(const ssh_matrix *m, int32_t *rows, int32_t *cols, int64_t *nnz);
+/* colptr (cols + 1), rowidx and vals (nnz); NULL arrays are skipped */
+int ssh_matrix_csc(const ssh_matrix *m, int32_t *colptr, int32_t *rowidx, double *vals);
+int ssh_matrix_destroy(ssh_matrix *m);
+/* write_mtx: coordinate real general, entries in CSC storage order */
+int ssh_mtx_write(const char *path, int32_t rows, int32_t cols, const int32_t *colptr,
+                  const int32_t *rowidx, const double *vals);
+/* InputStream::new: read the file (square), zero its stored diagonal */
+int ssh_stream_open(const char *path, const char *dataset_name, ssh_stream **out);
+/* vertices, stored entries, diagonal positions absent from the file */
+int ssh_stream_info(const ssh_stream *s, int32_t *num_nodes, int64_t *stored, int32_t *diag_zeros);
+/* run_stream: a sparsifier of num_nodes vertices with parameters p gets
+ * every stored (row, col, value), row < col, in CSC order, then sparsify;
+ * with writeout (and not test) the sparsified Laplacian goes to
+ * output_file.  *out receives the sparsifier (destroy it with
+ * ssh_sparsifier_destroy); before / after are SparsificationStats. */
+int ssh_stream_run(const ssh_stream *s, ss_context *ctx, const ssh_params *p, int test, int writeout,
+                   const char *output_file, ssh_sparsifier **out, int64_t *before, int64_t *after);
+int ssh_stream_destroy(ssh_stream *s);
+
 #ifdef __cplusplus
 }
 #endif

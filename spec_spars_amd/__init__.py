@@ -13,4 +13,4 @@ _lib.load()
 
 from .api import Batch, Context, Factor, Graph, set_factor_threads, solve_opts  # noqa: E402,F401
 from . import graphs  # noqa: E402,F401
-from .sparsifier import SparsifierParameters, Sparsifier  # noqa: E402,F401
+from .sparsifier import InputStream, SparsifierParameters, Sparsifier, read_mtx, write_mtx  # noqa: E402,F401

@@ -179,6 +179,16 @@ SIGNATURES = {
     "ssh_sparsifier_laplacian": (ctypes.c_int, [c_vp, c_i64p, c_i32p, c_i32p, c_f64p]),
     "ssh_sparsifier_info": (ctypes.c_int, [c_vp, c_f64p, c_i32p, c_i64p]),
     "ssh_sparsifier_sketch": (ctypes.c_int, [c_vp, c_f64p]),
+    "ssh_mtx_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(c_vp)]),
+    "ssh_matrix_info": (ctypes.c_int, [c_vp, c_i32p, c_i32p, c_i64p]),
+    "ssh_matrix_csc": (ctypes.c_int, [c_vp, c_i32p, c_i32p, c_f64p]),
+    "ssh_matrix_destroy": (ctypes.c_int, [c_vp]),
+    "ssh_mtx_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32, c_i32p, c_i32p, c_f64p]),
+    "ssh_stream_open": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(c_vp)]),
+    "ssh_stream_info": (ctypes.c_int, [c_vp, c_i32p, c_i64p, c_i32p]),
+    "ssh_stream_run": (ctypes.c_int, [c_vp, c_vp, ctypes.POINTER(HostParams), ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_char_p, ctypes.POINTER(c_vp), c_i64p, c_i64p]),
+    "ssh_stream_destroy": (ctypes.c_int, [c_vp]),
 }
 
 _lib = None

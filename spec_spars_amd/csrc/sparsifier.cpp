@@ -333,9 +333,7 @@ using spec_spars::Sparsifier;
 using spec_spars::SparsifierParameters;
 using ss::fail;
 
-struct ssh_sparsifier {
-    Sparsifier sp;
-};
+#include "host_handles.h"
 
 static SparsifierParameters to_params(const ssh_params *p) {
     SparsifierParameters q = SparsifierParameters::new_default(false);
@@ -497,3 +495,25 @@ int ssh_sparsifier_sketch(const ssh_sparsifier *s, double *Y) {
 }
 
 }  // extern "C"
+
+extern "C" int ssh_stream_run(const ssh_stream *st, ss_context *ctx, const ssh_params *p, int test, int writeout,
+                              const char *output_file, ssh_sparsifier **out, int64_t *before, int64_t *after) {
+    if (!st || !out) return fail(SS_ERR_INVALID, "null argument");
+    try {
+        SparsifierParameters q = to_params(p);
+        if (output_file) q.output_file = output_file;
+        ssh_sparsifier *s = new ssh_sparsifier{Sparsifier(ctx, st->is.num_nodes, q)};
+        spec_spars::SparsificationStats stats;
+        const int rc = st->is.run_stream(s->sp, test != 0, writeout != 0, &stats);
+        if (rc != SS_OK) {
+            delete s;
+            return rc;
+        }
+        if (before) *before = (int64_t)stats.original_num_edges;
+        if (after) *after = (int64_t)stats.final_num_edges;
+        *out = s;
+        return SS_OK;
+    } catch (const std::bad_alloc &) {
+        return fail(SS_ERR_NOMEM, "host allocation failed");
+    }
+}

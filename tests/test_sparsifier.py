@@ -132,3 +132,44 @@ def test_check_for_additions_and_valid_output(ctx, ordering):
     a, b = _graph_edges(cp, ri, n)
     ge = set(zip(u.tolist(), v.tolist()))
     assert all((x, y) in ge for x, y in zip(a.tolist(), b.tolist()))
+
+
+@pytest.mark.gpu
+def test_input_stream_runs_like_inserting_by_hand(ctx, tmp_path):
+    """InputStream::run_stream (src/stream.rs:41-64) on a MatrixMarket file
+    written here -- a weighted grid stored as its lower triangle with
+    diagonal entries and some weights split into two duplicate entries --
+    inserts the same edges in the same order as inserting the graph's
+    canonical edges by hand, so the sparsified Laplacians and the stats are
+    identical; writeout writes that Laplacian, which reads back exactly."""
+    import spec_spars_amd as ss
+    from spec_spars_amd import graphs
+    n, u, v = graphs.grid2d(20, 15)
+    w = graphs.loguniform_weights(len(u), seed=9, lo=0.1, hi=10.0)
+    rng = np.random.default_rng(4)
+    split = rng.random(len(u)) < 0.3
+    lines = []
+    for a, b, x, s in zip(u, v, w, split):
+        if s:
+            lines += ["%d %d %.17g" % (b + 1, a + 1, x / 2.0)] * 2       # halves sum exactly
+        else:
+            lines.append("%d %d %.17g" % (b + 1, a + 1, x))
+    lines += ["%d %d 5.0" % (i + 1, i + 1) for i in range(0, n, 7)]
+    path = tmp_path / "grid.mtx"
+    path.write_text("%%%%MatrixMarket matrix coordinate real symmetric\n%% written by the test\n%d %d %d\n%s\n" %
+                    (n, n, len(lines), "\n".join(lines)))
+    params = _params(epsilon=0.5, jl_factor=2.0, sketch_uniform=0, sketch_seed=3, sampling_seed=4)
+    stream = ss.InputStream(path, "grid")
+    assert stream.num_nodes == n
+    out = tmp_path / "h.mtx"
+    s1, st1 = stream.run_stream(params, ctx=ctx, writeout=True, output_file=str(out))
+    s2 = ss.Sparsifier(n, params, ctx)
+    s2.insert_many(u, v, w)
+    st2 = s2.sparsify()
+    assert st1 == st2 and st1[1] < st1[0]
+    L1, L2 = s1.current_laplacian(), s2.current_laplacian()
+    assert all(np.array_equal(a, b) for a, b in zip(L1, L2))
+    rows, cols, cp, ri, va = ss.read_mtx(out)
+    assert rows == cols == n
+    assert np.array_equal(cp, L1[0]) and np.array_equal(ri, L1[1]) and np.array_equal(va, L1[2])
+    s1.check_diagonal()
