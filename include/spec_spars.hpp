@@ -37,6 +37,12 @@ struct SparsifierParameters {
     int32_t stop_rule = SS_STOP_REF_ABS;
     double tol = 2e-7;
     int32_t max_iter = 400;
+    // Device-resident pass: the Laplacian goes to the device once per pass
+    // and the solution, probabilities, coins, sample and reweighted
+    // Laplacian are computed there (only the kept edges and the new diagonal
+    // come back, for the next host factor).  Results are the same bits as
+    // the host-driven pass.
+    bool resident = false;
 
     // SparsifierParameters::new_default (src/sparsifier.rs:182-209)
     static SparsifierParameters new_default(bool bench);
@@ -96,6 +102,9 @@ public:
     int apply_reweightings(Triplet &reweightings, bool report, SparsificationStats *stats);   // :682-691
     // The whole pass (:722-770).
     int sparsify(bool check, SparsificationStats *stats);
+    // Passes the streaming trigger ran (insert: num_edges() > threshold,
+    // :370-374).
+    int64_t triggered = 0;
     // JL sketch of the incidence matrix of new_entries (:490-539), row-major n x jl_dim.
     int jl_sketch(std::vector<double> &Y) const;
 
@@ -118,6 +127,8 @@ public:
 private:
     // canonical upper-triangle edges of current_laplacian (:547-548)
     void upper_edges(std::vector<int32_t> &u, std::vector<int32_t> &v, std::vector<double> &lval) const;
+    // the pass after the sketch and form_laplacian, on the device
+    int sparsify_resident(const std::vector<double> &Y, SparsificationStats *stats);
     int reweight_from_mask(const std::vector<int32_t> &u, const std::vector<int32_t> &v,
                            const std::vector<double> &lval, const std::vector<double> &p,
                            const std::vector<uint8_t> &mask, Triplet &reweightings) const;

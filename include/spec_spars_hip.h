@@ -184,6 +184,11 @@ int ss_graph_destroy(ss_graph *g);
 /* Device bytes the graph would need for k columns at a precision. */
 int ss_graph_bytes(int64_t n, int64_t m, int64_t factor_nnz, int32_t k,
                    int32_t precision, size_t *bytes);
+/* A device graph from a CSC Laplacian with its diagonal (as ss_solve_lap
+ * reads it): the edges are its strictly upper entries (weight -L_uv, canonical
+ * order), the diagonal is kept as given. */
+int ss_graph_create_csc(ss_context *ctx, int64_t n, const int32_t *colptr, const int32_t *rowidx,
+                        const double *vals, ss_graph **out);
 int ss_graph_set_factor(ss_graph *g, const ss_factor *f);
 /* Device bytes the graph holds now (structure, factor, work blocks), and
  * the bytes a run with k columns at a precision would add (0 when the work
@@ -243,6 +248,17 @@ int ss_graph_read_kept_many(const ss_graph *g, int32_t r, int64_t cap, int32_t *
  * forward result L^{-1} r and its output L^{-T} D^+ L^{-1} r. */
 int ss_graph_read_block(const ss_graph *g, int which /*0=Y,1=Z,2,3*/, double *out);
 int ss_graph_read_probs(const ss_graph *g, double *p);
+/* The fused ER/coin/sample/compaction pass alone over the solution of the
+ * last solve (k columns; ss_batch_write_rhs + ss_batch_solve), with beta and
+ * sampling seed: compute_probs + flip_coins + sample_and_reweight
+ * (src/sparsifier.rs:542-680).  Kept edges readable with ss_graph_read_kept. */
+int ss_graph_sample(ss_graph *g, int32_t k, double beta, uint64_t sampling_seed, int64_t *kept);
+/* apply_reweightings of the last sample on the device (src/sparsifier.rs:
+ * 682-691): the new Laplacian is the kept edges (ss_graph_read_kept: L_uv =
+ * -kw) and diag_out (host, n values) = the old diagonal plus each vertex's
+ * changes summed in edge order -- the values Triplet + sparse add produce.
+ * Exact zeros (diagonals of isolated vertices) are for the caller to drop. */
+int ss_graph_apply_reweight(ss_graph *g, double *diag_out);
 
 /* ---------------------------------------------------------------------- */
 /* Batches.  No reference counterpart: the reference sparsifies one graph per

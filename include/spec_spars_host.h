@@ -34,7 +34,7 @@ typedef struct ssh_params {
     int32_t precision;
     int32_t stop_rule;
     int32_t max_iter;
-    int32_t reserved;
+    int32_t resident;       /* device-resident pass (see spec_spars.hpp)   */
     double tol;
 } ssh_params;
 
@@ -62,6 +62,8 @@ int ssh_sparsifier_sample_apply(ssh_sparsifier *s, const double *probs, int64_t 
 int ssh_sparsifier_laplacian(const ssh_sparsifier *s, int64_t *nnz, int32_t *colptr, int32_t *rowidx,
                              double *vals);
 int ssh_sparsifier_info(const ssh_sparsifier *s, double *beta, int32_t *jl_dim, int64_t *threshold);
+/* passes the streaming trigger ran (src/sparsifier.rs:370-374) */
+int ssh_sparsifier_triggered(const ssh_sparsifier *s, int64_t *count);
 /* jl_sketch_colwise_flat of new_entries (src/sparsifier.rs:532-539), n x jl_dim row-major */
 int ssh_sparsifier_sketch(const ssh_sparsifier *s, double *Y);
 

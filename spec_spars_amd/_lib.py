@@ -72,7 +72,7 @@ class HostParams(ctypes.Structure):
                 ("sampling_seed", ctypes.c_uint64), ("streaming", ctypes.c_int32),
                 ("ordering", ctypes.c_int32), ("factor_seed", ctypes.c_uint64),
                 ("precision", ctypes.c_int32), ("stop_rule", ctypes.c_int32),
-                ("max_iter", ctypes.c_int32), ("reserved", ctypes.c_int32), ("tol", ctypes.c_double)]
+                ("max_iter", ctypes.c_int32), ("resident", ctypes.c_int32), ("tol", ctypes.c_double)]
 
 
 KERNELS = ["sketch", "spmm", "tri_fwd", "tri_bwd", "er_sample", "tri_fwd_small", "tri_bwd_small", "vector",
@@ -130,6 +130,9 @@ SIGNATURES = {
                                       ctypes.c_int32, ctypes.c_int32,
                                       ctypes.POINTER(ctypes.c_size_t)]),
     "ss_graph_set_factor": (ctypes.c_int, [c_vp, c_vp]),
+    "ss_graph_create_csc": (ctypes.c_int, [c_vp, ctypes.c_int64, c_i32p, c_i32p, c_f64p, ctypes.POINTER(c_vp)]),
+    "ss_graph_sample": (ctypes.c_int, [c_vp, ctypes.c_int32, ctypes.c_double, ctypes.c_uint64, c_i64p]),
+    "ss_graph_apply_reweight": (ctypes.c_int, [c_vp, c_f64p]),
     "ss_graph_resident_bytes": (ctypes.c_int, [c_vp, ctypes.POINTER(ctypes.c_size_t)]),
     "ss_graph_work_bytes": (ctypes.c_int, [c_vp, ctypes.c_int32, ctypes.c_int32,
                                            ctypes.POINTER(ctypes.c_size_t)]),
@@ -179,6 +182,7 @@ SIGNATURES = {
     "ssh_sparsifier_laplacian": (ctypes.c_int, [c_vp, c_i64p, c_i32p, c_i32p, c_f64p]),
     "ssh_sparsifier_info": (ctypes.c_int, [c_vp, c_f64p, c_i32p, c_i64p]),
     "ssh_sparsifier_sketch": (ctypes.c_int, [c_vp, c_f64p]),
+    "ssh_sparsifier_triggered": (ctypes.c_int, [c_vp, c_i64p]),
     "ssh_mtx_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(c_vp)]),
     "ssh_matrix_info": (ctypes.c_int, [c_vp, c_i32p, c_i32p, c_i64p]),
     "ssh_matrix_csc": (ctypes.c_int, [c_vp, c_i32p, c_i32p, c_f64p]),

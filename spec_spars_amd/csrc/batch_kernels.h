@@ -101,6 +101,9 @@ void bt_residual64(hipStream_t s, Tiles rt, const int32_t *rowptr, const int32_t
                    const double *X, const double *B, double *out, double *partial);
 void bt_convert_masked(hipStream_t s, Tiles rt, int k, const double *src, const int *keep, float *dst);
 void bt_accumulate(hipStream_t s, Tiles rt, int k, const float *d, double *x);
+void bt_apply_diag(hipStream_t s, int32_t n, const int32_t *rowptr, const int32_t *col, const double *val,
+                   const int32_t *iptr, const int32_t *ieid, const double *ew, const double *p, const uint8_t *mask,
+                   double *diag_new);
 void bt_er_sample(hipStream_t s, const BatchSampleArgs &a, int ntiles);
 
 }  // namespace ss

@@ -810,6 +810,48 @@ void bt_convert_masked(hipStream_t s, Tiles rt, int k, const double *src, const 
 void bt_accumulate(hipStream_t s, Tiles rt, int k, const float *d, double *x) {
     hipLaunchKernelGGL(kb_accumulate, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, k, d, x);
 }
+// New diagonal of a sampled Laplacian, as Triplet::insert / process_diagonal
+// / to_csc and the sparse add build it (src/sparsifier.rs:48-87, :645-691):
+// each edge e contributes delta_e = -change_e if kept (change = w - w /
+// sqrt(p)), -w if dropped, to both endpoints; vertex x sums its edges'
+// deltas in edge order from zero and adds the sum to its old diagonal.
+__global__ __launch_bounds__(256) void kb_apply_diag(int32_t n, const int32_t *__restrict__ rowptr,
+                                                     const int32_t *__restrict__ col, const double *__restrict__ val,
+                                                     const int32_t *__restrict__ iptr,
+                                                     const int32_t *__restrict__ ieid, const double *__restrict__ ew,
+                                                     const double *__restrict__ p, const uint8_t *__restrict__ mask,
+                                                     double *__restrict__ diag_new) {
+#pragma clang fp contract(off)
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    double s = 0.0;
+    for (int q = iptr[x]; q < iptr[x + 1]; q++) {
+        const int e = ieid[q];
+        const double true_value = ew[e];
+        double delta;
+        if (mask[e]) {
+            const double target = true_value / sqrt(p[e]);
+            const double change = true_value - target;
+            delta = change * -1.0;
+        } else {
+            delta = true_value * -1.0;
+        }
+        s = s + delta;
+    }
+    double old = 0.0;
+    for (int q = rowptr[x]; q < rowptr[x + 1]; q++)
+        if (col[q] == x) old = val[q];
+    diag_new[x] = old + s;
+}
+
+void bt_apply_diag(hipStream_t s, int32_t n, const int32_t *rowptr, const int32_t *col, const double *val,
+                   const int32_t *iptr, const int32_t *ieid, const double *ew, const double *p, const uint8_t *mask,
+                   double *diag_new) {
+    if (n > 0)
+        hipLaunchKernelGGL(kb_apply_diag, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, rowptr, col, val,
+                           iptr, ieid, ew, p, mask, diag_new);
+}
+
 void bt_er_sample(hipStream_t s, const BatchSampleArgs &a, int ntiles) {
     hipLaunchKernelGGL(kb_er_sample, dim3(ntiles), dim3(256), 0, s, a);
 }

@@ -786,9 +786,12 @@ int ss_sketch(ss_context *ctx, const int32_t *v1, const int32_t *v2, const doubl
 }
 
 // Canonical edges of a CSC Laplacian's strict upper triangle.
+// Canonical edges (u < v, weight -L_uv) and the diagonal of a CSC Laplacian.
 static int csc_upper_edges(int64_t n, const int32_t *colptr, const int32_t *rowidx, const double *vals,
-                           std::vector<int32_t> &u, std::vector<int32_t> &v, std::vector<double> &w) {
+                           std::vector<int32_t> &u, std::vector<int32_t> &v, std::vector<double> &w,
+                           std::vector<double> &diag) {
     if (colptr[0] != 0) return fail(SS_ERR_INVALID, "colptr[0] must be 0");
+    diag.assign((size_t)n, 0.0);
     for (int64_t c = 0; c < n; c++) {
         if (colptr[c + 1] < colptr[c]) return fail(SS_ERR_INVALID, "colptr not monotone");
         for (int32_t q = colptr[c]; q < colptr[c + 1]; q++) {
@@ -798,10 +801,48 @@ static int csc_upper_edges(int64_t n, const int32_t *colptr, const int32_t *rowi
                 u.push_back(r);
                 v.push_back((int32_t)c);
                 w.push_back(-1.0 * vals[q]);
+            } else if (r == c) {
+                diag[(size_t)c] = vals[q];
             }
         }
     }
     return SS_OK;
+}
+
+int ss_graph_create_csc(ss_context *ctx, int64_t n, const int32_t *colptr, const int32_t *rowidx,
+                        const double *vals, ss_graph **out) {
+    if (!ctx || !colptr || !out || (colptr[n] > 0 && (!rowidx || !vals))) return fail(SS_ERR_INVALID, "null argument");
+    if (n <= 0) return fail(SS_ERR_INVALID, "n must be positive");
+    SS_GUARD_BEGIN
+    std::vector<int32_t> u, v;
+    std::vector<double> w, diag;
+    SS_TRY(csc_upper_edges(n, colptr, rowidx, vals, u, v, w, diag));
+    HostGraph G;
+    SS_TRY(graph_build(n, (int64_t)u.size(), u.data(), v.data(), w.data(), G, diag.data()));
+    return batch_upload(ctx, {&G}, out);
+    SS_GUARD_END
+}
+
+int ss_graph_sample(ss_graph *g, int32_t k, double beta, uint64_t sampling_seed, int64_t *kept) {
+    if (!g) return fail(SS_ERR_INVALID, "null graph");
+    SS_TRY(need_single(g));
+    if (!(beta > 0.0) || !std::isfinite(beta)) return fail(SS_ERR_INVALID, "bad beta %g", beta);
+    SS_GUARD_BEGIN
+    if (g->wk != k) return fail(SS_ERR_INVALID, "no solution block for k=%d (the last solve had k=%d)", k, g->wk);
+    SS_TRY(graph_set_sampling(g, 0, beta, sampling_seed));
+    SS_TRY(graph_sample(g, k, nullptr, false));
+    SS_TRY(prof_resolve(g));
+    if (kept) *kept = g->kept_count[0];
+    return SS_OK;
+    SS_GUARD_END
+}
+
+int ss_graph_apply_reweight(ss_graph *g, double *diag_out) {
+    if (!g) return fail(SS_ERR_INVALID, "null graph");
+    SS_TRY(need_single(g));
+    SS_GUARD_BEGIN
+    return graph_apply_reweight(g, diag_out);
+    SS_GUARD_END
 }
 
 int ss_solve_lap(ss_context *ctx, const double *Y, int64_t n, int32_t k, const int32_t *colptr,
@@ -815,10 +856,10 @@ int ss_solve_lap(ss_context *ctx, const double *Y, int64_t n, int32_t k, const i
     if (opts) o = *opts;
     else ss_solve_opts_default(&o);
     std::vector<int32_t> u, v;
-    std::vector<double> w;
-    SS_TRY(csc_upper_edges(n, colptr, rowidx, vals, u, v, w));
+    std::vector<double> w, diag;
+    SS_TRY(csc_upper_edges(n, colptr, rowidx, vals, u, v, w, diag));
     HostGraph G;
-    SS_TRY(graph_build(n, (int64_t)u.size(), u.data(), v.data(), w.data(), G));
+    SS_TRY(graph_build(n, (int64_t)u.size(), u.data(), v.data(), w.data(), G, diag.data()));
     Factor F;
     SS_TRY(factor_build(n, colptr, rowidx, vals, o.factor_seed, o.ordering, F));
     ss_graph *g = nullptr;

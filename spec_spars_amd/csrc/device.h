@@ -257,6 +257,10 @@ int graph_sketch(ss_graph *g, int k, int uniform, int prec);
 int graph_autotune(ss_graph *g, int k, int prec, int reps, ss_tuning_state &best, double &ms_level,
                    double &ms_tile);
 int graph_sample(ss_graph *g, int k, const double *p_in_dev, bool write_d);
+// The last sample applied to the Laplacian as apply_reweightings does
+// (src/sparsifier.rs:682-691): the new diagonal, old diagonal plus each
+// vertex's changes summed in edge order, into diag_out (host, n values).
+int graph_apply_reweight(ss_graph *g, double *diag_out);
 // Sampling from columns [zoff, zoff + k) of the n x ldz solution block into
 // the given kept arrays (capacity M); per graph start and count.
 int graph_sample_cols(ss_graph *g, int k, int zoff, int ldz, const double *p_in_dev, bool write_d, int32_t *ku,

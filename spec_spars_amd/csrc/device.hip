@@ -1318,4 +1318,19 @@ int graph_sample_cols(ss_graph *g, int k, int zoff, int ldz, const double *p_in_
     return SS_OK;
 }
 
+int graph_apply_reweight(ss_graph *g, double *diag_out) {
+    if (g->G != 1) return fail(SS_ERR_INVALID, "apply_reweight takes a single graph");
+    if (!diag_out) return fail(SS_ERR_INVALID, "null output");
+    SS_HIP(hipSetDevice(g->ctx->device));
+    hipStream_t s = g->ctx->stream;
+    double *dn = nullptr;
+    DevArena tmp;
+    SS_TRY(tmp.alloc(&dn, (size_t)g->N));
+    bt_apply_diag(s, (int32_t)g->N, g->rowptr, g->col, g->val, g->iptr, g->ieid, g->ew, g->p, g->mask, dn);
+    SS_HIP(hipGetLastError());
+    SS_HIP(hipMemcpyAsync(diag_out, dn, sizeof(double) * (size_t)g->N, hipMemcpyDeviceToHost, s));
+    SS_HIP(hipStreamSynchronize(s));
+    return SS_OK;
+}
+
 }  // namespace ss

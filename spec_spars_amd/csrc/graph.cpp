@@ -25,7 +25,8 @@ int graph_check_edges(int64_t n, int64_t m, const int32_t *u, const int32_t *v, 
     return SS_OK;
 }
 
-int graph_build(int64_t n, int64_t m, const int32_t *u, const int32_t *v, const double *w, HostGraph &G) {
+int graph_build(int64_t n, int64_t m, const int32_t *u, const int32_t *v, const double *w, HostGraph &G,
+                const double *diag_in) {
     SS_TRY(graph_check_edges(n, m, u, v, w));
     G.n = n;
     G.m = m;
@@ -38,6 +39,11 @@ int graph_build(int64_t n, int64_t m, const int32_t *u, const int32_t *v, const 
         diag[u[e]] += w[e];
         diag[v[e]] += w[e];
     }
+    // a Laplacian handed over as a matrix keeps its own diagonal (after a
+    // sparsification pass it is the old diagonal plus the changes, summed
+    // in another order than the weights)
+    if (diag_in)
+        for (int64_t i = 0; i < n; i++) diag[i] = diag_in[i];
     G.rowptr.assign((size_t)n + 1, 0);
     for (int64_t i = 0; i < n; i++) G.rowptr[i + 1] = G.rowptr[i] + deg[i] + 1;
     const size_t nnz = (size_t)G.rowptr[n];

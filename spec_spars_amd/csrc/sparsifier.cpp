@@ -140,7 +140,10 @@ int Sparsifier::insert(int32_t v1, int32_t v2, double value) {
         int rc = new_entries.insert(v1, v2, value);
         if (rc != SS_OK) return rc;
     }
-    if (streaming && num_edges() > threshold) return sparsify(false, nullptr);
+    if (streaming && num_edges() > threshold) {
+        triggered++;
+        return sparsify(false, nullptr);
+    }
     return SS_OK;
 }
 
@@ -303,6 +306,7 @@ int Sparsifier::sparsify(bool check, SparsificationStats *stats) {
     std::vector<int32_t> u, v;
     std::vector<double> lval;
     upper_edges(u, v, lval);
+    if (params.resident && num_nnz) return sparsify_resident(Y, stats);
     std::vector<double> probs(num_nnz, 1.0);
     std::vector<uint8_t> mask(num_nnz, 1);
     if (num_nnz) {
@@ -322,6 +326,85 @@ int Sparsifier::sparsify(bool check, SparsificationStats *stats) {
     if ((rc = reweight_from_mask(u, v, lval, probs, mask, rw)) != SS_OK) return rc;
     if ((rc = apply_reweightings(rw, true, stats)) != SS_OK) return rc;
     return check ? check_diagonal() : SS_OK;
+}
+
+int Sparsifier::sparsify_resident(const std::vector<double> &Y, SparsificationStats *stats) {
+    // The same pass as sparsify() from the solve on: the Laplacian and Y go
+    // to the device, the factor is the host's (same seed and order); the
+    // solution stays on the device for the fused ER/coin/sample pass, and
+    // apply_reweightings runs there too.  Only the kept edges and the new
+    // diagonal come back.
+    if (!ctx) return fail(SS_ERR_INVALID, "sparsify: no device context");
+    const CscMatrix &A = current_laplacian;
+    const size_t before = num_edges();
+    struct Handles {
+        ss_graph *g = nullptr;
+        ss_factor *f = nullptr;
+        ~Handles() {
+            if (g) ss_graph_destroy(g);
+            if (f) ss_factor_destroy(f);
+        }
+    } h;
+    int rc;
+    if ((rc = ss_graph_create_csc(ctx, num_nodes, A.colptr.data(), A.rowidx.data(), A.vals.data(), &h.g)) != SS_OK)
+        return rc;
+    ss_solve_opts o = solve_opts_of(params);
+    if ((rc = ss_factor_build_ordered(num_nodes, A.colptr.data(), A.rowidx.data(), A.vals.data(), o.factor_seed,
+                                      o.ordering, &h.f)) != SS_OK)
+        return rc;
+    if ((rc = ss_graph_set_factor(h.g, h.f)) != SS_OK) return rc;
+    if ((rc = ss_batch_write_rhs(h.g, 0, jl_dim, o.precision, Y.data())) != SS_OK) return rc;
+    if ((rc = ss_batch_solve(h.g, jl_dim, &o, nullptr, nullptr, nullptr)) != SS_OK) return rc;
+    int64_t kept = 0;
+    if ((rc = ss_graph_sample(h.g, jl_dim, beta, sampling_seed, &kept)) != SS_OK) return rc;
+    std::vector<double> diag((size_t)num_nodes);
+    if ((rc = ss_graph_apply_reweight(h.g, diag.data())) != SS_OK) return rc;
+    std::vector<int32_t> ku((size_t)std::max<int64_t>(kept, 1)), kv(ku.size());
+    std::vector<double> kw(ku.size());
+    if ((rc = ss_graph_read_kept(h.g, kept, ku.data(), kv.data(), kw.data(), &kept)) != SS_OK) return rc;
+    // the new Laplacian: kept off-diagonals -kw (canonical order, so each
+    // column's rows come out sorted), the diagonal where it is non-zero
+    std::vector<int32_t> cnt((size_t)num_nodes + 1, 0);
+    for (int64_t e = 0; e < kept; e++) {
+        cnt[(size_t)ku[e] + 1]++;
+        cnt[(size_t)kv[e] + 1]++;
+    }
+    for (int32_t c = 0; c < num_nodes; c++) cnt[(size_t)c + 1] += cnt[(size_t)c] + (diag[(size_t)c] != 0.0);
+    CscMatrix H;
+    H.n = num_nodes;
+    H.colptr = cnt;
+    H.rowidx.resize((size_t)cnt[(size_t)num_nodes]);
+    H.vals.resize(H.rowidx.size());
+    // column c: its rows below c (edges (u, c), u ascending), then the
+    // diagonal, then its rows above c (edges (c, v), v ascending)
+    std::vector<int32_t> pos(cnt.begin(), cnt.end() - 1);
+    for (int64_t e = 0; e < kept; e++) {
+        const int32_t c = kv[e];
+        H.rowidx[(size_t)pos[c]] = ku[e];
+        H.vals[(size_t)pos[c]++] = -1.0 * kw[e];
+    }
+    for (int32_t c = 0; c < num_nodes; c++)
+        if (diag[(size_t)c] != 0.0) {
+            H.rowidx[(size_t)pos[c]] = c;
+            H.vals[(size_t)pos[c]++] = diag[(size_t)c];
+        }
+    for (int64_t e = 0; e < kept; e++) {
+        const int32_t c = ku[e];
+        H.rowidx[(size_t)pos[c]] = kv[e];
+        H.vals[(size_t)pos[c]++] = -1.0 * kw[e];
+    }
+    current_laplacian.colptr.swap(H.colptr);
+    current_laplacian.rowidx.swap(H.rowidx);
+    current_laplacian.vals.swap(H.vals);
+    const size_t after = num_edges();
+    if (verbose)
+        printf("edges before sparsification: %zu, after: %zu (%.2f%%)\n", before, after,
+               before ? 100.0 * (double)after / (double)before : 0.0);
+    if (stats) {
+        stats->original_num_edges = before;
+        stats->final_num_edges = after;
+    }
+    return SS_OK;
 }
 
 }  // namespace spec_spars
@@ -354,6 +437,7 @@ static SparsifierParameters to_params(const ssh_params *p) {
     q.stop_rule = p->stop_rule;
     q.max_iter = p->max_iter;
     q.tol = p->tol;
+    q.resident = p->resident != 0;
     return q;
 }
 
@@ -378,6 +462,7 @@ void ssh_params_default(ssh_params *p, int bench) {
     p->stop_rule = q.stop_rule;
     p->max_iter = q.max_iter;
     p->tol = q.tol;
+    p->resident = q.resident;
 }
 
 int ssh_sparsifier_create(ss_context *ctx, int32_t num_nodes, const ssh_params *p, ssh_sparsifier **out) {
@@ -516,4 +601,10 @@ extern "C" int ssh_stream_run(const ssh_stream *st, ss_context *ctx, const ssh_p
     } catch (const std::bad_alloc &) {
         return fail(SS_ERR_NOMEM, "host allocation failed");
     }
+}
+
+extern "C" int ssh_sparsifier_triggered(const ssh_sparsifier *s, int64_t *count) {
+    if (!s || !count) return fail(SS_ERR_INVALID, "null argument");
+    *count = s->sp.triggered;
+    return SS_OK;
 }

@@ -115,7 +115,8 @@ struct HostGraph {
     std::vector<double> val;
 };
 int graph_check_edges(int64_t n, int64_t m, const int32_t *u, const int32_t *v, const double *w);
+// diag (n values, may be NULL) replaces the diagonal summed from the weights.
 int graph_build(int64_t n, int64_t m, const int32_t *u, const int32_t *v, const double *w,
-                HostGraph &G);
+                HostGraph &G, const double *diag = nullptr);
 
 }  // namespace ss

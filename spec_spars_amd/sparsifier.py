@@ -15,7 +15,7 @@ from ._lib import check, load
 class SparsifierParameters:
     """SparsifierParameters (src/sparsifier.rs:146-210)."""
 
-    FIELDS = [f for f, _ in _lib.HostParams._fields_ if f != "reserved"]
+    FIELDS = [f for f, _ in _lib.HostParams._fields_]
 
     def __init__(self, **kw):
         p = _lib.HostParams()
@@ -93,6 +93,12 @@ class Sparsifier:
         a, b = ctypes.c_int64(), ctypes.c_int64()
         check(load().ssh_sparsifier_sparsify(self._h, int(check_), ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+    def triggered(self):
+        """Passes the streaming trigger ran (src/sparsifier.rs:370-374)."""
+        x = ctypes.c_int64()
+        check(load().ssh_sparsifier_triggered(self._h, ctypes.byref(x)))
+        return x.value
 
     def sample_and_apply(self, probs):
         """sample_and_reweight(probs) then apply_reweightings(report=false)."""

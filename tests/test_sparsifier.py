@@ -158,7 +158,7 @@ def test_input_stream_runs_like_inserting_by_hand(ctx, tmp_path):
     path = tmp_path / "grid.mtx"
     path.write_text("%%%%MatrixMarket matrix coordinate real symmetric\n%% written by the test\n%d %d %d\n%s\n" %
                     (n, n, len(lines), "\n".join(lines)))
-    params = _params(epsilon=0.5, jl_factor=2.0, sketch_uniform=0, sketch_seed=3, sampling_seed=4)
+    params = _params(epsilon=4.0, jl_factor=2.0, sketch_uniform=0, sketch_seed=3, sampling_seed=4)
     stream = ss.InputStream(path, "grid")
     assert stream.num_nodes == n
     out = tmp_path / "h.mtx"
@@ -173,3 +173,44 @@ def test_input_stream_runs_like_inserting_by_hand(ctx, tmp_path):
     assert rows == cols == n
     assert np.array_equal(cp, L1[0]) and np.array_equal(ri, L1[1]) and np.array_equal(va, L1[2])
     s1.check_diagonal()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_streaming_trigger_resident_equals_host(ctx, precision):
+    """Streaming (src/sparsifier.rs:357-376): once the formed Laplacian holds
+    more than threshold = n * round(beta) * row_constant edges, every insert
+    runs a pass.  Edges go in in batches, the trigger fires, and the
+    device-resident pass (solution, probabilities, coins, sample and the
+    reweighted Laplacian computed on the device, only the kept edges and the
+    new diagonal read back) leaves the same Laplacian, bit for bit, as the
+    host-driven pass after every batch."""
+    import spec_spars_amd as ss
+    from spec_spars_amd import graphs
+    n, u, v = graphs.grid2d(40, 40)
+    w = graphs.loguniform_weights(len(u), seed=6, lo=0.1, hi=10.0)
+    m = len(u)
+    base = dict(epsilon=5.0, beta_constant=4, row_constant=1, sketch_uniform=0, sketch_seed=2, sampling_seed=3,
+                streaming=1, precision=precision, stop_rule=ss.STOP_REL, tol=1e-10, max_iter=2000)
+    runs = []
+    for resident in (0, 1):
+        s = ss.Sparsifier(n, _params(resident=resident, **base), ctx)
+        assert s.threshold == n * 1 * 1 and s.threshold < m
+        snaps = []
+        s.insert_many(u[:2000], v[:2000], w[:2000])          # nothing formed yet: no trigger
+        assert s.triggered() == 0
+        s.form_laplacian()
+        snaps.append(s.current_laplacian())
+        for a in range(2000, m, 280):                       # the formed graph is above the threshold
+            s.insert_many(u[a:a + 280], v[a:a + 280], w[a:a + 280])
+            snaps.append(s.current_laplacian())
+        stats = s.sparsify()
+        snaps.append(s.current_laplacian())
+        s.check_diagonal()
+        runs.append((s.triggered(), stats, snaps))
+        s.close()
+    (t0, st0, sn0), (t1, st1, sn1) = runs
+    assert t0 == t1 and t0 >= 1
+    assert st0 == st1
+    for a, b in zip(sn0, sn1):
+        assert all(np.array_equal(x, y) for x, y in zip(a, b))
