@@ -175,6 +175,27 @@ int ss_er_sample(ss_context *ctx, const double *Z, int64_t n, int32_t k,
                  uint8_t *mask_out, int64_t *kept, int32_t *ku, int32_t *kv,
                  double *kw);
 
+/* No counterpart in the reference: ss_er_sample for callers that cannot
+ * allocate device memory -- code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  Same arguments and results, but
+ * Z, u, v, lval, p_in and every output array except `kept` are DEVICE
+ * pointers (ku, kv, kw of capacity m; p_in, p_out, d_out, mask_out may be
+ * NULL); `kept` is host memory (may be NULL).  The call allocates nothing:
+ * its counters and look-back words live in a fixed per-device array of
+ * 8 * (8 + SS_NOALLOC_MAX_EDGES / 256) bytes (2 MiB), held until the process
+ * ends and set up by ss_context_create; m > SS_NOALLOC_MAX_EDGES is
+ * rejected.  Edges are checked on the device (u < v < n, L_uv < 0, p_in in
+ * [0,1]; SS_ERR_INVALID).  Runs on the context's stream and returns after
+ * it drains.  Every call on one GPU shares that array: calls that overlap
+ * on one device race. */
+#define SS_NOALLOC_MAX_EDGES ((int64_t)1 << 26)
+int ss_er_sample_noalloc(ss_context *ctx, const double *Z, int64_t n, int32_t k,
+                         const int32_t *u, const int32_t *v, const double *lval,
+                         int64_t m, double beta, uint64_t sampling_seed,
+                         const double *p_in, double *p_out, double *d_out,
+                         uint8_t *mask_out, int64_t *kept, int32_t *ku, int32_t *kv,
+                         double *kw);
+
 /* ---------------------------------------------------------------------- */
 /* Device-resident graph: canonical edge list, Laplacian CSR, incidence ids,
  * factor, sketch/solution blocks and sampling output stay in HBM.          */

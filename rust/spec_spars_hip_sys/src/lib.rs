@@ -110,6 +110,12 @@ extern "C" {
                         lval: *const f64, m: i64, beta: f64, sampling_seed: u64, p_in: *const f64,
                         p_out: *mut f64, d_out: *mut f64, mask_out: *mut u8, kept: *mut i64, ku: *mut i32,
                         kv: *mut i32, kw: *mut f64) -> c_int;
+    // no counterpart in the reference: ss_er_sample over device pointers,
+    // allocating nothing (at most SS_NOALLOC_MAX_EDGES = 2^26 edges)
+    pub fn ss_er_sample_noalloc(ctx: *mut ss_context, z: *const f64, n: i64, k: i32, u: *const i32,
+                                v: *const i32, lval: *const f64, m: i64, beta: f64, sampling_seed: u64,
+                                p_in: *const f64, p_out: *mut f64, d_out: *mut f64, mask_out: *mut u8,
+                                kept: *mut i64, ku: *mut i32, kv: *mut i32, kw: *mut f64) -> c_int;
 
     // resident graphs: src/sparsifier.rs:722-770 on the device
     pub fn ss_graph_create(ctx: *mut ss_context, n: i64, m: i64, u: *const i32, v: *const i32, w: *const f64,

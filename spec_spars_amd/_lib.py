@@ -123,6 +123,9 @@ SIGNATURES = {
                                     c_f64p, ctypes.c_int64, ctypes.c_double, ctypes.c_uint64,
                                     c_f64p, c_f64p, c_f64p, c_u8p, c_i64p, c_i32p, c_i32p,
                                     c_f64p]),
+    "ss_er_sample_noalloc": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int64, ctypes.c_int32, c_vp, c_vp, c_vp,
+                                            ctypes.c_int64, ctypes.c_double, ctypes.c_uint64, c_vp, c_vp,
+                                            c_vp, c_vp, c_i64p, c_vp, c_vp, c_vp]),
     "ss_graph_create": (ctypes.c_int, [c_vp, ctypes.c_int64, ctypes.c_int64, c_i32p, c_i32p,
                                        c_f64p, ctypes.POINTER(c_vp)]),
     "ss_graph_destroy": (ctypes.c_int, [c_vp]),

@@ -12,7 +12,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libspecspars_hip.so")
 
-HIP_SOURCES = ["kernels.hip", "core_kernels.hip", "batch_kernels.hip", "device.hip"]
+HIP_SOURCES = ["kernels.hip", "core_kernels.hip", "batch_kernels.hip", "device.hip", "noalloc.hip"]
 CPP_SOURCES = ["factor.cpp", "graph.cpp", "capi.cpp", "sparsifier.cpp", "mtx.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

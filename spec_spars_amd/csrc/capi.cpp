@@ -83,9 +83,21 @@ int ss_context_create(int device, ss_context **out) {
         return fail(SS_ERR_HIP, "hipStreamCreate failed");
     }
     for (auto &ev : c->ev) (void)hipEventCreate(&ev);
-    if (hipHostMalloc((void **)&c->h_status, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void **)&c->h_status, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_noalloc, 2 * sizeof(int64_t), hipHostMallocDefault) != hipSuccess) {
+        if (c->h_status) (void)hipHostFree(c->h_status);
+    if (c->h_noalloc) (void)hipHostFree(c->h_noalloc);
         delete c;
         return fail(SS_ERR_NOMEM, "pinned host allocation failed");
+    }
+    {
+        int st = noalloc_setup(device);
+        if (st != SS_OK) {
+            (void)hipHostFree(c->h_status);
+            (void)hipHostFree(c->h_noalloc);
+            delete c;
+            return st;
+        }
     }
     *out = c;
     return SS_OK;
@@ -101,6 +113,7 @@ int ss_context_destroy(ss_context *c) {
     for (auto &ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (c->h_status) (void)hipHostFree(c->h_status);
+    if (c->h_noalloc) (void)hipHostFree(c->h_noalloc);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return SS_OK;
@@ -966,9 +979,11 @@ int ss_er_sample(ss_context *ctx, const double *Z, int64_t n, int32_t k, const i
         ErSampleArgs<double> a;
         a.m = m;
         a.ntiles = ntiles;
+        a.n = n;
         a.eu = d_u;
         a.ev = d_v;
         a.ew = d_w;
+        a.ew_is_lval = 0;
         a.Z = d_Z;
         a.k = k;
         a.beta = beta;
@@ -992,6 +1007,7 @@ int ss_er_sample(ss_context *ctx, const double *Z, int64_t n, int32_t k, const i
     SS_HIP(hipMemcpyAsync(&h_kept, d_kept, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     SS_HIP(hipMemcpyAsync(&h_err, d_err, sizeof(int), hipMemcpyDeviceToHost, s));
     SS_HIP(hipStreamSynchronize(s));
+    if (h_err == 2) return fail(SS_ERR_INVALID, "an edge is not u < v < n with L_uv < 0, or a probability is outside [0,1]");
     if (h_err) return fail(SS_ERR_TIMEOUT, "device wait exceeded its spin limit (compaction look-back)");
     if (kept) *kept = h_kept;
     if (m) {

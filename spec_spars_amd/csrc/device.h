@@ -23,10 +23,15 @@ struct ss_context {
     hipStream_t stream = nullptr;
     hipEvent_t ev[8] = {};
     int *h_status = nullptr;      // pinned: {nactive, err}
+    int64_t *h_noalloc = nullptr; // pinned: {kept, err} of ss_er_sample_noalloc
     int live_graphs = 0;          // graphs created on this context and not yet destroyed
 };
 
 namespace ss {
+
+// Device address of the allocation-free sampling scratch array, once per
+// device with that device current (noalloc.hip).
+int noalloc_setup(int device);
 
 // A set of device allocations released together.
 struct DevArena {

@@ -23,9 +23,11 @@ template <typename TZ>
 struct ErSampleArgs {
     int64_t m;
     int64_t ntiles;
+    int64_t n;              // vertex count: edges not u < v < n are flagged (err 2), never read
     const int32_t *eu;
     const int32_t *ev;
-    const double *ew;
+    const double *ew;       // w = -L_uv, or L_uv itself when ew_is_lval
+    int ew_is_lval;
     const TZ *Z;
     int k;
     double beta;
@@ -41,7 +43,7 @@ struct ErSampleArgs {
     uint64_t *tile_state;
     uint32_t epoch;
     int64_t *kept_total;
-    int *err;
+    int *err;               // 1: a look-back wait hit spin_limit; 2: an invalid edge or probability
     int spin_limit;
 };
 

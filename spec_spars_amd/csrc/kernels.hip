@@ -623,12 +623,20 @@ __global__ __launch_bounds__(256) void k_er_sample(ErSampleArgs<TZ> a) {
     if (e < a.m) {
         eu = a.eu[e];
         ev = a.ev[e];
-        const double w = a.ew[e];
+        const double w = a.ew_is_lval ? -1.0 * a.ew[e] : a.ew[e];
         const double lval = -1.0 * w;
-        double p;
-        if (a.p_in) {
+        double p = 0.0;
+        // an edge that is not u < v < n with L_uv < 0 (or a probability
+        // outside [0, 1]) is flagged and dropped before Z is touched
+        bool bad = eu < 0 || ev >= a.n || eu >= ev || !(w > 0.0);
+        if (!bad && a.p_in) {
             p = a.p_in[e];
-        } else {
+            bad = !(p >= 0.0 && p <= 1.0);
+        }
+        if (bad) {
+            __hip_atomic_store(a.err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            p = 0.0;
+        } else if (!a.p_in) {
             const TZ *zu = a.Z + (size_t)eu * a.k;
             const TZ *zv = a.Z + (size_t)ev * a.k;
             double s = 0.0;
