@@ -178,74 +178,105 @@ template <> struct Acc4<double> { typedef f64x4 type; };
 
 // One piece of a core product: rows [r0, r0 + kCT) of graph g, inner range
 // [m0, m1) -> partial tile part[piece] (kCT x k).  Forward A = M, backward
-// A = M^T.  Four waves per 64 x 64 output tile (a 64-column chunk of k),
-// each a 32 x 32 quarter as 2 x 2 blocks of 16 x 16 on the matrix cores;
-// the inner dimension is staged in LDS kCK rows at a time.
-template <typename T, bool BWD>
+// A = M^T.  The workgroup holds the tile's 64 rows by all k columns (k
+// rounded up to KC = 64 RB): its four waves each take RB of the four 16-row
+// blocks by the four 16-column blocks of one 64-column chunk, on the matrix
+// cores (16 x 16 x 4), so every A tile is read from memory once whatever k
+// is.  The inner dimension is staged in LDS kAK rows at a time, double
+// buffered: the next stage's loads are in flight while the current one is
+// multiplied, with one barrier per stage.  Each output adds its terms in
+// ascending inner order, four at a time, for every k and RB.
+constexpr int kAK = 16;   // inner rows per stage of k_core_apply
+template <typename T, bool BWD, int RB>
 __global__ __launch_bounds__(256) void k_core_apply(const CorePiece *__restrict__ pieces,
                                                     const int32_t *__restrict__ cpad,
                                                     const int64_t *__restrict__ dense_off,
                                                     const int64_t *__restrict__ tc_off, const T *__restrict__ M,
                                                     int k, const T *__restrict__ tc, T *__restrict__ part) {
+    static_assert(RB == 1 || RB == 2 || RB == 4, "row blocks per wave");
     typedef typename Acc4<T>::type AccT;
-    __shared__ T as[kCK][kCT + 4];   // as[m][r]: A(r, m)
-    __shared__ T bs[kCK][kCT + 4];   // bs[m][c]: Tc(m, c)
+    constexpr int KC = 64 * RB;              // columns held
+    constexpr int NA = kCT * kAK / 256;      // A elements per thread per stage
+    constexpr int NB = kAK * KC / 256;       // B elements per thread per stage
+    __shared__ T as[2][kAK][kCT + 4];        // as[.][m][r]: A(r, m)
+    __shared__ T bs[2][kAK][KC + 4];         // bs[.][m][c]: Tc(m, c)
     const CorePiece pc = pieces[blockIdx.x];
     const int cp = cpad[pc.graph];
     const T *Mg = M + dense_off[pc.graph];
     const T *tg = tc + (size_t)tc_off[pc.graph] * k;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;   // this wave's quarter
+    const int wr = (wv % (4 / RB)) * 16 * RB;   // this wave's first row
+    const int wc = (wv / (4 / RB)) * 64;        // and first column
     const int li = lane & 15, lk = lane >> 4;
-    T *po = part + (size_t)blockIdx.x * kCT * k;
-    for (int jb = 0; jb < k; jb += kCT) {
-        AccT acc[2][2];
+    T ra[NA], rb[NB];
+    // stage loads: A element e = t + 256 u; forward A(r, m) = M[r0 + r][m]
+    // (rows of kAK contiguous entries), backward A(r, m) = M[m][r0 + r]
+    // (rows of kCT contiguous entries); B element e: Tc row m0 + e / KC,
+    // column e % KC (zero past k)
+    auto load = [&](int m0) {
 #pragma unroll
-        for (int x = 0; x < 2; x++)
-#pragma unroll
-            for (int y = 0; y < 2; y++) acc[x][y] = AccT{0, 0, 0, 0};
-        for (int m0 = pc.m0; m0 < pc.m1; m0 += kCK) {
-            if (BWD) {
-                for (int e = t; e < kCK * kCT; e += 256) {
-                    const int mm = e / kCT, r = e % kCT;
-                    as[mm][r] = Mg[(size_t)(m0 + mm) * cp + pc.r0 + r];
-                }
-            } else {
-                for (int e = t; e < kCT * kCK; e += 256) {
-                    const int r = e / kCK, mm = e % kCK;
-                    as[mm][r] = Mg[(size_t)(pc.r0 + r) * cp + m0 + mm];
-                }
-            }
-            for (int e = t; e < kCK * kCT; e += 256) {
-                const int mm = e / kCT, c = e % kCT;
-                bs[mm][c] = jb + c < k ? tg[(size_t)(m0 + mm) * k + jb + c] : (T)0;
-            }
-            __syncthreads();
-#pragma unroll
-            for (int m4 = 0; m4 < kCK; m4 += 4) {
-                T a[2], b[2];
-#pragma unroll
-                for (int x = 0; x < 2; x++) a[x] = as[m4 + lk][wr + 16 * x + li];
-#pragma unroll
-                for (int y = 0; y < 2; y++) b[y] = bs[m4 + lk][wc + 16 * y + li];
-#pragma unroll
-                for (int x = 0; x < 2; x++)
-#pragma unroll
-                    for (int y = 0; y < 2; y++) acc[x][y] = mfma16(a[x], b[y], acc[x][y]);
-            }
-            __syncthreads();
+        for (int u = 0; u < NA; u++) {
+            const int e = t + 256 * u;
+            ra[u] = BWD ? Mg[(size_t)(m0 + e / kCT) * cp + pc.r0 + e % kCT]
+                        : Mg[(size_t)(pc.r0 + e / kAK) * cp + m0 + e % kAK];
         }
 #pragma unroll
-        for (int x = 0; x < 2; x++)
+        for (int u = 0; u < NB; u++) {
+            const int e = t + 256 * u, c = e % KC;
+            rb[u] = c < k ? tg[(size_t)(m0 + e / KC) * k + c] : (T)0;
+        }
+    };
+    auto store = [&](int b) {
 #pragma unroll
-            for (int y = 0; y < 2; y++) {
-                const int c = jb + wc + 16 * y + li;
-                if (c < k)
+        for (int u = 0; u < NA; u++) {
+            const int e = t + 256 * u;
+            if (BWD)
+                as[b][e / kCT][e % kCT] = ra[u];
+            else
+                as[b][e % kAK][e / kAK] = ra[u];
+        }
 #pragma unroll
-                    for (int i = 0; i < 4; i++)
-                        po[(size_t)(wr + 16 * x + acc_row<T>(lk, i)) * k + c] = acc[x][y][i];
-            }
+        for (int u = 0; u < NB; u++) {
+            const int e = t + 256 * u;
+            bs[b][e / KC][e % KC] = rb[u];
+        }
+    };
+    AccT acc[RB][4];
+#pragma unroll
+    for (int x = 0; x < RB; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) acc[x][y] = AccT{0, 0, 0, 0};
+    load(pc.m0);
+    int b = 0;
+    for (int m0 = pc.m0; m0 < pc.m1; m0 += kAK, b ^= 1) {
+        // buffer b was last read two stages ago: every wave has passed the
+        // previous stage's barrier since
+        store(b);
+        __syncthreads();
+        if (m0 + kAK < pc.m1) load(m0 + kAK);
+#pragma unroll
+        for (int m4 = 0; m4 < kAK; m4 += 4) {
+            T a[RB], bb[4];
+#pragma unroll
+            for (int x = 0; x < RB; x++) a[x] = as[b][m4 + lk][wr + 16 * x + li];
+#pragma unroll
+            for (int y = 0; y < 4; y++) bb[y] = bs[b][m4 + lk][wc + 16 * y + li];
+#pragma unroll
+            for (int x = 0; x < RB; x++)
+#pragma unroll
+                for (int y = 0; y < 4; y++) acc[x][y] = mfma16(a[x], bb[y], acc[x][y]);
+        }
     }
+    T *po = part + (size_t)blockIdx.x * kCT * k;
+#pragma unroll
+    for (int x = 0; x < RB; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) {
+            const int c = wc + 16 * y + li;
+            if (c < k)
+#pragma unroll
+                for (int i = 0; i < 4; i++) po[(size_t)(wr + 16 * x + acc_row<T>(lk, i)) * k + c] = acc[x][y][i];
+        }
 }
 
 // out[vert[r]] = sum of the row's pieces, in piece order.
@@ -301,8 +332,16 @@ void launch_core_apply(hipStream_t s, const CoreApply &c, const T *M, int k, con
     if (c.pieces <= 0) return;
     hipLaunchKernelGGL((k_core_gather<T, BWD>), dim3(stream_grid(c.rows * k)), dim3(256), 0, s, c.rows, c.vert, diag,
                        k, src, tc);
-    hipLaunchKernelGGL((k_core_apply<T, BWD>), dim3((unsigned)c.pieces), dim3(256), 0, s,
-                       BWD ? c.piece_bwd : c.piece_fwd, c.cpad, c.dense_off, c.tc_off, M, k, tc, part);
+    const CorePiece *pc = BWD ? c.piece_bwd : c.piece_fwd;
+    if (k <= 64)
+        hipLaunchKernelGGL((k_core_apply<T, BWD, 1>), dim3((unsigned)c.pieces), dim3(256), 0, s, pc, c.cpad,
+                           c.dense_off, c.tc_off, M, k, tc, part);
+    else if (k <= 128)
+        hipLaunchKernelGGL((k_core_apply<T, BWD, 2>), dim3((unsigned)c.pieces), dim3(256), 0, s, pc, c.cpad,
+                           c.dense_off, c.tc_off, M, k, tc, part);
+    else
+        hipLaunchKernelGGL((k_core_apply<T, BWD, 4>), dim3((unsigned)c.pieces), dim3(256), 0, s, pc, c.cpad,
+                           c.dense_off, c.tc_off, M, k, tc, part);
     hipLaunchKernelGGL(k_core_reduce<T>, dim3(stream_grid((int64_t)c.tiles * kCT * k)), dim3(256), 0, s, c.tiles,
                        BWD ? c.tile_piece_bwd : c.tile_piece_fwd, c.tile_row, c.vert, k, part, out);
 }
