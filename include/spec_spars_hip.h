@@ -121,6 +121,15 @@ int ss_sketch(ss_context *ctx, const int32_t *v1, const int32_t *v2,
 /* (b)+(c) Laplacian solve L Z = Y - mean, replaces ffi::run_solve_lap
  * (src/lib.rs:103, include/example.h:18, src/example.cc:668-703) and the
  * MKL PCG it drives (include/custom_cg.hpp:95-274).                        */
+/* Mixed precision (SS_PREC_MIXED) around an FP32 PCG with FP64 dots:
+ * RELIABLE keeps one PCG running and replaces a column's recursive residual
+ * by the FP64 true residual b - A x whenever it has fallen by inner_tol
+ * (reliable updates; the search directions are kept); RESTART runs FP32
+ * PCG sweeps, each on the FP64 residual of the last, until the FP64 rule
+ * holds (iterative refinement). */
+#define SS_MIXED_RELIABLE 0
+#define SS_MIXED_RESTART 1
+
 typedef struct ss_solve_opts {
     uint64_t factor_seed;   /* seed of the randomized factor                 */
     int32_t stop_rule;      /* SS_STOP_REF_ABS or SS_STOP_REL                */
@@ -130,8 +139,12 @@ typedef struct ss_solve_opts {
                                1e-2 in the reference (custom_cg.hpp:254)     */
     int32_t precision;      /* SS_PREC_FP64 or SS_PREC_MIXED                 */
     int32_t ordering;       /* SS_ORDER_* of the factor ss_solve_lap builds  */
-    double inner_tol;       /* SS_PREC_MIXED: relative residual reduction each
-                               FP32 PCG sweep asks for (0 = 1e-3)            */
+    double inner_tol;       /* SS_PREC_MIXED: relative residual reduction
+                               between two FP64 residual replacements
+                               (reliable; 0 = 3e-3) or asked of each FP32
+                               PCG sweep (restart; 0 = 1e-3)                 */
+    int32_t mixed_mode;     /* SS_PREC_MIXED: SS_MIXED_RELIABLE (default) or
+                               SS_MIXED_RESTART                              */
 } ss_solve_opts;
 
 typedef struct ss_solve_report {

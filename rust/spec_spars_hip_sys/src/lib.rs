@@ -43,6 +43,7 @@ pub struct ss_solve_opts {
     pub precision: i32,
     pub ordering: i32,
     pub inner_tol: f64,
+    pub mixed_mode: i32,
 }
 
 #[repr(C)]

@@ -35,7 +35,8 @@ class SolveOpts(ctypes.Structure):
     _fields_ = [("factor_seed", ctypes.c_uint64), ("stop_rule", ctypes.c_int32),
                 ("max_iter", ctypes.c_int32), ("tol", ctypes.c_double),
                 ("fail_relres", ctypes.c_double), ("precision", ctypes.c_int32),
-                ("ordering", ctypes.c_int32), ("inner_tol", ctypes.c_double)]
+                ("ordering", ctypes.c_int32), ("inner_tol", ctypes.c_double),
+                ("mixed_mode", ctypes.c_int32)]
 
 
 class SolveReport(ctypes.Structure):

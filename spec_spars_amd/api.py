@@ -21,7 +21,8 @@ def _p(a, ct):
 
 
 def solve_opts(factor_seed=0, stop_rule=_lib.STOP_REF_ABS, max_iter=400, tol=None,
-               fail_relres=1e-2, precision=_lib.PREC_FP64, ordering=_lib.ORDER_ROUNDS, inner_tol=0.0):
+               fail_relres=1e-2, precision=_lib.PREC_FP64, ordering=_lib.ORDER_ROUNDS, inner_tol=0.0,
+               mixed_mode=0):
     o = _lib.SolveOpts()
     load().ss_solve_opts_default(ctypes.byref(o))
     o.factor_seed = factor_seed
@@ -32,6 +33,7 @@ def solve_opts(factor_seed=0, stop_rule=_lib.STOP_REF_ABS, max_iter=400, tol=Non
     o.precision = precision
     o.ordering = ordering
     o.inner_tol = inner_tol
+    o.mixed_mode = mixed_mode
     return o
 
 

@@ -73,7 +73,8 @@ void bt_tile_partial(hipStream_t s, Tiles rt, int k, const TA *A, const TB *B, c
                      double *partial);
 void bt_pcg_init(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p_bb, const double *p_rz,
                  PcgScalars sc);
-void bt_pcg_alpha(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, PcgScalars sc, int *nactive);
+void bt_pcg_alpha(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, PcgScalars sc, int *nactive,
+                  int *flag = nullptr);
 void bt_pcg_check(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, int rule, double tol,
                   PcgScalars sc, int *nactive);
 void bt_pcg_beta(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, PcgScalars sc);
@@ -97,10 +98,29 @@ void bt_spmm_dot(hipStream_t s, Tiles rt, const int32_t *rowptr, const int32_t *
 // R -= alpha Q, per-tile partials of |R|^2.
 template <typename T>
 void bt_update_r_rr(hipStream_t s, Tiles rt, int k, const double *alpha, const T *Q, T *R, double *partial);
+// out = B - A X (FP64) with per-tile partials of |out|^2; mask (per
+// (graph, column), or NULL for all) selects the columns computed.
 void bt_residual64(hipStream_t s, Tiles rt, const int32_t *rowptr, const int32_t *col, const double *val, int k,
-                   const double *X, const double *B, double *out, double *partial);
+                   const double *X, const double *B, double *out, double *partial, const int *mask = nullptr);
 void bt_convert_masked(hipStream_t s, Tiles rt, int k, const double *src, const int *keep, float *dst);
 void bt_accumulate(hipStream_t s, Tiles rt, int k, const float *d, double *x);
+// Mixed precision with reliable updates (per (graph, column)): bnorm from
+// rr_ref = |B64|^2; the recursive residual check flagging columns for
+// replacement; the fold of flagged columns' FP32 iterate x into X64
+// (X32 = x); R64 -= A x on the flagged columns; the FP64 stop test and new
+// reference; R32 = (float) R64 and X32 = 0 on the replaced columns.
+// status = {active, err, flagged}.
+void bt_reliable_init(hipStream_t s, int gk, PcgScalars sc, double *rr_ref, int *flag, int *status);
+void bt_reliable_check(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, int rule, double tol,
+                       double delta2, PcgScalars sc, const double *rr_ref, int *flag, int *status);
+void bt_reliable_fold(hipStream_t s, Tiles rt, int k, const int *flag, const double *alpha, const float *P,
+                      float *X32, double *X64);
+void bt_reliable_apply(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, int rule, double tol,
+                       PcgScalars sc, double *rr_ref, int *flag, int *keep, int *status);
+void bt_reliable_residual(hipStream_t s, Tiles rt, const int32_t *rowptr, const int32_t *col, const double *val,
+                          int k, const int *flag, const float *X, double *R, double *partial);
+void bt_reliable_replace(hipStream_t s, Tiles rt, int k, const double *src, const int *keep, float *dst,
+                         float *X32);
 void bt_apply_diag(hipStream_t s, int32_t n, const int32_t *rowptr, const int32_t *col, const double *val,
                    const int32_t *iptr, const int32_t *ieid, const double *ew, const double *p, const uint8_t *mask,
                    double *diag_new);

@@ -195,14 +195,18 @@ __global__ __launch_bounds__(64) void kb_pcg_init(int k, const int32_t *gtile, c
     }
 }
 
+// A column whose p . A p is not positive stops; with flag (reliable
+// updates) it is flagged for a residual replacement (2: with a new search
+// direction) instead.
 __global__ __launch_bounds__(64) void kb_pcg_alpha(int k, const int32_t *gtile, const double *p_pq, PcgScalars s,
-                                                   int *nactive) {
+                                                   int *nactive, int *flag) {
     const int gj = blockIdx.x, g = gj / k, j = gj - g * k;
     const double pq = bt_graph_sum(p_pq, gtile, g, k, j);
     if (BT_LANE == 0) {
         double a = 0.0;
         if (s.active[gj]) {
             if (pq > 0.0) a = s.rz[gj] / pq;
+            else if (flag) flag[gj] = 2;
             else s.active[gj] = 0;
         }
         s.alpha[gj] = a;
@@ -518,21 +522,25 @@ __global__ __launch_bounds__(1024) void kb_update_r_rr(Tiles rt, int k, const do
     }
 }
 
-// out = B - A X in FP64, with per-tile partials of |out|^2
+// out = B - A X in FP64, with per-tile partials of |out|^2; with a mask,
+// only the (graph, column) pairs it marks are computed (the others keep
+// out and get zero partials)
 __global__ __launch_bounds__(1024) void kb_residual64(Tiles rt, const int32_t *__restrict__ rowptr,
                                                      const int32_t *__restrict__ col, const double *__restrict__ val,
                                                      int k, const double *__restrict__ X, const double *__restrict__ B,
-                                                     double *__restrict__ out, double *__restrict__ partial) {
+                                                     const int *__restrict__ mask, double *__restrict__ out,
+                                                     double *__restrict__ partial) {
     __shared__ double sm[kMaxTileWaves][64];
-    const int t = blockIdx.x;
+    const int t = blockIdx.x, g = rt.graph[t];
     const int r0 = rt.start[t], r1 = rt.start[t + 1];
     for (int jb = 0; jb < k; jb += 64) {
         const int j = jb + BT_LANE;
+        const bool on = j < k && (!mask || mask[(size_t)g * k + j] != 0);
         double acc = 0.0;
         for (int row = r0 + BT_WAVE; row < r1; row += BT_NW) {
             const int r = bt_uniform(row);
             const int q0 = rowptr[r], q1 = rowptr[r + 1];
-            if (j < k) {
+            if (on) {
                 const size_t o = (size_t)r * k + j;
                 const double v = bt_dot_sub<double>(B[o], q0, q1, col, val, X, k, j);
                 out[o] = v;
@@ -568,6 +576,169 @@ __global__ __launch_bounds__(1024) void kb_accumulate(Tiles rt, int k, const flo
         const int j = jb + BT_LANE;
         if (j >= k) break;
         for (int r = r0 + BT_WAVE; r < r1; r += BT_NW) x[(size_t)r * k + j] += (double)d[(size_t)r * k + j];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision with reliable updates.  One FP32 PCG runs on; a column
+// whose recursive FP32 residual has fallen by delta since its last
+// replacement (or to the stop target) folds its FP32 iterate into the FP64
+// solution and takes the FP64 true residual b - A x as its residual, its
+// search direction kept.  Every step is per (graph, column): a column's
+// arithmetic never depends on when the others replace.
+// ---------------------------------------------------------------------------
+
+// bnorm = |B64| and rr_ref = |B64|^2 from rr_ref (uploaded |B64|^2); flags
+// and the flag count cleared.
+__global__ __launch_bounds__(256) void kb_reliable_init(int gk, PcgScalars s, double *rr_ref, int *flag, int *status) {
+    const int gj = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gj == 0) status[2] = 0;
+    if (gj >= gk) return;
+    s.bnorm[gj] = sqrt(rr_ref[gj]);
+    flag[gj] = 0;
+}
+
+// The recursive residual's |R|^2 from the tiles; an active column is
+// flagged when it reached rr_ref * delta2 or the stop target.  Counts the
+// active (status[0], cleared by kb_pcg_alpha) and flagged (status[2])
+// columns.
+__global__ __launch_bounds__(64) void kb_reliable_check(int k, const int32_t *gtile, const double *p_rr, int rule,
+                                                        double tol, double delta2, PcgScalars s,
+                                                        const double *rr_ref, int *flag, int *status) {
+    const int gj = blockIdx.x, g = gj / k, j = gj - g * k;
+    const double rr = bt_graph_sum(p_rr, gtile, g, k, j);
+    if (BT_LANE == 0) {
+        s.rr[gj] = rr;
+        if (s.active[gj]) {
+            s.iters[gj] += 1;
+            const bool target = rule == 0 ? rr <= tol : sqrt(rr) <= tol * s.bnorm[gj];
+            if (flag[gj] || target || rr <= delta2 * rr_ref[gj]) {
+                if (!flag[gj]) flag[gj] = 1;
+                atomicAdd(status + 2, 1);
+            }
+            atomicAdd(status, 1);
+        }
+    }
+}
+
+// Replacement, step 1 (flagged columns): x = X32 + alpha P (the FP32
+// iterate with the iteration's pending update), X64 += x, X32 = x.
+__global__ __launch_bounds__(1024) void kb_reliable_fold(Tiles rt, int k, const int *__restrict__ flag,
+                                                        const double *__restrict__ alpha, const float *__restrict__ P,
+                                                        float *__restrict__ X32, double *__restrict__ X64) {
+    const int t = blockIdx.x, g = rt.graph[t];
+    const int r0 = rt.start[t], r1 = rt.start[t + 1];
+    for (int jb = 0; jb < k; jb += 64) {
+        const int j = jb + BT_LANE;
+        if (j >= k) break;
+        if (!flag[(size_t)g * k + j]) continue;
+        const float a = (float)alpha[(size_t)g * k + j];
+        for (int r = r0 + BT_WAVE; r < r1; r += BT_NW) {
+            const size_t o = (size_t)r * k + j;
+            const float x = a != 0.0f ? bt_fma(a, P[o], X32[o]) : X32[o];
+            X64[o] += (double)x;
+            X32[o] = x;
+        }
+    }
+}
+
+// Replacement, step 2 (flagged columns): R64 -= A x with x the folded
+// FP32 iterate (R64 held b - A X64 before the fold, so it becomes the FP64
+// true residual of the new X64; the gathers read FP32 rows), with per-tile
+// partials of |R64|^2 (zero for the other columns).
+__global__ __launch_bounds__(1024) void kb_reliable_residual(Tiles rt, const int32_t *__restrict__ rowptr,
+                                                            const int32_t *__restrict__ col,
+                                                            const double *__restrict__ val, int k,
+                                                            const int *__restrict__ flag,
+                                                            const float *__restrict__ X, double *__restrict__ R,
+                                                            double *__restrict__ partial) {
+    __shared__ double sm[kMaxTileWaves][64];
+    const int t = blockIdx.x, g = rt.graph[t];
+    const int r0 = rt.start[t], r1 = rt.start[t + 1];
+    for (int jb = 0; jb < k; jb += 64) {
+        const int j = jb + BT_LANE;
+        const bool on = j < k && flag[(size_t)g * k + j] != 0;
+        double acc = 0.0;
+        for (int row = r0 + BT_WAVE; row < r1; row += BT_NW) {
+            const int r = bt_uniform(row);
+            const int q0 = rowptr[r], q1 = rowptr[r + 1];
+            if (on) {
+                const size_t o = (size_t)r * k + j;
+                double v = R[o];
+                int q = q0;
+                for (; q + 7 < q1; q += 8) {
+                    float x[8];
+                    double w[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        x[u] = X[(size_t)col[q + u] * k + j];
+                        w[u] = val[q + u];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; u++) v = bt_fma(-w[u], (double)x[u], v);
+                }
+                for (; q < q1; q++) v = bt_fma(-val[q], (double)X[(size_t)col[q] * k + j], v);
+                R[o] = v;
+                acc = bt_fma(v, v, acc);
+            }
+        }
+        bt_tile_store(acc, k, jb, partial, sm);
+    }
+}
+
+// Replacement, step 3, with |B64 - A X64|^2 of the flagged columns from the
+// SpMM tiles: a flagged column that meets the FP64 stop rule is done; the
+// others take it as their reference (keep = 1: their FP32 residual is
+// replaced; keep = 2 marks the done ones, whose FP32 iterate is cleared
+// too).  Where the true residual exceeds the recursive one by more
+// than kReliableJump (the FP32 recursion ran below what its arithmetic
+// resolves) or p . A p broke down, the column's next search direction is
+// its preconditioned residual alone (rz = 0 makes beta zero).  Flagged
+// columns' alpha is cleared (their update is folded); flags and the flag
+// count are cleared.
+constexpr double kReliableJump = 4.0;   // |r|^2 ratio, true over recursive
+__global__ __launch_bounds__(64) void kb_reliable_apply(int k, const int32_t *gtile, const double *p_rr, int rule,
+                                                        double tol, PcgScalars s, double *rr_ref, int *flag,
+                                                        int *keep, int *status) {
+    const int gj = blockIdx.x, g = gj / k, j = gj - g * k;
+    const double rr = bt_graph_sum(p_rr, gtile, g, k, j);
+    if (BT_LANE == 0) {
+        keep[gj] = 0;
+        if (flag[gj]) {
+            s.alpha[gj] = 0.0;
+            const bool done = rr == 0.0 || (rule == 0 ? rr <= tol : sqrt(rr) <= tol * s.bnorm[gj]);
+            if (done) {
+                s.active[gj] = 0;
+                keep[gj] = 2;
+                atomicSub(status, 1);
+            } else {
+                if (flag[gj] == 2 || rr > kReliableJump * s.rr[gj]) s.rz[gj] = 0.0;
+                rr_ref[gj] = rr;
+                keep[gj] = 1;
+            }
+            flag[gj] = 0;
+        }
+        if (gj == 0) status[2] = 0;
+    }
+}
+
+// Replacement, step 4: X32 = 0 on the replaced columns, and R32 =
+// (float) R64 on those that go on (keep 1); the others untouched.
+__global__ __launch_bounds__(1024) void kb_reliable_replace(Tiles rt, int k, const double *__restrict__ src,
+                                                           const int *__restrict__ keep, float *__restrict__ dst,
+                                                           float *__restrict__ X32) {
+    const int t = blockIdx.x, g = rt.graph[t];
+    const int r0 = rt.start[t], r1 = rt.start[t + 1];
+    for (int jb = 0; jb < k; jb += 64) {
+        const int j = jb + BT_LANE;
+        if (j >= k) break;
+        const int kp = keep[(size_t)g * k + j];
+        if (!kp) continue;
+        for (int r = r0 + BT_WAVE; r < r1; r += BT_NW) {
+            const size_t o = (size_t)r * k + j;
+            if (kp == 1) dst[o] = (float)src[o];
+            X32[o] = 0.0f;
+        }
     }
 }
 
@@ -730,8 +901,9 @@ void bt_pcg_init(hipStream_t s, int gk, int k, const int32_t *gtile, const doubl
                  PcgScalars sc) {
     hipLaunchKernelGGL(kb_pcg_init, dim3(gk), dim3(64), 0, s, k, gtile, p_bb, p_rz, sc);
 }
-void bt_pcg_alpha(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, PcgScalars sc, int *nactive) {
-    hipLaunchKernelGGL(kb_pcg_alpha, dim3(gk), dim3(64), 0, s, k, gtile, p, sc, nactive);
+void bt_pcg_alpha(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, PcgScalars sc, int *nactive,
+                  int *flag) {
+    hipLaunchKernelGGL(kb_pcg_alpha, dim3(gk), dim3(64), 0, s, k, gtile, p, sc, nactive, flag);
 }
 void bt_pcg_check(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, int rule, double tol,
                   PcgScalars sc, int *nactive) {
@@ -801,14 +973,41 @@ void bt_update_r_rr(hipStream_t s, Tiles rt, int k, const double *alpha, const T
 template void bt_update_r_rr<double>(hipStream_t, Tiles, int, const double *, const double *, double *, double *);
 template void bt_update_r_rr<float>(hipStream_t, Tiles, int, const double *, const float *, float *, double *);
 void bt_residual64(hipStream_t s, Tiles rt, const int32_t *rowptr, const int32_t *col, const double *val, int k,
-                   const double *X, const double *B, double *out, double *partial) {
-    hipLaunchKernelGGL(kb_residual64, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, rowptr, col, val, k, X, B, out, partial);
+                   const double *X, const double *B, double *out, double *partial, const int *mask) {
+    hipLaunchKernelGGL(kb_residual64, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, rowptr, col, val, k, X, B, mask,
+                       out, partial);
 }
 void bt_convert_masked(hipStream_t s, Tiles rt, int k, const double *src, const int *keep, float *dst) {
     hipLaunchKernelGGL(kb_convert_masked, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, k, src, keep, dst);
 }
 void bt_accumulate(hipStream_t s, Tiles rt, int k, const float *d, double *x) {
     hipLaunchKernelGGL(kb_accumulate, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, k, d, x);
+}
+void bt_reliable_init(hipStream_t s, int gk, PcgScalars sc, double *rr_ref, int *flag, int *status) {
+    hipLaunchKernelGGL(kb_reliable_init, dim3(bt_grid(gk, 256)), dim3(256), 0, s, gk, sc, rr_ref, flag, status);
+}
+void bt_reliable_check(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, int rule, double tol,
+                       double delta2, PcgScalars sc, const double *rr_ref, int *flag, int *status) {
+    hipLaunchKernelGGL(kb_reliable_check, dim3(gk), dim3(64), 0, s, k, gtile, p, rule, tol, delta2, sc, rr_ref, flag,
+                       status);
+}
+void bt_reliable_fold(hipStream_t s, Tiles rt, int k, const int *flag, const double *alpha, const float *P,
+                      float *X32, double *X64) {
+    hipLaunchKernelGGL(kb_reliable_fold, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, k, flag, alpha, P, X32, X64);
+}
+void bt_reliable_apply(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, int rule, double tol,
+                       PcgScalars sc, double *rr_ref, int *flag, int *keep, int *status) {
+    hipLaunchKernelGGL(kb_reliable_apply, dim3(gk), dim3(64), 0, s, k, gtile, p, rule, tol, sc, rr_ref, flag, keep,
+                       status);
+}
+void bt_reliable_residual(hipStream_t s, Tiles rt, const int32_t *rowptr, const int32_t *col, const double *val,
+                          int k, const int *flag, const float *X, double *R, double *partial) {
+    hipLaunchKernelGGL(kb_reliable_residual, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, rowptr, col, val, k, flag,
+                       X, R, partial);
+}
+void bt_reliable_replace(hipStream_t s, Tiles rt, int k, const double *src, const int *keep, float *dst,
+                         float *X32) {
+    hipLaunchKernelGGL(kb_reliable_replace, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, k, src, keep, dst, X32);
 }
 // New diagonal of a sampled Laplacian, as Triplet::insert / process_diagonal
 // / to_csc and the sparse add build it (src/sparsifier.rs:48-87, :645-691):

@@ -62,6 +62,7 @@ void ss_solve_opts_default(ss_solve_opts *o) {
     // rounds order: its level-scheduled solves are shallow (depth 147 on a
     // 128^3 grid, where the natural order's is about 1M)
     o->ordering = SS_ORDER_ROUNDS;
+    o->mixed_mode = SS_MIXED_RELIABLE;
 }
 
 // ---- context ---------------------------------------------------------------
@@ -83,7 +84,7 @@ int ss_context_create(int device, ss_context **out) {
         return fail(SS_ERR_HIP, "hipStreamCreate failed");
     }
     for (auto &ev : c->ev) (void)hipEventCreate(&ev);
-    if (hipHostMalloc((void **)&c->h_status, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess ||
+    if (hipHostMalloc((void **)&c->h_status, 3 * sizeof(int), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void **)&c->h_noalloc, 2 * sizeof(int64_t), hipHostMallocDefault) != hipSuccess) {
         if (c->h_status) (void)hipHostFree(c->h_status);
     if (c->h_noalloc) (void)hipHostFree(c->h_noalloc);
@@ -342,7 +343,7 @@ static float elapsed(hipEvent_t a, hipEvent_t b) {
 
 static bool same_solve(const ss_solve_opts &a, const ss_solve_opts &b) {
     return a.stop_rule == b.stop_rule && a.max_iter == b.max_iter && a.tol == b.tol && a.precision == b.precision &&
-           a.fail_relres == b.fail_relres && a.inner_tol == b.inner_tol;
+           a.fail_relres == b.fail_relres && a.inner_tol == b.inner_tol && a.mixed_mode == b.mixed_mode;
 }
 
 // Sketch -> solve -> sample over the whole batch (opts has G entries).

@@ -22,7 +22,7 @@ struct ss_context {
     ss_tuning_state tune;
     hipStream_t stream = nullptr;
     hipEvent_t ev[8] = {};
-    int *h_status = nullptr;      // pinned: {nactive, err}
+    int *h_status = nullptr;      // pinned: {nactive, err, flagged}
     int64_t *h_noalloc = nullptr; // pinned: {kept, err} of ss_er_sample_noalloc
     int live_graphs = 0;          // graphs created on this context and not yet destroyed
 };
@@ -163,7 +163,7 @@ struct ss_graph {
     void *core_tc = nullptr, *core_part = nullptr;
     ss_schedule_info sched_info[2] = {};
     // status
-    int *status = nullptr;           // {nactive, err}
+    int *status = nullptr;           // {nactive, err, flagged (reliable updates)}
     // work (allocated for wk columns at wprec)
     int wk = 0, wcap = 0, wprec = -1;   // columns in use, columns allocated, precision
     double *Y64 = nullptr, *X64 = nullptr, *R64 = nullptr, *B64 = nullptr;
@@ -175,6 +175,8 @@ struct ss_graph {
     double *sc_buf = nullptr;
     int *sc_int = nullptr, *colkeep = nullptr;
     ss::PcgScalars sc{};
+    double *rel_ref = nullptr;       // per (graph, column): |r|^2 at the last reliable update
+    int *rel_flag = nullptr;         // per (graph, column): replacement due
     // per-call parameters (per graph)
     uint64_t *p_colseed = nullptr;   // per (graph, column), G x kMaxK: sketch seed,
     double *p_colscale = nullptr;    // scale,

@@ -303,7 +303,7 @@ int batch_upload(ss_context *ctx, const std::vector<const HostGraph *> &graphs, 
     SS_TRY(A.alloc(&g->cptr, cptr.size()));
     SS_TRY(A.alloc(&g->cvert, cvert.size()));
     SS_TRY(A.alloc(&g->small_id, small_id.size()));
-    SS_TRY(A.alloc(&g->status, 2));
+    SS_TRY(A.alloc(&g->status, 3));
     SS_TRY(A.alloc(&g->p, M));
     SS_TRY(A.alloc(&g->dnorm, M));
     SS_TRY(A.alloc(&g->mask, M));
@@ -345,7 +345,7 @@ int batch_upload(ss_context *ctx, const std::vector<const HostGraph *> &graphs, 
     SS_TRY(h2d(s, g->cptr, cptr.data(), cptr.size()));
     SS_TRY(h2d(s, g->cvert, cvert.data(), cvert.size()));
     SS_TRY(h2d(s, g->small_id, small_id.data(), small_id.size()));
-    SS_HIP(hipMemsetAsync(g->status, 0, sizeof(int) * 2, s));
+    SS_HIP(hipMemsetAsync(g->status, 0, sizeof(int) * 3, s));
     SS_HIP(hipMemsetAsync(g->tile_state, 0, sizeof(uint64_t) * ((size_t)g->net + 1), s));
     SS_HIP(hipStreamSynchronize(s));
     g->kept_start.assign((size_t)G, 0);
@@ -696,6 +696,8 @@ static void bind_scalars(ss_graph *g, int k) {
     g->sc.bnorm = g->sc_buf + 4 * gk;
     g->sc.active = g->sc_int;
     g->sc.iters = g->sc_int + gk;
+    g->rel_ref = g->sc_buf + 5 * gk;
+    g->rel_flag = g->sc_int + 2 * gk;
 }
 
 size_t graph_work_bytes(const ss_graph *g, int k, int prec) {
@@ -706,7 +708,7 @@ size_t graph_work_bytes(const ss_graph *g, int k, int prec) {
     b += prec == SS_PREC_FP64 ? nk * 8 * 6 : nk * (8 * 2 + 4 * 2 + 4 * 6);
     b += ((size_t)std::max(std::max(g->nrt, g->nst), 1) + (size_t)std::max(g->nrt, 1)) * cap * 8 + gk * 8 +
          (size_t)g->ncomp * cap * 8 * 2;
-    b += gk * (5 * 8 + 2 * 4 + 4);
+    b += gk * (6 * 8 + 3 * 4 + 4);
     // the dense cores' gathered block and product pieces
     b += ((size_t)g->core.rows + (size_t)g->core.pieces * kCoreTile) * cap * (prec == SS_PREC_FP64 ? 8 : 4);
     return b;
@@ -762,8 +764,8 @@ int graph_ensure_work(ss_graph *g, int k, int prec) {
     SS_TRY(A.alloc(&g->gsum, gk));
     SS_TRY(A.alloc(&g->csum, (size_t)g->ncomp * cap));
     SS_TRY(A.alloc(&g->cmean, (size_t)g->ncomp * cap));
-    SS_TRY(A.alloc(&g->sc_buf, 5 * gk));
-    SS_TRY(A.alloc(&g->sc_int, 2 * gk));
+    SS_TRY(A.alloc(&g->sc_buf, 6 * gk));
+    SS_TRY(A.alloc(&g->sc_int, 3 * gk));
     SS_TRY(A.alloc(&g->colkeep, gk));
     g->wk = k;
     g->wcap = cap;
@@ -863,16 +865,17 @@ static int precond(ss_graph *g, int k, const T *R, T *W, T *Zp, double *rz_parti
     return SS_OK;
 }
 
-static int read_status(ss_graph *g, int *nactive) {
+static int read_status(ss_graph *g, int *nactive, int *nflagged = nullptr) {
     ss_context *c = g->ctx;
-    SS_HIP(hipMemcpyAsync(c->h_status, g->status, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    SS_HIP(hipMemcpyAsync(c->h_status, g->status, 3 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     SS_HIP(hipStreamSynchronize(c->stream));
     if (c->h_status[1] != 0) {
-        (void)hipMemsetAsync(g->status, 0, 2 * sizeof(int), c->stream);
+        (void)hipMemsetAsync(g->status, 0, 3 * sizeof(int), c->stream);
         (void)hipStreamSynchronize(c->stream);
         return fail(SS_ERR_TIMEOUT, "device wait exceeded its spin limit (compaction look-back)");
     }
     if (nactive) *nactive = c->h_status[0];
+    if (nflagged) *nflagged = c->h_status[2];
     return SS_OK;
 }
 
@@ -889,8 +892,11 @@ static void component_means(ss_graph *g, int k, const double *Y) {
 // the previous iteration's X += alpha P; Q = A P; alpha; R -= alpha Q; stop
 // test.  With tail, also the next preconditioner application and beta
 // (what the next iteration needs).  The solve ends with the last X update.
+// With delta2 >= 0 (reliable updates), the check flags columns for a
+// residual replacement instead of stopping them.
 template <typename T>
-static int pcg_iteration(ss_graph *g, int k, T *X, const PcgWork<T> &w, int rule, double tol, bool tail) {
+static int pcg_iteration(ss_graph *g, int k, T *X, const PcgWork<T> &w, int rule, double tol, bool tail,
+                         double delta2 = -1.0) {
     hipStream_t s = g->ctx->stream;
     const Tiles rt = rtiles(g);
     const int gk = g->G * k;
@@ -903,12 +909,16 @@ static int pcg_iteration(ss_graph *g, int k, T *X, const PcgWork<T> &w, int rule
         ProfScope ps(g, SS_KERNEL_SPMM);
         bt_spmm_dot<T>(s, stiles(g), g->rowptr, g->col, avals<T>(g), k, w.P, w.Q, g->partial);   // Q = A P, P.Q
     }
-    bt_pcg_alpha(s, gk, k, g->sgtile, g->partial, g->sc, g->status);
+    bt_pcg_alpha(s, gk, k, g->sgtile, g->partial, g->sc, g->status, delta2 >= 0.0 ? g->rel_flag : nullptr);
     {
         ProfScope ps(g, SS_KERNEL_VECTOR);
         bt_update_r_rr<T>(s, rt, k, g->sc.alpha, w.Q, w.R, g->partial);
     }
-    bt_pcg_check(s, gk, k, g->gtile, g->partial, rule, tol, g->sc, g->status);
+    if (delta2 >= 0.0)
+        bt_reliable_check(s, gk, k, g->gtile, g->partial, rule, tol, delta2, g->sc, g->rel_ref, g->rel_flag,
+                          g->status);
+    else
+        bt_pcg_check(s, gk, k, g->gtile, g->partial, rule, tol, g->sc, g->status);
     if (tail) {
         SS_TRY(precond<T>(g, k, w.R, w.W, w.Zp, g->partial2));
         bt_pcg_beta(s, gk, k, g->gtile, g->partial2, g->sc);
@@ -1016,6 +1026,78 @@ static int pcg_run(ss_graph *g, int k, const T *rhs, const double *mean_src, T *
     return SS_OK;
 }
 
+// Mixed precision with reliable updates (FP64 B64 projected, X64 = 0 and
+// R64 = B64 on entry; bn2 = |B64|^2 per (graph, column)).  One FP32 PCG
+// (FP32 SpMM and triangular solves, FP64 dots and scalars) runs on
+// (float) B64; a column whose recursive residual has fallen to delta^2
+// times its reference (|r|^2 at its last replacement) or to the stop
+// target folds its FP32 iterate x into X64 and replaces its residual by the
+// FP64 true residual B64 - A X64 (kept current in R64 as R64 - A x, so the
+// replacement gathers FP32 rows), keeping its search direction unless the
+// true residual jumped above the recursive one; it stops when that true
+// residual meets the stop rule.  Unlike restarting the FP32 PCG after every
+// refinement, the Krylov space built so far is kept.  rounds = replacement
+// rounds (launches of the FP64 residual).
+static int pcg_reliable(ss_graph *g, int k, int rule, double tol, double delta, int max_iter,
+                        const std::vector<double> &bn2, std::vector<int32_t> &iters, int &rounds) {
+    hipStream_t s = g->ctx->stream;
+    const Tiles rt = rtiles(g);
+    const int gk = g->G * k;
+    const PcgWork<float> &w = g->w32;
+    const double delta2 = delta * delta;
+    {
+        ProfScope ps(g, SS_KERNEL_VECTOR);
+        bt_convert_masked(s, rt, k, g->R64, nullptr, g->Y32);
+        launch_fill_u32(s, (int64_t)2 * g->ncomp * k, reinterpret_cast<uint32_t *>(g->cmean), 0u);
+        bt_project<float, float>(s, rt, k, g->Y32, g->comp_of_row, g->cmean, w.R, w.B, g->X32);
+    }
+    SS_TRY(precond<float>(g, k, w.R, w.W, w.Zp, g->partial2));
+    bt_tile_partial<float, float>(s, rt, k, w.B, w.B, nullptr, 0, g->partial);
+    bt_pcg_init(s, gk, k, g->gtile, g->partial, g->partial2, g->sc);
+    SS_HIP(hipMemcpyAsync(g->rel_ref, bn2.data(), sizeof(double) * gk, hipMemcpyHostToDevice, s));
+    bt_reliable_init(s, gk, g->sc, g->rel_ref, g->rel_flag, g->status);
+    rounds = 0;
+    for (int it = 0; it < max_iter; it++) {
+        SS_TRY(pcg_iteration<float>(g, k, g->X32, w, rule, tol, false, delta2));
+        int nactive = 0, nflag = 0;
+        SS_TRY(read_status(g, &nactive, &nflag));
+        if (nflag > 0) {
+            rounds++;
+            {
+                ProfScope ps(g, SS_KERNEL_VECTOR);
+                bt_reliable_fold(s, rt, k, g->rel_flag, g->sc.alpha, w.P, g->X32, g->X64);
+            }
+            {
+                ProfScope ps(g, SS_KERNEL_SPMM);
+                bt_reliable_residual(s, stiles(g), g->rowptr, g->col, g->val, k, g->rel_flag, g->X32, g->R64,
+                                     g->partial);
+            }
+            bt_reliable_apply(s, gk, k, g->sgtile, g->partial, rule, tol, g->sc, g->rel_ref, g->rel_flag,
+                              g->colkeep, g->status);
+            {
+                ProfScope ps(g, SS_KERNEL_VECTOR);
+                bt_reliable_replace(s, rt, k, g->R64, g->colkeep, w.R, g->X32);
+            }
+            SS_TRY(read_status(g, &nactive));
+        }
+        if (nactive == 0) break;
+        SS_TRY(precond<float>(g, k, w.R, w.W, w.Zp, g->partial2));
+        bt_pcg_beta(s, gk, k, g->gtile, g->partial2, g->sc);
+    }
+    {
+        // what the FP32 iterate holds past the last replacement (nothing
+        // for the columns that stopped: alpha is zero and X32 was cleared)
+        ProfScope ps(g, SS_KERNEL_VECTOR);
+        bt_update_x<float>(s, rt, k, g->sc.alpha, w.P, g->X32);
+        bt_accumulate(s, rt, k, g->X32, g->X64);
+    }
+    SS_TRY(read_status(g, nullptr));
+    iters.resize(gk);
+    SS_HIP(hipMemcpyAsync(iters.data(), g->sc.iters, sizeof(int32_t) * gk, hipMemcpyDeviceToHost, s));
+    SS_HIP(hipStreamSynchronize(s));
+    return SS_OK;
+}
+
 // |v|^2 per (graph, column) of an FP64 block, on the host.
 static int block_norms2(ss_graph *g, int k, const double *V, std::vector<double> &out) {
     hipStream_t s = g->ctx->stream;
@@ -1066,9 +1148,19 @@ int graph_solve(ss_graph *g, int k, const ss_solve_opts &o, SolveStats &st) {
         component_means(g, k, g->Y64);
         bt_project<double, double>(s, rt, k, g->Y64, g->comp_of_row, g->cmean, g->R64, g->B64, g->X64);
         SS_TRY(block_norms2(g, k, g->B64, bn2));
-        // Each FP32 sweep cuts the FP64 residual by ~1e-3 (well above FP32's
-        // stagnation level); sweeps repeat until the FP64 stop rule holds.
         if (!(o.inner_tol >= 0.0 && o.inner_tol < 1.0)) return fail(SS_ERR_INVALID, "inner_tol %g outside [0, 1)", o.inner_tol);
+        if (o.mixed_mode != SS_MIXED_RELIABLE && o.mixed_mode != SS_MIXED_RESTART)
+            return fail(SS_ERR_INVALID, "unknown mixed_mode %d", o.mixed_mode);
+        if (o.mixed_mode == SS_MIXED_RELIABLE) {
+            SS_TRY(pcg_reliable(g, k, o.stop_rule, o.tol, o.inner_tol > 0.0 ? o.inner_tol : 3e-3, o.max_iter, bn2,
+                                st.iters, st.refinements));
+            SS_TRY(residual_norms(g, k, g->X64, g->B64, g->R64, rn2));
+            for (int i = 0; i < gk; i++) st.relres[i] = bn2[i] > 0.0 ? std::sqrt(rn2[i] / bn2[i]) : 0.0;
+            return SS_OK;
+        }
+        // Restarts: each FP32 sweep cuts the FP64 residual by ~1e-3 (well
+        // above FP32's stagnation level); sweeps repeat until the FP64 stop
+        // rule holds.
         const double inner_tol = std::max(o.inner_tol > 0.0 ? o.inner_tol : 1e-3, o.stop_rule == SS_STOP_REL ? o.tol : 0.0);
         const int inner_max = std::min(o.max_iter, 300);
         std::vector<int32_t> it_sweep, keep(gk);
