@@ -122,6 +122,7 @@ def kernel_bytes(n, m, G, k, kept, s=8):
         # ER/sample: edges (u, v, w), Z (FP64) once, p out, mask out, kept (u, v, w) out
         "er_sample": m * (4 + 4 + 8) + n * k * 8 + 8 * m + m + kept * 16,
         "vector": None,
+        "residual": None,
     }
 
 

@@ -335,7 +335,7 @@ int ss_batch_read_probs(const ss_graph *g, int32_t i, double *p);
 /* Per-kernel device time, measured with HIP events on the graph's stream
  * around every launch while enabled (enabling clears the counters). */
 #define SS_KERNEL_SKETCH 0      /* (a) sketch                                */
-#define SS_KERNEL_SPMM 1        /* (b) Laplacian SpMM (+ p.q) and residuals  */
+#define SS_KERNEL_SPMM 1        /* (b) Laplacian SpMM (+ p.q)                */
 #define SS_KERNEL_TRI_FWD 2     /* (c) one level of the forward solve        */
 #define SS_KERNEL_TRI_BWD 3     /* (c) one level of the backward solve       */
 #define SS_KERNEL_ER_SAMPLE 4   /* (d) ER + coin + sample + compaction       */
@@ -346,7 +346,8 @@ int ss_batch_read_probs(const ss_graph *g, int32_t i, double *p);
                                    inverse product, reduction               */
 #define SS_KERNEL_CORE_BWD 9    /* dense cores, backward: gather of y/D,
                                    transposed inverse product, reduction    */
-#define SS_KERNEL_COUNT 10
+#define SS_KERNEL_RESIDUAL 10   /* FP64 true residuals b - A x (refinement)  */
+#define SS_KERNEL_COUNT 11
 typedef struct ss_kernel_stat {
     double ms_total;
     int64_t launches;

@@ -18,6 +18,8 @@ SS_ERR_TIMEOUT = 5
 SS_ERR_CAPACITY = 6
 PREC_FP64 = 0
 PREC_MIXED = 1
+MIXED_RELIABLE = 0
+MIXED_RESTART = 1
 STOP_REF_ABS = 0
 STOP_REL = 1
 ORDER_NATURAL = 0
@@ -77,7 +79,7 @@ class HostParams(ctypes.Structure):
 
 
 KERNELS = ["sketch", "spmm", "tri_fwd", "tri_bwd", "er_sample", "tri_fwd_small", "tri_bwd_small", "vector",
-           "core_fwd", "core_bwd"]
+           "core_fwd", "core_bwd", "residual"]
 
 
 class ScheduleInfo(ctypes.Structure):

@@ -1068,7 +1068,7 @@ static int pcg_reliable(ss_graph *g, int k, int rule, double tol, double delta, 
                 bt_reliable_fold(s, rt, k, g->rel_flag, g->sc.alpha, w.P, g->X32, g->X64);
             }
             {
-                ProfScope ps(g, SS_KERNEL_SPMM);
+                ProfScope ps(g, SS_KERNEL_RESIDUAL);
                 bt_reliable_residual(s, stiles(g), g->rowptr, g->col, g->val, k, g->rel_flag, g->X32, g->R64,
                                      g->partial);
             }
@@ -1115,7 +1115,7 @@ static int residual_norms(ss_graph *g, int k, const double *X, const double *B, 
     hipStream_t s = g->ctx->stream;
     const int gk = g->G * k;
     {
-        ProfScope ps(g, SS_KERNEL_SPMM);
+        ProfScope ps(g, SS_KERNEL_RESIDUAL);
         bt_residual64(s, stiles(g), g->rowptr, g->col, g->val, k, X, B, R, g->partial);
     }
     bt_graph_sums(s, gk, k, g->sgtile, g->partial, g->gsum);

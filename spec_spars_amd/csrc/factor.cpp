@@ -350,18 +350,28 @@ static int factor_rounds(int64_t n, const int32_t *colptr, const int32_t *rowidx
                 double val;
             };
             std::vector<TailFill> tf;
+            // SPEC_SPARS_VERBOSE >= 2: time of the row scans, eliminations
+            // and fill additions
+            double t_scan = 0, t_elim = 0, t_fill = 0;
+            auto tick = [&]() { return trace ? std::chrono::steady_clock::now() : std::chrono::steady_clock::time_point(); };
+            auto secs = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+                return std::chrono::duration<double>(b - a).count();
+            };
             for (int32_t v : cand) {
                 order.push_back(v);
                 const int32_t tv = tid[v];
                 const double *row = W.data() + (size_t)tv * L;
+                const auto c0 = tick();
                 P.work.clear();
                 for (int64_t t = 0; t < L; t++)
                     if (row[t] != 0.0 && !tdone[t]) P.work.push_back({live[t], row[t]});
+                const auto c1 = tick();
                 ColumnRng rng(seed, v);
                 tf.clear();
                 vdiag[v] = eliminate_merged(P.work, P.cum, rng, P.col, [&](int32_t a, int32_t b, double fv) {
                     tf.push_back({tid[a], tid[b], fv});
                 });
+                const auto c2 = tick();
                 // one elimination adds to each entry of W at most once (its
                 // fill pairs are distinct), so the additions may go in any
                 // order: threads take the rows of their range
@@ -375,6 +385,12 @@ static int factor_rounds(int64_t n, const int32_t *colptr, const int32_t *rowidx
                         if (f.b >= lo && f.b < hi) w[(size_t)f.b * L + f.a] += f.val;
                     }
                 }, fact);
+                if (trace) {
+                    const auto c3 = tick();
+                    t_scan += secs(c0, c1);
+                    t_elim += secs(c1, c2);
+                    t_fill += secs(c2, c3);
+                }
                 vstart[v] = (int64_t)crow.size();
                 vcount[v] = (int64_t)P.col.size();
                 for (const Ent &e : P.col) { crow.push_back(e.row); cval.push_back(e.val); }
@@ -384,6 +400,9 @@ static int factor_rounds(int64_t n, const int32_t *colptr, const int32_t *rowidx
             }
             tail_rows = (int64_t)cand.size();
             live.clear();
+            if (trace)
+                std::fprintf(stderr, "[factor] dense tail: row scans %.1f s, eliminations %.1f s, fill additions %.1f s\n",
+                             t_scan, t_elim, t_fill);
             if (verbose) {
                 std::fprintf(stderr, "[factor] dense tail of %lld vertices from round %d: %.1f s\n",
                              (long long)tail_rows, rounds - (int32_t)tail_rows,

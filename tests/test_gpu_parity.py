@@ -3,6 +3,7 @@ import math
 
 import numpy as np
 import pytest
+import scipy.sparse as sp
 
 from conftest import components, golden
 
@@ -121,6 +122,34 @@ def test_solve_mixed_precision(ctx, oracle_mod):
     assert info["max_relres"] <= 1e-8
     assert info["refinements"] >= 1
     assert _rel_centered(Z, Zo).max() < 1e-5
+
+
+@pytest.mark.parametrize("kind", ["grid3d", "rgg"])
+def test_mixed_reliable_and_restart_agree(ctx, oracle_mod, kind):
+    """Both mixed-precision schemes (reliable updates, the default, and
+    restarted FP32 sweeps) reach the FP64 tolerance and agree with the FP64
+    oracle; reliable updates keep the Krylov space, so they take no more
+    iterations than the restarts, and every inner_tol gives the same
+    tolerance."""
+    import spec_spars_amd as ss
+    n, u, v, w = _graph(kind)
+    k = 12
+    Y = ctx.sketch(u, v, w, n, k, seed=4, scale=1.0 / math.sqrt(k))
+    cp, ri, va, Zo, it = _oracle_solution(oracle_mod, n, u, v, w, Y, 3, 1e-12)
+    out = {}
+    for mode, inner in ((ss.MIXED_RELIABLE, 0.0), (ss.MIXED_RELIABLE, 0.1), (ss.MIXED_RELIABLE, 1e-4),
+                        (ss.MIXED_RESTART, 0.0)):
+        Z, info = ctx.solve_lap(Y, cp, ri, va, ss.solve_opts(factor_seed=3, stop_rule=ss.STOP_REL, tol=1e-8,
+                                                             precision=ss.PREC_MIXED, max_iter=2000,
+                                                             inner_tol=inner, mixed_mode=mode))
+        L = sp.csc_matrix((va, ri, cp), shape=(n, n))
+        Yp = Y - Y.mean(0)
+        rr = np.linalg.norm(Yp - L @ Z, axis=0) / np.linalg.norm(Yp, axis=0)
+        assert rr.max() <= 1.01e-8, (mode, inner, rr.max())
+        assert info["max_relres"] <= 1e-8
+        assert _rel_centered(Z, Zo).max() < 1e-5
+        out[(mode, inner)] = info["max_iters"]
+    assert out[(ss.MIXED_RELIABLE, 0.0)] <= out[(ss.MIXED_RESTART, 0.0)], out
 
 
 def test_solve_not_converged_status(ctx, oracle_mod):
