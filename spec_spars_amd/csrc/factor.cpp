@@ -49,6 +49,11 @@ struct ColumnRng {
         s += 0x9e3779b97f4a7c15ULL;
         return (double)(mix64(s) >> 11) * (1.0 / 9007199254740992.0);
     }
+    // The value the (i+1)-th call of uniform() returns, without the calls
+    // before it (the stream is counter-based).
+    double at(uint64_t i) const {
+        return (double)(mix64(s + (i + 1) * 0x9e3779b97f4a7c15ULL) >> 11) * (1.0 / 9007199254740992.0);
+    }
 };
 
 struct Ent {
@@ -231,6 +236,10 @@ static int factor_rounds(int64_t n, const int32_t *colptr, const int32_t *rowidx
     std::vector<int64_t> vcount((size_t)n, 0);
     crow.clear();
     cval.clear();
+    // the factor holds about 2-3x the Laplacian's entries: room reserved up
+    // front (untouched pages cost nothing) spares the copies of regrowth
+    crow.reserve((size_t)colptr[n] * 3);
+    cval.reserve((size_t)colptr[n] * 3);
     vdiag.assign((size_t)n, 0.0);
     rounds = 0;
     tail_rows = 0;
@@ -257,7 +266,17 @@ static int factor_rounds(int64_t n, const int32_t *colptr, const int32_t *rowidx
     auto last_report = std::chrono::steady_clock::now();
     const auto t_start = last_report;
     int64_t last_sel = (int64_t)n;
+    std::vector<int64_t> lsplit;
+    // SPEC_SPARS_VERBOSE >= 2: seconds per step kind, summed over rounds
+    double st[7] = {0, 0, 0, 0, 0, 0, 0};
+    auto lap = [&](int i, std::chrono::steady_clock::time_point &a) {
+        if (!trace) return;
+        const auto b = std::chrono::steady_clock::now();
+        st[i] += std::chrono::duration<double>(b - a).count();
+        a = b;
+    };
     while (!live.empty()) {
+        auto tl = trace ? std::chrono::steady_clock::now() : std::chrono::steady_clock::time_point();
         if (verbose && std::chrono::steady_clock::now() - last_report > std::chrono::seconds(20)) {
             last_report = std::chrono::steady_clock::now();
             std::fprintf(stderr, "[factor] round %d: %zu of %lld vertices left, %zu factor entries\n", rounds,
@@ -267,10 +286,23 @@ static int factor_rounds(int64_t n, const int32_t *colptr, const int32_t *rowidx
         const int64_t L = (int64_t)live.size();
         const int active = L >= 4096 ? T : 1;
         const uint64_t rkey = mix64(seed ^ (0x94d049bb133111ebULL * (uint64_t)(rounds + 1)));
-        // 1. drop eliminated neighbours, key = (live degree, random)
+        // 1. drop eliminated neighbours, key = (live degree, random); the
+        // live set is split by list lengths, not by count (the hubs, which
+        // hold most entries, have low ids)
+        lsplit.assign((size_t)active + 1, L);
+        lsplit[0] = 0;
+        if (active > 1) {
+            int64_t total = 0;
+            for (int64_t x = 0; x < L; x++) total += 1 + (int64_t)adj[live[x]].size();
+            int64_t run = 0;
+            int t = 1;
+            for (int64_t x = 0; x < L && t < active; x++) {
+                run += 1 + (int64_t)adj[live[x]].size();
+                while (t < active && run >= total * t / active) lsplit[(size_t)t++] = x + 1;
+            }
+        }
         pool.run([&](int t) {
-            int64_t lo, hi;
-            HostPool::range(L, active, t, lo, hi);
+            const int64_t lo = lsplit[(size_t)t], hi = lsplit[(size_t)t + 1];
             uint64_t md = ~0ULL, sd = 0;
             for (int64_t x = lo; x < hi; x++) {
                 const int32_t v = live[x];
@@ -287,6 +319,7 @@ static int factor_rounds(int64_t n, const int32_t *colptr, const int32_t *rowidx
             pt[t].mindeg = md;
             pt[t].sumdeg = sd;
         }, active);
+        lap(0, tl);
         uint64_t mindeg = ~0ULL, sumdeg = 0;
         for (int t = 0; t < active; t++) {
             mindeg = std::min(mindeg, pt[t].mindeg);
@@ -328,7 +361,6 @@ static int factor_rounds(int64_t n, const int32_t *colptr, const int32_t *rowidx
             cand.assign(live.begin(), live.end());
             std::sort(cand.begin(), cand.end(),
                       [&](int32_t x, int32_t y) { return key[x] != key[y] ? key[x] < key[y] : x < y; });
-            PerThread &P = pt[0];
             // the tail graph as a dense symmetric matrix over tail ids (in
             // vertex order): its live entries accumulate fills in
             // elimination order, and a row scan lists a vertex's live
@@ -345,64 +377,150 @@ static int factor_rounds(int64_t n, const int32_t *colptr, const int32_t *rowidx
                 }
             }, active);
             std::vector<char> tdone((size_t)L, 0);
+            // One elimination per step, every step on all threads, phases
+            // separated by spin barriers (a step is tens of microseconds):
+            //   A  each thread scans its slice of the vertex's W row (live
+            //      entries, rows ascending) and sorts a copy by (value, row);
+            //   M  the sorted slices merge pairwise, a tree of log2 T levels;
+            //   C  thread 0: prefix sums, D, the factor column's place;
+            //   S  each thread samples the partners of its slice of entries
+            //      (draw i is the i-th of the vertex's stream, computed
+            //      directly) and writes its slice of the factor column;
+            //   F  each thread adds the fills that land in its rows of W.
+            // The factor is the one the sequential loop gives: the same
+            // sort (a total order), sums in entry order, draws and fills.
+            const int TT = pool.size();
+            SpinBarrier bar(TT);
+            std::vector<std::vector<Ent>> scan((size_t)TT), run((size_t)TT);
+            std::vector<int64_t> soff((size_t)TT + 1, 0);
+            std::vector<Ent> mA, mB;
+            std::vector<double> cum;
             struct TailFill {
                 int32_t a, b;   // tail ids
                 double val;
             };
             std::vector<TailFill> tf;
-            // SPEC_SPARS_VERBOSE >= 2: time of the row scans, eliminations
-            // and fill additions
-            double t_scan = 0, t_elim = 0, t_fill = 0;
-            auto tick = [&]() { return trace ? std::chrono::steady_clock::now() : std::chrono::steady_clock::time_point(); };
-            auto secs = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
-                return std::chrono::duration<double>(b - a).count();
-            };
-            for (int32_t v : cand) {
-                order.push_back(v);
-                const int32_t tv = tid[v];
-                const double *row = W.data() + (size_t)tv * L;
-                const auto c0 = tick();
-                P.work.clear();
-                for (int64_t t = 0; t < L; t++)
-                    if (row[t] != 0.0 && !tdone[t]) P.work.push_back({live[t], row[t]});
-                const auto c1 = tick();
-                ColumnRng rng(seed, v);
-                tf.clear();
-                vdiag[v] = eliminate_merged(P.work, P.cum, rng, P.col, [&](int32_t a, int32_t b, double fv) {
-                    tf.push_back({tid[a], tid[b], fv});
-                });
-                const auto c2 = tick();
-                // one elimination adds to each entry of W at most once (its
-                // fill pairs are distinct), so the additions may go in any
-                // order: threads take the rows of their range
-                const int fact = tf.size() >= 2048 ? T : 1;
-                pool.run([&](int t) {
-                    int64_t lo, hi;
-                    HostPool::range(L, fact, t, lo, hi);
+            double tot = 0.0;
+            int64_t cnt = 0, cbase = 0;
+            const Ent *sorted = nullptr;
+            // SPEC_SPARS_VERBOSE >= 2: time of the phases (thread 0's view)
+            double t_scan = 0, t_sample = 0, t_fill = 0;
+            using clk = std::chrono::steady_clock;
+            auto secs = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
+            pool.run([&](int t) {
+                int64_t clo, chi, rlo, rhi;
+                HostPool::range(L, TT, t, clo, chi);   // W columns scanned
+                HostPool::range(L, TT, t, rlo, rhi);   // W rows updated
+                for (const int32_t v : cand) {
+                    const int32_t tv = tid[v];
+                    const double *row = W.data() + (size_t)tv * L;
+                    const clk::time_point c0 = trace && t == 0 ? clk::now() : clk::time_point();
+                    // A
+                    auto &sl = scan[t];
+                    sl.clear();
+                    for (int64_t x = clo; x < chi; x++)
+                        if (row[x] != 0.0 && !tdone[x]) sl.push_back({live[x], row[x]});
+                    run[t].assign(sl.begin(), sl.end());
+                    std::sort(run[t].begin(), run[t].end(), [](const Ent &a, const Ent &b) {
+                        return a.val != b.val ? a.val < b.val : a.row < b.row;
+                    });
+                    bar.wait();
+                    // M: the slices' offsets, then pairwise merges
+                    if (t == 0) {
+                        soff[0] = 0;
+                        for (int u = 0; u < TT; u++) soff[u + 1] = soff[u] + (int64_t)scan[u].size();
+                        cnt = soff[TT];
+                        if ((int64_t)mA.size() < cnt) {
+                            mA.resize((size_t)cnt);
+                            mB.resize((size_t)cnt);
+                        }
+                    }
+                    bar.wait();
+                    std::copy(run[t].begin(), run[t].end(), mA.begin() + soff[t]);
+                    bar.wait();
+                    Ent *src = mA.data(), *dst = mB.data();
+                    for (int width = 1; width < TT; width *= 2) {
+                        const int first = 2 * width * t;   // this thread's pair of run groups
+                        if (first < TT) {
+                            const int mid = std::min(first + width, TT), last = std::min(first + 2 * width, TT);
+                            std::merge(src + soff[first], src + soff[mid], src + soff[mid], src + soff[last],
+                                       dst + soff[first], [](const Ent &a, const Ent &b) {
+                                           return a.val != b.val ? a.val < b.val : a.row < b.row;
+                                       });
+                        }
+                        std::swap(src, dst);
+                        bar.wait();
+                    }
+                    const clk::time_point c1 = trace && t == 0 ? clk::now() : clk::time_point();
+                    // C
+                    if (t == 0) {
+                        sorted = src;
+                        order.push_back(v);
+                        cum.resize((size_t)std::max<int64_t>(cnt, 1));
+                        double r = 0.0;
+                        for (int64_t i = 0; i < cnt; i++) { r += sorted[i].val; cum[(size_t)i] = r; }
+                        tot = r;
+                        vdiag[v] = cnt > 0 ? tot : 0.0;
+                        cbase = (int64_t)crow.size();
+                        crow.resize((size_t)(cbase + cnt));
+                        cval.resize((size_t)(cbase + cnt));
+                        vstart[v] = cbase;
+                        vcount[v] = cnt;
+                        tf.resize((size_t)std::max<int64_t>(cnt - 1, 0));
+                    }
+                    bar.wait();
+                    // S: partners of entries [lo, hi) of the cnt - 1 that
+                    // sample one, and this thread's slice of the column
+                    if (cnt > 0) {
+                        const ColumnRng rng(seed, v);
+                        int64_t lo, hi;
+                        HostPool::range(cnt - 1, TT, t, lo, hi);
+                        for (int64_t i = lo; i < hi; i++) {
+                            const double rest = tot - cum[(size_t)i];
+                            const double target = rng.at((uint64_t)i) * rest;
+                            int64_t a = i + 1, b = cnt;
+                            while (a < b) {
+                                const int64_t mid = (a + b) / 2;
+                                if (cum[(size_t)mid] - cum[(size_t)i] < target) a = mid + 1; else b = mid;
+                            }
+                            const int64_t j = a < cnt ? a : cnt - 1;
+                            tf[(size_t)i] = {tid[sorted[i].row], tid[sorted[j].row], sorted[i].val * rest / tot};
+                        }
+                        const int64_t o = cbase + soff[t];
+                        for (size_t i = 0; i < sl.size(); i++) {
+                            crow[(size_t)o + i] = sl[i].row;
+                            cval[(size_t)o + i] = -(sl[i].val / tot);
+                        }
+                    }
+                    bar.wait();
+                    const clk::time_point c2 = trace && t == 0 ? clk::now() : clk::time_point();
+                    // F: one elimination adds to each entry of W at most
+                    // once (its fill pairs are distinct), so the additions
+                    // may go in any order: threads take their rows
                     double *w = W.data();
                     for (const TailFill &f : tf) {
-                        if (f.a >= lo && f.a < hi) w[(size_t)f.a * L + f.b] += f.val;
-                        if (f.b >= lo && f.b < hi) w[(size_t)f.b * L + f.a] += f.val;
+                        if (f.a >= rlo && f.a < rhi) w[(size_t)f.a * L + f.b] += f.val;
+                        if (f.b >= rlo && f.b < rhi) w[(size_t)f.b * L + f.a] += f.val;
                     }
-                }, fact);
-                if (trace) {
-                    const auto c3 = tick();
-                    t_scan += secs(c0, c1);
-                    t_elim += secs(c1, c2);
-                    t_fill += secs(c2, c3);
+                    if (t == 0) {
+                        elim[v] = 1;
+                        tdone[tv] = 1;
+                        rounds++;
+                    }
+                    bar.wait();
+                    if (trace && t == 0) {
+                        const clk::time_point c3 = clk::now();
+                        t_scan += secs(c0, c1);
+                        t_sample += secs(c1, c2);
+                        t_fill += secs(c2, c3);
+                    }
                 }
-                vstart[v] = (int64_t)crow.size();
-                vcount[v] = (int64_t)P.col.size();
-                for (const Ent &e : P.col) { crow.push_back(e.row); cval.push_back(e.val); }
-                elim[v] = 1;
-                tdone[tv] = 1;
-                rounds++;
-            }
+            });
+            if (trace)
+                std::fprintf(stderr, "[factor] dense tail: scans + sorts + merges %.1f s, sampling %.1f s, fill additions %.1f s\n",
+                             t_scan, t_sample, t_fill);
             tail_rows = (int64_t)cand.size();
             live.clear();
-            if (trace)
-                std::fprintf(stderr, "[factor] dense tail: row scans %.1f s, eliminations %.1f s, fill additions %.1f s\n",
-                             t_scan, t_elim, t_fill);
             if (verbose) {
                 std::fprintf(stderr, "[factor] dense tail of %lld vertices from round %d: %.1f s\n",
                              (long long)tail_rows, rounds - (int32_t)tail_rows,
@@ -424,6 +542,7 @@ static int factor_rounds(int64_t n, const int32_t *colptr, const int32_t *rowidx
         for (int t = 0; t < active; t++) cand.insert(cand.end(), pt[t].cand.begin(), pt[t].cand.end());
         std::sort(cand.begin(), cand.end(),
                   [&](int32_t x, int32_t y) { return key[x] != key[y] ? key[x] < key[y] : x < y; });
+        lap(1, tl);
         // 3. greedy maximal independent set (sequential, in key order)
         sel.clear();
         for (int32_t v : cand) {
@@ -433,6 +552,7 @@ static int factor_rounds(int64_t n, const int32_t *colptr, const int32_t *rowidx
         }
         std::sort(sel.begin(), sel.end());
         last_sel = (int64_t)sel.size();
+        lap(2, tl);
         // 4. eliminate the set: thread t takes a contiguous range of sel
         const int64_t S = (int64_t)sel.size();
         const int sact = S >= 256 ? T : 1;
@@ -457,35 +577,41 @@ static int factor_rounds(int64_t n, const int32_t *colptr, const int32_t *rowidx
                 std::vector<Ent>().swap(adj[v]);
             }
         }, sact);
-        // columns in sel order
+        lap(3, tl);
+        // columns in sel order, each thread copying its own
         {
-            int64_t base = (int64_t)crow.size(), total = 0;
-            for (int t = 0; t < sact; t++) total += (int64_t)pt[t].cols.size();
-            crow.resize((size_t)(base + total));
-            cval.resize((size_t)(base + total));
-            for (int t = 0; t < sact; t++) {
+            const int64_t obase = (int64_t)order.size();
+            order.resize((size_t)(obase + S));
+            std::vector<int64_t> cb((size_t)sact + 1);
+            cb[0] = (int64_t)crow.size();
+            for (int t = 0; t < sact; t++) cb[(size_t)t + 1] = cb[(size_t)t] + (int64_t)pt[t].cols.size();
+            crow.resize((size_t)cb[(size_t)sact]);
+            cval.resize((size_t)cb[(size_t)sact]);
+            pool.run([&](int t) {
                 int64_t lo, hi;
                 HostPool::range(S, sact, t, lo, hi);
-                int64_t off = 0;
+                int64_t off = cb[(size_t)t];
                 for (int64_t x = lo; x < hi; x++) {
                     const int32_t v = sel[x];
-                    order.push_back(v);
-                    vstart[v] = base + off;
+                    order[(size_t)(obase + x)] = v;
+                    vstart[v] = off;
                     vcount[v] = pt[t].ccount[x - lo];
                     off += vcount[v];
                 }
-                for (int64_t q = 0; q < (int64_t)pt[t].cols.size(); q++) {
-                    crow[(size_t)(base + q)] = pt[t].cols[q].row;
-                    cval[(size_t)(base + q)] = pt[t].cols[q].val;
+                const auto &cols = pt[t].cols;
+                for (size_t q = 0; q < cols.size(); q++) {
+                    crow[(size_t)cb[(size_t)t] + q] = cols[q].row;
+                    cval[(size_t)cb[(size_t)t] + q] = cols[q].val;
                 }
-                base += (int64_t)pt[t].cols.size();
-            }
+            }, sact);
         }
+        lap(4, tl);
         // 5. fill edges, appended by the owner of their target
         pool.run([&](int o) {
             for (int t = 0; t < sact; t++)
                 for (const Fill3 &f : pt[t].fills[o]) adj[f.target].push_back({f.other, f.val});
         });
+        lap(5, tl);
         // 6. the live set shrinks (order kept)
         pool.run([&](int t) {
             int64_t lo, hi;
@@ -500,8 +626,13 @@ static int factor_rounds(int64_t n, const int32_t *colptr, const int32_t *rowidx
             w += (int64_t)pt[t].next.size();
         }
         live.resize((size_t)w);
+        lap(6, tl);
         rounds++;
     }
+    if (trace)
+        std::fprintf(stderr, "[factor] rounds: degrees+keys %.1f s, candidates %.1f s, greedy set %.1f s, "
+                     "eliminations %.1f s, column copy %.1f s, fill exchange %.1f s, live set %.1f s\n",
+                     st[0], st[1], st[2], st[3], st[4], st[5], st[6]);
     // vstart/vcount are per vertex; pack them as [start, start+count)
     std::vector<int64_t> packed((size_t)n * 2);
     for (int64_t v = 0; v < n; v++) { packed[2 * v] = vstart[v]; packed[2 * v + 1] = vcount[v]; }
@@ -554,9 +685,15 @@ int factor_build(int64_t n, const int32_t *colptr, const int32_t *rowidx, const 
         F.ucol.resize((size_t)tot);
         F.uval.resize((size_t)tot);
         HostPool pool(factor_threads());
+        // positions split by entries, not by count: the dense tail's
+        // columns at the end hold most of them
+        auto entry_split = [&](int t) {
+            const int64_t want = tot * t / pool.size();
+            return (int64_t)(std::lower_bound(F.urowptr.begin(), F.urowptr.begin() + n, (int32_t)want) -
+                             F.urowptr.begin());
+        };
         pool.run([&](int t) {
-            int64_t lo, hi;
-            HostPool::range(n, pool.size(), t, lo, hi);
+            const int64_t lo = entry_split(t), hi = t + 1 == pool.size() ? n : entry_split(t + 1);
             std::vector<Ent> tmp;
             for (int64_t p = lo; p < hi; p++) {
                 const int32_t v = F.order[p];

@@ -4,6 +4,7 @@
 // always by a fixed rule (contiguous ranges, owner = id % size), so results
 // never depend on timing.
 #pragma once
+#include <atomic>
 #include <condition_variable>
 #include <cstdint>
 #include <cstdlib>
@@ -100,6 +101,30 @@ private:
     uint64_t gen_ = 0;
     int active_ = 0, pending_ = 0;
     bool stop_ = false;
+};
+
+// A barrier for the threads of one HostPool::run whose phases are short:
+// the last thread to arrive bumps the generation the others spin on (they
+// yield after a while, in case the machine is oversubscribed).
+class SpinBarrier {
+public:
+    explicit SpinBarrier(int n) : n_(n) {}
+    void wait() {
+        if (n_ <= 1) return;
+        const unsigned g = gen_.load(std::memory_order_acquire);
+        if (count_.fetch_add(1, std::memory_order_acq_rel) == n_ - 1) {
+            count_.store(0, std::memory_order_relaxed);
+            gen_.fetch_add(1, std::memory_order_acq_rel);
+            return;
+        }
+        for (int spins = 0; gen_.load(std::memory_order_acquire) == g;)
+            if (++spins > 4096) std::this_thread::yield();
+    }
+
+private:
+    const int n_;
+    std::atomic<int> count_{0};
+    std::atomic<unsigned> gen_{0};
 };
 
 }  // namespace ss

@@ -103,6 +103,30 @@ def test_factor_deterministic_and_seeded():
     assert not all(len(x) == len(y) and np.array_equal(x, y) for x, y in zip(a, c))
 
 
+def test_rounds_factor_with_dense_tail_same_for_every_thread_count():
+    """A power-law graph whose rounds order ends in a dense tail (its
+    eliminations spread over all threads, phases behind spin barriers):
+    the same factor bits with 1, 2, 3 and the default number of threads."""
+    import spec_spars_amd as ss
+    from spec_spars_amd import graphs
+    n, u, v = graphs.chung_lu(5000, 100000, gamma=2.2, seed=1)
+    w = graphs.uniform_weights(len(u), seed=1)
+    cp, ri, va = graphs.laplacian_csc(n, u, v, w)
+    out = []
+    try:
+        for threads in (1, 2, 3, 0):
+            ss.set_factor_threads(threads)
+            f = ss.Factor(cp, ri, va, seed=4, ordering=ss.ORDER_ROUNDS)
+            assert f.tail() > 100
+            out.append((f.export(), f.order()[0], f.tail()))
+    finally:
+        ss.set_factor_threads(0)
+    for exp, order, tail in out[1:]:
+        assert tail == out[0][2]
+        assert np.array_equal(order, out[0][1])
+        assert all(np.array_equal(x, y) for x, y in zip(exp, out[0][0]))
+
+
 def test_levels_match_oracle_dp():
     import oracle
     from spec_spars_amd import graphs
