@@ -36,6 +36,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "sparsified edges/s at 1 GPU vs plain at ε=0.3; p50/p99 request latency"
 PEAK_HBM_GBS = 8000.0
+# a 4 KiB copy (k_copy_f64, 512 work-items) just before and just after the
+# profiled served pass: the markers --trace-summary finds in a kernel trace
+TRACE_MARK_BYTES = 4096
 
 
 def log(*a):
@@ -144,6 +147,18 @@ def summarize(prof, kb, t_timed):
                     "unit": "GB/s", "frac": round(kd["GBps"] / PEAK_HBM_GBS, 4), "traffic": None,
                     "bytes_per_launch": kd["bytes_per_launch"], "avg_launch_ms": kd["avg_ms"]}
     return kernels, roofline
+
+
+def add_traffic(roofline, traffic):
+    """The counter-measured HBM/fabric bytes per launch of the roofline's
+    kernel and the rate they move at: how close the kernel runs to the
+    memory system when the gathers beyond first touch are counted too."""
+    roofline["traffic"] = traffic
+    if traffic and roofline.get("avg_launch_ms"):
+        gbs = traffic / (roofline["avg_launch_ms"] * 1e-3) / 1e9
+        roofline["traffic_GBps"] = round(gbs, 1)
+        roofline["traffic_frac"] = round(gbs / PEAK_HBM_GBS, 4)
+        roofline["traffic_over_algorithmic"] = round(traffic / roofline["bytes_per_launch"], 2)
 
 
 def measured_traffic(workload, precision, kind):
@@ -380,12 +395,14 @@ def bench_single(W, args):
         value = R * m / t_served
         prof, t_prof = {}, 1.0
         if not args.no_profile:
+            ctx.bench_copy(TRACE_MARK_BYTES, reps=1)
             G.profile(True)
             t0 = time.perf_counter()
             G.sparsify_many(reqs, solve=svM)
             t_prof = time.perf_counter() - t0
             prof = G.profile_read()
             G.profile(False)
+            ctx.bench_copy(TRACE_MARK_BYTES, reps=1)
         kept = sum(r["kept"] for r in rep_served)
         kb = kernel_bytes(n, m, G, K, kept, s=8 if PM == ss.PREC_FP64 else 4)
         kernels, roofline = summarize(prof, kb, t_prof)
@@ -395,8 +412,8 @@ def bench_single(W, args):
             ke["bytes_per_launch"] = kernel_bytes(n, m, G, k, kept // R, s=4)["er_sample"]
             ke["GBps"] = ke["bytes_per_launch"] / (ke["avg_ms"] * 1e-3) / 1e9
         if roofline:
-            roofline["traffic"] = measured_traffic(W["name"], "mixed" if PM == ss.PREC_MIXED else "fp64",
-                                                   roofline["kernel"])
+            add_traffic(roofline, measured_traffic(W["name"], "mixed" if PM == ss.PREC_MIXED else "fp64",
+                                                   roofline["kernel"]))
         copy_ms = ctx.bench_copy(1 << 30, reps=10)
         copy_gbs = 2.0 * (1 << 30) / (copy_ms * 1e-3) / 1e9
 
@@ -543,7 +560,7 @@ def bench_batch(W, args):
         kb = kernel_bytes(n, m, B, k, kept, s=8 if prec == ss.PREC_FP64 else 4)
         kernels, roofline = summarize(prof, kb, t_prof)
         if roofline:
-            roofline["traffic"] = measured_traffic(W["name"], args.precision, roofline["kernel"])
+            add_traffic(roofline, measured_traffic(W["name"], args.precision, roofline["kernel"]))
         sched = {"fwd": B.schedule_info(0), "bwd": B.schedule_info(1)}
     finally:
         if B is not None:
@@ -630,6 +647,37 @@ def probe(args):
     return 0
 
 
+def trace_summary(path):
+    """Per-kernel device time of the profiled served pass from a rocprofv3
+    kernel trace of this script (--kernel-trace --output-format csv): the
+    kernels between the two marker copies around that pass (the last
+    marker launch before it, the first after it).  Prints markdown."""
+    import csv
+    rows = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"], int(r["Grid_Size"])))
+    rows.sort()
+    marks = [i for i, r in enumerate(rows) if "k_copy_f64" in r[2] and r[3] <= TRACE_MARK_BYTES // 8]
+    if len(marks) < 4:
+        raise SystemExit("trace-summary: the marker copies are not in %s" % path)
+    lo, hi = marks[1], marks[2]   # each marker is a warm launch plus one timed launch
+    stats = {}
+    for t0, t1, name, _ in rows[lo + 1:hi]:
+        key = name.split("(")[0].replace("void ", "")
+        s = stats.setdefault(key, [0, 0.0])
+        s[0] += 1
+        s[1] += (t1 - t0) / 1e6
+    span = (rows[hi][0] - rows[lo][1]) / 1e6
+    print("| kernel | launches | total ms | avg us |")
+    print("|---|---:|---:|---:|")
+    for k, (c, ms) in sorted(stats.items(), key=lambda x: -x[1][1]):
+        print("| `%s` | %d | %.2f | %.1f |" % (k, c, ms, 1e3 * ms / c))
+    print("\nKernel time in the window %.1f ms; the window spans %.1f ms." %
+          (sum(v[1] for v in stats.values()), span))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -653,7 +701,11 @@ def main():
     ap.add_argument("--duration", type=float, default=60.0, help="serve workload: arrival window in seconds")
     ap.add_argument("--pool", type=int, default=12, help="serve workload: resident graphs")
     ap.add_argument("--slots", type=int, default=4, help="serve workload: requests on the device at once")
+    ap.add_argument("--trace-summary", metavar="CSV", default=None,
+                    help="summarize the profiled served pass of a rocprofv3 kernel trace of this script, and exit")
     args = ap.parse_args()
+    if args.trace_summary:
+        return trace_summary(args.trace_summary)
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (the multi-GPU "
               "sharding design is documented, not built)" % args.gpus, file=sys.stderr)

@@ -35,3 +35,27 @@ def test_traffic_figures_are_read_from_the_committed_counter_summary():
     assert t_fwd and t_bwd and 5e7 < t_fwd < 2e8 and 5e7 < t_bwd < 2e8
     assert bench.measured_traffic("grid3d128", "fp64", "spmm") is None
     assert bench.measured_traffic("no-such-workload", "fp64", "tri_fwd") is None
+
+
+def test_trace_summary_takes_the_window_between_the_markers(tmp_path):
+    """--trace-summary keeps the kernels between the two marker copies
+    (a warm and a timed 512-item k_copy_f64 on either side of the profiled
+    pass) and nothing outside them; the 1 GiB copy is no marker."""
+    head = ('"Correlation_Id","Dispatch_Id","Kernel_Name","Grid_Size","Start_Timestamp","End_Timestamp"\n')
+    rows = [("ss::kb_sketch(x)", 1000, 0, 500),                 # before the window
+            ("ss::k_copy_f64(long, double const*, double*)", 512, 1000, 1010),
+            ("ss::k_copy_f64(long, double const*, double*)", 512, 1020, 1030),
+            ("void ss::kb_spmm_dot<float>(a)", 4096, 2000, 4000),
+            ("void ss::kb_spmm_dot<float>(a)", 4096, 5000, 6000),
+            ("void ss::k_trisolve_level<float, true, 1>(b)", 64, 6000, 6500),
+            ("ss::k_copy_f64(long, double const*, double*)", 512, 7000, 7010),
+            ("ss::k_copy_f64(long, double const*, double*)", 512, 7020, 7030),
+            ("ss::k_copy_f64(long, double const*, double*)", 1 << 27, 8000, 9000)]
+    p = tmp_path / "run_kernel_trace.csv"
+    p.write_text(head + "".join('%d,%d,"%s",%d,%d,%d\n' % (i, i, n, g, a, b) for i, (n, g, a, b) in enumerate(rows)))
+    r = _bench("--trace-summary", str(p))
+    assert r.returncode == 0, r.stderr
+    out = r.stdout
+    assert "| `ss::kb_spmm_dot<float>` | 2 | 0.00 | 1.5 |" in out
+    assert "k_trisolve_level<float, true, 1>` | 1 |" in out
+    assert "kb_sketch" not in out and "k_copy_f64" not in out
