@@ -655,8 +655,16 @@ def trace_summary(path):
     import csv
     rows = []
     with open(path) as f:
-        for r in csv.DictReader(f):
-            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"], int(r["Grid_Size"])))
+        rd = csv.DictReader(f)
+        cols = rd.fieldnames or []
+        if "Grid_Size" in cols:
+            grid = lambda r: int(r["Grid_Size"])
+        elif "Grid_Size_X" in cols:
+            grid = lambda r: int(r["Grid_Size_X"]) * int(r.get("Grid_Size_Y") or 1) * int(r.get("Grid_Size_Z") or 1)
+        else:
+            raise SystemExit("trace-summary: no grid size column in %s (columns: %s)" % (path, ", ".join(cols)))
+        for r in rd:
+            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"], grid(r)))
     rows.sort()
     marks = [i for i, r in enumerate(rows) if "k_copy_f64" in r[2] and r[3] <= TRACE_MARK_BYTES // 8]
     if len(marks) < 4:

@@ -3,6 +3,8 @@ import os
 import subprocess
 import sys
 
+import pytest
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
@@ -37,11 +39,14 @@ def test_traffic_figures_are_read_from_the_committed_counter_summary():
     assert bench.measured_traffic("no-such-workload", "fp64", "tri_fwd") is None
 
 
-def test_trace_summary_takes_the_window_between_the_markers(tmp_path):
+@pytest.mark.parametrize("xyz", [False, True])
+def test_trace_summary_takes_the_window_between_the_markers(tmp_path, xyz):
     """--trace-summary keeps the kernels between the two marker copies
     (a warm and a timed 512-item k_copy_f64 on either side of the profiled
-    pass) and nothing outside them; the 1 GiB copy is no marker."""
-    head = ('"Correlation_Id","Dispatch_Id","Kernel_Name","Grid_Size","Start_Timestamp","End_Timestamp"\n')
+    pass) and nothing outside them; the 1 GiB copy is no marker.  The grid
+    comes as one column or as X, Y, Z."""
+    grid_cols = '"Grid_Size_X","Grid_Size_Y","Grid_Size_Z"' if xyz else '"Grid_Size"'
+    head = ('"Correlation_Id","Dispatch_Id","Kernel_Name",%s,"Start_Timestamp","End_Timestamp"\n' % grid_cols)
     rows = [("ss::kb_sketch(x)", 1000, 0, 500),                 # before the window
             ("ss::k_copy_f64(long, double const*, double*)", 512, 1000, 1010),
             ("ss::k_copy_f64(long, double const*, double*)", 512, 1020, 1030),
@@ -52,7 +57,8 @@ def test_trace_summary_takes_the_window_between_the_markers(tmp_path):
             ("ss::k_copy_f64(long, double const*, double*)", 512, 7020, 7030),
             ("ss::k_copy_f64(long, double const*, double*)", 1 << 27, 8000, 9000)]
     p = tmp_path / "run_kernel_trace.csv"
-    p.write_text(head + "".join('%d,%d,"%s",%d,%d,%d\n' % (i, i, n, g, a, b) for i, (n, g, a, b) in enumerate(rows)))
+    gfmt = (lambda g: "%d,1,1" % g) if xyz else (lambda g: "%d" % g)
+    p.write_text(head + "".join('%d,%d,"%s",%s,%d,%d\n' % (i, i, n, gfmt(g), a, b) for i, (n, g, a, b) in enumerate(rows)))
     r = _bench("--trace-summary", str(p))
     assert r.returncode == 0, r.stderr
     out = r.stdout
