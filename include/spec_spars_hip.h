@@ -124,9 +124,10 @@ int ss_sketch(ss_context *ctx, const int32_t *v1, const int32_t *v2,
 /* Mixed precision (SS_PREC_MIXED) around an FP32 PCG with FP64 dots:
  * RELIABLE keeps one PCG running and replaces a column's recursive residual
  * by the FP64 true residual b - A x whenever it has fallen by inner_tol
- * (reliable updates; the search directions are kept); RESTART runs FP32
- * PCG sweeps, each on the FP64 residual of the last, until the FP64 rule
- * holds (iterative refinement). */
+ * (reliable updates; the search directions are kept; max_iter bounds the
+ * iterations); RESTART runs FP32 PCG sweeps, each on the FP64 residual of
+ * the last, until the FP64 rule holds (iterative refinement; at most 16
+ * sweeps of at most min(max_iter, 300) iterations each). */
 #define SS_MIXED_RELIABLE 0
 #define SS_MIXED_RESTART 1
 

@@ -152,6 +152,58 @@ def test_mixed_reliable_and_restart_agree(ctx, oracle_mod, kind):
     assert out[(ss.MIXED_RELIABLE, 0.0)] <= out[(ss.MIXED_RESTART, 0.0)], out
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_mixed_edge_cases(ctx, oracle_mod, mode):
+    """Mixed precision (reliable updates and restart sweeps) on the cases
+    its per-column logic has to get right: a graph of three components and
+    an isolated vertex (every component's residual at the tolerance, its
+    own mean removed), a zero right-hand side (a zero solution, no
+    iterations), the reference's absolute stop rule (|r|^2 <= 2e-7), and a
+    budget too small to converge (SS_ERR_NOT_CONVERGED, not an abort)."""
+    import spec_spars_amd as ss
+    from spec_spars_amd import graphs
+    n1, u1, v1 = graphs.grid2d(20, 15)
+    n2, u2, v2 = graphs.rgg(400, seed=0, mean_degree=12.0)
+    n3, u3, v3 = graphs.grid2d(7, 9)
+    u = np.concatenate([u1, u2 + n1, u3 + n1 + n2])
+    v = np.concatenate([v1, v2 + n1, v3 + n1 + n2])
+    n = n1 + n2 + n3 + 1   # the last vertex is isolated
+    u, v, _ = graphs.canonical(u, v)
+    w = graphs.loguniform_weights(len(u), seed=2, lo=0.1, hi=10.0)
+    ncomp, lab = __import__("scipy.sparse.csgraph", fromlist=["x"]).connected_components(
+        sp.coo_matrix((np.ones(len(u)), (u, v)), shape=(n, n)), directed=False)
+    assert ncomp >= 4
+    cp, ri, va = oracle_mod.laplacian(n, u, v, w)
+    L = sp.csc_matrix((va, ri, cp), shape=(n, n))
+    rng = np.random.default_rng(5)
+    Y = rng.standard_normal((n, 6))
+    Y[:, 3] = 0.0
+    Yp = Y.copy()
+    for c in range(ncomp):
+        Yp[lab == c] -= Y[lab == c].mean(0)
+    o = ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-8, max_iter=3000, precision=ss.PREC_MIXED, mixed_mode=mode)
+    Z, info = ctx.solve_lap(Y, cp, ri, va, o)
+    R = Yp - L @ Z
+    for c in range(ncomp):
+        on = lab == c
+        b = np.linalg.norm(Yp[on], axis=0)
+        r = np.linalg.norm(R[on], axis=0)
+        assert np.all(r <= 1.01e-8 * np.maximum(np.linalg.norm(Yp, axis=0), 1e-300)), (c, r / np.maximum(b, 1e-300))
+    assert np.array_equal(Z[:, 3], np.zeros(n)) and info["iters"][3] == 0
+    # the reference's absolute rule
+    o = ss.solve_opts(stop_rule=ss.STOP_REF_ABS, precision=ss.PREC_MIXED, max_iter=3000, mixed_mode=mode)
+    Z, info = ctx.solve_lap(Y, cp, ri, va, o)
+    rr = (np.linalg.norm(Yp - L @ Z, axis=0) ** 2)
+    assert np.all(rr <= 2e-7 * (1 + 1e-9)), rr
+    # too few iterations: a status, not an abort (max_iter bounds the
+    # reliable solve's iterations; restart sweeps take it per sweep)
+    if mode == ss.MIXED_RELIABLE:
+        o = ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-12, max_iter=1, precision=ss.PREC_MIXED, mixed_mode=mode)
+        with pytest.raises(ss.SSError) as ei:
+            ctx.solve_lap(Y, cp, ri, va, o)
+        assert ei.value.code == 3
+
+
 def test_solve_not_converged_status(ctx, oracle_mod):
     import spec_spars_amd as ss
     n, u, v, w = _graph("grid")
