@@ -713,9 +713,9 @@ __global__ __launch_bounds__(64) void kb_reliable_apply(int k, const int32_t *gt
                 atomicSub(status, 1);
             } else {
                 if (flag[gj] == 2 || rr > kReliableJump * s.rr[gj]) s.rz[gj] = 0.0;
-                rr_ref[gj] = rr;
                 keep[gj] = 1;
             }
+            rr_ref[gj] = rr;   // a done column's final |b - A x|^2
             flag[gj] = 0;
         }
         if (gj == 0) status[2] = 0;
@@ -890,7 +890,10 @@ void bt_sketch(hipStream_t s, Tiles rt, const int32_t *iptr, const int32_t *ieid
 template <typename TA, typename TB>
 void bt_tile_partial(hipStream_t s, Tiles rt, int k, const TA *A, const TB *B, const double *diag, int op,
                      double *partial) {
-    hipLaunchKernelGGL((kb_tile_partial<TA, TB>), dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, k, A, B, diag, op, partial);
+    // all 16 waves whatever the tuned tile width: the reduction reads one
+    // block per tile with nothing else to hide its load latency
+    hipLaunchKernelGGL((kb_tile_partial<TA, TB>), dim3(rt.count), dim3(64 * kMaxTileWaves), 0, s, rt, k, A, B, diag,
+                       op, partial);
 }
 template void bt_tile_partial<double, double>(hipStream_t, Tiles, int, const double *, const double *,
                                               const double *, int, double *);

@@ -1039,7 +1039,7 @@ static int pcg_run(ss_graph *g, int k, const T *rhs, const double *mean_src, T *
 // refinement, the Krylov space built so far is kept.  rounds = replacement
 // rounds (launches of the FP64 residual).
 static int pcg_reliable(ss_graph *g, int k, int rule, double tol, double delta, int max_iter,
-                        const std::vector<double> &bn2, std::vector<int32_t> &iters, int &rounds) {
+                        const std::vector<double> &bn2, std::vector<int32_t> &iters, int &rounds, bool &all_done) {
     hipStream_t s = g->ctx->stream;
     const Tiles rt = rtiles(g);
     const int gk = g->G * k;
@@ -1057,6 +1057,7 @@ static int pcg_reliable(ss_graph *g, int k, int rule, double tol, double delta, 
     SS_HIP(hipMemcpyAsync(g->rel_ref, bn2.data(), sizeof(double) * gk, hipMemcpyHostToDevice, s));
     bt_reliable_init(s, gk, g->sc, g->rel_ref, g->rel_flag, g->status);
     rounds = 0;
+    all_done = false;
     for (int it = 0; it < max_iter; it++) {
         SS_TRY(pcg_iteration<float>(g, k, g->X32, w, rule, tol, false, delta2));
         int nactive = 0, nflag = 0;
@@ -1080,7 +1081,10 @@ static int pcg_reliable(ss_graph *g, int k, int rule, double tol, double delta, 
             }
             SS_TRY(read_status(g, &nactive));
         }
-        if (nactive == 0) break;
+        if (nactive == 0) {
+            all_done = true;
+            break;
+        }
         SS_TRY(precond<float>(g, k, w.R, w.W, w.Zp, g->partial2));
         bt_pcg_beta(s, gk, k, g->gtile, g->partial2, g->sc);
     }
@@ -1152,9 +1156,19 @@ int graph_solve(ss_graph *g, int k, const ss_solve_opts &o, SolveStats &st) {
         if (o.mixed_mode != SS_MIXED_RELIABLE && o.mixed_mode != SS_MIXED_RESTART)
             return fail(SS_ERR_INVALID, "unknown mixed_mode %d", o.mixed_mode);
         if (o.mixed_mode == SS_MIXED_RELIABLE) {
+            bool all_done = false;
             SS_TRY(pcg_reliable(g, k, o.stop_rule, o.tol, o.inner_tol > 0.0 ? o.inner_tol : 3e-3, o.max_iter, bn2,
-                                st.iters, st.refinements));
-            SS_TRY(residual_norms(g, k, g->X64, g->B64, g->R64, rn2));
+                                st.iters, st.refinements, all_done));
+            if (all_done) {
+                // every column stopped on its FP64 true residual, which the
+                // last replacement left in rel_ref (a zero right-hand side
+                // keeps its zero); X64 has not changed since
+                rn2.resize(gk);
+                SS_HIP(hipMemcpyAsync(rn2.data(), g->rel_ref, sizeof(double) * gk, hipMemcpyDeviceToHost, s));
+                SS_HIP(hipStreamSynchronize(s));
+            } else {
+                SS_TRY(residual_norms(g, k, g->X64, g->B64, g->R64, rn2));
+            }
             for (int i = 0; i < gk; i++) st.relres[i] = bn2[i] > 0.0 ? std::sqrt(rn2[i] / bn2[i]) : 0.0;
             return SS_OK;
         }
