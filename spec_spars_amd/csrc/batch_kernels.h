@@ -113,6 +113,10 @@ void bt_accumulate(hipStream_t s, Tiles rt, int k, const float *d, double *x);
 void bt_reliable_init(hipStream_t s, int gk, PcgScalars sc, double *rr_ref, int *flag, int *status);
 void bt_reliable_check(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, int rule, double tol,
                        double delta2, PcgScalars sc, const double *rr_ref, int *flag, int *status);
+// Flag every active column of a group (columns [gs[j], ge[j]) of its
+// graph) that has a flagged column.
+void bt_reliable_group(hipStream_t s, int gk, int k, const int32_t *gs, const int32_t *ge, PcgScalars sc, int *flag,
+                       int *status);
 void bt_reliable_fold(hipStream_t s, Tiles rt, int k, const int *flag, const double *alpha, const float *P,
                       float *X32, double *X64);
 void bt_reliable_apply(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, int rule, double tol,

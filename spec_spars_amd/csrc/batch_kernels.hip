@@ -621,6 +621,26 @@ __global__ __launch_bounds__(64) void kb_reliable_check(int k, const int32_t *gt
     }
 }
 
+// Replacement groups: an active column whose group (columns [gs[j],
+// ge[j]) of its graph, the columns of one request) has a flagged column is
+// flagged too, so a request's columns replace together.  A flag seen here
+// always goes back to a column flagged by its own residual, so the outcome
+// does not depend on the order the threads run in.
+__global__ __launch_bounds__(256) void kb_reliable_group(int gk, int k, const int32_t *__restrict__ gs,
+                                                        const int32_t *__restrict__ ge, PcgScalars s, int *flag,
+                                                        int *status) {
+    const int gj = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gj >= gk) return;
+    const int g = gj / k, j = gj - g * k;
+    if (!s.active[gj] || flag[gj]) return;
+    for (int c = gs[j]; c < ge[j]; c++)
+        if (__hip_atomic_load(&flag[g * k + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            __hip_atomic_store(&flag[gj], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            atomicAdd(status + 2, 1);
+            return;
+        }
+}
+
 // Replacement, step 1 (flagged columns): x = X32 + alpha P (the FP32
 // iterate with the iteration's pending update), X64 += x, X32 = x.
 __global__ __launch_bounds__(1024) void kb_reliable_fold(Tiles rt, int k, const int *__restrict__ flag,
@@ -659,7 +679,9 @@ __global__ __launch_bounds__(1024) void kb_reliable_residual(Tiles rt, const int
         const int j = jb + BT_LANE;
         const bool on = j < k && flag[(size_t)g * k + j] != 0;
         double acc = 0.0;
-        for (int row = r0 + BT_WAVE; row < r1; row += BT_NW) {
+        // a chunk with no replaced column skips its rows (the waves still
+        // meet at the partial's barrier)
+        for (int row = r0 + BT_WAVE; __ballot(on) != 0 && row < r1; row += BT_NW) {
             const int r = bt_uniform(row);
             const int q0 = rowptr[r], q1 = rowptr[r + 1];
             if (on) {
@@ -993,6 +1015,10 @@ void bt_reliable_check(hipStream_t s, int gk, int k, const int32_t *gtile, const
                        double delta2, PcgScalars sc, const double *rr_ref, int *flag, int *status) {
     hipLaunchKernelGGL(kb_reliable_check, dim3(gk), dim3(64), 0, s, k, gtile, p, rule, tol, delta2, sc, rr_ref, flag,
                        status);
+}
+void bt_reliable_group(hipStream_t s, int gk, int k, const int32_t *gs, const int32_t *ge, PcgScalars sc, int *flag,
+                       int *status) {
+    hipLaunchKernelGGL(kb_reliable_group, dim3(bt_grid(gk, 256)), dim3(256), 0, s, gk, k, gs, ge, sc, flag, status);
 }
 void bt_reliable_fold(hipStream_t s, Tiles rt, int k, const int *flag, const double *alpha, const float *P,
                       float *X32, double *X64) {

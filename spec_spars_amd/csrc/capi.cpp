@@ -373,7 +373,8 @@ static int batch_sparsify(ss_graph *g, const ss_sparsify_opts *opts, ss_sparsify
     SS_TRY(graph_sketch(g, k, o0.uniform, o0.solve.precision));
     SS_HIP(hipEventRecord(c->ev[1], s));
     SolveStats st;
-    SS_TRY(graph_solve(g, k, o0.solve, st));
+    const std::vector<int32_t> one_request((size_t)k, 0);   // a graph's columns replace together
+    SS_TRY(graph_solve(g, k, o0.solve, st, &one_request));
     SS_HIP(hipEventRecord(c->ev[2], s));
     SS_TRY(graph_sample(g, k, nullptr, false));
     SS_HIP(hipEventRecord(c->ev[3], s));
@@ -463,7 +464,9 @@ static int graph_sparsify_many(ss_graph *g, int32_t count, const ss_sparsify_opt
     SS_TRY(graph_sketch(g, K, o0.uniform, o0.solve.precision));
     SS_HIP(hipEventRecord(c->ev[1], s));
     SolveStats st;
-    SS_TRY(graph_solve(g, K, o0.solve, st));
+    std::vector<int32_t> request_of;   // each request's columns replace together
+    for (int r = 0; r < count; r++) request_of.insert(request_of.end(), (size_t)opts[r].k, r);
+    SS_TRY(graph_solve(g, K, o0.solve, st, &request_of));
     SS_HIP(hipEventRecord(c->ev[2], s));
     const size_t M = (size_t)std::max<int64_t>(g->M, 1);
     for (int r = 0; r < count; r++) {

@@ -177,6 +177,7 @@ struct ss_graph {
     ss::PcgScalars sc{};
     double *rel_ref = nullptr;       // per (graph, column): |r|^2 at the last reliable update
     int *rel_flag = nullptr;         // per (graph, column): replacement due
+    int32_t *rel_gs = nullptr, *rel_ge = nullptr;   // per column: its replacement group [gs, ge)
     // per-call parameters (per graph)
     uint64_t *p_colseed = nullptr;   // per (graph, column), G x kMaxK: sketch seed,
     double *p_colscale = nullptr;    // scale,
@@ -244,7 +245,11 @@ struct SolveStats {
 };
 // Solve L Z = Y - (component means) for every graph; Y64 must hold the
 // right-hand sides; the FP64 solution lands in X64.
-int graph_solve(ss_graph *g, int k, const ss_solve_opts &o, SolveStats &st);
+// groups (mixed precision, reliable updates): k non-decreasing group ids
+// of the columns (the requests of a sparsify pass), whose columns replace
+// their residuals together; NULL: every column on its own.
+int graph_solve(ss_graph *g, int k, const ss_solve_opts &o, SolveStats &st,
+                const std::vector<int32_t> *groups = nullptr);
 // Per-graph parameters of the next sketch / sample.
 struct GraphParams {
     uint64_t sketch_seed = 0, sampling_seed = 0;

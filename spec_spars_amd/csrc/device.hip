@@ -708,7 +708,7 @@ size_t graph_work_bytes(const ss_graph *g, int k, int prec) {
     b += prec == SS_PREC_FP64 ? nk * 8 * 6 : nk * (8 * 2 + 4 * 2 + 4 * 6);
     b += ((size_t)std::max(std::max(g->nrt, g->nst), 1) + (size_t)std::max(g->nrt, 1)) * cap * 8 + gk * 8 +
          (size_t)g->ncomp * cap * 8 * 2;
-    b += gk * (6 * 8 + 3 * 4 + 4);
+    b += gk * (6 * 8 + 3 * 4 + 4) + (size_t)cap * 8;
     // the dense cores' gathered block and product pieces
     b += ((size_t)g->core.rows + (size_t)g->core.pieces * kCoreTile) * cap * (prec == SS_PREC_FP64 ? 8 : 4);
     return b;
@@ -767,6 +767,8 @@ int graph_ensure_work(ss_graph *g, int k, int prec) {
     SS_TRY(A.alloc(&g->sc_buf, 6 * gk));
     SS_TRY(A.alloc(&g->sc_int, 3 * gk));
     SS_TRY(A.alloc(&g->colkeep, gk));
+    SS_TRY(A.alloc(&g->rel_gs, (size_t)cap));
+    SS_TRY(A.alloc(&g->rel_ge, (size_t)cap));
     g->wk = k;
     g->wcap = cap;
     g->wprec = prec;
@@ -1038,8 +1040,15 @@ static int pcg_run(ss_graph *g, int k, const T *rhs, const double *mean_src, T *
 // residual meets the stop rule.  Unlike restarting the FP32 PCG after every
 // refinement, the Krylov space built so far is kept.  rounds = replacement
 // rounds (launches of the FP64 residual).
+// SPEC_SPARS_VERBOSE >= 2: one line per replacement round on stderr
+static const bool g_trace_rounds = [] {
+    const char *v = std::getenv("SPEC_SPARS_VERBOSE");
+    return v && std::atoi(v) >= 2;
+}();
+
 static int pcg_reliable(ss_graph *g, int k, int rule, double tol, double delta, int max_iter,
-                        const std::vector<double> &bn2, std::vector<int32_t> &iters, int &rounds, bool &all_done) {
+                        const std::vector<double> &bn2, std::vector<int32_t> &iters, int &rounds, bool &all_done,
+                        bool grouped) {
     hipStream_t s = g->ctx->stream;
     const Tiles rt = rtiles(g);
     const int gk = g->G * k;
@@ -1060,10 +1069,12 @@ static int pcg_reliable(ss_graph *g, int k, int rule, double tol, double delta, 
     all_done = false;
     for (int it = 0; it < max_iter; it++) {
         SS_TRY(pcg_iteration<float>(g, k, g->X32, w, rule, tol, false, delta2));
+        if (grouped) bt_reliable_group(s, gk, k, g->rel_gs, g->rel_ge, g->sc, g->rel_flag, g->status);
         int nactive = 0, nflag = 0;
         SS_TRY(read_status(g, &nactive, &nflag));
         if (nflag > 0) {
             rounds++;
+            if (g_trace_rounds) std::fprintf(stderr, "[reliable] iteration %d: %d of %d active columns replaced\n", it + 1, nflag, nactive);
             {
                 ProfScope ps(g, SS_KERNEL_VECTOR);
                 bt_reliable_fold(s, rt, k, g->rel_flag, g->sc.alpha, w.P, g->X32, g->X64);
@@ -1129,7 +1140,7 @@ static int residual_norms(ss_graph *g, int k, const double *X, const double *B, 
     return SS_OK;
 }
 
-int graph_solve(ss_graph *g, int k, const ss_solve_opts &o, SolveStats &st) {
+int graph_solve(ss_graph *g, int k, const ss_solve_opts &o, SolveStats &st, const std::vector<int32_t> *groups) {
     if (!g->has_factor) return fail(SS_ERR_INVALID, "no factor uploaded");
     if (g->wk != k) return fail(SS_ERR_INVALID, "work not allocated for k=%d", k);
     if (o.max_iter < 0 || !(o.tol >= 0.0)) return fail(SS_ERR_INVALID, "bad solve options");
@@ -1156,9 +1167,25 @@ int graph_solve(ss_graph *g, int k, const ss_solve_opts &o, SolveStats &st) {
         if (o.mixed_mode != SS_MIXED_RELIABLE && o.mixed_mode != SS_MIXED_RESTART)
             return fail(SS_ERR_INVALID, "unknown mixed_mode %d", o.mixed_mode);
         if (o.mixed_mode == SS_MIXED_RELIABLE) {
+            bool grouped = false;
+            if (groups) {
+                if ((int)groups->size() != k) return fail(SS_ERR_INVALID, "replacement groups: %zu for k=%d", groups->size(), k);
+                std::vector<int32_t> gs((size_t)k), ge((size_t)k);
+                for (int j = 0; j < k; j++) {
+                    if (j > 0 && (*groups)[j] < (*groups)[j - 1])
+                        return fail(SS_ERR_INVALID, "replacement groups must not decrease");
+                    gs[j] = j > 0 && (*groups)[j] == (*groups)[j - 1] ? gs[j - 1] : j;
+                }
+                for (int j = k - 1; j >= 0; j--)
+                    ge[j] = j + 1 < k && (*groups)[j] == (*groups)[j + 1] ? ge[j + 1] : j + 1;
+                for (int j = 0; j < k; j++) grouped |= ge[j] - gs[j] > 1;
+                SS_TRY(h2d(s, g->rel_gs, gs.data(), (size_t)k));
+                SS_TRY(h2d(s, g->rel_ge, ge.data(), (size_t)k));
+                SS_HIP(hipStreamSynchronize(s));   // gs / ge leave scope
+            }
             bool all_done = false;
             SS_TRY(pcg_reliable(g, k, o.stop_rule, o.tol, o.inner_tol > 0.0 ? o.inner_tol : 3e-3, o.max_iter, bn2,
-                                st.iters, st.refinements, all_done));
+                                st.iters, st.refinements, all_done, grouped));
             if (all_done) {
                 // every column stopped on its FP64 true residual, which the
                 // last replacement left in rel_ref (a zero right-hand side
