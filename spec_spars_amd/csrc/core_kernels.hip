@@ -23,7 +23,6 @@
 namespace ss {
 
 constexpr int kCT = kCoreTile;   // tile rows and columns
-constexpr int kCK = 32;          // inner (m) depth staged in LDS per pass
 
 // L_cc, strictly lower part, FP64: dense[pos[e]] = val[e].
 __global__ __launch_bounds__(256) void k_core_scatter(int64_t count, const int64_t *__restrict__ pos,
@@ -56,73 +55,6 @@ __global__ __launch_bounds__(kCT) void k_core_inv_diag(int cp, const double *__r
         xs[i][j] = x;
     }
     for (int i = 0; i < kCT; i++) M[(o + i) * cp + o + j] = xs[i][j];
-}
-
-// One block row of the inverse: for every tile column jt < it,
-//   M_{it,jt} = -Minv_{it,it} * sum_{jt <= mt < it} L_{it,mt} M_{mt,jt}
-// (block forward substitution of L M = I, column tiles independent).  A
-// workgroup of 256 threads per jt, 4 x 4 outputs per thread.
-__global__ __launch_bounds__(256) void k_core_inv_row(int it, int cp, const double *__restrict__ L,
-                                                      double *__restrict__ M) {
-    __shared__ double as[kCT][kCK + 1];     // L rows x m (or the diagonal inverse)
-    __shared__ double ts[kCT][kCT + 1];     // M m-rows (first kCK rows), then the sum tile
-    const int jt = blockIdx.x, t = threadIdx.x;
-    const int tr = t >> 4, tc = t & 15;
-    double acc[4][4] = {};
-    const size_t r0 = (size_t)it * kCT, c0 = (size_t)jt * kCT;
-    for (size_t m0 = c0; m0 < r0; m0 += kCK) {
-        for (int e = t; e < kCT * kCK; e += 256) {
-            const int r = e / kCK, mm = e % kCK;
-            as[r][mm] = L[(r0 + r) * cp + m0 + mm];
-        }
-        for (int e = t; e < kCK * kCT; e += 256) {
-            const int mm = e / kCT, c = e % kCT;
-            ts[mm][c] = M[(m0 + mm) * cp + c0 + c];
-        }
-        __syncthreads();
-#pragma unroll 4
-        for (int mm = 0; mm < kCK; mm++) {
-            double a[4], bb[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) a[u] = as[tr * 4 + u][mm];
-#pragma unroll
-            for (int v = 0; v < 4; v++) bb[v] = ts[mm][tc * 4 + v];
-#pragma unroll
-            for (int u = 0; u < 4; u++)
-#pragma unroll
-                for (int v = 0; v < 4; v++) acc[u][v] = __builtin_fma(a[u], bb[v], acc[u][v]);
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int u = 0; u < 4; u++)
-#pragma unroll
-        for (int v = 0; v < 4; v++) ts[tr * 4 + u][tc * 4 + v] = acc[u][v];
-    double out[4][4] = {};
-    for (int m0 = 0; m0 < kCT; m0 += kCK) {
-        __syncthreads();
-        for (int e = t; e < kCT * kCK; e += 256) {
-            const int r = e / kCK, mm = e % kCK;
-            as[r][mm] = M[(r0 + r) * cp + r0 + m0 + mm];
-        }
-        __syncthreads();
-#pragma unroll 4
-        for (int mm = 0; mm < kCK; mm++) {
-            double a[4], bb[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) a[u] = as[tr * 4 + u][mm];
-#pragma unroll
-            for (int v = 0; v < 4; v++) bb[v] = ts[m0 + mm][tc * 4 + v];
-#pragma unroll
-            for (int u = 0; u < 4; u++)
-#pragma unroll
-                for (int v = 0; v < 4; v++) out[u][v] = __builtin_fma(a[u], bb[v], out[u][v]);
-        }
-    }
-#pragma unroll
-    for (int u = 0; u < 4; u++)
-#pragma unroll
-        for (int v = 0; v < 4; v++) M[(r0 + tr * 4 + u) * cp + c0 + tc * 4 + v] = -out[u][v];
 }
 
 __global__ __launch_bounds__(256) void k_to_f32(int64_t count, const double *__restrict__ a, float *__restrict__ b) {
@@ -277,6 +209,86 @@ __global__ __launch_bounds__(256) void k_core_apply(const CorePiece *__restrict_
 #pragma unroll
                 for (int i = 0; i < 4; i++) po[(size_t)(wr + 16 * x + acc_row<T>(lk, i)) * k + c] = acc[x][y][i];
         }
+}
+
+// One block row of the inverse: for every tile column jt < it,
+//   M_{it,jt} = -Minv_{it,it} * sum_{jt <= mt < it} L_{it,mt} M_{mt,jt}
+// (block forward substitution of L M = I, column tiles independent).  A
+// workgroup per jt: its four waves each hold 16 rows x 64 columns of the
+// 64 x 64 tile on the matrix cores (16 x 16 x 4 FP64).  The inner range is
+// staged in LDS kAK rows at a time, double buffered; the sum tile is then
+// multiplied by the diagonal tile's inverse (k_core_inv_diag) the same way.
+__global__ __launch_bounds__(256) void k_core_inv_row(int it, int cp, const double *__restrict__ L,
+                                                      double *__restrict__ M) {
+    constexpr int NA = kCT * kAK / 256, NB = kAK * kCT / 256;
+    __shared__ double as[2][kAK][kCT + 4];   // as[.][m][r] = L(r0 + r, m0 + m)
+    __shared__ double bs[2][kAK][kCT + 4];   // bs[.][m][c] = M(m0 + m, c0 + c)
+    __shared__ double ts[kCT][kCT + 4];      // the sum tile
+    __shared__ double ms[kCT][kCT + 4];      // ms[m][r] = Minv_{it,it}(r, m)
+    const int jt = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int li = lane & 15, lk = lane >> 4, wr = 16 * wv;
+    const size_t r0 = (size_t)it * kCT, c0 = (size_t)jt * kCT;
+    f64x4 acc[4];
+#pragma unroll
+    for (int y = 0; y < 4; y++) acc[y] = f64x4{0, 0, 0, 0};
+    double ra[NA], rb[NB];
+    auto load = [&](size_t m0) {
+#pragma unroll
+        for (int u = 0; u < NA; u++) {
+            const int e = t + 256 * u;
+            ra[u] = L[(r0 + e / kAK) * cp + m0 + e % kAK];
+        }
+#pragma unroll
+        for (int u = 0; u < NB; u++) {
+            const int e = t + 256 * u;
+            rb[u] = M[(m0 + e / kCT) * cp + c0 + e % kCT];
+        }
+    };
+    auto store = [&](int b) {
+#pragma unroll
+        for (int u = 0; u < NA; u++) {
+            const int e = t + 256 * u;
+            as[b][e % kAK][e / kAK] = ra[u];
+        }
+#pragma unroll
+        for (int u = 0; u < NB; u++) {
+            const int e = t + 256 * u;
+            bs[b][e / kCT][e % kCT] = rb[u];
+        }
+    };
+    // the diagonal tile's inverse, transposed into LDS
+    for (int e = t; e < kCT * kCT; e += 256) ms[e % kCT][e / kCT] = M[(r0 + e / kCT) * cp + r0 + e % kCT];
+    load(c0);
+    int b = 0;
+    for (size_t m0 = c0; m0 < r0; m0 += kAK, b ^= 1) {
+        store(b);
+        __syncthreads();
+        if (m0 + kAK < r0) load(m0 + kAK);
+#pragma unroll
+        for (int m4 = 0; m4 < kAK; m4 += 4) {
+            const double a = as[b][m4 + lk][wr + li];
+#pragma unroll
+            for (int y = 0; y < 4; y++) acc[y] = mfma16(a, bs[b][m4 + lk][16 * y + li], acc[y]);
+        }
+    }
+#pragma unroll
+    for (int y = 0; y < 4; y++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) ts[wr + acc_row<double>(lk, i)][16 * y + li] = acc[y][i];
+    __syncthreads();
+    f64x4 out[4];
+#pragma unroll
+    for (int y = 0; y < 4; y++) out[y] = f64x4{0, 0, 0, 0};
+#pragma unroll 4
+    for (int m4 = 0; m4 < kCT; m4 += 4) {
+        const double a = ms[m4 + lk][wr + li];
+#pragma unroll
+        for (int y = 0; y < 4; y++) out[y] = mfma16(a, ts[m4 + lk][16 * y + li], out[y]);
+    }
+#pragma unroll
+    for (int y = 0; y < 4; y++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) M[(r0 + wr + acc_row<double>(lk, i)) * cp + c0 + 16 * y + li] = -out[y][i];
 }
 
 // out[vert[r]] = sum of the row's pieces, in piece order.

@@ -15,10 +15,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <numeric>
+#include <string>
 #include <type_traits>
 
 #include "device.h"
@@ -26,6 +30,27 @@
 namespace ss {
 
 static const int kSpinLimit = 1 << 22;   // polls per look-back wait (~seconds)
+
+// SPEC_SPARS_VERBOSE set: the host-side phases of an upload on stderr
+struct UploadClock {
+    const char *what;
+    const bool on;
+    std::chrono::steady_clock::time_point t;
+    std::string line;
+    explicit UploadClock(const char *w)
+        : what(w), on(std::getenv("SPEC_SPARS_VERBOSE") != nullptr), t(std::chrono::steady_clock::now()) {}
+    void mark(const char *phase) {
+        if (!on) return;
+        const auto now = std::chrono::steady_clock::now();
+        char buf[96];
+        std::snprintf(buf, sizeof buf, " %s %.2fs", phase, std::chrono::duration<double>(now - t).count());
+        line += buf;
+        t = now;
+    }
+    ~UploadClock() {
+        if (on) std::fprintf(stderr, "[upload] %s:%s\n", what, line.c_str());
+    }
+};
 
 // ---------------------------------------------------------------------------
 // per-kernel timing
@@ -176,6 +201,7 @@ int batch_upload(ss_context *ctx, const std::vector<const HostGraph *> &graphs, 
     g->nrt = g->rtile_off[G];
     g->net = g->etile_off[G];
     const int64_t N = g->N, M = g->M, nnz = N + 2 * M;
+    UploadClock clk("graph upload");
 
     // tiles
     std::vector<int32_t> rt_start((size_t)g->nrt + 1), rt_graph((size_t)std::max(g->nrt, 1)),
@@ -276,6 +302,7 @@ int batch_upload(ss_context *ctx, const std::vector<const HostGraph *> &graphs, 
     g->ncomp = (int)comp_graph.size();
     g->nsmall = (int)small_id.size();
     std::vector<float> valf(val.begin(), val.end());
+    clk.mark("host arrays");
 
     auto &A = g->arena_struct;
     SS_TRY(A.alloc(&g->rt_start, rt_start.size()));
@@ -348,6 +375,7 @@ int batch_upload(ss_context *ctx, const std::vector<const HostGraph *> &graphs, 
     SS_HIP(hipMemsetAsync(g->status, 0, sizeof(int) * 3, s));
     SS_HIP(hipMemsetAsync(g->tile_state, 0, sizeof(uint64_t) * ((size_t)g->net + 1), s));
     SS_HIP(hipStreamSynchronize(s));
+    clk.mark("copies");
     g->kept_start.assign((size_t)G, 0);
     g->kept_count.assign((size_t)G, 0);
     ctx->live_graphs++;
@@ -475,6 +503,7 @@ int batch_upload_factors(ss_graph *g, const std::vector<const Factor *> &factors
             return fail(SS_ERR_INVALID, "factor %d does not match graph %d (n=%lld)", i, i,
                         (long long)(g->row_off[i + 1] - g->row_off[i]));
     SS_HIP(hipSetDevice(g->ctx->device));
+    UploadClock clk("factor upload");
     Factor Fc;
     const Factor *Fp = factors[0];
     if (g->G > 1) {
@@ -501,7 +530,9 @@ int batch_upload_factors(ss_graph *g, const std::vector<const Factor *> &factors
         tc_off[i + 1] = tc_off[i] + cpad[i];
     }
     Schedule fw, bw;
+    clk.mark("cores");
     SS_TRY(factor_schedules(F, core, fw, bw, kLongRow, kSmallRows));
+    clk.mark("schedules");
     // product pieces: per row tile of each core, inner ranges of at most
     // kCoreSeg rows -- forward [0, r0 + kCoreTile), backward [r0, cp)
     std::vector<CorePiece> pf, pb;
@@ -584,8 +615,11 @@ int batch_upload_factors(ss_graph *g, const std::vector<const Factor *> &factors
     SS_TRY(h2d(s, d_tile_row, tile_row.data(), tile_row.size()));
     SS_TRY(h2d(s, d_tpf, tpf.data(), tpf.size()));
     SS_TRY(h2d(s, d_tpb, tpb.data(), tpb.size()));
+    SS_HIP(hipStreamSynchronize(s));
+    clk.mark("copies");
     SS_TRY(core_invert(g, F, core_c, cpad, dense_off));
     SS_HIP(hipStreamSynchronize(s));
+    clk.mark("core inverse");
     g->core.rows = tc_off[g->G];
     g->core.pieces = (int32_t)pf.size();
     g->core.tiles = (int32_t)tile_row.size();
