@@ -161,15 +161,20 @@ def add_traffic(roofline, traffic):
         roofline["traffic_over_algorithmic"] = round(traffic / roofline["bytes_per_launch"], 2)
 
 
-def measured_traffic(workload, precision, kind):
+def measured_traffic(workload, precision, kind, columns):
     """HBM bytes per launch of a kernel kind, from the committed counter
     summary profiles/sparsify_kernels_<workload>_<precision>_pmc.md
     (FETCH_SIZE and WRITE_SIZE per dispatch, calibrated on a copy kernel),
-    when one exists."""
+    when one exists for a pass of the same number of columns (its
+    `traffic_columns:` line)."""
     path = os.path.join(ROOT, "profiles", "sparsify_kernels_%s_%s_pmc.md" % (workload, precision))
     if not os.path.exists(path):
         return None
-    for line in open(path):
+    lines = open(path).read().splitlines()
+    cols = [int(x.split(":")[1]) for x in lines if x.startswith("traffic_columns:")]
+    if cols != [columns]:
+        return None
+    for line in lines:
         if line.startswith("traffic_bytes_per_launch %s:" % kind):
             return int(line.split(":")[1])
     return None
@@ -413,7 +418,7 @@ def bench_single(W, args):
             ke["GBps"] = ke["bytes_per_launch"] / (ke["avg_ms"] * 1e-3) / 1e9
         if roofline:
             add_traffic(roofline, measured_traffic(W["name"], "mixed" if PM == ss.PREC_MIXED else "fp64",
-                                                   roofline["kernel"]))
+                                                   roofline["kernel"], K))
         copy_ms = ctx.bench_copy(1 << 30, reps=10)
         copy_gbs = 2.0 * (1 << 30) / (copy_ms * 1e-3) / 1e9
 
@@ -562,7 +567,7 @@ def bench_batch(W, args):
         kb = kernel_bytes(n, m, B, k, kept, s=8 if prec == ss.PREC_FP64 else 4)
         kernels, roofline = summarize(prof, kb, t_prof)
         if roofline:
-            add_traffic(roofline, measured_traffic(W["name"], args.precision, roofline["kernel"]))
+            add_traffic(roofline, measured_traffic(W["name"], args.precision, roofline["kernel"], k))
         sched = {"fwd": B.schedule_info(0), "bwd": B.schedule_info(1)}
     finally:
         if B is not None:

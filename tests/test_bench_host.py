@@ -32,11 +32,14 @@ def test_traffic_figures_are_read_from_the_committed_counter_summary():
     profiles/, one line per kernel kind."""
     sys.path.insert(0, ROOT)
     import bench
-    t_fwd = bench.measured_traffic("grid3d128", "fp64", "tri_fwd")
-    t_bwd = bench.measured_traffic("grid3d128", "fp64", "tri_bwd")
+    t_fwd = bench.measured_traffic("grid3d128", "fp64", "tri_fwd", 48)
+    t_bwd = bench.measured_traffic("grid3d128", "fp64", "tri_bwd", 48)
     assert t_fwd and t_bwd and 5e7 < t_fwd < 2e8 and 5e7 < t_bwd < 2e8
-    assert bench.measured_traffic("grid3d128", "fp64", "spmm") is None
-    assert bench.measured_traffic("no-such-workload", "fp64", "tri_fwd") is None
+    assert bench.measured_traffic("grid3d128", "fp64", "spmm", 48) is None
+    assert bench.measured_traffic("no-such-workload", "fp64", "tri_fwd", 48) is None
+    # a pass of another width is not the one the counters measured
+    assert bench.measured_traffic("grid3d128", "fp64", "tri_fwd", 192) is None
+    assert bench.measured_traffic("chunglu5m", "mixed", "tri_bwd", 256) > 1e9
 
 
 @pytest.mark.parametrize("xyz", [False, True])
