@@ -338,6 +338,10 @@ def step_loop(fn, warmup, steps, tag):
     return (time.perf_counter() - t0) / steps, last
 
 
+# mixed-precision preconditioner rows -> ss_solve_opts.mixed_mode
+MIXED_MODES = {"fp16": 0, "fp32": 2}
+
+
 def bench_single(W, args):
     """One resident graph: plain FP64 leg, served mixed leg (merged requests),
     column-wise sample, leverage check, serving window, CPU baseline."""
@@ -394,7 +398,8 @@ def bench_single(W, args):
         reqs = [dict(k=k, beta=graphs.beta_for(n, e), sketch_seed=i, sampling_seed=i) for i, e in enumerate(eps_list)]
         K = R * k
         tuneM = ctx.autotune(G, K, PM, reps=3) if not args.no_tune else ctx.tuning()
-        svM = ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-8, max_iter=2000, precision=PM)
+        svM = ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-8, max_iter=2000, precision=PM,
+                            mixed_mode=MIXED_MODES[args.mixed_rows])
         t_served, rep_served = step_loop(lambda: G.sparsify_many(reqs, solve=svM), args.warmup, args.steps,
                                          "served x%d" % R)
         value = R * m / t_served
@@ -703,6 +708,8 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default=None,
                     help="served leg / batch precision (default: the workload's)")
     ap.add_argument("--requests", type=int, default=4, help="requests merged per served pass")
+    ap.add_argument("--mixed-rows", choices=sorted(MIXED_MODES), default="fp16",
+                    help="mixed precision: the preconditioner's intermediate rows in FP16 (default) or FP32")
     ap.add_argument("--col-sample", type=int, default=2, help="single-RHS solves timed for the column-wise plain leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel event timing")

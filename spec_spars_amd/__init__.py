@@ -7,7 +7,8 @@ fails if it has not been built.
 """
 from . import _lib
 from ._lib import (SSError, PREC_FP64, PREC_MIXED, STOP_REF_ABS, STOP_REL,  # noqa: F401
-                   ORDER_NATURAL, ORDER_RANDOM, ORDER_ROUNDS, MIXED_RELIABLE, MIXED_RESTART)
+                   ORDER_NATURAL, ORDER_RANDOM, ORDER_ROUNDS, MIXED_RELIABLE, MIXED_RESTART,
+                   MIXED_RELIABLE_F32)
 
 _lib.load()
 

@@ -71,13 +71,21 @@ void bt_sketch(hipStream_t s, Tiles rt, const int32_t *iptr, const int32_t *ieid
 template <typename TA, typename TB>
 void bt_tile_partial(hipStream_t s, Tiles rt, int k, const TA *A, const TB *B, const double *diag, int op,
                      double *partial);
+// psc (may be null): the FP16 preconditioner rows' per-(graph, column)
+// scale, which the r . z partials (p_rz) carry squared.
 void bt_pcg_init(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p_bb, const double *p_rz,
-                 PcgScalars sc);
+                 PcgScalars sc, const float *psc = nullptr);
+// FP16 preconditioner rows: sc[gj] = sqrt(n_g / |r|^2) and inv[gj] = 1 /
+// sc[gj] (0 for r = 0), |r|^2 from the tile partials p_rr or, with p_rr
+// null, from rr[gj]; n_g from row_off.
+void bt_prec_scale(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p_rr, const double *rr,
+                   const int32_t *row_off, float *sc, float *inv);
 void bt_pcg_alpha(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, PcgScalars sc, int *nactive,
                   int *flag = nullptr);
 void bt_pcg_check(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, int rule, double tol,
                   PcgScalars sc, int *nactive);
-void bt_pcg_beta(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, PcgScalars sc);
+void bt_pcg_beta(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, PcgScalars sc,
+                 const float *psc = nullptr);
 void bt_graph_sums(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, double *out);
 void bt_small_comp_sums(hipStream_t s, int nsmall, const int32_t *small_id, const int32_t *cptr, const int32_t *cvert,
                         int k, const double *Y, double *csum);
@@ -87,8 +95,11 @@ template <typename TS, typename T>
 void bt_project(hipStream_t s, Tiles rt, int k, const TS *Y, const int32_t *comp_of_row, const double *cmean, T *R,
                 T *B, T *X);
 // P = Z + beta P, with the previous iteration's X += alpha P folded in.
-template <typename T>
-void bt_update_px(hipStream_t s, Tiles rt, int k, const double *alpha, const double *beta, const T *Z, T *P, T *X);
+// zinv (may be null): Z holds FP16 rows scaled per (graph, column), and z
+// = Z zinv[gj].
+template <typename T, typename TZ>
+void bt_update_px(hipStream_t s, Tiles rt, int k, const double *alpha, const double *beta, const TZ *Z,
+                  const float *zinv, T *P, T *X);
 // X += alpha P (the last iteration's solution update).
 template <typename T>
 void bt_update_x(hipStream_t s, Tiles rt, int k, const double *alpha, const T *P, T *X);

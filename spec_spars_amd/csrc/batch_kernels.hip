@@ -180,11 +180,19 @@ __device__ __forceinline__ double bt_graph_sum(const double *partial, const int3
 // ---------------------------------------------------------------------------
 // PCG scalars, one 64-lane block per (graph, column), index gj = g*k + j
 // ---------------------------------------------------------------------------
+// psc (FP16 preconditioner rows): r . z was summed over rows scaled by
+// psc[gj], so it is divided by psc[gj]^2.
+__device__ __forceinline__ double bt_unscale_rz(double rz, const float *psc, int gj) {
+    if (!psc) return rz;
+    const double sc = (double)psc[gj];
+    return sc > 0.0 ? rz / (sc * sc) : 0.0;
+}
+
 __global__ __launch_bounds__(64) void kb_pcg_init(int k, const int32_t *gtile, const double *p_bb,
-                                                  const double *p_rz, PcgScalars s) {
+                                                  const double *p_rz, PcgScalars s, const float *psc) {
     const int gj = blockIdx.x, g = gj / k, j = gj - g * k;
     const double bb = bt_graph_sum(p_bb, gtile, g, k, j);
-    const double rz = bt_graph_sum(p_rz, gtile, g, k, j);
+    const double rz = bt_unscale_rz(bt_graph_sum(p_rz, gtile, g, k, j), psc, gj);
     if (BT_LANE == 0) {
         s.bnorm[gj] = sqrt(bb);
         s.rz[gj] = rz;
@@ -230,14 +238,31 @@ __global__ __launch_bounds__(64) void kb_pcg_check(int k, const int32_t *gtile, 
     }
 }
 
-__global__ __launch_bounds__(64) void kb_pcg_beta(int k, const int32_t *gtile, const double *p_rz, PcgScalars s) {
+__global__ __launch_bounds__(64) void kb_pcg_beta(int k, const int32_t *gtile, const double *p_rz, PcgScalars s,
+                                                  const float *psc) {
     const int gj = blockIdx.x, g = gj / k, j = gj - g * k;
-    const double rzn = bt_graph_sum(p_rz, gtile, g, k, j);
+    const double rzn = bt_unscale_rz(bt_graph_sum(p_rz, gtile, g, k, j), psc, gj);
     if (BT_LANE == 0) {
         double b = 0.0;
         if (s.active[gj] && s.rz[gj] != 0.0) b = rzn / s.rz[gj];
         s.beta[gj] = b;
         s.rz[gj] = rzn;
+    }
+}
+
+// Scale of the FP16 preconditioner rows of (graph, column) gj: s =
+// sqrt(n_g / |r|^2), so the scaled residual has unit mean square and its
+// preconditioned rows stay far inside FP16's range; inv = 1 / s (both 0
+// for r = 0).  |r|^2 from the tile partials p_rr, else from rr[gj].
+__global__ __launch_bounds__(64) void kb_prec_scale(int k, const int32_t *gtile, const double *p_rr, const double *rr,
+                                                    const int32_t *row_off, float *sc, float *inv) {
+    const int gj = blockIdx.x, g = gj / k, j = gj - g * k;
+    const double v = p_rr ? bt_graph_sum(p_rr, gtile, g, k, j) : rr[gj];
+    if (BT_LANE == 0) {
+        const double n = (double)(row_off[g + 1] - row_off[g]);
+        const bool on = v > 0.0 && n > 0.0;
+        sc[gj] = on ? (float)sqrt(n / v) : 0.0f;
+        inv[gj] = on ? (float)sqrt(v / n) : 0.0f;
     }
 }
 
@@ -320,10 +345,15 @@ __global__ __launch_bounds__(1024) void kb_project(Tiles rt, int k, const TS *__
 // tile's block works on column j = i % k of rows i / k, i / k + RP, ...
 // (RP = blockDim / k rows per pass), so a pass moves RP whole rows as one
 // contiguous run and no lane idles when k is below 64.
-template <typename T>
+// With zinv (FP16 preconditioner rows, Z scaled by its column's s), z =
+// Z zinv[gj]; a value outside FP16's range (never seen; it would only
+// slow the column's convergence) counts as zero rather than spreading
+// through P.
+template <typename T, typename TZ>
 __global__ __launch_bounds__(1024) void kb_update_px(Tiles rt, int k, const double *__restrict__ alpha,
-                                                    const double *__restrict__ beta, const T *__restrict__ Z,
-                                                    T *__restrict__ P, T *__restrict__ X) {
+                                                    const double *__restrict__ beta, const TZ *__restrict__ Z,
+                                                    const float *__restrict__ zinv, T *__restrict__ P,
+                                                    T *__restrict__ X) {
     const int t = blockIdx.x, g = rt.graph[t];
     const int r0 = rt.start[t], r1 = rt.start[t + 1];
     const int RP = (int)blockDim.x / k, i = (int)threadIdx.x;
@@ -331,6 +361,7 @@ __global__ __launch_bounds__(1024) void kb_update_px(Tiles rt, int k, const doub
     const int j = i % k, ro = i / k;
     const T a = (T)alpha[(size_t)g * k + j];
     const T b = (T)beta[(size_t)g * k + j];
+    const T zs = zinv ? (T)zinv[(size_t)g * k + j] : (T)1;
     for (int rb = r0 + ro; rb < r1; rb += kBtRows * RP) {
         T p[kBtRows], z[kBtRows], x[kBtRows];
 #pragma unroll
@@ -339,8 +370,12 @@ __global__ __launch_bounds__(1024) void kb_update_px(Tiles rt, int k, const doub
             const bool on = r < r1;
             const size_t o = (size_t)r * k + j;
             p[u] = on && (a != (T)0 || b != (T)0) ? P[o] : (T)0;
-            z[u] = on ? Z[o] : (T)0;
+            z[u] = on ? (T)Z[o] : (T)0;
             x[u] = on && a != (T)0 ? X[o] : (T)0;
+        }
+        if (zinv) {
+#pragma unroll
+            for (int u = 0; u < kBtRows; u++) z[u] = __builtin_isfinite(z[u]) ? z[u] * zs : (T)0;
         }
 #pragma unroll
         for (int u = 0; u < kBtRows; u++) {
@@ -736,6 +771,7 @@ __global__ __launch_bounds__(64) void kb_reliable_apply(int k, const int32_t *gt
             } else {
                 if (flag[gj] == 2 || rr > kReliableJump * s.rr[gj]) s.rz[gj] = 0.0;
                 keep[gj] = 1;
+                s.rr[gj] = rr;   // the replaced FP32 residual's |r|^2
             }
             rr_ref[gj] = rr;   // a done column's final |b - A x|^2
             flag[gj] = 0;
@@ -921,10 +957,16 @@ template void bt_tile_partial<double, double>(hipStream_t, Tiles, int, const dou
                                               const double *, int, double *);
 template void bt_tile_partial<float, float>(hipStream_t, Tiles, int, const float *, const float *, const double *,
                                             int, double *);
+template void bt_tile_partial<_Float16, _Float16>(hipStream_t, Tiles, int, const _Float16 *, const _Float16 *,
+                                                  const double *, int, double *);
 
 void bt_pcg_init(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p_bb, const double *p_rz,
-                 PcgScalars sc) {
-    hipLaunchKernelGGL(kb_pcg_init, dim3(gk), dim3(64), 0, s, k, gtile, p_bb, p_rz, sc);
+                 PcgScalars sc, const float *psc) {
+    hipLaunchKernelGGL(kb_pcg_init, dim3(gk), dim3(64), 0, s, k, gtile, p_bb, p_rz, sc, psc);
+}
+void bt_prec_scale(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p_rr, const double *rr,
+                   const int32_t *row_off, float *sc, float *inv) {
+    hipLaunchKernelGGL(kb_prec_scale, dim3(gk), dim3(64), 0, s, k, gtile, p_rr, rr, row_off, sc, inv);
 }
 void bt_pcg_alpha(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, PcgScalars sc, int *nactive,
                   int *flag) {
@@ -934,8 +976,9 @@ void bt_pcg_check(hipStream_t s, int gk, int k, const int32_t *gtile, const doub
                   PcgScalars sc, int *nactive) {
     hipLaunchKernelGGL(kb_pcg_check, dim3(gk), dim3(64), 0, s, k, gtile, p, rule, tol, sc, nactive);
 }
-void bt_pcg_beta(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, PcgScalars sc) {
-    hipLaunchKernelGGL(kb_pcg_beta, dim3(gk), dim3(64), 0, s, k, gtile, p, sc);
+void bt_pcg_beta(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, PcgScalars sc,
+                 const float *psc) {
+    hipLaunchKernelGGL(kb_pcg_beta, dim3(gk), dim3(64), 0, s, k, gtile, p, sc, psc);
 }
 void bt_graph_sums(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, double *out) {
     hipLaunchKernelGGL(kb_graph_sums, dim3(gk), dim3(64), 0, s, k, gtile, p, out);
@@ -967,15 +1010,18 @@ static int rowblock_threads(const Tiles &rt, int k) {
     const int w = 64 * rt.waves, need = (k + 63) / 64 * 64;
     return w > need ? w : need;
 }
-template <typename T>
-void bt_update_px(hipStream_t s, Tiles rt, int k, const double *alpha, const double *beta, const T *Z, T *P, T *X) {
-    hipLaunchKernelGGL(kb_update_px<T>, dim3(rt.count), dim3(rowblock_threads(rt, k)), 0, s, rt, k, alpha, beta, Z, P,
-                       X);
+template <typename T, typename TZ>
+void bt_update_px(hipStream_t s, Tiles rt, int k, const double *alpha, const double *beta, const TZ *Z,
+                  const float *zinv, T *P, T *X) {
+    hipLaunchKernelGGL((kb_update_px<T, TZ>), dim3(rt.count), dim3(rowblock_threads(rt, k)), 0, s, rt, k, alpha, beta,
+                       Z, zinv, P, X);
 }
-template void bt_update_px<double>(hipStream_t, Tiles, int, const double *, const double *, const double *, double *,
-                                   double *);
-template void bt_update_px<float>(hipStream_t, Tiles, int, const double *, const double *, const float *, float *,
-                                  float *);
+template void bt_update_px<double, double>(hipStream_t, Tiles, int, const double *, const double *, const double *,
+                                           const float *, double *, double *);
+template void bt_update_px<float, float>(hipStream_t, Tiles, int, const double *, const double *, const float *,
+                                         const float *, float *, float *);
+template void bt_update_px<float, _Float16>(hipStream_t, Tiles, int, const double *, const double *,
+                                            const _Float16 *, const float *, float *, float *);
 template <typename T>
 void bt_update_x(hipStream_t s, Tiles rt, int k, const double *alpha, const T *P, T *X) {
     hipLaunchKernelGGL(kb_update_x<T>, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, k, alpha, P, X);

@@ -66,10 +66,10 @@ __global__ __launch_bounds__(256) void k_to_f32(int64_t count, const double *__r
 // Tc[r] = forward: out[vert[r]] (the head's r_c - L_co y_o);
 //         backward: in[vert[r]] / D (D == 0 keeps the value, as the level
 //         solves do); zero on padding rows (vert < 0).
-template <typename T, bool BWD>
+template <typename T, typename V, bool BWD>
 __global__ __launch_bounds__(256) void k_core_gather(int64_t rows, const int32_t *__restrict__ vert,
                                                      const double *__restrict__ diag, int k,
-                                                     const T *__restrict__ src, T *__restrict__ tc) {
+                                                     const V *__restrict__ src, T *__restrict__ tc) {
     const int64_t total = rows * k;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
          e += (int64_t)gridDim.x * blockDim.x) {
@@ -78,7 +78,7 @@ __global__ __launch_bounds__(256) void k_core_gather(int64_t rows, const int32_t
         const int v = vert[r];
         T x = (T)0;
         if (v >= 0) {
-            x = src[(size_t)v * k + j];
+            x = (T)src[(size_t)v * k + j];
             if (BWD) {
                 const double d = diag[v];
                 if (d != 0.0) x = (T)((double)x / d);
@@ -292,11 +292,11 @@ __global__ __launch_bounds__(256) void k_core_inv_row(int it, int cp, const doub
 }
 
 // out[vert[r]] = sum of the row's pieces, in piece order.
-template <typename T>
+template <typename T, typename V>
 __global__ __launch_bounds__(256) void k_core_reduce(int32_t tiles, const int32_t *__restrict__ tile_piece,
                                                      const int32_t *__restrict__ tile_row,
                                                      const int32_t *__restrict__ vert, int k,
-                                                     const T *__restrict__ part, T *__restrict__ out) {
+                                                     const T *__restrict__ part, V *__restrict__ out) {
     const int64_t total = (int64_t)tiles * kCT * k;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
          e += (int64_t)gridDim.x * blockDim.x) {
@@ -307,7 +307,7 @@ __global__ __launch_bounds__(256) void k_core_reduce(int32_t tiles, const int32_
         const int p0 = tile_piece[tr], p1 = tile_piece[tr + 1];
         T s = part[(size_t)p0 * kCT * k + rem];
         for (int p = p0 + 1; p < p1; p++) s = s + part[(size_t)p * kCT * k + rem];
-        out[(size_t)v * k + rem % k] = s;
+        out[(size_t)v * k + rem % k] = (V)s;
     }
 }
 
@@ -338,12 +338,12 @@ void launch_to_f32(hipStream_t s, int64_t count, const double *a, float *b) {
     if (count > 0) hipLaunchKernelGGL(k_to_f32, dim3(stream_grid(count)), dim3(256), 0, s, count, a, b);
 }
 
-template <typename T, bool BWD>
-void launch_core_apply(hipStream_t s, const CoreApply &c, const T *M, int k, const double *diag, const T *src,
-                       T *tc, T *part, T *out) {
+template <typename T, typename V, bool BWD>
+void launch_core_apply(hipStream_t s, const CoreApply &c, const T *M, int k, const double *diag, const V *src,
+                       T *tc, T *part, V *out) {
     if (c.pieces <= 0) return;
-    hipLaunchKernelGGL((k_core_gather<T, BWD>), dim3(stream_grid(c.rows * k)), dim3(256), 0, s, c.rows, c.vert, diag,
-                       k, src, tc);
+    hipLaunchKernelGGL((k_core_gather<T, V, BWD>), dim3(stream_grid(c.rows * k)), dim3(256), 0, s, c.rows, c.vert,
+                       diag, k, src, tc);
     const CorePiece *pc = BWD ? c.piece_bwd : c.piece_fwd;
     if (k <= 64)
         hipLaunchKernelGGL((k_core_apply<T, BWD, 1>), dim3((unsigned)c.pieces), dim3(256), 0, s, pc, c.cpad,
@@ -354,15 +354,17 @@ void launch_core_apply(hipStream_t s, const CoreApply &c, const T *M, int k, con
     else
         hipLaunchKernelGGL((k_core_apply<T, BWD, 4>), dim3((unsigned)c.pieces), dim3(256), 0, s, pc, c.cpad,
                            c.dense_off, c.tc_off, M, k, tc, part);
-    hipLaunchKernelGGL(k_core_reduce<T>, dim3(stream_grid((int64_t)c.tiles * kCT * k)), dim3(256), 0, s, c.tiles,
-                       BWD ? c.tile_piece_bwd : c.tile_piece_fwd, c.tile_row, c.vert, k, part, out);
+    hipLaunchKernelGGL((k_core_reduce<T, V>), dim3(stream_grid((int64_t)c.tiles * kCT * k)), dim3(256), 0, s,
+                       c.tiles, BWD ? c.tile_piece_bwd : c.tile_piece_fwd, c.tile_row, c.vert, k, part, out);
 }
-#define SS_INST_CORE_APPLY(T, B)                                                                             \
-    template void launch_core_apply<T, B>(hipStream_t, const CoreApply &, const T *, int, const double *,     \
-                                          const T *, T *, T *, T *);
-SS_INST_CORE_APPLY(double, false)
-SS_INST_CORE_APPLY(double, true)
-SS_INST_CORE_APPLY(float, false)
-SS_INST_CORE_APPLY(float, true)
+#define SS_INST_CORE_APPLY(T, V, B)                                                                          \
+    template void launch_core_apply<T, V, B>(hipStream_t, const CoreApply &, const T *, int, const double *,  \
+                                             const V *, T *, T *, V *);
+SS_INST_CORE_APPLY(double, double, false)
+SS_INST_CORE_APPLY(double, double, true)
+SS_INST_CORE_APPLY(float, float, false)
+SS_INST_CORE_APPLY(float, float, true)
+SS_INST_CORE_APPLY(float, _Float16, false)
+SS_INST_CORE_APPLY(float, _Float16, true)
 
 }  // namespace ss

@@ -111,6 +111,8 @@ struct ss_graph {
     int nst = 0;
     int32_t *st_start = nullptr, *st_graph = nullptr, *sgtile = nullptr;
     int32_t *edge_off_dev = nullptr; // device copy of edge_off (int32)
+    int32_t *row_off_dev = nullptr;  // device copy of row_off (int32)
+    int32_t *row_graph = nullptr;    // per row: its graph (batches of more than one graph; else null)
     // canonical edges (global vertex ids)
     int32_t *eu = nullptr, *ev = nullptr;
     double *ew = nullptr;
@@ -174,6 +176,9 @@ struct ss_graph {
     double *gsum = nullptr, *csum = nullptr, *cmean = nullptr;
     double *sc_buf = nullptr;
     int *sc_int = nullptr, *colkeep = nullptr;
+    // mixed precision with FP16 preconditioner rows: per (graph, column)
+    // the rows' scale s (G x k), then 1 / s (G x k)
+    float *prec_sc = nullptr, *prec_inv = nullptr;
     ss::PcgScalars sc{};
     double *rel_ref = nullptr;       // per (graph, column): |r|^2 at the last reliable update
     int *rel_flag = nullptr;         // per (graph, column): replacement due

@@ -24,6 +24,7 @@
 #include <numeric>
 #include <string>
 #include <type_traits>
+#include <type_traits>
 
 #include "device.h"
 
@@ -314,6 +315,15 @@ int batch_upload(ss_context *ctx, const std::vector<const HostGraph *> &graphs, 
     SS_TRY(A.alloc(&g->st_graph, st_graph.size()));
     SS_TRY(A.alloc(&g->sgtile, g->stile_off.size()));
     SS_TRY(A.alloc(&g->edge_off_dev, edge_off32.size()));
+    std::vector<int32_t> row_off32(g->row_off.begin(), g->row_off.end());
+    SS_TRY(A.alloc(&g->row_off_dev, row_off32.size()));
+    std::vector<int32_t> row_graph;
+    if (G > 1) {
+        row_graph.resize((size_t)N);
+        for (int gi = 0; gi < G; gi++)
+            std::fill(row_graph.begin() + g->row_off[gi], row_graph.begin() + g->row_off[gi + 1], gi);
+        SS_TRY(A.alloc(&g->row_graph, (size_t)N));
+    }
     SS_TRY(A.alloc(&g->eu, M));
     SS_TRY(A.alloc(&g->ev, M));
     SS_TRY(A.alloc(&g->ew, M));
@@ -356,6 +366,8 @@ int batch_upload(ss_context *ctx, const std::vector<const HostGraph *> &graphs, 
     SS_TRY(h2d(s, g->st_graph, st_graph.data(), st_graph.size()));
     SS_TRY(h2d(s, g->sgtile, g->stile_off.data(), g->stile_off.size()));
     SS_TRY(h2d(s, g->edge_off_dev, edge_off32.data(), edge_off32.size()));
+    SS_TRY(h2d(s, g->row_off_dev, row_off32.data(), row_off32.size()));
+    if (G > 1) SS_TRY(h2d(s, g->row_graph, row_graph.data(), (size_t)N));
     SS_TRY(h2d(s, g->eu, eu.data(), M));
     SS_TRY(h2d(s, g->ev, ev.data(), M));
     SS_TRY(h2d(s, g->ew, ew.data(), M));
@@ -732,6 +744,7 @@ static void bind_scalars(ss_graph *g, int k) {
     g->sc.iters = g->sc_int + gk;
     g->rel_ref = g->sc_buf + 5 * gk;
     g->rel_flag = g->sc_int + 2 * gk;
+    if (g->prec_sc) g->prec_inv = g->prec_sc + gk;
 }
 
 size_t graph_work_bytes(const ss_graph *g, int k, int prec) {
@@ -742,7 +755,7 @@ size_t graph_work_bytes(const ss_graph *g, int k, int prec) {
     b += prec == SS_PREC_FP64 ? nk * 8 * 6 : nk * (8 * 2 + 4 * 2 + 4 * 6);
     b += ((size_t)std::max(std::max(g->nrt, g->nst), 1) + (size_t)std::max(g->nrt, 1)) * cap * 8 + gk * 8 +
          (size_t)g->ncomp * cap * 8 * 2;
-    b += gk * (6 * 8 + 3 * 4 + 4) + (size_t)cap * 8;
+    b += gk * (6 * 8 + 3 * 4 + 4 + (prec == SS_PREC_FP64 ? 0 : 2 * 4)) + (size_t)cap * 8;
     // the dense cores' gathered block and product pieces
     b += ((size_t)g->core.rows + (size_t)g->core.pieces * kCoreTile) * cap * (prec == SS_PREC_FP64 ? 8 : 4);
     return b;
@@ -769,6 +782,7 @@ int graph_ensure_work(ss_graph *g, int k, int prec) {
     g->wprec = -1;
     g->Y64 = g->X64 = g->R64 = g->B64 = nullptr;
     g->Y32 = g->X32 = nullptr;
+    g->prec_sc = g->prec_inv = nullptr;
     g->w64 = PcgWork<double>();
     g->w32 = PcgWork<float>();
     auto &A = g->arena_work;
@@ -792,6 +806,7 @@ int graph_ensure_work(ss_graph *g, int k, int prec) {
         SS_TRY(A.alloc(&g->Y32, nk));
         SS_TRY(A.alloc(&g->X32, nk));
         SS_TRY(alloc_work(g->w32));
+        SS_TRY(A.alloc(&g->prec_sc, 2 * gk));
     }
     SS_TRY(A.alloc(&g->partial, (size_t)std::max(std::max(g->nrt, g->nst), 1) * cap));
     SS_TRY(A.alloc(&g->partial2, (size_t)std::max(g->nrt, 1) * cap));
@@ -833,8 +848,11 @@ template <> const float *core_M<float>(const ss_graph *g) { return g->Mf; }
 
 // One direction of the sparse part of the preconditioner: per-level launches
 // between the runs of small levels, one single-workgroup launch per run.
-template <typename T, bool BWD>
-static void solve_levels(ss_graph *g, int k, const T *in, T *out) {
+// Arithmetic in T; start values read as TI (the solve's input), rows
+// written and gathered as V (T, or pairs of columns: FP32, or FP16 scaled
+// by rs).
+template <typename T, typename TI, typename V, bool BWD>
+static void solve_levels(ss_graph *g, int k, const TI *in, V *out, RowScale rs) {
     hipStream_t s = g->ctx->stream;
     const int rpw = g->ctx->tune.level_rows_per_wave;
     const std::vector<int32_t> &lp = BWD ? g->b_level_ptr : g->f_level_ptr;
@@ -853,7 +871,7 @@ static void solve_levels(ss_graph *g, int k, const T *in, T *out) {
         if (r < runs.size() && runs[r].first == l) {
             const int32_t e = runs[r++].second;
             ProfScope ps(g, BWD ? SS_KERNEL_TRI_BWD_SMALL : SS_KERNEL_TRI_FWD_SMALL);
-            launch_tri_small<T, BWD>(s, l, e, lp_dev, rows, ptr, col, val, g->diag, k, in, out);
+            launch_tri_small<T, TI, V, BWD>(s, l, e, lp_dev, rows, ptr, col, val, g->diag, k, in, out, rs);
             l = e;
             continue;
         }
@@ -861,44 +879,80 @@ static void solve_levels(ss_graph *g, int k, const T *in, T *out) {
         const int32_t e = r < runs.size() ? runs[r].first : depth;
         ProfScope ps(g, BWD ? SS_KERNEL_TRI_BWD : SS_KERNEL_TRI_FWD, e - l);
         for (; l < e; l++)
-            launch_trisolve_level<T, BWD>(s, lp[l], nhuge[l], hms[l], nlong[l] - nhuge[l], nmed[l], lp[l + 1],
-                                          waves[l], rpw,
-                                          g->ctx->tune.xcd_slices, rows, ptr, col, val, g->diag, k, in, out,
-                                          reinterpret_cast<T *>(g->huge_scratch), g->huge_counter);
+            launch_trisolve_level<T, TI, V, BWD>(s, lp[l], nhuge[l], hms[l], nlong[l] - nhuge[l], nmed[l], lp[l + 1],
+                                                 waves[l], rpw, g->ctx->tune.xcd_slices, rows, ptr, col, val,
+                                                 g->diag, k, in, out, g->huge_scratch, g->huge_counter, rs);
     }
 }
 
 // The dense cores of one direction: forward = their rows' entries outside
 // the cores (the head), then y_c = M (that); backward = z_c = M^T (D^+ y_c).
-template <typename T, bool BWD>
-static void solve_cores(ss_graph *g, int k, const T *in, T *out) {
+// A row-major n x k block of elements V seen as k scalars per row (the
+// dense-core and reduction kernels take one column per thread).
+template <typename V> struct ScalarOf { typedef V type; };
+template <> struct ScalarOf<f32x2> { typedef float type; };
+template <> struct ScalarOf<f16x2> { typedef _Float16 type; };
+
+template <typename T, typename TI, typename V, bool BWD>
+static void solve_cores(ss_graph *g, int k, const TI *in, V *out, RowScale rs) {
+    typedef typename ScalarOf<V>::type S;
     if (g->core_total == 0) return;
     hipStream_t s = g->ctx->stream;
     ProfScope ps(g, BWD ? SS_KERNEL_CORE_BWD : SS_KERNEL_CORE_FWD, BWD ? 3 : 4);
-    if (!BWD)
-        launch_core_head<T>(s, g->f_core_t0, g->core_huge, g->f_nrows, g->f_rows, g->f_ptr, g->f_col, fvals<T>(g),
-                            g->diag, k, in, out);
-    launch_core_apply<T, BWD>(s, g->core, core_M<T>(g), k, g->diag, BWD ? in : out, static_cast<T *>(g->core_tc),
-                              static_cast<T *>(g->core_part), out);
+    T *tc = static_cast<T *>(g->core_tc), *part = static_cast<T *>(g->core_part);
+    S *outs = reinterpret_cast<S *>(out);
+    if constexpr (BWD) {
+        launch_core_apply<T, S, true>(s, g->core, core_M<T>(g), k, g->diag, reinterpret_cast<const S *>(in), tc,
+                                      part, outs);
+    } else {
+        launch_core_head<T, TI, V>(s, g->f_core_t0, g->core_huge, g->f_nrows, g->f_rows, g->f_ptr, g->f_col,
+                                   fvals<T>(g), g->diag, k, in, out, rs);
+        launch_core_apply<T, S, false>(s, g->core, core_M<T>(g), k, g->diag, outs, tc, part, outs);
+    }
 }
 
 // Zp = L^{-T} D^+ L^{-1} R (W holds the forward result y).  rz_partial
 // receives the per-tile partials of r . Zp = y^T D^+ y.  Forward: the
 // sparse levels, then the dense cores (which only read finished rows);
 // backward: the cores first (their rows only need core rows), then the
-// sparse levels.
-template <typename T>
-static int precond(ss_graph *g, int k, const T *R, T *W, T *Zp, double *rz_partial) {
-    solve_levels<T, false>(g, k, R, W);
-    solve_cores<T, false>(g, k, R, W);
+// sparse levels.  V = FP16 pairs (mixed precision): W and Zp hold the
+// rows scaled by rs (the partials then carry the scale squared).
+template <typename T, typename VI, typename V>
+static int precond_v(ss_graph *g, int k, const VI *R, V *W, V *Zp, double *rz_partial, RowScale rs) {
+    typedef typename ScalarOf<V>::type S;
+    solve_levels<T, VI, V, false>(g, k, R, W, rs);
+    solve_cores<T, VI, V, false>(g, k, R, W, rs);
     {
         ProfScope ps(g, SS_KERNEL_VECTOR);
-        bt_tile_partial<T, T>(g->ctx->stream, rtiles(g), k, W, W, g->diag, 2, rz_partial);
+        const S *ws = reinterpret_cast<const S *>(W);
+        bt_tile_partial<S, S>(g->ctx->stream, rtiles(g), k, ws, ws, g->diag, 2, rz_partial);
     }
-    solve_cores<T, true>(g, k, W, Zp);
-    solve_levels<T, true>(g, k, W, Zp);
+    solve_cores<T, V, V, true>(g, k, W, Zp, rs);
+    solve_levels<T, V, V, true>(g, k, W, Zp, rs);
     SS_HIP(hipGetLastError());
     return SS_OK;
+}
+// FP64, or FP32 rows: two columns per lane where k is even (the same
+// arithmetic per column, half the gather instructions).
+template <typename T>
+static int precond(ss_graph *g, int k, const T *R, T *W, T *Zp, double *rz_partial) {
+    const RowScale none{nullptr, nullptr};
+    if constexpr (std::is_same<T, float>::value) {
+        if (k % 2 == 0)
+            return precond_v<float, f32x2, f32x2>(g, k, reinterpret_cast<const f32x2 *>(R),
+                                                  reinterpret_cast<f32x2 *>(W), reinterpret_cast<f32x2 *>(Zp),
+                                                  rz_partial, none);
+    }
+    return precond_v<T, T, T>(g, k, R, W, Zp, rz_partial, none);
+}
+
+// The mixed path's FP16 preconditioner rows, two columns per lane (k
+// even), in the FP32 W and Zp blocks (which hold twice the room), scaled per
+// (graph, column) by prec_sc.
+static RowScale prec_scale(const ss_graph *g) { return RowScale{g->prec_sc, g->row_graph}; }
+static int precond_half(ss_graph *g, int k, const PcgWork<float> &w) {
+    return precond_v<float, f32x2, f16x2>(g, k, reinterpret_cast<const f32x2 *>(w.R), reinterpret_cast<f16x2 *>(w.W),
+                                          reinterpret_cast<f16x2 *>(w.Zp), g->partial2, prec_scale(g));
 }
 
 static int read_status(ss_graph *g, int *nactive, int *nflagged = nullptr) {
@@ -932,14 +986,22 @@ static void component_means(ss_graph *g, int k, const double *Y) {
 // residual replacement instead of stopping them.
 template <typename T>
 static int pcg_iteration(ss_graph *g, int k, T *X, const PcgWork<T> &w, int rule, double tol, bool tail,
-                         double delta2 = -1.0) {
+                         double delta2 = -1.0, bool half = false) {
     hipStream_t s = g->ctx->stream;
     const Tiles rt = rtiles(g);
     const int gk = g->G * k;
     {
         ProfScope ps(g, SS_KERNEL_VECTOR);
         // P = Z + beta P; X += alpha P of the previous iteration
-        bt_update_px<T>(s, rt, k, g->sc.alpha, g->sc.beta, w.Zp, w.P, X);
+        if constexpr (std::is_same<T, float>::value) {
+            if (half)
+                bt_update_px<float, _Float16>(s, rt, k, g->sc.alpha, g->sc.beta,
+                                              reinterpret_cast<const _Float16 *>(w.Zp), g->prec_inv, w.P, X);
+            else
+                bt_update_px<T, T>(s, rt, k, g->sc.alpha, g->sc.beta, w.Zp, nullptr, w.P, X);
+        } else {
+            bt_update_px<T, T>(s, rt, k, g->sc.alpha, g->sc.beta, w.Zp, nullptr, w.P, X);
+        }
     }
     {
         ProfScope ps(g, SS_KERNEL_SPMM);
@@ -1082,7 +1144,7 @@ static const bool g_trace_rounds = [] {
 
 static int pcg_reliable(ss_graph *g, int k, int rule, double tol, double delta, int max_iter,
                         const std::vector<double> &bn2, std::vector<int32_t> &iters, int &rounds, bool &all_done,
-                        bool grouped) {
+                        bool grouped, bool half) {
     hipStream_t s = g->ctx->stream;
     const Tiles rt = rtiles(g);
     const int gk = g->G * k;
@@ -1094,15 +1156,21 @@ static int pcg_reliable(ss_graph *g, int k, int rule, double tol, double delta, 
         launch_fill_u32(s, (int64_t)2 * g->ncomp * k, reinterpret_cast<uint32_t *>(g->cmean), 0u);
         bt_project<float, float>(s, rt, k, g->Y32, g->comp_of_row, g->cmean, w.R, w.B, g->X32);
     }
-    SS_TRY(precond<float>(g, k, w.R, w.W, w.Zp, g->partial2));
     bt_tile_partial<float, float>(s, rt, k, w.B, w.B, nullptr, 0, g->partial);
-    bt_pcg_init(s, gk, k, g->gtile, g->partial, g->partial2, g->sc);
+    if (half) {
+        // FP16 preconditioner rows: R (= B) scaled to unit mean square
+        bt_prec_scale(s, gk, k, g->gtile, g->partial, nullptr, g->row_off_dev, g->prec_sc, g->prec_inv);
+        SS_TRY(precond_half(g, k, w));
+    } else {
+        SS_TRY(precond<float>(g, k, w.R, w.W, w.Zp, g->partial2));
+    }
+    bt_pcg_init(s, gk, k, g->gtile, g->partial, g->partial2, g->sc, half ? g->prec_sc : nullptr);
     SS_HIP(hipMemcpyAsync(g->rel_ref, bn2.data(), sizeof(double) * gk, hipMemcpyHostToDevice, s));
     bt_reliable_init(s, gk, g->sc, g->rel_ref, g->rel_flag, g->status);
     rounds = 0;
     all_done = false;
     for (int it = 0; it < max_iter; it++) {
-        SS_TRY(pcg_iteration<float>(g, k, g->X32, w, rule, tol, false, delta2));
+        SS_TRY(pcg_iteration<float>(g, k, g->X32, w, rule, tol, false, delta2, half));
         if (grouped) bt_reliable_group(s, gk, k, g->rel_gs, g->rel_ge, g->sc, g->rel_flag, g->status);
         int nactive = 0, nflag = 0;
         SS_TRY(read_status(g, &nactive, &nflag));
@@ -1130,8 +1198,15 @@ static int pcg_reliable(ss_graph *g, int k, int rule, double tol, double delta, 
             all_done = true;
             break;
         }
-        SS_TRY(precond<float>(g, k, w.R, w.W, w.Zp, g->partial2));
-        bt_pcg_beta(s, gk, k, g->gtile, g->partial2, g->sc);
+        if (half) {
+            // |R|^2 of this iteration's check (the replaced columns': their
+            // FP64 residual's)
+            bt_prec_scale(s, gk, k, g->gtile, nullptr, g->sc.rr, g->row_off_dev, g->prec_sc, g->prec_inv);
+            SS_TRY(precond_half(g, k, w));
+        } else {
+            SS_TRY(precond<float>(g, k, w.R, w.W, w.Zp, g->partial2));
+        }
+        bt_pcg_beta(s, gk, k, g->gtile, g->partial2, g->sc, half ? g->prec_sc : nullptr);
     }
     {
         // what the FP32 iterate holds past the last replacement (nothing
@@ -1198,9 +1273,10 @@ int graph_solve(ss_graph *g, int k, const ss_solve_opts &o, SolveStats &st, cons
         bt_project<double, double>(s, rt, k, g->Y64, g->comp_of_row, g->cmean, g->R64, g->B64, g->X64);
         SS_TRY(block_norms2(g, k, g->B64, bn2));
         if (!(o.inner_tol >= 0.0 && o.inner_tol < 1.0)) return fail(SS_ERR_INVALID, "inner_tol %g outside [0, 1)", o.inner_tol);
-        if (o.mixed_mode != SS_MIXED_RELIABLE && o.mixed_mode != SS_MIXED_RESTART)
+        if (o.mixed_mode != SS_MIXED_RELIABLE && o.mixed_mode != SS_MIXED_RESTART &&
+            o.mixed_mode != SS_MIXED_RELIABLE_F32)
             return fail(SS_ERR_INVALID, "unknown mixed_mode %d", o.mixed_mode);
-        if (o.mixed_mode == SS_MIXED_RELIABLE) {
+        if (o.mixed_mode == SS_MIXED_RELIABLE || o.mixed_mode == SS_MIXED_RELIABLE_F32) {
             bool grouped = false;
             if (groups) {
                 if ((int)groups->size() != k) return fail(SS_ERR_INVALID, "replacement groups: %zu for k=%d", groups->size(), k);
@@ -1219,7 +1295,8 @@ int graph_solve(ss_graph *g, int k, const ss_solve_opts &o, SolveStats &st, cons
             }
             bool all_done = false;
             SS_TRY(pcg_reliable(g, k, o.stop_rule, o.tol, o.inner_tol > 0.0 ? o.inner_tol : 3e-3, o.max_iter, bn2,
-                                st.iters, st.refinements, all_done, grouped));
+                                st.iters, st.refinements, all_done, grouped,
+                                o.mixed_mode == SS_MIXED_RELIABLE && k % 2 == 0));
             if (all_done) {
                 // every column stopped on its FP64 true residual, which the
                 // last replacement left in rel_ref (a zero right-hand side
@@ -1302,6 +1379,10 @@ static int autotune_t(ss_graph *g, int k, const PcgWork<T> &w, int reps, ss_tuni
     } else {
         bt_project<double, double>(s, rtiles(g), k, g->Y64, g->comp_of_row, g->cmean, g->R64, g->B64, g->X64);
         bt_convert_masked(s, rtiles(g), k, g->R64, nullptr, (float *)w.R);
+        // the mixed path's preconditioner (FP16 rows) is the one timed
+        bt_tile_partial<float, float>(s, rtiles(g), k, (const float *)w.R, (const float *)w.R, nullptr, 0,
+                                      g->partial);
+        bt_prec_scale(s, g->G * k, k, g->gtile, g->partial, nullptr, g->row_off_dev, g->prec_sc, g->prec_inv);
     }
     const ss_tuning_state orig = g->ctx->tune;
     const int orig_waves = g->tile_waves;
@@ -1311,7 +1392,12 @@ static int autotune_t(ss_graph *g, int k, const PcgWork<T> &w, int reps, ss_tuni
             g->ctx->tune.level_rows_per_wave = rpw;
             g->ctx->tune.xcd_slices = xs;
             double ms = 0;
-            SS_TRY(time_median(g, reps, [&]() { return precond<T>(g, k, w.R, w.W, w.Zp, g->partial2); }, ms));
+            SS_TRY(time_median(g, reps, [&]() {
+                if constexpr (std::is_same<T, float>::value) {
+                    if (k % 2 == 0) return precond_half(g, k, w);
+                }
+                return precond<T>(g, k, w.R, w.W, w.Zp, g->partial2);
+            }, ms));
             if (ms < ms_level) {
                 ms_level = ms;
                 best.level_rows_per_wave = rpw;
@@ -1326,7 +1412,7 @@ static int autotune_t(ss_graph *g, int k, const PcgWork<T> &w, int reps, ss_tuni
         const Tiles rt = rtiles(g);
         double ms = 0;
         SS_TRY(time_median(g, reps, [&]() {
-            bt_update_px<T>(s, rt, k, g->sc.alpha, g->sc.beta, w.Zp, w.P, w.B);
+            bt_update_px<T, T>(s, rt, k, g->sc.alpha, g->sc.beta, w.Zp, nullptr, w.P, w.B);
             bt_spmm_dot<T>(s, stiles(g), g->rowptr, g->col, avals<T>(g), k, w.P, w.Q, g->partial);
             bt_update_r_rr<T>(s, rt, k, g->sc.alpha, w.Q, w.W, g->partial);
             SS_HIP(hipGetLastError());

@@ -19,6 +19,20 @@ struct PcgScalars {
     int *iters;      // updates applied
 };
 
+// Two adjacent columns of a row: FP32 pairs and FP16 pairs (one lane of a
+// triangular solve moves both with one access).
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+
+// Scaling of FP16 solve rows (mixed precision): the forward solve's start
+// values are r times s[g * k + j] for a row of graph g (rg[row], graph 0
+// without rg); s == nullptr: no scaling (rows stored in the arithmetic's
+// own type).
+struct RowScale {
+    const float *s;
+    const int32_t *rg;
+};
+
 template <typename TZ>
 struct ErSampleArgs {
     int64_t m;
@@ -60,25 +74,30 @@ void launch_copy_f64(hipStream_t s, int64_t count, const double *a, double *b);
 // most kTinyRow entries each) go rpw to a wave.
 // waves == 0: no row of the level has entries (a flat copy of start values).
 // xcd != 0: each of the 8 XCDs takes one contiguous slice of the wave rows.
-template <typename T, bool BWD>
+// Arithmetic in T; start values read as TI (the solve's input), rows
+// written and gathered as V: T, or with FP32 arithmetic pairs of columns
+// (f32x2, or f16x2 scaled by rs; k even), TI the same width.  scratch holds
+// the huge rows' subtrees in the lane type (FP32 / FP64 values).
+template <typename T, typename TI, typename V, bool BWD>
 void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nhuge, int32_t hm, int32_t nlong, int32_t nmed,
                            int32_t t1, int waves, int rpw, int xcd, const int32_t *rows, const int32_t *ptr,
-                           const int32_t *col, const T *val, const double *diag, int k, const T *in, T *out,
-                           T *scratch, int *counter);
+                           const int32_t *col, const T *val, const double *diag, int k, const TI *in, V *out,
+                           void *scratch, int *counter, RowScale rs);
 // Small levels [l0, l1) (each of at most kSmallRows rows) walked by one
 // workgroup; level l holds positions [level_ptr[l], level_ptr[l+1]).
-template <typename T, bool BWD>
+template <typename T, typename TI, typename V, bool BWD>
 void launch_tri_small(hipStream_t s, int32_t l0, int32_t l1, const int32_t *level_ptr, const int32_t *rows,
-                      const int32_t *ptr, const int32_t *col, const T *val, const double *diag, int k, const T *in,
-                      T *out);
+                      const int32_t *ptr, const int32_t *col, const T *val, const double *diag, int k, const TI *in,
+                      V *out, RowScale rs);
 // Dense cores (per graph, the last rows of its elimination order; see
 // core_kernels.hip).  Forward: launch_core_head (the core rows' entries
 // outside the core, schedule positions [t0, t1), the first nhuge of more
 // than kWgRowMax entries, writing r_c - L_co y_o to the core rows), then
 // launch_core_apply<T, false>.  Backward: launch_core_apply<T, true> first.
-template <typename T>
+template <typename T, typename TI, typename V>
 void launch_core_head(hipStream_t s, int32_t t0, int32_t nhuge, int32_t t1, const int32_t *rows, const int32_t *ptr,
-                      const int32_t *col, const T *val, const double *diag, int k, const T *in, T *out);
+                      const int32_t *col, const T *val, const double *diag, int k, const TI *in, V *out,
+                      RowScale rs);
 // A piece of a core product: rows [r0, r0 + kCoreTile) of graph `graph`'s
 // core times inner rows [m0, m1) (at most kCoreSeg of them).
 struct CorePiece {
@@ -102,9 +121,10 @@ void launch_core_invert(hipStream_t s, int cp, const double *L, double *M);
 void launch_to_f32(hipStream_t s, int64_t count, const double *a, float *b);
 // Forward: out[core rows] = M (out[core rows]); backward: out[core rows] =
 // M^T (D^+ src[core rows]).  tc: rows x k scratch; part: pieces x kCoreTile x k.
-template <typename T, bool BWD>
-void launch_core_apply(hipStream_t s, const CoreApply &c, const T *M, int k, const double *diag, const T *src,
-                       T *tc, T *part, T *out);
+// src and out rows are stored as V (T, or scaled FP16).
+template <typename T, typename V, bool BWD>
+void launch_core_apply(hipStream_t s, const CoreApply &c, const T *M, int k, const double *diag, const V *src,
+                       T *tc, T *part, V *out);
 template <typename TZ>
 void launch_er_sample(hipStream_t s, const ErSampleArgs<TZ> &a);
 

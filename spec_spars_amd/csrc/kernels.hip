@@ -25,27 +25,6 @@ __device__ __forceinline__ int wave_uniform(int x) { return __builtin_amdgcn_rea
 __device__ __forceinline__ double fnms(double v, double x, double acc) { return __builtin_fma(-v, x, acc); }
 __device__ __forceinline__ float fnms(float v, float x, float acc) { return __builtin_fmaf(-v, x, acc); }
 
-// acc -= sum_q val[q] * X[col[q]*k + j] over q in [q0, q1) with stride
-// `step`, 8 gathers in flight, terms subtracted in q order.
-template <typename T>
-__device__ __forceinline__ T row_dot_sub(T acc, int q0, int q1, int step, const int32_t *__restrict__ col,
-                                         const T *__restrict__ val, const T *X, int k, int j) {
-    int q = q0;
-    for (; q + 7 * step < q1; q += 8 * step) {
-        T x[8], v[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-            x[u] = X[(size_t)col[q + u * step] * k + j];
-            v[u] = val[q + u * step];
-        }
-#pragma unroll
-        for (int u = 0; u < 8; u++) acc = fnms(v[u], x[u], acc);
-    }
-    for (; q < q1; q += step) acc = fnms(val[q], X[(size_t)col[q] * k + j], acc);
-    return acc;
-}
-
-
 // ---------------------------------------------------------------------------
 // (a) sketch, gather form: Y[x, j] = sum over incident edges e of x, in
 // increasing e, of  fl(scale * sign(e, j)) * b(x, e),  b = +-sqrt(w).
@@ -99,12 +78,100 @@ __global__ void k_fill_u32(int64_t count, uint32_t *p, uint32_t v) {
 //   backward (BWD=true):  z_i = y_i / D_i - sum_{j>i} L[j,i] z_j
 //                         (division skipped where D_i == 0, as
 //                          include/custom_cg.hpp:227-233 does)
+//
+// A row of the solve is stored as ke = k / n elements of type V, each
+// holding n = Lane<V>::n adjacent columns: V = T (n = 1), or for FP32
+// arithmetic two columns per element, as FP32 (f32x2) or FP16 (f16x2), so
+// one lane moves two columns with one 8- or 4-byte access.  A lane's value
+// is Lane<V>::type; the arithmetic of every column is the same whatever n
+// is.  The solve's input (forward: the PCG residual; backward: the forward
+// result) is read the same way, as elements of type VI.
 // ---------------------------------------------------------------------------
-// Start value of row i: forward r_i; backward y_i / D_i (D_i == 0 keeps y_i).
-template <typename T, bool BWD>
-__device__ __forceinline__ T row_start(int i, int j, int k, double d, const T *in) {
-    T acc = in[(size_t)i * k + j];
-    if (BWD && d != 0.0) acc = (T)((double)acc / d);
+template <typename V> struct Lane;
+template <> struct Lane<double> { typedef double type; static constexpr int n = 1; };
+template <> struct Lane<float> { typedef float type; static constexpr int n = 1; };
+template <> struct Lane<f32x2> { typedef f32x2 type; static constexpr int n = 2; };
+template <> struct Lane<f16x2> { typedef f32x2 type; static constexpr int n = 2; };
+
+// An element as stored -> the lane value.  The gather loops hold the
+// stored elements and widen them where they are used, so all of a group's
+// loads are issued before the first wait.  Stored elements that need
+// widening (FP16) are never loaded under a branch: a slot past the row's
+// entries loads an entry of the group that exists and its term is
+// selected away, so the compiler cannot tie a widening, and a wait, to
+// each load.
+__device__ __forceinline__ double cvt(double x) { return x; }
+__device__ __forceinline__ float cvt(float x) { return x; }
+__device__ __forceinline__ f32x2 cvt(f32x2 x) { return x; }
+__device__ __forceinline__ f32x2 cvt(f16x2 x) { return __builtin_convertvector(x, f32x2); }
+template <typename V> __device__ __forceinline__ typename Lane<V>::type ldv(const V *p) { return cvt(*p); }
+template <typename V> constexpr bool widened() { return sizeof(typename Lane<V>::type) > sizeof(V); }
+__device__ __forceinline__ void stv(double *p, double v) { *p = v; }
+__device__ __forceinline__ void stv(float *p, float v) { *p = v; }
+__device__ __forceinline__ void stv(f32x2 *p, f32x2 v) { *p = v; }
+__device__ __forceinline__ void stv(f16x2 *p, f32x2 v) { *p = __builtin_convertvector(v, f16x2); }
+
+// Two-column lanes: each column its own fused multiply-add / rounding.
+__device__ __forceinline__ f32x2 fnms(float v, f32x2 x, f32x2 acc) {
+    f32x2 r;
+    r.x = __builtin_fmaf(-v, x.x, acc.x);
+    r.y = __builtin_fmaf(-v, x.y, acc.y);
+    return r;
+}
+__device__ __forceinline__ double div_d(double x, double d) { return x / d; }
+__device__ __forceinline__ float div_d(float x, double d) { return (float)((double)x / d); }
+__device__ __forceinline__ f32x2 div_d(f32x2 x, double d) {
+    f32x2 r;
+    r.x = (float)((double)x.x / d);
+    r.y = (float)((double)x.y / d);
+    return r;
+}
+
+// Column scale of element e of a row of graph g (FP16 rows): n floats at
+// s[g * k + n e].
+template <typename LV> __device__ __forceinline__ LV scale_at(const float *s, size_t base, int e);
+template <> __device__ __forceinline__ float scale_at<float>(const float *s, size_t base, int e) {
+    return s[base + e];
+}
+template <> __device__ __forceinline__ f32x2 scale_at<f32x2>(const float *s, size_t base, int e) {
+    return reinterpret_cast<const f32x2 *>(s + base)[e];
+}
+template <> __device__ __forceinline__ double scale_at<double>(const float *s, size_t base, int e) {
+    return (double)s[base + e];
+}
+
+// acc -= sum_q val[q] * X[col[q]*ke + e] over q in [q0, q1) with stride
+// `step`, 8 gathers in flight, terms subtracted in q order.
+template <typename T, typename V>
+__device__ __forceinline__ typename Lane<V>::type row_dot_sub(typename Lane<V>::type acc, int q0, int q1, int step,
+                                                              const int32_t *__restrict__ col,
+                                                              const T *__restrict__ val, const V *X, int ke, int e) {
+    int q = q0;
+    for (; q + 7 * step < q1; q += 8 * step) {
+        V x[8];
+        T v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            x[u] = X[(size_t)col[q + u * step] * ke + e];
+            v[u] = val[q + u * step];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc = fnms(v[u], cvt(x[u]), acc);
+    }
+    for (; q < q1; q += step) acc = fnms(val[q], ldv(X + (size_t)col[q] * ke + e), acc);
+    return acc;
+}
+
+// Start value of element e of row i: forward r_i; backward y_i / D_i (D_i
+// == 0 keeps y_i).  With rs.s (FP16 rows), the forward start is r_i times
+// its columns' scales rs.s[g * k + column] (g = the row's graph), so every
+// row of the solve holds the scaled vector; the backward reads rows already
+// scaled.
+template <typename LV, bool BWD, typename VI>
+__device__ __forceinline__ LV row_start(int i, int e, int k, int ke, double d, const VI *in, RowScale rs) {
+    LV acc = ldv(in + (size_t)i * ke + e);
+    if (BWD && d != 0.0) acc = div_d(acc, d);
+    if (!BWD && rs.s) acc = acc * scale_at<LV>(rs.s, (size_t)(rs.rg ? rs.rg[i] : 0) * k, e);
     return acc;
 }
 
@@ -117,7 +184,7 @@ __device__ __forceinline__ void tree_fold(T (&v)[N]) {
         for (int u = 0; u + step < N; u += 2 * step) v[u] = v[u] + v[u + step];
 }
 
-// Tree fold of sm[first .. first + N) (column lane).
+// Tree fold of sm[first .. first + N) (element lane).
 template <typename T, int N>
 __device__ __forceinline__ T lds_tree_n(T (*sm)[64], int first, int lane) {
     T v[N];
@@ -149,45 +216,56 @@ __device__ __forceinline__ double lane_get(double v, int l) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Parts of a row whose entries sit one per lane (entry e in lane e & 63 of
-// c0/v0 below 64, of c1/v1 from 64): gathered 16 at a time, entry e into
-// part e % P.  Lanes past the row's end hold zero values, and their terms
+// Parts of a row whose entries sit one per lane (entry q in lane q & 63 of
+// c0/v0 below 64, of c1/v1 from 64): gathered 16 at a time, entry q into
+// part q % P.  Lanes past the row's end hold zero values, and their terms
 // (zero times zero) leave a part exact.
-template <typename T, int P>
-__device__ __forceinline__ T wave_parts(int len, int c0, int c1, T v0, T v1, const T *X, int k, int jj) {
-    T part[P];
+template <typename T, typename V, int P>
+__device__ __forceinline__ typename Lane<V>::type wave_parts(int len, int c0, int c1, T v0, T v1, const V *X, int ke,
+                                                             int ee) {
+    typedef typename Lane<V>::type LV;
+    constexpr int GW = 16;
+    LV part[P];
 #pragma unroll
-    for (int u = 0; u < P; u++) part[u] = (T)0;
-    for (int e0 = 0; e0 < len; e0 += 16) {
+    for (int u = 0; u < P; u++) part[u] = LV();
+    for (int e0 = 0; e0 < len; e0 += GW) {
         const bool hi = e0 >= 64;
         const int cs = hi ? c1 : c0;
         const T vs = hi ? v1 : v0;
-        T x[16];
+        V x[GW];
+        const int cf = lane_get(cs, e0 & 63);   // entry e0 exists
 #pragma unroll
-        for (int u = 0; u < 16; u++) {
+        for (int u = 0; u < GW; u++) {
             const int c = lane_get(cs, (e0 + u) & 63);
-            x[u] = e0 + u < len ? X[(size_t)c * k + jj] : (T)0;
+            if constexpr (widened<V>())
+                x[u] = X[(size_t)(e0 + u < len ? c : cf) * ke + ee];
+            else
+                x[u] = e0 + u < len ? X[(size_t)c * ke + ee] : V();
         }
 #pragma unroll
-        for (int u = 0; u < 16; u++) part[u % P] = fnms(lane_get(vs, (e0 + u) & 63), x[u], part[u % P]);
+        for (int u = 0; u < GW; u++)
+            part[u % P] = fnms(lane_get(vs, (e0 + u) & 63), e0 + u < len ? cvt(x[u]) : LV(), part[u % P]);
     }
-    tree_fold<T, P>(part);
+    tree_fold<LV, P>(part);
     return part[0];
 }
 
 // Tree fold of parts [s0, s0 + NA) of segment [a, b) split in P parts, on
 // one wave: slot l of a 64-slot round holds entry a + s0 + l % NA + P (l / NA)
 // (so part s0 + u takes its entries in order), the slots' entries load one
-// per lane, then gather 16 at a time.  Slots past the segment's end hold
+// per lane, then gather GW at a time.  Slots past the segment's end hold
 // zero values, whose terms leave a part exact.
-template <typename T, int NA, int GW = 16>
-__device__ __forceinline__ T wave_subtree(int a, int b, int P, int s0, const int32_t *__restrict__ col,
-                                          const T *__restrict__ val, const T *X, int k, int jj, int lane) {
+template <typename T, typename V, int NA, int GW = 16>
+__device__ __forceinline__ typename Lane<V>::type wave_subtree(int a, int b, int P, int s0,
+                                                               const int32_t *__restrict__ col,
+                                                               const T *__restrict__ val, const V *X, int ke, int ee,
+                                                               int lane) {
+    typedef typename Lane<V>::type LV;
     static_assert(NA == 1 || NA == 2 || NA == 4 || NA == 8 || NA == 16, "parts per wave");
     static_assert((GW == 8 || GW == 16) && NA <= GW, "gathers in flight: a whole number of slots per part");
-    T part[NA];
+    LV part[NA];
 #pragma unroll
-    for (int u = 0; u < NA; u++) part[u] = (T)0;
+    for (int u = 0; u < NA; u++) part[u] = LV();
     const int a0 = a + s0;
     for (int base = 0; a0 + P * (base / NA) < b; base += 64) {
         const int l = base + lane;
@@ -195,41 +273,54 @@ __device__ __forceinline__ T wave_subtree(int a, int b, int P, int s0, const int
         const int c = q < b ? col[q] : 0;
         const T v = q < b ? val[q] : (T)0;
         for (int e0 = 0; e0 < 64 && a0 + P * ((base + e0) / NA) < b; e0 += GW) {
-            T x[GW];
+            V x[GW];
+            const int cf = lane_get(c, e0);   // slot e0's entry exists (the loop's test)
 #pragma unroll
             for (int u = 0; u < GW; u++) {
                 const int cu = lane_get(c, e0 + u);
                 const int lu = base + e0 + u;
-                x[u] = a0 + lu % NA + P * (lu / NA) < b ? X[(size_t)cu * k + jj] : (T)0;
+                const bool on = a0 + lu % NA + P * (lu / NA) < b;
+                if constexpr (widened<V>())
+                    x[u] = X[(size_t)(on ? cu : cf) * ke + ee];
+                else
+                    x[u] = on ? X[(size_t)cu * ke + ee] : V();
             }
 #pragma unroll
-            for (int u = 0; u < GW; u++) part[u % NA] = fnms(lane_get(v, e0 + u), x[u], part[u % NA]);
+            for (int u = 0; u < GW; u++) {
+                const int lu = base + e0 + u;
+                const bool on = a0 + lu % NA + P * (lu / NA) < b;
+                part[u % NA] = fnms(lane_get(v, e0 + u), on ? cvt(x[u]) : LV(), part[u % NA]);
+            }
         }
     }
-    tree_fold<T, NA>(part);
+    tree_fold<LV, NA>(part);
     return part[0];
 }
 
 // na in {1, 2, 4, 8, 16}, na <= MAXNA (the bound keeps wider variants out
 // of a kernel's registers).
-template <typename T, int MAXNA>
-__device__ __forceinline__ T wave_subtree_n(int na, int a, int b, int P, int s0, const int32_t *__restrict__ col,
-                                            const T *__restrict__ val, const T *X, int k, int jj, int lane) {
-    if (na == 1 || MAXNA == 1) return wave_subtree<T, 1>(a, b, P, s0, col, val, X, k, jj, lane);
-    if (na == 2 || MAXNA == 2) return wave_subtree<T, (MAXNA < 2 ? MAXNA : 2)>(a, b, P, s0, col, val, X, k, jj, lane);
-    if (na == 4 || MAXNA == 4) return wave_subtree<T, (MAXNA < 4 ? MAXNA : 4)>(a, b, P, s0, col, val, X, k, jj, lane);
-    if (na == 8 || MAXNA == 8) return wave_subtree<T, (MAXNA < 8 ? MAXNA : 8)>(a, b, P, s0, col, val, X, k, jj, lane);
-    return wave_subtree<T, MAXNA>(a, b, P, s0, col, val, X, k, jj, lane);
+template <typename T, typename V, int MAXNA>
+__device__ __forceinline__ typename Lane<V>::type wave_subtree_n(int na, int a, int b, int P, int s0,
+                                                                 const int32_t *__restrict__ col,
+                                                                 const T *__restrict__ val, const V *X, int ke,
+                                                                 int ee, int lane) {
+    constexpr int M = MAXNA;
+    if (na == 1 || M == 1) return wave_subtree<T, V, 1>(a, b, P, s0, col, val, X, ke, ee, lane);
+    if (na == 2 || M == 2) return wave_subtree<T, V, (M < 2 ? M : 2)>(a, b, P, s0, col, val, X, ke, ee, lane);
+    if (na == 4 || M == 4) return wave_subtree<T, V, (M < 4 ? M : 4)>(a, b, P, s0, col, val, X, ke, ee, lane);
+    if (na == 8 || M == 8) return wave_subtree<T, V, (M < 8 ? M : 8)>(a, b, P, s0, col, val, X, ke, ee, lane);
+    return wave_subtree<T, V, M>(a, b, P, s0, col, val, X, ke, ee, lane);
 }
 
 // One row of at most kWaveRowMax entries on one wave: the row's entries are
 // loaded one per lane, so a single dependent load separates the row record
 // from all of its gathers.
-template <typename T, bool BWD>
+template <typename T, typename VI, typename V, bool BWD>
 __device__ __forceinline__ void wave_row(int t, const int32_t *__restrict__ rows, const int32_t *__restrict__ ptr,
                                          const int32_t *__restrict__ col, const T *__restrict__ val,
-                                         const double *__restrict__ diag, int k, const T *__restrict__ in, T *out,
-                                         int lane) {
+                                         const double *__restrict__ diag, int k, int ke, const VI *__restrict__ in,
+                                         V *out, RowScale rs, int lane) {
+    typedef typename Lane<V>::type LV;
     const int i = wave_uniform(rows[t]);
     const int q0 = wave_uniform(ptr[t]), len = wave_uniform(ptr[t + 1]) - q0;
     const int c0 = lane < len ? col[q0 + lane] : 0;
@@ -238,17 +329,18 @@ __device__ __forceinline__ void wave_row(int t, const int32_t *__restrict__ rows
     const T v1 = 64 + lane < len ? val[q0 + 64 + lane] : (T)0;
     const double d = BWD ? diag[i] : 1.0;
     const int P = seg_parts(len);
-    for (int jb = 0; jb < k; jb += 64) {
-        const int j = jb + lane, jj = j < k ? j : k - 1;
-        T acc = row_start<T, BWD>(i, jj, k, d, in);
+    for (int eb = 0; eb < ke; eb += 64) {
+        const int e = eb + lane, ee = e < ke ? e : ke - 1;
+        LV acc = row_start<LV, BWD>(i, ee, k, ke, d, in, rs);
         if (P == 1) {
-            for (int e = 0; e < len; e++) acc = fnms(lane_get(v0, e), out[(size_t)lane_get(c0, e) * k + jj], acc);
+            for (int q = 0; q < len; q++)
+                acc = fnms(lane_get(v0, q), ldv(out + (size_t)lane_get(c0, q) * ke + ee), acc);
         } else if (P == 4) {
-            acc = acc + wave_parts<T, 4>(len, c0, c1, v0, v1, out, k, jj);
+            acc = acc + wave_parts<T, V, 4>(len, c0, c1, v0, v1, out, ke, ee);
         } else {
-            acc = acc + wave_parts<T, 8>(len, c0, c1, v0, v1, out, k, jj);
+            acc = acc + wave_parts<T, V, 8>(len, c0, c1, v0, v1, out, ke, ee);
         }
-        if (j < k) out[(size_t)i * k + j] = acc;
+        if (e < ke) stv(out + (size_t)i * ke + e, acc);
     }
 }
 
@@ -257,11 +349,12 @@ __device__ __forceinline__ void wave_row(int t, const int32_t *__restrict__ rows
 // schedule arrays carry kTinyGroup entries of padding), their entries
 // (contiguous, at most 32) one per lane, every gather of the group is in
 // flight before the first row finishes, and the stores come last.
-template <typename T, bool BWD, int R>
+template <typename T, typename VI, typename V, bool BWD, int R>
 __device__ __forceinline__ void wave_tiny_rows(int tb, int cnt, const int32_t *__restrict__ rows,
                                                const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
                                                const T *__restrict__ val, const double *__restrict__ diag, int k,
-                                               const T *__restrict__ in, T *out, int lane) {
+                                               int ke, const VI *__restrict__ in, V *out, RowScale rs, int lane) {
+    typedef typename Lane<V>::type LV;
     static_assert(R >= 1 && R <= kTinyGroup, "at most kTinyGroup tiny rows at a time");
     constexpr int E = kTinyRow;
     int ir[R], pr[R + 1];
@@ -281,69 +374,76 @@ __device__ __forceinline__ void wave_tiny_rows(int tb, int cnt, const int32_t *_
     }
     if (ne == 0) {
         // no entries in the group: every row is its start value
-        for (int jb = 0; jb < k; jb += 64) {
-            const int j = jb + lane;
-            if (j >= k) break;
-            T st[R];
+        for (int eb = 0; eb < ke; eb += 64) {
+            const int e = eb + lane;
+            if (e >= ke) break;
+            LV st[R];
 #pragma unroll
-            for (int r = 0; r < R; r++) st[r] = r < cnt ? row_start<T, BWD>(ir[r], j, k, dr[r], in) : (T)0;
+            for (int r = 0; r < R; r++) st[r] = r < cnt ? row_start<LV, BWD>(ir[r], e, k, ke, dr[r], in, rs) : LV();
 #pragma unroll
             for (int r = 0; r < R; r++)
-                if (r < cnt) out[(size_t)ir[r] * k + j] = st[r];
+                if (r < cnt) stv(out + (size_t)ir[r] * ke + e, st[r]);
         }
         return;
     }
     const int lc = lane < ne ? col[base + lane] : 0;
     const T lv = lane < ne ? val[base + lane] : (T)0;
-    for (int jb = 0; jb < k; jb += 64) {
-        const int j = jb + lane, jj = j < k ? j : k - 1;
-        T st[R], x[R][E];
+    for (int eb = 0; eb < ke; eb += 64) {
+        const int e = eb + lane, ee = e < ke ? e : ke - 1;
+        LV st[R];
+        V x[R][E];
+        const int cf = lane_get(lc, 0);   // the group's first entry (ne > 0)
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            st[r] = r < cnt ? row_start<T, BWD>(ir[r], jj, k, dr[r], in) : (T)0;
+            st[r] = r < cnt ? row_start<LV, BWD>(ir[r], ee, k, ke, dr[r], in, rs) : LV();
 #pragma unroll
-            for (int e = 0; e < E; e++) {
-                const int c = lane_get(lc, (ar[r] + e) & 63);
-                x[r][e] = e < nr[r] ? out[(size_t)c * k + jj] : (T)0;
+            for (int q = 0; q < E; q++) {
+                const int c = lane_get(lc, (ar[r] + q) & 63);
+                if constexpr (widened<V>())
+                    x[r][q] = out[(size_t)(q < nr[r] ? c : cf) * ke + ee];
+                else
+                    x[r][q] = q < nr[r] ? out[(size_t)c * ke + ee] : V();
             }
         }
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            T acc = st[r];
+            LV acc = st[r];
 #pragma unroll
-            for (int e = 0; e < E; e++) {
-                const T v = e < nr[r] ? lane_get(lv, (ar[r] + e) & 63) : (T)0;
-                acc = fnms(v, x[r][e], acc);
+            for (int q = 0; q < E; q++) {
+                const T v = q < nr[r] ? lane_get(lv, (ar[r] + q) & 63) : (T)0;
+                acc = fnms(v, q < nr[r] ? cvt(x[r][q]) : LV(), acc);
             }
-            if (r < cnt && j < k) out[(size_t)ir[r] * k + j] = acc;
+            if (r < cnt && e < ke) stv(out + (size_t)ir[r] * ke + e, acc);
         }
     }
 }
 
 // A level whose rows have no entries (the first forward level: the rows
 // that depend on nothing): every row is its start value.  Thread i of a
-// block copies column i % k of rows i / k, i / k + 256 / k, ..., four rows
-// in flight, so a block pass moves whole rows as one contiguous run.
-template <typename T, bool BWD>
+// block copies element i % ke of rows i / ke, i / ke + 256 / ke, ..., four
+// rows in flight, so a block pass moves whole rows as one contiguous run.
+template <typename T, typename VI, typename V, bool BWD>
 __global__ __launch_bounds__(256) void k_level_copy(int32_t t0, int32_t nrows, const int32_t *__restrict__ rows,
                                                     const double *__restrict__ diag, int k,
-                                                    const T *__restrict__ in, T *__restrict__ out) {
-    // thread i copies column i % k of every (256 / k)-th row, 4 rows in flight
-    const int RP = (int)blockDim.x / k, i = (int)threadIdx.x;
-    if (i >= RP * k) return;
-    const int j = i % k, ro = i / k;
+                                                    const VI *__restrict__ in, V *__restrict__ out, RowScale rs) {
+    typedef typename Lane<V>::type LV;
+    const int ke = k / Lane<V>::n;
+    // thread i copies element i % ke of every (256 / ke)-th row, 4 rows in flight
+    const int RP = (int)blockDim.x / ke, i = (int)threadIdx.x;
+    if (i >= RP * ke) return;
+    const int e = i % ke, ro = i / ke;
     const int64_t step = (int64_t)gridDim.x * RP * 4;
     for (int64_t tb = (int64_t)blockIdx.x * RP * 4 + ro; tb < nrows; tb += step) {
         int r[4];
-        T v[4];
+        LV v[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) r[u] = tb + u * RP < nrows ? rows[t0 + tb + u * RP] : 0;
 #pragma unroll
         for (int u = 0; u < 4; u++)
-            v[u] = tb + u * RP < nrows ? row_start<T, BWD>(r[u], j, k, BWD ? diag[r[u]] : 1.0, in) : (T)0;
+            v[u] = tb + u * RP < nrows ? row_start<LV, BWD>(r[u], e, k, ke, BWD ? diag[r[u]] : 1.0, in, rs) : LV();
 #pragma unroll
         for (int u = 0; u < 4; u++)
-            if (tb + u * RP < nrows) out[(size_t)r[u] * k + j] = v[u];
+            if (tb + u * RP < nrows) stv(out + (size_t)r[u] * ke + e, v[u]);
     }
 }
 
@@ -353,20 +453,23 @@ __global__ __launch_bounds__(256) void k_level_copy(int32_t t0, int32_t nrows, c
 // bits.  hm == 1: one workgroup per row, wave w folding parts [w P/16,
 // (w+1) P/16).  hm > 1: H = P / 16 workgroups per row (hm launched per
 // row), workgroup h folding parts [16h, 16h + 16) with one wave per part and
-// storing that subtree of every column chunk in its scratch slot; the last
-// of the row's workgroups to arrive (one device-scope counter per row, reset
-// by that workgroup) folds the H subtrees, adds them to the start value and
-// stores the row.
-template <typename T, bool BWD>
+// storing that subtree of every element in its scratch slot (lane values,
+// full precision); the last of the row's workgroups to arrive (one
+// device-scope counter per row, reset by that workgroup) folds the H
+// subtrees, adds them to the start value and stores the row.
+template <typename T, typename VI, typename V, bool BWD>
 __global__ __launch_bounds__(64 * kLevelWaves) void k_trisolve_huge(int32_t t0, int32_t hm,
                                                                     const int32_t *__restrict__ rows,
                                                                     const int32_t *__restrict__ ptr,
                                                                     const int32_t *__restrict__ col,
                                                                     const T *__restrict__ val,
                                                                     const double *__restrict__ diag, int k,
-                                                                    const T *__restrict__ in, T *out, T *scratch,
-                                                                    int *counter) {
-    __shared__ T sm[kWgParts][64];
+                                                                    const VI *__restrict__ in, V *out,
+                                                                    typename Lane<V>::type *scratch, int *counter,
+                                                                    RowScale rs) {
+    typedef typename Lane<V>::type LV;
+    const int ke = k / Lane<V>::n;
+    __shared__ LV sm[kWgParts][64];
     const int lane = SS_LANE, wv = SS_WAVE_IN_BLOCK;
     const int slot = (int)blockIdx.x / hm, h = (int)blockIdx.x - slot * hm;
     const int t = t0 + slot;
@@ -377,37 +480,38 @@ __global__ __launch_bounds__(64 * kLevelWaves) void k_trisolve_huge(int32_t t0, 
     const double d = BWD ? diag[i] : 1.0;
     if (hm == 1) {
         const int na = P / kWgParts;
-        for (int jb = 0; jb < k; jb += 64) {
-            const int j = jb + lane, jj = j < k ? j : k - 1;
-            if constexpr (sizeof(T) == 8) {
-                // FP64: 8 parts as two subtrees of 4 (the same tree), 8
-                // gathers in flight, which keeps the kernel within the 128
-                // registers a 1,024-thread workgroup leaves each lane
+        for (int eb = 0; eb < ke; eb += 64) {
+            const int e = eb + lane, ee = e < ke ? e : ke - 1;
+            if constexpr (sizeof(LV) == 8) {
+                // 8-byte lane values: 8 parts as two subtrees of 4 (the
+                // same tree), 8 gathers in flight, which keeps the kernel
+                // within the 128 registers a 1,024-thread workgroup leaves
+                // each lane
                 if (na == 4)
-                    sm[wv][lane] = wave_subtree<T, 4, 8>(q0, q1, P, wv * 4, col, val, out, k, jj, lane);
+                    sm[wv][lane] = wave_subtree<T, V, 4, 8>(q0, q1, P, wv * 4, col, val, out, ke, ee, lane);
                 else
-                    sm[wv][lane] = wave_subtree<T, 4, 8>(q0, q1, P, wv * 8, col, val, out, k, jj, lane) +
-                                   wave_subtree<T, 4, 8>(q0, q1, P, wv * 8 + 4, col, val, out, k, jj, lane);
+                    sm[wv][lane] = wave_subtree<T, V, 4, 8>(q0, q1, P, wv * 8, col, val, out, ke, ee, lane) +
+                                   wave_subtree<T, V, 4, 8>(q0, q1, P, wv * 8 + 4, col, val, out, ke, ee, lane);
             } else {
-                sm[wv][lane] = wave_subtree_n<T, kMaxWg>(na, q0, q1, P, wv * na, col, val, out, k, jj, lane);
+                sm[wv][lane] = wave_subtree_n<T, V, kMaxWg>(na, q0, q1, P, wv * na, col, val, out, ke, ee, lane);
             }
             __syncthreads();
-            if (wv == 0 && j < k) {
-                const T acc = row_start<T, BWD>(i, jj, k, d, in);
-                out[(size_t)i * k + j] = acc + lds_tree_n<T, kWgParts>(sm, 0, lane);
+            if (wv == 0 && e < ke) {
+                const LV acc = row_start<LV, BWD>(i, ee, k, ke, d, in, rs);
+                stv(out + (size_t)i * ke + e, acc + lds_tree_n<LV, kWgParts>(sm, 0, lane));
             }
             __syncthreads();
         }
         return;
     }
     if (h >= H) return;   // uniform over the workgroup
-    T *sc = scratch + ((size_t)slot * kMaxWg + h) * k;
-    for (int jb = 0; jb < k; jb += 64) {
-        const int j = jb + lane, jj = j < k ? j : k - 1;
-        sm[wv][lane] = wave_subtree<T, 1, sizeof(T) == 8 ? 8 : 16>(q0, q1, P, h * kWgParts + wv, col, val, out, k,
-                                                                 jj, lane);
+    LV *sc = scratch + ((size_t)slot * kMaxWg + h) * ke;
+    for (int eb = 0; eb < ke; eb += 64) {
+        const int e = eb + lane, ee = e < ke ? e : ke - 1;
+        sm[wv][lane] = wave_subtree<T, V, 1, sizeof(LV) == 8 ? 8 : 16>(q0, q1, P, h * kWgParts + wv, col, val, out,
+                                                                     ke, ee, lane);
         __syncthreads();
-        if (wv == 0 && j < k) sc[j] = lds_tree_n<T, kWgParts>(sm, 0, lane);
+        if (wv == 0 && e < ke) sc[e] = lds_tree_n<LV, kWgParts>(sm, 0, lane);
         __syncthreads();
     }
     if (wv != 0) return;
@@ -419,26 +523,26 @@ __global__ __launch_bounds__(64 * kLevelWaves) void k_trisolve_huge(int32_t t0, 
     old = __shfl(old, 0, 64);
     if (old != H - 1) return;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    const T *rs = scratch + (size_t)slot * kMaxWg * k;
-    for (int jb = 0; jb < k; jb += 64) {
-        const int j = jb + lane, jj = j < k ? j : k - 1;
-        T acc = row_start<T, BWD>(i, jj, k, d, in);
-        T fold;
+    const LV *rsub = scratch + (size_t)slot * kMaxWg * ke;
+    for (int eb = 0; eb < ke; eb += 64) {
+        const int e = eb + lane, ee = e < ke ? e : ke - 1;
+        LV acc = row_start<LV, BWD>(i, ee, k, ke, d, in, rs);
+        LV fold;
         if (H == 4) {
-            T v[4];
+            LV v[4];
 #pragma unroll
-            for (int u = 0; u < 4; u++) v[u] = rs[(size_t)u * k + jj];
-            tree_fold<T, 4>(v);
+            for (int u = 0; u < 4; u++) v[u] = rsub[(size_t)u * ke + ee];
+            tree_fold<LV, 4>(v);
             fold = v[0];
         } else {
-            T v[kMaxWg];
+            LV v[kMaxWg];
 #pragma unroll
-            for (int u = 0; u < kMaxWg; u++) v[u] = rs[(size_t)u * k + jj];
-            tree_fold<T, kMaxWg>(v);
+            for (int u = 0; u < kMaxWg; u++) v[u] = rsub[(size_t)u * ke + ee];
+            tree_fold<LV, kMaxWg>(v);
             fold = v[0];
         }
         acc = acc + fold;
-        if (j < k) out[(size_t)i * k + j] = acc;
+        if (e < ke) stv(out + (size_t)i * ke + e, acc);
     }
     if (lane == 0) counter[slot] = 0;
 }
@@ -448,7 +552,7 @@ __global__ __launch_bounds__(64 * kLevelWaves) void k_trisolve_huge(int32_t t0, 
 // row's parts), the next nmed one wave each, the rest (tiny rows) rpw
 // consecutive rows per wave, TG at a time (the group size sets how many
 // gathers a wave holds in flight, and so its registers).
-template <typename T, bool BWD, int TG>
+template <typename T, typename VI, typename V, bool BWD, int TG>
 __global__ __launch_bounds__(64 * kLevelWaves) void k_trisolve_level(int32_t t0, int32_t nlong, int32_t nmed,
                                                                      int32_t t1, int rpw, int xcd_slices,
                                                                      const int32_t *__restrict__ rows,
@@ -456,31 +560,34 @@ __global__ __launch_bounds__(64 * kLevelWaves) void k_trisolve_level(int32_t t0,
                                                                      const int32_t *__restrict__ col,
                                                                      const T *__restrict__ val,
                                                                      const double *__restrict__ diag, int k,
-                                                                     const T *__restrict__ in, T *__restrict__ out) {
+                                                                     const VI *__restrict__ in, V *__restrict__ out,
+                                                                     RowScale rs) {
+    typedef typename Lane<V>::type LV;
+    const int ke = k / Lane<V>::n;
     const int lane = SS_LANE, wv = SS_WAVE_IN_BLOCK, W = (int)(blockDim.x >> 6);
     if ((int)blockIdx.x < nlong) {
         // a long row: gg of the W waves hold P / gg consecutive parts each
         // (subtrees of the row's part tree), wave 0 folds them
-        __shared__ T sm[kLevelWaves][64];
+        __shared__ LV sm[kLevelWaves][64];
         const int t = t0 + blockIdx.x;
         const int i = wave_uniform(rows[t]);
         const int q0 = wave_uniform(ptr[t]), q1 = wave_uniform(ptr[t + 1]);
         const double d = BWD ? diag[i] : 1.0;
         const int P = seg_parts(q1 - q0);
         const int gg = P < W ? P : W, na = P / gg;
-        for (int jb = 0; jb < k; jb += 64) {
-            const int j = jb + lane, jj = j < k ? j : k - 1;
-            T acc = wv == 0 ? row_start<T, BWD>(i, jj, k, d, in) : (T)0;
+        for (int eb = 0; eb < ke; eb += 64) {
+            const int e = eb + lane, ee = e < ke ? e : ke - 1;
+            LV acc = wv == 0 ? row_start<LV, BWD>(i, ee, k, ke, d, in, rs) : LV();
             if (P == 1) {
-                if (wv == 0) acc = row_dot_sub<T>(acc, q0, q1, 1, col, val, out, k, jj);
+                if (wv == 0) acc = row_dot_sub<T, V>(acc, q0, q1, 1, col, val, out, ke, ee);
             } else if (wv < gg) {
                 const int s0 = wv * na;
-                sm[wv][lane] = wave_subtree_n<T, kWgParts / 4>(na, q0, q1, P, s0, col, val, out, k, jj, lane);
+                sm[wv][lane] = wave_subtree_n<T, V, kWgParts / 4>(na, q0, q1, P, s0, col, val, out, ke, ee, lane);
             }
             __syncthreads();
             if (wv == 0) {
-                if (P > 1) acc = acc + lds_tree<T>(sm, 0, gg, lane);
-                if (j < k) out[(size_t)i * k + j] = acc;
+                if (P > 1) acc = acc + lds_tree<LV>(sm, 0, gg, lane);
+                if (e < ke) stv(out + (size_t)i * ke + e, acc);
             }
             __syncthreads();
         }
@@ -496,7 +603,7 @@ __global__ __launch_bounds__(64 * kLevelWaves) void k_trisolve_level(int32_t t0,
     const int chunk = xcd_slices ? (b % kXcds) * per + b / kXcds : b;
     const int w = wave_uniform(chunk * W + wv);
     if (w < nmed) {
-        wave_row<T, BWD>(t0 + nlong + w, rows, ptr, col, val, diag, k, in, out, lane);
+        wave_row<T, VI, V, BWD>(t0 + nlong + w, rows, ptr, col, val, diag, k, ke, in, out, rs, lane);
         return;
     }
     const int tb = t0 + nlong + nmed + (w - nmed) * rpw;
@@ -505,7 +612,7 @@ __global__ __launch_bounds__(64 * kLevelWaves) void k_trisolve_level(int32_t t0,
         if (t >= t1) break;
         int cnt = rpw - g0 < TG ? rpw - g0 : TG;
         if (t + cnt > t1) cnt = t1 - t;
-        wave_tiny_rows<T, BWD, TG>(t, cnt, rows, ptr, col, val, diag, k, in, out, lane);
+        wave_tiny_rows<T, VI, V, BWD, TG>(t, cnt, rows, ptr, col, val, diag, k, ke, in, out, rs, lane);
     }
 }
 
@@ -522,14 +629,16 @@ __global__ __launch_bounds__(64 * kLevelWaves) void k_trisolve_level(int32_t t0,
 // levels whose rows and entries fit the stage; one small level always fits.
 constexpr int kStageLevels = 256, kStageRows = 512, kStageNnz = 2 * kSmallNnz;
 static_assert(kStageRows >= kSmallRows && kStageNnz >= kSmallNnz, "one small level fits the stage");
-template <typename T, bool BWD>
+template <typename T, typename VI, typename V, bool BWD>
 __global__ __launch_bounds__(1024) void k_tri_small(int32_t l0, int32_t l1, const int32_t *__restrict__ level_ptr,
                                                     const int32_t *__restrict__ rows,
                                                     const int32_t *__restrict__ ptr,
                                                     const int32_t *__restrict__ col, const T *__restrict__ val,
                                                     const double *__restrict__ diag, int k,
-                                                    const T *__restrict__ in, T *out) {
-    __shared__ T sm[16][64];
+                                                    const VI *__restrict__ in, V *out, RowScale rs) {
+    typedef typename Lane<V>::type LV;
+    const int ke = k / Lane<V>::n;
+    __shared__ LV sm[16][64];
     __shared__ int32_t s_lp[kStageLevels + 1];
     __shared__ int32_t s_rows[kStageRows];
     __shared__ int32_t s_ptr[kStageRows + 1];
@@ -548,16 +657,16 @@ __global__ __launch_bounds__(1024) void k_tri_small(int32_t l0, int32_t l1, cons
             if (u < np) s_rows[u] = rows[tb + u];
         }
         __syncthreads();
-        const int eb = s_ptr[0];
+        const int eb0 = s_ptr[0];
         // levels 1 .. nc end inside the stage (a prefix: bounds only grow)
         const int u = (int)threadIdx.x + 1;
-        const bool fits = u <= nl && s_lp[u] - tb <= np && s_ptr[s_lp[u] - tb] - eb <= kStageNnz;
+        const bool fits = u <= nl && s_lp[u] - tb <= np && s_ptr[s_lp[u] - tb] - eb0 <= kStageNnz;
         const int nc = __syncthreads_count(fits);
         if (nc == 0) return;   // not reached: every level of a run is small, and one small level fits
-        const int ne = s_ptr[s_lp[nc] - tb] - eb;
-        for (int e = threadIdx.x; e < ne; e += blockDim.x) {
-            s_col[e] = col[eb + e];
-            s_val[e] = val[eb + e];
+        const int ne = s_ptr[s_lp[nc] - tb] - eb0;
+        for (int q = threadIdx.x; q < ne; q += blockDim.x) {
+            s_col[q] = col[eb0 + q];
+            s_val[q] = val[eb0 + q];
         }
         __syncthreads();
         for (int l = 0; l < nc; l++) {
@@ -570,27 +679,28 @@ __global__ __launch_bounds__(1024) void k_tri_small(int32_t l0, int32_t l1, cons
             const int pt = pb + slot;
             const bool act = pt < pe;
             const int i = act ? s_rows[pt] : 0;
-            const int q0 = act ? s_ptr[pt] - eb : 0, q1 = act ? s_ptr[pt + 1] - eb : 0;
+            const int q0 = act ? s_ptr[pt] - eb0 : 0, q1 = act ? s_ptr[pt + 1] - eb0 : 0;
             const double d = (BWD && act) ? diag[i] : 1.0;
             const int P = act ? seg_parts(q1 - q0) : 1;
             const int gg = P < g ? P : g;             // waves that hold parts, P / gg each
-            for (int jb = 0; jb < k; jb += 64) {
-                const int j = jb + lane, jj = j < k ? j : k - 1;
-                T acc = (T)0;
-                if (act && sub == 0) acc = row_start<T, BWD>(i, jj, k, d, in);
-                if (act && P == 1 && sub == 0) acc = row_dot_sub<T>(acc, q0, q1, 1, s_col, s_val, out, k, jj);
+            for (int eb = 0; eb < ke; eb += 64) {
+                const int e = eb + lane, ee = e < ke ? e : ke - 1;
+                LV acc = LV();
+                if (act && sub == 0) acc = row_start<LV, BWD>(i, ee, k, ke, d, in, rs);
+                if (act && P == 1 && sub == 0) acc = row_dot_sub<T, V>(acc, q0, q1, 1, s_col, s_val, out, ke, ee);
                 if (act && P > 1 && sub < gg)
-                    sm[slot * g + sub][lane] = wave_subtree_n<T, kWgParts>(P / gg, q0, q1, P, sub * (P / gg), s_col,
-                                                                           s_val, out, k, jj, lane);
+                    sm[slot * g + sub][lane] = wave_subtree_n<T, V, kWgParts>(P / gg, q0, q1, P, sub * (P / gg),
+                                                                              s_col, s_val, out, ke, ee, lane);
                 __syncthreads();
-                if (act && P > 1 && sub == 0) acc = acc + lds_tree<T>(sm, slot * g, gg, lane);
-                if (act && sub == 0 && j < k) out[(size_t)i * k + j] = acc;
+                if (act && P > 1 && sub == 0) acc = acc + lds_tree<LV>(sm, slot * g, gg, lane);
+                if (act && sub == 0 && e < ke) stv(out + (size_t)i * ke + e, acc);
                 __syncthreads();   // also orders this level's stores before the next level's loads
             }
         }
         c0 += nc;
     }
 }
+
 
 // ---------------------------------------------------------------------------
 // (d) fused effective resistance + probability + coin + sample + compaction.
@@ -743,52 +853,62 @@ void launch_fill_u32(hipStream_t s, int64_t count, uint32_t *p, uint32_t v) {
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(count, 256, kStreamGrid)), dim3(256), 0, s, count, p, v);
 }
 
-template <typename T>
+template <typename T, typename TI, typename V>
 void launch_core_head(hipStream_t s, int32_t t0, int32_t nhuge, int32_t t1, const int32_t *rows, const int32_t *ptr,
-                      const int32_t *col, const T *val, const double *diag, int k, const T *in, T *out) {
+                      const int32_t *col, const T *val, const double *diag, int k, const TI *in, V *out,
+                      RowScale rs) {
     // every core row as a long row of a level, a workgroup each (the core
     // rows are many): the rows of more than kWgRowMax entries first
-    launch_trisolve_level<T, false>(s, t0, nhuge, 1, t1 - t0 - nhuge, 0, t1, kLevelWaves, 1, 0, rows, ptr, col, val,
-                                    diag, k, in, out, nullptr, nullptr);
+    launch_trisolve_level<T, TI, V, false>(s, t0, nhuge, 1, t1 - t0 - nhuge, 0, t1, kLevelWaves, 1, 0, rows, ptr,
+                                           col, val, diag, k, in, out, nullptr, nullptr, rs);
 }
-template void launch_core_head<double>(hipStream_t, int32_t, int32_t, int32_t, const int32_t *, const int32_t *,
-                                       const int32_t *, const double *, const double *, int, const double *,
-                                       double *);
-template void launch_core_head<float>(hipStream_t, int32_t, int32_t, int32_t, const int32_t *, const int32_t *,
-                                      const int32_t *, const float *, const double *, int, const float *, float *);
+#define SS_INST_HEAD(T, TI, V)                                                                                  \
+    template void launch_core_head<T, TI, V>(hipStream_t, int32_t, int32_t, int32_t, const int32_t *,            \
+                                             const int32_t *, const int32_t *, const T *, const double *, int,   \
+                                             const TI *, V *, RowScale);
+SS_INST_HEAD(double, double, double)
+SS_INST_HEAD(float, float, float)
+SS_INST_HEAD(float, f32x2, f32x2)
+SS_INST_HEAD(float, f32x2, f16x2)
 
-template <typename T, bool BWD>
+template <typename T, typename TI, typename V, bool BWD>
 void launch_tri_small(hipStream_t s, int32_t l0, int32_t l1, const int32_t *level_ptr, const int32_t *rows,
-                      const int32_t *ptr, const int32_t *col, const T *val, const double *diag, int k, const T *in,
-                      T *out) {
+                      const int32_t *ptr, const int32_t *col, const T *val, const double *diag, int k, const TI *in,
+                      V *out, RowScale rs) {
     if (l1 <= l0) return;
-    hipLaunchKernelGGL((k_tri_small<T, BWD>), dim3(1), dim3(1024), 0, s, l0, l1, level_ptr, rows, ptr, col, val, diag,
-                       k, in, out);
+    hipLaunchKernelGGL((k_tri_small<T, TI, V, BWD>), dim3(1), dim3(1024), 0, s, l0, l1, level_ptr, rows, ptr, col,
+                       val, diag, k, in, out, rs);
 }
-#define SS_INST_SMALL(T, B)                                                                                       \
-    template void launch_tri_small<T, B>(hipStream_t, int32_t, int32_t, const int32_t *, const int32_t *,          \
-                                         const int32_t *, const int32_t *, const T *, const double *, int,          \
-                                         const T *, T *);
-SS_INST_SMALL(double, false)
-SS_INST_SMALL(double, true)
-SS_INST_SMALL(float, false)
-SS_INST_SMALL(float, true)
+#define SS_INST_SMALL(T, TI, V, B)                                                                              \
+    template void launch_tri_small<T, TI, V, B>(hipStream_t, int32_t, int32_t, const int32_t *, const int32_t *,  \
+                                                const int32_t *, const int32_t *, const T *, const double *,    \
+                                                int, const TI *, V *, RowScale);
+SS_INST_SMALL(double, double, double, false)
+SS_INST_SMALL(double, double, double, true)
+SS_INST_SMALL(float, float, float, false)
+SS_INST_SMALL(float, float, float, true)
+SS_INST_SMALL(float, f32x2, f32x2, false)
+SS_INST_SMALL(float, f32x2, f32x2, true)
+SS_INST_SMALL(float, f32x2, f16x2, false)
+SS_INST_SMALL(float, f16x2, f16x2, true)
 
-template <typename T, bool BWD>
+template <typename T, typename TI, typename V, bool BWD>
 void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nhuge, int32_t hm, int32_t nlong, int32_t nmed,
                            int32_t t1, int waves, int rpw, int xcd, const int32_t *rows, const int32_t *ptr,
-                           const int32_t *col, const T *val, const double *diag, int k, const T *in, T *out,
-                           T *scratch, int *counter) {
+                           const int32_t *col, const T *val, const double *diag, int k, const TI *in, V *out,
+                           void *scratch, int *counter, RowScale rs) {
+    const int ke = k / Lane<V>::n;
     if (nhuge > 0) {
-        hipLaunchKernelGGL((k_trisolve_huge<T, BWD>), dim3((unsigned)nhuge * (unsigned)hm), dim3(64 * kWgParts), 0, s,
-                           t0, hm, rows, ptr, col, val, diag, k, in, out, scratch, counter);
+        hipLaunchKernelGGL((k_trisolve_huge<T, TI, V, BWD>), dim3((unsigned)nhuge * (unsigned)hm),
+                           dim3(64 * kWgParts), 0, s, t0, hm, rows, ptr, col, val, diag, k, in, out,
+                           static_cast<typename Lane<V>::type *>(scratch), counter, rs);
         t0 += nhuge;
     }
     if (t1 <= t0) return;
     if (waves == 0) {   // a level of rows without entries
         const int32_t nrows = t1 - t0;
-        hipLaunchKernelGGL((k_level_copy<T, BWD>), dim3(grid_for(nrows, 4 * (256 / k), 1 << 16)), dim3(256), 0, s, t0,
-                           nrows, rows, diag, k, in, out);
+        hipLaunchKernelGGL((k_level_copy<T, TI, V, BWD>), dim3(grid_for(nrows, 4 * (256 / ke), 1 << 16)), dim3(256),
+                           0, s, t0, nrows, rows, diag, k, in, out, rs);
         return;
     }
     if (rpw < 1) rpw = 1;
@@ -802,23 +922,28 @@ void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nhuge, int32_t hm,
     if (xcd_slices) blocks = (blocks + kXcds - 1) / kXcds * kXcds;
     const unsigned grid = (unsigned)nlong + (unsigned)blocks;
     if (rpw == 1)
-        hipLaunchKernelGGL((k_trisolve_level<T, BWD, 1>), dim3(grid), dim3(64 * waves), 0, s, t0, nlong, nmed, t1,
-                           rpw, xcd_slices, rows, ptr, col, val, diag, k, in, out);
+        hipLaunchKernelGGL((k_trisolve_level<T, TI, V, BWD, 1>), dim3(grid), dim3(64 * waves), 0, s, t0, nlong, nmed,
+                           t1, rpw, xcd_slices, rows, ptr, col, val, diag, k, in, out, rs);
     else if (rpw == 2)
-        hipLaunchKernelGGL((k_trisolve_level<T, BWD, 2>), dim3(grid), dim3(64 * waves), 0, s, t0, nlong, nmed, t1,
-                           rpw, xcd_slices, rows, ptr, col, val, diag, k, in, out);
+        hipLaunchKernelGGL((k_trisolve_level<T, TI, V, BWD, 2>), dim3(grid), dim3(64 * waves), 0, s, t0, nlong, nmed,
+                           t1, rpw, xcd_slices, rows, ptr, col, val, diag, k, in, out, rs);
     else
-        hipLaunchKernelGGL((k_trisolve_level<T, BWD, kTinyGroup>), dim3(grid), dim3(64 * waves), 0, s, t0, nlong,
-                           nmed, t1, rpw, xcd_slices, rows, ptr, col, val, diag, k, in, out);
+        hipLaunchKernelGGL((k_trisolve_level<T, TI, V, BWD, kTinyGroup>), dim3(grid), dim3(64 * waves), 0, s, t0,
+                           nlong, nmed, t1, rpw, xcd_slices, rows, ptr, col, val, diag, k, in, out, rs);
 }
-#define SS_INST_TRI(T, B)                                                                                      \
-    template void launch_trisolve_level<T, B>(hipStream_t, int32_t, int32_t, int32_t, int32_t, int32_t, int32_t,   \
-                                              int, int, int, const int32_t *, const int32_t *, const int32_t *,  \
-                                              const T *, const double *, int, const T *, T *, T *, int *);
-SS_INST_TRI(double, false)
-SS_INST_TRI(double, true)
-SS_INST_TRI(float, false)
-SS_INST_TRI(float, true)
+#define SS_INST_TRI(T, TI, V, B)                                                                                \
+    template void launch_trisolve_level<T, TI, V, B>(hipStream_t, int32_t, int32_t, int32_t, int32_t, int32_t,   \
+                                                     int32_t, int, int, int, const int32_t *, const int32_t *,   \
+                                                     const int32_t *, const T *, const double *, int, const TI *, \
+                                                     V *, void *, int *, RowScale);
+SS_INST_TRI(double, double, double, false)
+SS_INST_TRI(double, double, double, true)
+SS_INST_TRI(float, float, float, false)
+SS_INST_TRI(float, float, float, true)
+SS_INST_TRI(float, f32x2, f32x2, false)
+SS_INST_TRI(float, f32x2, f32x2, true)
+SS_INST_TRI(float, f32x2, f16x2, false)
+SS_INST_TRI(float, f16x2, f16x2, true)
 
 
 template <typename TZ>

@@ -126,11 +126,12 @@ def test_solve_mixed_precision(ctx, oracle_mod):
 
 @pytest.mark.parametrize("kind", ["grid3d", "rgg"])
 def test_mixed_reliable_and_restart_agree(ctx, oracle_mod, kind):
-    """Both mixed-precision schemes (reliable updates, the default, and
-    restarted FP32 sweeps) reach the FP64 tolerance and agree with the FP64
-    oracle; reliable updates keep the Krylov space, so they take no more
-    iterations than the restarts, and every inner_tol gives the same
-    tolerance."""
+    """Every mixed-precision scheme (reliable updates with FP16
+    preconditioner rows, the default; the same with FP32 rows; restarted
+    FP32 sweeps) reaches the FP64 tolerance and agrees with the FP64 oracle;
+    reliable updates keep the Krylov space, so they take no more iterations
+    than the restarts, every inner_tol gives the same tolerance, and the
+    FP16 rows cost at most two iterations over FP32 rows."""
     import spec_spars_amd as ss
     n, u, v, w = _graph(kind)
     k = 12
@@ -138,7 +139,7 @@ def test_mixed_reliable_and_restart_agree(ctx, oracle_mod, kind):
     cp, ri, va, Zo, it = _oracle_solution(oracle_mod, n, u, v, w, Y, 3, 1e-12)
     out = {}
     for mode, inner in ((ss.MIXED_RELIABLE, 0.0), (ss.MIXED_RELIABLE, 0.1), (ss.MIXED_RELIABLE, 1e-4),
-                        (ss.MIXED_RESTART, 0.0)):
+                        (ss.MIXED_RESTART, 0.0), (ss.MIXED_RELIABLE_F32, 0.0)):
         Z, info = ctx.solve_lap(Y, cp, ri, va, ss.solve_opts(factor_seed=3, stop_rule=ss.STOP_REL, tol=1e-8,
                                                              precision=ss.PREC_MIXED, max_iter=2000,
                                                              inner_tol=inner, mixed_mode=mode))
@@ -149,12 +150,14 @@ def test_mixed_reliable_and_restart_agree(ctx, oracle_mod, kind):
         assert info["max_relres"] <= 1e-8
         assert _rel_centered(Z, Zo).max() < 1e-5
         out[(mode, inner)] = info["max_iters"]
-    assert out[(ss.MIXED_RELIABLE, 0.0)] <= out[(ss.MIXED_RESTART, 0.0)], out
+    assert out[(ss.MIXED_RELIABLE_F32, 0.0)] <= out[(ss.MIXED_RESTART, 0.0)], out
+    assert out[(ss.MIXED_RELIABLE, 0.0)] <= out[(ss.MIXED_RELIABLE_F32, 0.0)] + 2, out
 
 
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode", [0, 1, 2])
 def test_mixed_edge_cases(ctx, oracle_mod, mode):
-    """Mixed precision (reliable updates and restart sweeps) on the cases
+    """Mixed precision (reliable updates with FP16 and FP32 preconditioner
+    rows, restart sweeps) on the cases
     its per-column logic has to get right: a graph of three components and
     an isolated vertex (every component's residual at the tolerance, its
     own mean removed), a zero right-hand side (a zero solution, no
@@ -197,7 +200,7 @@ def test_mixed_edge_cases(ctx, oracle_mod, mode):
     assert np.all(rr <= 2e-7 * (1 + 1e-9)), rr
     # too few iterations: a status, not an abort (max_iter bounds the
     # reliable solve's iterations; restart sweeps take it per sweep)
-    if mode == ss.MIXED_RELIABLE:
+    if mode != ss.MIXED_RESTART:
         o = ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-12, max_iter=1, precision=ss.PREC_MIXED, mixed_mode=mode)
         with pytest.raises(ss.SSError) as ei:
             ctx.solve_lap(Y, cp, ri, va, o)
