@@ -80,25 +80,30 @@ def workload(name):
     raise SystemExit("unknown workload " + name)
 
 
-def kernel_bytes(n, m, G, k, kept, s=8):
+def kernel_bytes(n, m, G, k, kept, s=8, srow=None):
     """Algorithmic (compulsory) HBM bytes per launch of each kernel kind:
     every array counted once per pass, gathers beyond first touch not
     counted.  n, m: total vertices / edges of the (batched) system; k the
-    columns of the pass; s the bytes of the solve's values."""
+    columns of the pass; s the bytes of the solve's values; srow the bytes
+    of a value of the preconditioner's intermediate rows (s, or 2 for FP16
+    rows: the forward reads the residual at s and writes its rows at srow,
+    the backward reads and writes rows at srow)."""
+    srow = s if srow is None else srow
     nnzL = n + 2 * m
     fi, bi = G.schedule_info(0), G.schedule_info(1)
+    row_io = {False: k * (s + srow), True: 2 * k * srow}   # forward / backward in-row + out-row
 
-    def per_level(info, extra_per_row):
-        # schedule row id + row pointer, (col, val) per entry, read in-row, write out-row
+    def per_level(info, bwd):
+        # schedule row id + row pointer, (col, val) per entry, read in-row, write out-row (+ D backward)
         if not info["level_launches"]:
             return None
-        b = info["level_rows"] * (4 + 4 + 2 * k * s + extra_per_row) + info["level_nnz"] * (4 + s)
+        b = info["level_rows"] * (4 + 4 + row_io[bwd] + (8 if bwd else 0)) + info["level_nnz"] * (4 + s)
         return b / info["level_launches"]
 
-    def small(info, extra_per_row):
+    def small(info, bwd):
         if not info["small_levels"]:
             return None
-        total = info["small_rows"] * (4 + 4 + 2 * k * s + extra_per_row) + info["small_nnz"] * (4 + s)
+        total = info["small_rows"] * (4 + 4 + row_io[bwd] + (8 if bwd else 0)) + info["small_nnz"] * (4 + s)
         return total / info["small_runs"]
 
     def core(info, launches):
@@ -107,7 +112,7 @@ def kernel_bytes(n, m, G, k, kept, s=8):
         # outside the core; over the direction's launches
         if not info["core_rows"]:
             return None
-        total = info["core_entries"] // 2 * s + info["core_rows"] * 2 * k * s + \
+        total = info["core_entries"] // 2 * s + info["core_rows"] * 2 * k * srow + \
             info["core_outside_nnz"] * (4 + s)
         return total / launches
 
@@ -116,10 +121,10 @@ def kernel_bytes(n, m, G, k, kept, s=8):
         "sketch": 4 * (n + 1) + 2 * m * (4 + 8) + n * k * 8,
         # SpMM: rowptr, (col, val) per nonzero, read P, write Q
         "spmm": 4 * (n + 1) + nnzL * (4 + s) + 2 * n * k * s,
-        "tri_fwd": per_level(fi, 0),
-        "tri_bwd": per_level(bi, 8),
-        "tri_fwd_small": small(fi, 0),
-        "tri_bwd_small": small(bi, 8),
+        "tri_fwd": per_level(fi, False),
+        "tri_bwd": per_level(bi, True),
+        "tri_fwd_small": small(fi, False),
+        "tri_bwd_small": small(bi, True),
         "core_fwd": core(fi, 4),
         "core_bwd": core(bi, 3),
         # ER/sample: edges (u, v, w), Z (FP64) once, p out, mask out, kept (u, v, w) out
@@ -414,7 +419,8 @@ def bench_single(W, args):
             G.profile(False)
             ctx.bench_copy(TRACE_MARK_BYTES, reps=1)
         kept = sum(r["kept"] for r in rep_served)
-        kb = kernel_bytes(n, m, G, K, kept, s=8 if PM == ss.PREC_FP64 else 4)
+        kb = kernel_bytes(n, m, G, K, kept, s=8 if PM == ss.PREC_FP64 else 4,
+                          srow=2 if PM == ss.PREC_MIXED and args.mixed_rows == "fp16" and K % 2 == 0 else None)
         kernels, roofline = summarize(prof, kb, t_prof)
         # er_sample runs once per request: its bytes are per request's k
         if "er_sample" in kernels:
@@ -569,7 +575,8 @@ def bench_batch(W, args):
             prof = B.profile_read()
             B.profile(False)
         kept = sum(r["kept"] for r in reps)
-        kb = kernel_bytes(n, m, B, k, kept, s=8 if prec == ss.PREC_FP64 else 4)
+        kb = kernel_bytes(n, m, B, k, kept, s=8 if prec == ss.PREC_FP64 else 4,
+                          srow=2 if prec == ss.PREC_MIXED and k % 2 == 0 else None)
         kernels, roofline = summarize(prof, kb, t_prof)
         if roofline:
             add_traffic(roofline, measured_traffic(W["name"], args.precision, roofline["kernel"], k))

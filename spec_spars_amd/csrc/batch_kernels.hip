@@ -427,20 +427,51 @@ __device__ __forceinline__ double bt_lane(double v, int l) {
 // the grouping does not change a bit.
 constexpr int kSpmmGroup = 4;
 constexpr int kSpmmShort = 8;
-template <typename T>
+// The partial of one column (lane value acc of column j) over the block's
+// waves, in wave order, into partial[blockIdx.x * k + j].
+__device__ __forceinline__ void bt_tile_store_col(double acc, int j, int k, double *partial, double (*sm)[64]) {
+    const int w = BT_WAVE, lane = BT_LANE;
+    sm[w][lane] = acc;
+    __syncthreads();
+    if (w == 0 && j < k) {
+        double s = sm[0][lane];
+        for (int u = 1; u < BT_NW; u++) s += sm[u][lane];
+        partial[(size_t)blockIdx.x * k + j] = s;
+    }
+    __syncthreads();
+}
+
+// Lane values of the SpMM: one column (T), or two adjacent FP32 columns
+// (f32x2, k even) moved by one 8-byte access.
+__device__ __forceinline__ f32x2 bt_fma(float a, f32x2 b, f32x2 c) {
+    f32x2 r;
+    r.x = __builtin_fmaf(a, b.x, c.x);
+    r.y = __builtin_fmaf(a, b.y, c.y);
+    return r;
+}
+template <typename V> __device__ __forceinline__ double bt_col(V v, int h) { return (double)v; }
+template <> __device__ __forceinline__ double bt_col<f32x2>(f32x2 v, int h) { return (double)(h ? v.y : v.x); }
+
+// V: T, or f32x2 (two columns per lane, rows of k / 2 elements).  Every
+// column's arithmetic is the same for either.
+template <typename T, typename V>
 __global__ __launch_bounds__(1024) void kb_spmm_dot(Tiles rt, const int32_t *__restrict__ rowptr,
                                                    const int32_t *__restrict__ col, const T *__restrict__ val, int k,
-                                                   const T *__restrict__ P, T *__restrict__ Q,
+                                                   const V *__restrict__ P, V *__restrict__ Q,
                                                    double *__restrict__ partial) {
-    // FP32 rows of up to 16 entries take the all-gathers-in-flight path (FP64
-    // at 16 would not fit the registers)
-    constexpr int G = kSpmmGroup, E = sizeof(T) == 4 ? 2 * kSpmmShort : kSpmmShort;
+    // rows of up to 16 entries (4-byte lanes; 8 for 8-byte lanes, which
+    // would not fit the registers at 16) take the all-gathers-in-flight path
+    constexpr int NC = sizeof(V) / sizeof(T);   // columns per lane
+    constexpr int G = kSpmmGroup, E = sizeof(V) == 4 ? 2 * kSpmmShort : kSpmmShort;
     __shared__ double sm[kMaxTileWaves][64];
     const int t = blockIdx.x, NW = BT_NW, lane = BT_LANE;
     const int r0 = rt.start[t], r1 = rt.start[t + 1];
-    for (int jb = 0; jb < k; jb += 64) {
-        const int j = jb + lane, jj = j < k ? j : k - 1;
-        double acc = 0.0;
+    const int ke = k / NC;
+    for (int eb = 0; eb < ke; eb += 64) {
+        const int e = eb + lane, ee = e < ke ? e : ke - 1;
+        double acc[NC];
+#pragma unroll
+        for (int h = 0; h < NC; h++) acc[h] = 0.0;
         for (int rb = bt_uniform(r0 + BT_WAVE); rb < r1; rb += G * NW) {
             int rr[G], q0[G], len[G];
             int maxlen = 0;
@@ -460,7 +491,8 @@ __global__ __launch_bounds__(1024) void kb_spmm_dot(Tiles rt, const int32_t *__r
             }
             if (maxlen <= E) {
                 int c[G];
-                T v[G], own[G], x[G][E];
+                T v[G];
+                V own[G], x[G][E];
 #pragma unroll
                 for (int u = 0; u < G; u++) {
                     c[u] = lane < len[u] ? col[q0[u] + lane] : 0;
@@ -468,18 +500,19 @@ __global__ __launch_bounds__(1024) void kb_spmm_dot(Tiles rt, const int32_t *__r
                 }
 #pragma unroll
                 for (int u = 0; u < G; u++) {
-                    own[u] = P[(size_t)rr[u] * k + jj];
+                    own[u] = P[(size_t)rr[u] * ke + ee];
 #pragma unroll
-                    for (int e = 0; e < E; e++) x[u][e] = e < len[u] ? P[(size_t)bt_lane(c[u], e) * k + jj] : (T)0;
+                    for (int q = 0; q < E; q++) x[u][q] = q < len[u] ? P[(size_t)bt_lane(c[u], q) * ke + ee] : V();
                 }
 #pragma unroll
                 for (int u = 0; u < G; u++) {
-                    T a = (T)0;
+                    V a = V();
 #pragma unroll
-                    for (int e = 0; e < E; e++) a = bt_fma(-bt_lane(v[u], e), x[u][e], a);
-                    if (rb + u * NW < r1 && j < k) {
-                        Q[(size_t)rr[u] * k + j] = -a;
-                        acc = bt_fma((double)own[u], (double)(-a), acc);
+                    for (int q = 0; q < E; q++) a = bt_fma(-bt_lane(v[u], q), x[u][q], a);
+                    if (rb + u * NW < r1 && e < ke) {
+                        Q[(size_t)rr[u] * ke + e] = -a;
+#pragma unroll
+                        for (int h = 0; h < NC; h++) acc[h] = bt_fma(bt_col(own[u], h), -bt_col(a, h), acc[h]);
                     }
                 }
             } else {
@@ -489,29 +522,32 @@ __global__ __launch_bounds__(1024) void kb_spmm_dot(Tiles rt, const int32_t *__r
 #pragma unroll
                 for (int u = 0; u < G; u++) {
                     if (rb + u * NW >= r1) break;
-                    T a = (T)0;
+                    V a = V();
                     for (int base = 0; base < len[u]; base += 64) {
                         const int n = len[u] - base;
                         const int cc = lane < n ? col[q0[u] + base + lane] : 0;
                         const T vv = lane < n ? val[q0[u] + base + lane] : (T)0;
                         for (int e0 = 0; e0 < 64 && e0 < n; e0 += 16) {
-                            T x[16];
+                            V x[16];
 #pragma unroll
-                            for (int e = 0; e < 16; e++)
-                                x[e] = e0 + e < n ? P[(size_t)bt_lane(cc, e0 + e) * k + jj] : (T)0;
+                            for (int q = 0; q < 16; q++)
+                                x[q] = e0 + q < n ? P[(size_t)bt_lane(cc, e0 + q) * ke + ee] : V();
 #pragma unroll
-                            for (int e = 0; e < 16; e++) a = bt_fma(-bt_lane(vv, e0 + e), x[e], a);
+                            for (int q = 0; q < 16; q++) a = bt_fma(-bt_lane(vv, e0 + q), x[q], a);
                         }
                     }
-                    if (j < k) {
-                        const size_t o = (size_t)rr[u] * k + j;
+                    if (e < ke) {
+                        const size_t o = (size_t)rr[u] * ke + e;
                         Q[o] = -a;
-                        acc = bt_fma((double)P[o], (double)(-a), acc);
+                        const V pv = P[o];
+#pragma unroll
+                        for (int h = 0; h < NC; h++) acc[h] = bt_fma(bt_col(pv, h), -bt_col(a, h), acc[h]);
                     }
                 }
             }
         }
-        bt_tile_store(acc, k, jb, partial, sm);
+#pragma unroll
+        for (int h = 0; h < NC; h++) bt_tile_store_col(acc[h], NC * e + h, k, partial, sm);
     }
 }
 
@@ -1031,7 +1067,16 @@ template void bt_update_x<float>(hipStream_t, Tiles, int, const double *, const 
 template <typename T>
 void bt_spmm_dot(hipStream_t s, Tiles rt, const int32_t *rowptr, const int32_t *col, const T *val, int k, const T *P,
                  T *Q, double *partial) {
-    hipLaunchKernelGGL(kb_spmm_dot<T>, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, rowptr, col, val, k, P, Q, partial);
+    if constexpr (sizeof(T) == 4) {
+        if (k % 2 == 0) {
+            // FP32 with k even: two columns per lane
+            hipLaunchKernelGGL((kb_spmm_dot<float, f32x2>), dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, rowptr,
+                               col, val, k, reinterpret_cast<const f32x2 *>(P), reinterpret_cast<f32x2 *>(Q), partial);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((kb_spmm_dot<T, T>), dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, rowptr, col, val, k, P, Q,
+                       partial);
 }
 template void bt_spmm_dot<double>(hipStream_t, Tiles, const int32_t *, const int32_t *, const double *, int,
                                   const double *, double *, double *);
