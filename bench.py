@@ -705,6 +705,109 @@ def trace_summary(path):
     return 0
 
 
+def _template_args(name):
+    """Top-level template arguments of a demangled kernel name."""
+    a = name.find("<")
+    if a < 0:
+        return []
+    depth, cur, out = 0, "", []
+    for ch in name[a + 1:]:
+        if ch in "<(":
+            depth += 1
+        elif ch in ">)":
+            if depth == 0:
+                out.append(cur.strip())
+                break
+            depth -= 1
+        if ch == "," and depth == 0:
+            out.append(cur.strip())
+            cur = ""
+        else:
+            cur += ch
+    return out
+
+
+def _counter_rows(path):
+    """(dispatch id, kernel name, value) of one rocprofv3 counter CSV."""
+    import csv
+    rows = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            rows.append((int(r["Dispatch_Id"]), r["Kernel_Name"], float(r["Counter_Value"])))
+    rows.sort()
+    return rows
+
+
+def pmc_summary(fetch_csv, write_csv):
+    """HBM traffic per dispatch from two rocprofv3 --pmc runs of the probe
+    (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch), each calibrated on the
+    probe's 1 GiB FP64 copy, and the traffic per launch of the kernel kinds
+    bench.py times (a level launch with its rows of more than 1,024 entries;
+    the forward core head, the level launch just before each forward core
+    gather, counted with the dense cores).  Prints markdown."""
+    fetch, write = _counter_rows(fetch_csv), _counter_rows(write_csv)
+    if len(fetch) != len(write) or any(a[1] != b[1] for a, b in zip(fetch, write)):
+        raise SystemExit("pmc-summary: the two runs dispatched different kernels")
+    copy = [i for i, r in enumerate(fetch) if "k_copy_f64" in r[1]]
+    if not copy:
+        raise SystemExit("pmc-summary: no calibration copy in %s" % fetch_csv)
+    cf = (1 << 20) / fetch[copy[-1]][2]    # the copy reads 1 GiB
+    cw = (1 << 20) / write[copy[-1]][2]    # and writes 1 GiB
+    print("Calibration: FETCH_SIZE reads %.0f KiB and WRITE_SIZE %.0f KiB for the copy's 1048576 KiB read and "
+          "written,\nso traffic = %.2f x FETCH_SIZE + %.2f x WRITE_SIZE (KiB).\n" %
+          (fetch[copy[-1]][2], write[copy[-1]][2], cf, cw))
+    traffic = [(f[1], 1024.0 * (cf * f[2] + cw * w[2])) for f, w in zip(fetch, write)]
+    short = [t[0].split("(")[0].replace("void ", "") for t in traffic]
+    per = {}
+    for nm, b in zip(short, (t[1] for t in traffic)):
+        e = per.setdefault(nm, [0, 0.0])
+        e[0] += 1
+        e[1] += b
+    print("| kernel | dispatches | traffic MB/dispatch |")
+    print("|---|---:|---:|")
+    for nm, (c, b) in sorted(per.items(), key=lambda x: -x[1][1]):
+        print("| `%s` | %d | %.1f |" % (nm, c, b / c / 1e6))
+    # kinds: walk the dispatches in order
+    kinds = {}
+    level = ("k_trisolve_level", "k_trisolve_huge", "k_level_copy")
+
+    def add(kind, b, launch):
+        e = kinds.setdefault(kind, [0, 0.0])
+        e[0] += launch
+        e[1] += b
+    names = [t[0] for t in traffic]
+
+    def fwd_level(i):
+        return any(x in short[i] for x in level) and not (len(_template_args(names[i])) >= 4 and
+                                                          _template_args(names[i])[3] == "true")
+    head = set()
+    for j, nm in enumerate(short):
+        if "k_core_gather" in nm and _template_args(names[j])[-1] == "false" and j > 0 and fwd_level(j - 1):
+            head.add(j - 1)
+            if "k_trisolve_level" in short[j - 1] and j > 1 and "k_trisolve_huge" in short[j - 2] and fwd_level(j - 2):
+                head.add(j - 2)
+    for i, (nm, b) in enumerate(traffic):
+        base = short[i]
+        if any(x in base for x in level):
+            args = _template_args(nm)
+            bwd = len(args) >= 4 and args[3] == "true"
+            kind = "core_fwd" if i in head else ("tri_bwd" if bwd else "tri_fwd")
+            add(kind, b, 0 if "k_trisolve_huge" in base else 1)
+        elif "kb_spmm_dot" in base:
+            add("spmm", b, 1)
+        elif "kb_er_sample" in base:
+            add("er_sample", b, 1)
+        elif "kb_sketch" in base:
+            add("sketch", b, 1)
+    print("\nPer launch of each kernel kind as bench.py times them (a level launch with its rows of more than\n"
+          "1,024 entries; the forward core head counted with the dense cores), bytes:\n\n```")
+    for kind in ("tri_bwd", "tri_fwd", "spmm", "er_sample", "sketch"):
+        if kind in kinds and kinds[kind][0]:
+            print("traffic_bytes_per_launch %s: %d" % (kind, kinds[kind][1] / kinds[kind][0]))
+    print("```")
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -732,9 +835,14 @@ def main():
     ap.add_argument("--slots", type=int, default=4, help="serve workload: requests on the device at once")
     ap.add_argument("--trace-summary", metavar="CSV", default=None,
                     help="summarize the profiled served pass of a rocprofv3 kernel trace of this script, and exit")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"), default=None,
+                    help="traffic per dispatch and per kernel kind from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of "
+                         "--probe, and exit")
     args = ap.parse_args()
     if args.trace_summary:
         return trace_summary(args.trace_summary)
+    if args.pmc_summary:
+        return pmc_summary(*args.pmc_summary)
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (the multi-GPU "
               "sharding design is documented, not built)" % args.gpus, file=sys.stderr)

@@ -12,6 +12,7 @@ from ._lib import (SSError, PREC_FP64, PREC_MIXED, STOP_REF_ABS, STOP_REL,  # no
 
 _lib.load()
 
-from .api import Batch, Context, Factor, Graph, set_factor_threads, solve_opts  # noqa: E402,F401
+from .api import (Batch, Context, Factor, Graph, set_factor_threads, solve_opts, mixed_mode_for,  # noqa: E402,F401
+                  FP16_ROWS_MAX_SPREAD)
 from . import graphs  # noqa: E402,F401
 from .sparsifier import InputStream, SparsifierParameters, Sparsifier, read_mtx, write_mtx  # noqa: E402,F401

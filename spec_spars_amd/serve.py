@@ -108,6 +108,8 @@ class Registry:
         ordering = ss.ORDER_ROUNDS if ordering is None else ordering
         self.specs = graphs
         self.ctxs, self.G, self.F = [], [], []
+        # per graph the mixed-precision scheme its weights allow
+        self.mixed_mode = [ss.mixed_mode_for(w) for _, _, _, w in graphs]
         for n, u, v, w in graphs:
             ctx = ss.Context(0)
             self.ctxs.append(ctx)
@@ -165,14 +167,16 @@ class Registry:
         if group[0].op == "sparsify":
             from . import graphs as gen
             n = self.specs[group[0].graph][0]
-            opts = self.ss.solve_opts(stop_rule=self.ss.STOP_REL, tol=tol, max_iter=2000, precision=precision)
+            opts = self.ss.solve_opts(stop_rule=self.ss.STOP_REL, tol=tol, max_iter=2000, precision=precision,
+                                      mixed_mode=self.mixed_mode[group[0].graph])
             reps = G.sparsify_many([dict(k=r.k, beta=gen.beta_for(n, r.eps), sketch_seed=r.seed,
                                          sampling_seed=r.seed) for r in group], solve=opts)
             return [{"kept": rep["kept"], "edges": G.m, "arrays": G.read_kept_many(i) if keep_arrays else None,
                      "iters": rep["max_iters"], "relres": rep["max_relres"], "group": len(group)}
                     for i, rep in enumerate(reps)]
         b = np.concatenate([self.rhs(r.graph, r.k, r.seed) for r in group], axis=1)
-        opts = self.ss.solve_opts(stop_rule=self.ss.STOP_REL, tol=tol, max_iter=2000, precision=precision)
+        opts = self.ss.solve_opts(stop_rule=self.ss.STOP_REL, tol=tol, max_iter=2000, precision=precision,
+                                  mixed_mode=self.mixed_mode[group[0].graph])
         Z, info = G.solve(b, opts)
         out, c = [], 0
         for r in group:
@@ -188,7 +192,8 @@ class Registry:
         from . import graphs as gen
         G = self.G[r.graph]
         n = self.specs[r.graph][0]
-        opts = ss.solve_opts(stop_rule=ss.STOP_REL, tol=tol, max_iter=2000, precision=precision)
+        opts = ss.solve_opts(stop_rule=ss.STOP_REL, tol=tol, max_iter=2000, precision=precision,
+                             mixed_mode=self.mixed_mode[r.graph])
         if r.op == "sparsify":
             rep = G.sparsify(r.k, gen.beta_for(n, r.eps), sketch_seed=r.seed, sampling_seed=r.seed, solve=opts)
             kept = G.read_kept() if keep_arrays else None

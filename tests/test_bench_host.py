@@ -68,3 +68,29 @@ def test_trace_summary_takes_the_window_between_the_markers(tmp_path, xyz):
     assert "| `ss::kb_spmm_dot<float>` | 2 | 0.00 | 1.5 |" in out
     assert "k_trisolve_level<float, true, 1>` | 1 |" in out
     assert "kb_sketch" not in out and "k_copy_f64" not in out
+
+
+def test_pmc_summary_kinds(tmp_path, capsys):
+    """The counter summary calibrates on the probe's copy and sorts level
+    dispatches into kinds: backward levels, forward levels, and the forward
+    core head (the level launch just before a forward core gather)."""
+    import bench
+    hdr = "Dispatch_Id,Kernel_Name,Counter_Name,Counter_Value\n"
+    lv = "void ss::k_trisolve_level<float, float __attribute__((ext_vector_type(2))), " \
+         "_Float16 __attribute__((ext_vector_type(2))), %s, 2>(int)"
+    hg = "void ss::k_trisolve_huge<float, float, float, %s>(int)"
+    rows = [(lv % "false", 10), (hg % "false", 5), (lv % "false", 7),
+            ("void ss::k_core_gather<float, _Float16, false>(long)", 1),
+            (lv % "true", 20), ("void ss::kb_spmm_dot<float, float, float>(int)", 100),
+            ("void ss::k_copy_f64(long, double const*, double*)", 1 << 19)]
+    for name, scale in (("f.csv", 1), ("w.csv", 2)):
+        with open(tmp_path / name, "w") as f:
+            f.write(hdr)
+            for i, (k, v) in enumerate(rows):
+                f.write('%d,"%s",X,%d\n' % (i + 1, k, v * scale))
+    assert bench.pmc_summary(str(tmp_path / "f.csv"), str(tmp_path / "w.csv")) == 0
+    out = capsys.readouterr().out
+    # copy: fetch 512 KiB-units -> factor 2; write 2^20 -> factor 1; traffic = 2 f + 1 w = 4 v KiB
+    assert "traffic_bytes_per_launch tri_bwd: %d" % (4 * 20 * 1024) in out
+    assert "traffic_bytes_per_launch tri_fwd: %d" % (4 * 10 * 1024) in out
+    assert "traffic_bytes_per_launch spmm: %d" % (4 * 100 * 1024) in out

@@ -259,3 +259,16 @@ def test_default_options_pick_the_shallow_order():
     f = ss.Factor(*graphs.laplacian_csc(n, u, v, w), seed=0, ordering=o.ordering)
     info = f.info()
     assert info["depth_fwd"] < 1000 and info["depth_bwd"] < 1000
+
+
+def test_mixed_mode_for_weight_spread():
+    """FP16 preconditioner rows for weights spanning at most 2^10 (the
+    Chung-Lu U[0.5, 2] and unit-weight families), FP32 rows beyond (the
+    grids' 10^U(-3, 3))."""
+    import spec_spars_amd as ss
+    from spec_spars_amd import graphs
+    assert ss.mixed_mode_for(graphs.uniform_weights(1000, seed=0)) == ss.MIXED_RELIABLE
+    assert ss.mixed_mode_for(np.ones(10)) == ss.MIXED_RELIABLE
+    assert ss.mixed_mode_for(graphs.loguniform_weights(1000, seed=0)) == ss.MIXED_RELIABLE_F32
+    assert ss.mixed_mode_for(np.array([1.0, 1024.0])) == ss.MIXED_RELIABLE
+    assert ss.mixed_mode_for(np.array([1.0, 1025.0])) == ss.MIXED_RELIABLE_F32
