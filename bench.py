@@ -166,18 +166,21 @@ def add_traffic(roofline, traffic):
         roofline["traffic_over_algorithmic"] = round(traffic / roofline["bytes_per_launch"], 2)
 
 
-def measured_traffic(workload, precision, kind, columns):
+def measured_traffic(workload, precision, kind, columns, rows=None):
     """HBM bytes per launch of a kernel kind, from the committed counter
     summary profiles/sparsify_kernels_<workload>_<precision>_pmc.md
     (FETCH_SIZE and WRITE_SIZE per dispatch, calibrated on a copy kernel),
     when one exists for a pass of the same number of columns (its
-    `traffic_columns:` line)."""
+    `traffic_columns:` line) and, for mixed precision, the same
+    preconditioner rows (its `traffic_rows:` line)."""
     path = os.path.join(ROOT, "profiles", "sparsify_kernels_%s_%s_pmc.md" % (workload, precision))
     if not os.path.exists(path):
         return None
     lines = open(path).read().splitlines()
     cols = [int(x.split(":")[1]) for x in lines if x.startswith("traffic_columns:")]
     if cols != [columns]:
+        return None
+    if rows is not None and [x.split(":")[1].strip() for x in lines if x.startswith("traffic_rows:")] != [rows]:
         return None
     for line in lines:
         if line.startswith("traffic_bytes_per_launch %s:" % kind):
@@ -429,7 +432,8 @@ def bench_single(W, args):
             ke["GBps"] = ke["bytes_per_launch"] / (ke["avg_ms"] * 1e-3) / 1e9
         if roofline:
             add_traffic(roofline, measured_traffic(W["name"], "mixed" if PM == ss.PREC_MIXED else "fp64",
-                                                   roofline["kernel"], K))
+                                                   roofline["kernel"], K,
+                                                   rows=args.mixed_rows if PM == ss.PREC_MIXED else None))
         copy_ms = ctx.bench_copy(1 << 30, reps=10)
         copy_gbs = 2.0 * (1 << 30) / (copy_ms * 1e-3) / 1e9
 
@@ -691,7 +695,7 @@ def trace_summary(path):
     lo, hi = marks[1], marks[2]   # each marker is a warm launch plus one timed launch
     stats = {}
     for t0, t1, name, _ in rows[lo + 1:hi]:
-        key = name.split("(")[0].replace("void ", "")
+        key = _kernel_key(name)
         s = stats.setdefault(key, [0, 0.0])
         s[0] += 1
         s[1] += (t1 - t0) / 1e6
@@ -727,6 +731,38 @@ def _template_args(name):
     return out
 
 
+def _kernel_base(name):
+    """A kernel's name without template arguments or parameters, from its
+    demangled or (for names rocprofv3 leaves mangled) Itanium-mangled form."""
+    import re
+    m = re.match(r"_ZN2ss(\d+)", name)
+    if m:
+        n = int(m.group(1))
+        return "ss::" + name[m.end():m.end() + n]
+    return name.split("(")[0].split("<")[0].replace("void ", "")
+
+
+def _kernel_bwd(name, pos):
+    """The bool template argument at position pos of a solve kernel (its
+    direction: true = backward), demangled or mangled (Lb0E / Lb1E)."""
+    if name.startswith("_Z"):
+        i = name.find("Lb")
+        return i >= 0 and name[i + 2] == "1"
+    args = _template_args(name)
+    return len(args) > pos and args[pos] == "true"
+
+
+_DIR_ARG = {"ss::k_trisolve_level": 3, "ss::k_trisolve_huge": 3, "ss::k_level_copy": 3, "ss::k_tri_small": 3,
+            "ss::k_core_gather": 2, "ss::k_core_apply": 1}
+
+
+def _kernel_key(name):
+    """A readable per-kernel key: the base name, with the direction of a
+    solve kernel."""
+    b = _kernel_base(name)
+    return b + (" (%s)" % ("bwd" if _kernel_bwd(name, _DIR_ARG[b]) else "fwd") if b in _DIR_ARG else "")
+
+
 def _counter_rows(path):
     """(dispatch id, kernel name, value) of one rocprofv3 counter CSV."""
     import csv
@@ -757,9 +793,9 @@ def pmc_summary(fetch_csv, write_csv):
           "written,\nso traffic = %.2f x FETCH_SIZE + %.2f x WRITE_SIZE (KiB).\n" %
           (fetch[copy[-1]][2], write[copy[-1]][2], cf, cw))
     traffic = [(f[1], 1024.0 * (cf * f[2] + cw * w[2])) for f, w in zip(fetch, write)]
-    short = [t[0].split("(")[0].replace("void ", "") for t in traffic]
+    short = [_kernel_base(t[0]) for t in traffic]
     per = {}
-    for nm, b in zip(short, (t[1] for t in traffic)):
+    for nm, b in zip((_kernel_key(t[0]) for t in traffic), (t[1] for t in traffic)):
         e = per.setdefault(nm, [0, 0.0])
         e[0] += 1
         e[1] += b
@@ -778,19 +814,17 @@ def pmc_summary(fetch_csv, write_csv):
     names = [t[0] for t in traffic]
 
     def fwd_level(i):
-        return any(x in short[i] for x in level) and not (len(_template_args(names[i])) >= 4 and
-                                                          _template_args(names[i])[3] == "true")
+        return any(x in short[i] for x in level) and not _kernel_bwd(names[i], 3)
     head = set()
     for j, nm in enumerate(short):
-        if "k_core_gather" in nm and _template_args(names[j])[-1] == "false" and j > 0 and fwd_level(j - 1):
+        if "k_core_gather" in nm and not _kernel_bwd(names[j], 2) and j > 0 and fwd_level(j - 1):
             head.add(j - 1)
             if "k_trisolve_level" in short[j - 1] and j > 1 and "k_trisolve_huge" in short[j - 2] and fwd_level(j - 2):
                 head.add(j - 2)
     for i, (nm, b) in enumerate(traffic):
         base = short[i]
         if any(x in base for x in level):
-            args = _template_args(nm)
-            bwd = len(args) >= 4 and args[3] == "true"
+            bwd = _kernel_bwd(nm, 3)
             kind = "core_fwd" if i in head else ("tri_bwd" if bwd else "tri_fwd")
             add(kind, b, 0 if "k_trisolve_huge" in base else 1)
         elif "kb_spmm_dot" in base:
