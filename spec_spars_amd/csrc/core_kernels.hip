@@ -63,13 +63,20 @@ __global__ __launch_bounds__(256) void k_to_f32(int64_t count, const double *__r
         b[e] = (float)a[e];
 }
 
+__global__ __launch_bounds__(256) void k_to_f16(int64_t count, const double *__restrict__ a,
+                                                _Float16 *__restrict__ b) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < count;
+         e += (int64_t)gridDim.x * blockDim.x)
+        b[e] = (_Float16)a[e];
+}
+
 // Tc[r] = forward: out[vert[r]] (the head's r_c - L_co y_o);
 //         backward: in[vert[r]] / D (D == 0 keeps the value, as the level
 //         solves do); zero on padding rows (vert < 0).
-template <typename T, typename V, bool BWD>
+template <typename T, typename V, typename TC, bool BWD>
 __global__ __launch_bounds__(256) void k_core_gather(int64_t rows, const int32_t *__restrict__ vert,
                                                      const double *__restrict__ diag, int k,
-                                                     const V *__restrict__ src, T *__restrict__ tc) {
+                                                     const V *__restrict__ src, TC *__restrict__ tc) {
     const int64_t total = rows * k;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
          e += (int64_t)gridDim.x * blockDim.x) {
@@ -84,7 +91,7 @@ __global__ __launch_bounds__(256) void k_core_gather(int64_t rows, const int32_t
                 if (d != 0.0) x = (T)((double)x / d);
             }
         }
-        tc[e] = x;
+        tc[e] = (TC)x;
     }
 }
 
@@ -208,6 +215,113 @@ __global__ __launch_bounds__(256) void k_core_apply(const CorePiece *__restrict_
             if (c < k)
 #pragma unroll
                 for (int i = 0; i < 4; i++) po[(size_t)(wr + 16 * x + acc_row<T>(lk, i)) * k + c] = acc[x][y][i];
+        }
+}
+
+// The dense-core products of the mixed path with FP16 preconditioner
+// rows: M and the gathered rows in FP16, FP32 accumulation on the matrix
+// cores (16 x 16 x 16 FP16), partials in FP32.  The pieces, tiles and
+// double-buffered stages of k_core_apply, kAKH inner rows per stage, held
+// inner-index-contiguous in LDS so that a lane's four inner values for one
+// matrix-core step are one 8-byte read: lane l holds A(row l % 16, inner
+// 4 (l / 16) .. + 3) and Tc(the same inner, column l % 16); the outputs land
+// as in the FP32 kernel (row 4 (l / 16) + i, column l % 16).  Each output
+// adds its terms in ascending inner order, sixteen at a time, for every k
+// and RB.
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+constexpr int kAKH = 32;   // inner rows per stage of k_core_apply_h
+constexpr int kAKHPad = 8; // LDS row padding (halves): rows stay 8-byte aligned
+template <bool BWD, int RB>
+__global__ __launch_bounds__(256) void k_core_apply_h(const CorePiece *__restrict__ pieces,
+                                                      const int32_t *__restrict__ cpad,
+                                                      const int64_t *__restrict__ dense_off,
+                                                      const int64_t *__restrict__ tc_off,
+                                                      const _Float16 *__restrict__ M, int k,
+                                                      const _Float16 *__restrict__ tc, float *__restrict__ part) {
+    static_assert(RB == 1 || RB == 2 || RB == 4, "row blocks per wave");
+    static_assert((kAKH + kAKHPad) % 4 == 0 && kAKH % 16 == 0, "8-byte LDS reads, whole matrix-core steps");
+    constexpr int KC = 64 * RB;                 // columns held
+    constexpr int NA = kCT * kAKH / 256;        // A elements per thread per stage
+    constexpr int NB = kAKH * KC / 256;         // B elements per thread per stage
+    __shared__ __align__(16) _Float16 as[2][kCT][kAKH + kAKHPad];   // as[.][r][m]: A(r, m)
+    __shared__ __align__(16) _Float16 bs[2][KC][kAKH + kAKHPad];    // bs[.][c][m]: Tc(m, c)
+    const CorePiece pc = pieces[blockIdx.x];
+    const int cp = cpad[pc.graph];
+    const _Float16 *Mg = M + dense_off[pc.graph];
+    const _Float16 *tg = tc + (size_t)tc_off[pc.graph] * k;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int wr = (wv % (4 / RB)) * 16 * RB;   // this wave's first row
+    const int wc = (wv / (4 / RB)) * 64;        // and first column
+    const int li = lane & 15, lk = lane >> 4;
+    _Float16 ra[NA], rb[NB];
+    // stage loads: forward A(r, m) = M[r0 + r][m0 + m] (element e: r = e /
+    // kAKH, m = e % kAKH), backward A(r, m) = M[m0 + m][r0 + r] (m = e /
+    // kCT, r = e % kCT); B element e: Tc row m0 + e / KC, column e % KC
+    // (zero past k)
+    auto load = [&](int m0) {
+#pragma unroll
+        for (int u = 0; u < NA; u++) {
+            const int e = t + 256 * u;
+            ra[u] = BWD ? Mg[(size_t)(m0 + e / kCT) * cp + pc.r0 + e % kCT]
+                        : Mg[(size_t)(pc.r0 + e / kAKH) * cp + m0 + e % kAKH];
+        }
+#pragma unroll
+        for (int u = 0; u < NB; u++) {
+            const int e = t + 256 * u, c = e % KC;
+            rb[u] = c < k ? tg[(size_t)(m0 + e / KC) * k + c] : (_Float16)0;
+        }
+    };
+    auto store = [&](int b) {
+#pragma unroll
+        for (int u = 0; u < NA; u++) {
+            const int e = t + 256 * u;
+            if (BWD)
+                as[b][e % kCT][e / kCT] = ra[u];
+            else
+                as[b][e / kAKH][e % kAKH] = ra[u];
+        }
+#pragma unroll
+        for (int u = 0; u < NB; u++) {
+            const int e = t + 256 * u;
+            bs[b][e % KC][e / KC] = rb[u];
+        }
+    };
+    f32x4 acc[RB][4];
+#pragma unroll
+    for (int x = 0; x < RB; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) acc[x][y] = f32x4{0, 0, 0, 0};
+    load(pc.m0);
+    int b = 0;
+    for (int m0 = pc.m0; m0 < pc.m1; m0 += kAKH, b ^= 1) {
+        // buffer b was last read two stages ago: every wave has passed the
+        // previous stage's barrier since
+        store(b);
+        __syncthreads();
+        if (m0 + kAKH < pc.m1) load(m0 + kAKH);
+#pragma unroll
+        for (int mm = 0; mm < kAKH; mm += 16) {
+            f16x4 a[RB], bb[4];
+#pragma unroll
+            for (int x = 0; x < RB; x++) a[x] = *reinterpret_cast<const f16x4 *>(&as[b][wr + 16 * x + li][mm + 4 * lk]);
+#pragma unroll
+            for (int y = 0; y < 4; y++) bb[y] = *reinterpret_cast<const f16x4 *>(&bs[b][wc + 16 * y + li][mm + 4 * lk]);
+#pragma unroll
+            for (int x = 0; x < RB; x++)
+#pragma unroll
+                for (int y = 0; y < 4; y++)
+                    acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x16f16(a[x], bb[y], acc[x][y], 0, 0, 0);
+        }
+    }
+    float *po = part + (size_t)blockIdx.x * kCT * k;
+#pragma unroll
+    for (int x = 0; x < RB; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) {
+            const int c = wc + 16 * y + li;
+            if (c < k)
+#pragma unroll
+                for (int i = 0; i < 4; i++) po[(size_t)(wr + 16 * x + 4 * lk + i) * k + c] = acc[x][y][i];
         }
 }
 
@@ -338,11 +452,15 @@ void launch_to_f32(hipStream_t s, int64_t count, const double *a, float *b) {
     if (count > 0) hipLaunchKernelGGL(k_to_f32, dim3(stream_grid(count)), dim3(256), 0, s, count, a, b);
 }
 
+void launch_to_f16(hipStream_t s, int64_t count, const double *a, _Float16 *b) {
+    if (count > 0) hipLaunchKernelGGL(k_to_f16, dim3(stream_grid(count)), dim3(256), 0, s, count, a, b);
+}
+
 template <typename T, typename V, bool BWD>
 void launch_core_apply(hipStream_t s, const CoreApply &c, const T *M, int k, const double *diag, const V *src,
                        T *tc, T *part, V *out) {
     if (c.pieces <= 0) return;
-    hipLaunchKernelGGL((k_core_gather<T, V, BWD>), dim3(stream_grid(c.rows * k)), dim3(256), 0, s, c.rows, c.vert,
+    hipLaunchKernelGGL((k_core_gather<T, V, T, BWD>), dim3(stream_grid(c.rows * k)), dim3(256), 0, s, c.rows, c.vert,
                        diag, k, src, tc);
     const CorePiece *pc = BWD ? c.piece_bwd : c.piece_fwd;
     if (k <= 64)
@@ -366,5 +484,29 @@ SS_INST_CORE_APPLY(float, float, false)
 SS_INST_CORE_APPLY(float, float, true)
 SS_INST_CORE_APPLY(float, _Float16, false)
 SS_INST_CORE_APPLY(float, _Float16, true)
+
+template <bool BWD>
+void launch_core_apply_h(hipStream_t s, const CoreApply &c, const _Float16 *M, int k, const double *diag,
+                         const _Float16 *src, _Float16 *tc, float *part, _Float16 *out) {
+    if (c.pieces <= 0) return;
+    hipLaunchKernelGGL((k_core_gather<float, _Float16, _Float16, BWD>), dim3(stream_grid(c.rows * k)), dim3(256), 0,
+                       s, c.rows, c.vert, diag, k, src, tc);
+    const CorePiece *pc = BWD ? c.piece_bwd : c.piece_fwd;
+    if (k <= 64)
+        hipLaunchKernelGGL((k_core_apply_h<BWD, 1>), dim3((unsigned)c.pieces), dim3(256), 0, s, pc, c.cpad,
+                           c.dense_off, c.tc_off, M, k, tc, part);
+    else if (k <= 128)
+        hipLaunchKernelGGL((k_core_apply_h<BWD, 2>), dim3((unsigned)c.pieces), dim3(256), 0, s, pc, c.cpad,
+                           c.dense_off, c.tc_off, M, k, tc, part);
+    else
+        hipLaunchKernelGGL((k_core_apply_h<BWD, 4>), dim3((unsigned)c.pieces), dim3(256), 0, s, pc, c.cpad,
+                           c.dense_off, c.tc_off, M, k, tc, part);
+    hipLaunchKernelGGL((k_core_reduce<float, _Float16>), dim3(stream_grid((int64_t)c.tiles * kCT * k)), dim3(256), 0,
+                       s, c.tiles, BWD ? c.tile_piece_bwd : c.tile_piece_fwd, c.tile_row, c.vert, k, part, out);
+}
+template void launch_core_apply_h<false>(hipStream_t, const CoreApply &, const _Float16 *, int, const double *,
+                                         const _Float16 *, _Float16 *, float *, _Float16 *);
+template void launch_core_apply_h<true>(hipStream_t, const CoreApply &, const _Float16 *, int, const double *,
+                                        const _Float16 *, _Float16 *, float *, _Float16 *);
 
 }  // namespace ss

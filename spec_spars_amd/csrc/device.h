@@ -158,8 +158,9 @@ struct ss_graph {
     int64_t core_total = 0;               // core rows of the batch
     int64_t core_entries = 0;             // entries of the padded inverses (sum of cp^2)
     ss::CoreApply core;                   // device arrays of the core products
-    double *Md = nullptr;                 // inverses, FP64 and FP32
+    double *Md = nullptr;                 // inverses, FP64, FP32 and FP16 (the mixed path's FP16 rows)
     float *Mf = nullptr;
+    _Float16 *Mh = nullptr;
     ss::DevArena arena_core_work;         // gathered right-hand sides and product pieces
     int core_wcap = 0, core_wprec = -1;   // columns / precision they are sized for
     void *core_tc = nullptr, *core_part = nullptr;

@@ -484,6 +484,7 @@ static int core_invert(ss_graph *g, const Factor &F, const std::vector<int32_t> 
     for (int i = 0; i < g->G; i++)
         if (cpad[i] > 0) launch_core_invert(s, cpad[i], L + dense_off[i], g->Md + dense_off[i]);
     launch_to_f32(s, total, g->Md, g->Mf);
+    launch_to_f16(s, total, g->Md, g->Mh);
     SS_HIP(hipGetLastError());
     SS_HIP(hipStreamSynchronize(s));   // the host vectors and tmp are freed on return
     return SS_OK;
@@ -598,6 +599,7 @@ int batch_upload_factors(ss_graph *g, const std::vector<const Factor *> &factors
     SS_TRY(A.alloc(&d_tpb, tpb.size()));
     SS_TRY(A.alloc(&g->Md, dense_total));
     SS_TRY(A.alloc(&g->Mf, dense_total));
+    SS_TRY(A.alloc(&g->Mh, dense_total));
     hipStream_t s = g->ctx->stream;
     std::vector<float> fvf(fw.val.begin(), fw.val.end()), bvf(bw.val.begin(), bw.val.end());
     std::vector<int32_t> frows(fw.rows), fptr(fw.ptr), brows(bw.rows), bptr(bw.ptr);
@@ -901,14 +903,16 @@ static void solve_cores(ss_graph *g, int k, const TI *in, V *out, RowScale rs) {
     ProfScope ps(g, BWD ? SS_KERNEL_CORE_BWD : SS_KERNEL_CORE_FWD, BWD ? 3 : 4);
     T *tc = static_cast<T *>(g->core_tc), *part = static_cast<T *>(g->core_part);
     S *outs = reinterpret_cast<S *>(out);
-    if constexpr (BWD) {
-        launch_core_apply<T, S, true>(s, g->core, core_M<T>(g), k, g->diag, reinterpret_cast<const S *>(in), tc,
-                                      part, outs);
-    } else {
+    if constexpr (!BWD)
         launch_core_head<T, TI, V>(s, g->f_core_t0, g->core_huge, g->f_nrows, g->f_rows, g->f_ptr, g->f_col,
                                    fvals<T>(g), g->diag, k, in, out, rs);
-        launch_core_apply<T, S, false>(s, g->core, core_M<T>(g), k, g->diag, outs, tc, part, outs);
-    }
+    const S *src = BWD ? reinterpret_cast<const S *>(in) : outs;
+    if constexpr (std::is_same<S, _Float16>::value)
+        // FP16 rows: FP16 products on the matrix cores
+        launch_core_apply_h<BWD>(s, g->core, g->Mh, k, g->diag, src, static_cast<_Float16 *>(g->core_tc),
+                                 static_cast<float *>(g->core_part), outs);
+    else
+        launch_core_apply<T, S, BWD>(s, g->core, core_M<T>(g), k, g->diag, src, tc, part, outs);
 }
 
 // Zp = L^{-T} D^+ L^{-1} R (W holds the forward result y).  rz_partial

@@ -119,12 +119,18 @@ void launch_core_scatter(hipStream_t s, int64_t count, const int64_t *pos, const
 // M = L^{-1} of one unit-lower cp x cp block (strictly lower part in L).
 void launch_core_invert(hipStream_t s, int cp, const double *L, double *M);
 void launch_to_f32(hipStream_t s, int64_t count, const double *a, float *b);
+void launch_to_f16(hipStream_t s, int64_t count, const double *a, _Float16 *b);
 // Forward: out[core rows] = M (out[core rows]); backward: out[core rows] =
 // M^T (D^+ src[core rows]).  tc: rows x k scratch; part: pieces x kCoreTile x k.
 // src and out rows are stored as V (T, or scaled FP16).
 template <typename T, typename V, bool BWD>
 void launch_core_apply(hipStream_t s, const CoreApply &c, const T *M, int k, const double *diag, const V *src,
                        T *tc, T *part, V *out);
+// The same for FP16 rows with M in FP16 (matrix-core FP16 products, FP32
+// accumulation): tc holds FP16 gathered rows, part FP32 partials.
+template <bool BWD>
+void launch_core_apply_h(hipStream_t s, const CoreApply &c, const _Float16 *M, int k, const double *diag,
+                         const _Float16 *src, _Float16 *tc, float *part, _Float16 *out);
 template <typename TZ>
 void launch_er_sample(hipStream_t s, const ErSampleArgs<TZ> &a);
 
