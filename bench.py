@@ -119,8 +119,9 @@ def kernel_bytes(n, m, G, k, kept, s=8, srow=None):
     return {
         # sketch: incidence lists (edge id + signed sqrt(w)) and Y (FP64)
         "sketch": 4 * (n + 1) + 2 * m * (4 + 8) + n * k * 8,
-        # SpMM: rowptr, (col, val) per nonzero, read P, write Q
-        "spmm": 4 * (n + 1) + nnzL * (4 + s) + 2 * n * k * s,
+        # SpMM: rowptr, (col, val) per nonzero, read P, write Q; with FP16
+        # preconditioner rows Q = A Z + beta Q: read Z (FP16), P, Q, write Q
+        "spmm": 4 * (n + 1) + nnzL * (4 + s) + (2 * n * k * s if srow == s else n * k * (srow + 3 * s)),
         "tri_fwd": per_level(fi, False),
         "tri_bwd": per_level(bi, True),
         "tri_fwd_small": small(fi, False),

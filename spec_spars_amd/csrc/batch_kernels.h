@@ -106,6 +106,11 @@ void bt_update_x(hipStream_t s, Tiles rt, int k, const double *alpha, const T *P
 template <typename T>
 void bt_spmm_dot(hipStream_t s, Tiles rt, const int32_t *rowptr, const int32_t *col, const T *val, int k, const T *P,
                  T *Q, double *partial);
+// Q = A Z zinv + beta Q (FP16 preconditioner rows, k even): the search
+// directions' recurrence with the gathers on the FP16 Z rows; partials of
+// P . Q from the FP32 P rows.
+void bt_spmm_z(hipStream_t s, Tiles rt, const int32_t *rowptr, const int32_t *col, const float *val, int k,
+               const _Float16 *Z, const float *zinv, const double *beta, const float *P, float *Q, double *partial);
 // R -= alpha Q, per-tile partials of |R|^2.
 template <typename T>
 void bt_update_r_rr(hipStream_t s, Tiles rt, int k, const double *alpha, const T *Q, T *R, double *partial);

@@ -1009,7 +1009,16 @@ static int pcg_iteration(ss_graph *g, int k, T *X, const PcgWork<T> &w, int rule
     }
     {
         ProfScope ps(g, SS_KERNEL_SPMM);
-        bt_spmm_dot<T>(s, stiles(g), g->rowptr, g->col, avals<T>(g), k, w.P, w.Q, g->partial);   // Q = A P, P.Q
+        // Q = A P, P.Q; with FP16 preconditioner rows as A Z + beta Q
+        if constexpr (std::is_same<T, float>::value) {
+            if (half)
+                bt_spmm_z(s, stiles(g), g->rowptr, g->col, g->valf, k, reinterpret_cast<const _Float16 *>(w.Zp),
+                          g->prec_inv, g->sc.beta, w.P, w.Q, g->partial);
+            else
+                bt_spmm_dot<T>(s, stiles(g), g->rowptr, g->col, avals<T>(g), k, w.P, w.Q, g->partial);
+        } else {
+            bt_spmm_dot<T>(s, stiles(g), g->rowptr, g->col, avals<T>(g), k, w.P, w.Q, g->partial);
+        }
     }
     bt_pcg_alpha(s, gk, k, g->sgtile, g->partial, g->sc, g->status, delta2 >= 0.0 ? g->rel_flag : nullptr);
     {
