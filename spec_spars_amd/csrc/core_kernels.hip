@@ -100,7 +100,6 @@ __global__ __launch_bounds__(256) void k_core_gather(int64_t rows, const int32_t
 // column l % 16, row acc_row<T>(l / 16, i): 4 (l / 16) + i in FP32,
 // (l / 16) + 4 i in FP64 (the two instructions deal their outputs out
 // differently; checked on the device).
-typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ f32x4 mfma16(float a, float b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
@@ -228,7 +227,6 @@ __global__ __launch_bounds__(256) void k_core_apply(const CorePiece *__restrict_
 // as in the FP32 kernel (row 4 (l / 16) + i, column l % 16).  Each output
 // adds its terms in ascending inner order, sixteen at a time, for every k
 // and RB.
-typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 constexpr int kAKH = 32;   // inner rows per stage of k_core_apply_h
 constexpr int kAKHPad = 8; // LDS row padding (halves): rows stay 8-byte aligned
 template <bool BWD, int RB>

@@ -894,6 +894,7 @@ static void solve_levels(ss_graph *g, int k, const TI *in, V *out, RowScale rs) 
 template <typename V> struct ScalarOf { typedef V type; };
 template <> struct ScalarOf<f32x2> { typedef float type; };
 template <> struct ScalarOf<f16x2> { typedef _Float16 type; };
+template <> struct ScalarOf<f16x4> { typedef _Float16 type; };
 
 template <typename T, typename TI, typename V, bool BWD>
 static void solve_cores(ss_graph *g, int k, const TI *in, V *out, RowScale rs) {
@@ -955,6 +956,10 @@ static int precond(ss_graph *g, int k, const T *R, T *W, T *Zp, double *rz_parti
 // (graph, column) by prec_sc.
 static RowScale prec_scale(const ss_graph *g) { return RowScale{g->prec_sc, g->row_graph}; }
 static int precond_half(ss_graph *g, int k, const PcgWork<float> &w) {
+    if (k % 4 == 0)   // four columns per lane: one 8-byte access
+        return precond_v<float, f32x4, f16x4>(g, k, reinterpret_cast<const f32x4 *>(w.R),
+                                              reinterpret_cast<f16x4 *>(w.W), reinterpret_cast<f16x4 *>(w.Zp),
+                                              g->partial2, prec_scale(g));
     return precond_v<float, f32x2, f16x2>(g, k, reinterpret_cast<const f32x2 *>(w.R), reinterpret_cast<f16x2 *>(w.W),
                                           reinterpret_cast<f16x2 *>(w.Zp), g->partial2, prec_scale(g));
 }

@@ -81,8 +81,9 @@ __global__ void k_fill_u32(int64_t count, uint32_t *p, uint32_t v) {
 //
 // A row of the solve is stored as ke = k / n elements of type V, each
 // holding n = Lane<V>::n adjacent columns: V = T (n = 1), or for FP32
-// arithmetic two columns per element, as FP32 (f32x2) or FP16 (f16x2), so
-// one lane moves two columns with one 8- or 4-byte access.  A lane's value
+// arithmetic two columns per element, as FP32 (f32x2) or FP16 (f16x2), or
+// four as FP16 (f16x4), so one lane moves two or four columns with one 4-
+// or 8-byte access.  A lane's value
 // is Lane<V>::type; the arithmetic of every column is the same whatever n
 // is.  The solve's input (forward: the PCG residual; backward: the forward
 // result) is read the same way, as elements of type VI.
@@ -92,6 +93,8 @@ template <> struct Lane<double> { typedef double type; static constexpr int n = 
 template <> struct Lane<float> { typedef float type; static constexpr int n = 1; };
 template <> struct Lane<f32x2> { typedef f32x2 type; static constexpr int n = 2; };
 template <> struct Lane<f16x2> { typedef f32x2 type; static constexpr int n = 2; };
+template <> struct Lane<f32x4> { typedef f32x4 type; static constexpr int n = 4; };
+template <> struct Lane<f16x4> { typedef f32x4 type; static constexpr int n = 4; };
 
 // An element as stored -> the lane value.  The gather loops hold the
 // stored elements and widen them where they are used, so all of a group's
@@ -104,12 +107,16 @@ __device__ __forceinline__ double cvt(double x) { return x; }
 __device__ __forceinline__ float cvt(float x) { return x; }
 __device__ __forceinline__ f32x2 cvt(f32x2 x) { return x; }
 __device__ __forceinline__ f32x2 cvt(f16x2 x) { return __builtin_convertvector(x, f32x2); }
+__device__ __forceinline__ f32x4 cvt(f32x4 x) { return x; }
+__device__ __forceinline__ f32x4 cvt(f16x4 x) { return __builtin_convertvector(x, f32x4); }
 template <typename V> __device__ __forceinline__ typename Lane<V>::type ldv(const V *p) { return cvt(*p); }
 template <typename V> constexpr bool widened() { return sizeof(typename Lane<V>::type) > sizeof(V); }
 __device__ __forceinline__ void stv(double *p, double v) { *p = v; }
 __device__ __forceinline__ void stv(float *p, float v) { *p = v; }
 __device__ __forceinline__ void stv(f32x2 *p, f32x2 v) { *p = v; }
 __device__ __forceinline__ void stv(f16x2 *p, f32x2 v) { *p = __builtin_convertvector(v, f16x2); }
+__device__ __forceinline__ void stv(f32x4 *p, f32x4 v) { *p = v; }
+__device__ __forceinline__ void stv(f16x4 *p, f32x4 v) { *p = __builtin_convertvector(v, f16x4); }
 
 // Two-column lanes: each column its own fused multiply-add / rounding.
 __device__ __forceinline__ f32x2 fnms(float v, f32x2 x, f32x2 acc) {
@@ -118,12 +125,28 @@ __device__ __forceinline__ f32x2 fnms(float v, f32x2 x, f32x2 acc) {
     r.y = __builtin_fmaf(-v, x.y, acc.y);
     return r;
 }
+__device__ __forceinline__ f32x4 fnms(float v, f32x4 x, f32x4 acc) {
+    f32x4 r;
+    r.x = __builtin_fmaf(-v, x.x, acc.x);
+    r.y = __builtin_fmaf(-v, x.y, acc.y);
+    r.z = __builtin_fmaf(-v, x.z, acc.z);
+    r.w = __builtin_fmaf(-v, x.w, acc.w);
+    return r;
+}
 __device__ __forceinline__ double div_d(double x, double d) { return x / d; }
 __device__ __forceinline__ float div_d(float x, double d) { return (float)((double)x / d); }
 __device__ __forceinline__ f32x2 div_d(f32x2 x, double d) {
     f32x2 r;
     r.x = (float)((double)x.x / d);
     r.y = (float)((double)x.y / d);
+    return r;
+}
+__device__ __forceinline__ f32x4 div_d(f32x4 x, double d) {
+    f32x4 r;
+    r.x = (float)((double)x.x / d);
+    r.y = (float)((double)x.y / d);
+    r.z = (float)((double)x.z / d);
+    r.w = (float)((double)x.w / d);
     return r;
 }
 
@@ -135,6 +158,9 @@ template <> __device__ __forceinline__ float scale_at<float>(const float *s, siz
 }
 template <> __device__ __forceinline__ f32x2 scale_at<f32x2>(const float *s, size_t base, int e) {
     return reinterpret_cast<const f32x2 *>(s + base)[e];
+}
+template <> __device__ __forceinline__ f32x4 scale_at<f32x4>(const float *s, size_t base, int e) {
+    return reinterpret_cast<const f32x4 *>(s + base)[e];
 }
 template <> __device__ __forceinline__ double scale_at<double>(const float *s, size_t base, int e) {
     return (double)s[base + e];
@@ -870,6 +896,7 @@ SS_INST_HEAD(double, double, double)
 SS_INST_HEAD(float, float, float)
 SS_INST_HEAD(float, f32x2, f32x2)
 SS_INST_HEAD(float, f32x2, f16x2)
+SS_INST_HEAD(float, f32x4, f16x4)
 
 template <typename T, typename TI, typename V, bool BWD>
 void launch_tri_small(hipStream_t s, int32_t l0, int32_t l1, const int32_t *level_ptr, const int32_t *rows,
@@ -891,6 +918,8 @@ SS_INST_SMALL(float, f32x2, f32x2, false)
 SS_INST_SMALL(float, f32x2, f32x2, true)
 SS_INST_SMALL(float, f32x2, f16x2, false)
 SS_INST_SMALL(float, f16x2, f16x2, true)
+SS_INST_SMALL(float, f32x4, f16x4, false)
+SS_INST_SMALL(float, f16x4, f16x4, true)
 
 template <typename T, typename TI, typename V, bool BWD>
 void launch_trisolve_level(hipStream_t s, int32_t t0, int32_t nhuge, int32_t hm, int32_t nlong, int32_t nmed,
@@ -944,6 +973,8 @@ SS_INST_TRI(float, f32x2, f32x2, false)
 SS_INST_TRI(float, f32x2, f32x2, true)
 SS_INST_TRI(float, f32x2, f16x2, false)
 SS_INST_TRI(float, f16x2, f16x2, true)
+SS_INST_TRI(float, f32x4, f16x4, false)
+SS_INST_TRI(float, f16x4, f16x4, true)
 
 
 template <typename TZ>
