@@ -452,6 +452,15 @@ __device__ __forceinline__ f32x2 bt_fma(float a, f32x2 b, f32x2 c) {
 template <typename V> __device__ __forceinline__ double bt_col(V v, int h) { return (double)v; }
 template <> __device__ __forceinline__ double bt_col<f32x2>(f32x2 v, int h) { return (double)(h ? v.y : v.x); }
 __device__ __forceinline__ f32x2 bt_widen(f16x2 x) { return __builtin_convertvector(x, f32x2); }
+__device__ __forceinline__ f32x4 bt_widen(f16x4 x) { return __builtin_convertvector(x, f32x4); }
+__device__ __forceinline__ f32x4 bt_fma(float a, f32x4 b, f32x4 c) {
+    f32x4 r;
+    r.x = __builtin_fmaf(a, b.x, c.x);
+    r.y = __builtin_fmaf(a, b.y, c.y);
+    r.z = __builtin_fmaf(a, b.z, c.z);
+    r.w = __builtin_fmaf(a, b.w, c.w);
+    return r;
+}
 
 // V: T, or f32x2 (two columns per lane, rows of k / 2 elements).  Every
 // column's arithmetic is the same for either.
@@ -563,27 +572,35 @@ __global__ __launch_bounds__(1024) void kb_spmm_dot(Tiles rt, const int32_t *__r
 // row's entries loads the row's own element and its term is selected
 // away).  Q drifts from A P by FP32 rounding across iterations; the
 // reliable updates' FP64 residuals are computed afresh, so the drift only
-// shows in the recursive residual between replacements.
+// shows in the recursive residual between replacements.  VZ: f16x2 (two
+// columns per lane, LV = f32x2) or f16x4 (four, LV = f32x4; k a multiple of
+// 4; 8 gathers in flight per row of a short group, for the registers).
+template <typename VZ, typename LV>
 __global__ __launch_bounds__(1024) void kb_spmm_z(Tiles rt, const int32_t *__restrict__ rowptr,
                                                  const int32_t *__restrict__ col, const float *__restrict__ val,
-                                                 int k, const f16x2 *__restrict__ Z, const float *__restrict__ zinv,
-                                                 const double *__restrict__ beta, const f32x2 *__restrict__ P,
-                                                 f32x2 *__restrict__ Q, double *__restrict__ partial) {
-    constexpr int G = kSpmmGroup, E = 2 * kSpmmShort;
+                                                 int k, const VZ *__restrict__ Z, const float *__restrict__ zinv,
+                                                 const double *__restrict__ beta, const LV *__restrict__ P,
+                                                 LV *__restrict__ Q, double *__restrict__ partial) {
+    constexpr int NC = sizeof(LV) / 4;   // columns per lane
+    constexpr int G = kSpmmGroup, E = NC == 2 ? 2 * kSpmmShort : kSpmmShort;
     __shared__ double sm[kMaxTileWaves][64];
     const int t = blockIdx.x, NW = BT_NW, lane = BT_LANE, g = rt.graph[t];
     const int r0 = rt.start[t], r1 = rt.start[t + 1];
-    const int ke = k / 2;
+    const int ke = k / NC;
     for (int eb = 0; eb < ke; eb += 64) {
         const int e = eb + lane, ee = e < ke ? e : ke - 1;
-        const size_t gc = (size_t)g * k + 2 * ee;
-        f32x2 zs, bq;
-        zs.x = zinv[gc];
-        zs.y = zinv[gc + 1];
-        bq.x = (float)beta[gc];
-        bq.y = (float)beta[gc + 1];
-        const bool use_old = bq.x != 0.0f || bq.y != 0.0f;
-        double acc[2] = {0.0, 0.0};
+        const size_t gc = (size_t)g * k + NC * ee;
+        LV zs, bq;
+        bool use_old = false;
+#pragma unroll
+        for (int h = 0; h < NC; h++) {
+            zs[h] = zinv[gc + h];
+            bq[h] = (float)beta[gc + h];
+            use_old = use_old || bq[h] != 0.0f;
+        }
+        double acc[NC];
+#pragma unroll
+        for (int h = 0; h < NC; h++) acc[h] = 0.0;
         for (int rb = bt_uniform(r0 + BT_WAVE); rb < r1; rb += G * NW) {
             int rr[G], q0[G], len[G];
             int maxlen = 0;
@@ -599,11 +616,11 @@ __global__ __launch_bounds__(1024) void kb_spmm_z(Tiles rt, const int32_t *__res
                 len[u] = on ? b - a : 0;
                 maxlen = len[u] > maxlen ? len[u] : maxlen;
             }
-            f32x2 az[G];
+            LV az[G];
             if (maxlen <= E) {
                 int c[G];
                 float v[G];
-                f16x2 x[G][E];
+                VZ x[G][E];
 #pragma unroll
                 for (int u = 0; u < G; u++) {
                     c[u] = lane < len[u] ? col[q0[u] + lane] : 0;
@@ -615,28 +632,27 @@ __global__ __launch_bounds__(1024) void kb_spmm_z(Tiles rt, const int32_t *__res
                     for (int q = 0; q < E; q++) x[u][q] = Z[(size_t)(q < len[u] ? bt_lane(c[u], q) : rr[u]) * ke + ee];
 #pragma unroll
                 for (int u = 0; u < G; u++) {
-                    f32x2 a = f32x2{0.0f, 0.0f};
+                    LV a = LV();
 #pragma unroll
-                    for (int q = 0; q < E; q++)
-                        a = bt_fma(-bt_lane(v[u], q), q < len[u] ? bt_widen(x[u][q]) : f32x2{0.0f, 0.0f}, a);
+                    for (int q = 0; q < E; q++) a = bt_fma(-bt_lane(v[u], q), q < len[u] ? bt_widen(x[u][q]) : LV(), a);
                     az[u] = -a;
                 }
             } else {
 #pragma unroll
                 for (int u = 0; u < G; u++) {
-                    f32x2 a = f32x2{0.0f, 0.0f};
+                    LV a = LV();
                     for (int base = 0; base < len[u]; base += 64) {
                         const int n = len[u] - base;
                         const int cc = lane < n ? col[q0[u] + base + lane] : 0;
                         const float vv = lane < n ? val[q0[u] + base + lane] : 0.0f;
                         for (int e0 = 0; e0 < 64 && e0 < n; e0 += 16) {
-                            f16x2 x[16];
+                            VZ x[16];
 #pragma unroll
                             for (int q = 0; q < 16; q++)
                                 x[q] = Z[(size_t)(e0 + q < n ? bt_lane(cc, e0 + q) : rr[u]) * ke + ee];
 #pragma unroll
                             for (int q = 0; q < 16; q++)
-                                a = bt_fma(-bt_lane(vv, e0 + q), e0 + q < n ? bt_widen(x[q]) : f32x2{0.0f, 0.0f}, a);
+                                a = bt_fma(-bt_lane(vv, e0 + q), e0 + q < n ? bt_widen(x[q]) : LV(), a);
                         }
                     }
                     az[u] = -a;
@@ -646,23 +662,21 @@ __global__ __launch_bounds__(1024) void kb_spmm_z(Tiles rt, const int32_t *__res
             for (int u = 0; u < G; u++) {
                 if (rb + u * NW < r1 && e < ke) {
                     const size_t o = (size_t)rr[u] * ke + e;
-                    f32x2 qv;
+                    LV qv = az[u] * zs;
                     if (use_old) {
-                        const f32x2 qo = Q[o];
-                        qv.x = __builtin_fmaf(bq.x, qo.x, az[u].x * zs.x);
-                        qv.y = __builtin_fmaf(bq.y, qo.y, az[u].y * zs.y);
-                    } else {
-                        qv = az[u] * zs;
+                        const LV qo = Q[o];
+#pragma unroll
+                        for (int h = 0; h < NC; h++) qv[h] = __builtin_fmaf(bq[h], qo[h], az[u][h] * zs[h]);
                     }
                     Q[o] = qv;
-                    const f32x2 pv = P[o];
-                    acc[0] = bt_fma((double)pv.x, (double)qv.x, acc[0]);
-                    acc[1] = bt_fma((double)pv.y, (double)qv.y, acc[1]);
+                    const LV pv = P[o];
+#pragma unroll
+                    for (int h = 0; h < NC; h++) acc[h] = bt_fma((double)pv[h], (double)qv[h], acc[h]);
                 }
             }
         }
-        bt_tile_store_col(acc[0], 2 * e, k, partial, sm);
-        bt_tile_store_col(acc[1], 2 * e + 1, k, partial, sm);
+#pragma unroll
+        for (int h = 0; h < NC; h++) bt_tile_store_col(acc[h], NC * e + h, k, partial, sm);
     }
 }
 
@@ -1181,9 +1195,14 @@ template void bt_update_x<double>(hipStream_t, Tiles, int, const double *, const
 template void bt_update_x<float>(hipStream_t, Tiles, int, const double *, const float *, float *);
 void bt_spmm_z(hipStream_t s, Tiles rt, const int32_t *rowptr, const int32_t *col, const float *val, int k,
                const _Float16 *Z, const float *zinv, const double *beta, const float *P, float *Q, double *partial) {
-    hipLaunchKernelGGL(kb_spmm_z, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, rowptr, col, val, k,
-                       reinterpret_cast<const f16x2 *>(Z), zinv, beta, reinterpret_cast<const f32x2 *>(P),
-                       reinterpret_cast<f32x2 *>(Q), partial);
+    if (k % 4 == 0)
+        hipLaunchKernelGGL((kb_spmm_z<f16x4, f32x4>), dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, rowptr, col, val,
+                           k, reinterpret_cast<const f16x4 *>(Z), zinv, beta, reinterpret_cast<const f32x4 *>(P),
+                           reinterpret_cast<f32x4 *>(Q), partial);
+    else
+        hipLaunchKernelGGL((kb_spmm_z<f16x2, f32x2>), dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, rowptr, col, val,
+                           k, reinterpret_cast<const f16x2 *>(Z), zinv, beta, reinterpret_cast<const f32x2 *>(P),
+                           reinterpret_cast<f32x2 *>(Q), partial);
 }
 template <typename T>
 void bt_spmm_dot(hipStream_t s, Tiles rt, const int32_t *rowptr, const int32_t *col, const T *val, int k, const T *P,
