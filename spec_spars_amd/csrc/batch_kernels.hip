@@ -908,6 +908,71 @@ __global__ __launch_bounds__(1024) void kb_reliable_residual(Tiles rt, const int
     }
 }
 
+// The same with four columns per lane (k a multiple of 4): FP32 rows of X
+// gathered 16 bytes at a time, FP64 rows of R 32; the lane's columns that
+// are not flagged keep their R and add nothing to the partials.  Every
+// column's arithmetic is kb_reliable_residual's.
+__global__ __launch_bounds__(1024) void kb_reliable_residual4(Tiles rt, const int32_t *__restrict__ rowptr,
+                                                             const int32_t *__restrict__ col,
+                                                             const double *__restrict__ val, int k,
+                                                             const int *__restrict__ flag,
+                                                             const f32x4 *__restrict__ X, f64x4 *__restrict__ R,
+                                                             double *__restrict__ partial) {
+    __shared__ double sm[kMaxTileWaves][64];
+    const int t = blockIdx.x, g = rt.graph[t];
+    const int r0 = rt.start[t], r1 = rt.start[t + 1];
+    const int ke = k / 4;
+    for (int eb = 0; eb < ke; eb += 64) {
+        const int e = eb + BT_LANE;
+        bool onh[4], on = false;
+#pragma unroll
+        for (int h = 0; h < 4; h++) {
+            onh[h] = e < ke && flag[(size_t)g * k + 4 * e + h] != 0;
+            on = on || onh[h];
+        }
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        // a chunk with no replaced column skips its rows (the waves still
+        // meet at the partials' barriers)
+        for (int row = r0 + BT_WAVE; __ballot(on) != 0 && row < r1; row += BT_NW) {
+            const int r = bt_uniform(row);
+            const int q0 = rowptr[r], q1 = rowptr[r + 1];
+            if (on) {
+                const size_t o = (size_t)r * ke + e;
+                const f64x4 v0 = R[o];
+                f64x4 v = v0;
+                int q = q0;
+                for (; q + 7 < q1; q += 8) {
+                    f32x4 x[8];
+                    double w[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        x[u] = X[(size_t)col[q + u] * ke + e];
+                        w[u] = val[q + u];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; u++)
+#pragma unroll
+                        for (int h = 0; h < 4; h++) v[h] = bt_fma(-w[u], (double)x[u][h], v[h]);
+                }
+                for (; q < q1; q++) {
+                    const f32x4 x = X[(size_t)col[q] * ke + e];
+                    const double w = val[q];
+#pragma unroll
+                    for (int h = 0; h < 4; h++) v[h] = bt_fma(-w, (double)x[h], v[h]);
+                }
+#pragma unroll
+                for (int h = 0; h < 4; h++) {
+                    if (!onh[h]) v[h] = v0[h];
+                    else acc[h] = bt_fma(v[h], v[h], acc[h]);
+                }
+                R[o] = v;
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < 4; h++) bt_tile_store_col(acc[h], 4 * e + h, k, partial, sm);
+    }
+}
+
 // Replacement, step 3, with |B64 - A X64|^2 of the flagged columns from the
 // SpMM tiles: a flagged column that meets the FP64 stop rule is done; the
 // others take it as their reference (keep = 1: their FP32 residual is
@@ -1262,8 +1327,12 @@ void bt_reliable_apply(hipStream_t s, int gk, int k, const int32_t *gtile, const
 }
 void bt_reliable_residual(hipStream_t s, Tiles rt, const int32_t *rowptr, const int32_t *col, const double *val,
                           int k, const int *flag, const float *X, double *R, double *partial) {
-    hipLaunchKernelGGL(kb_reliable_residual, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, rowptr, col, val, k, flag,
-                       X, R, partial);
+    if (k % 4 == 0)
+        hipLaunchKernelGGL(kb_reliable_residual4, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, rowptr, col, val, k,
+                           flag, reinterpret_cast<const f32x4 *>(X), reinterpret_cast<f64x4 *>(R), partial);
+    else
+        hipLaunchKernelGGL(kb_reliable_residual, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, rowptr, col, val, k,
+                           flag, X, R, partial);
 }
 void bt_reliable_replace(hipStream_t s, Tiles rt, int k, const double *src, const int *keep, float *dst,
                          float *X32) {

@@ -25,6 +25,7 @@ typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 // Scaling of FP16 solve rows (mixed precision): the forward solve's start
 // values are r times s[g * k + j] for a row of graph g (rg[row], graph 0
