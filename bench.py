@@ -754,7 +754,7 @@ def _kernel_bwd(name, pos):
 
 
 _DIR_ARG = {"ss::k_trisolve_level": 3, "ss::k_trisolve_huge": 3, "ss::k_level_copy": 3, "ss::k_tri_small": 3,
-            "ss::k_core_gather": 2, "ss::k_core_apply": 1}
+            "ss::k_core_gather": 2, "ss::k_core_apply": 1, "ss::k_core_apply_h": 0}
 
 
 def _kernel_key(name):
@@ -828,7 +828,7 @@ def pmc_summary(fetch_csv, write_csv):
             bwd = _kernel_bwd(nm, 3)
             kind = "core_fwd" if i in head else ("tri_bwd" if bwd else "tri_fwd")
             add(kind, b, 0 if "k_trisolve_huge" in base else 1)
-        elif "kb_spmm_dot" in base:
+        elif "kb_spmm_dot" in base or "kb_spmm_z" in base:
             add("spmm", b, 1)
         elif "kb_er_sample" in base:
             add("er_sample", b, 1)
