@@ -52,7 +52,7 @@ def workload(name):
         n, u, v = graphs.chung_lu(5_000_000, 50_000_000, gamma=2.2, seed=0)
         w = graphs.uniform_weights(len(u), seed=0)
         return dict(name=name, desc="Chung-Lu n=5e6 m=5e7 gamma=2.2, weights U[0.5,2], k=64, PCG rel 1e-8, eps=0.3",
-                    n=n, u=u, v=v, w=w, k=64, eps=0.3, precision="mixed")
+                    n=n, u=u, v=v, w=w, k=64, eps=0.3, precision="mixed", rows="fp16")
     if name == "grid3d128":
         n, u, v = graphs.grid3d(128)
         w = graphs.loguniform_weights(len(u), seed=0)
@@ -348,7 +348,7 @@ def step_loop(fn, warmup, steps, tag):
 
 
 # mixed-precision preconditioner rows -> ss_solve_opts.mixed_mode
-MIXED_MODES = {"fp16": 0, "fp32": 2}
+MIXED_MODES = {"fp16": 2, "fp32": 0}
 
 
 def bench_single(W, args):
@@ -567,7 +567,8 @@ def bench_batch(W, args):
         log("batch %.1fs factors %.1fs (nnz %d) upload %.1fs" % (t_graph, t_factor, fnnz, t_upload))
         tune = ctx.autotune(B, k, prec, reps=3) if not args.no_tune else ctx.tuning()
         betas = [graphs.beta_for(g[0], W["eps"]) for g in gs]
-        solve = ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-8, max_iter=2000, precision=prec)
+        solve = ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-8, max_iter=2000, precision=prec,
+                              mixed_mode=MIXED_MODES[args.mixed_rows])
         t_step, reps = step_loop(lambda: B.sparsify(k, betas, sketch_seeds=[0] * len(gs),
                                                     sampling_seeds=[0] * len(gs), solve=solve),
                                  args.warmup, args.steps, "batch")
@@ -581,7 +582,7 @@ def bench_batch(W, args):
             B.profile(False)
         kept = sum(r["kept"] for r in reps)
         kb = kernel_bytes(n, m, B, k, kept, s=8 if prec == ss.PREC_FP64 else 4,
-                          srow=2 if prec == ss.PREC_MIXED and k % 2 == 0 else None)
+                          srow=2 if prec == ss.PREC_MIXED and args.mixed_rows == "fp16" and k % 2 == 0 else None)
         kernels, roofline = summarize(prof, kb, t_prof)
         if roofline:
             add_traffic(roofline, measured_traffic(W["name"], args.precision, roofline["kernel"], k))
@@ -655,7 +656,9 @@ def probe(args):
         G.set_factor(ss.Factor(cp, ri, va, seed=0, ordering=ss.ORDER_ROUNDS))
         log("probe: factor uploaded")
         prec = ss.PREC_FP64 if (args.precision or W["precision"]) == "fp64" else ss.PREC_MIXED
-        solve = ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-8, max_iter=3, precision=prec)
+        rows = args.mixed_rows or W.get("rows", "fp32")
+        solve = ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-8, max_iter=3, precision=prec,
+                              mixed_mode=MIXED_MODES[rows])
         solve.fail_relres = 1e300
         if args.requests > 1:
             reqs = [dict(k=k, beta=graphs.beta_for(n, W["eps"]), sketch_seed=i, sampling_seed=i)
@@ -853,8 +856,9 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default=None,
                     help="served leg / batch precision (default: the workload's)")
     ap.add_argument("--requests", type=int, default=4, help="requests merged per served pass")
-    ap.add_argument("--mixed-rows", choices=sorted(MIXED_MODES), default="fp16",
-                    help="mixed precision: the preconditioner's intermediate rows in FP16 (default) or FP32")
+    ap.add_argument("--mixed-rows", choices=sorted(MIXED_MODES), default=None,
+                    help="mixed precision: the preconditioner's intermediate rows in FP16 or FP32 (default: FP16 "
+                         "for the merged Chung-Lu pass, FP32 elsewhere)")
     ap.add_argument("--col-sample", type=int, default=2, help="single-RHS solves timed for the column-wise plain leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel event timing")
@@ -892,6 +896,8 @@ def main():
     W = workload(args.workload)
     if args.precision is None:
         args.precision = W["precision"]
+    if args.mixed_rows is None:
+        args.mixed_rows = W.get("rows", "fp32")
     if "batch" in W:
         return bench_batch(W, args)
     return bench_single(W, args)

@@ -127,15 +127,17 @@ int ss_sketch(ss_context *ctx, const int32_t *v1, const int32_t *v2,
  * (reliable updates; the search directions are kept; max_iter bounds the
  * iterations); RESTART runs FP32 PCG sweeps, each on the FP64 residual of
  * the last, until the FP64 rule holds (iterative refinement; at most 16
- * sweeps of at most min(max_iter, 300) iterations each).  RELIABLE holds
- * the preconditioner's intermediate rows (the triangular solves' outputs,
- * which every factor entry gathers) in FP16, each column scaled to unit mean
- * square before the solve, with FP32 arithmetic; RELIABLE_F32 holds them in
- * FP32.  The preconditioner's precision changes the iteration count only:
- * both stop on the FP64 true residual. */
+ * sweeps of at most min(max_iter, 300) iterations each).  RELIABLE_F16 is
+ * RELIABLE with the preconditioner's intermediate rows (the triangular
+ * solves' outputs, which every factor entry gathers) stored in FP16, each
+ * column scaled to unit mean square before the solve (FP32 arithmetic; k
+ * even, else FP32 rows): fewer bytes per gather, which pays on wide merged
+ * requests over weights spanning little, at the cost of iterations
+ * elsewhere.  The preconditioner's precision changes the iteration count
+ * only: every scheme stops on the FP64 true residual. */
 #define SS_MIXED_RELIABLE 0
 #define SS_MIXED_RESTART 1
-#define SS_MIXED_RELIABLE_F32 2
+#define SS_MIXED_RELIABLE_F16 2
 
 typedef struct ss_solve_opts {
     uint64_t factor_seed;   /* seed of the randomized factor                 */
@@ -151,7 +153,7 @@ typedef struct ss_solve_opts {
                                (reliable; 0 = 3e-3) or asked of each FP32
                                PCG sweep (restart; 0 = 1e-3)                 */
     int32_t mixed_mode;     /* SS_PREC_MIXED: SS_MIXED_RELIABLE (default),
-                               SS_MIXED_RESTART or SS_MIXED_RELIABLE_F32     */
+                               SS_MIXED_RESTART or SS_MIXED_RELIABLE_F16     */
 } ss_solve_opts;
 
 typedef struct ss_solve_report {

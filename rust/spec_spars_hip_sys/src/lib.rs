@@ -20,7 +20,7 @@ pub const SS_ORDER_RANDOM: i32 = 1;
 pub const SS_ORDER_ROUNDS: i32 = 2;
 pub const SS_MIXED_RELIABLE: i32 = 0;
 pub const SS_MIXED_RESTART: i32 = 1;
-pub const SS_MIXED_RELIABLE_F32: i32 = 2;
+pub const SS_MIXED_RELIABLE_F16: i32 = 2;
 
 #[repr(C)]
 pub struct ss_context {

@@ -8,7 +8,7 @@ fails if it has not been built.
 from . import _lib
 from ._lib import (SSError, PREC_FP64, PREC_MIXED, STOP_REF_ABS, STOP_REL,  # noqa: F401
                    ORDER_NATURAL, ORDER_RANDOM, ORDER_ROUNDS, MIXED_RELIABLE, MIXED_RESTART,
-                   MIXED_RELIABLE_F32)
+                   MIXED_RELIABLE_F16)
 
 _lib.load()
 

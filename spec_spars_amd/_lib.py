@@ -20,7 +20,7 @@ PREC_FP64 = 0
 PREC_MIXED = 1
 MIXED_RELIABLE = 0
 MIXED_RESTART = 1
-MIXED_RELIABLE_F32 = 2   # reliable updates with FP32 preconditioner rows
+MIXED_RELIABLE_F16 = 2   # reliable updates with FP16 preconditioner rows
 STOP_REF_ABS = 0
 STOP_REL = 1
 ORDER_NATURAL = 0

@@ -43,10 +43,10 @@ FP16_ROWS_MAX_SPREAD = 1024.0
 
 
 def mixed_mode_for(w):
-    """Mixed-precision scheme for a graph with edge weights w: reliable
-    updates with FP16 preconditioner rows (MIXED_RELIABLE) when the weights
-    span at most FP16_ROWS_MAX_SPREAD, else with FP32 rows
-    (MIXED_RELIABLE_F32).  A wider spread spreads the preconditioned
+    """Mixed-precision scheme for wide merged requests on a graph with edge
+    weights w: reliable updates with FP16 preconditioner rows
+    (MIXED_RELIABLE_F16) when the weights span at most FP16_ROWS_MAX_SPREAD,
+    else with FP32 rows (MIXED_RELIABLE).  A wider spread spreads the preconditioned
     vector's entries over more binary orders than FP16's 11-bit significand
     resolves, and the PCG then stalls between residual replacements (a
     random geometric graph with weights 10^U(-3, 3) left half its columns
@@ -55,8 +55,8 @@ def mixed_mode_for(w):
     w = np.asarray(w, dtype=np.float64)
     w = w[w > 0]
     if len(w) == 0:
-        return _lib.MIXED_RELIABLE
-    return _lib.MIXED_RELIABLE if w.max() <= FP16_ROWS_MAX_SPREAD * w.min() else _lib.MIXED_RELIABLE_F32
+        return _lib.MIXED_RELIABLE_F16
+    return _lib.MIXED_RELIABLE_F16 if w.max() <= FP16_ROWS_MAX_SPREAD * w.min() else _lib.MIXED_RELIABLE
 
 
 def set_factor_threads(threads=0):

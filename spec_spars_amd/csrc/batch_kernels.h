@@ -123,8 +123,8 @@ void bt_accumulate(hipStream_t s, Tiles rt, int k, const float *d, double *x);
 // Mixed precision with reliable updates (per (graph, column)): bnorm from
 // rr_ref = |B64|^2; the recursive residual check flagging columns for
 // replacement; the fold of flagged columns' FP32 iterate x into X64
-// (X32 = x); R64 -= A x on the flagged columns; the FP64 stop test and new
-// reference; R32 = (float) R64 and X32 = 0 on the replaced columns.
+// (X32 = x); R64 -= A x and R32 = (float) R64 on the flagged columns; the
+// FP64 stop test and new reference; X32 = 0 on the replaced columns.
 // status = {active, err, flagged}.
 void bt_reliable_init(hipStream_t s, int gk, PcgScalars sc, double *rr_ref, int *flag, int *status);
 void bt_reliable_check(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, int rule, double tol,
@@ -138,9 +138,8 @@ void bt_reliable_fold(hipStream_t s, Tiles rt, int k, const int *flag, const dou
 void bt_reliable_apply(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, int rule, double tol,
                        PcgScalars sc, double *rr_ref, int *flag, int *keep, int *status);
 void bt_reliable_residual(hipStream_t s, Tiles rt, const int32_t *rowptr, const int32_t *col, const double *val,
-                          int k, const int *flag, const float *X, double *R, double *partial);
-void bt_reliable_replace(hipStream_t s, Tiles rt, int k, const double *src, const int *keep, float *dst,
-                         float *X32);
+                          int k, const int *flag, const float *X, double *R, float *R32, double *partial);
+void bt_reliable_replace(hipStream_t s, Tiles rt, int k, const int *keep, float *X32);
 void bt_apply_diag(hipStream_t s, int32_t n, const int32_t *rowptr, const int32_t *col, const double *val,
                    const int32_t *iptr, const int32_t *ieid, const double *ew, const double *p, const uint8_t *mask,
                    double *diag_new);

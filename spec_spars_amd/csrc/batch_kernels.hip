@@ -865,13 +865,15 @@ __global__ __launch_bounds__(1024) void kb_reliable_fold(Tiles rt, int k, const 
 // Replacement, step 2 (flagged columns): R64 -= A x with x the folded
 // FP32 iterate (R64 held b - A X64 before the fold, so it becomes the FP64
 // true residual of the new X64; the gathers read FP32 rows), with per-tile
-// partials of |R64|^2 (zero for the other columns).
+// partials of |R64|^2 (zero for the other columns); R32 = (float) R64 on
+// the same columns (a column the next step finds done keeps it unused:
+// it no longer iterates).
 __global__ __launch_bounds__(1024) void kb_reliable_residual(Tiles rt, const int32_t *__restrict__ rowptr,
                                                             const int32_t *__restrict__ col,
                                                             const double *__restrict__ val, int k,
                                                             const int *__restrict__ flag,
                                                             const float *__restrict__ X, double *__restrict__ R,
-                                                            double *__restrict__ partial) {
+                                                            float *__restrict__ R32, double *__restrict__ partial) {
     __shared__ double sm[kMaxTileWaves][64];
     const int t = blockIdx.x, g = rt.graph[t];
     const int r0 = rt.start[t], r1 = rt.start[t + 1];
@@ -901,6 +903,7 @@ __global__ __launch_bounds__(1024) void kb_reliable_residual(Tiles rt, const int
                 }
                 for (; q < q1; q++) v = bt_fma(-val[q], (double)X[(size_t)col[q] * k + j], v);
                 R[o] = v;
+                R32[o] = (float)v;
                 acc = bt_fma(v, v, acc);
             }
         }
@@ -917,7 +920,7 @@ __global__ __launch_bounds__(1024) void kb_reliable_residual4(Tiles rt, const in
                                                              const double *__restrict__ val, int k,
                                                              const int *__restrict__ flag,
                                                              const f32x4 *__restrict__ X, f64x4 *__restrict__ R,
-                                                             double *__restrict__ partial) {
+                                                             float *__restrict__ R32, double *__restrict__ partial) {
     __shared__ double sm[kMaxTileWaves][64];
     const int t = blockIdx.x, g = rt.graph[t];
     const int r0 = rt.start[t], r1 = rt.start[t + 1];
@@ -962,8 +965,12 @@ __global__ __launch_bounds__(1024) void kb_reliable_residual4(Tiles rt, const in
                 }
 #pragma unroll
                 for (int h = 0; h < 4; h++) {
-                    if (!onh[h]) v[h] = v0[h];
-                    else acc[h] = bt_fma(v[h], v[h], acc[h]);
+                    if (!onh[h]) {
+                        v[h] = v0[h];
+                    } else {
+                        acc[h] = bt_fma(v[h], v[h], acc[h]);
+                        R32[4 * o + h] = (float)v[h];
+                    }
                 }
                 R[o] = v;
             }
@@ -1010,23 +1017,17 @@ __global__ __launch_bounds__(64) void kb_reliable_apply(int k, const int32_t *gt
     }
 }
 
-// Replacement, step 4: X32 = 0 on the replaced columns, and R32 =
-// (float) R64 on those that go on (keep 1); the others untouched.
-__global__ __launch_bounds__(1024) void kb_reliable_replace(Tiles rt, int k, const double *__restrict__ src,
-                                                           const int *__restrict__ keep, float *__restrict__ dst,
+// Replacement, step 4: X32 = 0 on the replaced columns (their FP32
+// residual was replaced in step 2).
+__global__ __launch_bounds__(1024) void kb_reliable_replace(Tiles rt, int k, const int *__restrict__ keep,
                                                            float *__restrict__ X32) {
     const int t = blockIdx.x, g = rt.graph[t];
     const int r0 = rt.start[t], r1 = rt.start[t + 1];
     for (int jb = 0; jb < k; jb += 64) {
         const int j = jb + BT_LANE;
         if (j >= k) break;
-        const int kp = keep[(size_t)g * k + j];
-        if (!kp) continue;
-        for (int r = r0 + BT_WAVE; r < r1; r += BT_NW) {
-            const size_t o = (size_t)r * k + j;
-            if (kp == 1) dst[o] = (float)src[o];
-            X32[o] = 0.0f;
-        }
+        if (!keep[(size_t)g * k + j]) continue;
+        for (int r = r0 + BT_WAVE; r < r1; r += BT_NW) X32[(size_t)r * k + j] = 0.0f;
     }
 }
 
@@ -1260,7 +1261,7 @@ template void bt_update_x<double>(hipStream_t, Tiles, int, const double *, const
 template void bt_update_x<float>(hipStream_t, Tiles, int, const double *, const float *, float *);
 void bt_spmm_z(hipStream_t s, Tiles rt, const int32_t *rowptr, const int32_t *col, const float *val, int k,
                const _Float16 *Z, const float *zinv, const double *beta, const float *P, float *Q, double *partial) {
-    if (k % 4 == 0)
+    if (k % 4 == 0 && k >= 256)
         hipLaunchKernelGGL((kb_spmm_z<f16x4, f32x4>), dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, rowptr, col, val,
                            k, reinterpret_cast<const f16x4 *>(Z), zinv, beta, reinterpret_cast<const f32x4 *>(P),
                            reinterpret_cast<f32x4 *>(Q), partial);
@@ -1273,8 +1274,8 @@ template <typename T>
 void bt_spmm_dot(hipStream_t s, Tiles rt, const int32_t *rowptr, const int32_t *col, const T *val, int k, const T *P,
                  T *Q, double *partial) {
     if constexpr (sizeof(T) == 4) {
-        if (k % 2 == 0) {
-            // FP32 with k even: two columns per lane
+        if (k % 2 == 0 && k >= 128) {
+            // FP32 with k even and at least 128: two columns per lane
             hipLaunchKernelGGL((kb_spmm_dot<float, f32x2>), dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, rowptr,
                                col, val, k, reinterpret_cast<const f32x2 *>(P), reinterpret_cast<f32x2 *>(Q), partial);
             return;
@@ -1326,17 +1327,16 @@ void bt_reliable_apply(hipStream_t s, int gk, int k, const int32_t *gtile, const
                        status);
 }
 void bt_reliable_residual(hipStream_t s, Tiles rt, const int32_t *rowptr, const int32_t *col, const double *val,
-                          int k, const int *flag, const float *X, double *R, double *partial) {
-    if (k % 4 == 0)
+                          int k, const int *flag, const float *X, double *R, float *R32, double *partial) {
+    if (k % 4 == 0 && k >= 256)
         hipLaunchKernelGGL(kb_reliable_residual4, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, rowptr, col, val, k,
-                           flag, reinterpret_cast<const f32x4 *>(X), reinterpret_cast<f64x4 *>(R), partial);
+                           flag, reinterpret_cast<const f32x4 *>(X), reinterpret_cast<f64x4 *>(R), R32, partial);
     else
         hipLaunchKernelGGL(kb_reliable_residual, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, rowptr, col, val, k,
-                           flag, X, R, partial);
+                           flag, X, R, R32, partial);
 }
-void bt_reliable_replace(hipStream_t s, Tiles rt, int k, const double *src, const int *keep, float *dst,
-                         float *X32) {
-    hipLaunchKernelGGL(kb_reliable_replace, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, k, src, keep, dst, X32);
+void bt_reliable_replace(hipStream_t s, Tiles rt, int k, const int *keep, float *X32) {
+    hipLaunchKernelGGL(kb_reliable_replace, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, k, keep, X32);
 }
 // New diagonal of a sampled Laplacian, as Triplet::insert / process_diagonal
 // / to_csc and the sparse add build it (src/sparsifier.rs:48-87, :645-691):

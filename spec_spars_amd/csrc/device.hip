@@ -937,13 +937,15 @@ static int precond_v(ss_graph *g, int k, const VI *R, V *W, V *Zp, double *rz_pa
     SS_HIP(hipGetLastError());
     return SS_OK;
 }
-// FP64, or FP32 rows: two columns per lane where k is even (the same
-// arithmetic per column, half the gather instructions).
+// FP64, or FP32 rows: two columns per lane where k is even and at least
+// 128 (the same arithmetic per column, half the gather instructions; a
+// narrower row would leave most of a wave's lanes idle at twice the
+// registers).
 template <typename T>
 static int precond(ss_graph *g, int k, const T *R, T *W, T *Zp, double *rz_partial) {
     const RowScale none{nullptr, nullptr};
     if constexpr (std::is_same<T, float>::value) {
-        if (k % 2 == 0)
+        if (k % 2 == 0 && k >= 128)
             return precond_v<float, f32x2, f32x2>(g, k, reinterpret_cast<const f32x2 *>(R),
                                                   reinterpret_cast<f32x2 *>(W), reinterpret_cast<f32x2 *>(Zp),
                                                   rz_partial, none);
@@ -956,7 +958,7 @@ static int precond(ss_graph *g, int k, const T *R, T *W, T *Zp, double *rz_parti
 // (graph, column) by prec_sc.
 static RowScale prec_scale(const ss_graph *g) { return RowScale{g->prec_sc, g->row_graph}; }
 static int precond_half(ss_graph *g, int k, const PcgWork<float> &w) {
-    if (k % 4 == 0)   // four columns per lane: one 8-byte access
+    if (k % 4 == 0 && k >= 256)   // four columns per lane: one 8-byte access
         return precond_v<float, f32x4, f16x4>(g, k, reinterpret_cast<const f32x4 *>(w.R),
                                               reinterpret_cast<f16x4 *>(w.W), reinterpret_cast<f16x4 *>(w.Zp),
                                               g->partial2, prec_scale(g));
@@ -1201,14 +1203,14 @@ static int pcg_reliable(ss_graph *g, int k, int rule, double tol, double delta, 
             }
             {
                 ProfScope ps(g, SS_KERNEL_RESIDUAL);
-                bt_reliable_residual(s, stiles(g), g->rowptr, g->col, g->val, k, g->rel_flag, g->X32, g->R64,
+                bt_reliable_residual(s, stiles(g), g->rowptr, g->col, g->val, k, g->rel_flag, g->X32, g->R64, w.R,
                                      g->partial);
             }
             bt_reliable_apply(s, gk, k, g->sgtile, g->partial, rule, tol, g->sc, g->rel_ref, g->rel_flag,
                               g->colkeep, g->status);
             {
                 ProfScope ps(g, SS_KERNEL_VECTOR);
-                bt_reliable_replace(s, rt, k, g->R64, g->colkeep, w.R, g->X32);
+                bt_reliable_replace(s, rt, k, g->colkeep, g->X32);
             }
             SS_TRY(read_status(g, &nactive));
         }
@@ -1292,9 +1294,9 @@ int graph_solve(ss_graph *g, int k, const ss_solve_opts &o, SolveStats &st, cons
         SS_TRY(block_norms2(g, k, g->B64, bn2));
         if (!(o.inner_tol >= 0.0 && o.inner_tol < 1.0)) return fail(SS_ERR_INVALID, "inner_tol %g outside [0, 1)", o.inner_tol);
         if (o.mixed_mode != SS_MIXED_RELIABLE && o.mixed_mode != SS_MIXED_RESTART &&
-            o.mixed_mode != SS_MIXED_RELIABLE_F32)
+            o.mixed_mode != SS_MIXED_RELIABLE_F16)
             return fail(SS_ERR_INVALID, "unknown mixed_mode %d", o.mixed_mode);
-        if (o.mixed_mode == SS_MIXED_RELIABLE || o.mixed_mode == SS_MIXED_RELIABLE_F32) {
+        if (o.mixed_mode == SS_MIXED_RELIABLE || o.mixed_mode == SS_MIXED_RELIABLE_F16) {
             bool grouped = false;
             if (groups) {
                 if ((int)groups->size() != k) return fail(SS_ERR_INVALID, "replacement groups: %zu for k=%d", groups->size(), k);
@@ -1314,7 +1316,7 @@ int graph_solve(ss_graph *g, int k, const ss_solve_opts &o, SolveStats &st, cons
             bool all_done = false;
             SS_TRY(pcg_reliable(g, k, o.stop_rule, o.tol, o.inner_tol > 0.0 ? o.inner_tol : 3e-3, o.max_iter, bn2,
                                 st.iters, st.refinements, all_done, grouped,
-                                o.mixed_mode == SS_MIXED_RELIABLE && k % 2 == 0));
+                                o.mixed_mode == SS_MIXED_RELIABLE_F16 && k % 2 == 0));
             if (all_done) {
                 // every column stopped on its FP64 true residual, which the
                 // last replacement left in rel_ref (a zero right-hand side

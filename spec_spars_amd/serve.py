@@ -108,8 +108,9 @@ class Registry:
         ordering = ss.ORDER_ROUNDS if ordering is None else ordering
         self.specs = graphs
         self.ctxs, self.G, self.F = [], [], []
-        # per graph the mixed-precision scheme its weights allow
-        self.mixed_mode = [ss.mixed_mode_for(w) for _, _, _, w in graphs]
+        # the default mixed scheme on every graph (FP16 preconditioner rows
+        # pay on wide merged requests; the served requests here are narrow)
+        self.mixed_mode = [ss.MIXED_RELIABLE] * len(graphs)
         for n, u, v, w in graphs:
             ctx = ss.Context(0)
             self.ctxs.append(ctx)

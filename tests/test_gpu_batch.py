@@ -519,3 +519,35 @@ def test_sparsify_many_matches_alone(ctx, prec):
         assert np.array_equal(ku, many[r][0]) and np.array_equal(kv, many[r][1])
         assert np.array_equal(kw, many[r][2])
     G.close()
+
+
+def test_sparsify_many_fp16_rows_match_alone(ctx):
+    """FP16 preconditioner rows (MIXED_RELIABLE_F16), the served form's
+    scheme: four merged requests of 64 columns (one 256-column pass, the
+    solves and SpMM four columns per lane) give every request the kept
+    edges, iterations and replacement rounds it gets alone (64 columns, two
+    per lane): the lane width and the merged width change no column's
+    arithmetic.  The residuals meet the FP64 tolerance."""
+    import spec_spars_amd as ss
+    from spec_spars_amd import graphs
+    n, u, v = graphs.chung_lu(20000, 200000, gamma=2.2, seed=7)
+    w = graphs.uniform_weights(len(u), seed=7)
+    assert ss.mixed_mode_for(w) == ss.MIXED_RELIABLE_F16
+    G = ss.Graph(ctx, n, u, v, w)
+    F = ss.Factor(*G.laplacian(), seed=1, ordering=ss.ORDER_ROUNDS)
+    G.set_factor(F)
+    solve = ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-8, max_iter=2000, precision=ss.PREC_MIXED,
+                          mixed_mode=ss.MIXED_RELIABLE_F16)
+    reqs = [dict(k=64, beta=graphs.beta_for(n, e), sketch_seed=i, sampling_seed=i)
+            for i, e in enumerate([0.3, 0.2, 0.5, 0.3])]
+    reps = G.sparsify_many(reqs, solve=solve)
+    many = [G.read_kept_many(r) for r in range(len(reqs))]
+    for r, q in enumerate(reqs):
+        rep = G.sparsify(q["k"], q["beta"], sketch_seed=q["sketch_seed"], sampling_seed=q["sampling_seed"],
+                         solve=solve)
+        ku, kv, kw = G.read_kept()
+        assert rep["max_relres"] <= 1e-8 and reps[r]["max_relres"] <= 1e-8
+        assert rep["kept"] == reps[r]["kept"] == len(ku) and rep["max_iters"] == reps[r]["max_iters"]
+        assert np.array_equal(ku, many[r][0]) and np.array_equal(kv, many[r][1])
+        assert np.array_equal(kw, many[r][2])
+    G.close()

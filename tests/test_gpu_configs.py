@@ -119,8 +119,9 @@ def test_c2_grid128_fp64_at_size(ctx, oracle_mod):
 def test_c3_chunglu5m_mixed_at_size(ctx, oracle_mod):
     """Chung-Lu n = 5,000,000, m = 50,000,000 (exponent 2.2), weights
     U[0.5, 2], k = 64, FP32 SpMM / triangular solves with FP64 reductions
-    and refinement to rel 1e-8, eps = 0.3; the leverage scores against the
-    FP64 oracle's PCG on two columns."""
+    and refinement to rel 1e-8, eps = 0.3, the preconditioner's rows in FP16
+    (the served form's scheme); the leverage scores against the FP64
+    oracle's PCG on two columns."""
     import spec_spars_amd as ss
     from spec_spars_amd import graphs
     n, u, v = graphs.chung_lu(5_000_000, 50_000_000, gamma=2.2, seed=0)
@@ -135,7 +136,9 @@ def test_c3_chunglu5m_mixed_at_size(ctx, oracle_mod):
     G.set_factor(F)
     _say("c3 factor uploaded (tail %d)" % F.tail())
     beta = graphs.beta_for(n, 0.3)
-    sv = ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-8, max_iter=2000, precision=ss.PREC_MIXED)
+    sv = ss.solve_opts(stop_rule=ss.STOP_REL, tol=1e-8, max_iter=2000, precision=ss.PREC_MIXED,
+                       mixed_mode=ss.mixed_mode_for(w))
+    assert sv.mixed_mode == ss.MIXED_RELIABLE_F16
     rep = G.sparsify(k, beta, scale=1.0 / math.sqrt(k), sketch_seed=0, sampling_seed=0, solve=sv)
     _say("c3 sparsify %.0f ms, %d iterations, %d sweeps, relres %.2e, kept %d" %
          (rep["ms_total"], rep["max_iters"], rep["refinements"], rep["max_relres"], rep["kept"]))

@@ -126,9 +126,8 @@ def test_solve_mixed_precision(ctx, oracle_mod):
 
 @pytest.mark.parametrize("kind", ["grid3d", "rgg"])
 def test_mixed_reliable_and_restart_agree(ctx, oracle_mod, kind):
-    """Every mixed-precision scheme (reliable updates with FP16
-    preconditioner rows, the default; the same with FP32 rows; restarted
-    FP32 sweeps) reaches the FP64 tolerance and agrees with the FP64 oracle;
+    """Every mixed-precision scheme (reliable updates, the default; the
+    same with FP16 preconditioner rows; restarted FP32 sweeps) reaches the FP64 tolerance and agrees with the FP64 oracle;
     reliable updates keep the Krylov space, so they take no more iterations
     than the restarts, every inner_tol gives the same tolerance, and the
     FP16 rows cost at most two iterations over FP32 rows."""
@@ -139,7 +138,7 @@ def test_mixed_reliable_and_restart_agree(ctx, oracle_mod, kind):
     cp, ri, va, Zo, it = _oracle_solution(oracle_mod, n, u, v, w, Y, 3, 1e-12)
     out = {}
     for mode, inner in ((ss.MIXED_RELIABLE, 0.0), (ss.MIXED_RELIABLE, 0.1), (ss.MIXED_RELIABLE, 1e-4),
-                        (ss.MIXED_RESTART, 0.0), (ss.MIXED_RELIABLE_F32, 0.0)):
+                        (ss.MIXED_RESTART, 0.0), (ss.MIXED_RELIABLE_F16, 0.0)):
         Z, info = ctx.solve_lap(Y, cp, ri, va, ss.solve_opts(factor_seed=3, stop_rule=ss.STOP_REL, tol=1e-8,
                                                              precision=ss.PREC_MIXED, max_iter=2000,
                                                              inner_tol=inner, mixed_mode=mode))
@@ -150,8 +149,8 @@ def test_mixed_reliable_and_restart_agree(ctx, oracle_mod, kind):
         assert info["max_relres"] <= 1e-8
         assert _rel_centered(Z, Zo).max() < 1e-5
         out[(mode, inner)] = info["max_iters"]
-    assert out[(ss.MIXED_RELIABLE_F32, 0.0)] <= out[(ss.MIXED_RESTART, 0.0)], out
-    assert out[(ss.MIXED_RELIABLE, 0.0)] <= out[(ss.MIXED_RELIABLE_F32, 0.0)] + 2, out
+    assert out[(ss.MIXED_RELIABLE, 0.0)] <= out[(ss.MIXED_RESTART, 0.0)], out
+    assert out[(ss.MIXED_RELIABLE_F16, 0.0)] <= out[(ss.MIXED_RELIABLE, 0.0)] + 2, out
 
 
 @pytest.mark.parametrize("mode", [0, 1, 2])

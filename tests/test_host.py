@@ -267,8 +267,8 @@ def test_mixed_mode_for_weight_spread():
     grids' 10^U(-3, 3))."""
     import spec_spars_amd as ss
     from spec_spars_amd import graphs
-    assert ss.mixed_mode_for(graphs.uniform_weights(1000, seed=0)) == ss.MIXED_RELIABLE
-    assert ss.mixed_mode_for(np.ones(10)) == ss.MIXED_RELIABLE
-    assert ss.mixed_mode_for(graphs.loguniform_weights(1000, seed=0)) == ss.MIXED_RELIABLE_F32
-    assert ss.mixed_mode_for(np.array([1.0, 1024.0])) == ss.MIXED_RELIABLE
-    assert ss.mixed_mode_for(np.array([1.0, 1025.0])) == ss.MIXED_RELIABLE_F32
+    assert ss.mixed_mode_for(graphs.uniform_weights(1000, seed=0)) == ss.MIXED_RELIABLE_F16
+    assert ss.mixed_mode_for(np.ones(10)) == ss.MIXED_RELIABLE_F16
+    assert ss.mixed_mode_for(graphs.loguniform_weights(1000, seed=0)) == ss.MIXED_RELIABLE
+    assert ss.mixed_mode_for(np.array([1.0, 1024.0])) == ss.MIXED_RELIABLE_F16
+    assert ss.mixed_mode_for(np.array([1.0, 1025.0])) == ss.MIXED_RELIABLE
