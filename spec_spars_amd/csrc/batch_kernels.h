@@ -100,6 +100,12 @@ void bt_project(hipStream_t s, Tiles rt, int k, const TS *Y, const int32_t *comp
 template <typename T, typename TZ>
 void bt_update_px(hipStream_t s, Tiles rt, int k, const double *alpha, const double *beta, const TZ *Z,
                   const float *zinv, T *P, T *X);
+// The same for FP16 preconditioner rows (k even): Z scaled by 1 / zinv per
+// column, two or four columns per thread.
+void bt_update_px_h(hipStream_t s, Tiles rt, int k, const double *alpha, const double *beta, const _Float16 *Z,
+                    const float *zinv, float *P, float *X);
+// r . z = y^T D^+ y per (tile, column) of FP16 preconditioner rows (k even).
+void bt_tile_rz_h(hipStream_t s, Tiles rt, int k, const _Float16 *W, const double *diag, double *partial);
 // X += alpha P (the last iteration's solution update).
 template <typename T>
 void bt_update_x(hipStream_t s, Tiles rt, int k, const double *alpha, const T *P, T *X);

@@ -9,6 +9,8 @@
 // results are bit-identical alone and inside any batch.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "batch_kernels.h"
 #include "hash_dev.h"
 
@@ -75,6 +77,20 @@ __device__ __forceinline__ void bt_tile_store32(double acc, int k, int jb, doubl
             s += sm[u][lane];
             s += sm[u][lane + 32];
         }
+        partial[(size_t)blockIdx.x * k + j] = s;
+    }
+    __syncthreads();
+}
+
+// The partial of one column (lane value acc of column j) over the block's
+// waves, in wave order, into partial[blockIdx.x * k + j].
+__device__ __forceinline__ void bt_tile_store_col(double acc, int j, int k, double *partial, double (*sm)[64]) {
+    const int w = BT_WAVE, lane = BT_LANE;
+    sm[w][lane] = acc;
+    __syncthreads();
+    if (w == 0 && j < k) {
+        double s = sm[0][lane];
+        for (int u = 1; u < BT_NW; u++) s += sm[u][lane];
         partial[(size_t)blockIdx.x * k + j] = s;
     }
     __syncthreads();
@@ -388,6 +404,107 @@ __global__ __launch_bounds__(1024) void kb_update_px(Tiles rt, int k, const doub
     }
 }
 
+// The same for FP16 preconditioner rows (Z scaled by 1 / zinv per column),
+// NC = 2 or 4 columns per thread (one 4- or 8-byte load of Z, 8 or 16 bytes
+// of P and X): thread i of the tile's block takes element i % ke of rows
+// i / ke, i / ke + RP, ... (ke = k / NC elements per row).  Every column's
+// arithmetic is kb_update_px's.
+template <typename VZ, typename LV>
+__global__ __launch_bounds__(1024) void kb_update_px_h(Tiles rt, int k, const double *__restrict__ alpha,
+                                                      const double *__restrict__ beta, const VZ *__restrict__ Z,
+                                                      const float *__restrict__ zinv, LV *__restrict__ P,
+                                                      LV *__restrict__ X) {
+    constexpr int NC = sizeof(LV) / 4;
+    const int t = blockIdx.x, g = rt.graph[t];
+    const int r0 = rt.start[t], r1 = rt.start[t + 1];
+    const int ke = k / NC;
+    const int RP = (int)blockDim.x / ke, i = (int)threadIdx.x;
+    if (i >= RP * ke) return;
+    const int e = i % ke, ro = i / ke;
+    const size_t gc = (size_t)g * k + NC * e;
+    LV a, b, zs;
+    bool any_a = false, any_ab = false;
+#pragma unroll
+    for (int h = 0; h < NC; h++) {
+        a[h] = (float)alpha[gc + h];
+        b[h] = (float)beta[gc + h];
+        zs[h] = zinv[gc + h];
+        any_a = any_a || a[h] != 0.0f;
+        any_ab = any_ab || a[h] != 0.0f || b[h] != 0.0f;
+    }
+    for (int rb = r0 + ro; rb < r1; rb += kBtRows * RP) {
+        LV p[kBtRows], x[kBtRows], z[kBtRows];
+#pragma unroll
+        for (int u = 0; u < kBtRows; u++) {
+            const int r = rb + u * RP;
+            const bool on = r < r1;
+            const size_t o = (size_t)r * ke + e;
+            p[u] = on && any_ab ? P[o] : LV();
+            z[u] = on ? __builtin_convertvector(Z[o], LV) : LV();
+            x[u] = on && any_a ? X[o] : LV();
+        }
+#pragma unroll
+        for (int u = 0; u < kBtRows; u++) {
+            const int r = rb + u * RP;
+            if (r >= r1) break;
+            const size_t o = (size_t)r * ke + e;
+            LV xn = x[u], pn;
+#pragma unroll
+            for (int h = 0; h < NC; h++) {
+                const float zh = __builtin_isfinite(z[u][h]) ? z[u][h] * zs[h] : 0.0f;
+                if (a[h] != 0.0f) xn[h] = __builtin_fmaf(a[h], p[u][h], x[u][h]);
+                pn[h] = b[h] == 0.0f ? zh : __builtin_fmaf(b[h], p[u][h], zh);
+            }
+            if (any_a) X[o] = xn;
+            P[o] = pn;
+        }
+    }
+}
+
+// y^T D^+ y per (tile, column) for FP16 preconditioner rows y (the r . z
+// partials of the FP16-row path; D == 0 skipped as in kb_tile_partial's op
+// 2): NC = 2 or 4 columns per lane (one 4- or 8-byte load), wave w of the
+// block taking rows r0 + w, r0 + w + NW, ... (kBtRows of them in flight),
+// each column's terms added in row order within a wave and the waves added
+// in order: an order fixed by the tile alone, whatever k and NC are.
+template <typename VZ, typename LV>
+__global__ __launch_bounds__(1024) void kb_tile_rz_h(Tiles rt, int k, const VZ *__restrict__ W,
+                                                    const double *__restrict__ diag, double *__restrict__ partial) {
+    constexpr int NC = sizeof(LV) / 4;
+    __shared__ double sm[kMaxTileWaves][64];
+    const int t = blockIdx.x, NW = BT_NW;
+    const int r0 = rt.start[t], r1 = rt.start[t + 1];
+    const int ke = k / NC;
+    for (int eb = 0; eb < ke; eb += 64) {
+        const int e = eb + BT_LANE, ee = e < ke ? e : ke - 1;
+        double acc[NC];
+#pragma unroll
+        for (int h = 0; h < NC; h++) acc[h] = 0.0;
+        for (int rb = r0 + BT_WAVE; rb < r1; rb += kBtRows * NW) {
+            LV y[kBtRows];
+            double d[kBtRows];
+#pragma unroll
+            for (int u = 0; u < kBtRows; u++) {
+                const int r = rb + u * NW;
+                const bool on = r < r1;
+                y[u] = on ? __builtin_convertvector(W[(size_t)r * ke + ee], LV) : LV();
+                d[u] = on ? diag[r] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < kBtRows; u++) {
+                if (rb + u * NW >= r1) break;
+#pragma unroll
+                for (int h = 0; h < NC; h++) {
+                    const double a = (double)y[u][h];
+                    acc[h] = d[u] != 0.0 ? bt_fma(a, a / d[u], acc[h]) : bt_fma(a, a, acc[h]);
+                }
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < NC; h++) bt_tile_store_col(acc[h], NC * e + h, k, partial, sm);
+    }
+}
+
 // X += alpha P: the last iteration's solution update.
 template <typename T>
 __global__ __launch_bounds__(1024) void kb_update_x(Tiles rt, int k, const double *__restrict__ alpha,
@@ -427,19 +544,6 @@ __device__ __forceinline__ double bt_lane(double v, int l) {
 // the grouping does not change a bit.
 constexpr int kSpmmGroup = 4;
 constexpr int kSpmmShort = 8;
-// The partial of one column (lane value acc of column j) over the block's
-// waves, in wave order, into partial[blockIdx.x * k + j].
-__device__ __forceinline__ void bt_tile_store_col(double acc, int j, int k, double *partial, double (*sm)[64]) {
-    const int w = BT_WAVE, lane = BT_LANE;
-    sm[w][lane] = acc;
-    __syncthreads();
-    if (w == 0 && j < k) {
-        double s = sm[0][lane];
-        for (int u = 1; u < BT_NW; u++) s += sm[u][lane];
-        partial[(size_t)blockIdx.x * k + j] = s;
-    }
-    __syncthreads();
-}
 
 // Lane values of the SpMM: one column (T), or two adjacent FP32 columns
 // (f32x2, k even) moved by one 8-byte access.
@@ -1253,6 +1357,28 @@ template void bt_update_px<float, float>(hipStream_t, Tiles, int, const double *
                                          const float *, float *, float *);
 template void bt_update_px<float, _Float16>(hipStream_t, Tiles, int, const double *, const double *,
                                             const _Float16 *, const float *, float *, float *);
+void bt_update_px_h(hipStream_t s, Tiles rt, int k, const double *alpha, const double *beta, const _Float16 *Z,
+                    const float *zinv, float *P, float *X) {
+    if (k % 4 == 0 && k >= 256) {
+        const int ke = k / 4, th = std::max(64 * rt.waves, (ke + 63) / 64 * 64);
+        hipLaunchKernelGGL((kb_update_px_h<f16x4, f32x4>), dim3(rt.count), dim3(th), 0, s, rt, k, alpha, beta,
+                           reinterpret_cast<const f16x4 *>(Z), zinv, reinterpret_cast<f32x4 *>(P),
+                           reinterpret_cast<f32x4 *>(X));
+    } else {
+        const int ke = k / 2, th = std::max(64 * rt.waves, (ke + 63) / 64 * 64);
+        hipLaunchKernelGGL((kb_update_px_h<f16x2, f32x2>), dim3(rt.count), dim3(th), 0, s, rt, k, alpha, beta,
+                           reinterpret_cast<const f16x2 *>(Z), zinv, reinterpret_cast<f32x2 *>(P),
+                           reinterpret_cast<f32x2 *>(X));
+    }
+}
+void bt_tile_rz_h(hipStream_t s, Tiles rt, int k, const _Float16 *W, const double *diag, double *partial) {
+    if (k % 4 == 0 && k >= 256)
+        hipLaunchKernelGGL((kb_tile_rz_h<f16x4, f32x4>), dim3(rt.count), dim3(64 * kMaxTileWaves), 0, s, rt, k,
+                           reinterpret_cast<const f16x4 *>(W), diag, partial);
+    else
+        hipLaunchKernelGGL((kb_tile_rz_h<f16x2, f32x2>), dim3(rt.count), dim3(64 * kMaxTileWaves), 0, s, rt, k,
+                           reinterpret_cast<const f16x2 *>(W), diag, partial);
+}
 template <typename T>
 void bt_update_x(hipStream_t s, Tiles rt, int k, const double *alpha, const T *P, T *X) {
     hipLaunchKernelGGL(kb_update_x<T>, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, k, alpha, P, X);

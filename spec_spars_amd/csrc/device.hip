@@ -930,7 +930,10 @@ static int precond_v(ss_graph *g, int k, const VI *R, V *W, V *Zp, double *rz_pa
     {
         ProfScope ps(g, SS_KERNEL_VECTOR);
         const S *ws = reinterpret_cast<const S *>(W);
-        bt_tile_partial<S, S>(g->ctx->stream, rtiles(g), k, ws, ws, g->diag, 2, rz_partial);
+        if constexpr (std::is_same<S, _Float16>::value)
+            bt_tile_rz_h(g->ctx->stream, rtiles(g), k, ws, g->diag, rz_partial);
+        else
+            bt_tile_partial<S, S>(g->ctx->stream, rtiles(g), k, ws, ws, g->diag, 2, rz_partial);
     }
     solve_cores<T, V, V, true>(g, k, W, Zp, rs);
     solve_levels<T, V, V, true>(g, k, W, Zp, rs);
@@ -1006,8 +1009,8 @@ static int pcg_iteration(ss_graph *g, int k, T *X, const PcgWork<T> &w, int rule
         // P = Z + beta P; X += alpha P of the previous iteration
         if constexpr (std::is_same<T, float>::value) {
             if (half)
-                bt_update_px<float, _Float16>(s, rt, k, g->sc.alpha, g->sc.beta,
-                                              reinterpret_cast<const _Float16 *>(w.Zp), g->prec_inv, w.P, X);
+                bt_update_px_h(s, rt, k, g->sc.alpha, g->sc.beta, reinterpret_cast<const _Float16 *>(w.Zp),
+                               g->prec_inv, w.P, X);
             else
                 bt_update_px<T, T>(s, rt, k, g->sc.alpha, g->sc.beta, w.Zp, nullptr, w.P, X);
         } else {
