@@ -133,10 +133,13 @@ void bt_accumulate(hipStream_t s, Tiles rt, int k, const float *d, double *x);
 // FP64 stop test and new reference; X32 = 0 on the replaced columns.
 // status = {active, err, flagged}.
 void bt_reliable_init(hipStream_t s, int gk, PcgScalars sc, double *rr_ref, int *flag, int *status);
+// defer: a column at its stop target waits for its replacement group
+// (bt_reliable_group must follow).
 void bt_reliable_check(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, int rule, double tol,
-                       double delta2, PcgScalars sc, const double *rr_ref, int *flag, int *status);
+                       double delta2, bool defer, PcgScalars sc, const double *rr_ref, int *flag, int *status);
 // Flag every active column of a group (columns [gs[j], ge[j]) of its
-// graph) that has a flagged column.
+// graph) that has a flagged column or whose active columns are all at
+// their stop target; unflag the other stop candidates.
 void bt_reliable_group(hipStream_t s, int gk, int k, const int32_t *gs, const int32_t *ge, PcgScalars sc, int *flag,
                        int *status);
 void bt_reliable_fold(hipStream_t s, Tiles rt, int k, const int *flag, const double *alpha, const float *P,

@@ -902,12 +902,18 @@ __global__ __launch_bounds__(256) void kb_reliable_init(int gk, PcgScalars s, do
     flag[gj] = 0;
 }
 
+// A column that reached its stop target while others of its replacement
+// group have not; it waits for them (kb_reliable_group).
+constexpr int kStopCandidate = 4;
+
 // The recursive residual's |R|^2 from the tiles; an active column is
-// flagged when it reached rr_ref * delta2 or the stop target.  Counts the
-// active (status[0], cleared by kb_pcg_alpha) and flagged (status[2])
-// columns.
+// flagged when it reached rr_ref * delta2 or the stop target.  With defer
+// (replacement groups), a column at its stop target alone is only marked
+// kStopCandidate: its group verifies its stop when every active column of
+// it is there, or at its next replacement.  Counts the active (status[0],
+// cleared by kb_pcg_alpha) and flagged (status[2]) columns.
 __global__ __launch_bounds__(64) void kb_reliable_check(int k, const int32_t *gtile, const double *p_rr, int rule,
-                                                        double tol, double delta2, PcgScalars s,
+                                                        double tol, double delta2, int defer, PcgScalars s,
                                                         const double *rr_ref, int *flag, int *status) {
     const int gj = blockIdx.x, g = gj / k, j = gj - g * k;
     const double rr = bt_graph_sum(p_rr, gtile, g, k, j);
@@ -916,33 +922,50 @@ __global__ __launch_bounds__(64) void kb_reliable_check(int k, const int32_t *gt
         if (s.active[gj]) {
             s.iters[gj] += 1;
             const bool target = rule == 0 ? rr <= tol : sqrt(rr) <= tol * s.bnorm[gj];
-            if (flag[gj] || target || rr <= delta2 * rr_ref[gj]) {
+            if (flag[gj] || rr <= delta2 * rr_ref[gj] || (target && !defer)) {
                 if (!flag[gj]) flag[gj] = 1;
                 atomicAdd(status + 2, 1);
+            } else if (target) {
+                flag[gj] = kStopCandidate;
             }
             atomicAdd(status, 1);
         }
     }
 }
 
-// Replacement groups: an active column whose group (columns [gs[j],
-// ge[j]) of its graph, the columns of one request) has a flagged column is
-// flagged too, so a request's columns replace together.  A flag seen here
-// always goes back to a column flagged by its own residual, so the outcome
-// does not depend on the order the threads run in.
+// Replacement groups (columns [gs[j], ge[j]) of a graph, the columns of
+// one request): a group replaces when one of its active columns is flagged
+// (drift, or a non-positive p . A p) or when every active column of it is
+// at its stop target; its other active columns are then flagged too, so a
+// request's columns replace together.  A stop candidate of a group that
+// does not replace goes back to unflagged and iterates on.  The decision is
+// a function of the flags kb_reliable_check left, and a flag rewritten here
+// is rewritten the way that decision goes (1 when the group replaces, 0 when
+// not), so it does not depend on the order the threads run in.
 __global__ __launch_bounds__(256) void kb_reliable_group(int gk, int k, const int32_t *__restrict__ gs,
                                                         const int32_t *__restrict__ ge, PcgScalars s, int *flag,
                                                         int *status) {
     const int gj = blockIdx.x * blockDim.x + threadIdx.x;
     if (gj >= gk) return;
     const int g = gj / k, j = gj - g * k;
-    if (!s.active[gj] || flag[gj]) return;
-    for (int c = gs[j]; c < ge[j]; c++)
-        if (__hip_atomic_load(&flag[g * k + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-            __hip_atomic_store(&flag[gj], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            atomicAdd(status + 2, 1);
-            return;
+    const int own = flag[gj];
+    if (!s.active[gj] || (own & 3)) return;
+    bool any = false, all = true;
+    for (int c = gs[j]; c < ge[j]; c++) {
+        if (!s.active[g * k + c]) continue;
+        const int f = __hip_atomic_load(&flag[g * k + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (f & 3) {
+            any = true;
+            break;
         }
+        if (f == 0) all = false;
+    }
+    if (any || all) {
+        __hip_atomic_store(&flag[gj], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        atomicAdd(status + 2, 1);
+    } else if (own) {
+        __hip_atomic_store(&flag[gj], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 // Replacement, step 1 (flagged columns): x = X32 + alpha P (the FP32
@@ -1435,9 +1458,9 @@ void bt_reliable_init(hipStream_t s, int gk, PcgScalars sc, double *rr_ref, int 
     hipLaunchKernelGGL(kb_reliable_init, dim3(bt_grid(gk, 256)), dim3(256), 0, s, gk, sc, rr_ref, flag, status);
 }
 void bt_reliable_check(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, int rule, double tol,
-                       double delta2, PcgScalars sc, const double *rr_ref, int *flag, int *status) {
-    hipLaunchKernelGGL(kb_reliable_check, dim3(gk), dim3(64), 0, s, k, gtile, p, rule, tol, delta2, sc, rr_ref, flag,
-                       status);
+                       double delta2, bool defer, PcgScalars sc, const double *rr_ref, int *flag, int *status) {
+    hipLaunchKernelGGL(kb_reliable_check, dim3(gk), dim3(64), 0, s, k, gtile, p, rule, tol, delta2, defer ? 1 : 0, sc,
+                       rr_ref, flag, status);
 }
 void bt_reliable_group(hipStream_t s, int gk, int k, const int32_t *gs, const int32_t *ge, PcgScalars sc, int *flag,
                        int *status) {

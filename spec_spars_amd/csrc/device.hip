@@ -997,10 +997,11 @@ static void component_means(ss_graph *g, int k, const double *Y) {
 // test.  With tail, also the next preconditioner application and beta
 // (what the next iteration needs).  The solve ends with the last X update.
 // With delta2 >= 0 (reliable updates), the check flags columns for a
-// residual replacement instead of stopping them.
+// residual replacement instead of stopping them (defer: a column at its
+// stop target waits for its replacement group).
 template <typename T>
 static int pcg_iteration(ss_graph *g, int k, T *X, const PcgWork<T> &w, int rule, double tol, bool tail,
-                         double delta2 = -1.0, bool half = false) {
+                         double delta2 = -1.0, bool half = false, bool defer = false) {
     hipStream_t s = g->ctx->stream;
     const Tiles rt = rtiles(g);
     const int gk = g->G * k;
@@ -1036,8 +1037,8 @@ static int pcg_iteration(ss_graph *g, int k, T *X, const PcgWork<T> &w, int rule
         bt_update_r_rr<T>(s, rt, k, g->sc.alpha, w.Q, w.R, g->partial);
     }
     if (delta2 >= 0.0)
-        bt_reliable_check(s, gk, k, g->gtile, g->partial, rule, tol, delta2, g->sc, g->rel_ref, g->rel_flag,
-                          g->status);
+        bt_reliable_check(s, gk, k, g->gtile, g->partial, rule, tol, delta2, defer, g->sc, g->rel_ref,
+                          g->rel_flag, g->status);
     else
         bt_pcg_check(s, gk, k, g->gtile, g->partial, rule, tol, g->sc, g->status);
     if (tail) {
@@ -1157,8 +1158,12 @@ static int pcg_run(ss_graph *g, int k, const T *rhs, const double *mean_src, T *
 // replacement gathers FP32 rows), keeping its search direction unless the
 // true residual jumped above the recursive one; it stops when that true
 // residual meets the stop rule.  Unlike restarting the FP32 PCG after every
-// refinement, the Krylov space built so far is kept.  rounds = replacement
-// rounds (launches of the FP64 residual).
+// refinement, the Krylov space built so far is kept.  With replacement
+// groups (grouped), a group's columns replace together, and a column at its
+// stop target waits (iterating on) until every active column of its group
+// is there or the group replaces for drift: one verifying round per group
+// instead of one per iteration in which some of its columns arrive.
+// rounds = replacement rounds (launches of the FP64 residual).
 // SPEC_SPARS_VERBOSE >= 2: one line per replacement round on stderr
 static const bool g_trace_rounds = [] {
     const char *v = std::getenv("SPEC_SPARS_VERBOSE");
@@ -1193,7 +1198,7 @@ static int pcg_reliable(ss_graph *g, int k, int rule, double tol, double delta, 
     rounds = 0;
     all_done = false;
     for (int it = 0; it < max_iter; it++) {
-        SS_TRY(pcg_iteration<float>(g, k, g->X32, w, rule, tol, false, delta2, half));
+        SS_TRY(pcg_iteration<float>(g, k, g->X32, w, rule, tol, false, delta2, half, grouped));
         if (grouped) bt_reliable_group(s, gk, k, g->rel_gs, g->rel_ge, g->sc, g->rel_flag, g->status);
         int nactive = 0, nflag = 0;
         SS_TRY(read_status(g, &nactive, &nflag));
