@@ -66,8 +66,12 @@ struct SketchCols {
     const double *scale;
     const int32_t *j;
 };
+// S (may be null; Rademacher only): scratch of ceil(k / 64) words per edge
+// of the edge tiles et (edge_off: per graph, its first global edge) for the
+// signs, hashed once per edge instead of once per incidence.
 void bt_sketch(hipStream_t s, Tiles rt, const int32_t *iptr, const int32_t *ieid, const double *ib, int k,
-               SketchCols cols, int uniform, double *Y);
+               SketchCols cols, int uniform, double *Y, Tiles et = Tiles{}, const int32_t *edge_off = nullptr,
+               uint64_t *S = nullptr);
 template <typename TA, typename TB>
 void bt_tile_partial(hipStream_t s, Tiles rt, int k, const TA *A, const TB *B, const double *diag, int op,
                      double *partial);
@@ -128,10 +132,9 @@ void bt_convert_masked(hipStream_t s, Tiles rt, int k, const double *src, const 
 void bt_accumulate(hipStream_t s, Tiles rt, int k, const float *d, double *x);
 // Mixed precision with reliable updates (per (graph, column)): bnorm from
 // rr_ref = |B64|^2; the recursive residual check flagging columns for
-// replacement; the fold of flagged columns' FP32 iterate x into X64
-// (X32 = x); R64 -= A x and R32 = (float) R64 on the flagged columns; the
-// FP64 stop test and new reference; X32 = 0 on the replaced columns.
-// status = {active, err, flagged}.
+// replacement; the fold of flagged columns' FP32 iterate x into X64 (XF =
+// x, X32 = 0); R64 -= A XF and R32 = (float) R64 on the flagged columns;
+// the FP64 stop test and new reference.  status = {active, err, flagged}.
 void bt_reliable_init(hipStream_t s, int gk, PcgScalars sc, double *rr_ref, int *flag, int *status);
 // defer: a column at its stop target waits for its replacement group
 // (bt_reliable_group must follow).
@@ -143,12 +146,11 @@ void bt_reliable_check(hipStream_t s, int gk, int k, const int32_t *gtile, const
 void bt_reliable_group(hipStream_t s, int gk, int k, const int32_t *gs, const int32_t *ge, PcgScalars sc, int *flag,
                        int *status);
 void bt_reliable_fold(hipStream_t s, Tiles rt, int k, const int *flag, const double *alpha, const float *P,
-                      float *X32, double *X64);
+                      float *X32, double *X64, float *XF);
 void bt_reliable_apply(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, int rule, double tol,
-                       PcgScalars sc, double *rr_ref, int *flag, int *keep, int *status);
+                       PcgScalars sc, double *rr_ref, int *flag, int *status);
 void bt_reliable_residual(hipStream_t s, Tiles rt, const int32_t *rowptr, const int32_t *col, const double *val,
                           int k, const int *flag, const float *X, double *R, float *R32, double *partial);
-void bt_reliable_replace(hipStream_t s, Tiles rt, int k, const int *keep, float *X32);
 void bt_apply_diag(hipStream_t s, int32_t n, const int32_t *rowptr, const int32_t *col, const double *val,
                    const int32_t *iptr, const int32_t *ieid, const double *ew, const double *p, const uint8_t *mask,
                    double *diag_new);

@@ -100,10 +100,45 @@ __device__ __forceinline__ void bt_tile_store_col(double acc, int j, int k, doub
 // (a) sketch: block per row tile, wave per row, seed/scale of the tile's graph
 // ---------------------------------------------------------------------------
 constexpr int kSketchSplit = 128;   // blocks per row tile in the sketch
+
+// The Rademacher signs once per edge (the sketch visits every edge from both
+// ends): bit l of S[E * nw + c] (nw = ceil(k / 64)) is 1 where column 64 c +
+// l of global edge E (graph-local index E - edge_off[g], the hash input) is
+// +1.  Block per edge tile, wave per edge, lane per column.
+__global__ __launch_bounds__(256) void kb_sketch_signs(Tiles et, const int32_t *__restrict__ edge_off, int k,
+                                                       SketchCols cols, int variant, uint64_t *__restrict__ S) {
+    const int t = blockIdx.x, g = et.graph[t];
+    const int e0 = et.start[t], e1 = et.start[t + 1];
+    const int nw = (k + 63) / 64;
+    const int eo = edge_off[g];
+    for (int jb = 0; jb < k; jb += 64) {
+        const int j = jb + BT_LANE, jc = g * k + (j < k ? j : k - 1);
+        const uint64_t seed = cols.seed[jc];
+        const uint64_t jh = (uint64_t)cols.j[jc];
+        for (int e = e0 + BT_WAVE; e < e1; e += BT_NW) {
+            const uint32_t x = (uint32_t)sign_hash((uint64_t)(e - eo), jh, seed, variant);
+            const uint64_t b = __ballot((x & 0x80000000u) != 0u);
+            if (BT_LANE == 0) S[(size_t)e * nw + jb / 64] = b;
+        }
+    }
+}
+
+// A lane's double from lane t (t wave-uniform).
+__device__ __forceinline__ double bt_readlane_f64(double v, int t) {
+    const uint64_t u = __double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, t);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), t);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// With SIGNS, the Rademacher signs come from kb_sketch_signs' bits (the
+// same values: +-1 times the scale, exactly) instead of a hash per incidence.
+template <bool SIGNS>
 __global__ __launch_bounds__(1024) void kb_sketch(Tiles rt, const int32_t *__restrict__ iptr,
                                                  const int32_t *__restrict__ ieid, const double *__restrict__ ib,
                                                  int k, SketchCols cols, int uniform, int variant,
-                                                 double *__restrict__ Y) {
+                                                 const int32_t *__restrict__ edge_off,
+                                                 const uint64_t *__restrict__ S, double *__restrict__ Y) {
 #pragma clang fp contract(off)
     // kSketchSplit blocks per row tile, one row per wave: a row's cost is
     // its degree, and the rows of a power-law graph's hubs often sit
@@ -113,6 +148,7 @@ __global__ __launch_bounds__(1024) void kb_sketch(Tiles rt, const int32_t *__res
     const int per = (rt.start[t + 1] - rt.start[t] + kSketchSplit - 1) / kSketchSplit;
     const int r0 = rt.start[t] + sb * per;
     const int r1 = r0 + per < rt.start[t + 1] ? r0 + per : rt.start[t + 1];
+    const int nw = (k + 63) / 64;
     for (int row = r0 + BT_WAVE; row < r1; row += BT_NW) {
         const int r = bt_uniform(row);
         const int q0 = iptr[r], q1 = iptr[r + 1];
@@ -123,10 +159,35 @@ __global__ __launch_bounds__(1024) void kb_sketch(Tiles rt, const int32_t *__res
             const double scale = cols.scale[jc];
             const uint64_t jh = (uint64_t)cols.j[jc];
             double acc = 0.0;
-            for (int q = q0; q < q1; q++) {
-                const double sv = sign_value((uint64_t)ieid[q], jh, seed, uniform, variant) * scale;
-                const double term = sv * ib[q];
-                acc = acc + term;
+            if constexpr (SIGNS) {
+                // 64 incidences at a time, one per lane (their sign words
+                // loaded together), then summed in incidence order with
+                // each one's word and value broadcast from its lane
+                const uint64_t *Sg = S + (size_t)edge_off[g] * nw + jb / 64;
+                for (int qb = q0; qb < q1; qb += 64) {
+                    const int qq = qb + BT_LANE, cnt = q1 - qb < 64 ? q1 - qb : 64;
+                    uint64_t wl = 0;
+                    double bl = 0.0;
+                    if (qq < q1) {
+                        wl = Sg[(size_t)ieid[qq] * nw];
+                        bl = ib[qq];
+                    }
+                    const uint32_t wlo = (uint32_t)wl, whi = (uint32_t)(wl >> 32);
+                    for (int t = 0; t < cnt; t++) {
+                        const uint32_t half = BT_LANE < 32 ? (uint32_t)__builtin_amdgcn_readlane((int)wlo, t)
+                                                           : (uint32_t)__builtin_amdgcn_readlane((int)whi, t);
+                        const double b = bt_readlane_f64(bl, t);
+                        const double sv = ((half >> (BT_LANE & 31)) & 1u) ? scale : -scale;
+                        const double term = sv * b;
+                        acc = acc + term;
+                    }
+                }
+            } else {
+                for (int q = q0; q < q1; q++) {
+                    const double sv = sign_value((uint64_t)ieid[q], jh, seed, uniform, variant) * scale;
+                    const double term = sv * ib[q];
+                    acc = acc + term;
+                }
             }
             if (j < k) Y[(size_t)r * k + j] = acc;
         }
@@ -969,10 +1030,13 @@ __global__ __launch_bounds__(256) void kb_reliable_group(int gk, int k, const in
 }
 
 // Replacement, step 1 (flagged columns): x = X32 + alpha P (the FP32
-// iterate with the iteration's pending update), X64 += x, X32 = x.
+// iterate with the iteration's pending update), X64 += x, XF = x (what the
+// FP64 residual gathers) and X32 = 0 (every flagged column's FP32 iterate
+// starts again: its residual is replaced, or it stops).
 __global__ __launch_bounds__(1024) void kb_reliable_fold(Tiles rt, int k, const int *__restrict__ flag,
                                                         const double *__restrict__ alpha, const float *__restrict__ P,
-                                                        float *__restrict__ X32, double *__restrict__ X64) {
+                                                        float *__restrict__ X32, double *__restrict__ X64,
+                                                        float *__restrict__ XF) {
     const int t = blockIdx.x, g = rt.graph[t];
     const int r0 = rt.start[t], r1 = rt.start[t + 1];
     for (int jb = 0; jb < k; jb += 64) {
@@ -984,7 +1048,8 @@ __global__ __launch_bounds__(1024) void kb_reliable_fold(Tiles rt, int k, const 
             const size_t o = (size_t)r * k + j;
             const float x = a != 0.0f ? bt_fma(a, P[o], X32[o]) : X32[o];
             X64[o] += (double)x;
-            X32[o] = x;
+            XF[o] = x;
+            X32[o] = 0.0f;
         }
     }
 }
@@ -1109,9 +1174,8 @@ __global__ __launch_bounds__(1024) void kb_reliable_residual4(Tiles rt, const in
 
 // Replacement, step 3, with |B64 - A X64|^2 of the flagged columns from the
 // SpMM tiles: a flagged column that meets the FP64 stop rule is done; the
-// others take it as their reference (keep = 1: their FP32 residual is
-// replaced; keep = 2 marks the done ones, whose FP32 iterate is cleared
-// too).  Where the true residual exceeds the recursive one by more
+// others take it as their reference (their FP32 residual was replaced in
+// step 2).  Where the true residual exceeds the recursive one by more
 // than kReliableJump (the FP32 recursion ran below what its arithmetic
 // resolves) or p . A p broke down, the column's next search direction is
 // its preconditioned residual alone (rz = 0 makes beta zero).  Flagged
@@ -1120,41 +1184,24 @@ __global__ __launch_bounds__(1024) void kb_reliable_residual4(Tiles rt, const in
 constexpr double kReliableJump = 4.0;   // |r|^2 ratio, true over recursive
 __global__ __launch_bounds__(64) void kb_reliable_apply(int k, const int32_t *gtile, const double *p_rr, int rule,
                                                         double tol, PcgScalars s, double *rr_ref, int *flag,
-                                                        int *keep, int *status) {
+                                                        int *status) {
     const int gj = blockIdx.x, g = gj / k, j = gj - g * k;
     const double rr = bt_graph_sum(p_rr, gtile, g, k, j);
     if (BT_LANE == 0) {
-        keep[gj] = 0;
         if (flag[gj]) {
             s.alpha[gj] = 0.0;
             const bool done = rr == 0.0 || (rule == 0 ? rr <= tol : sqrt(rr) <= tol * s.bnorm[gj]);
             if (done) {
                 s.active[gj] = 0;
-                keep[gj] = 2;
                 atomicSub(status, 1);
             } else {
                 if (flag[gj] == 2 || rr > kReliableJump * s.rr[gj]) s.rz[gj] = 0.0;
-                keep[gj] = 1;
                 s.rr[gj] = rr;   // the replaced FP32 residual's |r|^2
             }
             rr_ref[gj] = rr;   // a done column's final |b - A x|^2
             flag[gj] = 0;
         }
         if (gj == 0) status[2] = 0;
-    }
-}
-
-// Replacement, step 4: X32 = 0 on the replaced columns (their FP32
-// residual was replaced in step 2).
-__global__ __launch_bounds__(1024) void kb_reliable_replace(Tiles rt, int k, const int *__restrict__ keep,
-                                                           float *__restrict__ X32) {
-    const int t = blockIdx.x, g = rt.graph[t];
-    const int r0 = rt.start[t], r1 = rt.start[t + 1];
-    for (int jb = 0; jb < k; jb += 64) {
-        const int j = jb + BT_LANE;
-        if (j >= k) break;
-        if (!keep[(size_t)g * k + j]) continue;
-        for (int r = r0 + BT_WAVE; r < r1; r += BT_NW) X32[(size_t)r * k + j] = 0.0f;
     }
 }
 
@@ -1298,9 +1345,15 @@ static inline unsigned bt_grid(int64_t items, int per) {
 }
 
 void bt_sketch(hipStream_t s, Tiles rt, const int32_t *iptr, const int32_t *ieid, const double *ib, int k,
-               SketchCols cols, int uniform, double *Y) {
-    hipLaunchKernelGGL(kb_sketch, dim3(rt.count * kSketchSplit), dim3(64 * rt.waves), 0, s, rt, iptr, ieid, ib, k,
-                       cols, uniform, 0, Y);
+               SketchCols cols, int uniform, double *Y, Tiles et, const int32_t *edge_off, uint64_t *S) {
+    if (S && !uniform && et.count > 0) {
+        hipLaunchKernelGGL(kb_sketch_signs, dim3(et.count), dim3(256), 0, s, et, edge_off, k, cols, 0, S);
+        hipLaunchKernelGGL(kb_sketch<true>, dim3(rt.count * kSketchSplit), dim3(64 * rt.waves), 0, s, rt, iptr, ieid,
+                           ib, k, cols, uniform, 0, edge_off, S, Y);
+    } else {
+        hipLaunchKernelGGL(kb_sketch<false>, dim3(rt.count * kSketchSplit), dim3(64 * rt.waves), 0, s, rt, iptr, ieid,
+                           ib, k, cols, uniform, 0, edge_off, S, Y);
+    }
 }
 
 template <typename TA, typename TB>
@@ -1467,13 +1520,13 @@ void bt_reliable_group(hipStream_t s, int gk, int k, const int32_t *gs, const in
     hipLaunchKernelGGL(kb_reliable_group, dim3(bt_grid(gk, 256)), dim3(256), 0, s, gk, k, gs, ge, sc, flag, status);
 }
 void bt_reliable_fold(hipStream_t s, Tiles rt, int k, const int *flag, const double *alpha, const float *P,
-                      float *X32, double *X64) {
-    hipLaunchKernelGGL(kb_reliable_fold, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, k, flag, alpha, P, X32, X64);
+                      float *X32, double *X64, float *XF) {
+    hipLaunchKernelGGL(kb_reliable_fold, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, k, flag, alpha, P, X32, X64,
+                       XF);
 }
 void bt_reliable_apply(hipStream_t s, int gk, int k, const int32_t *gtile, const double *p, int rule, double tol,
-                       PcgScalars sc, double *rr_ref, int *flag, int *keep, int *status) {
-    hipLaunchKernelGGL(kb_reliable_apply, dim3(gk), dim3(64), 0, s, k, gtile, p, rule, tol, sc, rr_ref, flag, keep,
-                       status);
+                       PcgScalars sc, double *rr_ref, int *flag, int *status) {
+    hipLaunchKernelGGL(kb_reliable_apply, dim3(gk), dim3(64), 0, s, k, gtile, p, rule, tol, sc, rr_ref, flag, status);
 }
 void bt_reliable_residual(hipStream_t s, Tiles rt, const int32_t *rowptr, const int32_t *col, const double *val,
                           int k, const int *flag, const float *X, double *R, float *R32, double *partial) {
@@ -1483,9 +1536,6 @@ void bt_reliable_residual(hipStream_t s, Tiles rt, const int32_t *rowptr, const 
     else
         hipLaunchKernelGGL(kb_reliable_residual, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, rowptr, col, val, k,
                            flag, X, R, R32, partial);
-}
-void bt_reliable_replace(hipStream_t s, Tiles rt, int k, const int *keep, float *X32) {
-    hipLaunchKernelGGL(kb_reliable_replace, dim3(rt.count), dim3(64 * rt.waves), 0, s, rt, k, keep, X32);
 }
 // New diagonal of a sampled Laplacian, as Triplet::insert / process_diagonal
 // / to_csc and the sparse add build it (src/sparsifier.rs:48-87, :645-691):

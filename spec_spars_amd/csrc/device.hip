@@ -1206,20 +1206,17 @@ static int pcg_reliable(ss_graph *g, int k, int rule, double tol, double delta, 
             rounds++;
             if (g_trace_rounds) std::fprintf(stderr, "[reliable] iteration %d: %d of %d active columns replaced\n", it + 1, nflag, nactive);
             {
+                // the folded iterate goes to Y32, free once the right-hand
+                // sides were projected
                 ProfScope ps(g, SS_KERNEL_VECTOR);
-                bt_reliable_fold(s, rt, k, g->rel_flag, g->sc.alpha, w.P, g->X32, g->X64);
+                bt_reliable_fold(s, rt, k, g->rel_flag, g->sc.alpha, w.P, g->X32, g->X64, g->Y32);
             }
             {
                 ProfScope ps(g, SS_KERNEL_RESIDUAL);
-                bt_reliable_residual(s, stiles(g), g->rowptr, g->col, g->val, k, g->rel_flag, g->X32, g->R64, w.R,
+                bt_reliable_residual(s, stiles(g), g->rowptr, g->col, g->val, k, g->rel_flag, g->Y32, g->R64, w.R,
                                      g->partial);
             }
-            bt_reliable_apply(s, gk, k, g->sgtile, g->partial, rule, tol, g->sc, g->rel_ref, g->rel_flag,
-                              g->colkeep, g->status);
-            {
-                ProfScope ps(g, SS_KERNEL_VECTOR);
-                bt_reliable_replace(s, rt, k, g->colkeep, g->X32);
-            }
+            bt_reliable_apply(s, gk, k, g->sgtile, g->partial, rule, tol, g->sc, g->rel_ref, g->rel_flag, g->status);
             SS_TRY(read_status(g, &nactive));
         }
         if (nactive == 0) {
@@ -1532,7 +1529,14 @@ int graph_sketch(ss_graph *g, int k, int uniform, int prec) {
     SS_TRY(graph_ensure_work(g, k, prec));
     SS_HIP(hipSetDevice(g->ctx->device));
     ProfScope ps(g, SS_KERNEL_SKETCH);
-    bt_sketch(g->ctx->stream, rtiles(g), g->iptr, g->ieid, g->ib, k, sketch_cols(g), uniform, g->Y64);
+    // the signs' bits in X64 (the solve's output, free until the solve) when
+    // they fit: ceil(k / 64) words per edge in N * cap doubles
+    const size_t nw = ((size_t)k + 63) / 64;
+    uint64_t *S = !uniform && g->M > 0 && (size_t)g->M * nw <= (size_t)g->N * (size_t)g->wcap
+                      ? reinterpret_cast<uint64_t *>(g->X64)
+                      : nullptr;
+    bt_sketch(g->ctx->stream, rtiles(g), g->iptr, g->ieid, g->ib, k, sketch_cols(g), uniform, g->Y64,
+              Tiles{g->et_start, g->et_graph, g->net, 4}, g->edge_off_dev, S);
     SS_HIP(hipGetLastError());
     return SS_OK;
 }
