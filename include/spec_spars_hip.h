@@ -266,7 +266,11 @@ typedef struct ss_sparsify_report {
 } ss_sparsify_report;
 
 /* Sketch -> solve -> ER/sample/compact, all on the device; the factor must
- * have been set.  Results stay resident until read back. */
+ * have been set.  Results stay resident until read back.  With
+ * SS_PREC_MIXED and a reliable scheme, a request's columns replace their
+ * residuals together, and a column at its stop target keeps iterating
+ * until every column of its request is there (one FP64 verification for
+ * the request); each still stops on its FP64 true residual. */
 int ss_graph_sparsify(ss_graph *g, const ss_sparsify_opts *o,
                       ss_sparsify_report *rep);
 int ss_graph_read_kept(const ss_graph *g, int64_t cap, int32_t *ku,
