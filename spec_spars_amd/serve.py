@@ -123,9 +123,14 @@ class Registry:
             self.F = list(ex.map(build, graphs))
         for G, F in zip(self.G, self.F):
             G.set_factor(F)
-        # per graph one projected Gaussian block; a solve request draws its
-        # right-hand sides from it by a seeded column permutation and signs
+        # per graph one projected Gaussian block, drawn at registration; a
+        # solve request takes seeded columns of it with seeded signs
+        self.rhs_width = 64
         self.rhs_base = {}
+        for gi, (n, _, _, _) in enumerate(graphs):
+            base = np.random.default_rng(1000 + gi).standard_normal((n, self.rhs_width))
+            base -= base.mean(0)
+            self.rhs_base[gi] = base
 
     def close(self):
         for G in self.G:
@@ -139,20 +144,28 @@ class Registry:
     def need(self, r, precision):
         return self.G[r.graph].work_bytes(r.k, precision)
 
-    def rhs(self, gi, k, seed, width=64):
-        """k right-hand sides for graph gi: columns of a projected N(0, I)
-        block (width columns, drawn once per graph), permuted and
-        sign-flipped by the seed."""
-        n = self.specs[gi][0]
-        base = self.rhs_base.get(gi)
-        if base is None:
-            base = np.random.default_rng(1000 + gi).standard_normal((n, width))
-            base -= base.mean(0)
-            self.rhs_base[gi] = base
+    def _rhs_cols(self, k, seed):
+        """A solve request's columns of the Gaussian block and their signs."""
+        if not 1 <= k <= self.rhs_width:
+            raise ValueError("a solve request takes 1..%d right-hand sides, not %d" % (self.rhs_width, k))
         rng = np.random.default_rng(seed)
-        cols = rng.permutation(width)[:k]
-        signs = rng.choice([-1.0, 1.0], size=k)
-        return base[:, cols] * signs
+        return rng.permutation(self.rhs_width)[:k], rng.choice([-1.0, 1.0], size=k)
+
+    def rhs_block(self, gi, reqs):
+        """Right-hand sides of solve requests reqs = [(k, seed), ...] on
+        graph gi side by side: request (k, seed) takes k seeded columns of
+        the graph's projected N(0, I) block, each sign-flipped by the seed.
+        One gather over every column at once: gathering each request's
+        columns apart made the host, not the device, the bottleneck of a
+        merged solve on a 2M-vertex graph."""
+        cols, signs = zip(*(self._rhs_cols(k, seed) for k, seed in reqs))
+        b = np.take(self.rhs_base[gi], np.concatenate(cols), axis=1)
+        b *= np.concatenate(signs)
+        return b
+
+    def rhs(self, gi, k, seed):
+        """k right-hand sides for graph gi (one request's)."""
+        return self.rhs_block(gi, [(k, seed)])
 
     def execute_group(self, group, precision, tol=1e-8, keep_arrays=True):
         """Run requests that share a graph (blocking).  Solve requests run as
@@ -175,14 +188,15 @@ class Registry:
             return [{"kept": rep["kept"], "edges": G.m, "arrays": G.read_kept_many(i) if keep_arrays else None,
                      "iters": rep["max_iters"], "relres": rep["max_relres"], "group": len(group)}
                     for i, rep in enumerate(reps)]
-        b = np.concatenate([self.rhs(r.graph, r.k, r.seed) for r in group], axis=1)
+        b = self.rhs_block(group[0].graph, [(r.k, r.seed) for r in group])
         opts = self.ss.solve_opts(stop_rule=self.ss.STOP_REL, tol=tol, max_iter=2000, precision=precision,
                                   mixed_mode=self.mixed_mode[group[0].graph])
         Z, info = G.solve(b, opts)
         out, c = [], 0
         for r in group:
+            # each request's Z is a view of the merged block
             sl = slice(c, c + r.k)
-            out.append({"Z": Z[:, sl].copy(), "rhs": r.k, "iters": int(info["iters"][sl].max()),
+            out.append({"Z": Z[:, sl], "rhs": r.k, "iters": int(info["iters"][sl].max()),
                         "relres": float(info["relres"][sl].max()), "group": len(group)})
             c += r.k
         return out
