@@ -154,3 +154,31 @@ def test_server_groups_solves_on_one_graph():
     g0 = [g for g in Rec.groups if g[0] in (1, 2, 3, 4, 5)]
     assert [len(g) for g in g0] == [3, 2]
     assert sorted(x for g in Rec.groups for x in g) == list(range(7))
+
+
+def test_rhs_block_is_each_requests_columns_side_by_side():
+    """A merged solve's right-hand sides (one gather over the graph's
+    Gaussian block) are each request's seeded columns with its seeded signs,
+    in request order: the block a request gets alone, bit for bit.  Requests
+    of more columns than the block holds are refused."""
+    import pytest
+    from spec_spars_amd.serve import Registry
+
+    class HostRegistry(Registry):
+        def __init__(self, n, width=64):
+            self.rhs_width = width
+            self.rhs_base = {0: np.random.default_rng(3).standard_normal((n, width))}
+
+    reg = HostRegistry(1000)
+    reqs = [(16, 5), (8, 6), (16, 7)]
+    b = reg.rhs_block(0, reqs)
+    c = 0
+    for k, seed in reqs:
+        rng = np.random.default_rng(seed)
+        cols, signs = rng.permutation(64)[:k], rng.choice([-1.0, 1.0], size=k)
+        assert np.array_equal(b[:, c:c + k], reg.rhs_base[0][:, cols] * signs)
+        assert np.array_equal(b[:, c:c + k], reg.rhs(0, k, seed))
+        c += k
+    assert b.shape == (1000, 40) and b.flags.c_contiguous
+    with pytest.raises(ValueError):
+        reg.rhs(0, 65, 1)
